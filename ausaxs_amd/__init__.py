@@ -1,0 +1,287 @@
+"""MI355X-native AUSAXS Debye hot path — Python binding of the C ABI.
+
+The product is the C-ABI library ``libausaxs_hip.so`` (HIP kernels for gfx950
+plus a C++20 host layer mirroring AUSAXS's histogram-manager API). This module
+is a thin ctypes binding used by the tests and the benchmark. There is no CPU
+fallback: loading fails loudly when the library is missing, and every call
+fails loudly when no HIP device is present.
+
+Reference interfaces mirrored (AUSAXS 1.2.9):
+  * ``Structure.calculate`` -> ``IHistogramManager::calculate_all``
+    (hist/histogram_manager/IHistogramManager.h:16-29)
+  * ``CompositeHistogram.debye_transform`` -> ``ICompositeDistanceHistogram(Exv)::debye_transform``
+    (hist/intensity_calculator/ICompositeDistanceHistogram.h:24-71)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libausaxs_hip.so"
+
+MODE_FFAVG = 0
+MODE_FFEXPLICIT = 1
+MODE_SIMPLE = 2
+AA, AW, WW, PTOT = 0, 1, 2, 3
+MAX_FF = 15
+
+# form-factor slots (form_factor/FormFactorType.h:14-33)
+FF = dict(EXV=0, WATER=1, OH=1, H=2, C=3, CH=4, CH2=5, CH3=6, N=7, NH=8, NH2=9, NH3=10, O=11, S=12, SH=13, OTHER=14)
+
+
+class AusaxsError(RuntimeError):
+    pass
+
+
+class _Settings(C.Structure):
+    _fields_ = [
+        ("bin_width", C.c_double),
+        ("bin_count", C.c_int),
+        ("weighted_bins", C.c_int),
+        ("debye_terms", C.c_int),
+        ("tile_j", C.c_int),
+    ]
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m ausaxs_amd.build` (hipcc, gfx950). "
+            "There is no CPU fallback for the GPU path."
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "ausaxs_default_settings": (None, [P(_Settings)]),
+        "ausaxs_get_last_error": (C.c_int, [C.c_char_p, C.c_int]),
+        "ausaxs_device_info": (C.c_int, [C.c_char_p, C.c_int, P(C.c_int), P(C.c_size_t)]),
+        "ausaxs_structure_create": (
+            C.c_int,
+            [C.c_int, vp, C.c_int, vp, C.c_int, P(_Settings), C.c_double, P(vp)],
+        ),
+        "ausaxs_structure_calculate": (C.c_int, [vp, P(vp)]),
+        "ausaxs_structure_free": (C.c_int, [vp]),
+        "ausaxs_hist_calculate": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int, P(_Settings), C.c_double, P(vp)]),
+        "ausaxs_histogram_info": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
+        "ausaxs_histogram_get": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+        "ausaxs_histogram_d_axis": (C.c_int, [vp, vp]),
+        "ausaxs_histogram_set_params": (C.c_int, [vp] + [C.c_double] * 5),
+        "ausaxs_histogram_debye": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+        "ausaxs_histogram_free": (C.c_int, [vp]),
+        "ausaxs_default_q_axis": (C.c_int, [C.c_double, C.c_double, vp, P(C.c_int)]),
+        "ausaxs_pair_histogram": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp, vp]),
+        "ausaxs_debye_contract": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp]),
+        "ausaxs_bin_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, vp]),
+        "ausaxs_structure_set_timing": (C.c_int, [vp, C.c_int]),
+        "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
+        "ausaxs_synth_globule": (
+            C.c_int,
+            [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp],
+        ),
+        "ausaxs_synth_shell": (
+            C.c_int,
+            [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp],
+        ),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc: int):
+    if rc != 0:
+        buf = C.create_string_buffer(4096)
+        lib.ausaxs_get_last_error(buf, 4096)
+        raise AusaxsError(buf.value.decode(errors="replace"))
+
+
+def default_settings(**kw) -> _Settings:
+    s = _Settings()
+    lib.ausaxs_default_settings(C.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cus, lds = C.c_int(), C.c_size_t()
+    _check(lib.ausaxs_device_info(name, 256, C.byref(cus), C.byref(lds)))
+    return dict(arch=name.value.decode(), compute_units=cus.value, lds_per_block=lds.value)
+
+
+def points_ff(xyz: np.ndarray, ff) -> np.ndarray:
+    """Pack FP32 coordinates and form-factor types into the XYZFF layout."""
+    xyz = np.asarray(xyz, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((len(xyz), 4), dtype=np.float32)
+    out[:, :3] = xyz
+    out.view(np.int32)[:, 3] = np.broadcast_to(np.asarray(ff, dtype=np.int32), (len(xyz),))
+    return out
+
+
+def points_w(xyz: np.ndarray, w) -> np.ndarray:
+    """Pack FP32 coordinates and weights into the XYZW layout."""
+    xyz = np.asarray(xyz, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((len(xyz), 4), dtype=np.float32)
+    out[:, :3] = xyz
+    out[:, 3] = np.broadcast_to(np.asarray(w, dtype=np.float32), (len(xyz),))
+    return out
+
+
+def _pts(a):
+    if a is None:
+        return np.zeros((0, 4), np.float32)
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("points must be an (n, 4) float32 array")
+    return a
+
+
+class CompositeHistogram:
+    """ICompositeDistanceHistogram(Exv) returned by calculate_all."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        mode, mb = C.c_int(), C.c_int()
+        _check(lib.ausaxs_histogram_info(self._h, C.byref(mode), C.byref(mb)))
+        self.mode, self.max_bin = mode.value, mb.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.ausaxs_histogram_free(h)
+            self._h = C.c_void_p(0)
+
+    def _shape(self, which):
+        mb = self.max_bin
+        if self.mode == MODE_SIMPLE or which in (WW, PTOT):
+            return (mb,)
+        return (MAX_FF, MAX_FF, mb) if which == AA else (MAX_FF, mb)
+
+    def get(self, which):
+        """(value, count, bin_center) arrays in the reference layout."""
+        shp = self._shape(which)
+        v = np.zeros(shp, np.float64)
+        c = np.zeros(shp, np.uint32)
+        b = np.zeros(shp, np.float64)
+        _check(lib.ausaxs_histogram_get(self._h, which, _ptr(v), _ptr(c), _ptr(b)))
+        return v, c, b
+
+    def d_axis(self):
+        d = np.zeros(self.max_bin, np.float64)
+        _check(lib.ausaxs_histogram_d_axis(self._h, _ptr(d)))
+        return d
+
+    def set_params(self, cw=1.0, cx=1.0, crho=1.0, dw_atomic=0.0, dw_exv=0.0):
+        _check(lib.ausaxs_histogram_set_params(self._h, cw, cx, crho, dw_atomic, dw_exv))
+
+    def debye_transform(self, q, profiles=False):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        I = np.zeros(len(q), np.float64)
+        prof = np.zeros((6, len(q)), np.float64) if profiles else None
+        _check(lib.ausaxs_histogram_debye(self._h, _ptr(q), len(q), _ptr(I), _ptr(prof)))
+        return (I, prof) if profiles else I
+
+
+class Structure:
+    """Resident coordinates + work lists; calculate() = calculate_all()."""
+
+    def __init__(self, mode, atoms, waters=None, settings=None, z_exv=1.0):
+        self._atoms = _pts(atoms)
+        self._waters = _pts(waters)
+        s = settings or default_settings()
+        h = C.c_void_p()
+        _check(
+            lib.ausaxs_structure_create(
+                mode, _ptr(self._atoms), len(self._atoms), _ptr(self._waters), len(self._waters),
+                C.byref(s), float(z_exv), C.byref(h),
+            )
+        )
+        self._s = h
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.ausaxs_structure_free(s)
+            self._s = C.c_void_p(0)
+
+    def set_timing(self, on=True):
+        _check(lib.ausaxs_structure_set_timing(self._s, int(on)))
+
+    def calculate(self) -> CompositeHistogram:
+        h = C.c_void_p()
+        _check(lib.ausaxs_structure_calculate(self._s, C.byref(h)))
+        return CompositeHistogram(h.value)
+
+    def last_timing(self):
+        pm, tm, pr, ni = C.c_double(), C.c_double(), C.c_longlong(), C.c_int()
+        _check(lib.ausaxs_last_timing(self._s, C.byref(pm), C.byref(tm), C.byref(pr), C.byref(ni)))
+        return dict(pair_ms=pm.value, total_ms=tm.value, pairs=pr.value, n_items=ni.value)
+
+
+def calculate_all(mode, atoms, waters=None, settings=None, z_exv=1.0) -> CompositeHistogram:
+    return Structure(mode, atoms, waters, settings, z_exv).calculate()
+
+
+def default_q_axis(qmin=1e-4, qmax=0.5):
+    q = np.zeros(200, np.float64)
+    n = C.c_int()
+    _check(lib.ausaxs_default_q_axis(qmin, qmax, _ptr(q), C.byref(n)))
+    return q[: n.value].copy()
+
+
+def pair_histogram(a, b=None, bin_width=0.25, bin_count=8000):
+    """Raw pair histogram of one set (reference loop geometry): (count, sum of distances)."""
+    a = _pts(a)
+    bb = None if b is None else _pts(b)
+    cnt = np.zeros(bin_count, np.uint64)
+    ds = np.zeros(bin_count, np.float64)
+    _check(
+        lib.ausaxs_pair_histogram(
+            _ptr(a), len(a), _ptr(bb), 0 if bb is None else len(bb), bin_width, bin_count, _ptr(cnt), _ptr(ds)
+        )
+    )
+    return cnt, ds
+
+
+def debye_contract(H, d, q, terms=3):
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    rows, K = H.shape
+    out = np.zeros((rows, len(q)), np.float64)
+    _check(lib.ausaxs_debye_contract(_ptr(H), rows, K, _ptr(d), _ptr(q), len(q), terms, _ptr(out)))
+    return out
+
+
+def bin_check(lo_bits, hi_bits, bin_width):
+    m = np.zeros(2, np.uint64)
+    _check(lib.ausaxs_bin_check(lo_bits, hi_bits, bin_width, _ptr(m)))
+    return int(m[0]), int(m[1])
+
+
+def synth_globule(seed, n, axes, r_excl):
+    out = np.zeros((n, 4), np.float32)
+    _check(lib.ausaxs_synth_globule(seed, n, *map(float, axes), float(r_excl), _ptr(out)))
+    return out
+
+
+def synth_shell(seed, m, axes, offset=3.0):
+    out = np.zeros((m, 4), np.float32)
+    _check(lib.ausaxs_synth_shell(seed, m, *map(float, axes), float(offset), _ptr(out)))
+    return out

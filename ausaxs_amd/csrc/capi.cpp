@@ -1,0 +1,280 @@
+// extern "C" layer (include/ausaxs_hip.h). Exceptions never cross the ABI:
+// each entry catches, stores the message in a thread-local last-error string
+// and returns a non-zero status (the reference's execute_with_catch,
+// source/api/api_helper.h:25-45).
+#include "../../include/ausaxs_hip.h"
+
+#include "hist.hpp"
+#include "synth.hpp"
+
+#include <cstring>
+#include <string>
+
+using namespace ausaxs::gpu;
+
+struct ausaxs_structure {
+    std::unique_ptr<ResidentStructure> s;
+    bool timed = false;
+    EngineTiming last{};
+};
+
+struct ausaxs_histogram {
+    std::unique_ptr<CompositeDistanceHistogramHIP> h;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return 1;
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw std::runtime_error(msg);
+}
+
+HistSettings to_settings(const ausaxs_settings* s) {
+    HistSettings h;
+    if (s) {
+        h.bin_width = s->bin_width;
+        require(s->bin_count >= 10, "bin_count must be >= 10");
+        h.bin_count = (uint32_t)s->bin_count;
+        h.weighted_bins = s->weighted_bins != 0;
+        h.debye_terms = s->debye_terms;
+        h.tile_j = s->tile_j;
+    }
+    require(h.bin_width > 0, "bin_width must be positive");
+    require(h.debye_terms >= 0 && h.debye_terms <= 3, "debye_terms must be 0, 1, 2 or 3");
+    return h;
+}
+
+Mode to_mode(int m) {
+    require(m == AUSAXS_MODE_FFAVG || m == AUSAXS_MODE_FFEXPLICIT || m == AUSAXS_MODE_SIMPLE, "unknown mode");
+    return (Mode)m;
+}
+}  // namespace
+
+extern "C" {
+
+void ausaxs_default_settings(ausaxs_settings* s) {
+    if (!s) return;
+    HistSettings h;
+    s->bin_width = h.bin_width;
+    s->bin_count = (int)h.bin_count;
+    s->weighted_bins = h.weighted_bins ? 1 : 0;
+    s->debye_terms = h.debye_terms;
+    s->tile_j = h.tile_j;
+}
+
+int ausaxs_get_last_error(char* buf, int len) {
+    if (!buf || len <= 0) return 1;
+    std::strncpy(buf, g_last_error.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+    return 0;
+}
+
+int ausaxs_device_info(char* name, int len, int* cus, size_t* lds) {
+    return guarded([&] {
+        Context& c = Context::get();
+        if (name && len > 0) {
+            std::strncpy(name, c.prop.gcnArchName, (size_t)len - 1);
+            name[len - 1] = 0;
+        }
+        if (cus) *cus = c.prop.multiProcessorCount;
+        if (lds) *lds = c.prop.sharedMemPerBlock;
+    });
+}
+
+int ausaxs_structure_create(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                            const ausaxs_settings* settings, double z_exv, ausaxs_structure** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || atoms, "atoms is NULL");
+        require(n_waters == 0 || waters, "waters is NULL");
+        auto s = std::make_unique<ausaxs_structure>();
+        s->s = std::make_unique<ResidentStructure>(to_mode(mode), atoms, n_atoms, waters, n_waters,
+                                                   to_settings(settings), z_exv);
+        *out = s.release();
+    });
+}
+
+int ausaxs_structure_calculate(ausaxs_structure* s, ausaxs_histogram** out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        auto h = std::make_unique<ausaxs_histogram>();
+        h->h = s->s->calculate(s->timed);
+        s->last = s->s->engine().timing();
+        *out = h.release();
+    });
+}
+
+int ausaxs_structure_free(ausaxs_structure* s) {
+    delete s;
+    return 0;
+}
+
+int ausaxs_structure_set_timing(ausaxs_structure* s, int on) {
+    return guarded([&] {
+        require(s != nullptr, "NULL structure");
+        s->timed = on != 0;
+    });
+}
+
+int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
+                       int* n_items) {
+    return guarded([&] {
+        require(s != nullptr, "NULL structure");
+        if (pair_ms) *pair_ms = s->last.pair_ms;
+        if (total_ms) *total_ms = s->last.total_ms;
+        if (pairs) *pairs = s->last.pairs;
+        if (n_items) *n_items = s->last.n_items;
+    });
+}
+
+int ausaxs_hist_calculate(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                          const ausaxs_settings* settings, double z_exv, ausaxs_histogram** out) {
+    ausaxs_structure* s = nullptr;
+    int rc = ausaxs_structure_create(mode, atoms, n_atoms, waters, n_waters, settings, z_exv, &s);
+    if (rc) return rc;
+    rc = ausaxs_structure_calculate(s, out);
+    ausaxs_structure_free(s);
+    return rc;
+}
+
+int ausaxs_histogram_info(const ausaxs_histogram* h, int* mode, int* max_bin) {
+    return guarded([&] {
+        require(h != nullptr, "NULL histogram");
+        if (mode) *mode = (int)h->h->mode;
+        if (max_bin) *max_bin = (int)h->h->max_bin;
+    });
+}
+
+int ausaxs_histogram_get(const ausaxs_histogram* h, int which, double* value, uint32_t* count, double* center) {
+    return guarded([&] {
+        require(h != nullptr, "NULL histogram");
+        const CompositeDistanceHistogramHIP& c = *h->h;
+        const std::vector<WeightedEntry>* v = nullptr;
+        switch (which) {
+            case AUSAXS_AA: v = &c.aa; break;
+            case AUSAXS_AW: v = &c.aw; break;
+            case AUSAXS_WW: v = &c.ww; break;
+            case AUSAXS_PTOT: v = &c.p_tot; break;
+            default: throw std::runtime_error("unknown distribution id");
+        }
+        for (size_t i = 0; i < v->size(); ++i) {
+            if (value) value[i] = (*v)[i].value;
+            if (count) count[i] = (*v)[i].count;
+            if (center) center[i] = (*v)[i].bin_center;
+        }
+    });
+}
+
+int ausaxs_histogram_d_axis(const ausaxs_histogram* h, double* d) {
+    return guarded([&] {
+        require(h && d, "NULL argument");
+        std::memcpy(d, h->h->d_axis.data(), h->h->d_axis.size() * sizeof(double));
+    });
+}
+
+int ausaxs_histogram_set_params(ausaxs_histogram* h, double cw, double cx, double crho, double dw_atomic,
+                                double dw_exv) {
+    return guarded([&] {
+        require(h != nullptr, "NULL histogram");
+        h->h->params = FreeParams{cw, cx, crho, dw_atomic, dw_exv};
+    });
+}
+
+int ausaxs_histogram_debye(ausaxs_histogram* h, const double* q, int nq, double* I, double* profiles) {
+    return guarded([&] {
+        require(h && q && I && nq >= 0, "bad argument");
+        std::vector<double> qv(q, q + nq), prof;
+        auto res = h->h->debye_transform(qv, profiles ? &prof : nullptr);
+        std::memcpy(I, res.data(), (size_t)nq * sizeof(double));
+        if (profiles) std::memcpy(profiles, prof.data(), prof.size() * sizeof(double));
+    });
+}
+
+int ausaxs_histogram_free(ausaxs_histogram* h) {
+    delete h;
+    return 0;
+}
+
+int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq) {
+    return guarded([&] {
+        require(q && nq, "NULL argument");
+        auto v = default_q_axis(qmin, qmax);
+        std::memcpy(q, v.data(), v.size() * sizeof(double));
+        *nq = (int)v.size();
+    });
+}
+
+int ausaxs_pair_histogram(const float* a, int n, const float* b, int m, double bin_width, int bin_count,
+                          uint64_t* count, double* dsum) {
+    return guarded([&] {
+        require(a && count && dsum && n >= 0 && m >= 0 && bin_count > 0, "bad argument");
+        HistSettings s;
+        s.bin_width = bin_width;
+        s.bin_count = (uint32_t)bin_count;
+        raw_pair_histogram(a, n, b, m, s, count, dsum);
+    });
+}
+
+int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, const double* q, int nq, int terms,
+                          double* out) {
+    return guarded([&] {
+        require(H && d && q && out && rows >= 0 && K >= 0 && nq >= 0, "bad argument");
+        require(terms >= 0 && terms <= 3, "terms must be 0..3");
+        Context& ctx = Context::get();
+        DevBuf<double> dH, dd, dq, dout;
+        DevBuf<char> scratch;
+        dH.upload(H, (size_t)rows * K, ctx.stream);
+        dd.upload(d, K, ctx.stream);
+        dq.upload(q, nq, ctx.stream);
+        dout.ensure((size_t)rows * nq);
+        scratch.ensure(debye_scratch_bytes(rows, K, nq));
+        DebyeLaunch dl{dH.p, dd.p, dq.p, rows, K, nq, terms, dout.p, scratch.p};
+        check(terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
+        check(hipMemcpyAsync(out, dout.p, (size_t)rows * nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+              "D2H");
+        ctx.sync();
+    });
+}
+
+int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long long* mism) {
+    return guarded([&] {
+        require(mism && lo <= hi, "bad argument");
+        Context& ctx = Context::get();
+        DevBuf<unsigned long long> d;
+        d.ensure(2);
+        check(hipMemsetAsync(d.p, 0, 2 * sizeof(unsigned long long), ctx.stream), "memset");
+        check(launch_bin_check(lo, hi, (float)(1.0 / bin_width), d.p, ctx.stream), "bin_check_kernel");
+        check(hipMemcpyAsync(mism, d.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream), "D2H");
+        ctx.sync();
+    });
+}
+
+int ausaxs_synth_globule(uint64_t seed, int n, double ax, double ay, double az, double r_excl, float* out) {
+    return guarded([&] {
+        require(out != nullptr || n == 0, "out is NULL");
+        synth_globule(seed, n, ax, ay, az, r_excl, out);
+    });
+}
+
+int ausaxs_synth_shell(uint64_t seed, int m, double ax, double ay, double az, double offset, float* out) {
+    return guarded([&] {
+        require(out != nullptr || m == 0, "out is NULL");
+        synth_shell(seed, m, ax, ay, az, offset, out);
+    });
+}
+
+}  // extern "C"

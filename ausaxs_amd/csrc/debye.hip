@@ -1,0 +1,175 @@
+// Debye contraction on MFMA for gfx950 (CDNA4).
+//
+//   out[r][k] = sum_b H[r][b] * sinc(q_k * d_b)
+//
+// restates CompositeDistanceHistogramFFAvgBase::cache_refresh_sinqd
+// (CompositeDistanceHistogramFFAvgBase.cpp:427-467, std::inner_product over a
+// VectorDebyeTable, VectorDebyeTable.cpp:24-38) as a GEMM with
+// M = q points, N = histogram rows, K = distance bins.
+//
+// Operands are split into bf16 terms, x = x0 + x1 (+ x2), each term holding the
+// next 8 significant bits; products of terms with t_a + t_b < terms are summed
+// by v_mfma_f32_16x16x32_bf16 in FP32 (bf16 x bf16 products are exact in
+// FP32), and every 32-wide K step is added to an FP64 accumulator, so FP32
+// rounding never spans more than one K step. terms = 3 carries ~24 bits per
+// operand (the FP32-level split); terms = 2 is the hi+lo split.
+#include "device_api.hpp"
+
+namespace ausaxs::gpu {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxTerms = 3;
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ double bf16_val(uint16_t h) { return (double)__uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void split_store(double v, int terms, uint16_t* dst, size_t plane) {
+    double r = v;
+    for (int t = 0; t < terms; ++t) {
+        const uint16_t h = bf16_rne((float)r);
+        dst[t * plane] = h;
+        r -= bf16_val(h);
+    }
+}
+
+// sinc table (VectorDebyeTable.cpp:24-38) split into bf16 planes [terms][nq_pad][K_pad]
+__global__ __launch_bounds__(256) void sinc_split_kernel(const double* __restrict__ q, const double* __restrict__ d,
+                                                         int nq, int K, int nq_pad, int K_pad, int terms,
+                                                         uint16_t* __restrict__ A) {
+    const size_t plane = (size_t)nq_pad * K_pad;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= plane) return;
+    const int k = (int)(e / K_pad), b = (int)(e % K_pad);
+    double s = 0.0;
+    if (k < nq && b < K) {
+        const double qd = q[k] * d[b];
+        if (qd < 1e-3) {
+            const double qd2 = qd * qd;
+            s = 1 - qd2 * (1. / 6) + qd2 * qd2 * (1. / 120);
+        } else {
+            s = sin(qd) / qd;
+        }
+    }
+    split_store(s, terms, A + e, plane);
+}
+
+__global__ __launch_bounds__(256) void h_split_kernel(const double* __restrict__ H, int rows, int K, int rows_pad,
+                                                      int K_pad, int terms, uint16_t* __restrict__ B) {
+    const size_t plane = (size_t)rows_pad * K_pad;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= plane) return;
+    const int r = (int)(e / K_pad), b = (int)(e % K_pad);
+    const double v = (r < rows && b < K) ? H[(size_t)r * K + b] : 0.0;
+    split_store(v, terms, B + e, plane);
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// One wave per 16 (q) x 16 (rows) output tile. Lane l feeds row/column l%16
+// and K sub-block l/16 (8 consecutive bf16); the accumulator holds
+// D[4*(l/16)+i][l%16].
+__global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                                        int nq, int rows, int nq_pad, int rows_pad, int K_pad,
+                                                        int terms, double* __restrict__ out) {
+    const int l = threadIdx.x;
+    const int q0 = blockIdx.x * 16, r0 = blockIdx.y * 16;
+    const int m = l & 15, kg = l >> 4;
+    const size_t pa = (size_t)nq_pad * K_pad, pb = (size_t)rows_pad * K_pad;
+    const uint16_t* arow = A + (size_t)(q0 + m) * K_pad + kg * 8;
+    const uint16_t* brow = B + (size_t)(r0 + m) * K_pad + kg * 8;
+    double acc64[4] = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < K_pad; k0 += 32) {
+        bf16x8 a[kMaxTerms], b[kMaxTerms];
+        for (int t = 0; t < terms; ++t) {
+            a[t] = load8(arow + t * pa + k0);
+            b[t] = load8(brow + t * pb + k0);
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        // smallest products first, largest last
+        if (terms >= 3) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+        }
+        if (terms >= 2) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc64[i] += (double)acc[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int qk = q0 + 4 * kg + i;
+        const int r = r0 + m;
+        if (qk < nq && r < rows) out[(size_t)r * nq + qk] = acc64[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void debye_f64_kernel(const double* __restrict__ H, const double* __restrict__ d,
+                                                        const double* __restrict__ q, int rows, int K, int nq,
+                                                        double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * nq) return;
+    const int r = e / nq, k = e % nq;
+    double s = 0.0;
+    for (int b = 0; b < K; ++b) {
+        const double qd = q[k] * d[b];
+        double v;
+        if (qd < 1e-3) {
+            const double qd2 = qd * qd;
+            v = 1 - qd2 * (1. / 6) + qd2 * qd2 * (1. / 120);
+        } else {
+            v = sin(qd) / qd;
+        }
+        s += H[(size_t)r * K + b] * v;
+    }
+    out[e] = s;
+}
+
+inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+size_t debye_scratch_bytes(int rows, int K, int nq) {
+    const size_t K_pad = pad_to(K, 32), nq_pad = pad_to(nq, 16), rows_pad = pad_to(rows, 16);
+    return (size_t)kMaxTerms * (nq_pad + rows_pad) * K_pad * sizeof(uint16_t);
+}
+
+hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s) {
+    if (p.rows == 0 || p.nq == 0) return hipSuccess;
+    if (p.terms < 1 || p.terms > kMaxTerms) return hipErrorInvalidValue;
+    const int K_pad = pad_to(p.K, 32), nq_pad = pad_to(p.nq, 16), rows_pad = pad_to(p.rows, 16);
+    uint16_t* A = static_cast<uint16_t*>(p.scratch);
+    uint16_t* B = A + (size_t)kMaxTerms * nq_pad * K_pad;
+    const size_t na = (size_t)nq_pad * K_pad, nb = (size_t)rows_pad * K_pad;
+    hipLaunchKernelGGL(sinc_split_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, p.q, p.d, p.nq, p.K,
+                       nq_pad, K_pad, p.terms, A);
+    hipLaunchKernelGGL(h_split_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, p.H, p.rows, p.K, rows_pad,
+                       K_pad, p.terms, B);
+    hipLaunchKernelGGL(debye_mfma_kernel, dim3(nq_pad / 16, rows_pad / 16), dim3(64), 0, s, A, B, p.nq, p.rows, nq_pad,
+                       rows_pad, K_pad, p.terms, p.out);
+    return hipGetLastError();
+}
+
+hipError_t launch_debye_f64(const DebyeLaunch& p, hipStream_t s) {
+    const int n = p.rows * p.nq;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(debye_f64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p.H, p.d, p.q, p.rows, p.K, p.nq,
+                       p.out);
+    return hipGetLastError();
+}
+
+}  // namespace ausaxs::gpu

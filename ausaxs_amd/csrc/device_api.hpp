@@ -1,0 +1,99 @@
+// Device-side entry points of the MI355X Debye hot path (gfx950).
+//
+// Host code (managers, C ABI) calls these launchers; the kernels themselves
+// live in pair_hist.hip and debye.hip. All launchers are asynchronous on the
+// given stream and return hipError_t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ausaxs::gpu {
+
+// Fixed-point encoding of one LDS/partial histogram cell (one uint64):
+//   bits 40..63  pair count (24 bits)
+//   bits  0..39  sum over pairs of (q + 2^15), q = round(delta * 2^16),
+//                delta = x - rint(x), x = d * inv_width (bin units)
+// A work item holds < 2^24 pairs, so neither field can overflow. Integer
+// accumulation keeps every histogram bit-reproducible.
+constexpr int kDeltaBits = 16;
+constexpr uint64_t kCountShift = 40;
+constexpr uint64_t kLowMask = (uint64_t(1) << kCountShift) - 1;
+constexpr int64_t kDeltaOffset = int64_t(1) << (kDeltaBits - 1);
+constexpr int64_t kMaxPairsPerItem = (int64_t(1) << 24) - 1;
+
+// One workgroup's rectangle of the pair space.
+struct PairWork {
+    int32_t i_begin, i_end;  // row points (indices into the point buffer)
+    int32_t j_begin, j_end;  // column points
+    int32_t row0, row1;      // global histogram rows: row0 if tag_i < tag_j, else row1
+    int32_t flags;           // kDiag | kOrient
+    int32_t slot;            // partial slot (= item index)
+};
+constexpr int32_t kDiag = 1;    // same point set and overlapping ranges: count j > i only
+constexpr int32_t kOrient = 2;  // two output rows chosen by original-index order
+
+// Pairs involving the reference's scalar-tail columns (last <= 3 points of a
+// loop), evaluated with per-pair rule selection and added with global atomics.
+struct StripWork {
+    int32_t n_rows;        // row points (original order)
+    int32_t t_begin;       // first column index handled (original order)
+    int32_t t_end;         // one past last
+    int32_t n_cols_total;  // column set size (n for self loops, m for cross)
+    int32_t self;          // 1: self loop (i < t), rule by (n-i-1) mod 4; 0: cross (always scalar rule)
+    int32_t n_col_classes; // row = rowmap[class_i * n_col_classes + class_t]
+};
+
+struct PairLaunch {
+    const float4* pts;        // point buffer (x, y, z, tag bits)
+    const PairWork* items;    // device
+    int n_items;
+    uint64_t* partial;        // [n_items][2][nbins+1]
+    float inv_w;
+    int nbins;                // LDS bins per row; slot nbins collects overflow
+    int weighted;             // reserved
+};
+
+hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s);
+
+// Sum partial slots into final rows. row_ptr/row_list: CSR over global rows,
+// entries = item*2 + local_row. cnt/dsum: [n_rows][nbins+1].
+hipError_t launch_merge(const uint64_t* partial, const int32_t* row_ptr, const int32_t* row_list, int n_rows,
+                        int nbins, uint64_t* cnt, int64_t* dsum, hipStream_t s);
+
+struct StripLaunch {
+    const float4* rows;       // original-order row points (tag = class id bits)
+    const float4* cols;       // original-order column points
+    StripWork w;
+    const int32_t* rowmap;    // device
+    uint64_t* cnt;
+    int64_t* dsum;
+    float inv_w;
+    int nbins;
+    int stride;               // nbins+1
+};
+hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
+
+// Exhaustive check of the binning arithmetic over float d2 in [lo, hi] (bit
+// patterns): counts lanes where the fast path (raw v_sqrt + tie fallback)
+// disagrees with the correctly rounded reference, and where the device's
+// correctly rounded sqrt is not the IEEE result (checked in FP64).
+hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* mismatches,
+                            hipStream_t s);
+
+// Debye contraction on MFMA: out[r][k] = sum_b H[r][b] * sinc(q_k * d_b).
+struct DebyeLaunch {
+    const double* H;   // [rows][K] device
+    const double* d;   // [K] device
+    const double* q;   // [nq] device
+    int rows, K, nq;
+    int terms;         // bf16 terms per operand (2: hi+lo, 3: hi+mid+lo)
+    double* out;       // [rows][nq] device
+    void* scratch;     // workspace, see debye_scratch_bytes
+};
+size_t debye_scratch_bytes(int rows, int K, int nq);
+hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s);
+// Plain FP64 contraction (same operands), used as the device reference path.
+hipError_t launch_debye_f64(const DebyeLaunch& p, hipStream_t s);
+
+}  // namespace ausaxs::gpu

@@ -1,0 +1,589 @@
+// Histogram managers and composite histograms (see hist.hpp).
+#include "hist.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+
+namespace ausaxs::gpu {
+
+float inv_width(const HistSettings& s) {
+    // ConstantWidth / VariableWidth (WidthControllers.h:10-27): 1./width in
+    // double, returned as float.
+    return (float)(1.0 / s.bin_width);
+}
+
+// ---------------------------------------------------------------------------
+// form factors
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Gauss5 {
+    double a[5], b[5], c;
+};
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kS2Q = 1. / (4 * 4 * kPi * kPi);  // FormFactorTable.h:15-23
+
+// FormFactorTable.h:28-120 (b in s units; converted on evaluation)
+const Gauss5 kH{{0.489918, 0.262003, 0.196767, 0.049879, 0}, {20.6593, 7.74039, 49.5519, 2.20159, 0}, 0.001305};
+const Gauss5 kO{{2.960427, 2.508818, 0.637853, 0.722838, 1.142756},
+                {14.182259, 5.936858, 0.112726, 34.958481, 0.390240},
+                0.027014};
+const Gauss5 kN{{11.893780, 3.277479, 1.858092, 0.858927, 0.912985},
+                {0.000158, 10.232723, 30.344690, 0.656065, 0.217287},
+                -11.804902};
+const Gauss5 kS{{6.362157, 5.154568, 1.473732, 1.635073, 1.209372},
+                {1.514347, 22.092528, 0.061373, 55.445176, 0.646925},
+                0.154722};
+const Gauss5 kC{{2.657506, 1.078079, 1.490909, -4.241070, 0.713791},
+                {14.780758, 0.776775, 42.086843, -0.000294, 0.239535},
+                4.297983};
+const Gauss5 kAr{{7.188004, 6.638454, 0.454180, 1.929593, 1.523654},
+                 {0.956221, 15.339877, 15.339862, 39.043824, 0.062409},
+                 0.265954};
+const Gauss5 kCH{{2.909530, 0.485267, 1.516151, 0.206905, 1.541626},
+                 {13.933084, 23.221524, 41.990403, 4.974183, 0.679266},
+                 0.337670};
+const Gauss5 kCH2{{3.275723, 0.870037, 1.534606, 0.395078, 1.544562},
+                  {13.408502, 23.785175, 41.922444, 5.019072, 0.724439},
+                  0.377096};
+const Gauss5 kCH3{{3.681341, 1.228691, 1.549320, 0.574033, 1.554377},
+                  {13.026207, 24.131974, 41.869426, 4.984373, 0.765769},
+                  0.409294};
+const Gauss5 kOH{{0.456221, 3.219608, 0.812773, 2.666928, 1.380927},
+                 {21.503498, 13.397134, 34.547137, 5.826620, 0.412902},
+                 0.463202};
+const Gauss5 kNH{{1.650531, 0.429639, 2.144736, 1.851894, 1.408921},
+                 {10.603730, 6.987283, 29.939901, 10.573859, 0.611678},
+                 0.510589};
+const Gauss5 kNH2{{1.904157, 1.942536, 2.435585, 0.730512, 1.379728},
+                  {10.803702, 10.792421, 29.610479, 6.847755, 0.709687},
+                  0.603738};
+const Gauss5 kNH3{{1.882162, 1.933200, 2.465843, 0.927311, 1.190889},
+                  {10.975157, 10.956008, 29.208572, 6.663555, 0.843650},
+                  0.597322};
+const Gauss5 kSH{{0.570042, 6.337416, 1.641643, 5.398549, 1.527982},
+                 {11.447986, 1.197657, 55.401032, 22.420955, 2.356552},
+                 1.523944};
+
+double gauss5(const Gauss5& g, double q) {
+    double s = 0;
+    for (int i = 0; i < 5; ++i) s += g.a[i] * std::exp(-(g.b[i] * kS2Q) * q * q);
+    return s + g.c;
+}
+
+const Gauss5* atomic_table(int type) {
+    // lookup::atomic::raw::get (FormFactor.h:93-117)
+    switch (type) {
+        case 1: return &kOH;  // WATER == OH -> OH_alc
+        case 2: return &kH;
+        case 3: return &kC;
+        case 4: return &kCH;   // CH_sp3
+        case 5: return &kCH2;  // CH2_sp3
+        case 6: return &kCH3;  // CH3_sp3
+        case 7: return &kN;
+        case 8: return &kNH;
+        case 9: return &kNH2;
+        case 10: return &kNH3;  // NH3_plus
+        case 11: return &kO;
+        case 12: return &kS;
+        case 13: return &kSH;
+        case 14: return &kAr;  // OTHER
+        default: return nullptr;
+    }
+}
+
+double exv_volume(int type) {
+    // constants::exv::MinimumFluctuation_implicit_H; OTHER -> volume(vdw Ar = 1.88)
+    static const double v[kMaxFF] = {0,      20.911, 0,      12.352, 11.640, 34.583, 41.851, 0.027,
+                                     2.181, 20.562, 20.722, 14.238, 15.413, 28.529, 0};
+    if (type == 14) return 4 * kPi / 3 * 1.88 * 1.88 * 1.88;
+    if (type < 1 || type >= kMaxFF) throw std::runtime_error("ausaxs_hip: no excluded-volume entry for this type");
+    return v[type];
+}
+
+}  // namespace
+
+double ff_atomic(int type, double q) {
+    if (type == kExvBin) {
+        const double r = 1.62;  // constants::radius::average_atomic_radius
+        return std::exp(-(r * r / 2) * q * q);
+    }
+    const Gauss5* g = atomic_table(type);
+    if (!g) throw std::runtime_error("ausaxs_hip: invalid form factor type " + std::to_string(type));
+    return gauss5(*g, q);
+}
+
+double ff_exv(int type, double q) {
+    const double v = exv_volume(type);
+    const double exponent = std::pow(v, 2. / 3) / (4 * kPi);
+    return v * 0.334 * std::exp(-exponent * q * q);
+}
+
+std::vector<double> default_q_axis(double qmin, double qmax) {
+    const double amin = 1e-4, amax = 1, bins = 200, width = (amax - amin) / bins;
+    auto get_bin = [&](double v) -> unsigned {
+        if (v <= amin) return 0;
+        if (v >= amax) return (unsigned)bins;
+        return (unsigned)std::floor((v + 1e-6 - amin) / width);
+    };
+    const unsigned b0 = get_bin(qmin), b1 = get_bin(qmax);
+    std::vector<double> q;
+    for (unsigned i = b0; i < b1; ++i) q.push_back(amin + i * width);
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// weighted entries
+// ---------------------------------------------------------------------------
+namespace {
+
+inline void add(WeightedEntry& a, const WeightedEntry& b) {
+    a.value += b.value;
+    a.count += b.count;
+    a.bin_center += b.bin_center;
+}
+
+inline void scale(WeightedEntry& e, double f) {  // WeightedEntry.h operator*
+    e.value *= f;
+    e.count = (uint32_t)(f * e.count);
+    e.bin_center *= f;
+}
+
+unsigned max_bin_of(const std::vector<WeightedEntry>& p) {
+    for (int i = (int)p.size() - 1; i >= 10; --i)
+        if (p[i].value != 0) return (unsigned)i + 1;
+    return 10;
+}
+
+std::vector<double> axis_of(const std::vector<WeightedEntry>& p, double bin_width, bool weighted) {
+    const int n = (int)p.size();
+    std::vector<double> d(n);
+    const double width = (n * bin_width) / n;
+    for (int i = 0; i < n; ++i) {
+        const double dv = i * width;
+        if (weighted) {
+            const double bc = p[i].bin_center;
+            d[i] = ((double)(!bc) * dv + bc) / (double)((!p[i].count) + p[i].count);
+        } else {
+            d[i] = dv;
+        }
+    }
+    if (n) d[0] = 0;
+    return d;
+}
+
+std::vector<float4> to_points(const float* xyzw, int n) {
+    std::vector<float4> v(n);
+    for (int i = 0; i < n; ++i) v[i] = make_float4(xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2], 0.f);
+    return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// ResidentStructure
+// ---------------------------------------------------------------------------
+ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                                     const HistSettings& s, double z_exv)
+    : mode_(mode), s_(s), z_exv_(z_exv), n_atoms_(n_atoms), n_waters_(n_waters) {
+    if (n_atoms < 0 || n_waters < 0) throw std::runtime_error("ausaxs_hip: negative point count");
+    if (!(s.bin_width > 0) || s.bin_count < 10) throw std::runtime_error("ausaxs_hip: invalid bin settings");
+    const float inv_w = inv_width(s);
+    PointSet A, W;
+    A.orig = to_points(atoms, n_atoms);
+    W.orig = to_points(waters, n_waters);
+    std::vector<PairJob> jobs;
+    int n_rows = 0;
+
+    if (mode == Mode::FFAvg || mode == Mode::FFExplicit) {
+        atom_ff_.resize(n_atoms);
+        std::set<int> present;
+        for (int i = 0; i < n_atoms; ++i) {
+            int32_t f;
+            std::memcpy(&f, &atoms[4 * i + 3], 4);
+            // CompactCoordinatesFF::setup (CompactCoordinatesFF.h:21-32) throws on UNKNOWN
+            if (f < 1 || f >= kMaxFF)
+                throw std::runtime_error("ausaxs_hip: atom " + std::to_string(i) + " has form factor type " +
+                                         std::to_string(f) + " outside [1, 14]");
+            atom_ff_[i] = f;
+            A.orig[i].w = float_of(f);
+            present.insert(f);
+        }
+        A.n_classes = kMaxFF;
+        W.n_classes = 1;
+        PairJob aa, aw, ww;
+        aa.row_set = 0;
+        aa.rowmap.assign(kMaxFF * kMaxFF, -1);
+        aw.row_set = 0;
+        aw.col_set = 1;
+        aw.rowmap.assign(kMaxFF, -1);
+        ww.row_set = 1;
+        ww.rowmap.assign(1, -1);
+        for (int f1 : present)
+            for (int f2 : present) {
+                aa.rowmap[f1 * kMaxFF + f2] = n_rows++;
+                rows_.push_back({0, f1, f2});
+            }
+        for (int f : present) {
+            aw.rowmap[f] = n_rows++;
+            rows_.push_back({1, f, 0});
+        }
+        ww.rowmap[0] = n_rows++;
+        rows_.push_back({2, 0, 0});
+        jobs = {aa, aw, ww};
+    } else {
+        // XYZW weights: one class per distinct weight value; w_i*w_j of a pair
+        // is then the float product of its two class weights.
+        auto classify = [](const float* p, int n, PointSet& ps, std::vector<float>& w, std::vector<float>& cw) {
+            std::map<uint32_t, int> id;
+            w.resize(n);
+            for (int i = 0; i < n; ++i) {
+                const float wi = p[4 * i + 3];
+                uint32_t key;
+                std::memcpy(&key, &wi, 4);
+                auto it = id.find(key);
+                if (it == id.end()) {
+                    it = id.emplace(key, (int)cw.size()).first;
+                    cw.push_back(wi);
+                }
+                w[i] = wi;
+                ps.orig[i].w = float_of(it->second);
+            }
+            if (cw.size() > 64)
+                throw std::runtime_error("ausaxs_hip: more than 64 distinct weights in one point set");
+            ps.n_classes = std::max<int>(1, (int)cw.size());
+        };
+        classify(atoms, n_atoms, A, atom_w_, atom_class_w_);
+        classify(waters, n_waters, W, water_w_, water_class_w_);
+        const int CA = A.n_classes, CW = W.n_classes;
+        PairJob aa, ww, aw;
+        aa.row_set = 0;
+        aa.rowmap.assign(CA * CA, -1);
+        for (int a = 0; a < CA; ++a)
+            for (int b = a; b < CA; ++b) {
+                aa.rowmap[a * CA + b] = aa.rowmap[b * CA + a] = n_rows++;
+                rows_.push_back({0, a, b});
+            }
+        ww.row_set = 1;
+        ww.rowmap.assign(CW * CW, -1);
+        for (int a = 0; a < CW; ++a)
+            for (int b = a; b < CW; ++b) {
+                ww.rowmap[a * CW + b] = ww.rowmap[b * CW + a] = n_rows++;
+                rows_.push_back({2, a, b});
+            }
+        // SimpleCalculator::enqueue_calculate_cross(atoms, waters): rows over waters
+        aw.row_set = 1;
+        aw.col_set = 0;
+        aw.rowmap.assign(CW * CA, -1);
+        for (int a = 0; a < CW; ++a)
+            for (int b = 0; b < CA; ++b) {
+                aw.rowmap[a * CA + b] = n_rows++;
+                rows_.push_back({1, a, b});
+            }
+        jobs = {aa, ww, aw};
+    }
+
+    const int bound = bin_bound({&A, &W}, inv_w);
+    const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, bound);
+    std::vector<PointSet> sets;
+    sets.push_back(std::move(A));
+    sets.push_back(std::move(W));
+    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j);
+}
+
+std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool timed) {
+    engine_->run(cnt_, dsum_, timed);
+    const int nbins = engine_->nbins();
+    const int stride = nbins + 1;
+    if (engine_->overflowed(cnt_))
+        throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
+                                 std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
+                                 " A); increase bin_count or bin_width");
+    const double inv_w = (double)engine_->inv_w();
+    const size_t nb = nbins;
+
+    // entry of one global row and bin, times the reference's template factor
+    auto entry = [&](int row, int b, int factor) {
+        const uint64_t c = cnt_[(size_t)row * stride + b];
+        const int64_t s = dsum_[(size_t)row * stride + b];
+        WeightedEntry e;
+        e.value = (double)factor * (double)c;
+        e.count = (uint32_t)(factor * c);
+        e.bin_center = factor * (((double)c * b + (double)s * (1.0 / 65536.0)) / inv_w);
+        return e;
+    };
+
+    auto res = std::make_unique<CompositeDistanceHistogramHIP>();
+    res->mode = mode_;
+    res->bin_width = s_.bin_width;
+    res->weighted_bins = s_.weighted_bins;
+    res->debye_terms = s_.debye_terms;
+
+    if (mode_ == Mode::FFAvg || mode_ == Mode::FFExplicit) {
+        std::vector<WeightedEntry> aa((size_t)kMaxFF * kMaxFF * nb), aw((size_t)kMaxFF * nb), ww(nb);
+        auto A3 = [&](int f1, int f2, int b) -> WeightedEntry& { return aa[((size_t)f1 * kMaxFF + f2) * nb + b]; };
+        auto A2 = [&](int f, int b) -> WeightedEntry& { return aw[(size_t)f * nb + b]; };
+        for (size_t r = 0; r < rows_.size(); ++r) {
+            const auto& k = rows_[r];
+            for (int b = 0; b < nbins; ++b) {
+                if (k[0] == 0) A3(k[1], k[2], b) = entry((int)r, b, 2);
+                else if (k[0] == 1) A2(k[1], b) = entry((int)r, b, 1);
+                else ww[b] = entry((int)r, b, 2);
+            }
+        }
+        if (mode_ == Mode::FFAvg) {
+            // HistogramManagerMTFFAvg.cpp:27-34,111-118: every aa pair also lands in
+            // [ff_i][exv] (ff of the lower original index) and [exv][exv]; every
+            // aw pair in aw[exv]. Derived from the ordered cells.
+            for (int f = 1; f < kMaxFF; ++f)
+                for (int g = 1; g < kMaxFF; ++g)
+                    for (int b = 0; b < nbins; ++b) {
+                        add(A3(f, kExvBin, b), A3(f, g, b));
+                        add(A3(kExvBin, kExvBin, b), A3(f, g, b));
+                    }
+            for (int f = 1; f < kMaxFF; ++f)
+                for (int b = 0; b < nbins; ++b) add(A2(kExvBin, b), A2(f, b));
+        }
+        // self correlations (FFAvg.cpp:308-323, FFExplicit.cpp:331-346)
+        for (int i = 0; i < n_atoms_; ++i) add(A3(atom_ff_[i], atom_ff_[i], 0), WeightedEntry{1, 1, 0});
+        if (mode_ == Mode::FFAvg) add(A3(kExvBin, kExvBin, 0), WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
+        add(ww[0], WeightedEntry{(double)n_waters_, (uint32_t)n_waters_, 0});
+
+        std::vector<WeightedEntry> ptot(nb);
+        for (int f1 = 1; f1 < kMaxFF; ++f1)
+            for (int f2 = 1; f2 < kMaxFF; ++f2)
+                for (int b = 0; b < nbins; ++b) add(ptot[b], A3(f1, f2, b));
+        for (int f = 1; f < kMaxFF; ++f)
+            for (int b = 0; b < nbins; ++b) add(ptot[b], A2(f, b));
+        for (int b = 0; b < nbins; ++b) add(ptot[b], ww[b]);
+        const unsigned max_bin = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
+
+        if (mode_ == Mode::FFAvg) {  // FFAvg.cpp:355-364
+            for (int f = 1; f < kMaxFF; ++f)
+                for (unsigned b = 0; b < max_bin; ++b) scale(A3(f, kExvBin, b), z_exv_);
+            for (unsigned b = 0; b < max_bin; ++b) scale(A3(kExvBin, kExvBin, b), z_exv_ * z_exv_);
+            for (unsigned b = 0; b < max_bin; ++b) scale(A2(kExvBin, b), z_exv_);
+        }
+        res->max_bin = max_bin;
+        res->aa.resize((size_t)kMaxFF * kMaxFF * max_bin);
+        res->aw.resize((size_t)kMaxFF * max_bin);
+        for (int f1 = 0; f1 < kMaxFF; ++f1) {
+            for (int f2 = 0; f2 < kMaxFF; ++f2)
+                std::copy_n(&A3(f1, f2, 0), max_bin, &res->aa[((size_t)f1 * kMaxFF + f2) * max_bin]);
+            std::copy_n(&A2(f1, 0), max_bin, &res->aw[(size_t)f1 * max_bin]);
+        }
+        res->ww.assign(ww.begin(), ww.begin() + max_bin);
+        res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
+    } else {
+        std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
+        for (size_t r = 0; r < rows_.size(); ++r) {
+            const auto& k = rows_[r];
+            double wprod;
+            std::vector<WeightedEntry>* dst;
+            if (k[0] == 0) wprod = (double)(atom_class_w_[k[1]] * atom_class_w_[k[2]]), dst = &aa;
+            else if (k[0] == 2) wprod = (double)(water_class_w_[k[1]] * water_class_w_[k[2]]), dst = &ww;
+            else wprod = (double)(water_class_w_[k[1]] * atom_class_w_[k[2]]), dst = &aw;
+            for (int b = 0; b < nbins; ++b) {
+                WeightedEntry e = entry((int)r, b, 2);  // add<2> (TemplateHelperSimple.h; SimpleCalculator.h:195-207)
+                e.value *= wprod;
+                add((*dst)[b], e);
+            }
+        }
+        // SimpleCalculator.h:144-160: bin 0 += (sum w^2, (unsigned) sum w^2, 0)
+        auto self_term = [](const std::vector<float>& w) {
+            double t = 0;
+            for (float x : w) t += (double)(x * x);
+            return t;
+        };
+        const double ta = self_term(atom_w_), tw = self_term(water_w_);
+        add(aa[0], WeightedEntry{ta, (uint32_t)ta, 0});
+        add(ww[0], WeightedEntry{tw, (uint32_t)tw, 0});
+        std::vector<WeightedEntry> ptot(nb);
+        for (size_t b = 0; b < nb; ++b) {
+            ptot[b] = aa[b];
+            add(ptot[b], ww[b]);
+            add(ptot[b], aw[b]);
+        }
+        const unsigned max_bin = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
+        res->max_bin = max_bin;
+        res->aa.assign(aa.begin(), aa.begin() + max_bin);
+        res->aw.assign(aw.begin(), aw.begin() + max_bin);
+        res->ww.assign(ww.begin(), ww.begin() + max_bin);
+        res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
+    }
+    res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins);
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Debye transform
+// ---------------------------------------------------------------------------
+const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vector<double>& q) {
+    if (!cached_sinqd_.empty() && q == cached_q_) return cached_sinqd_;
+    const int K = (int)max_bin;
+    std::vector<double> H;
+    row_keys_.clear();
+    auto push_row = [&](int kind, int f1, int f2, const WeightedEntry* src) {
+        bool nz = false;
+        for (int b = 0; b < K; ++b) nz |= src[b].value != 0;
+        if (!nz) return;
+        row_keys_.push_back({kind, f1, f2});
+        for (int b = 0; b < K; ++b) H.push_back(src[b].value);
+    };
+    if (mode == Mode::Simple) {
+        push_row(0, 0, 0, aa.data());
+        push_row(1, 0, 0, aw.data());
+        push_row(2, 0, 0, ww.data());
+    } else {
+        for (int f1 = 0; f1 < kMaxFF; ++f1)
+            for (int f2 = 0; f2 < kMaxFF; ++f2) push_row(0, f1, f2, &aa[((size_t)f1 * kMaxFF + f2) * K]);
+        for (int f = 0; f < kMaxFF; ++f) push_row(1, f, 0, &aw[(size_t)f * K]);
+        push_row(2, 0, 0, ww.data());
+    }
+    const int rows = (int)row_keys_.size(), nq = (int)q.size();
+    cached_sinqd_.assign((size_t)rows * nq, 0.0);
+    cached_q_ = q;
+    if (rows == 0 || nq == 0) return cached_sinqd_;
+
+    Context& ctx = Context::get();
+    static DevBuf<double> dH, dd, dq, dout;
+    static DevBuf<char> scratch;
+    dH.upload(H.data(), H.size(), ctx.stream);
+    dd.upload(d_axis.data(), d_axis.size(), ctx.stream);
+    dq.upload(q.data(), q.size(), ctx.stream);
+    dout.ensure((size_t)rows * nq);
+    scratch.ensure(debye_scratch_bytes(rows, K, nq));
+    DebyeLaunch dl{dH.p, dd.p, dq.p, rows, K, nq, debye_terms, dout.p, scratch.p};
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
+    check(debye_terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
+    check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
+    check(hipMemcpyAsync(cached_sinqd_.data(), dout.p, cached_sinqd_.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "D2H");
+    ctx.sync();
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    last_debye_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return cached_sinqd_;
+}
+
+std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::vector<double>& q,
+                                                                   std::vector<double>* profiles) {
+    const std::vector<double>& S = sinqd(q);
+    const int nq = (int)q.size();
+    std::vector<double> prof((size_t)6 * nq, 0.0);
+    double* aa_ = prof.data();
+    double* ax_ = aa_ + nq;
+    double* aw_ = aa_ + 2 * nq;
+    double* xx_ = aa_ + 3 * nq;
+    double* wx_ = aa_ + 4 * nq;
+    double* ww_ = aa_ + 5 * nq;
+    const FreeParams& p = params;
+    if (mode == Mode::Simple) {
+        // DistanceHistogram::debye_transform (DistanceHistogram.cpp:54-87) with
+        // p = aa + k*aw + k^2*ww (CompositeDistanceHistogram.cpp:55-57)
+        for (size_t r = 0; r < row_keys_.size(); ++r) {
+            const int kind = row_keys_[r][0];
+            double* dst = kind == 0 ? aa_ : kind == 1 ? aw_ : ww_;
+            for (int k = 0; k < nq; ++k) dst[k] += S[r * nq + k] * std::exp(-q[k] * q[k]);
+        }
+        std::vector<double> I(nq);
+        for (int k = 0; k < nq; ++k) I[k] = aa_[k] + p.cw * aw_[k] + p.cw * p.cw * ww_[k];
+        if (profiles) *profiles = prof;
+        return I;
+    }
+    const double rm2 = 1.62 * 1.62 / 4;  // FFExplicitBase.cpp:54-57
+    for (int k = 0; k < nq; ++k) {
+        const double qk = q[k];
+        const double cx = mode == Mode::FFAvg ? p.cx : std::pow(p.cx, 3) * std::exp(-rm2 * (p.cx * p.cx - 1) * qk * qk);
+        const double fw = ff_atomic(kWaterBin, qk);
+        double fa[kMaxFF], fe[kMaxFF];
+        for (int f = 0; f < kMaxFF; ++f) fa[f] = ff_atomic(f, qk);
+        for (int f = 1; f < kMaxFF; ++f) fe[f] = ff_exv(f, qk);
+        for (size_t r = 0; r < row_keys_.size(); ++r) {
+            const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
+            const double s = S[r * nq + k];
+            if (mode == Mode::FFAvg) {
+                // FFAvgBase.cpp:470-558 with the raw_atomic_table products
+                if (kind == 0) {
+                    if (f1 >= 1 && f2 >= 1) aa_[k] += s * fa[f1] * fa[f2];
+                    else if (f1 >= 1 && f2 == kExvBin) ax_[k] += 2 * p.crho * cx * s * fa[f1] * fa[kExvBin];
+                    else if (f1 == kExvBin && f2 == kExvBin) xx_[k] += std::pow(cx * p.crho, 2) * s * fa[0] * fa[0];
+                } else if (kind == 1) {
+                    if (f1 >= 1) aw_[k] += 2 * p.cw * s * fa[f1] * fw;
+                    else wx_[k] += 2 * p.crho * cx * p.cw * s * fa[kExvBin] * fw;
+                } else {
+                    ww_[k] += p.cw * p.cw * s * fw * fw;
+                }
+            } else {
+                // FFExplicitBase.cpp:105-212: raw_atomic, raw_cross (atomic x exv), raw_exv tables
+                if (kind == 0) {
+                    if (f1 < 1 || f2 < 1) continue;
+                    aa_[k] += s * fa[f1] * fa[f2];
+                    ax_[k] += p.crho * cx * s * (fa[f1] * fe[f2] + fa[f2] * fe[f1]);
+                    xx_[k] += std::pow(cx * p.crho, 2) * s * fe[f1] * fe[f2];
+                } else if (kind == 1) {
+                    if (f1 < 1) continue;
+                    aw_[k] += 2 * p.cw * s * fa[f1] * fw;
+                    wx_[k] += p.crho * cx * p.cw * s * (fw * fe[f1] + fa[f1] * fe[kWaterBin]);
+                } else {
+                    ww_[k] += p.cw * p.cw * s * fw * fw;
+                }
+            }
+        }
+        if (p.dw_atomic != 0 || p.dw_exv != 0) {  // FFAvgBase.cpp:310-374
+            const double ba = std::exp(-qk * qk * p.dw_atomic * p.dw_atomic * 0.5);
+            const double bx = std::exp(-qk * qk * p.dw_exv * p.dw_exv * 0.5);
+            aa_[k] *= ba * ba;
+            xx_[k] *= bx * bx;
+            ax_[k] *= ba * bx;
+            aw_[k] *= ba;
+            wx_[k] *= bx;
+        }
+    }
+    std::vector<double> I(nq);
+    for (int k = 0; k < nq; ++k) I[k] = aa_[k] - ax_[k] + aw_[k] + xx_[k] - wx_[k] + ww_[k];
+    if (profiles) *profiles = prof;
+    return I;
+}
+
+// ---------------------------------------------------------------------------
+// raw pair set
+// ---------------------------------------------------------------------------
+void raw_pair_histogram(const float* a, int n, const float* b, int m, const HistSettings& s, uint64_t* count,
+                        double* dsum) {
+    const float inv_w = inv_width(s);
+    PointSet A, B;
+    A.orig = to_points(a, n);
+    std::vector<PointSet> sets;
+    PairJob job;
+    job.row_set = 0;
+    job.rowmap = {0};
+    if (b) {
+        B.orig = to_points(b, m);
+        job.col_set = 1;
+    }
+    const int nbins = (int)s.bin_count;
+    sets.push_back(std::move(A));
+    if (b) sets.push_back(std::move(B));
+    PairEngine eng(std::move(sets), {job}, 1, inv_w, nbins, s.tile_j);
+    std::vector<uint64_t> c;
+    std::vector<int64_t> ds;
+    eng.run(c, ds);
+    if (eng.overflowed(c)) throw std::runtime_error("ausaxs_hip: pair distance beyond bin_count*bin_width");
+    for (int k = 0; k < nbins; ++k) {
+        count[k] = c[k];
+        dsum[k] = ((double)c[k] * k + (double)ds[k] * (1.0 / 65536.0)) / (double)inv_w;
+    }
+}
+
+}  // namespace ausaxs::gpu

@@ -1,0 +1,110 @@
+// Histogram managers and composite histograms of the MI355X Debye path.
+//
+// Mirrors the reference's seam (IHistogramManager::calculate_all returning an
+// ICompositeDistanceHistogram, IHistogramManager.h:16-29) for the managers
+//   HistogramManagerMTFFAvg       (HistogramManagerMTFFAvg.cpp:194-371)
+//   HistogramManagerMTFFExplicit  (HistogramManagerMTFFExplicit.cpp:216-436)
+//   HistogramManagerMT            (HistogramManagerMT.cpp:23-75)
+// with the pair work on the GPU (engine.hpp) and the O(rows x bins)
+// bookkeeping (self terms, p_tot, max_bin, exv rows) on the host in FP64.
+#pragma once
+
+#include "engine.hpp"
+
+#include <array>
+#include <memory>
+#include <vector>
+
+namespace ausaxs::gpu {
+
+constexpr int kMaxFF = 15;  // settings/FormFactorSettings.h:12
+constexpr int kExvBin = 0;  // form_factor/FormFactorType.h:35
+constexpr int kWaterBin = 1;
+
+struct WeightedEntry {  // distribution/detail/WeightedEntry.h:16-50
+    double value = 0;
+    uint32_t count = 0;
+    double bin_center = 0;
+};
+
+struct HistSettings {
+    double bin_width = 0.25;    // settings::axes::bin_width (d_axis width)
+    uint32_t bin_count = 8000;  // settings::axes::bin_count
+    bool weighted_bins = true;  // settings::hist::weighted_bins
+    int debye_terms = 3;        // bf16 terms per operand; 0 = FP64 contraction
+    int tile_j = 8192;          // columns per work item
+};
+// WidthControllers.h:10-27: inverse width as float
+float inv_width(const HistSettings& s);
+
+struct FreeParams {  // CompositeDistanceHistogramFFAvgBase free parameters
+    double cw = 1, cx = 1, crho = 1, dw_atomic = 0, dw_exv = 0;
+};
+
+enum class Mode : int { FFAvg = 0, FFExplicit = 1, Simple = 2 };
+
+// form factors restated from FormFactor.h:33-39 / FormFactorTable.h:28-120 /
+// ExvFormFactor.h:16-42 (default MinimumFluctuation_implicit_H volumes)
+double ff_atomic(int type, double q);
+double ff_exv(int type, double q);
+
+// The reference's default q axis and its sub-axis between qmin and qmax
+// (ConstantsAxes.h:14, Axis.cpp:40-59).
+std::vector<double> default_q_axis(double qmin, double qmax);
+
+class CompositeDistanceHistogramHIP {
+  public:
+    Mode mode = Mode::FFExplicit;
+    unsigned max_bin = 0;
+    double bin_width = 0.25;
+    bool weighted_bins = true;
+    int debye_terms = 3;
+    // FF modes: aa [15][15][max_bin], aw [15][max_bin], ww [max_bin]
+    // Simple:   aa, aw, ww [max_bin]
+    std::vector<WeightedEntry> aa, aw, ww, p_tot;
+    std::vector<double> d_axis;
+    FreeParams params;
+
+    // ICompositeDistanceHistogram(Exv)::debye_transform on an explicit q grid;
+    // profiles (optional, [6][nq]) = {aa, ax, aw, xx, wx, ww}.
+    std::vector<double> debye_transform(const std::vector<double>& q, std::vector<double>* profiles = nullptr);
+    // sinqd rows on the q grid, computed on the GPU (cached per q grid)
+    const std::vector<double>& sinqd(const std::vector<double>& q);
+
+    double last_debye_ms = 0;
+
+  private:
+    std::vector<double> cached_q_, cached_sinqd_;
+    std::vector<std::array<int, 3>> row_keys_;  // (kind, f1, f2): kind 0 aa, 1 aw, 2 ww
+};
+
+// Resident structure: coordinates uploaded and work lists built once;
+// calculate() is HistogramManager*::calculate_all on the resident data.
+class ResidentStructure {
+  public:
+    // atoms: n x 4 floats {x,y,z, ff (int bits) | weight}; waters likewise.
+    ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                      const HistSettings& s, double z_exv);
+    std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
+    PairEngine& engine() { return *engine_; }
+    Mode mode() const { return mode_; }
+
+  private:
+    Mode mode_;
+    HistSettings s_;
+    double z_exv_;
+    int n_atoms_, n_waters_;
+    std::vector<int32_t> atom_ff_;                // FF modes
+    std::vector<float> atom_w_, water_w_;         // Simple mode
+    std::vector<float> atom_class_w_, water_class_w_;
+    std::vector<std::array<int, 3>> rows_;        // global row -> (kind, a, b)
+    std::unique_ptr<PairEngine> engine_;
+    std::vector<uint64_t> cnt_;
+    std::vector<int64_t> dsum_;
+};
+
+// single pair set with the reference's loop geometry (test entry)
+void raw_pair_histogram(const float* a, int n, const float* b, int m, const HistSettings& s, uint64_t* count,
+                        double* dsum);
+
+}  // namespace ausaxs::gpu

@@ -1,0 +1,139 @@
+/*
+ * ausaxs_hip.h — C ABI of the MI355X-native AUSAXS Debye hot path.
+ *
+ * Plain C: pointers and sizes only. Every function returns 0 on success and a
+ * non-zero status on failure; the message of the last failure on the calling
+ * thread is available from ausaxs_get_last_error. This mirrors the reference's
+ * C API conventions (source/api/api_helper.h:15-45: status out-parameter plus
+ * get_last_error_msg), with the status as the return value.
+ *
+ * Point arrays are n x 4 floats in the reference's CompactCoordinates layout
+ * (hist/detail/data/CompactCoordinatesXYZFF.h:107-169, ...XYZW.h:106-165):
+ *   XYZFF: {x, y, z, form-factor type as int32 bits}   (FF managers)
+ *   XYZW : {x, y, z, weight}                           (plain manager)
+ * Coordinates are the FP64 atom coordinates cast to float by the caller, as
+ * the reference's CompactCoordinates constructors do.
+ */
+#ifndef AUSAXS_HIP_H
+#define AUSAXS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* manager choices (settings/HistogramSettings.h:23-42) */
+#define AUSAXS_MODE_FFAVG 0      /* HistogramManagerMTFFAvg       */
+#define AUSAXS_MODE_FFEXPLICIT 1 /* HistogramManagerMTFFExplicit  */
+#define AUSAXS_MODE_SIMPLE 2     /* HistogramManagerMT (XYZW)     */
+
+/* which distribution ausaxs_histogram_get returns */
+#define AUSAXS_AA 0
+#define AUSAXS_AW 1
+#define AUSAXS_WW 2
+#define AUSAXS_PTOT 3
+
+typedef struct {
+    double bin_width;  /* settings::axes::bin_width (default 0.25)  */
+    int bin_count;     /* settings::axes::bin_count (default 8000)  */
+    int weighted_bins; /* settings::hist::weighted_bins (default 1) */
+    int debye_terms;   /* bf16 terms per GEMM operand: 2 = hi+lo, 3 = hi+mid+lo (default); 0 = FP64 */
+    int tile_j;        /* columns per pair work item (default 8192)  */
+} ausaxs_settings;
+
+typedef struct ausaxs_structure ausaxs_structure; /* resident coordinates + work lists */
+typedef struct ausaxs_histogram ausaxs_histogram; /* ICompositeDistanceHistogram(Exv)  */
+
+void ausaxs_default_settings(ausaxs_settings* s);
+int ausaxs_get_last_error(char* buf, int len);
+int ausaxs_device_info(char* name, int len, int* compute_units, size_t* lds_per_block);
+
+/* Upload coordinates and build the pair work lists once.
+ * Replaces the construction half of HistogramManagerMTFFAvg/FFExplicit/MT
+ * (HistogramManagerMTFFAvg.cpp:203-208: CompactCoordinatesFF of bodies and waters).
+ * z_exv: FFAvg only, V_grid * 0.334 / N_atoms (HistogramManagerMTFFAvg.cpp:355-359). */
+int ausaxs_structure_create(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                            const ausaxs_settings* settings, double z_exv, ausaxs_structure** out);
+
+/* IHistogramManager::calculate_all (IHistogramManager.h:16-29) on resident data:
+ * HistogramManagerMTFFAvg.cpp:194-371, HistogramManagerMTFFExplicit.cpp:216-436,
+ * HistogramManagerMT.cpp:23-75. */
+int ausaxs_structure_calculate(ausaxs_structure* s, ausaxs_histogram** out);
+int ausaxs_structure_free(ausaxs_structure* s);
+
+/* One-shot manager: create + calculate + free the structure. */
+int ausaxs_hist_calculate(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
+                          const ausaxs_settings* settings, double z_exv, ausaxs_histogram** out);
+
+/* Shape of a composite histogram: max_bin after truncation
+ * (HistogramManagerMTFFAvg.cpp:340-352). */
+int ausaxs_histogram_info(const ausaxs_histogram* h, int* mode, int* max_bin);
+
+/* Reference-layout distributions (WeightedEntry fields split into arrays).
+ * FF modes: AA is [15][15][max_bin] (Container3D.h:48), AW is [15][max_bin],
+ * WW and PTOT are [max_bin]. SIMPLE mode: all are [max_bin].
+ * Any of value/count/bin_center may be NULL. */
+int ausaxs_histogram_get(const ausaxs_histogram* h, int which, double* value, uint32_t* count, double* bin_center);
+
+/* Weighted distance axis (WeightedDistribution1D::get_weighted_axis,
+ * WeightedDistribution1D.cpp:51-58, with d[0] = 0 per DistanceHistogram.cpp:44-47). */
+int ausaxs_histogram_d_axis(const ausaxs_histogram* h, double* d);
+
+/* apply_water_scaling_factor, apply_excluded_volume_scaling_factor,
+ * apply_solvent_density_scaling_factor, apply_{atomic,exv}_debye_waller_factor
+ * (ICompositeDistanceHistogramExv.h:12-107). */
+int ausaxs_histogram_set_params(ausaxs_histogram* h, double cw, double cx, double crho, double dw_atomic,
+                                double dw_exv);
+
+/* debye_transform on an explicit q grid (DistanceHistogram.cpp:70-87 for the
+ * plain histogram; CompositeDistanceHistogramFFAvgBase.cpp:57-76 for the
+ * composite ones). I: [nq]. profiles (optional): [6][nq] = aa, ax, aw, xx, wx, ww. */
+int ausaxs_histogram_debye(ausaxs_histogram* h, const double* q, int nq, double* I, double* profiles);
+
+int ausaxs_histogram_free(ausaxs_histogram* h);
+
+/* The reference's default q sub-axis (ConstantsAxes.h:14, Axis::sub_axis,
+ * Axis.cpp:40-59). q must hold 200 doubles; *nq receives the length. */
+int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq);
+
+/* ---- kernel-level entries (tests, tools) -------------------------------- */
+
+/* One pair set with the reference's loop geometry (self if b == NULL, else
+ * rows a x columns b): per-bin pair counts and summed distances. Replaces the
+ * evaluate/increment loop of SimpleCalculator.h:94-215 for one set. */
+int ausaxs_pair_histogram(const float* a, int n, const float* b, int m, double bin_width, int bin_count,
+                          uint64_t* count, double* dsum);
+
+/* Debye contraction out[r][k] = sum_b H[r][b] sinc(q_k d_b) on the device
+ * (cache_refresh_sinqd, CompositeDistanceHistogramFFAvgBase.cpp:427-467).
+ * terms: 2/3 = bf16-split MFMA, 0 = FP64. */
+int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, const double* q, int nq, int terms,
+                          double* out);
+
+/* Exhaustive device check of the binning arithmetic over float d2 bit
+ * patterns [lo_bits, hi_bits]: mismatches[0] = fast-path bins that differ
+ * from the correctly rounded rule, mismatches[1] = device sqrt results that
+ * are not correctly rounded. */
+int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsigned long long* mismatches);
+
+/* HIP-event timings of the last ausaxs_structure_calculate with timing on. */
+int ausaxs_structure_set_timing(ausaxs_structure* s, int on);
+int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
+                       int* n_items);
+
+/* ---- synthetic structures (deterministic, xoshiro256** seeded by SplitMix64) */
+
+/* Self-avoiding 3.8 A chain packed in an ellipsoid with semi-axes (ax, ay, az),
+ * minimum non-bonded distance r_excl; elements C 62% / N 17% / O 20% / S 1%
+ * written as form-factor types (C=3, N=7, O=11, S=12) in the fourth slot. */
+int ausaxs_synth_globule(uint64_t seed, int n, double ax, double ay, double az, double r_excl, float* out);
+/* m waters on the ellipsoid surface pushed out by offset along the normal;
+ * fourth slot = WATER type (1). */
+int ausaxs_synth_shell(uint64_t seed, int m, double ax, double ay, double az, double offset, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

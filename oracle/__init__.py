@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker. Imported by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg — never by the
+product (ausaxs_amd).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+FFAVG, FFEXPLICIT = 0, 1
+MAX_FF = 15
+
+ENTRY = np.dtype([("value", np.float64), ("count", np.uint32), ("bin_center", np.float64)], align=True)
+assert ENTRY.itemsize == 24
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def _load():
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "orc_bin_pair": (None, [vp, vp, C.c_float, i, vp, vp]),
+        "orc_rule_self": (i, [i, i, i]),
+        "orc_rule_cross": (i, [i, i]),
+        "orc_hist_ff": (i, [i, vp, i, vp, i, C.c_float, i, d, i, vp, vp, vp, vp]),
+        "orc_pair_hist": (i, [vp, i, vp, i, C.c_float, i, i, vp]),
+        "orc_hist_simple": (i, [vp, i, vp, i, C.c_float, i, i, vp, vp, vp, vp]),
+        "orc_weighted_axis": (None, [vp, i, d, vp]),
+        "orc_sinc_table": (None, [vp, i, vp, i, vp]),
+        "orc_contract": (None, [vp, i, i, vp, i, vp]),
+        "orc_contract_ld": (None, [vp, i, i, vp, i, vp]),
+        "orc_ff_atomic": (d, [i, d]),
+        "orc_ff_exv": (d, [i, d]),
+        "orc_total": (None, [vp, i, vp]),
+        "orc_exact_debye": (None, [vp, i, vp, i, vp]),
+        "orc_lsq": (None, [vp, vp, vp, i, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+
+    class Params(C.Structure):
+        _fields_ = [(k, d) for k in ("cw", "cx", "crho", "dw_atomic", "dw_exv")]
+
+    lib.orc_profiles.restype = None
+    lib.orc_profiles.argtypes = [i, vp, i, vp, vp, vp, Params, vp]
+    lib.Params = Params
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def inv_width(bin_width: float) -> np.float32:
+    return np.float32(1.0 / bin_width)
+
+
+def bin_pair(a, b, bin_width, rule):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    d = np.zeros(1, np.float32)
+    k = np.zeros(1, np.int32)
+    lib.orc_bin_pair(_p(a), _p(b), C.c_float(inv_width(bin_width)), rule, _p(d), _p(k))
+    return float(d[0]), int(k[0])
+
+
+def hist_ff(mode, atoms, waters, bin_width=0.25, bin_count=8000, z_exv=1.0, nthreads=1):
+    """HistogramManagerMTFF{Avg,Explicit}::calculate_all -> dict of entry arrays (full bin_count) + max_bin."""
+    atoms = np.ascontiguousarray(atoms, np.float32)
+    waters = np.ascontiguousarray(waters if waters is not None else np.zeros((0, 4), np.float32), np.float32)
+    aa = np.zeros(MAX_FF * MAX_FF * bin_count, ENTRY)
+    aw = np.zeros(MAX_FF * bin_count, ENTRY)
+    ww = np.zeros(bin_count, ENTRY)
+    pt = np.zeros(bin_count, ENTRY)
+    mb = lib.orc_hist_ff(mode, _p(atoms), len(atoms), _p(waters), len(waters), C.c_float(inv_width(bin_width)),
+                         bin_count, z_exv, nthreads, _p(aa), _p(aw), _p(ww), _p(pt))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    return dict(
+        max_bin=mb,
+        aa=aa.reshape(MAX_FF, MAX_FF, bin_count)[:, :, :mb],
+        aw=aw.reshape(MAX_FF, bin_count)[:, :mb],
+        ww=ww[:mb],
+        p_tot=pt[:mb],
+    )
+
+
+def hist_simple(atoms, waters, bin_width=0.25, bin_count=8000, nthreads=1):
+    atoms = np.ascontiguousarray(atoms, np.float32)
+    waters = np.ascontiguousarray(waters if waters is not None else np.zeros((0, 4), np.float32), np.float32)
+    out = [np.zeros(bin_count, ENTRY) for _ in range(4)]
+    mb = lib.orc_hist_simple(_p(atoms), len(atoms), _p(waters), len(waters), C.c_float(inv_width(bin_width)),
+                             bin_count, nthreads, *map(_p, out))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
+
+
+def pair_hist(a, b=None, bin_width=0.25, bin_count=8000, factor=1):
+    a = np.ascontiguousarray(a, np.float32)
+    bb = None if b is None else np.ascontiguousarray(b, np.float32)
+    out = np.zeros(bin_count, ENTRY)
+    rc = lib.orc_pair_hist(_p(a), len(a), _p(bb), 0 if bb is None else len(bb), C.c_float(inv_width(bin_width)),
+                           bin_count, factor, _p(out))
+    if rc:
+        raise RuntimeError(f"oracle status {rc}")
+    return out
+
+
+def weighted_axis(p, bin_width):
+    p = np.ascontiguousarray(p, ENTRY)
+    d = np.zeros(len(p), np.float64)
+    lib.orc_weighted_axis(_p(p), len(p), bin_width, _p(d))
+    return d
+
+
+def sinc_table(q, d):
+    q = np.ascontiguousarray(q, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    out = np.zeros((len(q), len(d)), np.float64)
+    lib.orc_sinc_table(_p(q), len(q), _p(d), len(d), _p(out))
+    return out
+
+
+def contract(H, S, long_double=False):
+    H = np.ascontiguousarray(H, np.float64)
+    S = np.ascontiguousarray(S, np.float64)
+    out = np.zeros((H.shape[0], S.shape[0]), np.float64)
+    f = lib.orc_contract_ld if long_double else lib.orc_contract
+    f(_p(H), H.shape[0], H.shape[1], _p(S), S.shape[0], _p(out))
+    return out
+
+
+def ff_atomic(t, q):
+    return lib.orc_ff_atomic(t, q)
+
+
+def ff_exv(t, q):
+    return lib.orc_ff_exv(t, q)
+
+
+def profiles(mode, q, sinqd_aa, sinqd_aw, sinqd_ww, cw=1.0, cx=1.0, crho=1.0, dw_atomic=0.0, dw_exv=0.0):
+    q = np.ascontiguousarray(q, np.float64)
+    nq = len(q)
+    a3 = np.ascontiguousarray(sinqd_aa, np.float64).reshape(MAX_FF * MAX_FF * nq)
+    a2 = np.ascontiguousarray(sinqd_aw, np.float64).reshape(MAX_FF * nq)
+    a1 = np.ascontiguousarray(sinqd_ww, np.float64).reshape(nq)
+    out = np.zeros((6, nq), np.float64)
+    lib.orc_profiles(mode, _p(q), nq, _p(a3), _p(a2), _p(a1), lib.Params(cw, cx, crho, dw_atomic, dw_exv), _p(out))
+    return out
+
+
+def total(prof):
+    prof = np.ascontiguousarray(prof, np.float64)
+    nq = prof.shape[1]
+    out = np.zeros(nq, np.float64)
+    lib.orc_total(_p(prof), nq, _p(out))
+    return out
+
+
+def exact_debye(xyzw, q):
+    xyzw = np.ascontiguousarray(xyzw, np.float32)
+    q = np.ascontiguousarray(q, np.float64)
+    out = np.zeros(len(q), np.float64)
+    lib.orc_exact_debye(_p(xyzw), len(xyzw), _p(q), len(q), _p(out))
+    return out
+
+
+def lsq(x, y, err=None):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    e = None if err is None else np.ascontiguousarray(err, np.float64)
+    out = np.zeros(4, np.float64)
+    lib.orc_lsq(_p(x), _p(y), _p(e), len(x), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# full FF pipeline: histograms -> weighted axis -> sinc -> sinqd -> profiles
+# ---------------------------------------------------------------------------
+def debye_ff(mode, hist, q, bin_width, weighted=True, long_double=False, **params):
+    """I(q) and the six profiles from an oracle FF histogram (FP64 throughout)."""
+    mb = hist["max_bin"]
+    if weighted:
+        d = weighted_axis(hist["p_tot"], bin_width)
+    else:
+        d = np.arange(mb) * ((mb * bin_width) / mb)
+        d[0] = 0
+    S = sinc_table(q, d)
+    nq = len(q)
+    aa = contract(hist["aa"]["value"].reshape(-1, mb), S, long_double).reshape(MAX_FF, MAX_FF, nq)
+    aw = contract(hist["aw"]["value"].reshape(-1, mb), S, long_double).reshape(MAX_FF, nq)
+    ww = contract(hist["ww"]["value"].reshape(1, mb), S, long_double).reshape(nq)
+    prof = profiles(mode, q, aa, aw, ww, **params)
+    return total(prof), prof

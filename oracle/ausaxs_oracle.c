@@ -1,0 +1,728 @@
+/*
+ * ausaxs_oracle.c — CPU restatement of AUSAXS's Debye hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker. Only tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg may load it; the product never does.
+ *
+ * Build: see oracle/Makefile (IEEE FP32: -ffp-contract=off, no fast-math, so
+ * the SIMD-body rule keeps its separately rounded products and sums and the
+ * scalar rule gets its explicit fmaf).
+ *
+ * Parity pins (tests/test_oracle.py):
+ *   - binning rules: lattice inputs with closed-form sqrt(k) distances,
+ *     brute force over float d2 ranges, hand-derived ties.
+ *   - manager conventions: 8-atom cube closed form (histogram_manager.cpp:30-146),
+ *     partial-sum invariants, count totals n(n-1)/2.
+ *   - sinc/Debye: closed forms of the cube I(q) (composite_distance_histogram_ff_avg.cpp:23-190),
+ *     exact Debye vs binned Debye, carbon_sphere_waxs golden curve.
+ *   - form factors: I0 = sum(a)+c closed form, q-monotonicity, ExvFormFactor closed form.
+ */
+#include "ausaxs_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* binning rules                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* SIMD body: CompactCoordinatesXYZFF.h:514-545 (AVX2), :397-420 (SSE),
+ * CompactCoordinatesXYZW.h:491-535. d = this - other, squares, (x+y)+z,
+ * sqrt, mul by inv_width, cvtps2dq (round-half-even). */
+static inline void rule_simd(const float* a, const float* b, float inv_w, float* d_out, int32_t* bin_out) {
+    float dx = a[0] - b[0];
+    float dy = a[1] - b[1];
+    float dz = a[2] - b[2];
+    float sx = dx * dx;
+    float sy = dy * dy;
+    float sz = dz * dz;
+    float d2 = (sx + sy) + sz;
+    float d = sqrtf(d2);
+    float x = d * inv_w;
+    *d_out = d;
+    *bin_out = (int32_t)nearbyintf(x);
+}
+
+/* Scalar tail: evaluate_scalar (CompactCoordinatesXYZFF.h:313-329,
+ * CompactCoordinatesXYZW.h:303-315) as compiled with the reference's release
+ * flags: FMA-contracted squared distance and std::round lowered to
+ * trunc(x + copysign(0.49999997f, x)). */
+static inline void rule_scalar(const float* a, const float* b, float inv_w, float* d_out, int32_t* bin_out) {
+    float dx = a[0] - b[0];
+    float dy = a[1] - b[1];
+    float dz = a[2] - b[2];
+    float d2 = fmaf(dz, dz, fmaf(dx, dx, dy * dy));
+    float d = sqrtf(d2);
+    float x = d * inv_w;
+    *d_out = d;
+    *bin_out = (int32_t)truncf(x + copysignf(0.49999997f, x));
+}
+
+void orc_bin_pair(const float* a, const float* b, float inv_w, int rule, float* d_out, int32_t* bin_out) {
+    if (rule) rule_scalar(a, b, inv_w, d_out, bin_out);
+    else rule_simd(a, b, inv_w, d_out, bin_out);
+}
+
+/* Loop geometry of every self loop in the reference (e.g.
+ * HistogramManagerMTFFAvg.cpp:214-234): j = i+1, then blocks of 16, 8, 4 while
+ * they fit, then evaluate1 for the rest: the last (n-i-1) mod 4 columns. */
+int orc_rule_self(int n, int i, int j) {
+    int tail_start = n - ((n - i - 1) % 4);
+    return j >= tail_start;
+}
+
+/* Cross loops (e.g. HistogramManagerMTFFAvg.cpp:237-257): j from 0 over m
+ * columns; the last m mod 4 use evaluate1. */
+int orc_rule_cross(int m, int j) {
+    return j >= m - (m % 4);
+}
+
+/* ------------------------------------------------------------------------ */
+/* weighted entries (WeightedEntry.h:64-81)                                 */
+/* ------------------------------------------------------------------------ */
+
+static inline void inc_n(orc_entry* e, int n, float d) {
+    e->count += (uint32_t)n;
+    e->bin_center += (double)((float)n * d);
+    e->value += n;
+}
+
+static inline void add_n(orc_entry* e, int n, float d, float w) {
+    e->count += (uint32_t)n;
+    e->bin_center += (double)((float)n * d);
+    e->value += n * (double)w;
+}
+
+static inline void entry_add(orc_entry* a, const orc_entry* b) {
+    a->value += b->value;
+    a->count += b->count;
+    a->bin_center += b->bin_center;
+}
+
+/* WeightedEntry operator*(entry, double) (WeightedEntry.h:113-115): the count
+ * goes through double and is truncated back to unsigned. */
+static inline void entry_scale(orc_entry* e, double f) {
+    e->value *= f;
+    e->count = (uint32_t)(f * e->count);
+    e->bin_center *= f;
+}
+
+static inline int ff_of(const float* p) {
+    int32_t v;
+    memcpy(&v, &p[3], 4);
+    return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* thread pool over 800-row tasks (GeneralSettings.cpp:50)                  */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_JOB 800
+
+typedef struct job_ctx job_ctx;
+typedef int (*row_fn)(job_ctx* c, void* priv, int i);
+
+struct job_ctx {
+    int n_rows;
+    atomic_int next;
+    atomic_int err;
+    row_fn fn;
+    void* args;
+    void** privs;
+    int tid_counter;
+    pthread_mutex_t mu;
+};
+
+typedef struct {
+    job_ctx* c;
+    int tid;
+} worker_arg;
+
+static void* worker(void* p) {
+    worker_arg* wa = (worker_arg*)p;
+    job_ctx* c = wa->c;
+    void* priv = c->privs[wa->tid];
+    for (;;) {
+        int start = atomic_fetch_add(&c->next, ORC_JOB);
+        if (start >= c->n_rows) break;
+        int end = start + ORC_JOB < c->n_rows ? start + ORC_JOB : c->n_rows;
+        for (int i = start; i < end; ++i) {
+            int e = c->fn(c, priv, i);
+            if (e) atomic_store(&c->err, e);
+        }
+    }
+    return NULL;
+}
+
+static int run_rows(int n_rows, int nthreads, row_fn fn, void* args, void** privs) {
+    job_ctx c;
+    c.n_rows = n_rows;
+    atomic_init(&c.next, 0);
+    atomic_init(&c.err, 0);
+    c.fn = fn;
+    c.args = args;
+    c.privs = privs;
+    if (nthreads <= 1) {
+        worker_arg wa = {&c, 0};
+        worker(&wa);
+        return atomic_load(&c.err);
+    }
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    worker_arg* wa = (worker_arg*)malloc(sizeof(worker_arg) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        wa[t].c = &c;
+        wa[t].tid = t;
+        pthread_create(&th[t], NULL, worker, &wa[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(wa);
+    return atomic_load(&c.err);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FF managers                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int mode;
+    const float* a;
+    int na;
+    const float* w;
+    int nw;
+    float inv_w;
+    int nb;
+} ff_args;
+
+typedef struct {
+    orc_entry* aa; /* [15][15][nb] */
+    orc_entry* aw; /* [15][nb] */
+    orc_entry* ww; /* [nb] */
+} ff_priv;
+
+#define IDX3(f1, f2, b, nb) (((size_t)(f1) * ORC_MAX_FF + (f2)) * (size_t)(nb) + (b))
+#define IDX2(f, b, nb) ((size_t)(f) * (size_t)(nb) + (b))
+
+/* evaluate a block of k columns with one rule, then scatter — the shape of
+ * add16/add8/add4 + increment_* in the reference (TemplateHelperBase.h). */
+static int aa_row(job_ctx* c, void* pv, int i) {
+    ff_args* A = (ff_args*)c->args;
+    ff_priv* P = (ff_priv*)pv;
+    const float* a = A->a;
+    int n = A->na, nb = A->nb;
+    const float* pi = a + 4 * (size_t)i;
+    int fi = ff_of(pi);
+    int tail = n - ((n - i - 1) % 4);
+    float d[16];
+    int32_t bin[16];
+    int j = i + 1;
+    while (j < n) {
+        int k;
+        if (j < tail) {
+            k = (j + 15 < n) ? 16 : (j + 7 < n) ? 8 : 4;
+            for (int t = 0; t < k; ++t) rule_simd(pi, a + 4 * (size_t)(j + t), A->inv_w, &d[t], &bin[t]);
+        } else {
+            k = 1;
+            rule_scalar(pi, a + 4 * (size_t)j, A->inv_w, &d[0], &bin[0]);
+        }
+        for (int t = 0; t < k; ++t) {
+            if (bin[t] < 0 || bin[t] >= nb) return -2;
+            int fj = ff_of(a + 4 * (size_t)(j + t));
+            inc_n(&P->aa[IDX3(fi, fj, bin[t], nb)], 2, d[t]);
+            if (A->mode == ORC_FFAVG) {
+                /* HistogramManagerMTFFAvg.cpp:27-34: [ff_i][exv] and [exv][exv] */
+                inc_n(&P->aa[IDX3(fi, ORC_EXV, bin[t], nb)], 2, d[t]);
+                inc_n(&P->aa[IDX3(ORC_EXV, ORC_EXV, bin[t], nb)], 2, d[t]);
+            }
+        }
+        j += k;
+    }
+    return 0;
+}
+
+/* atoms (rows) x waters (columns): FFAvg :237-257, FFExplicit :260-280 */
+static int aw_row(job_ctx* c, void* pv, int i) {
+    ff_args* A = (ff_args*)c->args;
+    ff_priv* P = (ff_priv*)pv;
+    int m = A->nw, nb = A->nb;
+    const float* pi = A->a + 4 * (size_t)i;
+    int fi = ff_of(pi);
+    int tail = m - (m % 4);
+    for (int j = 0; j < m; ++j) {
+        float d;
+        int32_t bin;
+        if (j < tail) rule_simd(pi, A->w + 4 * (size_t)j, A->inv_w, &d, &bin);
+        else rule_scalar(pi, A->w + 4 * (size_t)j, A->inv_w, &d, &bin);
+        if (bin < 0 || bin >= nb) return -2;
+        inc_n(&P->aw[IDX2(fi, bin, nb)], 1, d);
+        if (A->mode == ORC_FFAVG) inc_n(&P->aw[IDX2(ORC_EXV, bin, nb)], 1, d); /* FFAvg.cpp:111-118 */
+    }
+    return 0;
+}
+
+static int ww_row(job_ctx* c, void* pv, int i) {
+    ff_args* A = (ff_args*)c->args;
+    ff_priv* P = (ff_priv*)pv;
+    int n = A->nw, nb = A->nb;
+    const float* pi = A->w + 4 * (size_t)i;
+    int tail = n - ((n - i - 1) % 4);
+    for (int j = i + 1; j < n; ++j) {
+        float d;
+        int32_t bin;
+        if (j < tail) rule_simd(pi, A->w + 4 * (size_t)j, A->inv_w, &d, &bin);
+        else rule_scalar(pi, A->w + 4 * (size_t)j, A->inv_w, &d, &bin);
+        if (bin < 0 || bin >= nb) return -2;
+        inc_n(&P->ww[bin], 2, d);
+    }
+    return 0;
+}
+
+static int max_bin_of(const orc_entry* p, int nb) {
+    /* HistogramManagerMTFFAvg.cpp:340-347 — compares `value` (operator==) */
+    for (int i = nb - 1; i >= 10; --i)
+        if (p[i].value != 0) return i + 1;
+    return 10;
+}
+
+int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int n_w,
+                float inv_w, int nb, double z_exv, int nthreads,
+                orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot) {
+    for (int i = 0; i < n_a; ++i) {
+        int f = ff_of(atoms + 4 * (size_t)i);
+        if (f < 1 || f >= ORC_MAX_FF) return -3; /* CompactCoordinatesFF.h:21-32 */
+    }
+    if (nthreads < 1) nthreads = 1;
+    size_t n3 = (size_t)ORC_MAX_FF * ORC_MAX_FF * nb, n2 = (size_t)ORC_MAX_FF * nb;
+    ff_priv* privs = (ff_priv*)calloc(nthreads, sizeof(ff_priv));
+    void** pp = (void**)malloc(sizeof(void*) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        privs[t].aa = (orc_entry*)calloc(n3, sizeof(orc_entry));
+        privs[t].aw = (orc_entry*)calloc(n2, sizeof(orc_entry));
+        privs[t].ww = (orc_entry*)calloc(nb, sizeof(orc_entry));
+        pp[t] = &privs[t];
+    }
+    ff_args A = {mode, atoms, n_a, waters, n_w, inv_w, nb};
+    int err = 0;
+    err |= run_rows(n_a, nthreads, aa_row, &A, pp);
+    err |= run_rows(n_a, nthreads, aw_row, &A, pp);
+    err |= run_rows(n_w, nthreads, ww_row, &A, pp);
+
+    /* ThreadLocalWrapper::merge (ThreadLocalWrapper.h:89-105) */
+    memset(aa, 0, n3 * sizeof(orc_entry));
+    memset(aw, 0, n2 * sizeof(orc_entry));
+    memset(ww, 0, nb * sizeof(orc_entry));
+    for (int t = 0; t < nthreads; ++t) {
+        for (size_t k = 0; k < n3; ++k) entry_add(&aa[k], &privs[t].aa[k]);
+        for (size_t k = 0; k < n2; ++k) entry_add(&aw[k], &privs[t].aw[k]);
+        for (int k = 0; k < nb; ++k) entry_add(&ww[k], &privs[t].ww[k]);
+        free(privs[t].aa);
+        free(privs[t].aw);
+        free(privs[t].ww);
+    }
+    free(privs);
+    free(pp);
+    if (err) return -2;
+
+    /* self correlations: FFAvg.cpp:308-323, FFExplicit.cpp:331-346 */
+    for (int i = 0; i < n_a; ++i) {
+        int f = ff_of(atoms + 4 * (size_t)i);
+        inc_n(&aa[IDX3(f, f, 0, nb)], 1, 0.0f);
+    }
+    if (mode == ORC_FFAVG) {
+        orc_entry e = {(double)n_a, (uint32_t)n_a, 0};
+        entry_add(&aa[IDX3(ORC_EXV, ORC_EXV, 0, nb)], &e);
+    }
+    {
+        orc_entry e = {(double)n_w, (uint32_t)n_w, 0};
+        entry_add(&ww[0], &e);
+    }
+
+    /* p_tot: FFAvg.cpp:325-337 (aa ff>=1 x ff>=1, aw ff>=1, ww) */
+    memset(ptot, 0, nb * sizeof(orc_entry));
+    for (int f1 = 1; f1 < ORC_MAX_FF; ++f1) {
+        for (int f2 = 1; f2 < ORC_MAX_FF; ++f2)
+            for (int b = 0; b < nb; ++b) entry_add(&ptot[b], &aa[IDX3(f1, f2, b, nb)]);
+        if (mode == ORC_FFEXPLICIT)
+            for (int b = 0; b < nb; ++b) entry_add(&ptot[b], &aw[IDX2(f1, b, nb)]);
+    }
+    if (mode == ORC_FFAVG)
+        for (int f1 = 1; f1 < ORC_MAX_FF; ++f1)
+            for (int b = 0; b < nb; ++b) entry_add(&ptot[b], &aw[IDX2(f1, b, nb)]);
+    for (int b = 0; b < nb; ++b) entry_add(&ptot[b], &ww[b]);
+    int max_bin = max_bin_of(ptot, nb);
+
+    /* Z_exv scaling of the exv rows: FFAvg.cpp:355-364 */
+    if (mode == ORC_FFAVG) {
+        for (int f = 1; f < ORC_MAX_FF; ++f)
+            for (int b = 0; b < max_bin; ++b) entry_scale(&aa[IDX3(f, ORC_EXV, b, nb)], z_exv);
+        for (int b = 0; b < max_bin; ++b) entry_scale(&aa[IDX3(ORC_EXV, ORC_EXV, b, nb)], z_exv * z_exv);
+        for (int b = 0; b < max_bin; ++b) entry_scale(&aw[IDX2(ORC_EXV, b, nb)], z_exv);
+    }
+    return max_bin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* raw pair histogram                                                       */
+/* ------------------------------------------------------------------------ */
+
+int orc_pair_hist(const float* a, int n, const float* b, int m, float inv_w, int nb, int factor, orc_entry* out) {
+    memset(out, 0, (size_t)nb * sizeof(orc_entry));
+    if (!b) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                float d;
+                int32_t bin;
+                orc_bin_pair(a + 4 * (size_t)i, a + 4 * (size_t)j, inv_w, orc_rule_self(n, i, j), &d, &bin);
+                if (bin < 0 || bin >= nb) return -2;
+                inc_n(&out[bin], factor, d);
+            }
+    } else {
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < m; ++j) {
+                float d;
+                int32_t bin;
+                orc_bin_pair(a + 4 * (size_t)i, b + 4 * (size_t)j, inv_w, orc_rule_cross(m, j), &d, &bin);
+                if (bin < 0 || bin >= nb) return -2;
+                inc_n(&out[bin], factor, d);
+            }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* simple (XYZW) manager                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const float* rows;
+    int nr;
+    const float* cols;
+    int nc;
+    float inv_w;
+    int nb;
+    int self;
+} simple_args;
+
+typedef struct {
+    orc_entry* h;
+} simple_priv;
+
+static int simple_row(job_ctx* c, void* pv, int i) {
+    simple_args* A = (simple_args*)c->args;
+    orc_entry* h = ((simple_priv*)pv)->h;
+    const float* pi = A->rows + 4 * (size_t)i;
+    int m = A->self ? A->nr : A->nc;
+    const float* cols = A->self ? A->rows : A->cols;
+    int j0 = A->self ? i + 1 : 0;
+    int tail = A->self ? m - ((m - i - 1) % 4) : m - (m % 4);
+    for (int j = j0; j < m; ++j) {
+        const float* pj = cols + 4 * (size_t)j;
+        float d;
+        int32_t bin;
+        if (j < tail) rule_simd(pi, pj, A->inv_w, &d, &bin);
+        else rule_scalar(pi, pj, A->inv_w, &d, &bin);
+        if (bin < 0 || bin >= A->nb) return -2;
+        /* TemplateHelperSimple.h:15-61: add<2>(bin, d, w_i*w_j) (float product) */
+        add_n(&h[bin], 2, d, pi[3] * pj[3]);
+    }
+    return 0;
+}
+
+static int run_simple(const float* rows, int nr, const float* cols, int nc, int self, float inv_w, int nb,
+                      int nthreads, orc_entry* out) {
+    simple_priv* privs = (simple_priv*)calloc(nthreads, sizeof(simple_priv));
+    void** pp = (void**)malloc(sizeof(void*) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        privs[t].h = (orc_entry*)calloc(nb, sizeof(orc_entry));
+        pp[t] = &privs[t];
+    }
+    simple_args A = {rows, nr, cols, nc, inv_w, nb, self};
+    int err = run_rows(nr, nthreads, simple_row, &A, pp);
+    memset(out, 0, (size_t)nb * sizeof(orc_entry));
+    for (int t = 0; t < nthreads; ++t) {
+        for (int b = 0; b < nb; ++b) entry_add(&out[b], &privs[t].h[b]);
+        free(privs[t].h);
+    }
+    free(privs);
+    free(pp);
+    if (self) {
+        /* SimpleCalculator.h:144-160: bin 0 += WeightedEntry(sum w^2, (unsigned) sum w^2, 0) */
+        double tw = 0;
+        for (int i = 0; i < nr; ++i) tw += (double)rows[4 * (size_t)i + 3] * rows[4 * (size_t)i + 3];
+        orc_entry e = {tw, (uint32_t)tw, 0};
+        entry_add(&out[0], &e);
+    }
+    return err;
+}
+
+int orc_hist_simple(const float* atoms, int n_a, const float* waters, int n_w, float inv_w, int nb, int nthreads,
+                    orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot) {
+    if (nthreads < 1) nthreads = 1;
+    int err = 0;
+    err |= run_simple(atoms, n_a, NULL, 0, 1, inv_w, nb, nthreads, aa);
+    err |= run_simple(waters, n_w, NULL, 0, 1, inv_w, nb, nthreads, ww);
+    /* enqueue_calculate_cross(data_a, data_w): rows over data_2 = waters,
+     * columns over data_1 = atoms (SimpleCalculator.h:188-211) */
+    err |= run_simple(waters, n_w, atoms, n_a, 0, inv_w, nb, nthreads, aw);
+    if (err) return -2;
+    for (int b = 0; b < nb; ++b) {
+        ptot[b] = aa[b];
+        entry_add(&ptot[b], &ww[b]);
+        entry_add(&ptot[b], &aw[b]);
+    }
+    return max_bin_of(ptot, nb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* weighted axis, sinc table, contraction                                   */
+/* ------------------------------------------------------------------------ */
+
+void orc_weighted_axis(const orc_entry* p, int n, double bin_width, double* d_out) {
+    /* Axis(0, n*bin_width, n).as_vector(): i*width with width = (n*bw)/n */
+    double width = (n * bin_width) / n;
+    for (int i = 0; i < n; ++i) {
+        double dv = 0 + i * width;
+        double bc = p[i].bin_center;
+        d_out[i] = ((double)(!bc) * dv + bc) / (double)((!p[i].count) + p[i].count);
+    }
+    if (n > 0) d_out[0] = 0;
+}
+
+void orc_sinc_table(const double* q, int nq, const double* d, int nd, double* out) {
+    const double tol = 1e-3, inv6 = 1. / 6, inv120 = 1. / 120;
+    for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < nd; ++j) {
+            double qd = q[i] * d[j];
+            double v;
+            if (qd < tol) {
+                double qd2 = qd * qd;
+                v = 1 - qd2 * inv6 + qd2 * qd2 * inv120;
+            } else {
+                v = sin(qd) / qd;
+            }
+            out[(size_t)i * nd + j] = v;
+        }
+}
+
+void orc_contract(const double* H, int rows, int nd, const double* S, int nq, double* out) {
+    for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < nq; ++i) {
+            double s = 0.0;
+            const double* h = H + (size_t)r * nd;
+            const double* t = S + (size_t)i * nd;
+            for (int b = 0; b < nd; ++b) s += h[b] * t[b];
+            out[(size_t)r * nq + i] = s;
+        }
+}
+
+void orc_contract_ld(const double* H, int rows, int nd, const double* S, int nq, double* out) {
+    for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < nq; ++i) {
+            long double s = 0.0L;
+            const double* h = H + (size_t)r * nd;
+            const double* t = S + (size_t)i * nd;
+            for (int b = 0; b < nd; ++b) s += (long double)h[b] * (long double)t[b];
+            out[(size_t)r * nq + i] = (double)s;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* form factors                                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double a[5], b[5], c;
+} ff5;
+
+#define PI_ 3.14159265358979323846
+/* FormFactorTable.h:15-23: b given in s units, converted by 1/(16 pi^2) */
+#define S2Q (1. / (4 * 4 * PI_ * PI_))
+
+/* coefficient sets: FormFactorTable.h:28-120 */
+static const ff5 FF_H = {{0.489918, 0.262003, 0.196767, 0.049879, 0}, {20.6593, 7.74039, 49.5519, 2.20159, 0}, 0.001305};
+static const ff5 FF_O = {{2.960427, 2.508818, 0.637853, 0.722838, 1.142756}, {14.182259, 5.936858, 0.112726, 34.958481, 0.390240}, 0.027014};
+static const ff5 FF_N = {{11.893780, 3.277479, 1.858092, 0.858927, 0.912985}, {0.000158, 10.232723, 30.344690, 0.656065, 0.217287}, -11.804902};
+static const ff5 FF_S = {{6.362157, 5.154568, 1.473732, 1.635073, 1.209372}, {1.514347, 22.092528, 0.061373, 55.445176, 0.646925}, 0.154722};
+static const ff5 FF_C = {{2.657506, 1.078079, 1.490909, -4.241070, 0.713791}, {14.780758, 0.776775, 42.086843, -0.000294, 0.239535}, 4.297983};
+static const ff5 FF_AR = {{7.188004, 6.638454, 0.454180, 1.929593, 1.523654}, {0.956221, 15.339877, 15.339862, 39.043824, 0.062409}, 0.265954};
+static const ff5 FF_CH = {{2.909530, 0.485267, 1.516151, 0.206905, 1.541626}, {13.933084, 23.221524, 41.990403, 4.974183, 0.679266}, 0.337670};
+static const ff5 FF_CH2 = {{3.275723, 0.870037, 1.534606, 0.395078, 1.544562}, {13.408502, 23.785175, 41.922444, 5.019072, 0.724439}, 0.377096};
+static const ff5 FF_CH3 = {{3.681341, 1.228691, 1.549320, 0.574033, 1.554377}, {13.026207, 24.131974, 41.869426, 4.984373, 0.765769}, 0.409294};
+static const ff5 FF_OH = {{0.456221, 3.219608, 0.812773, 2.666928, 1.380927}, {21.503498, 13.397134, 34.547137, 5.826620, 0.412902}, 0.463202};
+static const ff5 FF_NH = {{1.650531, 0.429639, 2.144736, 1.851894, 1.408921}, {10.603730, 6.987283, 29.939901, 10.573859, 0.611678}, 0.510589};
+static const ff5 FF_NH2 = {{1.904157, 1.942536, 2.435585, 0.730512, 1.379728}, {10.803702, 10.792421, 29.610479, 6.847755, 0.709687}, 0.603738};
+static const ff5 FF_NH3 = {{1.882162, 1.933200, 2.465843, 0.927311, 1.190889}, {10.975157, 10.956008, 29.208572, 6.663555, 0.843650}, 0.597322};
+static const ff5 FF_SH = {{0.570042, 6.337416, 1.641643, 5.398549, 1.527982}, {11.447986, 1.197657, 55.401032, 22.420955, 2.356552}, 1.523944};
+
+static double eval5(const ff5* f, double q, int convert) {
+    double s = 0;
+    for (int i = 0; i < 5; ++i) {
+        double b = convert ? f->b[i] * S2Q : f->b[i];
+        s += f->a[i] * exp(-b * q * q);
+    }
+    return s + f->c;
+}
+
+double orc_ff_atomic(int type, double q) {
+    switch (type) {
+        case 0: { /* excluded_volume: a={1}, b={r_avg^2/2} in q units (FormFactorTable.h) */
+            double r = 1.62;
+            return exp(-(r * r / 2) * q * q);
+        }
+        case 1: return eval5(&FF_OH, q, 1);  /* WATER == OH -> OH_alc */
+        case 2: return eval5(&FF_H, q, 1);
+        case 3: return eval5(&FF_C, q, 1);
+        case 4: return eval5(&FF_CH, q, 1);  /* CH_sp3 */
+        case 5: return eval5(&FF_CH2, q, 1); /* CH2_sp3 */
+        case 6: return eval5(&FF_CH3, q, 1); /* CH3_sp3 */
+        case 7: return eval5(&FF_N, q, 1);
+        case 8: return eval5(&FF_NH, q, 1);
+        case 9: return eval5(&FF_NH2, q, 1);
+        case 10: return eval5(&FF_NH3, q, 1); /* NH3_plus */
+        case 11: return eval5(&FF_O, q, 1);
+        case 12: return eval5(&FF_S, q, 1);
+        case 13: return eval5(&FF_SH, q, 1);
+        case 14: return eval5(&FF_AR, q, 1); /* OTHER: argon */
+        default: return NAN;
+    }
+}
+
+/* MinimumFluctuation_implicit_H (ExvTable.h), OTHER -> Ar = 4pi/3 1.88^3 */
+static double exv_volume(int type) {
+    switch (type) {
+        case 1: return 20.911;  /* WATER == OH */
+        case 2: return 0;       /* H */
+        case 3: return 12.352;  /* C */
+        case 4: return 11.640;  /* CH */
+        case 5: return 34.583;  /* CH2 */
+        case 6: return 41.851;  /* CH3 */
+        case 7: return 0.027;   /* N */
+        case 8: return 2.181;   /* NH */
+        case 9: return 20.562;  /* NH2 */
+        case 10: return 20.722; /* NH3 */
+        case 11: return 14.238; /* O */
+        case 12: return 15.413; /* S */
+        case 13: return 28.529; /* SH */
+        case 14: return 4 * PI_ / 3 * 1.88 * 1.88 * 1.88;
+        default: return NAN;
+    }
+}
+
+double orc_ff_exv(int type, double q) {
+    double v = exv_volume(type);
+    double exponent = pow(v, 2. / 3) / (4 * PI_);
+    double q0 = v * 0.334;
+    return q0 * exp(-exponent * q * q);
+}
+
+/* ------------------------------------------------------------------------ */
+/* intensity profiles                                                       */
+/* ------------------------------------------------------------------------ */
+
+#define S3(f1, f2, k) sinqd_aa[((size_t)(f1) * ORC_MAX_FF + (f2)) * nq + (k)]
+#define S2(f, k) sinqd_aw[(size_t)(f) * nq + (k)]
+
+void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, const double* sinqd_aw,
+                  const double* sinqd_ww, orc_params p, double* out) {
+    double* aa = out;
+    double* ax = out + nq;
+    double* aw = out + 2 * nq;
+    double* xx = out + 3 * nq;
+    double* wx = out + 4 * nq;
+    double* ww = out + 5 * nq;
+    for (int k = 0; k < 6 * nq; ++k) out[k] = 0;
+    const double rm2 = 1.62 * 1.62 / 4;
+    for (int k = 0; k < nq; ++k) {
+        double qk = q[k];
+        double cx = mode == ORC_FFAVG ? p.cx : pow(p.cx, 3) * exp(-rm2 * (pow(p.cx, 2) - 1) * qk * qk);
+        double fw = orc_ff_atomic(ORC_WATER, qk);
+        for (int f1 = 1; f1 < ORC_MAX_FF; ++f1) {
+            double a1 = orc_ff_atomic(f1, qk);
+            for (int f2 = 1; f2 < ORC_MAX_FF; ++f2) {
+                double a2 = orc_ff_atomic(f2, qk);
+                double s = S3(f1, f2, k);
+                aa[k] += s * a1 * a2;
+                if (mode == ORC_FFEXPLICIT) {
+                    /* raw_cross_table(i,j) = atomic(i)*exv(j); raw_exv_table = exv*exv */
+                    double cross = a1 * orc_ff_exv(f2, qk) + a2 * orc_ff_exv(f1, qk);
+                    ax[k] += p.crho * cx * s * cross;
+                    xx[k] += pow(cx * p.crho, 2) * s * orc_ff_exv(f1, qk) * orc_ff_exv(f2, qk);
+                }
+            }
+            if (mode == ORC_FFAVG) {
+                double fx = orc_ff_atomic(ORC_EXV, qk);
+                ax[k] += 2 * p.crho * cx * S3(f1, ORC_EXV, k) * a1 * fx;
+                aw[k] += 2 * p.cw * S2(f1, k) * a1 * fw;
+            } else {
+                aw[k] += 2 * p.cw * S2(f1, k) * a1 * fw;
+                wx[k] += p.crho * cx * p.cw * S2(f1, k) * (fw * orc_ff_exv(f1, qk) + a1 * orc_ff_exv(ORC_WATER, qk));
+            }
+        }
+        if (mode == ORC_FFAVG) {
+            double fx = orc_ff_atomic(ORC_EXV, qk);
+            xx[k] += pow(cx * p.crho, 2) * S3(ORC_EXV, ORC_EXV, k) * fx * fx;
+            wx[k] += 2 * p.crho * cx * p.cw * S2(ORC_EXV, k) * fx * fw;
+        }
+        ww[k] += p.cw * p.cw * sinqd_ww[k] * fw * fw;
+
+        /* Debye-Waller: FFAvgBase.cpp:310-374 */
+        if (p.dw_atomic != 0 || p.dw_exv != 0) {
+            double ba = exp(-qk * qk * p.dw_atomic * p.dw_atomic * 0.5);
+            double bx = exp(-qk * qk * p.dw_exv * p.dw_exv * 0.5);
+            aa[k] *= ba * ba;
+            xx[k] *= bx * bx;
+            ax[k] *= ba * bx;
+            aw[k] *= ba;
+            wx[k] *= bx;
+        }
+    }
+}
+
+void orc_total(const double* prof, int nq, double* out) {
+    for (int k = 0; k < nq; ++k)
+        out[k] = prof[k] - prof[nq + k] + prof[2 * nq + k] + prof[3 * nq + k] - prof[4 * nq + k] + prof[5 * nq + k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact Debye and LSQ                                                      */
+/* ------------------------------------------------------------------------ */
+
+void orc_exact_debye(const float* p, int n, const double* q, int nq, double* out) {
+    for (int k = 0; k < nq; ++k) {
+        double sum = 0;
+        for (int i = 0; i < n; ++i) {
+            const float* pi = p + 4 * (size_t)i;
+            for (int j = i + 1; j < n; ++j) {
+                float d;
+                int32_t bin;
+                orc_bin_pair(pi, p + 4 * (size_t)j, 1.0f, orc_rule_self(n, i, j), &d, &bin);
+                double qr = q[k] * d;
+                float w2 = 2 * (pi[3] * p[4 * (size_t)j + 3]);
+                sum += qr < 1e-9 ? w2 : w2 * sin(qr) / qr;
+            }
+            sum += pow(pi[3], 2);
+        }
+        out[k] = sum * exp(-q[k] * q[k]);
+    }
+}
+
+void orc_lsq(const double* x, const double* y, const double* err, int n, double* out) {
+    double S = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0;
+    for (int i = 0; i < n; ++i) {
+        double is = err ? 1. / err[i] : 1.;
+        double w = is * is;
+        S += w;
+        Sx += x[i] * w;
+        Sy += y[i] * w;
+        Sxx += x[i] * x[i] * w;
+        Sxy += x[i] * y[i] * w;
+    }
+    double delta = S * Sxx - Sx * Sx;
+    out[0] = (S * Sxy - Sx * Sy) / delta;
+    out[1] = (Sxx * Sy - Sx * Sxy) / delta;
+    out[2] = S / delta;
+    out[3] = Sxx / delta;
+}

@@ -1,0 +1,127 @@
+/*
+ * ausaxs_oracle — CPU restatement of AUSAXS's Debye hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg may load it. The product
+ * path (ausaxs_amd) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the AUSAXS 1.2.9 checkout). Parity pins are listed per function; a function
+ * without one is marked "parity unpinned".
+ */
+#ifndef AUSAXS_ORACLE_H
+#define AUSAXS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_FF 15          /* settings/FormFactorSettings.h:12 max_ff_types   */
+#define ORC_EXV 0              /* form_factor/FormFactorType.h:14-33 EXCLUDED_VOLUME */
+#define ORC_WATER 1            /* ... WATER == OH                                  */
+
+/* WeightedEntry{double value; unsigned count; double bin_center}
+ * distribution/detail/WeightedEntry.h:16-50 */
+typedef struct {
+    double value;
+    uint32_t count;
+    double bin_center;
+} orc_entry;
+
+enum { ORC_FFAVG = 0, ORC_FFEXPLICIT = 1 };
+
+/* ---- binning rules (CompactCoordinatesXYZFF.h) -------------------------- */
+/* rule 0: SIMD body (AVX2 :514-545, SSE :397-420): d2=(dx²+dy²)+dz², sqrt_rn,
+ *         x=d*inv_w, round-half-even.
+ * rule 1: scalar tail (evaluate_scalar :313-329 built with -ffast-math):
+ *         d2=fma(dz,dz,fma(dx,dx,dy*dy)), sqrt_rn, trunc(x+0.49999997f). */
+void orc_bin_pair(const float* a, const float* b, float inv_w, int rule, float* d_out, int32_t* bin_out);
+
+/* Which rule the reference loop uses for a pair (Appendix of the loop
+ * geometry in HistogramManagerMTFFAvg.cpp:214-280): self loops j>i use the
+ * scalar rule for the last (n-i-1) mod 4 columns; cross loops for the last
+ * m mod 4 columns. */
+int orc_rule_self(int n, int i, int j);
+int orc_rule_cross(int m, int j);
+
+/* ---- FF managers ---------------------------------------------------------
+ * HistogramManagerMTFFAvg::calculate_all (HistogramManagerMTFFAvg.cpp:194-371)
+ * HistogramManagerMTFFExplicit::calculate_all (...FFExplicit.cpp:216-436)
+ * atoms/waters: n x 4 floats {x,y,z, ff as int32 bits}; ff already remapped.
+ * aa: [15][15][bin_count], aw: [15][bin_count], ww/ptot: [bin_count].
+ * z_exv: FFAvg only (V_grid*0.334/N); ignored for FFExplicit.
+ * nthreads: worker threads (800-row tasks, thread-private histograms, merge,
+ * GeneralSettings.cpp:50, ThreadLocalWrapper.h:89-105).
+ * Returns max_bin (>= 10) or a negative status: -2 bin out of range, -3 bad ff. */
+int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int n_w,
+                float inv_w, int bin_count, double z_exv, int nthreads,
+                orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
+/* Raw pair histogram of one pair set without manager conventions:
+ * self (b == NULL) or cross (rows a, cols b); factor = reference template N.
+ * out[bin_count] entries. Returns 0 or -2 on a bin >= bin_count. */
+int orc_pair_hist(const float* a, int n, const float* b, int m, float inv_w, int bin_count,
+                  int factor, orc_entry* out);
+
+/* HistogramManagerMT::calculate_all with SimpleCalculator (XYZW weights):
+ * HistogramManagerMT.cpp:23-75, SimpleCalculator.h:94-284, TemplateHelperSimple.h:15-61.
+ * atoms/waters: n x 4 floats {x,y,z,w}. aa/aw/ww/ptot: [bin_count].
+ * Returns max_bin or negative status. */
+int orc_hist_simple(const float* atoms, int n_a, const float* waters, int n_w,
+                    float inv_w, int bin_count, int nthreads,
+                    orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
+/* ---- weighted axis, sinc, Debye ------------------------------------------ */
+/* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
+ * followed by DistanceHistogram::initialize (DistanceHistogram.cpp:44-47, d[0]=0). */
+void orc_weighted_axis(const orc_entry* p, int n, double bin_width, double* d_out);
+
+/* VectorDebyeTable::initialize (VectorDebyeTable.cpp:24-38): out[iq*nd + id]. */
+void orc_sinc_table(const double* q, int nq, const double* d, int nd, double* out);
+
+/* std::inner_product per (row, q): out[r*nq + iq] = sum_b H[r*nd+b]*S[iq*nd+b]
+ * (CompositeDistanceHistogramFFAvgBase.cpp:427-467). */
+void orc_contract(const double* H, int rows, int nd, const double* S, int nq, double* out);
+
+/* Kahan-free long-double contraction used as the accuracy reference for the
+ * device GEMM (same operands as orc_contract, 80-bit accumulation). */
+void orc_contract_ld(const double* H, int rows, int nd, const double* S, int nq, double* out);
+
+/* ---- form factors -------------------------------------------------------- */
+/* FormFactor::evaluate (FormFactor.h:33-39) with the raw 5-Gaussian tables of
+ * FormFactorTable.h:28-120 mapped per lookup::atomic::raw::get (FormFactor.h:93-117). */
+double orc_ff_atomic(int type, double q);
+/* ExvFormFactor(volume).evaluate (ExvFormFactor.h:16-42) with the default
+ * MinimumFluctuation_implicit_H set (ExvTable.h, ExvSettings.h:57-69). */
+double orc_ff_exv(int type, double q);
+
+/* Free parameters of ICompositeDistanceHistogramExv (FFAvgBase.h). */
+typedef struct {
+    double cw, cx, crho, dw_atomic, dw_exv;
+} orc_params;
+
+/* Six partial profiles {aa, ax, aw, xx, wx, ww} on an arbitrary q grid:
+ * FFAvg: cache_refresh_intensity_profiles (FFAvgBase.cpp:470-558) + DW
+ * (:310-374); FFExplicit: FFExplicitBase.cpp:105-212 with exv_factor :54-57.
+ * sinqd_aa: [15][15][nq], sinqd_aw: [15][nq], sinqd_ww: [nq] (FFAvg also
+ * reads the EXV rows/columns of aa/aw). out: [6][nq]. */
+void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, const double* sinqd_aw,
+                  const double* sinqd_ww, orc_params p, double* out);
+
+/* I = aa - ax + aw + xx - wx + ww (FFAvgBase.cpp:57-76). */
+void orc_total(const double* profiles, int nq, double* out);
+
+/* Exact O(N^2) Debye with f = exp(-q^2/2) per atom, FP64 over FP32-rounded
+ * distances (ExactDebyeCalculator.cpp:13-57). xyzw: n x 4 floats. */
+void orc_exact_debye(const float* xyzw, int n, const double* q, int nq, double* out);
+
+/* LinearLeastSquares::fit_params_only (LinearLeastSquares.cpp:26-44):
+ * weighted fit y ~ a*x + b with weights 1/err^2. out[0]=a, out[1]=b. */
+void orc_lsq(const double* x, const double* y, const double* err, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

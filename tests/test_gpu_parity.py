@@ -1,0 +1,256 @@
+"""GPU parity tests: the HIP path through the C ABI against the CPU oracle.
+
+Bar: bin indices and counts bit-exact; summed distances (weighted bins)
+within 1e-5 relative; I(q) within 1e-4 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import ausaxs_amd as A
+import oracle as O
+from conftest import compare_hist_approx, cube_atoms, globule
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def rel_close(a, b, rtol, atol=1e-9):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= atol + rtol * np.abs(b))
+
+
+def test_device_is_gfx950():
+    info = A.device_info()
+    print(info)
+    assert info["arch"].startswith("gfx950")
+
+
+@pytest.mark.parametrize("bw", [0.25, 0.1, 1.0])
+def test_binning_arithmetic_exhaustive(bw):
+    # every float32 d2 in [0, 6.4e5] (distances up to 800 A): the fast path
+    # (raw v_sqrt_f32 + exact fallback near ties) bins like sqrt_rn, and the
+    # device sqrtf is correctly rounded (verified in FP64)
+    hi = int(np.array(6.4e5, np.float32).view(np.uint32))
+    assert A.bin_check(0, hi, bw) == (0, 0)
+
+
+def check_pairs(a, b, bw, nb):
+    cnt, ds = A.pair_histogram(a, b, bw, nb)
+    ref = O.pair_hist(a, b, bw, nb, factor=1)
+    assert np.array_equal(cnt, ref["count"].astype(np.uint64))
+    m = ref["count"] > 0
+    assert rel_close(ds[m], ref["bin_center"][m], 1e-5, atol=1e-6)
+    assert np.all(ds[~m] == 0)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 5, 7, 64, 100, 513, 1027, 2049])
+def test_self_pairs_random(n):
+    rng = np.random.default_rng(n)
+    a = np.zeros((n, 4), F32)
+    a[:, :3] = rng.standard_normal((n, 3)) * 12
+    check_pairs(a, None, 0.25, 800)
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (100, 37), (513, 514), (700, 1029), (1500, 4)])
+def test_cross_pairs_random(n, m):
+    rng = np.random.default_rng(n * 7 + m)
+    a = np.zeros((n, 4), F32)
+    b = np.zeros((m, 4), F32)
+    a[:, :3] = rng.standard_normal((n, 3)) * 12
+    b[:, :3] = rng.standard_normal((m, 3)) * 12 + 3
+    check_pairs(a, b, 0.1, 1200)
+
+
+@pytest.mark.parametrize("n", [9, 10, 11, 12, 131])
+def test_exact_ties_follow_reference_rules(n):
+    # points 0.625 A apart on a line: x = 2.5 k, an exact tie for odd k, where the
+    # SIMD body (half to even) and the scalar tail (half up) disagree
+    a = np.zeros((n, 4), F32)
+    a[:, 0] = np.arange(n) * F32(0.625)
+    check_pairs(a, None, 0.25, 400)
+    check_pairs(a, a[::-1].copy(), 0.25, 400)
+
+
+def test_integer_lattice():
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(9), np.arange(7), indexing="ij"), -1).reshape(-1, 3)
+    a = np.zeros((len(g), 4), F32)
+    a[:, :3] = g
+    check_pairs(a, None, 0.25, 200)
+    check_pairs(a, None, 1.0, 60)
+
+
+# --------------------------------------------------------------------------
+# managers
+# --------------------------------------------------------------------------
+def compare_entries(gpu, ref):
+    v, c, b = gpu
+    assert np.array_equal(c, ref["count"]), np.argwhere(c != ref["count"])[:5]
+    assert np.array_equal(v, ref["value"])
+    m = ref["count"] > 0
+    assert rel_close(b[m], ref["bin_center"][m], 1e-5, atol=1e-6)
+
+
+def compare_ff_hist(h, ref, mode):
+    assert h.max_bin == ref["max_bin"]
+    if mode == A.MODE_SIMPLE:
+        for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww"), (A.PTOT, "p_tot")):
+            v, c, b = h.get(which)
+            assert np.array_equal(c, ref[key]["count"]), key
+            np.testing.assert_allclose(v, ref[key]["value"], rtol=1e-12)
+            m = ref[key]["count"] > 0
+            assert rel_close(b[m], ref[key]["bin_center"][m], 1e-5, atol=1e-6)
+        return
+    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww")):
+        v, c, b = h.get(which)
+        rv = ref[key]
+        assert np.array_equal(c, rv["count"]), key
+        np.testing.assert_allclose(v, rv["value"], rtol=1e-14, err_msg=key)
+        m = rv["count"] > 0
+        assert rel_close(b[m], rv["bin_center"][m], 1e-5, atol=1e-6), key
+    v, c, b = h.get(A.PTOT)
+    assert np.array_equal(c, ref["p_tot"]["count"])
+    m = ref["p_tot"]["count"] > 0
+    assert rel_close(b[m], ref["p_tot"]["bin_center"][m], 1e-5, atol=1e-6)
+    d = h.d_axis()
+    np.testing.assert_allclose(d, O.weighted_axis(ref["p_tot"], 0.25 if len(d) else 0.25), rtol=1e-5, atol=1e-6)
+
+
+def config1(n=2000, waters=300):
+    atoms = globule(n, 20.0, 1)
+    w = A.synth_shell(5, waters, (20, 20, 20), 3.0)
+    return atoms, w
+
+
+@pytest.mark.parametrize("mode", [A.MODE_FFEXPLICIT, A.MODE_FFAVG])
+def test_ff_manager_parity(mode):
+    atoms, waters = config1()
+    z = 0.37
+    h = A.calculate_all(mode, atoms, waters, z_exv=z)
+    ref = O.hist_ff(O.FFAVG if mode == A.MODE_FFAVG else O.FFEXPLICIT, atoms, waters, z_exv=z, nthreads=4)
+    compare_ff_hist(h, ref, mode)
+
+
+def test_ff_manager_types_and_tails():
+    # every active atom type, sizes with all residues mod 4, types interleaved
+    rng = np.random.default_rng(4)
+    for n, m in ((37, 0), (38, 5), (41, 6), (400, 3)):
+        xyz = rng.standard_normal((n, 3)) * 9
+        ff = rng.choice([2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14], n)
+        atoms = A.points_ff(xyz, ff)
+        waters = A.points_ff(rng.standard_normal((m, 3)) * 11, 1)
+        for mode in (A.MODE_FFEXPLICIT, A.MODE_FFAVG):
+            h = A.calculate_all(mode, atoms, waters, z_exv=1.3)
+            ref = O.hist_ff(O.FFAVG if mode == A.MODE_FFAVG else O.FFEXPLICIT, atoms, waters, z_exv=1.3)
+            compare_ff_hist(h, ref, mode)
+
+
+def test_simple_manager_parity():
+    rng = np.random.default_rng(8)
+    atoms = globule(1200, 16.0, 3)
+    atoms[:, 3] = rng.choice([6.0, 7.0, 8.0, 5.5], len(atoms)).astype(F32)
+    waters = A.synth_shell(6, 203, (16, 16, 16), 3.0)
+    waters[:, 3] = 1.0
+    h = A.calculate_all(A.MODE_SIMPLE, atoms, waters)
+    ref = O.hist_simple(atoms, waters)
+    compare_ff_hist(h, ref, A.MODE_SIMPLE)
+
+
+def test_cube_known_counts():
+    atoms = A.points_ff(cube_atoms(), 3)
+    waters = A.points_ff(cube_atoms(), 1)
+    h = A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters)
+    v, c, b = h.get(A.AA)
+    assert h.max_bin == 15
+    assert list(v[3, 3, [0, 8, 11, 14]]) == [8, 24, 24, 8]
+    assert v.sum() == 64
+    d = h.d_axis()
+    assert abs(d[8] - 2) < 1e-6 and abs(d[11] - 8 ** 0.5) < 1e-6 and abs(d[14] - 12 ** 0.5) < 1e-6
+
+
+def test_determinism_across_tilings():
+    atoms, waters = config1(3000, 500)
+    outs = []
+    for tj in (256, 1000, 8192):
+        h = A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters, settings=A.default_settings(tile_j=tj))
+        outs.append([h.get(w) for w in (A.AA, A.AW, A.WW)])
+    for o in outs[1:]:
+        for (v0, c0, b0), (v1, c1, b1) in zip(outs[0], o):
+            assert np.array_equal(c0, c1) and np.array_equal(v0, v1) and np.array_equal(b0, b1)
+
+
+def test_bin_overflow_is_an_error():
+    a = A.points_ff(np.array([[0, 0, 0], [30, 0, 0]]), 3)
+    with pytest.raises(A.AusaxsError, match="bin_count"):
+        A.calculate_all(A.MODE_FFEXPLICIT, a, settings=A.default_settings(bin_count=100))
+
+
+def test_unknown_form_factor_is_an_error():
+    a = A.points_ff(np.zeros((3, 3)), 0)
+    with pytest.raises(A.AusaxsError, match="form factor"):
+        A.calculate_all(A.MODE_FFEXPLICIT, a)
+
+
+# --------------------------------------------------------------------------
+# Debye
+# --------------------------------------------------------------------------
+def relerr(x, ref):
+    return np.max(np.abs(x - ref) / np.abs(ref))
+
+
+@pytest.mark.parametrize("mode", [A.MODE_FFEXPLICIT, A.MODE_FFAVG])
+def test_intensity_parity(mode):
+    atoms, waters = config1()
+    omode = O.FFAVG if mode == A.MODE_FFAVG else O.FFEXPLICIT
+    ref_h = O.hist_ff(omode, atoms, waters, z_exv=0.5, nthreads=4)
+    q = np.linspace(0.0, 0.5, 200)
+    params = dict(cw=1.1, cx=1.05, crho=0.97, dw_atomic=0.2, dw_exv=0.1)
+    I_ref, P_ref = O.debye_ff(omode, ref_h, q, 0.25, long_double=True, **params)
+    for terms, tol in ((3, 1e-4), (0, 1e-9)):
+        h = A.calculate_all(mode, atoms, waters, settings=A.default_settings(debye_terms=terms), z_exv=0.5)
+        h.set_params(**params)
+        I, P = h.debye_transform(q, profiles=True)
+        err = relerr(I, I_ref)
+        print(f"mode {mode} terms {terms}: max rel err {err:.3e}")
+        assert err <= tol
+
+
+def test_debye_contract_wide_angle():
+    # 250k-atom-like histogram shape, WAXS q up to 1.5, 0.1 A bins
+    atoms = globule(6000, 30.0, 2, 1.8)
+    ref_h = O.hist_ff(O.FFEXPLICIT, atoms, None, bin_width=0.1, nthreads=8)
+    mb = ref_h["max_bin"]
+    d = O.weighted_axis(ref_h["p_tot"], 0.1)
+    H = ref_h["aa"]["value"].reshape(-1, mb)
+    H = H[np.any(H != 0, axis=1)]
+    q = np.linspace(0.0015, 1.5, 1000)
+    ref = O.contract(H, O.sinc_table(q, d), long_double=True)
+    I_ref = ref.sum(0)
+    for terms, tol in ((3, 1e-4), (2, None)):
+        out = A.debye_contract(H, d, q, terms)
+        err = relerr(out.sum(0), I_ref)
+        print(f"terms {terms}: I(q) max rel err {err:.3e}")
+        if tol:
+            assert err <= tol
+    f64 = A.debye_contract(H, d, q, 0)
+    assert relerr(f64.sum(0), I_ref) <= 1e-10
+
+
+def test_golden_carbon_sphere_gpu(golden_carbon):
+    xyz, q, I_ref = golden_carbon
+    atoms = A.points_w(xyz, 1.0)
+    h = A.calculate_all(A.MODE_SIMPLE, atoms, None, settings=A.default_settings(bin_width=0.1))
+    I = h.debye_transform(q) * np.exp(q * q)
+    ok, i = compare_hist_approx(I, I_ref, 1e-6, 0.01)
+    assert ok, (i, I[i], I_ref[i])
+
+
+def test_large_structure_invariants():
+    # beyond oracle sizes: exact pair-count identities
+    atoms = A.synth_globule(3, 60000, (60, 45, 35), 1.8)
+    waters = A.synth_shell(4, 9000, (60, 45, 35), 3.0)
+    h = A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters, settings=A.default_settings(bin_width=0.1))
+    n, m = len(atoms), len(waters)
+    assert h.get(A.AA)[1].sum(dtype=np.int64) == n * (n - 1) + n
+    assert h.get(A.AW)[1].sum(dtype=np.int64) == n * m
+    assert h.get(A.WW)[1].sum(dtype=np.int64) == m * (m - 1) + m
