@@ -11,16 +11,20 @@
 namespace ausaxs::gpu {
 
 // Fixed-point encoding of one LDS/partial histogram cell (one uint64):
-//   bits 40..63  pair count (24 bits)
-//   bits  0..39  sum over pairs of (q + 2^15), q = round(delta * 2^16),
+//   bits 42..63  pair count (22 bits)
+//   bits  0..41  sum over pairs of (q + 2^19), q = round(delta * 2^20),
 //                delta = x - rint(x), x = d * inv_width (bin units)
-// A work item holds < 2^24 pairs, so neither field can overflow. Integer
-// accumulation keeps every histogram bit-reproducible.
-constexpr int kDeltaBits = 16;
-constexpr uint64_t kCountShift = 40;
+// A work item holds < 2^22 pairs, so neither field can overflow (each pair
+// adds at most 2^20 to the low field). The mean distance of a bin is then
+// known to 2^-21 bins even when every pair in it shares one delta (lattices,
+// symmetric test bodies). Integer accumulation keeps every histogram
+// bit-reproducible.
+constexpr int kDeltaBits = 20;
+constexpr uint64_t kCountShift = 42;
 constexpr uint64_t kLowMask = (uint64_t(1) << kCountShift) - 1;
 constexpr int64_t kDeltaOffset = int64_t(1) << (kDeltaBits - 1);
-constexpr int64_t kMaxPairsPerItem = (int64_t(1) << 24) - 1;
+constexpr int64_t kMaxPairsPerItem = (int64_t(1) << 22) - 1;
+constexpr double kDeltaScale = 1.0 / (double)(int64_t(1) << kDeltaBits);
 
 // One workgroup's rectangle of the pair space.
 struct PairWork {

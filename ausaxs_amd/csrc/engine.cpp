@@ -54,7 +54,7 @@ PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, in
                        int tile_j)
     : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w) {
     if (tile_j < 1 || (int64_t)kTileI * tile_j > kMaxPairsPerItem)
-        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item below 2^24 pairs");
+        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item below 2^22 pairs (tile_j <= 8191)");
     if (nbins < 1) throw std::runtime_error("ausaxs_hip: nbins must be positive");
     // dynamic histogram rows + the static column tile (257 float4) of pair_hist_kernel
     const size_t lds = (size_t)2 * (nbins + 2) * sizeof(uint64_t) + 257 * sizeof(float4);

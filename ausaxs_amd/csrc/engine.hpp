@@ -93,7 +93,7 @@ struct EngineTiming {
 class PairEngine {
   public:
     PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-               int tile_j = 8192);
+               int tile_j = 8000);
     PairEngine(const PairEngine&) = delete;
     PairEngine& operator=(const PairEngine&) = delete;
 

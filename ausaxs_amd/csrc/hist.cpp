@@ -255,6 +255,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
             }
             if (cw.size() > 64)
                 throw std::runtime_error("ausaxs_hip: more than 64 distinct weights in one point set");
+            if (cw.empty()) cw.push_back(0.f);  // empty set: one (unused) class keeps row tables non-empty
             ps.n_classes = std::max<int>(1, (int)cw.size());
         };
         classify(atoms, n_atoms, A, atom_w_, atom_class_w_);
@@ -313,7 +314,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         WeightedEntry e;
         e.value = (double)factor * (double)c;
         e.count = (uint32_t)(factor * c);
-        e.bin_center = factor * (((double)c * b + (double)s * (1.0 / 65536.0)) / inv_w);
+        e.bin_center = factor * (((double)c * b + (double)s * kDeltaScale) / inv_w);
         return e;
     };
 
@@ -582,7 +583,7 @@ void raw_pair_histogram(const float* a, int n, const float* b, int m, const Hist
     if (eng.overflowed(c)) throw std::runtime_error("ausaxs_hip: pair distance beyond bin_count*bin_width");
     for (int k = 0; k < nbins; ++k) {
         count[k] = c[k];
-        dsum[k] = ((double)c[k] * k + (double)ds[k] * (1.0 / 65536.0)) / (double)inv_w;
+        dsum[k] = ((double)c[k] * k + (double)ds[k] * kDeltaScale) / (double)inv_w;
     }
 }
 

@@ -57,10 +57,13 @@ __device__ __forceinline__ void simd_rule_exact(float d2, float inv_w, float& x,
     dl = x - b;
 }
 
+// q + 2^19 with q = round(dl * 2^20); dl in [-0.5, 0.5] -> [0, 2^20]
+__device__ __forceinline__ uint32_t quantize(float dl) {
+    return (uint32_t)__builtin_fmaf(dl, 1048576.0f, 524288.5f);
+}
+
 __device__ __forceinline__ uint64_t cell_add(float dl) {
-    // q + 2^15 with q = round(dl * 2^16); dl in [-0.5, 0.5] -> [0, 65536]
-    const uint32_t low = (uint32_t)__builtin_fmaf(dl, 65536.0f, 32768.5f);
-    return (uint64_t(1) << kCountShift) | (uint64_t)low;
+    return (uint64_t(1) << kCountShift) | (uint64_t)quantize(dl);
 }
 
 __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     const int row = p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
     if (row < 0) return;  // the host guarantees a row for every present class pair
     const unsigned bin = min((unsigned)(int)b, (unsigned)p.nbins);
-    const uint32_t low = (uint32_t)__builtin_fmaf(dl, 65536.0f, 32768.5f);
+    const uint32_t low = quantize(dl);
     const size_t cell = (size_t)row * p.stride + bin;
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
     atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)low - kDeltaOffset));

@@ -107,7 +107,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
     ap.add_argument("--debye-terms", type=int, default=3)
-    ap.add_argument("--tile-j", type=int, default=8192)
+    ap.add_argument("--tile-j", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
