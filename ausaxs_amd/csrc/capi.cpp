@@ -4,6 +4,7 @@
 // source/api/api_helper.h:25-45).
 #include "../../include/ausaxs_hip.h"
 
+#include "grid.hpp"
 #include "hist.hpp"
 #include "synth.hpp"
 
@@ -58,7 +59,8 @@ HistSettings to_settings(const ausaxs_settings* s) {
 }
 
 Mode to_mode(int m) {
-    require(m == AUSAXS_MODE_FFAVG || m == AUSAXS_MODE_FFEXPLICIT || m == AUSAXS_MODE_SIMPLE, "unknown mode");
+    require(m == AUSAXS_MODE_FFAVG || m == AUSAXS_MODE_FFEXPLICIT || m == AUSAXS_MODE_SIMPLE,
+            "unknown mode (FFGrid structures are made by ausaxs_structure_create_grid)");
     return (Mode)m;
 }
 }  // namespace
@@ -105,6 +107,60 @@ int ausaxs_structure_create(int mode, const float* atoms, int n_atoms, const flo
         s->s = std::make_unique<ResidentStructure>(to_mode(mode), atoms, n_atoms, waters, n_waters,
                                                    to_settings(settings), z_exv);
         *out = s.release();
+    });
+}
+
+int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                 const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                 ausaxs_structure** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_waters == 0 || waters_xyz, "waters is NULL");
+        for (int i = 0; i < n_atoms; ++i)
+            require(atom_ff[i] >= 1 && atom_ff[i] < kMaxFF, "atom form factor type outside [1, 14]");
+        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters);
+        // CompactCoordinatesFF: float casts of the FP64 coordinates
+        std::vector<float> a(4 * (size_t)n_atoms), w(4 * (size_t)n_waters);
+        for (int i = 0; i < n_atoms; ++i) {
+            for (int k = 0; k < 3; ++k) a[4 * i + k] = (float)atoms_xyz[3 * i + k];
+            std::memcpy(&a[4 * i + 3], &atom_ff[i], 4);
+        }
+        const int32_t water = kWaterBin;
+        for (int i = 0; i < n_waters; ++i) {
+            for (int k = 0; k < 3; ++k) w[4 * i + k] = (float)waters_xyz[3 * i + k];
+            std::memcpy(&w[4 * i + 3], &water, 4);
+        }
+        auto s = std::make_unique<ausaxs_structure>();
+        s->s = std::make_unique<ResidentStructure>(Mode::FFGrid, a.data(), n_atoms, w.data(), n_waters,
+                                                   to_settings(settings), 0.0, g.xyz.data(),
+                                                   (int)(g.xyz.size() / 3), g.volume);
+        *out = s.release();
+    });
+}
+
+int ausaxs_grid_exv(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                    int n_waters, double* out_xyz, int capacity, int* n_out, double* volume) {
+    return guarded([&] {
+        require(n_out != nullptr, "n_out is NULL");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_waters == 0 || waters_xyz, "waters is NULL");
+        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters);
+        const int n = (int)(g.xyz.size() / 3);
+        *n_out = n;
+        if (volume) *volume = g.volume;
+        require(capacity >= n && (n == 0 || out_xyz), "out_xyz capacity too small");
+        std::memcpy(out_xyz, g.xyz.data(), g.xyz.size() * sizeof(double));
+    });
+}
+
+int ausaxs_histogram_grid_axes(const ausaxs_histogram* h, double* d_ax, double* d_xx) {
+    return guarded([&] {
+        require(h && d_ax && d_xx, "NULL argument");
+        require(h->h->mode == Mode::FFGrid, "not an FFGrid histogram");
+        std::memcpy(d_ax, h->h->d_axis_ax.data(), h->h->d_axis_ax.size() * sizeof(double));
+        std::memcpy(d_xx, h->h->d_axis_xx.data(), h->h->d_axis_xx.size() * sizeof(double));
     });
 }
 
@@ -169,6 +225,8 @@ int ausaxs_histogram_get(const ausaxs_histogram* h, int which, double* value, ui
             case AUSAXS_AW: v = &c.aw; break;
             case AUSAXS_WW: v = &c.ww; break;
             case AUSAXS_PTOT: v = &c.p_tot; break;
+            case AUSAXS_PTOT_AX: v = &c.p_tot_ax; break;
+            case AUSAXS_XX: v = &c.p_xx; break;
             default: throw std::runtime_error("unknown distribution id");
         }
         for (size_t i = 0; i < v->size(); ++i) {

@@ -1,0 +1,35 @@
+// Excluded-volume dummy atoms on the integer lattice (grid exv).
+//
+// Restates grid::Grid(bodies) (Grid.cpp:34-110), the atomic expansion of
+// SphericalExpander<true,false> (SphericalExpander.h:29-90) and
+// RawGridExv::create (RawGridExv.cpp:10-59) with the default settings
+// (cell_width = exv.width = 1 A, min_exv_radius = 2.15 A, AtomicOnly).
+//
+// With those settings the marking procedure has an order-free result, which
+// is what this builder computes directly:
+//   dummies = {voxel of every atom centre (to_bins: round half away)}
+//           u {lattice points p with |p - a|^2 <= max(r_vdw(a), 2.15)^2 for
+//              some atom a, evaluated as ((x-ax)^2 + (y-ay)^2) + (z-az)^2 in
+//              FP64, unless p is the voxel of a water centre and of no atom
+//              centre}
+// listed in lexicographic (x, y, z) voxel order; the grid volume is their count.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace ausaxs::gpu {
+
+struct GridExv {
+    std::vector<double> xyz;  // n x 3, integer-valued lattice points
+    double volume = 0;        // Grid::get_volume(): count * cell_width^3
+    int origin[3]{};          // grid minimum (integer)
+};
+
+// atoms: n x 3 FP64 coordinates + form-factor types; waters: m x 3 FP64.
+GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m);
+
+// constants::radius::vdw per form-factor type (FormFactorType.h:174-193)
+double vdw_radius(int ff_type);
+
+}  // namespace ausaxs::gpu

@@ -188,18 +188,21 @@ std::vector<float4> to_points(const float* xyzw, int n) {
 // ResidentStructure
 // ---------------------------------------------------------------------------
 ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
-                                     const HistSettings& s, double z_exv)
-    : mode_(mode), s_(s), z_exv_(z_exv), n_atoms_(n_atoms), n_waters_(n_waters) {
-    if (n_atoms < 0 || n_waters < 0) throw std::runtime_error("ausaxs_hip: negative point count");
+                                     const HistSettings& s, double z_exv, const double* exv_xyz, int n_exv,
+                                     double grid_volume)
+    : mode_(mode), s_(s), z_exv_(z_exv), n_atoms_(n_atoms), n_waters_(n_waters), n_exv_(n_exv),
+      grid_volume_(grid_volume) {
+    if (n_atoms < 0 || n_waters < 0 || n_exv < 0) throw std::runtime_error("ausaxs_hip: negative point count");
     if (!(s.bin_width > 0) || s.bin_count < 10) throw std::runtime_error("ausaxs_hip: invalid bin settings");
+    if (mode == Mode::FFGrid && n_exv > 0 && !exv_xyz) throw std::runtime_error("ausaxs_hip: FFGrid needs dummies");
     const float inv_w = inv_width(s);
-    PointSet A, W;
+    PointSet A, W, X;
     A.orig = to_points(atoms, n_atoms);
     W.orig = to_points(waters, n_waters);
     std::vector<PairJob> jobs;
     int n_rows = 0;
 
-    if (mode == Mode::FFAvg || mode == Mode::FFExplicit) {
+    if (mode == Mode::FFAvg || mode == Mode::FFExplicit || mode == Mode::FFGrid) {
         atom_ff_.resize(n_atoms);
         std::set<int> present;
         for (int i = 0; i < n_atoms; ++i) {
@@ -235,6 +238,32 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
         ww.rowmap[0] = n_rows++;
         rows_.push_back({2, 0, 0});
         jobs = {aa, aw, ww};
+        if (mode == Mode::FFGrid) {
+            // dummies (CompactCoordinatesFF of EXCLUDED_VOLUME atoms, HistogramManagerMTFFGrid.cpp:49-57)
+            X.orig.resize(n_exv);
+            for (int i = 0; i < n_exv; ++i)
+                X.orig[i] = make_float4((float)exv_xyz[3 * i], (float)exv_xyz[3 * i + 1], (float)exv_xyz[3 * i + 2], 0.f);
+            X.n_classes = 1;
+            // xx (self, :68-89), ax (atoms x dummies by atom type, :92-113), wx (waters x dummies, :116-137)
+            PairJob xx, ax, wx;
+            xx.row_set = 2;
+            xx.rowmap = {n_rows++};
+            rows_.push_back({3, 0, 0});
+            ax.row_set = 0;
+            ax.col_set = 2;
+            ax.rowmap.assign(kMaxFF, -1);
+            for (int f : present) {
+                ax.rowmap[f] = n_rows++;
+                rows_.push_back({4, f, 0});
+            }
+            wx.row_set = 1;
+            wx.col_set = 2;
+            wx.rowmap = {n_rows++};
+            rows_.push_back({5, 0, 0});
+            jobs.push_back(xx);
+            jobs.push_back(ax);
+            jobs.push_back(wx);
+        }
     } else {
         // XYZW weights: one class per distinct weight value; w_i*w_j of a pair
         // is then the float product of its two class weights.
@@ -288,11 +317,12 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
         jobs = {aa, ww, aw};
     }
 
-    const int bound = bin_bound({&A, &W}, inv_w);
+    const int bound = bin_bound({&A, &W, &X}, inv_w);
     const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, bound);
     std::vector<PointSet> sets;
     sets.push_back(std::move(A));
     sets.push_back(std::move(W));
+    if (mode == Mode::FFGrid) sets.push_back(std::move(X));
     engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j);
 }
 
@@ -324,19 +354,30 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
     res->weighted_bins = s_.weighted_bins;
     res->debye_terms = s_.debye_terms;
 
-    if (mode_ == Mode::FFAvg || mode_ == Mode::FFExplicit) {
+    if (mode_ != Mode::Simple) {
+        // HistogramManagerMTFFGrid starts from HistogramManagerMTFFAvg<true, vbw>
+        // (HistogramManagerMTFFGrid.cpp:47): FFAvg conventions, always weighted.
+        const bool avg = mode_ != Mode::FFExplicit;
+        const double z = mode_ == Mode::FFGrid ? (n_atoms_ ? grid_volume_ * 0.334 / n_atoms_ : 0.0) : z_exv_;
         std::vector<WeightedEntry> aa((size_t)kMaxFF * kMaxFF * nb), aw((size_t)kMaxFF * nb), ww(nb);
+        std::vector<WeightedEntry> xx, ax, wx;
+        if (mode_ == Mode::FFGrid) xx.resize(nb), ax.resize((size_t)kMaxFF * nb), wx.resize(nb);
         auto A3 = [&](int f1, int f2, int b) -> WeightedEntry& { return aa[((size_t)f1 * kMaxFF + f2) * nb + b]; };
         auto A2 = [&](int f, int b) -> WeightedEntry& { return aw[(size_t)f * nb + b]; };
         for (size_t r = 0; r < rows_.size(); ++r) {
             const auto& k = rows_[r];
             for (int b = 0; b < nbins; ++b) {
-                if (k[0] == 0) A3(k[1], k[2], b) = entry((int)r, b, 2);
-                else if (k[0] == 1) A2(k[1], b) = entry((int)r, b, 1);
-                else ww[b] = entry((int)r, b, 2);
+                switch (k[0]) {
+                    case 0: A3(k[1], k[2], b) = entry((int)r, b, 2); break;
+                    case 1: A2(k[1], b) = entry((int)r, b, 1); break;
+                    case 2: ww[b] = entry((int)r, b, 2); break;
+                    case 3: xx[b] = entry((int)r, b, 2); break;
+                    case 4: ax[(size_t)k[1] * nb + b] = entry((int)r, b, 1); break;
+                    default: wx[b] = entry((int)r, b, 1); break;
+                }
             }
         }
-        if (mode_ == Mode::FFAvg) {
+        if (avg) {
             // HistogramManagerMTFFAvg.cpp:27-34,111-118: every aa pair also lands in
             // [ff_i][exv] (ff of the lower original index) and [exv][exv]; every
             // aw pair in aw[exv]. Derived from the ordered cells.
@@ -351,7 +392,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         }
         // self correlations (FFAvg.cpp:308-323, FFExplicit.cpp:331-346)
         for (int i = 0; i < n_atoms_; ++i) add(A3(atom_ff_[i], atom_ff_[i], 0), WeightedEntry{1, 1, 0});
-        if (mode_ == Mode::FFAvg) add(A3(kExvBin, kExvBin, 0), WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
+        if (avg) add(A3(kExvBin, kExvBin, 0), WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
         add(ww[0], WeightedEntry{(double)n_waters_, (uint32_t)n_waters_, 0});
 
         std::vector<WeightedEntry> ptot(nb);
@@ -361,13 +402,79 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         for (int f = 1; f < kMaxFF; ++f)
             for (int b = 0; b < nbins; ++b) add(ptot[b], A2(f, b));
         for (int b = 0; b < nbins; ++b) add(ptot[b], ww[b]);
-        const unsigned max_bin = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
+        const unsigned base = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
 
-        if (mode_ == Mode::FFAvg) {  // FFAvg.cpp:355-364
+        if (avg) {  // FFAvg.cpp:355-364
             for (int f = 1; f < kMaxFF; ++f)
-                for (unsigned b = 0; b < max_bin; ++b) scale(A3(f, kExvBin, b), z_exv_);
-            for (unsigned b = 0; b < max_bin; ++b) scale(A3(kExvBin, kExvBin, b), z_exv_ * z_exv_);
-            for (unsigned b = 0; b < max_bin; ++b) scale(A2(kExvBin, b), z_exv_);
+                for (unsigned b = 0; b < base; ++b) scale(A3(f, kExvBin, b), z);
+            for (unsigned b = 0; b < base; ++b) scale(A3(kExvBin, kExvBin, b), z * z);
+            for (unsigned b = 0; b < base; ++b) scale(A2(kExvBin, b), z);
+        }
+        unsigned max_bin = base;
+        ptot.resize(base);
+        if (mode_ == Mode::FFGrid) {
+            // HistogramManagerMTFFGrid.cpp:159-224
+            add(xx[0], WeightedEntry{(double)n_exv_, (uint32_t)n_exv_, 0});
+            unsigned mbx = 10;
+            for (int i = nbins - 1; i >= 10; --i)
+                if (xx[i].value != 0 || wx[i].value != 0) {
+                    mbx = (unsigned)i + 1;
+                    break;
+                }
+            // p_tot = FFAvg weighted counts (FFAvgBase.cpp get_weighted_counts: raw
+            // products at the first q of the default axis, cw = 1), centres = FFAvg d axis
+            const std::vector<double> base_axis = axis_of(ptot, s_.bin_width, true);
+            const double q0 = 1e-4;
+            std::vector<double> wc(base, 0.0);
+            for (int f1 = 1; f1 < kMaxFF; ++f1)
+                for (int f2 = 1; f2 < kMaxFF; ++f2) {
+                    const double w = ff_atomic(f1, q0) * ff_atomic(f2, q0);
+                    for (unsigned b = 0; b < base; ++b) wc[b] += A3(f1, f2, b).value * w;
+                }
+            for (int f = 1; f < kMaxFF; ++f) {
+                const double w = 2 * 1.0 * (ff_atomic(f, q0) * ff_atomic(kWaterBin, q0));
+                for (unsigned b = 0; b < base; ++b) wc[b] += A2(f, b).value * w;
+            }
+            {
+                const double w = 1.0 * 1.0 * (ff_atomic(kWaterBin, q0) * ff_atomic(kWaterBin, q0));
+                for (unsigned b = 0; b < base; ++b) wc[b] += ww[b].value * w;
+            }
+            max_bin = std::max(mbx, base);
+            std::vector<WeightedEntry> grid_ptot(max_bin);
+            for (unsigned b = 0; b < base; ++b) grid_ptot[b] = WeightedEntry{wc[b], 0, base_axis[b]};
+            // FFAvg rows beyond the FFAvg size are zero; exv rows become ax / wx / xx
+            for (int f1 = 1; f1 < kMaxFF; ++f1) {
+                for (int f2 = 1; f2 < kMaxFF; ++f2)
+                    for (unsigned b = base; b < max_bin; ++b) A3(f1, f2, b) = WeightedEntry{};
+                for (unsigned b = base; b < max_bin; ++b) A2(f1, b) = WeightedEntry{};
+            }
+            for (unsigned b = base; b < max_bin; ++b) ww[b] = WeightedEntry{};
+            std::vector<WeightedEntry> pax((size_t)max_bin);
+            for (unsigned b = 0; b < max_bin; ++b) add(pax[b], wx[b]);
+            for (int f = 0; f < kMaxFF; ++f)
+                for (unsigned b = 0; b < max_bin; ++b) add(pax[b], ax[(size_t)f * nb + b]);
+            for (int f = 0; f < kMaxFF; ++f)
+                for (unsigned b = 0; b < max_bin; ++b) A3(f, kExvBin, b) = ax[(size_t)f * nb + b];
+            for (unsigned b = 0; b < max_bin; ++b) A2(kExvBin, b) = wx[b];
+            for (unsigned b = 0; b < max_bin; ++b) A3(kExvBin, kExvBin, b) = xx[b];
+            // both axes live on histograms of bin_count entries (WeightedDistribution1D
+            // of the manager), and neither has its first bin forced to zero
+            auto axis_full = [&](const std::vector<WeightedEntry>& p) {
+                const double n = (double)s_.bin_count;
+                const double width = (n * s_.bin_width) / n;
+                std::vector<double> d(max_bin);
+                for (unsigned b = 0; b < max_bin; ++b) {
+                    const double bc = p[b].bin_center;
+                    d[b] = ((double)(!bc) * (b * width) + bc) / (double)((!p[b].count) + p[b].count);
+                }
+                return d;
+            };
+            res->p_tot_ax = pax;
+            res->p_xx.assign(xx.begin(), xx.begin() + max_bin);
+            res->d_axis_ax = axis_full(pax);
+            res->d_axis_xx = axis_full(res->p_xx);
+            res->grid_volume = grid_volume_;
+            ptot = std::move(grid_ptot);
         }
         res->max_bin = max_bin;
         res->aa.resize((size_t)kMaxFF * kMaxFF * max_bin);
@@ -378,7 +485,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             std::copy_n(&A2(f1, 0), max_bin, &res->aw[(size_t)f1 * max_bin]);
         }
         res->ww.assign(ww.begin(), ww.begin() + max_bin);
-        res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
+        res->p_tot = std::move(ptot);
     } else {
         std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
         for (size_t r = 0; r < rows_.size(); ++r) {
@@ -416,7 +523,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         res->ww.assign(ww.begin(), ww.begin() + max_bin);
         res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
     }
-    res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins);
+    res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins || mode_ == Mode::FFGrid);
     return res;
 }
 
@@ -426,26 +533,39 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
 const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vector<double>& q) {
     if (!cached_sinqd_.empty() && q == cached_q_) return cached_sinqd_;
     const int K = (int)max_bin;
-    std::vector<double> H;
-    row_keys_.clear();
-    auto push_row = [&](int kind, int f1, int f2, const WeightedEntry* src) {
+    const bool grid = mode == Mode::FFGrid;
+    // rows grouped by distance axis; H[axis] holds that group's rows
+    std::vector<double> H[3];
+    std::vector<std::array<int, 3>> keys[3];
+    auto push_row = [&](int axis, int kind, int f1, int f2, const WeightedEntry* src) {
         bool nz = false;
         for (int b = 0; b < K; ++b) nz |= src[b].value != 0;
         if (!nz) return;
-        row_keys_.push_back({kind, f1, f2});
-        for (int b = 0; b < K; ++b) H.push_back(src[b].value);
+        keys[axis].push_back({kind, f1, f2});
+        for (int b = 0; b < K; ++b) H[axis].push_back(src[b].value);
     };
     if (mode == Mode::Simple) {
-        push_row(0, 0, 0, aa.data());
-        push_row(1, 0, 0, aw.data());
-        push_row(2, 0, 0, ww.data());
+        push_row(0, 0, 0, 0, aa.data());
+        push_row(0, 1, 0, 0, aw.data());
+        push_row(0, 2, 0, 0, ww.data());
     } else {
+        // FFGrid (CompositeDistanceHistogramFFGrid.cpp:77-119): [ff][exv] and
+        // aw[exv] on the ax axis, [exv][exv] on the xx axis, the rest on p_tot's
         for (int f1 = 0; f1 < kMaxFF; ++f1)
-            for (int f2 = 0; f2 < kMaxFF; ++f2) push_row(0, f1, f2, &aa[((size_t)f1 * kMaxFF + f2) * K]);
-        for (int f = 0; f < kMaxFF; ++f) push_row(1, f, 0, &aw[(size_t)f * K]);
-        push_row(2, 0, 0, ww.data());
+            for (int f2 = 0; f2 < kMaxFF; ++f2) {
+                const int axis = !grid ? 0 : (f1 == kExvBin && f2 == kExvBin) ? 2 : (f2 == kExvBin ? 1 : 0);
+                push_row(axis, 0, f1, f2, &aa[((size_t)f1 * kMaxFF + f2) * K]);
+            }
+        for (int f = 0; f < kMaxFF; ++f) push_row(grid && f == kExvBin ? 1 : 0, 1, f, 0, &aw[(size_t)f * K]);
+        push_row(0, 2, 0, 0, ww.data());
     }
-    const int rows = (int)row_keys_.size(), nq = (int)q.size();
+    const std::vector<double>* axes[3] = {&d_axis, &d_axis_ax, &d_axis_xx};
+    const int nq = (int)q.size();
+    row_keys_.clear();
+    axis_of_row_.clear();
+    for (int a = 0; a < 3; ++a)
+        for (auto& k : keys[a]) row_keys_.push_back(k), axis_of_row_.push_back(a);
+    const int rows = (int)row_keys_.size();
     cached_sinqd_.assign((size_t)rows * nq, 0.0);
     cached_q_ = q;
     if (rows == 0 || nq == 0) return cached_sinqd_;
@@ -453,17 +573,27 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     Context& ctx = Context::get();
     static DevBuf<double> dH, dd, dq, dout;
     static DevBuf<char> scratch;
-    dH.upload(H.data(), H.size(), ctx.stream);
-    dd.upload(d_axis.data(), d_axis.size(), ctx.stream);
-    dq.upload(q.data(), q.size(), ctx.stream);
-    dout.ensure((size_t)rows * nq);
-    scratch.ensure(debye_scratch_bytes(rows, K, nq));
-    DebyeLaunch dl{dH.p, dd.p, dq.p, rows, K, nq, debye_terms, dout.p, scratch.p};
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
+    dq.upload(q.data(), q.size(), ctx.stream);
+    dout.ensure((size_t)rows * nq);
     check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
-    check(debye_terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
+    size_t row0 = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int ra = (int)keys[a].size();
+        if (ra == 0) continue;
+        if ((int)axes[a]->size() < K) throw std::runtime_error("ausaxs_hip: distance axis shorter than the histogram");
+        // the uploads are ordered on the stream before this group's kernels
+        dH.upload(H[a].data(), H[a].size(), ctx.stream);
+        dd.upload(axes[a]->data(), K, ctx.stream);
+        scratch.ensure(debye_scratch_bytes(ra, K, nq));
+        DebyeLaunch dl{dH.p, dd.p, dq.p, ra, K, nq, debye_terms, dout.p + row0 * nq, scratch.p};
+        check(debye_terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
+        // dH/dd are reused by the next group: wait before overwriting them
+        ctx.sync();
+        row0 += ra;
+    }
     check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
     check(hipMemcpyAsync(cached_sinqd_.data(), dout.p, cached_sinqd_.size() * sizeof(double),
                          hipMemcpyDeviceToHost, ctx.stream),
@@ -505,23 +635,27 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
     const double rm2 = 1.62 * 1.62 / 4;  // FFExplicitBase.cpp:54-57
     for (int k = 0; k < nq; ++k) {
         const double qk = q[k];
-        const double cx = mode == Mode::FFAvg ? p.cx : std::pow(p.cx, 3) * std::exp(-rm2 * (p.cx * p.cx - 1) * qk * qk);
+        const double cx = mode != Mode::FFExplicit ? p.cx : std::pow(p.cx, 3) * std::exp(-rm2 * (p.cx * p.cx - 1) * qk * qk);
         const double fw = ff_atomic(kWaterBin, qk);
         double fa[kMaxFF], fe[kMaxFF];
         for (int f = 0; f < kMaxFF; ++f) fa[f] = ff_atomic(f, qk);
         for (int f = 1; f < kMaxFF; ++f) fe[f] = ff_exv(f, qk);
+        // exv form factor of the FFAvg table, or of the grid dummies (V = 1 A^3)
+        const double fx = mode == Mode::FFGrid ? 0.334 * std::exp(-(1.0 / (4 * kPi)) * qk * qk) : fa[kExvBin];
         for (size_t r = 0; r < row_keys_.size(); ++r) {
             const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
             const double s = S[r * nq + k];
-            if (mode == Mode::FFAvg) {
-                // FFAvgBase.cpp:470-558 with the raw_atomic_table products
+            if (mode != Mode::FFExplicit) {
+                // FFAvgBase.cpp:470-558 with the raw_atomic_table products; FFGrid
+                // replaces the exv form factor by ExvFormFactor(V = 1 A^3)
+                // (CompositeDistanceHistogramFFGrid.cpp:36-75)
                 if (kind == 0) {
                     if (f1 >= 1 && f2 >= 1) aa_[k] += s * fa[f1] * fa[f2];
-                    else if (f1 >= 1 && f2 == kExvBin) ax_[k] += 2 * p.crho * cx * s * fa[f1] * fa[kExvBin];
-                    else if (f1 == kExvBin && f2 == kExvBin) xx_[k] += std::pow(cx * p.crho, 2) * s * fa[0] * fa[0];
+                    else if (f1 >= 1 && f2 == kExvBin) ax_[k] += 2 * p.crho * cx * s * (fa[f1] * fx);
+                    else if (f1 == kExvBin && f2 == kExvBin) xx_[k] += std::pow(cx * p.crho, 2) * s * (fx * fx);
                 } else if (kind == 1) {
-                    if (f1 >= 1) aw_[k] += 2 * p.cw * s * fa[f1] * fw;
-                    else wx_[k] += 2 * p.crho * cx * p.cw * s * fa[kExvBin] * fw;
+                    if (f1 >= 1) aw_[k] += 2 * p.cw * s * (fa[f1] * fw);
+                    else wx_[k] += 2 * p.crho * cx * p.cw * s * (fx * fw);
                 } else {
                     ww_[k] += p.cw * p.cw * s * fw * fw;
                 }

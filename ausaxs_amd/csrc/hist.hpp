@@ -41,7 +41,7 @@ struct FreeParams {  // CompositeDistanceHistogramFFAvgBase free parameters
     double cw = 1, cx = 1, crho = 1, dw_atomic = 0, dw_exv = 0;
 };
 
-enum class Mode : int { FFAvg = 0, FFExplicit = 1, Simple = 2 };
+enum class Mode : int { FFAvg = 0, FFExplicit = 1, Simple = 2, FFGrid = 3 };
 
 // form factors restated from FormFactor.h:33-39 / FormFactorTable.h:28-120 /
 // ExvFormFactor.h:16-42 (default MinimumFluctuation_implicit_H volumes)
@@ -63,6 +63,11 @@ class CompositeDistanceHistogramHIP {
     // Simple:   aa, aw, ww [max_bin]
     std::vector<WeightedEntry> aa, aw, ww, p_tot;
     std::vector<double> d_axis;
+    // FFGrid only: distance axes of the atom-dummy and dummy-dummy rows
+    // (CompositeDistanceHistogramFFGrid.cpp:130-136), first max_bin entries
+    std::vector<double> d_axis_ax, d_axis_xx;
+    std::vector<WeightedEntry> p_tot_ax, p_xx;
+    double grid_volume = 0;
     FreeParams params;
 
     // ICompositeDistanceHistogram(Exv)::debye_transform on an explicit q grid;
@@ -75,7 +80,9 @@ class CompositeDistanceHistogramHIP {
 
   private:
     std::vector<double> cached_q_, cached_sinqd_;
-    std::vector<std::array<int, 3>> row_keys_;  // (kind, f1, f2): kind 0 aa, 1 aw, 2 ww
+    // (kind, f1, f2): kind 0 aa, 1 aw, 2 ww; axis_of_row_: 0 d_axis, 1 d_axis_ax, 2 d_axis_xx
+    std::vector<std::array<int, 3>> row_keys_;
+    std::vector<int> axis_of_row_;
 };
 
 // Resident structure: coordinates uploaded and work lists built once;
@@ -83,8 +90,11 @@ class CompositeDistanceHistogramHIP {
 class ResidentStructure {
   public:
     // atoms: n x 4 floats {x,y,z, ff (int bits) | weight}; waters likewise.
+    // FFGrid: exv_xyz holds the n_exv dummy lattice points (grid.hpp) and
+    // grid_volume the grid volume used for the FFAvg base (Z_exv).
     ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
-                      const HistSettings& s, double z_exv);
+                      const HistSettings& s, double z_exv, const double* exv_xyz = nullptr, int n_exv = 0,
+                      double grid_volume = 0);
     std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
     PairEngine& engine() { return *engine_; }
     Mode mode() const { return mode_; }
@@ -93,7 +103,8 @@ class ResidentStructure {
     Mode mode_;
     HistSettings s_;
     double z_exv_;
-    int n_atoms_, n_waters_;
+    int n_atoms_, n_waters_, n_exv_ = 0;
+    double grid_volume_ = 0;
     std::vector<int32_t> atom_ff_;                // FF modes
     std::vector<float> atom_w_, water_w_;         // Simple mode
     std::vector<float> atom_class_w_, water_class_w_;

@@ -28,12 +28,15 @@ extern "C" {
 #define AUSAXS_MODE_FFAVG 0      /* HistogramManagerMTFFAvg       */
 #define AUSAXS_MODE_FFEXPLICIT 1 /* HistogramManagerMTFFExplicit  */
 #define AUSAXS_MODE_SIMPLE 2     /* HistogramManagerMT (XYZW)     */
+#define AUSAXS_MODE_FFGRID 3     /* HistogramManagerMTFFGrid      */
 
 /* which distribution ausaxs_histogram_get returns */
 #define AUSAXS_AA 0
 #define AUSAXS_AW 1
 #define AUSAXS_WW 2
 #define AUSAXS_PTOT 3
+#define AUSAXS_PTOT_AX 4 /* FFGrid: weighted atom/water-dummy totals [max_bin] */
+#define AUSAXS_XX 5      /* FFGrid: weighted dummy-dummy histogram [max_bin]   */
 
 typedef struct {
     double bin_width;  /* settings::axes::bin_width (default 0.25)  */
@@ -56,6 +59,25 @@ int ausaxs_device_info(char* name, int len, int* compute_units, size_t* lds_per_
  * z_exv: FFAvg only, V_grid * 0.334 / N_atoms (HistogramManagerMTFFAvg.cpp:355-359). */
 int ausaxs_structure_create(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
                             const ausaxs_settings* settings, double z_exv, ausaxs_structure** out);
+
+/* Grid excluded-volume structure (HistogramManagerMTFFGrid, HistogramManagerMTFFGrid.cpp:43-234).
+ * Coordinates are FP64 (n x 3), as the grid is built from the FP64 atoms
+ * (Grid.cpp:34-110); they are cast to float for the pair kernels like
+ * CompactCoordinatesFF does. Dummies come from ausaxs_grid_exv. */
+int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                 const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                 ausaxs_structure** out);
+
+/* Excluded-volume dummy atoms of the default grid (RawGridExv::create,
+ * RawGridExv.cpp:10-59; Grid.cpp:34-110; SphericalExpander.h:29-90) in the
+ * reference's order, and the grid volume (Grid::get_volume, Grid.cpp:429-432).
+ * out_xyz: capacity x 3 doubles; if capacity is too small, *n_out still
+ * receives the required count and status 1 is returned. */
+int ausaxs_grid_exv(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                    int n_waters, double* out_xyz, int capacity, int* n_out, double* volume);
+
+/* FFGrid distance axes of the ax and xx rows (CompositeDistanceHistogramFFGrid.cpp:130-136): [max_bin] each. */
+int ausaxs_histogram_grid_axes(const ausaxs_histogram* h, double* d_ax, double* d_xx);
 
 /* IHistogramManager::calculate_all (IHistogramManager.h:16-29) on resident data:
  * HistogramManagerMTFFAvg.cpp:194-371, HistogramManagerMTFFExplicit.cpp:216-436,
