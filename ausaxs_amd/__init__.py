@@ -49,7 +49,7 @@ class _Settings(C.Structure):
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is not built; run `python -m ausaxs_amd.build` (hipcc, gfx950). "
+            f"{LIB_PATH} is not built; run `python ausaxs_amd/build.py` (hipcc, gfx950). "
             "There is no CPU fallback for the GPU path."
         )
     lib = C.CDLL(str(LIB_PATH))
@@ -77,6 +77,7 @@ def _load():
         "ausaxs_debye_contract": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_bin_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, vp]),
         "ausaxs_structure_set_timing": (C.c_int, [vp, C.c_int]),
+        "ausaxs_copy_probe": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_synth_globule": (
             C.c_int,
@@ -169,7 +170,7 @@ class CompositeHistogram:
 
     def _shape(self, which):
         mb = self.max_bin
-        if self.mode == MODE_SIMPLE or which in (WW, PTOT):
+        if self.mode == MODE_SIMPLE or which in (WW, PTOT, 4, 5):  # 4, 5: FFGrid p_tot_ax, xx
             return (mb,)
         return (MAX_FF, MAX_FF, mb) if which == AA else (MAX_FF, mb)
 
@@ -285,3 +286,57 @@ def synth_shell(seed, m, axes, offset=3.0):
     out = np.zeros((m, 4), np.float32)
     _check(lib.ausaxs_synth_shell(seed, m, *map(float, axes), float(offset), _ptr(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# grid excluded volume (HistogramManagerMTFFGrid)
+# ---------------------------------------------------------------------------
+MODE_FFGRID = 3
+PTOT_AX, XX = 4, 5
+_P = C.POINTER
+lib.ausaxs_structure_create_grid.restype = C.c_int
+lib.ausaxs_structure_create_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, _P(_Settings),
+                                             _P(C.c_void_p)]
+lib.ausaxs_grid_exv.restype = C.c_int
+lib.ausaxs_grid_exv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                _P(C.c_int), _P(C.c_double)]
+lib.ausaxs_histogram_grid_axes.restype = C.c_int
+lib.ausaxs_histogram_grid_axes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+
+
+def grid_exv(atoms_xyz, ff, waters_xyz=None):
+    """Excluded-volume dummies (reference order) and grid volume; host-only."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+    n, vol = C.c_int(), C.c_double()
+    rc = lib.ausaxs_grid_exv(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), None, 0, C.byref(n), C.byref(vol))
+    out = np.zeros((max(1, n.value), 3), np.float64)
+    _check(lib.ausaxs_grid_exv(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), _ptr(out), n.value, C.byref(n),
+                               C.byref(vol)))
+    return out[: n.value].copy(), vol.value
+
+
+class GridStructure(Structure):
+    """HistogramManagerMTFFGrid on resident data (FP64 input coordinates)."""
+
+    def __init__(self, atoms_xyz, ff, waters_xyz=None, settings=None):
+        self._a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+        self._f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+        self._w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)),
+                                       np.float64).reshape(-1, 3)
+        s = settings or default_settings()
+        h = C.c_void_p()
+        _check(lib.ausaxs_structure_create_grid(_ptr(self._a), _ptr(self._f), len(self._a), _ptr(self._w),
+                                                len(self._w), C.byref(s), C.byref(h)))
+        self._s = h
+
+
+def _grid_axes(self):
+    d_ax = np.zeros(self.max_bin)
+    d_xx = np.zeros(self.max_bin)
+    _check(lib.ausaxs_histogram_grid_axes(self._h, _ptr(d_ax), _ptr(d_xx)))
+    return d_ax, d_xx
+
+
+CompositeHistogram.grid_axes = _grid_axes

@@ -1,6 +1,6 @@
 """Build the in-tree HIP library (gfx950) with hipcc.
 
-    python -m ausaxs_amd.build            # -> ausaxs_amd/libausaxs_hip.so
+    python ausaxs_amd/build.py            # -> ausaxs_amd/libausaxs_hip.so
 
 Every source is compiled with -ffp-contract=off (the binning rules need
 separately rounded products and sums; FMAs are written explicitly) and with

@@ -321,6 +321,29 @@ int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long l
     });
 }
 
+int ausaxs_copy_probe(size_t bytes, int reps, double* ms) {
+    return guarded([&] {
+        require(bytes % 16 == 0 && reps > 0, "bytes must be a multiple of 16, reps > 0");
+        Context& ctx = Context::get();
+        DevBuf<char> a, b;
+        a.ensure(bytes);
+        b.ensure(bytes);
+        check(hipMemsetAsync(a.p, 1, bytes, ctx.stream), "memset");
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) check(launch_copy(a.p, b.p, bytes, ctx.stream), "copy_kernel");
+        check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
+        ctx.sync();
+        float t = 0;
+        check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        if (ms) *ms = t / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int ausaxs_synth_globule(uint64_t seed, int n, double ax, double ay, double az, double r_excl, float* out) {
     return guarded([&] {
         require(out != nullptr || n == 0, "out is NULL");

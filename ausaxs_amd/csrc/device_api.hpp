@@ -85,6 +85,10 @@ hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
 hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* mismatches,
                             hipStream_t s);
 
+// Plain copy of `bytes` (a multiple of 16) from src to dst: the reference
+// stream for calibrating HBM traffic counters.
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
 // Debye contraction on MFMA: out[r][k] = sum_b H[r][b] * sinc(q_k * d_b).
 struct DebyeLaunch {
     const double* H;   // [rows][K] device

@@ -265,7 +265,20 @@ __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi
     if (bad_cr) atomicAdd(&mism[1], bad_cr);
 }
 
+// plain streaming copy of a known number of bytes (calibration of HBM traffic counters)
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / sizeof(uint4);
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
 
 hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
     if (p.n_items == 0) return hipSuccess;

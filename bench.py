@@ -109,7 +109,19 @@ def main():
     ap.add_argument("--debye-terms", type=int, default=3)
     ap.add_argument("--tile-j", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+
+        import ausaxs_amd as A
+
+        ms = ctypes.c_double()
+        A._check(A.lib.ausaxs_copy_probe(ctypes.c_size_t(args.copy_probe), 3, ctypes.byref(ms)))
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms.value,
+                          "GB/s": 2 * args.copy_probe / ms.value / 1e6}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is design-only).",
               file=sys.stderr)

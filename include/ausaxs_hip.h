@@ -140,6 +140,11 @@ int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, con
  * are not correctly rounded. */
 int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsigned long long* mismatches);
 
+/* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming
+ * kernel, `reps` times; *ms = mean kernel time. A traffic-counter reference of
+ * known size (no counterpart in the reference). */
+int ausaxs_copy_probe(size_t bytes, int reps, double* ms);
+
 /* HIP-event timings of the last ausaxs_structure_calculate with timing on. */
 int ausaxs_structure_set_timing(ausaxs_structure* s, int on);
 int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,

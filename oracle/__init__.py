@@ -209,3 +209,82 @@ def debye_ff(mode, hist, q, bin_width, weighted=True, long_double=False, **param
     ww = contract(hist["ww"]["value"].reshape(1, mb), S, long_double).reshape(nq)
     prof = profiles(mode, q, aa, aw, ww, **params)
     return total(prof), prof
+
+
+# ---------------------------------------------------------------------------
+# grid excluded volume (FFGrid)
+# ---------------------------------------------------------------------------
+lib.orc_grid_exv.restype = C.c_int
+lib.orc_grid_exv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                             C.c_void_p]
+lib.orc_hist_ffgrid.restype = C.c_int
+lib.orc_hist_ffgrid.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
+                                C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 6
+lib.orc_weighted_axis_n.restype = None
+lib.orc_weighted_axis_n.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
+FFGRID = 2
+
+
+def grid_exv(atoms_xyz, ff, waters_xyz=None):
+    """Grid dummies (n x 3 FP64, reference order) and grid volume."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+    cap = max(1, len(a) * 64)
+    out = np.zeros((cap, 3), np.float64)
+    vol = C.c_double()
+    n = lib.orc_grid_exv(_p(a), _p(f), len(a), _p(w), len(w), _p(out), cap, C.byref(vol))
+    if n < 0:
+        raise RuntimeError("grid_exv capacity")
+    return out[:n].copy(), vol.value
+
+
+def pack_ff(xyz, ff):
+    out = np.zeros((len(xyz), 4), np.float32)
+    out[:, :3] = np.asarray(xyz, np.float64).astype(np.float32)
+    out.view(np.int32)[:, 3] = ff
+    return out
+
+
+def hist_ffgrid(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1):
+    """HistogramManagerMTFFGrid::calculate_all from FP64 coordinates."""
+    w = np.zeros((0, 3)) if waters_xyz is None else np.asarray(waters_xyz, np.float64)
+    dummies, vol = grid_exv(atoms_xyz, ff, w)
+    atoms = pack_ff(atoms_xyz, ff)
+    waters = pack_ff(w, np.ones(len(w), np.int32))
+    xs = pack_ff(dummies, np.zeros(len(dummies), np.int32))
+    nb = bin_count
+    aa = np.zeros(MAX_FF * MAX_FF * nb, ENTRY)
+    aw = np.zeros(MAX_FF * nb, ENTRY)
+    ww, pt, pax, pxx = (np.zeros(nb, ENTRY) for _ in range(4))
+    mb = lib.orc_hist_ffgrid(_p(atoms), len(atoms), _p(waters), len(waters), _p(xs), len(xs), vol,
+                             C.c_float(inv_width(bin_width)), nb, nthreads, _p(aa), _p(aw), _p(ww), _p(pt), _p(pax),
+                             _p(pxx))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    d_ax = np.zeros(mb)
+    d_xx = np.zeros(mb)
+    lib.orc_weighted_axis_n(_p(pax), mb, nb, bin_width, _p(d_ax))
+    lib.orc_weighted_axis_n(_p(pxx), mb, nb, bin_width, _p(d_xx))
+    return dict(max_bin=mb, aa=aa.reshape(MAX_FF, MAX_FF, nb)[:, :, :mb], aw=aw.reshape(MAX_FF, nb)[:, :mb],
+                ww=ww[:mb], p_tot=pt[:mb], p_tot_ax=pax[:mb], p_xx=pxx[:mb], d_ax=d_ax, d_xx=d_xx, volume=vol,
+                dummies=dummies)
+
+
+def debye_ffgrid(hist, q, bin_width, long_double=False, **params):
+    """I(q) and profiles of an oracle FFGrid histogram (CompositeDistanceHistogramFFGrid.cpp:77-119)."""
+    mb = hist["max_bin"]
+    d = weighted_axis(hist["p_tot"], bin_width)  # count 0: the centres themselves, d[0] = 0
+    S_aa, S_ax, S_xx = sinc_table(q, d), sinc_table(q, hist["d_ax"]), sinc_table(q, hist["d_xx"])
+    nq = len(q)
+    aa_v = hist["aa"]["value"]
+    sa = contract(aa_v.reshape(-1, mb), S_aa, long_double).reshape(MAX_FF, MAX_FF, nq)
+    sx = contract(aa_v[:, 0, :], S_ax, long_double)
+    sa[:, 0, :] = sx
+    sa[0, 0, :] = contract(aa_v[0, 0][None, :], S_xx, long_double)[0]
+    aw_v = hist["aw"]["value"]
+    sw = contract(aw_v, S_aa, long_double)
+    sw[0] = contract(aw_v[0][None, :], S_ax, long_double)[0]
+    s_ww = contract(hist["ww"]["value"][None, :], S_aa, long_double)[0]
+    prof = profiles(FFGRID, q, sa, sw, s_ww, **params)
+    return total(prof), prof

@@ -365,6 +365,279 @@ int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int 
 }
 
 /* ------------------------------------------------------------------------ */
+/* grid excluded volume                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    orc_entry* h;
+} simple_priv;
+
+enum { G_EMPTY = 0, G_ACENTER = 1, G_AAREA = 2, G_WCENTER = 4, G_WAREA = 8, G_VOLUME = 16 };
+
+/* constants::radius::get_vdw_radius(form_factor_t) (FormFactorType.h:174-193), vdwTable.h */
+static double orc_vdw(int t) {
+    switch (t) {
+        case 2: return 1.1;
+        case 3: case 4: case 5: case 6: return 1.96;
+        case 7: case 8: case 9: case 10: return 1.79;
+        case 1: case 11: return 1.71;
+        case 12: case 13: return 2.06;
+        case 14: return 1.88;
+        default: return -1;
+    }
+}
+
+int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
+                 int cap, double* volume) {
+    const double width = 1.0, rvol = 2.15;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (atoms[3 * i + k] < mn[k]) mn[k] = atoms[3 * i + k];
+            if (atoms[3 * i + k] > mx[k]) mx[k] = atoms[3 * i + k];
+        }
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (waters[3 * i + k] < mn[k]) mn[k] = waters[3 * i + k];
+            if (waters[3 * i + k] > mx[k]) mx[k] = waters[3 * i + k];
+        }
+    double scaling = 0.25;
+    if (mx[0] - mn[0] < 50 || mx[1] - mn[1] < 50 || mx[2] - mn[2] < 50) scaling = 1;
+    const double margin = 2.14 + 1.71 + (-0.35) + 3 * width; /* vdw P + vdw O + shell_correction + 3 cells */
+    double gmin[3];
+    int bins[3];
+    for (int k = 0; k < 3; ++k) {
+        double diff = mx[k] - mn[k];
+        double expand = 0.5 * diff * scaling > margin ? 0.5 * diff * scaling : margin;
+        gmin[k] = floor(mn[k] - expand - width);
+        double gmax = ceil(mx[k] + expand + width);
+        bins[k] = (int)((gmax - gmin[k]) / width);
+    }
+    size_t total = (size_t)bins[0] * bins[1] * bins[2];
+    uint8_t* g = (uint8_t*)calloc(total, 1);
+#define G(i, j, k) g[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
+    long vol = 0;
+    int* loc = (int*)malloc(sizeof(int) * 3 * (size_t)(n > 0 ? n : 1));
+    /* Grid::add: atoms then waters (one body) */
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) loc[3 * i + k] = (int)round((atoms[3 * i + k] - gmin[k]) / width);
+        uint8_t* s = &G(loc[3 * i], loc[3 * i + 1], loc[3 * i + 2]);
+        vol += (*s == G_EMPTY || (*s & (G_WAREA | G_WCENTER))); /* is_empty_or_water */
+        *s |= G_ACENTER;
+    }
+    for (int i = 0; i < m; ++i) {
+        int l[3];
+        for (int k = 0; k < 3; ++k) l[k] = (int)round((waters[3 * i + k] - gmin[k]) / width);
+        G(l[0], l[1], l[2]) |= G_WCENTER;
+    }
+    /* expand_volume: atoms first (waters afterwards only add W_AREA marks) */
+    for (int a = 0; a < n; ++a) {
+        double rvdw = orc_vdw(ff[a]);
+        double rvdw2 = pow(rvdw, 2), rvol2 = pow(rvol, 2);
+        double br = ceil((rvdw > rvol ? rvdw : rvol) / width);
+        int bx = loc[3 * a], by = loc[3 * a + 1], bz = loc[3 * a + 2];
+        int xm = bx - br > 0 ? (int)(bx - br) : 0, xp = bx + br + 1 < bins[0] ? (int)(bx + br + 1) : bins[0];
+        int ym = by - br > 0 ? (int)(by - br) : 0, yp = by + br + 1 < bins[1] ? (int)(by + br + 1) : bins[1];
+        int zm = bz - br > 0 ? (int)(bz - br) : 0, zp = bz + br + 1 < bins[2] ? (int)(bz + br + 1) : bins[2];
+        const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+        for (int i = xm; i < xp; ++i) {
+            double x2 = pow(gmin[0] + i * width - ax, 2);
+            for (int j = ym; j < yp; ++j) {
+                double x2y2 = x2 + pow(gmin[1] + j * width - ay, 2);
+                for (int k = zm; k < zp; ++k) {
+                    double dist = x2y2 + pow(gmin[2] + k * width - az, 2);
+                    uint8_t* s = &G(i, j, k);
+                    int fill = dist <= rvdw2;
+                    if (fill) {
+                        if (!(*s == G_EMPTY || (*s & G_VOLUME))) continue; /* is_empty_or_volume */
+                        vol += !(*s & G_VOLUME);
+                        *s |= G_AAREA;
+                    }
+                    if (!fill && dist <= rvol2) {
+                        if (*s != G_EMPTY) continue;
+                        *s |= G_VOLUME;
+                        vol++;
+                    }
+                }
+            }
+        }
+    }
+    /* RawGridExv::create over the atoms' bounding_box_index +- buffer */
+    int imin[3] = {1 << 30, 1 << 30, 1 << 30}, imax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int a = 0; a < n; ++a)
+        for (int k = 0; k < 3; ++k) {
+            if (imin[k] > loc[3 * a + k]) imin[k] = loc[3 * a + k];
+            if (imax[k] < loc[3 * a + k]) imax[k] = loc[3 * a + k] + 1;
+        }
+    if (n == 0)
+        for (int k = 0; k < 3; ++k) imin[k] = imax[k] = 0;
+    const int buffer = 2; /* max(1, round(max(2.15, 2)/1)) */
+    int count = 0, ok = 1;
+    for (int i = imin[0] - buffer > 0 ? imin[0] - buffer : 0; i < (imax[0] + buffer + 1 < bins[0] ? imax[0] + buffer + 1 : bins[0]); ++i)
+        for (int j = imin[1] - buffer > 0 ? imin[1] - buffer : 0; j < (imax[1] + buffer + 1 < bins[1] ? imax[1] + buffer + 1 : bins[1]); ++j)
+            for (int k = imin[2] - buffer > 0 ? imin[2] - buffer : 0; k < (imax[2] + buffer + 1 < bins[2] ? imax[2] + buffer + 1 : bins[2]); ++k) {
+                if (!(G(i, j, k) & (G_VOLUME | G_ACENTER | G_AAREA))) continue;
+                if (count < cap) {
+                    out_xyz[3 * count] = gmin[0] + i * width;
+                    out_xyz[3 * count + 1] = gmin[1] + j * width;
+                    out_xyz[3 * count + 2] = gmin[2] + k * width;
+                } else {
+                    ok = 0;
+                }
+                ++count;
+            }
+#undef G
+    free(g);
+    free(loc);
+    if (volume) *volume = vol * width * width * width;
+    return ok ? count : -1;
+}
+
+/* grid pair loops: xx self (factor 2, 1D), ax atoms x dummies (factor 1,
+ * [ff_atom]), wx waters x dummies (factor 1, 1D) —
+ * HistogramManagerMTFFGrid.cpp:68-137, TemplateHelperGrid.h */
+typedef struct {
+    const float* rows;
+    int nr;
+    const float* cols;
+    int nc;
+    float inv_w;
+    int nb;
+    int self;  /* xx */
+    int typed; /* ax: 2D by row ff */
+} grid_args;
+
+static int grid_row(job_ctx* c, void* pv, int i) {
+    grid_args* A = (grid_args*)c->args;
+    orc_entry* h = ((simple_priv*)pv)->h;
+    const float* pi = A->rows + 4 * (size_t)i;
+    const int m = A->self ? A->nr : A->nc;
+    const float* cols = A->self ? A->rows : A->cols;
+    const int j0 = A->self ? i + 1 : 0;
+    const int tail = A->self ? m - ((m - i - 1) % 4) : m - (m % 4);
+    const int factor = A->self ? 2 : 1;
+    const size_t off = A->typed ? (size_t)ff_of(pi) * A->nb : 0;
+    for (int j = j0; j < m; ++j) {
+        float d;
+        int32_t bin;
+        if (j < tail) rule_simd(pi, cols + 4 * (size_t)j, A->inv_w, &d, &bin);
+        else rule_scalar(pi, cols + 4 * (size_t)j, A->inv_w, &d, &bin);
+        if (bin < 0 || bin >= A->nb) return -2;
+        inc_n(&h[off + bin], factor, d);
+    }
+    return 0;
+}
+
+static int run_grid(const float* rows, int nr, const float* cols, int nc, int self, int typed, float inv_w, int nb,
+                    int nthreads, orc_entry* out) {
+    const size_t n = (size_t)(typed ? ORC_MAX_FF : 1) * nb;
+    simple_priv* privs = (simple_priv*)calloc(nthreads, sizeof(simple_priv));
+    void** pp = (void**)malloc(sizeof(void*) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        privs[t].h = (orc_entry*)calloc(n, sizeof(orc_entry));
+        pp[t] = &privs[t];
+    }
+    grid_args A = {rows, nr, cols, nc, inv_w, nb, self, typed};
+    int err = run_rows(nr, nthreads, grid_row, &A, pp);
+    memset(out, 0, n * sizeof(orc_entry));
+    for (int t = 0; t < nthreads; ++t) {
+        for (size_t b = 0; b < n; ++b) entry_add(&out[b], &privs[t].h[b]);
+        free(privs[t].h);
+    }
+    free(privs);
+    free(pp);
+    return err;
+}
+
+static void weighted_axis_n(const orc_entry* p, int n_entries, int n_axis, double bin_width, double* d_out) {
+    double width = (n_axis * bin_width) / n_axis;
+    for (int i = 0; i < n_entries; ++i) {
+        double dv = i * width;
+        double bc = p[i].bin_center;
+        d_out[i] = ((double)(!bc) * dv + bc) / (double)((!p[i].count) + p[i].count);
+    }
+}
+
+int orc_hist_ffgrid(const float* atoms, int n_a, const float* waters, int n_w, const float* dummies, int n_x,
+                    double volume, float inv_w, int nb, int nthreads, orc_entry* aa, orc_entry* aw, orc_entry* ww,
+                    orc_entry* ptot, orc_entry* ptot_ax, orc_entry* pxx) {
+    if (nthreads < 1) nthreads = 1;
+    /* HistogramManagerMTFFAvg<true, vbw>::calculate_all with Z from the grid volume */
+    double z = n_a == 0 ? 0 : volume * 0.334 / n_a;
+    int base = orc_hist_ff(ORC_FFAVG, atoms, n_a, waters, n_w, inv_w, nb, z, nthreads, aa, aw, ww, ptot);
+    if (base < 0) return base;
+    orc_entry* xx = (orc_entry*)calloc(nb, sizeof(orc_entry));
+    orc_entry* ax = (orc_entry*)calloc((size_t)ORC_MAX_FF * nb, sizeof(orc_entry));
+    orc_entry* wx = (orc_entry*)calloc(nb, sizeof(orc_entry));
+    int err = 0;
+    err |= run_grid(dummies, n_x, NULL, 0, 1, 0, inv_w, nb, nthreads, xx);
+    err |= run_grid(atoms, n_a, dummies, n_x, 0, 1, inv_w, nb, nthreads, ax);
+    err |= run_grid(waters, n_w, dummies, n_x, 0, 0, inv_w, nb, nthreads, wx);
+    if (err) {
+        free(xx), free(ax), free(wx);
+        return -2;
+    }
+    {
+        orc_entry e = {(double)n_x, (uint32_t)n_x, 0};
+        entry_add(&xx[0], &e);
+    }
+    int mbx = 10;
+    for (int i = nb - 1; i >= 10; --i)
+        if (xx[i].value != 0 || wx[i].value != 0) {
+            mbx = i + 1;
+            break;
+        }
+    /* weighted counts of the FFAvg result at the first default q (FFAvgBase.cpp:77-109) */
+    double* dax = (double*)calloc(base, sizeof(double));
+    orc_weighted_axis(ptot, base, 1.0 / inv_w, dax);
+    double* wc = (double*)calloc(base, sizeof(double));
+    const double q0 = 1e-4;
+    for (int f1 = 1; f1 < ORC_MAX_FF; ++f1)
+        for (int f2 = 1; f2 < ORC_MAX_FF; ++f2) {
+            double w = orc_ff_atomic(f1, q0) * orc_ff_atomic(f2, q0);
+            for (int b = 0; b < base; ++b) wc[b] += aa[IDX3(f1, f2, b, nb)].value * w;
+        }
+    for (int f = 1; f < ORC_MAX_FF; ++f) {
+        double w = 2 * 1.0 * (orc_ff_atomic(f, q0) * orc_ff_atomic(ORC_WATER, q0));
+        for (int b = 0; b < base; ++b) wc[b] += aw[IDX2(f, b, nb)].value * w;
+    }
+    {
+        double w = 1.0 * 1.0 * (orc_ff_atomic(ORC_WATER, q0) * orc_ff_atomic(ORC_WATER, q0));
+        for (int b = 0; b < base; ++b) wc[b] += ww[b].value * w;
+    }
+    int max_bin = mbx > base ? mbx : base;
+    memset(ptot, 0, (size_t)nb * sizeof(orc_entry));
+    for (int b = 0; b < base; ++b) {
+        ptot[b].value = wc[b];
+        ptot[b].count = 0;
+        ptot[b].bin_center = dax[b];
+    }
+    /* FFAvg rows beyond its own size are zero */
+    for (int f1 = 0; f1 < ORC_MAX_FF; ++f1) {
+        for (int f2 = 0; f2 < ORC_MAX_FF; ++f2)
+            for (int b = base; b < nb; ++b) memset(&aa[IDX3(f1, f2, b, nb)], 0, sizeof(orc_entry));
+        for (int b = base; b < nb; ++b) memset(&aw[IDX2(f1, b, nb)], 0, sizeof(orc_entry));
+    }
+    for (int b = base; b < nb; ++b) memset(&ww[b], 0, sizeof(orc_entry));
+    memset(ptot_ax, 0, (size_t)nb * sizeof(orc_entry));
+    for (int b = 0; b < max_bin; ++b) entry_add(&ptot_ax[b], &wx[b]);
+    for (int f = 0; f < ORC_MAX_FF; ++f)
+        for (int b = 0; b < max_bin; ++b) entry_add(&ptot_ax[b], &ax[IDX2(f, b, nb)]);
+    for (int f = 0; f < ORC_MAX_FF; ++f)
+        for (int b = 0; b < max_bin; ++b) aa[IDX3(f, ORC_EXV, b, nb)] = ax[IDX2(f, b, nb)];
+    for (int b = 0; b < max_bin; ++b) aw[IDX2(ORC_EXV, b, nb)] = wx[b];
+    for (int b = 0; b < max_bin; ++b) aa[IDX3(ORC_EXV, ORC_EXV, b, nb)] = xx[b];
+    memcpy(pxx, xx, (size_t)nb * sizeof(orc_entry));
+    free(xx), free(ax), free(wx), free(dax), free(wc);
+    return max_bin;
+}
+
+/* weighted axis of a histogram that is n_axis bins long (first n_entries) */
+void orc_weighted_axis_n(const orc_entry* p, int n_entries, int n_axis, double bin_width, double* d_out) {
+    weighted_axis_n(p, n_entries, n_axis, bin_width, d_out);
+}
+
+/* ------------------------------------------------------------------------ */
 /* raw pair histogram                                                       */
 /* ------------------------------------------------------------------------ */
 
@@ -406,9 +679,6 @@ typedef struct {
     int self;
 } simple_args;
 
-typedef struct {
-    orc_entry* h;
-} simple_priv;
 
 static int simple_row(job_ctx* c, void* pv, int i) {
     simple_args* A = (simple_args*)c->args;
@@ -637,8 +907,11 @@ void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, con
     const double rm2 = 1.62 * 1.62 / 4;
     for (int k = 0; k < nq; ++k) {
         double qk = q[k];
-        double cx = mode == ORC_FFAVG ? p.cx : pow(p.cx, 3) * exp(-rm2 * (pow(p.cx, 2) - 1) * qk * qk);
+        double cx = mode != ORC_FFEXPLICIT ? p.cx : pow(p.cx, 3) * exp(-rm2 * (pow(p.cx, 2) - 1) * qk * qk);
         double fw = orc_ff_atomic(ORC_WATER, qk);
+        /* exv form factor: FFAvg's raw EXCLUDED_VOLUME entry, or FFGrid's
+         * ExvFormFactor(V = 1 A^3) (CompositeDistanceHistogramFFGrid.cpp:36-75) */
+        double fx = mode == ORC_FFGRID ? 0.334 * exp(-(1.0 / (4 * PI_)) * qk * qk) : orc_ff_atomic(ORC_EXV, qk);
         for (int f1 = 1; f1 < ORC_MAX_FF; ++f1) {
             double a1 = orc_ff_atomic(f1, qk);
             for (int f2 = 1; f2 < ORC_MAX_FF; ++f2) {
@@ -652,8 +925,7 @@ void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, con
                     xx[k] += pow(cx * p.crho, 2) * s * orc_ff_exv(f1, qk) * orc_ff_exv(f2, qk);
                 }
             }
-            if (mode == ORC_FFAVG) {
-                double fx = orc_ff_atomic(ORC_EXV, qk);
+            if (mode != ORC_FFEXPLICIT) {
                 ax[k] += 2 * p.crho * cx * S3(f1, ORC_EXV, k) * a1 * fx;
                 aw[k] += 2 * p.cw * S2(f1, k) * a1 * fw;
             } else {
@@ -661,8 +933,7 @@ void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, con
                 wx[k] += p.crho * cx * p.cw * S2(f1, k) * (fw * orc_ff_exv(f1, qk) + a1 * orc_ff_exv(ORC_WATER, qk));
             }
         }
-        if (mode == ORC_FFAVG) {
-            double fx = orc_ff_atomic(ORC_EXV, qk);
+        if (mode != ORC_FFEXPLICIT) {
             xx[k] += pow(cx * p.crho, 2) * S3(ORC_EXV, ORC_EXV, k) * fx * fx;
             wx[k] += 2 * p.crho * cx * p.cw * S2(ORC_EXV, k) * fx * fw;
         }

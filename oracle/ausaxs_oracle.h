@@ -30,7 +30,7 @@ typedef struct {
     double bin_center;
 } orc_entry;
 
-enum { ORC_FFAVG = 0, ORC_FFEXPLICIT = 1 };
+enum { ORC_FFAVG = 0, ORC_FFEXPLICIT = 1, ORC_FFGRID = 2 };
 
 /* ---- binning rules (CompactCoordinatesXYZFF.h) -------------------------- */
 /* rule 0: SIMD body (AVX2 :514-545, SSE :397-420): d2=(dx²+dy²)+dz², sqrt_rn,
@@ -58,6 +58,30 @@ int orc_rule_cross(int m, int j);
 int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int n_w,
                 float inv_w, int bin_count, double z_exv, int nthreads,
                 orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
+/* Grid excluded-volume dummies, restating the reference's voxel-state
+ * procedure literally: Grid(bodies) (Grid.cpp:34-110, to_bins :414-419),
+ * Grid::add (:300-352), SphericalExpander<true,false>::expand_volume for the
+ * atoms (SphericalExpander.h:29-90), GridObj state predicates
+ * (GridObj.cpp:16-43) and RawGridExv::create (RawGridExv.cpp:10-59).
+ * atoms/waters: n x 3 FP64; out_xyz: cap x 3. Returns the dummy count
+ * (or -1 if cap is too small); *volume = Grid::get_volume(). */
+int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
+                 int cap, double* volume);
+
+/* HistogramManagerMTFFGrid::calculate_all (HistogramManagerMTFFGrid.cpp:43-234)
+ * on top of the FFAvg manager. atoms/waters: n x 4 floats {x,y,z,ff} as for
+ * orc_hist_ff; dummies: nx x 4 floats (integer lattice points); volume: grid
+ * volume. Outputs (each sized as in orc_hist_ff, bin_count wide): aa, aw, ww,
+ * ptot (FFGrid p_tot: value = weighted counts, bin_center = FFAvg d axis),
+ * ptot_ax, pxx. Returns max_bin or a negative status. */
+int orc_hist_ffgrid(const float* atoms, int n_a, const float* waters, int n_w, const float* dummies, int n_x,
+                    double volume, float inv_w, int bin_count, int nthreads, orc_entry* aa, orc_entry* aw,
+                    orc_entry* ww, orc_entry* ptot, orc_entry* ptot_ax, orc_entry* pxx);
+
+/* weighted axis of a histogram n_axis bins long, first n_entries values, no
+ * forced zero at bin 0 (the FFGrid ax/xx axes, CompositeDistanceHistogramFFGrid.cpp:25-27) */
+void orc_weighted_axis_n(const orc_entry* p, int n_entries, int n_axis, double bin_width, double* d_out);
 
 /* Raw pair histogram of one pair set without manager conventions:
  * self (b == NULL) or cross (rows a, cols b); factor = reference template N.

@@ -20,6 +20,14 @@ def rel_close(a, b, rtol, atol=1e-9):
     return np.all(np.abs(a - b) <= atol + rtol * np.abs(b))
 
 
+def axis_ok(d, d_ref, bin_width):
+    """Bin centres from the device: fixed-point delta sums (mean error <= 2^-21
+    bins) of distances from v_sqrt_f32 (within 1 ulp, so <= 2^-23 relative;
+    bins themselves are exact)."""
+    d, d_ref = np.asarray(d), np.asarray(d_ref)
+    return np.all(np.abs(d - d_ref) <= bin_width * 2.0 ** -21 + 2.0 ** -22 * np.abs(d_ref))
+
+
 def test_device_is_gfx950():
     info = A.device_info()
     print(info)
@@ -212,8 +220,7 @@ def test_intensity_parity(mode):
     # shift of every centre moves the oracle's own I(q) by up to 1.5e-6.
     for terms, tol in ((3, 1e-4), (0, 2e-5)):
         h = A.calculate_all(mode, atoms, waters, settings=A.default_settings(debye_terms=terms), z_exv=0.5)
-        d_ref = O.weighted_axis(ref_h["p_tot"], 0.25)
-        assert np.max(np.abs(h.d_axis() - d_ref)) <= 0.25 * 2.0 ** -21 * 1.01
+        assert axis_ok(h.d_axis(), O.weighted_axis(ref_h["p_tot"], 0.25), 0.25)
         h.set_params(**params)
         I, P = h.debye_transform(q, profiles=True)
         err = relerr(I, I_ref)
@@ -260,3 +267,51 @@ def test_large_structure_invariants():
     assert h.get(A.AA)[1].sum(dtype=np.int64) == n * (n - 1) + n
     assert h.get(A.AW)[1].sum(dtype=np.int64) == n * m
     assert h.get(A.WW)[1].sum(dtype=np.int64) == m * (m - 1) + m
+
+
+# --------------------------------------------------------------------------
+# grid excluded volume (HistogramManagerMTFFGrid)
+# --------------------------------------------------------------------------
+def test_ffgrid_parity():
+    a = A.synth_globule(6, 700, (12, 12, 12), 2.0)
+    xyz = a[:, :3].astype(np.float64)
+    ff = a.view(np.int32)[:, 3]
+    w = A.synth_shell(7, 180, (12, 12, 12), 3.0)[:, :3].astype(np.float64)
+    h = A.GridStructure(xyz, ff, w).calculate()
+    ref = O.hist_ffgrid(xyz, ff, w, nthreads=4)
+    assert h.max_bin == ref["max_bin"]
+    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww"), (A.PTOT_AX, "p_tot_ax"), (A.XX, "p_xx")):
+        v, c, b = h.get(which)
+        rv = ref[key]
+        assert np.array_equal(c, rv["count"]), key
+        np.testing.assert_allclose(v, rv["value"], rtol=1e-14, err_msg=key)
+        m = rv["count"] > 0
+        assert rel_close(b[m], rv["bin_center"][m], 1e-5, atol=1e-6), key
+    v, c, b = h.get(A.PTOT)
+    np.testing.assert_allclose(v, ref["p_tot"]["value"], rtol=1e-12)
+    assert np.all(c == 0)
+    np.testing.assert_allclose(b, ref["p_tot"]["bin_center"], rtol=1e-6, atol=1e-7)
+    d_ax, d_xx = h.grid_axes()
+    assert axis_ok(d_ax, ref["d_ax"], 0.25) and axis_ok(d_xx, ref["d_xx"], 0.25)
+    q = np.linspace(0.0, 0.5, 150)
+    params = dict(cw=1.05, cx=0.98, crho=1.02, dw_atomic=0.1, dw_exv=0.05)
+    I_ref, _ = O.debye_ffgrid(ref, q, 0.25, long_double=True, **params)
+    h.set_params(**params)
+    err = relerr(h.debye_transform(q), I_ref)
+    print(f"ffgrid I(q) max rel err {err:.3e}")
+    assert err <= 1e-4
+
+
+def test_ffgrid_large_invariants():
+    # 20k-atom globule (radius 45 A) + 6k waters: exact pair-count identities
+    a = A.synth_globule(2, 20000, (45, 45, 45), 2.0)
+    xyz = a[:, :3].astype(np.float64)
+    ff = a.view(np.int32)[:, 3]
+    w = A.synth_shell(3, 6000, (45, 45, 45), 3.0)[:, :3].astype(np.float64)
+    nx = len(A.grid_exv(xyz, ff, w)[0])
+    h = A.GridStructure(xyz, ff, w).calculate()
+    n, m = len(xyz), len(w)
+    _, c, _ = h.get(A.AA)
+    assert c[0, 0].sum(dtype=np.int64) == nx * (nx - 1) + nx
+    assert c[1:, 0].sum(dtype=np.int64) == n * nx
+    assert h.get(A.AW)[1][0].sum(dtype=np.int64) == m * nx
