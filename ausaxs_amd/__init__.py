@@ -271,9 +271,9 @@ def debye_contract(H, d, q, terms=3):
 
 
 def bin_check(lo_bits, hi_bits, bin_width):
-    m = np.zeros(2, np.uint64)
+    m = np.zeros(3, np.uint64)
     _check(lib.ausaxs_bin_check(lo_bits, hi_bits, bin_width, _ptr(m)))
-    return int(m[0]), int(m[1])
+    return int(m[0]), int(m[1]), int(m[2])
 
 
 def synth_globule(seed, n, axes, r_excl):

@@ -313,10 +313,10 @@ int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long l
         require(mism && lo <= hi, "bad argument");
         Context& ctx = Context::get();
         DevBuf<unsigned long long> d;
-        d.ensure(2);
-        check(hipMemsetAsync(d.p, 0, 2 * sizeof(unsigned long long), ctx.stream), "memset");
+        d.ensure(3);
+        check(hipMemsetAsync(d.p, 0, 3 * sizeof(unsigned long long), ctx.stream), "memset");
         check(launch_bin_check(lo, hi, (float)(1.0 / bin_width), d.p, ctx.stream), "bin_check_kernel");
-        check(hipMemcpyAsync(mism, d.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream), "D2H");
+        check(hipMemcpyAsync(mism, d.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream), "D2H");
         ctx.sync();
     });
 }

@@ -55,15 +55,31 @@ struct PairLaunch {
     uint64_t* partial;        // [n_items][2][nbins+1]
     float inv_w;
     int nbins;                // LDS bins per row; slot nbins collects overflow
-    int weighted;             // reserved
+    int clamp;                // 0 only if every pair's bin is known to be < nbins
+    int grid;                 // workgroups; 0 or >= n_items: one per item, else persistent (strided)
 };
 
 hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s);
 
-// Sum partial slots into final rows. row_ptr/row_list: CSR over global rows,
-// entries = item*2 + local_row. cnt/dsum: [n_rows][nbins+1].
-hipError_t launch_merge(const uint64_t* partial, const int32_t* row_ptr, const int32_t* row_list, int n_rows,
-                        int nbins, uint64_t* cnt, int64_t* dsum, hipStream_t s);
+// Sum partial slots into final rows in two passes. row_list holds partial
+// entries (item*2 + local_row) grouped by output row; the entries of one row
+// are cut into chunks of at most kMergeChunk: chunk c = row_list[chunk_ptr[c]
+// .. chunk_ptr[c+1]), and row r owns chunks row_chunk[r] .. row_chunk[r+1].
+// tmp_cnt/tmp_dsum: [n_chunks][nbins+1]; cnt/dsum: [n_rows][nbins+1]. Every
+// output cell is written (rows without entries become zero).
+constexpr int kMergeChunk = 32;
+struct MergeLaunch {
+    const uint64_t* partial;
+    const int32_t* row_list;
+    const int32_t* chunk_ptr;  // [n_chunks+1]
+    const int32_t* row_chunk;  // [n_rows+1]
+    int n_chunks, n_rows, nbins;
+    uint64_t* tmp_cnt;
+    int64_t* tmp_dsum;
+    uint64_t* cnt;
+    int64_t* dsum;
+};
+hipError_t launch_merge(const MergeLaunch& m, hipStream_t s);
 
 struct StripLaunch {
     const float4* rows;       // original-order row points (tag = class id bits)

@@ -63,6 +63,11 @@ PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, in
         throw std::runtime_error("ausaxs_hip: " + std::to_string(nbins) + " bins need " + std::to_string(lds) +
                                  " B of LDS per workgroup, device offers " +
                                  std::to_string(ctx.prop.sharedMemPerBlock));
+    // bins need no clamp in the pair kernel when the bounding-box diagonal
+    // already keeps every pair below nbins
+    std::vector<const PointSet*> ps;
+    for (const auto& x : sets_) ps.push_back(&x);
+    clamp_ = nbins_ < bin_bound(ps, inv_w_);
     build(tile_j);
     for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
 }
@@ -208,11 +213,17 @@ void PairEngine::build(int tile_j) {
         per_row[items_[k].row0].push_back((int32_t)(2 * k));
         if (items_[k].flags & kOrient) per_row[items_[k].row1].push_back((int32_t)(2 * k + 1));
     }
-    row_ptr_.assign(1, 0);
+    // entries grouped by row, cut into merge chunks of at most kMergeChunk
     row_list_.clear();
+    chunk_ptr_.assign(1, 0);
+    row_chunk_.assign(1, 0);
     for (auto& v : per_row) {
-        row_list_.insert(row_list_.end(), v.begin(), v.end());
-        row_ptr_.push_back((int32_t)row_list_.size());
+        for (size_t k = 0; k < v.size(); k += kMergeChunk) {
+            const size_t k1 = std::min(v.size(), k + kMergeChunk);
+            row_list_.insert(row_list_.end(), v.begin() + k, v.begin() + k1);
+            chunk_ptr_.push_back((int32_t)row_list_.size());
+        }
+        row_chunk_.push_back((int32_t)chunk_ptr_.size() - 1);
     }
 
     // ---- device residency -------------------------------------------------
@@ -224,8 +235,11 @@ void PairEngine::build(int tile_j) {
         d_orig_.back().upload(ps.orig.data(), ps.orig.size(), s);
     }
     d_items_.upload(items_.data(), items_.size(), s);
-    d_row_ptr_.upload(row_ptr_.data(), row_ptr_.size(), s);
     d_row_list_.upload(row_list_.data(), row_list_.size(), s);
+    d_chunk_ptr_.upload(chunk_ptr_.data(), chunk_ptr_.size(), s);
+    d_row_chunk_.upload(row_chunk_.data(), row_chunk_.size(), s);
+    d_tmp_cnt_.ensure((chunk_ptr_.size() - 1) * (nbins_ + 1));
+    d_tmp_dsum_.ensure((chunk_ptr_.size() - 1) * (nbins_ + 1));
     d_rowmaps_.upload(rowmaps_.data(), rowmaps_.size(), s);
     d_partial_.ensure(std::max<size_t>(items_.size(), 1) * 2 * (nbins_ + 1));
     d_cnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
@@ -237,16 +251,12 @@ void PairEngine::enqueue(bool timed) {
     Context& ctx = Context::get();
     hipStream_t s = ctx.stream;
     if (timed) check(hipEventRecord(ev_[0], s), "hipEventRecord");
-    PairLaunch pl{d_pts_.p, d_items_.p, (int)items_.size(), d_partial_.p, inv_w_, nbins_, 1};
+    PairLaunch pl{d_pts_.p, d_items_.p, (int)items_.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     if (timed) check(hipEventRecord(ev_[1], s), "hipEventRecord");
-    if (items_.empty()) {
-        check(hipMemsetAsync(d_cnt_.p, 0, (size_t)n_rows_ * (nbins_ + 1) * sizeof(uint64_t), s), "memset");
-        check(hipMemsetAsync(d_dsum_.p, 0, (size_t)n_rows_ * (nbins_ + 1) * sizeof(int64_t), s), "memset");
-    } else {
-        check(launch_merge(d_partial_.p, d_row_ptr_.p, d_row_list_.p, n_rows_, nbins_, d_cnt_.p, d_dsum_.p, s),
-              "merge_kernel");
-    }
+    MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, (int)chunk_ptr_.size() - 1,
+                   n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_cnt_.p, d_dsum_.p};
+    check(launch_merge(ml, s), "merge kernels");
     for (const Strip& st : strips_) {
         StripLaunch sl{d_orig_[st.set_rows].p, d_orig_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
                        d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};

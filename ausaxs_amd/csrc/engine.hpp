@@ -112,6 +112,9 @@ class PairEngine {
     const uint64_t* device_cnt() const { return d_cnt_.p; }
     const int64_t* device_dsum() const { return d_dsum_.p; }
     bool overflowed(const std::vector<uint64_t>& cnt) const;
+    // workgroups of the pair kernel: 0 = one per item, else a persistent grid
+    void set_grid(int g) { grid_ = g; }
+    bool clamped() const { return clamp_; }
 
   private:
     struct Layout {
@@ -123,7 +126,7 @@ class PairEngine {
     std::vector<PairJob> jobs_;
     std::vector<Layout> layout_;
     std::vector<PairWork> items_;
-    std::vector<int32_t> row_ptr_, row_list_;
+    std::vector<int32_t> row_list_, chunk_ptr_, row_chunk_;
     struct Strip {
         int set_rows, set_cols;
         StripWork w;
@@ -133,14 +136,16 @@ class PairEngine {
     std::vector<int32_t> rowmaps_;
     int n_rows_, nbins_;
     float inv_w_;
+    bool clamp_ = true;
+    int grid_ = 0;
     int64_t pairs_ = 0;
 
     DevBuf<float4> d_pts_;
     std::vector<DevBuf<float4>> d_orig_;
     DevBuf<PairWork> d_items_;
-    DevBuf<int32_t> d_row_ptr_, d_row_list_, d_rowmaps_;
-    DevBuf<uint64_t> d_partial_, d_cnt_;
-    DevBuf<int64_t> d_dsum_;
+    DevBuf<int32_t> d_row_list_, d_chunk_ptr_, d_row_chunk_, d_rowmaps_;
+    DevBuf<uint64_t> d_partial_, d_cnt_, d_tmp_cnt_;
+    DevBuf<int64_t> d_dsum_, d_tmp_dsum_;
     hipEvent_t ev_[3]{};
     bool have_timing_ = false;
 

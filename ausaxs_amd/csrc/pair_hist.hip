@@ -4,19 +4,20 @@
 // evaluate_{16,8,4,1}, HistogramManagerMTFFAvg.cpp:214-280) as:
 //   * one workgroup per rectangle of the pair space (PairWork);
 //   * row points in VGPRs (2 per lane); column points staged through LDS in
-//     tiles of 256 and read back as wave-wide broadcasts, so each lane
-//     evaluates 2 pairs per column point (LDS reads and LDS atomics retire in
-//     order, so the broadcast read of the next column never waits on them);
+//     tiles of 256 and read back as wave-wide broadcasts in double-buffered
+//     groups of 4, so each lane evaluates 2 pairs per column point (LDS reads
+//     and LDS atomics retire in order, so the broadcast reads of the next
+//     group never wait on the atomics of the current one);
 //   * a workgroup-private histogram of packed uint64 cells in LDS updated
 //     with ds_add_u64 (one atomic per pair);
-//   * a flush of the LDS histogram to a partial slot, summed by merge_kernel.
+//   * a flush of the LDS histogram to a partial slot, summed by the merge kernels.
 //
 // Bit-exact binning (SIMD-body rule of the reference):
 //   d2 = (dx*dx + dy*dy) + dz*dz   separately rounded (file built with
 //                                  -ffp-contract=off)
 //   d  = sqrt_rn(d2), x = d*inv_w, bin = rint(x)  (round half to even)
 // v_sqrt_f32 is accurate to 1 ulp but not correctly rounded; it is used when
-// x is at least 4 ulp away from a .5 tie, otherwise the correctly rounded
+// x is more than 1.25 * 2^-22 * x away from a .5 tie, otherwise the correctly rounded
 // square root (sqrtf: v_sqrt_f32 plus an FMA-residual correction; note that
 // __fsqrt_rn lowers to the uncorrected instruction) decides.
 // launch_bin_check verifies both exhaustively on device.
@@ -29,6 +30,7 @@ constexpr int kThreads = 256;
 constexpr int kRowsPerLane = 2;
 constexpr int kTileI = kThreads * kRowsPerLane;  // row points per work item
 constexpr int kColTile = kThreads;               // column points per LDS tile
+constexpr int kColGroup = 4;                     // column points per double-buffered group
 
 // correctly rounded square root (IEEE); see the note above
 __device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
@@ -46,12 +48,39 @@ __device__ __forceinline__ void simd_rule_fast(float dx, float dy, float dz, flo
 }
 
 __device__ __forceinline__ bool near_tie(float x, float dl) {
-    // 2^-21 * x >= 4 ulp(x): the raw sqrt is within 1 ulp, the product adds < 3 ulp.
-    return fabsf(dl) > __builtin_fmaf(x, -4.76837158203125e-7f, 0.5f);
+    // The raw sqrt is within 1 ulp (<= 2^-23 relative) of the correctly rounded
+    // one, and each of the two products rounds by <= 2^-24, so x is within
+    // 2^-22 * x of the correctly rounded x. Outside a window of 1.25 * 2^-22 * x
+    // around a .5 tie both round to the same bin (the margin covers the
+    // rounding of the threshold itself).
+    return fabsf(dl) > __builtin_fmaf(x, -2.98023223876953125e-7f, 0.5f);
 }
 
 __device__ __forceinline__ void simd_rule_exact(float d2, float inv_w, float& x, float& b, float& dl) {
     const float r = sqrt_rn(d2);
+    x = r * inv_w;
+    b = __builtin_rintf(x);
+    dl = x - b;
+}
+
+// Correctly rounded sqrt(d2) from a raw v_sqrt_f32 result s (within 1 ulp),
+// for the tie fallback, where d2 >= (bin_width/2)^2 is normal. With s- and s+
+// the neighbours of s, s- s and s s+ are exact multiples of ulp(s)^2, as is
+// d2, so d2 > s- s implies d2 >= s- s + ulp^2 > ((s- + s)/2)^2 and d2 <= s- s
+// implies sqrt(d2) < (s- + s)/2 (likewise for s+). The FMA residuals'
+// signs are exact, so they pick the correctly rounded neighbour.
+__device__ __forceinline__ float sqrt_fix(float d2, float s) {
+    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_dn = __builtin_fmaf(-s_dn, s, d2);
+    const float r_up = __builtin_fmaf(-s_up, s, d2);
+    s = r_dn <= 0.f ? s_dn : s;
+    return r_up > 0.f ? s_up : s;
+}
+
+// fast-path result near a tie: redo the bin with the correctly rounded sqrt
+__device__ __forceinline__ void tie_fix(float d2, float inv_w, float& x, float& b, float& dl) {
+    const float r = sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2));
     x = r * inv_w;
     b = __builtin_rintf(x);
     dl = x - b;
@@ -70,13 +99,55 @@ __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
     __hip_atomic_fetch_add(cell, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <bool FULL, bool DIAG, bool ORIENT>
+// Row points of one lane (kRowsPerLane of them) for the current item.
+struct Rows {
+    float x[kRowsPerLane], y[kRowsPerLane], z[kRowsPerLane];
+    int tag[kRowsPerLane], idx[kRowsPerLane];
+    bool valid[kRowsPerLane];
+};
+
+// All kRowsPerLane pairs of one column point: bin, tie fallback, LDS add.
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP>
+__device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, float inv_w,
+                                            unsigned ovf, int off1) {
+    const int tagj = __float_as_int(pj.w);
+    float x[kRowsPerLane], b[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane];
+    bool tie = false;
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) {
+        simd_rule_fast(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z, inv_w, x[k], b[k], dl[k], d2[k]);
+        tie |= near_tie(x[k], dl[k]);
+    }
+    if (tie) {
+#pragma unroll
+        for (int k = 0; k < kRowsPerLane; ++k)
+            if (near_tie(x[k], dl[k])) tie_fix(d2[k], inv_w, x[k], b[k], dl[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) {
+        unsigned bin = (unsigned)(int)b[k];
+        if (CLAMP) bin = min(bin, ovf);
+        if (!FULL || DIAG) {
+            bool keep = r.valid[k];
+            if (DIAG) keep = keep && (j > r.idx[k]);
+            bin = keep ? bin : ovf + 1u;
+        }
+        int off = 0;
+        if (ORIENT) off = r.tag[k] < tagj ? 0 : off1;
+        lds_add(&hist[off + (int)bin], cell_add(dl[k]));
+    }
+}
+
+// One work item. Column points pass through LDS 256 at a time; within a tile
+// they are consumed in double-buffered groups of kColGroup: the reads of one
+// group are issued before the atomics of the other, so a wait for them never
+// waits for atomics (LDS operations retire in order) and no register copies
+// are needed between groups.
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP>
 __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
                                          float4* cols, float inv_w, int nbins) {
     const int t = threadIdx.x;
-    float xi[kRowsPerLane], yi[kRowsPerLane], zi[kRowsPerLane];
-    int tagi[kRowsPerLane], idx[kRowsPerLane];
-    bool valid[kRowsPerLane];
+    Rows r;
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
         // lanes of one wave take points 8 apart: spatially spread rows give
@@ -84,16 +155,15 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         // on one LDS cell.
         const int li = k * kThreads + t;
         const int i = w.i_begin + ((li & 63) * 8 + (li >> 6));
-        valid[k] = FULL || i < w.i_end;
-        const float4 p = valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        xi[k] = p.x;
-        yi[k] = p.y;
-        zi[k] = p.z;
-        tagi[k] = __float_as_int(p.w);
-        idx[k] = i;
+        r.valid[k] = FULL || i < w.i_end;
+        const float4 p = r.valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        r.x[k] = p.x;
+        r.y[k] = p.y;
+        r.z[k] = p.z;
+        r.tag[k] = __float_as_int(p.w);
+        r.idx[k] = i;
     }
-    const int stride = nbins + 2;  // slot nbins: overflow, slot nbins+1: masked lanes
-    const int off1 = ORIENT ? stride : 0;
+    const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
 
     for (int jc = w.j_begin; jc < w.j_end; jc += kColTile) {
@@ -101,96 +171,112 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         __syncthreads();  // previous tile consumed
         if (t < nc) cols[t] = pts[jc + t];
         __syncthreads();
-        // the read of column c+1 is issued before the atomics of column c, so
-        // waiting for it never waits for those atomics (LDS ops retire in order)
-        float4 pn = cols[0];
-        // completes the first read before the loop, so the loop header needs
-        // no wait that would also drain the atomics in flight
+        const int n2 = nc & ~(2 * kColGroup - 1);
+        float4 a[kColGroup], bq[kColGroup];
+#pragma unroll
+        for (int u = 0; u < kColGroup; ++u) a[u] = cols[u];
+        // completes the first reads before the loop, so the loop needs no
+        // wait that would also drain atomics in flight
         __syncthreads();
-        for (int c = 0; c < nc; ++c) {
-            const float4 pj = pn;
-            pn = cols[c + 1];  // cols has kColTile+1 slots; the last read is unused
-            const int j = jc + c;
-            const int tagj = __float_as_int(pj.w);
-            float x[kRowsPerLane], b[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane];
-            bool tie = false;
+        for (int c = 0; c < n2; c += 2 * kColGroup) {
 #pragma unroll
-            for (int k = 0; k < kRowsPerLane; ++k) {
-                simd_rule_fast(xi[k] - pj.x, yi[k] - pj.y, zi[k] - pj.z, inv_w, x[k], b[k], dl[k], d2[k]);
-                tie |= near_tie(x[k], dl[k]);
-            }
-            if (tie) {
+            for (int u = 0; u < kColGroup; ++u) bq[u] = cols[c + kColGroup + u];
 #pragma unroll
-                for (int k = 0; k < kRowsPerLane; ++k)
-                    if (near_tie(x[k], dl[k])) simd_rule_exact(d2[k], inv_w, x[k], b[k], dl[k]);
-            }
+            for (int u = 0; u < kColGroup; ++u)
+                pair_column<FULL, DIAG, ORIENT, CLAMP>(r, a[u], jc + c + u, hist, inv_w, ovf, off1);
+            // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
-            for (int k = 0; k < kRowsPerLane; ++k) {
-                unsigned bin = min((unsigned)(int)b[k], ovf);
-                if (!FULL || DIAG) {
-                    bool keep = valid[k];
-                    if (DIAG) keep = keep && (j > idx[k]);
-                    bin = keep ? bin : ovf + 1u;
-                }
-                int off = 0;
-                if (ORIENT) off = tagi[k] < tagj ? 0 : off1;
-                lds_add(&hist[off + (int)bin], cell_add(dl[k]));
-            }
+            for (int u = 0; u < kColGroup; ++u) a[u] = cols[c + 2 * kColGroup + u];
+#pragma unroll
+            for (int u = 0; u < kColGroup; ++u)
+                pair_column<FULL, DIAG, ORIENT, CLAMP>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, ovf, off1);
         }
+        for (int c = n2; c < nc; ++c)
+            pair_column<FULL, DIAG, ORIENT, CLAMP>(r, cols[c], jc + c, hist, inv_w, ovf, off1);
     }
 }
 
-__global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __restrict__ pts,
-                                                             const PairWork* __restrict__ items,
-                                                             uint64_t* __restrict__ partial, float inv_w,
-                                                             int nbins) {
-    extern __shared__ uint64_t hist[];
-    __shared__ float4 cols[kColTile + 1];
-    const PairWork w = items[blockIdx.x];
-    const int stride = nbins + 2;
+template <bool CLAMP>
+__device__ __forceinline__ void run_any(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
+                                        float4* cols, float inv_w, int nbins) {
     const bool orient = (w.flags & kOrient) != 0;
-    const int cells = (orient ? 2 : 1) * stride;
-    for (int e = threadIdx.x; e < cells; e += kThreads) hist[e] = 0;
-    __syncthreads();
-
-    const bool full = (w.i_end - w.i_begin) >= kTileI;
-    const bool diag = (w.flags & kDiag) != 0;
-    if (diag) {
-        if (orient) run_item<false, true, true>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<false, true, false>(pts, w, hist, cols, inv_w, nbins);
-    } else if (full) {
-        if (orient) run_item<true, false, true>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<true, false, false>(pts, w, hist, cols, inv_w, nbins);
+    if (w.flags & kDiag) {
+        if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        else run_item<false, true, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+    } else if (w.i_end - w.i_begin >= kTileI) {
+        if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        else run_item<true, false, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
     } else {
-        if (orient) run_item<false, false, true>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<false, false, false>(pts, w, hist, cols, inv_w, nbins);
+        if (orient) run_item<false, false, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        else run_item<false, false, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
     }
-    __syncthreads();
-
-    // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
-    uint64_t* out = partial + (size_t)w.slot * 2 * (nbins + 1);
-    const int out_stride = nbins + 1;
-    for (int r = 0; r < (orient ? 2 : 1); ++r)
-        for (int e = threadIdx.x; e < out_stride; e += kThreads) out[r * out_stride + e] = hist[r * stride + e];
 }
 
-__global__ __launch_bounds__(256) void merge_kernel(const uint64_t* __restrict__ partial,
-                                                    const int32_t* __restrict__ row_ptr,
-                                                    const int32_t* __restrict__ row_list, int nbins,
-                                                    uint64_t* __restrict__ cnt, int64_t* __restrict__ dsum) {
+// One workgroup per work item (grid = items), or a persistent grid that
+// strides over the item list (grid < items): both visit every item once.
+// clamp = 0 is allowed only when the host has bounded every pair's bin below
+// nbins (bounding-box diagonal); otherwise bins are clamped into slot nbins.
+__global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __restrict__ pts,
+                                                             const PairWork* __restrict__ items, int n_items,
+                                                             uint64_t* __restrict__ partial, float inv_w,
+                                                             int nbins, int clamp) {
+    extern __shared__ uint64_t hist[];
+    __shared__ float4 cols[kColTile + kColGroup];
+    const int stride = nbins + 2;
+    const int out_stride = nbins + 1;
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PairWork w = items[it];
+        const bool orient = (w.flags & kOrient) != 0;
+        const int cells = (orient ? 2 : 1) * stride;
+        __syncthreads();  // previous item's flush done
+        for (int e = threadIdx.x; e < cells; e += kThreads) hist[e] = 0;
+        __syncthreads();
+        if (clamp) run_any<true>(pts, w, hist, cols, inv_w, nbins);
+        else run_any<false>(pts, w, hist, cols, inv_w, nbins);
+        __syncthreads();
+        // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
+        uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
+        for (int rr = 0; rr < (orient ? 2 : 1); ++rr)
+            for (int e = threadIdx.x; e < out_stride; e += kThreads) out[rr * out_stride + e] = hist[rr * stride + e];
+    }
+}
+
+__global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __restrict__ partial,
+                                                           const int32_t* __restrict__ row_list,
+                                                           const int32_t* __restrict__ chunk_ptr, int nbins,
+                                                           uint64_t* __restrict__ tcnt, int64_t* __restrict__ tdsum) {
+    const int c = blockIdx.y;
+    const int stride = nbins + 1;
+    const int bin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bin >= stride) return;
+    uint64_t n_sum = 0, low_sum = 0;
+    const int e0 = chunk_ptr[c], e1 = chunk_ptr[c + 1];
+#pragma unroll 8
+    for (int e = e0; e < e1; ++e) {
+        const uint64_t v = partial[(size_t)row_list[e] * stride + bin];
+        n_sum += v >> kCountShift;
+        low_sum += v & kLowMask;
+    }
+    tcnt[(size_t)c * stride + bin] = n_sum;
+    tdsum[(size_t)c * stride + bin] = (int64_t)low_sum - (int64_t)n_sum * kDeltaOffset;
+}
+
+// Merge phase 2: output row r = sum of its chunks row_chunk[r] .. row_chunk[r+1].
+__global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* __restrict__ tcnt,
+                                                         const int64_t* __restrict__ tdsum,
+                                                         const int32_t* __restrict__ row_chunk, int nbins,
+                                                         uint64_t* __restrict__ cnt, int64_t* __restrict__ dsum) {
     const int row = blockIdx.y;
     const int stride = nbins + 1;
     const int bin = blockIdx.x * blockDim.x + threadIdx.x;
     if (bin >= stride) return;
     uint64_t c = 0;
     int64_t s = 0;
-    const int e0 = row_ptr[row], e1 = row_ptr[row + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int ent = row_list[e];
-        const uint64_t v = partial[(size_t)ent * stride + bin];
-        const uint64_t n = v >> kCountShift;
-        c += n;
-        s += (int64_t)(v & kLowMask) - (int64_t)n * kDeltaOffset;
+    const int c0 = row_chunk[row], c1 = row_chunk[row + 1];
+#pragma unroll 4
+    for (int k = c0; k < c1; ++k) {
+        c += tcnt[(size_t)k * stride + bin];
+        s += tdsum[(size_t)k * stride + bin];
     }
     cnt[(size_t)row * stride + bin] = c;
     dsum[(size_t)row * stride + bin] = s;
@@ -233,7 +319,7 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
 
 __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi, float inv_w,
                                                         unsigned long long* mism) {
-    unsigned long long bad_fast = 0, bad_cr = 0;
+    unsigned long long bad_fast = 0, bad_cr = 0, bad_fix = 0;
     const uint64_t n = (uint64_t)hi - lo + 1;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (uint64_t)gridDim.x * blockDim.x) {
@@ -255,14 +341,14 @@ __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi
         float x = __builtin_amdgcn_sqrtf(d2) * inv_w;
         float b = __builtin_rintf(x);
         float dl = x - b;
-        if (near_tie(x, dl)) {
-            float d2c = d2;
-            simd_rule_exact(d2c, inv_w, x, b, dl);
-        }
+        if (near_tie(x, dl)) tie_fix(d2, inv_w, x, b, dl);
         if (b != br) ++bad_fast;
+        // the residual correction of the tie fallback, on every normal d2
+        if (d2 >= 1e-30f && sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2)) != r) ++bad_fix;
     }
     if (bad_fast) atomicAdd(&mism[0], bad_fast);
     if (bad_cr) atomicAdd(&mism[1], bad_cr);
+    if (bad_fix) atomicAdd(&mism[2], bad_fix);
 }
 
 // plain streaming copy of a known number of bytes (calibration of HBM traffic counters)
@@ -283,16 +369,22 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
 hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
     if (p.n_items == 0) return hipSuccess;
     const size_t lds = (size_t)2 * (p.nbins + 2) * sizeof(uint64_t);
-    hipLaunchKernelGGL(pair_hist_kernel, dim3(p.n_items), dim3(kThreads), lds, s, p.pts, p.items, p.partial, p.inv_w,
-                       p.nbins);
+    const int grid = p.grid > 0 ? std::min(p.grid, p.n_items) : p.n_items;
+    hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.items, p.n_items, p.partial,
+                       p.inv_w, p.nbins, p.clamp);
     return hipGetLastError();
 }
 
-hipError_t launch_merge(const uint64_t* partial, const int32_t* row_ptr, const int32_t* row_list, int n_rows,
-                        int nbins, uint64_t* cnt, int64_t* dsum, hipStream_t s) {
-    if (n_rows == 0) return hipSuccess;
-    dim3 grid((nbins + 1 + 255) / 256, n_rows);
-    hipLaunchKernelGGL(merge_kernel, grid, dim3(256), 0, s, partial, row_ptr, row_list, nbins, cnt, dsum);
+hipError_t launch_merge(const MergeLaunch& m, hipStream_t s) {
+    if (m.n_rows == 0) return hipSuccess;
+    const unsigned bx = (unsigned)((m.nbins + 1 + 255) / 256);
+    if (m.n_chunks > 0) {
+        hipLaunchKernelGGL(merge_chunks_kernel, dim3(bx, m.n_chunks), dim3(256), 0, s, m.partial, m.row_list,
+                           m.chunk_ptr, m.nbins, m.tmp_cnt, m.tmp_dsum);
+        if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(merge_rows_kernel, dim3(bx, m.n_rows), dim3(256), 0, s, m.tmp_cnt, m.tmp_dsum, m.row_chunk,
+                       m.nbins, m.cnt, m.dsum);
     return hipGetLastError();
 }
 

@@ -63,10 +63,11 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(cfg, atoms, waters, q, budget_s=25.0):
+def cpu_baseline(cfg, atoms, waters, q, budget_s=15.0):
     """The oracle's multithreaded FF managers (800-row tasks, thread-private
     24-byte entries, merge) + FP64 Debye on this host, on the full workload if
-    it fits the time budget, else on a prefix sample."""
+    it fits the time budget, else on a prefix sample grown until one run takes
+    at least 40 % of the budget."""
     import oracle as O
 
     try:
@@ -80,19 +81,17 @@ def cpu_baseline(cfg, atoms, waters, q, budget_s=25.0):
     def pairs_of(n, m):
         return n * (n - 1) / 2 + n * m + m * (m - 1) / 2
 
-    # calibrate on a small prefix
-    nc = min(len(atoms), 12000)
-    mc = min(len(w), int(len(w) * nc / max(1, len(atoms))))
-    t = time.perf_counter()
-    O.hist_ff(mode, atoms[:nc], w[:mc], cfg["bw"], 8000, 1.0, threads)
-    rate = pairs_of(nc, mc) / (time.perf_counter() - t)
-    full = pairs_of(len(atoms), len(w))
-    frac = 1.0 if full / rate <= budget_s else float(np.sqrt(budget_s * rate / full))
-    ns, ms = int(len(atoms) * frac), int(len(w) * frac)
-    t = time.perf_counter()
-    h = O.hist_ff(mode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, threads)
-    O.debye_ff(mode, h, q, cfg["bw"])
-    dt = time.perf_counter() - t
+    frac = min(1.0, 12000 / max(1, len(atoms)))
+    while True:
+        ns, ms = int(len(atoms) * frac), int(len(w) * frac)
+        t = time.perf_counter()
+        h = O.hist_ff(mode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, threads)
+        O.debye_ff(mode, h, q, cfg["bw"])
+        dt = time.perf_counter() - t
+        if frac >= 1.0 or dt >= 0.4 * budget_s:
+            break
+        # pair work grows with the square of the prefix
+        frac = min(1.0, frac * float(np.sqrt(0.8 * budget_s / max(dt, 1e-3))))
     pairs = pairs_of(ns, ms)
     sample = "full workload" if frac >= 1.0 else f"first {ns} atoms + {ms} waters of the workload"
     return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=threads, kind="port",

@@ -137,7 +137,9 @@ int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, con
 /* Exhaustive device check of the binning arithmetic over float d2 bit
  * patterns [lo_bits, hi_bits]: mismatches[0] = fast-path bins that differ
  * from the correctly rounded rule, mismatches[1] = device sqrt results that
- * are not correctly rounded. */
+ * are not correctly rounded, mismatches[2] = results of the tie fallback's
+ * residual correction (normal d2) that differ from the correctly rounded sqrt.
+ * mismatches: 3 entries. */
 int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsigned long long* mismatches);
 
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming

@@ -34,13 +34,14 @@ def test_device_is_gfx950():
     assert info["arch"].startswith("gfx950")
 
 
-@pytest.mark.parametrize("bw", [0.25, 0.1, 1.0])
+@pytest.mark.parametrize("bw", [0.25, 0.1, 1.0, 0.05, 0.3, 2.0])
 def test_binning_arithmetic_exhaustive(bw):
     # every float32 d2 in [0, 6.4e5] (distances up to 800 A): the fast path
-    # (raw v_sqrt_f32 + exact fallback near ties) bins like sqrt_rn, and the
-    # device sqrtf is correctly rounded (verified in FP64)
+    # (raw v_sqrt_f32 + residual-corrected fallback near ties) bins like
+    # sqrt_rn, the device sqrtf is correctly rounded (verified in FP64), and the
+    # fallback's correction equals it on every normal d2
     hi = int(np.array(6.4e5, np.float32).view(np.uint32))
-    assert A.bin_check(0, hi, bw) == (0, 0)
+    assert A.bin_check(0, hi, bw) == (0, 0, 0)
 
 
 def check_pairs(a, b, bw, nb):
