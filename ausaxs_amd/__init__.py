@@ -78,6 +78,12 @@ def _load():
         "ausaxs_bin_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, vp]),
         "ausaxs_structure_set_timing": (C.c_int, [vp, C.c_int]),
         "ausaxs_copy_probe": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
+        "ausaxs_rigid_create": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, C.c_int, vp, P(vp)]),
+        "ausaxs_rigid_scan": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]),
+        "ausaxs_rigid_histogram": (C.c_int, [vp, C.c_int, P(vp)]),
+        "ausaxs_rigid_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_longlong),
+                                        P(C.c_double)]),
+        "ausaxs_rigid_free": (C.c_int, [vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_synth_globule": (
             C.c_int,
@@ -340,3 +346,69 @@ def _grid_axes(self):
 
 
 CompositeHistogram.grid_axes = _grid_axes
+
+
+class RigidScan:
+    """PartialHistogramManagerMT over many rigid-body conformations (one hipGraph per batch).
+
+    bodies: list of (n_b x 3) FP64 coordinate arrays, weights: list of per-atom
+    weights (XYZW). Self terms are cached from these coordinates; scan() moves
+    every body by its transform T[c, b] = (R row-major, t) per conformation."""
+
+    def __init__(self, bodies, weights, waters_xyz=None, waters_w=None, settings=None, autotune=False):
+        self.sizes = np.array([len(b) for b in bodies], np.int32)
+        self._xyz = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64).reshape(-1, 3) for b in bodies])
+                                         if len(bodies) else np.zeros((0, 3)), np.float64)
+        self._w = np.ascontiguousarray(np.concatenate([np.broadcast_to(np.asarray(w, np.float32), (len(b),))
+                                                       for b, w in zip(bodies, weights)]), np.float32)
+        m = 0 if waters_xyz is None else len(waters_xyz)
+        self._wx = np.ascontiguousarray(waters_xyz if m else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+        self._ww = np.ascontiguousarray(np.broadcast_to(np.asarray(waters_w if m else 1.0, np.float32), (m,)),
+                                        np.float32)
+        s = settings or default_settings()
+        if autotune:
+            s.tile_j = 0
+        h = C.c_void_p()
+        _check(lib.ausaxs_rigid_create(_ptr(self._xyz), _ptr(self._w), _ptr(self.sizes), len(self.sizes),
+                                       _ptr(self._wx), _ptr(self._ww), m, C.byref(s), C.byref(h)))
+        self._r = h
+
+    def __del__(self):
+        if getattr(self, "_r", None):
+            lib.ausaxs_rigid_free(self._r)
+            self._r = None
+
+    def scan(self, T, q, cw=1.0):
+        T = np.ascontiguousarray(T, np.float64).reshape(-1, len(self.sizes), 12)
+        q = np.ascontiguousarray(q, np.float64)
+        I = np.zeros((len(T), len(q)))
+        _check(lib.ausaxs_rigid_scan(self._r, _ptr(T), len(T), _ptr(q), len(q), cw, _ptr(I)))
+        return I
+
+    def histogram(self, c) -> CompositeHistogram:
+        h = C.c_void_p()
+        _check(lib.ausaxs_rigid_histogram(self._r, c, C.byref(h)))
+        return CompositeHistogram(h.value)
+
+    def info(self):
+        v = [C.c_int() for _ in range(4)]
+        pairs, ms = C.c_longlong(), C.c_double()
+        _check(lib.ausaxs_rigid_info(self._r, *map(C.byref, v), C.byref(pairs), C.byref(ms)))
+        return dict(tile_j=v[0].value, grid=v[1].value, batch=v[2].value, n_items=v[3].value, pairs=pairs.value,
+                    scan_ms=ms.value)
+
+
+def rigid_transform(R, t):
+    """12-entry transform (R row-major, t)."""
+    return np.concatenate([np.asarray(R, np.float64).reshape(9), np.asarray(t, np.float64).reshape(3)])
+
+
+def random_rotation(rng):
+    """Uniform random rotation (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.random(3)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    x, y, z, w = a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2), b * np.sin(2 * np.pi * u3), \
+        b * np.cos(2 * np.pi * u3)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])

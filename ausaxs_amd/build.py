@@ -32,7 +32,8 @@ COMMON = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["pair_hist.hip", "debye.hip", "engine.cpp", "hist.cpp", "grid.cpp", "synth.cpp", "capi.cpp"]
+SOURCES = ["pair_hist.hip", "debye.hip", "rigid.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "synth.cpp",
+           "capi.cpp"]
 
 
 def _compile(src: str) -> Path:

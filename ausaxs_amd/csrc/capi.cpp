@@ -6,6 +6,7 @@
 
 #include "grid.hpp"
 #include "hist.hpp"
+#include "rigid.hpp"
 #include "synth.hpp"
 
 #include <cstring>
@@ -21,6 +22,10 @@ struct ausaxs_structure {
 
 struct ausaxs_histogram {
     std::unique_ptr<CompositeDistanceHistogramHIP> h;
+};
+
+struct ausaxs_rigid {
+    std::unique_ptr<RigidScan> r;
 };
 
 namespace {
@@ -356,6 +361,60 @@ int ausaxs_synth_shell(uint64_t seed, int m, double ax, double ay, double az, do
         require(out != nullptr || m == 0, "out is NULL");
         synth_shell(seed, m, ax, ay, az, offset, out);
     });
+}
+
+
+int ausaxs_rigid_create(const double* xyz, const float* w, const int32_t* body_sizes, int n_bodies,
+                        const double* waters_xyz, const float* waters_w, int n_waters, const ausaxs_settings* settings,
+                        ausaxs_rigid** out) {
+    return guarded([&] {
+        require(out && body_sizes && n_bodies >= 1 && n_waters >= 0, "bad argument");
+        int64_t n = 0;
+        for (int b = 0; b < n_bodies; ++b) n += body_sizes[b];
+        require(n == 0 || (xyz && w), "bad argument: body coordinates");
+        require(n_waters == 0 || (waters_xyz && waters_w), "bad argument: waters");
+        HistSettings s = to_settings(settings);
+        if (settings) s.tile_j = settings->tile_j;  // 0: autotune
+        auto r = std::make_unique<ausaxs_rigid>();
+        r->r = std::make_unique<RigidScan>(xyz, w, body_sizes, n_bodies, waters_xyz, waters_w, n_waters, s);
+        *out = r.release();
+    });
+}
+
+int ausaxs_rigid_scan(ausaxs_rigid* r, const double* transforms, int n_conf, const double* q, int nq, double cw,
+                      double* I) {
+    return guarded([&] {
+        require(r && r->r && n_conf >= 0 && nq >= 0, "bad argument");
+        require((n_conf == 0 || transforms) && (nq == 0 || q) && (n_conf == 0 || nq == 0 || I), "bad argument");
+        r->r->scan(transforms, n_conf, q, nq, cw, I);
+    });
+}
+
+int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram** out) {
+    return guarded([&] {
+        require(r && r->r && out, "bad argument");
+        auto h = std::make_unique<ausaxs_histogram>();
+        h->h = r->r->histogram(conformation);
+        *out = h.release();
+    });
+}
+
+int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* batch, int* n_items, long long* pairs,
+                      double* scan_ms) {
+    return guarded([&] {
+        require(r && r->r, "bad argument");
+        const RigidInfo& i = r->r->info();
+        if (tile_j) *tile_j = i.tile_j;
+        if (grid) *grid = i.grid;
+        if (batch) *batch = i.batch;
+        if (n_items) *n_items = i.n_items;
+        if (pairs) *pairs = (long long)i.pairs;
+        if (scan_ms) *scan_ms = i.total_ms;
+    });
+}
+
+int ausaxs_rigid_free(ausaxs_rigid* r) {
+    return guarded([&] { delete r; });
 }
 
 }  // extern "C"

@@ -91,6 +91,11 @@ struct StripLaunch {
     float inv_w;
     int nbins;
     int stride;               // nbins+1
+    // replicas (grid.y): replica r reads rows + r*rows_rep, cols + r*cols_rep
+    // (points) and adds r*out_rep to the histogram row
+    int n_rep = 1;
+    int64_t rows_rep = 0, cols_rep = 0;
+    int out_rep = 0;
 };
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
 

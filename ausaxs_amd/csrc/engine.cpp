@@ -50,29 +50,60 @@ int bin_bound(const std::vector<const PointSet*>& sets, float inv_w) {
     return (int)std::ceil(x);
 }
 
-PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-                       int tile_j)
-    : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w) {
+void check_tile_j(int tile_j) {
     if (tile_j < 1 || (int64_t)kTileI * tile_j > kMaxPairsPerItem)
         throw std::runtime_error("ausaxs_hip: tile_j must keep a work item below 2^22 pairs (tile_j <= 8191)");
+}
+
+size_t check_pair_lds(int nbins) {
     if (nbins < 1) throw std::runtime_error("ausaxs_hip: nbins must be positive");
-    // dynamic histogram rows + the static column tile (257 float4) of pair_hist_kernel
-    const size_t lds = (size_t)2 * (nbins + 2) * sizeof(uint64_t) + 257 * sizeof(float4);
+    // dynamic histogram rows + the static column tile (260 float4) of pair_hist_kernel
+    const size_t lds = (size_t)2 * (nbins + 2) * sizeof(uint64_t) + 260 * sizeof(float4);
     Context& ctx = Context::get();
     if (lds > ctx.prop.sharedMemPerBlock)
         throw std::runtime_error("ausaxs_hip: " + std::to_string(nbins) + " bins need " + std::to_string(lds) +
                                  " B of LDS per workgroup, device offers " +
                                  std::to_string(ctx.prop.sharedMemPerBlock));
-    // bins need no clamp in the pair kernel when the bounding-box diagonal
-    // already keeps every pair below nbins
-    std::vector<const PointSet*> ps;
-    for (const auto& x : sets_) ps.push_back(&x);
-    clamp_ = nbins_ < bin_bound(ps, inv_w_);
-    build(tile_j);
-    for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+    return lds;
 }
 
-void PairEngine::add_items(int i0, int i1, int j0, int j1, int row0, int row1, bool diag, int tile_j) {
+SetLayout layout_set(const PointSet& ps, std::vector<float4>& buf, std::vector<int32_t>* perm) {
+    const int n = ps.size(), C = ps.n_classes;
+    const int t_self = n - std::min(3, n);  // self-loop tail candidates: [t_self, n)
+    const int t_col = n - (n % 4);          // cross-loop scalar columns: [t_col, n)
+    SetLayout L;
+    L.base = (int)buf.size();
+    L.seg_start.assign(C, 0);
+    L.seg_len.assign(C, 0);
+    L.seg_body.assign(C, 0);
+    L.seg_colbody.assign(C, 0);
+    std::vector<std::vector<int>> body(C), mid(C), tail(C);
+    for (int i = 0; i < n; ++i) {
+        const int c = bits_of(ps.orig[i].w);
+        if (c < 0 || c >= C) throw std::runtime_error("ausaxs_hip: point class out of range");
+        if (i < t_self) body[c].push_back(i);
+        else if (i < t_col) mid[c].push_back(i);
+        else tail[c].push_back(i);
+    }
+    for (int c = 0; c < C; ++c) {
+        L.seg_start[c] = (int)buf.size();
+        for (auto* v : {&body[c], &mid[c], &tail[c]})
+            for (int i : *v) {
+                float4 p = ps.orig[i];
+                p.w = float_of(i);
+                buf.push_back(p);
+                if (perm) perm->push_back(i);
+            }
+        L.seg_len[c] = (int)(body[c].size() + mid[c].size() + tail[c].size());
+        L.seg_body[c] = (int)body[c].size();
+        L.seg_colbody[c] = (int)(body[c].size() + mid[c].size());
+    }
+    return L;
+}
+
+namespace {
+void add_items(std::vector<PairWork>& items, int i0, int i1, int j0, int j1, int row0, int row1, bool diag,
+               int tile_j) {
     const bool orient = row0 != row1;
     for (int a = i0; a < i1; a += kTileI) {
         const int a1 = std::min(a + kTileI, i1);
@@ -85,146 +116,128 @@ void PairEngine::add_items(int i0, int i1, int j0, int j1, int row0, int row1, b
             w.row0 = row0;
             w.row1 = orient ? row1 : row0;
             w.flags = (diag ? kDiag : 0) | (orient ? kOrient : 0);
-            items_.push_back(w);
+            items.push_back(w);
         }
     }
+}
+}  // namespace
+
+void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j) {
+    const bool self = job.col_set < 0;
+    const PointSet& R = sets[job.row_set];
+    const PointSet& Cs = self ? R : sets[job.col_set];
+    const SetLayout& LR = layouts[job.row_set];
+    const SetLayout& LC = self ? LR : layouts[job.col_set];
+    const int CR = R.n_classes, CC = Cs.n_classes;
+    if ((int)job.rowmap.size() != CR * CC) throw std::runtime_error("ausaxs_hip: rowmap size mismatch");
+    for (int32_t r : job.rowmap)
+        if (r < -1 || r >= n_rows) throw std::runtime_error("ausaxs_hip: rowmap entry out of range");
+    auto row_of = [&](int a, int b) {
+        const int32_t r = job.rowmap[a * CC + b];
+        if (r < 0) throw std::runtime_error("ausaxs_hip: missing histogram row for a present class pair");
+        return r;
+    };
+    auto add_strip = [&](const StripWork& w) {
+        StripSpec st{};
+        st.set_rows = job.row_set;
+        st.set_cols = self ? job.row_set : job.col_set;
+        st.w = w;
+        st.rowmap_off = (int)wl.rowmaps.size();
+        wl.strips.push_back(st);
+        wl.rowmaps.insert(wl.rowmaps.end(), job.rowmap.begin(), job.rowmap.end());
+    };
+    if (self) {
+        const int64_t n = R.size();
+        wl.pairs += n * (n - 1) / 2;
+        for (int a = 0; a < CR; ++a) {
+            if (LR.seg_body[a] == 0) continue;
+            const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_body[a];
+            // triangle: diagonal blocks masked to j > i, the rest plain
+            const int ra = row_of(a, a);
+            for (int i0 = s0; i0 < s1; i0 += kTileI) {
+                const int i1 = std::min(i0 + kTileI, s1);
+                add_items(wl.items, i0, i1, i0, i1, ra, ra, true, tile_j);
+                if (i1 < s1) add_items(wl.items, i0, i1, i1, s1, ra, ra, false, tile_j);
+            }
+            for (int b = a + 1; b < CR; ++b) {
+                if (LR.seg_body[b] == 0) continue;
+                const int t0 = LR.seg_start[b], t1 = t0 + LR.seg_body[b];
+                add_items(wl.items, s0, s1, t0, t1, row_of(a, b), row_of(b, a), false, tile_j);
+            }
+        }
+        const int nn = R.size();
+        if (nn >= 2) add_strip(StripWork{nn, nn - std::min(3, nn), nn, nn, 1, CC});
+    } else {
+        const int64_t n = R.size(), m = Cs.size();
+        wl.pairs += n * m;
+        for (int a = 0; a < CR; ++a) {
+            if (LR.seg_len[a] == 0) continue;
+            const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_len[a];
+            for (int b = 0; b < CC; ++b) {
+                if (LC.seg_colbody[b] == 0) continue;
+                const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
+                const int r = row_of(a, b);
+                add_items(wl.items, s0, s1, t0, t1, r, r, false, tile_j);
+            }
+        }
+        const int mm = Cs.size();
+        if (mm % 4 && R.size() > 0) add_strip(StripWork{R.size(), mm - (mm % 4), mm, mm, 0, CC});
+    }
+}
+
+void sort_items(std::vector<PairWork>& items) {
+    std::stable_sort(items.begin(), items.end(), [](const PairWork& x, const PairWork& y) {
+        return (int64_t)(x.i_end - x.i_begin) * (x.j_end - x.j_begin) >
+               (int64_t)(y.i_end - y.i_begin) * (y.j_end - y.j_begin);
+    });
+}
+
+MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
+    std::vector<std::vector<int32_t>> per_row(n_rows);
+    for (size_t k = 0; k < items.size(); ++k) {
+        items[k].slot = (int32_t)k;
+        per_row[items[k].row0].push_back((int32_t)(2 * k));
+        if (items[k].flags & kOrient) per_row[items[k].row1].push_back((int32_t)(2 * k + 1));
+    }
+    // entries grouped by row, cut into merge chunks of at most kMergeChunk
+    MergePlan mp;
+    mp.chunk_ptr.assign(1, 0);
+    mp.row_chunk.assign(1, 0);
+    for (auto& v : per_row) {
+        for (size_t k = 0; k < v.size(); k += kMergeChunk) {
+            const size_t k1 = std::min(v.size(), k + kMergeChunk);
+            mp.row_list.insert(mp.row_list.end(), v.begin() + k, v.begin() + k1);
+            mp.chunk_ptr.push_back((int32_t)mp.row_list.size());
+        }
+        mp.row_chunk.push_back((int32_t)mp.chunk_ptr.size() - 1);
+    }
+    return mp;
+}
+
+PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
+                       int tile_j)
+    : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w) {
+    check_tile_j(tile_j);
+    check_pair_lds(nbins);
+    // bins need no clamp in the pair kernel when the bounding-box diagonal
+    // already keeps every pair below nbins
+    std::vector<const PointSet*> ps;
+    for (const auto& x : sets_) ps.push_back(&x);
+    clamp_ = nbins_ < bin_bound(ps, inv_w_);
+    build(tile_j);
+    for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
 }
 
 void PairEngine::build(int tile_j) {
     Context& ctx = Context::get();
-    // ---- layout of every set in the point buffer --------------------------
     std::vector<float4> buf;
-    layout_.resize(sets_.size());
-    for (size_t s = 0; s < sets_.size(); ++s) {
-        const PointSet& ps = sets_[s];
-        const int n = ps.size(), C = ps.n_classes;
-        const int t_self = n - std::min(3, n);  // self-loop tail candidates: [t_self, n)
-        const int t_col = n - (n % 4);          // cross-loop scalar columns: [t_col, n)
-        Layout& L = layout_[s];
-        L.base = (int)buf.size();
-        L.seg_start.assign(C, 0);
-        L.seg_len.assign(C, 0);
-        L.seg_body.assign(C, 0);
-        L.seg_colbody.assign(C, 0);
-        std::vector<std::vector<int>> body(C), mid(C), tail(C);
-        for (int i = 0; i < n; ++i) {
-            const int c = bits_of(ps.orig[i].w);
-            if (c < 0 || c >= C) throw std::runtime_error("ausaxs_hip: point class out of range");
-            if (i < t_self) body[c].push_back(i);
-            else if (i < t_col) mid[c].push_back(i);
-            else tail[c].push_back(i);
-        }
-        for (int c = 0; c < C; ++c) {
-            L.seg_start[c] = (int)buf.size();
-            for (auto* v : {&body[c], &mid[c], &tail[c]})
-                for (int i : *v) {
-                    float4 p = ps.orig[i];
-                    p.w = float_of(i);
-                    buf.push_back(p);
-                }
-            L.seg_len[c] = (int)(body[c].size() + mid[c].size() + tail[c].size());
-            L.seg_body[c] = (int)body[c].size();
-            L.seg_colbody[c] = (int)(body[c].size() + mid[c].size());
-        }
-    }
-
-    // ---- work items and strips --------------------------------------------
-    items_.clear();
-    strips_.clear();
-    rowmaps_.clear();
-    pairs_ = 0;
-    for (const PairJob& job : jobs_) {
-        const bool self = job.col_set < 0;
-        const PointSet& R = sets_[job.row_set];
-        const PointSet& Cs = self ? R : sets_[job.col_set];
-        const Layout& LR = layout_[job.row_set];
-        const Layout& LC = self ? LR : layout_[job.col_set];
-        const int CR = R.n_classes, CC = Cs.n_classes;
-        if ((int)job.rowmap.size() != CR * CC) throw std::runtime_error("ausaxs_hip: rowmap size mismatch");
-        for (int32_t r : job.rowmap)
-            if (r < -1 || r >= n_rows_) throw std::runtime_error("ausaxs_hip: rowmap entry out of range");
-        auto row_of = [&](int a, int b) {
-            const int32_t r = job.rowmap[a * CC + b];
-            if (r < 0) throw std::runtime_error("ausaxs_hip: missing histogram row for a present class pair");
-            return r;
-        };
-        if (self) {
-            const int64_t n = R.size();
-            pairs_ += n * (n - 1) / 2;
-            for (int a = 0; a < CR; ++a) {
-                if (LR.seg_body[a] == 0) continue;
-                const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_body[a];
-                // triangle: diagonal blocks masked to j > i, the rest plain
-                const int ra = row_of(a, a);
-                for (int i0 = s0; i0 < s1; i0 += kTileI) {
-                    const int i1 = std::min(i0 + kTileI, s1);
-                    add_items(i0, i1, i0, i1, ra, ra, true, tile_j);
-                    if (i1 < s1) add_items(i0, i1, i1, s1, ra, ra, false, tile_j);
-                }
-                for (int b = a + 1; b < CR; ++b) {
-                    if (LR.seg_body[b] == 0) continue;
-                    const int t0 = LR.seg_start[b], t1 = t0 + LR.seg_body[b];
-                    add_items(s0, s1, t0, t1, row_of(a, b), row_of(b, a), false, tile_j);
-                }
-            }
-            const int nn = R.size();
-            if (nn >= 2) {
-                Strip st{};
-                st.set_rows = job.row_set;
-                st.set_cols = job.row_set;
-                st.w = StripWork{nn, nn - std::min(3, nn), nn, nn, 1, CC};
-                st.rowmap_off = (int)rowmaps_.size();
-                strips_.push_back(st);
-                rowmaps_.insert(rowmaps_.end(), job.rowmap.begin(), job.rowmap.end());
-            }
-        } else {
-            const int64_t n = R.size(), m = Cs.size();
-            pairs_ += n * m;
-            for (int a = 0; a < CR; ++a) {
-                if (LR.seg_len[a] == 0) continue;
-                const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_len[a];
-                for (int b = 0; b < CC; ++b) {
-                    if (LC.seg_colbody[b] == 0) continue;
-                    const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
-                    const int r = row_of(a, b);
-                    add_items(s0, s1, t0, t1, r, r, false, tile_j);
-                }
-            }
-            const int mm = Cs.size();
-            if (mm % 4 && R.size() > 0) {
-                Strip st{};
-                st.set_rows = job.row_set;
-                st.set_cols = job.col_set;
-                st.w = StripWork{R.size(), mm - (mm % 4), mm, mm, 0, CC};
-                st.rowmap_off = (int)rowmaps_.size();
-                strips_.push_back(st);
-                rowmaps_.insert(rowmaps_.end(), job.rowmap.begin(), job.rowmap.end());
-            }
-        }
-    }
-    // largest items first: the tail of the launch is made of short items
-    std::stable_sort(items_.begin(), items_.end(), [](const PairWork& x, const PairWork& y) {
-        return (int64_t)(x.i_end - x.i_begin) * (x.j_end - x.j_begin) >
-               (int64_t)(y.i_end - y.i_begin) * (y.j_end - y.j_begin);
-    });
-    std::vector<std::vector<int32_t>> per_row(n_rows_);
-    for (size_t k = 0; k < items_.size(); ++k) {
-        items_[k].slot = (int32_t)k;
-        per_row[items_[k].row0].push_back((int32_t)(2 * k));
-        if (items_[k].flags & kOrient) per_row[items_[k].row1].push_back((int32_t)(2 * k + 1));
-    }
-    // entries grouped by row, cut into merge chunks of at most kMergeChunk
-    row_list_.clear();
-    chunk_ptr_.assign(1, 0);
-    row_chunk_.assign(1, 0);
-    for (auto& v : per_row) {
-        for (size_t k = 0; k < v.size(); k += kMergeChunk) {
-            const size_t k1 = std::min(v.size(), k + kMergeChunk);
-            row_list_.insert(row_list_.end(), v.begin() + k, v.begin() + k1);
-            chunk_ptr_.push_back((int32_t)row_list_.size());
-        }
-        row_chunk_.push_back((int32_t)chunk_ptr_.size() - 1);
-    }
+    layout_.clear();
+    for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf));
+    wl_ = WorkList{};
+    for (const PairJob& job : jobs_) append_job(wl_, job, sets_, layout_, n_rows_, tile_j);
+    sort_items(wl_.items);
+    mp_ = plan_merge(wl_.items, n_rows_);
 
     // ---- device residency -------------------------------------------------
     hipStream_t s = ctx.stream;
@@ -234,14 +247,14 @@ void PairEngine::build(int tile_j) {
         d_orig_.emplace_back();
         d_orig_.back().upload(ps.orig.data(), ps.orig.size(), s);
     }
-    d_items_.upload(items_.data(), items_.size(), s);
-    d_row_list_.upload(row_list_.data(), row_list_.size(), s);
-    d_chunk_ptr_.upload(chunk_ptr_.data(), chunk_ptr_.size(), s);
-    d_row_chunk_.upload(row_chunk_.data(), row_chunk_.size(), s);
-    d_tmp_cnt_.ensure((chunk_ptr_.size() - 1) * (nbins_ + 1));
-    d_tmp_dsum_.ensure((chunk_ptr_.size() - 1) * (nbins_ + 1));
-    d_rowmaps_.upload(rowmaps_.data(), rowmaps_.size(), s);
-    d_partial_.ensure(std::max<size_t>(items_.size(), 1) * 2 * (nbins_ + 1));
+    d_items_.upload(wl_.items.data(), wl_.items.size(), s);
+    d_row_list_.upload(mp_.row_list.data(), mp_.row_list.size(), s);
+    d_chunk_ptr_.upload(mp_.chunk_ptr.data(), mp_.chunk_ptr.size(), s);
+    d_row_chunk_.upload(mp_.row_chunk.data(), mp_.row_chunk.size(), s);
+    d_tmp_cnt_.ensure((size_t)mp_.n_chunks() * (nbins_ + 1));
+    d_tmp_dsum_.ensure((size_t)mp_.n_chunks() * (nbins_ + 1));
+    d_rowmaps_.upload(wl_.rowmaps.data(), wl_.rowmaps.size(), s);
+    d_partial_.ensure(std::max<size_t>(wl_.items.size(), 1) * 2 * (nbins_ + 1));
     d_cnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
     d_dsum_.ensure((size_t)n_rows_ * (nbins_ + 1));
     ctx.sync();
@@ -251,13 +264,13 @@ void PairEngine::enqueue(bool timed) {
     Context& ctx = Context::get();
     hipStream_t s = ctx.stream;
     if (timed) check(hipEventRecord(ev_[0], s), "hipEventRecord");
-    PairLaunch pl{d_pts_.p, d_items_.p, (int)items_.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_};
+    PairLaunch pl{d_pts_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     if (timed) check(hipEventRecord(ev_[1], s), "hipEventRecord");
-    MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, (int)chunk_ptr_.size() - 1,
+    MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
                    n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_cnt_.p, d_dsum_.p};
     check(launch_merge(ml, s), "merge kernels");
-    for (const Strip& st : strips_) {
+    for (const StripSpec& st : wl_.strips) {
         StripLaunch sl{d_orig_[st.set_rows].p, d_orig_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
                        d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};
         check(launch_strip(sl, s), "strip_kernel");
@@ -283,8 +296,8 @@ void PairEngine::run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, boo
 
 EngineTiming PairEngine::timing() {
     EngineTiming t;
-    t.pairs = pairs_;
-    t.n_items = (int)items_.size();
+    t.pairs = wl_.pairs;
+    t.n_items = (int)wl_.items.size();
     if (have_timing_) {
         check(hipEventSynchronize(ev_[2]), "hipEventSynchronize");
         check(hipEventElapsedTime(&t.pair_ms, ev_[0], ev_[1]), "hipEventElapsedTime");

@@ -64,6 +64,28 @@ struct DevBuf {
     }
 };
 
+// Pinned host buffer (hipHostMalloc), for copies captured in graphs.
+template <class T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        if (p) check(hipHostFree(p), "hipHostFree");
+        p = nullptr;
+        n = 0;
+        check(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault),
+              "hipHostMalloc");
+        n = std::max<size_t>(count, 1);
+    }
+};
+
 // A point set in the reference's original order with a class id per point
 // (form-factor type for XYZFF sets, weight class for XYZW sets).
 struct PointSet {
@@ -80,6 +102,53 @@ struct PairJob {
     int col_set = -1;             // -1: self job
     std::vector<int32_t> rowmap;  // [n_row_classes][n_col_classes]
 };
+
+// Class-sorted layout of one point set inside a point buffer. Within each
+// class the points follow the reference's loop geometry: body (all rules
+// SIMD), then the self-loop tail candidates, then the cross-loop scalar
+// columns (the last m mod 4 points).
+struct SetLayout {
+    int base = 0;                            // offset of the set in the buffer
+    std::vector<int> seg_start, seg_len;     // per class: full segment
+    std::vector<int> seg_body, seg_colbody;  // per class: self-body length, column-body length
+};
+// Appends the class-sorted points of ps to buf (w = original index bits);
+// perm (optional) receives the original index of every appended point.
+SetLayout layout_set(const PointSet& ps, std::vector<float4>& buf, std::vector<int32_t>* perm = nullptr);
+
+// Tail pairs of one job, evaluated by the strip kernel on original-order points.
+struct StripSpec {
+    int set_rows, set_cols;
+    StripWork w;
+    int rowmap_off;  // into WorkList::rowmaps
+};
+
+// Work items, strips and pair count of a list of jobs.
+struct WorkList {
+    std::vector<PairWork> items;
+    std::vector<StripSpec> strips;
+    std::vector<int32_t> rowmaps;
+    int64_t pairs = 0;
+};
+void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j);
+// largest items first: the tail of a launch is made of short items
+void sort_items(std::vector<PairWork>& items);
+
+// Merge plan: partial entries grouped by output row and cut into chunks.
+struct MergePlan {
+    std::vector<int32_t> row_list, chunk_ptr, row_chunk;
+    int n_chunks() const { return (int)chunk_ptr.size() - 1; }
+};
+// assigns slot = index to every item and plans the merge into n_rows rows
+MergePlan plan_merge(std::vector<PairWork>& items, int n_rows);
+
+// throws unless a work item of 512 rows x tile_j columns stays below the
+// fixed-point capacity of one LDS cell
+void check_tile_j(int tile_j);
+// LDS bytes per workgroup of pair_hist_kernel for nbins bins; throws if the
+// device offers less.
+size_t check_pair_lds(int nbins);
 
 struct EngineTiming {
     float pair_ms = 0;   // pair_hist_kernel on the engine stream (HIP events)
@@ -107,8 +176,8 @@ class PairEngine {
     int n_rows() const { return n_rows_; }
     int nbins() const { return nbins_; }
     float inv_w() const { return inv_w_; }
-    int64_t pairs() const { return pairs_; }
-    int n_items() const { return (int)items_.size(); }
+    int64_t pairs() const { return wl_.pairs; }
+    int n_items() const { return (int)wl_.items.size(); }
     const uint64_t* device_cnt() const { return d_cnt_.p; }
     const int64_t* device_dsum() const { return d_dsum_.p; }
     bool overflowed(const std::vector<uint64_t>& cnt) const;
@@ -117,28 +186,15 @@ class PairEngine {
     bool clamped() const { return clamp_; }
 
   private:
-    struct Layout {
-        int base = 0;                            // offset in the point buffer
-        std::vector<int> seg_start, seg_len;     // per class: full segment
-        std::vector<int> seg_body, seg_colbody;  // per class: self-body length, column-body length
-    };
     std::vector<PointSet> sets_;
     std::vector<PairJob> jobs_;
-    std::vector<Layout> layout_;
-    std::vector<PairWork> items_;
-    std::vector<int32_t> row_list_, chunk_ptr_, row_chunk_;
-    struct Strip {
-        int set_rows, set_cols;
-        StripWork w;
-        int rowmap_off;
-    };
-    std::vector<Strip> strips_;
-    std::vector<int32_t> rowmaps_;
+    std::vector<SetLayout> layout_;
+    WorkList wl_;
+    MergePlan mp_;
     int n_rows_, nbins_;
     float inv_w_;
     bool clamp_ = true;
     int grid_ = 0;
-    int64_t pairs_ = 0;
 
     DevBuf<float4> d_pts_;
     std::vector<DevBuf<float4>> d_orig_;
@@ -150,7 +206,6 @@ class PairEngine {
     bool have_timing_ = false;
 
     void build(int tile_j);
-    void add_items(int i0, int i1, int j0, int j1, int row0, int row1, bool diag, int tile_j);
 };
 
 // Upper bound on the bin of any pair in the given points (bounding-box

@@ -140,18 +140,24 @@ std::vector<double> default_q_axis(double qmin, double qmax) {
 // weighted entries
 // ---------------------------------------------------------------------------
 namespace {
+inline void add(WeightedEntry& a, const WeightedEntry& b) { entry_add(a, b); }
+}  // namespace
 
-inline void add(WeightedEntry& a, const WeightedEntry& b) {
+void entry_add(WeightedEntry& a, const WeightedEntry& b) {
     a.value += b.value;
     a.count += b.count;
     a.bin_center += b.bin_center;
 }
+
+namespace {
 
 inline void scale(WeightedEntry& e, double f) {  // WeightedEntry.h operator*
     e.value *= f;
     e.count = (uint32_t)(f * e.count);
     e.bin_center *= f;
 }
+
+}  // namespace
 
 unsigned max_bin_of(const std::vector<WeightedEntry>& p) {
     for (int i = (int)p.size() - 1; i >= 10; --i)
@@ -175,6 +181,8 @@ std::vector<double> axis_of(const std::vector<WeightedEntry>& p, double bin_widt
     if (n) d[0] = 0;
     return d;
 }
+
+namespace {
 
 std::vector<float4> to_points(const float* xyzw, int n) {
     std::vector<float4> v(n);

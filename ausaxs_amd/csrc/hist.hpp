@@ -27,6 +27,13 @@ struct WeightedEntry {  // distribution/detail/WeightedEntry.h:16-50
     double bin_center = 0;
 };
 
+void entry_add(WeightedEntry& a, const WeightedEntry& b);
+// HistogramManagerMTFFAvg.cpp:340-352: last bin >= 10 with a non-zero value, + 1 (at least 10)
+unsigned max_bin_of(const std::vector<WeightedEntry>& p);
+// WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58) with
+// d[0] = 0 (DistanceHistogram.cpp:44-47); plain bin positions if !weighted
+std::vector<double> axis_of(const std::vector<WeightedEntry>& p, double bin_width, bool weighted);
+
 struct HistSettings {
     double bin_width = 0.25;    // settings::axes::bin_width (d_axis width)
     uint32_t bin_count = 8000;  // settings::axes::bin_count

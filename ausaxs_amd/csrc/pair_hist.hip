@@ -291,8 +291,9 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     const int i = (int)(tid / nt);
     const int t = w.t_begin + (int)(tid % nt);
     if (w.self && i >= t) return;
-    const float4 a = p.rows[i];
-    const float4 c = p.cols[t];
+    const int rep = blockIdx.y;
+    const float4 a = p.rows[p.rows_rep * rep + i];
+    const float4 c = p.cols[p.cols_rep * rep + t];
     const float dx = a.x - c.x, dy = a.y - c.y, dz = a.z - c.z;
     bool scalar_rule = true;
     if (w.self) scalar_rule = t >= w.n_cols_total - ((w.n_cols_total - i - 1) % 4);
@@ -308,8 +309,9 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
         const float d2 = (dx * dx + dy * dy) + dz * dz;
         simd_rule_exact(d2, p.inv_w, x, b, dl);
     }
-    const int row = p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
+    int row = p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
     if (row < 0) return;  // the host guarantees a row for every present class pair
+    row += rep * p.out_rep;
     const unsigned bin = min((unsigned)(int)b, (unsigned)p.nbins);
     const uint32_t low = quantize(dl);
     const size_t cell = (size_t)row * p.stride + bin;
@@ -391,8 +393,9 @@ hipError_t launch_merge(const MergeLaunch& m, hipStream_t s) {
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s) {
     const long long n = (long long)p.w.n_rows * (p.w.t_end - p.w.t_begin);
     if (n <= 0) return hipSuccess;
+    if (p.n_rep <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(strip_kernel, dim3(blocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(strip_kernel, dim3(blocks, p.n_rep), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

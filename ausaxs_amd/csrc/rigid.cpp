@@ -1,0 +1,687 @@
+// Batched rigid-body scan (see rigid.hpp).
+#include "rigid.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace ausaxs::gpu {
+
+namespace {
+constexpr int kMaxBatch = 1024;  // conformations per graph launch
+constexpr int kMaxClasses = 64;
+
+// bounding sphere (centre, radius) of n points given as x,y,z with stride
+void sphere_of(const double* p, int n, int stride, double c[3], double& r) {
+    c[0] = c[1] = c[2] = 0;
+    r = n ? 0 : -1;
+    if (!n) return;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) c[k] += p[(size_t)i * stride + k];
+    for (int k = 0; k < 3; ++k) c[k] /= n;
+    for (int i = 0; i < n; ++i) {
+        double d2 = 0;
+        for (int k = 0; k < 3; ++k) d2 += (p[(size_t)i * stride + k] - c[k]) * (p[(size_t)i * stride + k] - c[k]);
+        r = std::max(r, std::sqrt(d2));
+    }
+}
+}  // namespace
+
+struct RigidScan::Plan {
+    int n_conf = 0, nbins = 0, tile_j = 0, grid = 0;
+    std::vector<int> moving;
+    int64_t S = 0, M = 0;  // shared points, points per replica
+    int n_slots = 0;
+    std::vector<PairWork> items;
+    MergePlan mp;
+    std::vector<StripLaunch> strips;  // device pointers filled at build
+    int64_t pairs = 0;
+    int nq_cap = 0;
+    double graph_cw = 0;
+    int graph_nq = -1;
+
+    DevBuf<float4> pts, orig;
+    DevBuf<PairWork> d_items;
+    DevBuf<int32_t> row_list, chunk_ptr, row_chunk, rowmaps;
+    DevBuf<uint64_t> partial, cnt, tmp_cnt;
+    DevBuf<int64_t> dsum, tmp_dsum;
+    DevBuf<int32_t> slot_src, slot_body, slot_dst, slot_dsto, slot_tag, slot_cls, body_moving;
+    DevBuf<double> T, q, H, dbar, I;
+    DevBuf<int> max_bin, flags;
+    HostBuf<double> hT, hq, hI;
+    HostBuf<int> hflags;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4]{};
+
+    Plan() {
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    }
+    ~Plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void drop_graph() {
+        if (exec) check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
+        if (graph) check(hipGraphDestroy(graph), "hipGraphDestroy");
+        exec = nullptr;
+        graph = nullptr;
+        graph_nq = -1;
+    }
+};
+
+RigidScan::~RigidScan() = default;
+
+int RigidScan::tri(int a, int b) const {
+    if (a > b) std::swap(a, b);
+    // rows of (a, b), a <= b, in row-major upper-triangle order
+    return a * C_ - a * (a - 1) / 2 + (b - a);
+}
+
+RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, int n_bodies, const double* wxyz,
+                     const float* ww, int n_waters, const HistSettings& s)
+    : s_(s), inv_w_(inv_width(s)) {
+    if (n_bodies < 1) throw std::runtime_error("ausaxs_hip: a rigid-body scan needs at least one body");
+    if (n_waters < 0) throw std::runtime_error("ausaxs_hip: negative water count");
+    if (!(s.bin_width > 0) || s.bin_count < 10) throw std::runtime_error("ausaxs_hip: invalid bin settings");
+    if (s.tile_j != 0) check_tile_j(s.tile_j);
+    body_off_.assign(1, 0);
+    for (int b = 0; b < n_bodies; ++b) {
+        if (sizes[b] < 0) throw std::runtime_error("ausaxs_hip: negative body size");
+        sizes_.push_back(sizes[b]);
+        body_off_.push_back(body_off_.back() + sizes[b]);
+    }
+    const int n = body_off_.back();
+    xyz_.assign(xyz, xyz + 3 * (size_t)n);
+    w_.assign(w, w + n);
+    for (double v : xyz_)
+        if (!std::isfinite(v)) throw std::runtime_error("ausaxs_hip: body coordinates must be finite");
+
+    // weight classes over bodies and waters
+    std::map<uint32_t, int> id;
+    auto class_of = [&](float wi) {
+        uint32_t key;
+        std::memcpy(&key, &wi, 4);
+        auto it = id.find(key);
+        if (it == id.end()) {
+            it = id.emplace(key, (int)class_w_.size()).first;
+            class_w_.push_back(wi);
+        }
+        return it->second;
+    };
+    point_cls_.resize(n);
+    for (int i = 0; i < n; ++i) point_cls_[i] = class_of(w_[i]);
+    waters_.orig.resize(n_waters);
+    std::vector<float> water_w(n_waters);
+    std::vector<bool> in_water;
+    for (int i = 0; i < n_waters; ++i) {
+        water_w[i] = ww[i];
+        const int c = class_of(ww[i]);
+        waters_.orig[i] = make_float4((float)wxyz[3 * i], (float)wxyz[3 * i + 1], (float)wxyz[3 * i + 2],
+                                      float_of(c));
+        if ((int)in_water.size() <= c) in_water.resize(c + 1, false);
+        in_water[c] = true;
+    }
+    if (class_w_.size() > (size_t)kMaxClasses)
+        throw std::runtime_error("ausaxs_hip: more than 64 distinct weights in a rigid-body scan");
+    C_ = std::max<int>(1, (int)class_w_.size());
+    if (class_w_.empty()) class_w_.push_back(0.f);
+    waters_.n_classes = C_;
+    for (int c = 0; c < (int)in_water.size(); ++c)
+        if (in_water[c]) water_classes_.push_back(c);
+
+    // rows of one conformation: aa class pairs, then aw (water class x class)
+    R_aa_ = C_ * (C_ + 1) / 2;
+    R_ = R_aa_ + (int)water_classes_.size() * C_;
+    row_kind_.assign(R_, 0);
+    row_w_.assign(R_, 0.0);
+    for (int a = 0; a < C_; ++a)
+        for (int b = a; b < C_; ++b) row_w_[tri(a, b)] = (double)(class_w_[a] * class_w_[b]);
+    for (size_t k = 0; k < water_classes_.size(); ++k)
+        for (int b = 0; b < C_; ++b) {
+            const int r = R_aa_ + (int)k * C_ + b;
+            row_kind_[r] = 1;
+            row_w_[r] = (double)(class_w_[water_classes_[k]] * class_w_[b]);
+        }
+
+    // bounding spheres for the bin bound of each scan
+    center_.resize(4 * (size_t)n_bodies);
+    for (int b = 0; b < n_bodies; ++b)
+        sphere_of(&xyz_[3 * (size_t)body_off_[b]], sizes_[b], 3, &center_[4 * b], center_[4 * b + 3]);
+    if (n_waters) {
+        std::vector<double> wd(3 * (size_t)n_waters);
+        for (int i = 0; i < n_waters; ++i)
+            for (int k = 0; k < 3; ++k) wd[3 * i + k] = (&waters_.orig[i].x)[k];
+        sphere_of(wd.data(), n_waters, 3, wcenter_, wradius_);
+    }
+
+    // cached self terms (calc_self_correlation, calc_ww; PartialHistogramManagerMT.cpp:259-294)
+    std::vector<PointSet> sets(n_bodies + 1);
+    std::vector<PairJob> jobs;
+    std::vector<int32_t> aa_map((size_t)C_ * C_), ww_map((size_t)C_ * C_);
+    for (int a = 0; a < C_; ++a)
+        for (int b = 0; b < C_; ++b) {
+            aa_map[a * C_ + b] = tri(a, b);
+            ww_map[a * C_ + b] = R_aa_ + tri(a, b);
+        }
+    for (int b = 0; b < n_bodies; ++b) {
+        PointSet& ps = sets[b];
+        ps.n_classes = C_;
+        ps.orig.resize(sizes_[b]);
+        for (int i = 0; i < sizes_[b]; ++i) {
+            const size_t g = (size_t)body_off_[b] + i;
+            ps.orig[i] = make_float4((float)xyz_[3 * g], (float)xyz_[3 * g + 1], (float)xyz_[3 * g + 2],
+                                     float_of(point_cls_[g]));
+        }
+        jobs.push_back(PairJob{b, -1, aa_map});
+    }
+    sets[n_bodies] = waters_;
+    jobs.push_back(PairJob{n_bodies, -1, ww_map});
+    std::vector<const PointSet*> ptrs;
+    for (auto& ps : sets) ptrs.push_back(&ps);
+    base_nb_ = std::max(10, std::min<int>((int)s_.bin_count, bin_bound(ptrs, inv_w_)));
+    std::vector<uint64_t> cnt;
+    std::vector<int64_t> dsum;
+    {
+        PairEngine eng(sets, jobs, 2 * R_aa_, inv_w_, base_nb_, s_.tile_j ? s_.tile_j : 8000);
+        eng.run(cnt, dsum);
+        if (eng.overflowed(cnt))
+            throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width");
+    }
+    const int stride = base_nb_ + 1;
+    base_aa_.assign(base_nb_, WeightedEntry{});
+    base_ww_.assign(base_nb_, WeightedEntry{});
+    for (int r = 0; r < 2 * R_aa_; ++r) {
+        const int rr = r % R_aa_;
+        std::vector<WeightedEntry>& dst = r < R_aa_ ? base_aa_ : base_ww_;
+        for (int b = 0; b < base_nb_; ++b) {
+            const uint64_t c = cnt[(size_t)r * stride + b];
+            if (!c) continue;
+            const int64_t ds = dsum[(size_t)r * stride + b];
+            WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
+            e.value = 2.0 * (double)c * row_w_[rr];
+            e.count = (uint32_t)(2 * c);
+            e.bin_center = 2.0 * (((double)c * b + (double)ds * kDeltaScale) / (double)inv_w_);
+            entry_add(dst[b], e);
+        }
+    }
+    // SimpleCalculator.h:144-160: each self job adds (sum w^2, (unsigned) sum w^2, 0) at bin 0
+    auto self_term = [](const float* v, int m) {
+        double t = 0;
+        for (int i = 0; i < m; ++i) t += (double)(v[i] * v[i]);
+        return WeightedEntry{t, (uint32_t)t, 0};
+    };
+    for (int b = 0; b < n_bodies; ++b) entry_add(base_aa_[0], self_term(&w_[body_off_[b]], sizes_[b]));
+    entry_add(base_ww_[0], self_term(water_w.data(), n_waters));
+
+    Context& ctx = Context::get();
+    std::vector<double> base(6 * (size_t)base_nb_);
+    for (int b = 0; b < base_nb_; ++b) {
+        base[b] = base_aa_[b].value;
+        base[base_nb_ + b] = base_aa_[b].count;
+        base[2 * base_nb_ + b] = base_aa_[b].bin_center;
+        base[3 * base_nb_ + b] = base_ww_[b].value;
+        base[4 * base_nb_ + b] = base_ww_[b].count;
+        base[5 * base_nb_ + b] = base_ww_[b].bin_center;
+    }
+    d_base_.upload(base.data(), base.size(), ctx.stream);
+    d_xyz_.upload(xyz_.data(), xyz_.size(), ctx.stream);
+    d_row_kind_.upload(row_kind_.data(), row_kind_.size(), ctx.stream);
+    d_row_w_.upload(row_w_.data(), row_w_.size(), ctx.stream);
+    ctx.sync();
+}
+
+int RigidScan::bound_bins(const double* T, int n_conf) const {
+    // bounding spheres moved by each conformation; ||R x|| <= ||R||_F ||x||
+    const int B = n_bodies();
+    double dmax = 2 * std::max(wradius_, 0.0), cmax = 0;
+    std::vector<double> c(3 * (size_t)B), r(B);
+    for (int k = 0; k < n_conf; ++k) {
+        for (int b = 0; b < B; ++b) {
+            const double* t = T + ((size_t)k * B + b) * 12;
+            double f = 0;
+            for (int e = 0; e < 9; ++e) f += t[e] * t[e];
+            const double* cb = &center_[4 * b];
+            for (int e = 0; e < 3; ++e) {
+                c[3 * b + e] = t[3 * e] * cb[0] + t[3 * e + 1] * cb[1] + t[3 * e + 2] * cb[2] + t[9 + e];
+                cmax = std::max(cmax, std::fabs(c[3 * b + e]));
+            }
+            r[b] = sizes_[b] ? cb[3] * std::sqrt(f) : -1;
+            if (!std::isfinite(r[b]) || !std::isfinite(c[3 * b]) || !std::isfinite(c[3 * b + 1]) ||
+                !std::isfinite(c[3 * b + 2]))
+                throw std::runtime_error("ausaxs_hip: transforms must be finite");
+        }
+        for (int i = 0; i < B; ++i) {
+            if (r[i] < 0) continue;
+            dmax = std::max(dmax, 2 * r[i]);
+            for (int j = 0; j < i; ++j) {
+                if (r[j] < 0) continue;
+                const double dx = c[3 * i] - c[3 * j], dy = c[3 * i + 1] - c[3 * j + 1], dz = c[3 * i + 2] - c[3 * j + 2];
+                dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz) + r[i] + r[j]);
+            }
+            if (wradius_ >= 0) {
+                const double dx = c[3 * i] - wcenter_[0], dy = c[3 * i + 1] - wcenter_[1], dz = c[3 * i + 2] - wcenter_[2];
+                dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz) + r[i] + wradius_);
+            }
+        }
+    }
+    // float rounding of the moved coordinates: a few ulp of their magnitude
+    dmax += 1e-5 * (cmax + dmax) + 1e-3;
+    const double x = dmax * (double)inv_w_ * (1 + 1e-5) + 2;
+    if (!(x < 1e9)) throw std::runtime_error("ausaxs_hip: conformations too far apart for the bin axis");
+    int nb = (int)std::ceil(x);
+    nb = (nb + 15) / 16 * 16;  // a little slack: nearby scans reuse one plan
+    return std::max(10, std::min<int>(nb, (int)s_.bin_count));
+}
+
+std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::vector<int>& moving, int nbins,
+                                                       int tile_j) {
+    check_pair_lds(nbins);
+    check_tile_j(tile_j);
+    const int B = n_bodies();
+    auto P = std::make_unique<Plan>();
+    Plan& p = *P;
+    p.n_conf = n_conf;
+    p.nbins = nbins;
+    p.tile_j = tile_j;
+    p.moving = moving;
+    const int n_rows = (n_conf + 1) * R_;
+
+    // set 0 = waters, 1 + b = body b; layout: waters, static bodies (shared), moving bodies (replica 0)
+    std::vector<PointSet> sets(1 + B);
+    sets[0] = waters_;
+    for (int b = 0; b < B; ++b) {
+        sets[1 + b].n_classes = C_;
+        sets[1 + b].orig.resize(sizes_[b]);
+        for (int i = 0; i < sizes_[b]; ++i)
+            sets[1 + b].orig[i] = make_float4(0.f, 0.f, 0.f, float_of(point_cls_[body_off_[b] + i]));
+    }
+    std::vector<SetLayout> L(1 + B);
+    std::vector<std::vector<int32_t>> perm(B);
+    std::vector<float4> buf;
+    L[0] = layout_set(sets[0], buf);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) p.S = (int64_t)buf.size();
+        for (int b = 0; b < B; ++b)
+            if ((moving[b] != 0) == (pass == 1)) L[1 + b] = layout_set(sets[1 + b], buf, &perm[b]);
+    }
+    p.M = (int64_t)buf.size() - p.S;
+    const int64_t total = p.S + (int64_t)n_conf * p.M;
+    if (total >= std::numeric_limits<int32_t>::max())
+        throw std::runtime_error("ausaxs_hip: too many points in one rigid-body batch");
+
+    // slots of the transform kernel
+    std::vector<int32_t> src, body, dst, dsto, tag, cls;
+    for (int b = 0; b < B; ++b)
+        for (int k = 0; k < sizes_[b]; ++k) {
+            const int o = perm[b][k];
+            src.push_back(body_off_[b] + o);
+            body.push_back(b);
+            dst.push_back(L[1 + b].base + k);
+            dsto.push_back(L[1 + b].base + o);
+            tag.push_back(o);
+            cls.push_back(point_cls_[body_off_[b] + o]);
+        }
+    p.n_slots = (int)src.size();
+
+    // jobs: inter-body blocks (rows body j, columns body i, j < i) and body-water blocks
+    std::vector<int32_t> aa_map((size_t)C_ * C_), aw_map((size_t)C_ * C_, -1);
+    for (int a = 0; a < C_; ++a)
+        for (int b = 0; b < C_; ++b) aa_map[a * C_ + b] = tri(a, b);
+    for (size_t k = 0; k < water_classes_.size(); ++k)
+        for (int b = 0; b < C_; ++b) aw_map[water_classes_[k] * C_ + b] = R_aa_ + (int)k * C_ + b;
+    auto shifted = [&](std::vector<int32_t> m) {
+        for (auto& r : m)
+            if (r >= 0) r += n_conf * R_;
+        return m;
+    };
+    WorkList shared, tmpl;
+    for (int i = 0; i < B; ++i) {
+        for (int j = 0; j < i; ++j) {
+            const bool mv = moving[i] || moving[j];
+            append_job(mv ? tmpl : shared, PairJob{1 + j, 1 + i, mv ? aa_map : shifted(aa_map)}, sets, L, n_rows,
+                       tile_j);
+        }
+        if (waters_.size() > 0)
+            append_job(moving[i] ? tmpl : shared, PairJob{0, 1 + i, moving[i] ? aw_map : shifted(aw_map)}, sets, L,
+                       n_rows, tile_j);
+    }
+    p.pairs = shared.pairs + (int64_t)n_conf * tmpl.pairs;
+
+    // items: shared ones, then each template item for every conformation
+    sort_items(shared.items);
+    sort_items(tmpl.items);
+    p.items = shared.items;
+    auto shift = [&](int32_t v, int c) { return v >= p.S ? (int32_t)(v + (int64_t)c * p.M) : v; };
+    for (const PairWork& t : tmpl.items)
+        for (int c = 0; c < n_conf; ++c) {
+            PairWork w = t;
+            w.i_begin = shift(t.i_begin, c);
+            w.i_end = t.i_begin >= p.S ? w.i_begin + (t.i_end - t.i_begin) : t.i_end;
+            w.j_begin = shift(t.j_begin, c);
+            w.j_end = t.j_begin >= p.S ? w.j_begin + (t.j_end - t.j_begin) : t.j_end;
+            w.row0 += c * R_;
+            w.row1 += c * R_;
+            p.items.push_back(w);
+        }
+    p.mp = plan_merge(p.items, n_rows);
+
+    // device residency
+    Context& ctx = Context::get();
+    hipStream_t s = ctx.stream;
+    p.pts.ensure(total);
+    p.orig.ensure(total);
+    if (waters_.size()) {
+        check(hipMemcpyAsync(p.pts.p, buf.data(), waters_.size() * sizeof(float4), hipMemcpyHostToDevice, s),
+              "H2D waters");
+        check(hipMemcpyAsync(p.orig.p, waters_.orig.data(), waters_.size() * sizeof(float4), hipMemcpyHostToDevice,
+                             s),
+              "H2D waters");
+    }
+    p.d_items.upload(p.items.data(), p.items.size(), s);
+    p.row_list.upload(p.mp.row_list.data(), p.mp.row_list.size(), s);
+    p.chunk_ptr.upload(p.mp.chunk_ptr.data(), p.mp.chunk_ptr.size(), s);
+    p.row_chunk.upload(p.mp.row_chunk.data(), p.mp.row_chunk.size(), s);
+    std::vector<int32_t> rowmaps = tmpl.rowmaps;
+    rowmaps.insert(rowmaps.end(), shared.rowmaps.begin(), shared.rowmaps.end());
+    p.rowmaps.upload(rowmaps.data(), rowmaps.size(), s);
+    p.partial.ensure(std::max<size_t>(p.items.size(), 1) * 2 * (nbins + 1));
+    p.tmp_cnt.ensure((size_t)p.mp.n_chunks() * (nbins + 1));
+    p.tmp_dsum.ensure((size_t)p.mp.n_chunks() * (nbins + 1));
+    p.cnt.ensure((size_t)n_rows * (nbins + 1));
+    p.dsum.ensure((size_t)n_rows * (nbins + 1));
+    p.slot_src.upload(src.data(), src.size(), s);
+    p.slot_body.upload(body.data(), body.size(), s);
+    p.slot_dst.upload(dst.data(), dst.size(), s);
+    p.slot_dsto.upload(dsto.data(), dsto.size(), s);
+    p.slot_tag.upload(tag.data(), tag.size(), s);
+    p.slot_cls.upload(cls.data(), cls.size(), s);
+    p.body_moving.upload(moving.data(), moving.size(), s);
+    p.T.ensure((size_t)n_conf * B * 12);
+    p.hT.ensure((size_t)n_conf * B * 12);
+    p.H.ensure((size_t)n_conf * nbins);
+    p.dbar.ensure((size_t)n_conf * nbins);
+    p.max_bin.ensure(n_conf);
+    p.flags.ensure(n_conf);
+    p.hflags.ensure(n_conf);
+
+    // strips: template strips once per conformation (replica strides), shared ones once
+    auto strip = [&](const StripSpec& st, int rowmap_off, bool replicated) {
+        const bool rm = st.set_rows > 0 && moving[st.set_rows - 1];
+        const bool cm = st.set_cols > 0 && moving[st.set_cols - 1];
+        StripLaunch sl{p.orig.p + L[st.set_rows].base, p.orig.p + L[st.set_cols].base, st.w,
+                       p.rowmaps.p + rowmap_off, p.cnt.p, p.dsum.p, inv_w_, nbins, nbins + 1};
+        sl.n_rep = replicated ? n_conf : 1;
+        sl.rows_rep = replicated && rm ? p.M : 0;
+        sl.cols_rep = replicated && cm ? p.M : 0;
+        sl.out_rep = replicated ? R_ : 0;
+        p.strips.push_back(sl);
+    };
+    for (const StripSpec& st : tmpl.strips) strip(st, st.rowmap_off, true);
+    for (const StripSpec& st : shared.strips) strip(st, (int)tmpl.rowmaps.size() + st.rowmap_off, false);
+
+    // every index the kernels will touch, checked on the host before any launch
+    auto fail = [](const char* what) {
+        throw std::runtime_error(std::string("ausaxs_hip: rigid-body plan out of bounds: ") + what);
+    };
+    for (const PairWork& w : p.items) {
+        if (w.i_begin < 0 || w.i_begin >= w.i_end || w.i_end > total || w.j_begin < 0 || w.j_begin >= w.j_end ||
+            w.j_end > total)
+            fail("item point range");
+        if (w.row0 < 0 || w.row0 >= n_rows || w.row1 < 0 || w.row1 >= n_rows) fail("item row");
+        if ((int64_t)(w.i_end - w.i_begin) * (w.j_end - w.j_begin) > kMaxPairsPerItem) fail("item size");
+    }
+    for (size_t k = 0; k < p.strips.size(); ++k) {
+        const StripLaunch& sl = p.strips[k];
+        const StripSpec& st = k < tmpl.strips.size() ? tmpl.strips[k] : shared.strips[k - tmpl.strips.size()];
+        const int64_t rb = L[st.set_rows].base, cb = L[st.set_cols].base;
+        const int64_t last = sl.n_rep - 1;
+        if (rb < 0 || rb + last * sl.rows_rep + sl.w.n_rows > total) fail("strip rows");
+        if (cb + sl.w.t_begin < 0 || cb + last * sl.cols_rep + sl.w.t_end > total) fail("strip columns");
+        const int off = (int)(sl.rowmap - p.rowmaps.p);
+        const size_t nmap = (size_t)C_ * sl.w.n_col_classes;
+        if (off < 0 || off + nmap > rowmaps.size()) fail("strip rowmap");
+        for (size_t e = 0; e < nmap; ++e)
+            if (rowmaps[off + e] >= 0 && rowmaps[off + e] + last * sl.out_rep >= n_rows) fail("strip output row");
+    }
+    for (int k = 0; k < p.n_slots; ++k) {
+        const int64_t reach = moving[body[k]] ? (int64_t)(n_conf - 1) * p.M : 0;
+        const int64_t lim = moving[body[k]] ? total : p.S;
+        if (dst[k] < 0 || dst[k] + reach >= lim || dsto[k] < 0 || dsto[k] + reach >= lim) fail("transform slot");
+        if (src[k] < 0 || src[k] >= body_off_.back()) fail("transform source");
+    }
+    ctx.sync();
+    return P;
+}
+
+void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
+    Context& ctx = Context::get();
+    hipStream_t s = ctx.stream;
+    const int B = n_bodies();
+    if (copies) {
+        check(hipMemcpyAsync(p.T.p, p.hT.p, (size_t)p.n_conf * B * 12 * sizeof(double), hipMemcpyHostToDevice, s),
+              "H2D transforms");
+        if (nq > 0)
+            check(hipMemcpyAsync(p.q.p, p.hq.p, (size_t)nq * sizeof(double), hipMemcpyHostToDevice, s), "H2D q");
+    }
+    TransformLaunch tl{d_xyz_.p, p.T.p, p.slot_src.p, p.slot_body.p, p.slot_dst.p, p.slot_dsto.p, p.slot_tag.p,
+                       p.slot_cls.p, p.body_moving.p, p.n_slots, B, p.n_conf, p.M, p.pts.p, p.orig.p};
+    check(launch_transform(tl, s), "transform_kernel");
+    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid};
+    check(launch_pair_hist(pl, s), "pair_hist_kernel");
+    MergeLaunch ml{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * R_,
+                   p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.cnt.p, p.dsum.p};
+    check(launch_merge(ml, s), "merge kernels");
+    for (const StripLaunch& sl : p.strips) check(launch_strip(sl, s), "strip_kernel");
+    FinalizeLaunch fl{p.cnt.p, p.dsum.p, R_, p.nbins, p.n_conf, d_row_kind_.p, d_row_w_.p, d_base_.p, base_nb_,
+                      (double)inv_w_, (s_.bin_count * s_.bin_width) / s_.bin_count, cw, s_.weighted_bins ? 1 : 0,
+                      p.H.p, p.dbar.p, p.max_bin.p, p.flags.p};
+    check(launch_finalize(fl, s), "finalize_kernel");
+    DebyeBatchLaunch dl{p.H.p, p.dbar.p, p.max_bin.p, p.q.p, p.nbins, p.n_conf, nq, p.I.p};
+    check(launch_debye_batch(dl, s), "debye_batch_kernel");
+    if (copies) {
+        if (nq > 0)
+            check(hipMemcpyAsync(p.hI.p, p.I.p, (size_t)p.n_conf * nq * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "D2H I");
+        check(hipMemcpyAsync(p.hflags.p, p.flags.p, (size_t)p.n_conf * sizeof(int), hipMemcpyDeviceToHost, s),
+              "D2H flags");
+    }
+}
+
+float RigidScan::time_pair(Plan& p, int grid) {
+    Context& ctx = Context::get();
+    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid};
+    check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");  // warm-up
+    check(hipEventRecord(p.ev[0], ctx.stream), "hipEventRecord");
+    check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");
+    check(hipEventRecord(p.ev[1], ctx.stream), "hipEventRecord");
+    check(hipEventSynchronize(p.ev[1]), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, p.ev[0], p.ev[1]), "hipEventElapsedTime");
+    return ms;
+}
+
+RigidScan::Plan& RigidScan::plan_for(int n_conf, const std::vector<int>& moving, int nbins, const double* T) {
+    auto key = std::make_tuple(n_conf, moving, nbins);
+    auto it = plans_.find(key);
+    if (it != plans_.end()) return *it->second;
+    Context& ctx = Context::get();
+    std::unique_ptr<Plan> best;
+    if (s_.tile_j != 0) {
+        best = build_plan(n_conf, moving, nbins, s_.tile_j);
+    } else {
+        // autotuner: the tile width first (one workgroup per item), then the
+        // persistent grid of the winner, each timed on this batch's transforms
+        const int cus = ctx.prop.multiProcessorCount;
+        float best_ms = std::numeric_limits<float>::max();
+        int cand = 0;
+        for (int tile : {1024, 2048, 4096, 8000}) {
+            auto p = build_plan(n_conf, moving, nbins, tile);
+            check(hipMemcpyAsync(p->T.p, T, (size_t)n_conf * n_bodies() * 12 * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx.stream),
+                  "H2D transforms");
+            TransformLaunch tl{d_xyz_.p, p->T.p, p->slot_src.p, p->slot_body.p, p->slot_dst.p, p->slot_dsto.p,
+                               p->slot_tag.p, p->slot_cls.p, p->body_moving.p, p->n_slots, n_bodies(), n_conf,
+                               p->M, p->pts.p, p->orig.p};
+            check(launch_transform(tl, ctx.stream), "transform_kernel");
+            const float ms = time_pair(*p, 0);
+            ++cand;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = std::move(p);
+            }
+        }
+        for (int g : {2 * cus, 4 * cus, 8 * cus}) {
+            if (g >= (int)best->items.size()) continue;
+            const float ms = time_pair(*best, g);
+            ++cand;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best->grid = g;
+            }
+        }
+        info_.pair_ms = best_ms;
+        info_.tuned_candidates = cand;
+    }
+    Plan& ref = *best;
+    plans_.emplace(key, std::move(best));
+    return ref;
+}
+
+void RigidScan::run_batch(Plan& p, const double* T, const double* q, int nq, double cw, double* I) {
+    Context& ctx = Context::get();
+    hipStream_t s = ctx.stream;
+    const int B = n_bodies();
+    std::memcpy(p.hT.p, T, (size_t)p.n_conf * B * 12 * sizeof(double));
+    if (nq > p.nq_cap || !p.q.p) {
+        p.drop_graph();
+        p.q.ensure(nq);
+        p.hq.ensure(nq);
+        p.I.ensure((size_t)p.n_conf * nq);
+        p.hI.ensure((size_t)p.n_conf * nq);
+        p.nq_cap = nq;
+    }
+    std::memcpy(p.hq.p, q, (size_t)nq * sizeof(double));
+    if (!p.exec || p.graph_nq != nq || p.graph_cw != cw) {
+        p.drop_graph();
+        // the whole batch as one graph: copies in, kernels, copies out (no
+        // memset nodes: every output the graph reads back is written by a kernel)
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            enqueue(p, nq, cw, true);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(s, &p.graph), "hipStreamEndCapture");
+        check(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        p.graph_nq = nq;
+        p.graph_cw = cw;
+    }
+    check(hipEventRecord(p.ev[2], s), "hipEventRecord");
+    check(hipGraphLaunch(p.exec, s), "hipGraphLaunch");
+    check(hipEventRecord(p.ev[3], s), "hipEventRecord");
+    ctx.sync();
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, p.ev[2], p.ev[3]), "hipEventElapsedTime");
+    info_.total_ms += ms;
+    for (int c = 0; c < p.n_conf; ++c)
+        if (p.hflags.p[c])
+            throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
+                                     std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) + " A)");
+    std::memcpy(I, p.hI.p, (size_t)p.n_conf * nq * sizeof(double));
+}
+
+void RigidScan::scan(const double* T, int n_conf, const double* q, int nq, double cw, double* I) {
+    if (n_conf < 0 || nq < 0) throw std::runtime_error("ausaxs_hip: negative conformation or q count");
+    const int B = n_bodies();
+    info_.total_ms = 0;
+    info_.pairs = 0;
+    for (int c0 = 0; c0 < n_conf; c0 += kMaxBatch) {
+        const int nc = std::min(kMaxBatch, n_conf - c0);
+        const double* Tc = T + (size_t)c0 * B * 12;
+        // bodies whose transform is the same in every conformation of the batch
+        std::vector<int> moving(B, 0);
+        for (int b = 0; b < B; ++b)
+            for (int c = 1; c < nc && !moving[b]; ++c)
+                moving[b] = std::memcmp(Tc + ((size_t)c * B + b) * 12, Tc + (size_t)b * 12, 12 * sizeof(double)) != 0;
+        const int nbins = bound_bins(Tc, nc);
+        Plan& p = plan_for(nc, moving, nbins, Tc);
+        run_batch(p, Tc, q, nq, cw, I + (size_t)c0 * nq);
+        info_.tile_j = p.tile_j;
+        info_.grid = p.grid;
+        info_.n_items = (int)p.items.size();
+        info_.batch = nc;
+        info_.pairs += p.pairs;
+        last_ = &p;
+        last_conf0_ = c0;
+        last_n_conf_ = nc;
+    }
+}
+
+std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
+    if (!last_ || c < last_conf0_ || c >= last_conf0_ + last_n_conf_)
+        throw std::runtime_error("ausaxs_hip: conformation " + std::to_string(c) +
+                                 " is not in the last batch of the last scan");
+    Plan& p = *last_;
+    Context& ctx = Context::get();
+    const int stride = p.nbins + 1;
+    const int lc = c - last_conf0_;
+    std::vector<uint64_t> cnt((size_t)2 * R_ * stride);
+    std::vector<int64_t> dsum((size_t)2 * R_ * stride);
+    const size_t rows_bytes = (size_t)R_ * stride;
+    for (int part = 0; part < 2; ++part) {
+        const size_t row0 = part == 0 ? (size_t)lc * R_ : (size_t)p.n_conf * R_;
+        check(hipMemcpyAsync(cnt.data() + part * rows_bytes, p.cnt.p + row0 * stride, rows_bytes * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, ctx.stream),
+              "D2H");
+        check(hipMemcpyAsync(dsum.data() + part * rows_bytes, p.dsum.p + row0 * stride,
+                             rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream),
+              "D2H");
+    }
+    ctx.sync();
+    const int nb = p.nbins;
+    std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
+    for (int b = 0; b < std::min(nb, base_nb_); ++b) {
+        aa[b] = base_aa_[b];
+        ww[b] = base_ww_[b];
+    }
+    for (int part = 0; part < 2; ++part)
+        for (int r = 0; r < R_; ++r)
+            for (int b = 0; b < nb; ++b) {
+                const size_t cell = (size_t)(part * R_ + r) * stride + b;
+                const uint64_t n = cnt[cell];
+                if (!n) continue;
+                WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
+                e.value = 2.0 * (double)n * row_w_[r];
+                e.count = (uint32_t)(2 * n);
+                e.bin_center = 2.0 * (((double)n * b + (double)dsum[cell] * kDeltaScale) / (double)inv_w_);
+                entry_add(row_kind_[r] == 0 ? aa[b] : aw[b], e);
+            }
+    std::vector<WeightedEntry> ptot(nb);
+    for (int b = 0; b < nb; ++b) {
+        ptot[b] = aa[b];
+        entry_add(ptot[b], aw[b]);
+        entry_add(ptot[b], ww[b]);
+    }
+    auto res = std::make_unique<CompositeDistanceHistogramHIP>();
+    res->mode = Mode::Simple;
+    res->bin_width = s_.bin_width;
+    res->weighted_bins = s_.weighted_bins;
+    res->debye_terms = s_.debye_terms;
+    const unsigned mb = std::min<unsigned>(max_bin_of(ptot), (unsigned)nb);
+    res->max_bin = mb;
+    res->aa.assign(aa.begin(), aa.begin() + mb);
+    res->aw.assign(aw.begin(), aw.begin() + mb);
+    res->ww.assign(ww.begin(), ww.begin() + mb);
+    res->p_tot.assign(ptot.begin(), ptot.begin() + mb);
+    res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins);
+    return res;
+}
+
+}  // namespace ausaxs::gpu

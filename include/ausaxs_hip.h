@@ -116,6 +116,46 @@ int ausaxs_histogram_debye(ausaxs_histogram* h, const double* q, int nq, double*
 
 int ausaxs_histogram_free(ausaxs_histogram* h);
 
+/* ---- batched rigid-body scan (PartialHistogramManagerMT) ------------------- */
+
+typedef struct ausaxs_rigid ausaxs_rigid;
+
+/* Bodies for PartialHistogramManagerMT (PartialHistogramManagerMT.cpp:42-341):
+ * xyz: FP64 coordinates (n x 3, bodies concatenated in order), w: XYZW weight
+ * per atom (CompactCoordinatesXYZW.h:106-165), body_sizes[n_bodies]. Waters
+ * (optional, FP64 m x 3 + weights) stay fixed. The self-correlation of every
+ * body and of the waters is computed here, once, from these coordinates
+ * (initialize / calc_self_correlation / calc_ww, :259-294), as the reference
+ * caches it while bodies only move rigidly. settings->tile_j = 0 selects the
+ * autotuner (tile width and persistent grid timed on the first batch). */
+int ausaxs_rigid_create(const double* xyz, const float* w, const int32_t* body_sizes, int n_bodies,
+                        const double* waters_xyz, const float* waters_w, int n_waters, const ausaxs_settings* settings,
+                        ausaxs_rigid** out);
+
+/* Score n_conf conformations. transforms: [n_conf][n_bodies][12] = {R00..R22
+ * row-major, t0, t1, t2}; each body's atoms become float(R x + t) (FP64,
+ * no contraction), as a CompactCoordinates rebuild of moved bodies holds
+ * (update_compact_representation_body, :165-169). Per conformation this is
+ * PartialHistogramManagerMT::calculate_all after every moved body's inter-body
+ * and water blocks were recomputed (calc_aa, calc_aw; :96-108), followed by
+ * CompositeDistanceHistogram::debye_transform on q with water scaling cw
+ * (CompositeDistanceHistogram.cpp:55-70, DistanceHistogram.cpp:54-87).
+ * Bodies whose transform is the same in every conformation keep their
+ * mutual blocks from one evaluation. I: [n_conf][nq]. */
+int ausaxs_rigid_scan(ausaxs_rigid* r, const double* transforms, int n_conf, const double* q, int nq, double cw,
+                      double* I);
+
+/* The composite histogram (SIMPLE layout) of one conformation of the last
+ * batch of the last scan (batches hold up to 1024 conformations). */
+int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram** out);
+
+/* Launch shape and timing of the last scan: tile width, persistent grid
+ * (0 = one workgroup per item), conformations per graph launch, work items
+ * per launch, pair evaluations, HIP-event time of the graph launches. */
+int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* batch, int* n_items, long long* pairs,
+                      double* scan_ms);
+int ausaxs_rigid_free(ausaxs_rigid* r);
+
 /* The reference's default q sub-axis (ConstantsAxes.h:14, Axis::sub_axis,
  * Axis.cpp:40-59). q must hold 200 doubles; *nq receives the length. */
 int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq);

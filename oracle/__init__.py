@@ -288,3 +288,46 @@ def debye_ffgrid(hist, q, bin_width, long_double=False, **params):
     s_ww = contract(hist["ww"]["value"][None, :], S_aa, long_double)[0]
     prof = profiles(FFGRID, q, sa, sw, s_ww, **params)
     return total(prof), prof
+
+
+# ---------------------------------------------------------------------------
+# rigid bodies (PartialHistogramManagerMT) and the plain composite I(q)
+# ---------------------------------------------------------------------------
+lib.orc_rigid_transform.restype = None
+lib.orc_rigid_transform.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+lib.orc_hist_partial.restype = C.c_int
+lib.orc_hist_partial.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
+                                 C.c_int, C.c_int] + [C.c_void_p] * 4
+
+
+def rigid_transform(xyz, w, T):
+    """FP64 rigid move then float cast (n x 3 doubles, n weights, T[12]) -> n x 4 XYZW floats."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    w = np.ascontiguousarray(w, np.float32)
+    T = np.ascontiguousarray(T, np.float64)
+    out = np.zeros((len(xyz), 4), np.float32)
+    lib.orc_rigid_transform(_p(xyz), _p(w), len(xyz), _p(T), _p(out))
+    return out
+
+
+def hist_partial(self_bodies, bodies, sizes, waters=None, bin_width=0.25, bin_count=8000, nthreads=1):
+    """PartialHistogramManagerMT::calculate_all after every body moved (see orc_hist_partial)."""
+    self_bodies = np.ascontiguousarray(self_bodies, np.float32)
+    bodies = np.ascontiguousarray(bodies, np.float32)
+    sizes = np.ascontiguousarray(sizes, np.int32)
+    waters = np.ascontiguousarray(waters if waters is not None else np.zeros((0, 4), np.float32), np.float32)
+    out = [np.zeros(bin_count, ENTRY) for _ in range(4)]
+    mb = lib.orc_hist_partial(_p(self_bodies), _p(bodies), _p(sizes), len(sizes), _p(waters), len(waters),
+                              C.c_float(inv_width(bin_width)), bin_count, nthreads, *map(_p, out))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
+
+
+def debye_simple(hist, q, bin_width, cw=1.0):
+    """CompositeDistanceHistogram::debye_transform (CompositeDistanceHistogram.cpp:55-70 with
+    DistanceHistogram.cpp:54-87): p = aa + cw*aw + cw^2*ww on p_tot's weighted axis, times exp(-q^2)."""
+    d = weighted_axis(hist["p_tot"], bin_width)
+    p = hist["aa"]["value"] + cw * hist["aw"]["value"] + cw * cw * hist["ww"]["value"]
+    q = np.asarray(q, np.float64)
+    return contract(p[None, :], sinc_table(q, d))[0] * np.exp(-q * q)

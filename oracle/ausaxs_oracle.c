@@ -746,6 +746,53 @@ int orc_hist_simple(const float* atoms, int n_a, const float* waters, int n_w, f
     return max_bin_of(ptot, nb);
 }
 
+void orc_rigid_transform(const double* xyz, const float* w, int n, const double* T, float* out) {
+    for (int i = 0; i < n; ++i) {
+        const double x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+        for (int k = 0; k < 3; ++k) {
+            const double* r = T + 3 * k;
+            const double v = ((r[0] * x + r[1] * y) + r[2] * z) + T[9 + k];
+            out[4 * (size_t)i + k] = (float)v;
+        }
+        out[4 * (size_t)i + 3] = w[i];
+    }
+}
+
+int orc_hist_partial(const float* self_bodies, const float* bodies, const int* sizes, int n_bodies,
+                     const float* waters, int n_w, float inv_w, int nb, int nthreads,
+                     orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot) {
+    if (nthreads < 1) nthreads = 1;
+    size_t* off = (size_t*)malloc(sizeof(size_t) * (n_bodies + 1));
+    off[0] = 0;
+    for (int b = 0; b < n_bodies; ++b) off[b + 1] = off[b] + (size_t)sizes[b];
+    orc_entry* tmp = (orc_entry*)malloc(sizeof(orc_entry) * (size_t)nb);
+    memset(aa, 0, sizeof(orc_entry) * (size_t)nb);
+    memset(aw, 0, sizeof(orc_entry) * (size_t)nb);
+    int err = 0;
+    for (int i = 0; i < n_bodies; ++i) {
+        /* self-correlation cached from the coordinates it was computed on */
+        err |= run_simple(self_bodies + 4 * off[i], sizes[i], NULL, 0, 1, inv_w, nb, nthreads, tmp);
+        for (int k = 0; k < nb; ++k) entry_add(&aa[k], &tmp[k]);
+        for (int j = 0; j < i; ++j) {
+            err |= run_simple(bodies + 4 * off[j], sizes[j], bodies + 4 * off[i], sizes[i], 0, inv_w, nb, nthreads,
+                              tmp);
+            for (int k = 0; k < nb; ++k) entry_add(&aa[k], &tmp[k]);
+        }
+        err |= run_simple(waters, n_w, bodies + 4 * off[i], sizes[i], 0, inv_w, nb, nthreads, tmp);
+        for (int k = 0; k < nb; ++k) entry_add(&aw[k], &tmp[k]);
+    }
+    err |= run_simple(waters, n_w, NULL, 0, 1, inv_w, nb, nthreads, ww);
+    free(tmp);
+    free(off);
+    if (err) return -2;
+    for (int b = 0; b < nb; ++b) {
+        ptot[b] = aa[b];
+        entry_add(&ptot[b], &aw[b]);
+        entry_add(&ptot[b], &ww[b]);
+    }
+    return max_bin_of(ptot, nb);
+}
+
 /* ------------------------------------------------------------------------ */
 /* weighted axis, sinc table, contraction                                   */
 /* ------------------------------------------------------------------------ */

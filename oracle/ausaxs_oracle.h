@@ -97,6 +97,26 @@ int orc_hist_simple(const float* atoms, int n_a, const float* waters, int n_w,
                     float inv_w, int bin_count, int nthreads,
                     orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
 
+/* Rigid-body transform of FP64 body coordinates (n x 3) to the float points
+ * a CompactCoordinates rebuild holds after the move (Body::rotate/translate
+ * act on FP64 atoms; CompactCoordinatesXYZW.h:106-165 casts each coordinate):
+ *   out_x = (float)(((R00*x + R01*y) + R02*z) + t0), likewise y, z;
+ * T = {R00,R01,R02, R10,R11,R12, R20,R21,R22, t0,t1,t2}; out[4*i+3] = w[i]. */
+void orc_rigid_transform(const double* xyz, const float* w, int n, const double* T, float* out);
+
+/* PartialHistogramManagerMT::calculate_all (PartialHistogramManagerMT.cpp:42-257)
+ * in the state reached after every body moved once the self terms were cached:
+ *   aa = sum_b self(self_bodies[b]) + sum_{i>j} cross(rows body j, cols body i)
+ *        (calc_aa(i, j) -> enqueue_calculate_cross(a[i], a[j]): rows over data_2)
+ *   aw = sum_i cross(rows waters, cols body i)      (calc_aw(i))
+ *   ww = self(waters)                               (calc_ww)
+ *   ptot = aa + aw + ww (the master histogram), max_bin as calculate() (:133-141).
+ * self_bodies/bodies: XYZW points concatenated per body, sizes[n_bodies].
+ * Returns max_bin or a negative status. */
+int orc_hist_partial(const float* self_bodies, const float* bodies, const int* sizes, int n_bodies,
+                     const float* waters, int n_w, float inv_w, int nb, int nthreads,
+                     orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
 /* ---- weighted axis, sinc, Debye ------------------------------------------ */
 /* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
  * followed by DistanceHistogram::initialize (DistanceHistogram.cpp:44-47, d[0]=0). */

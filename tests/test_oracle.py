@@ -322,3 +322,126 @@ def test_threads_do_not_change_counts():
     h4 = O.hist_ff(O.FFEXPLICIT, atoms, None, nthreads=4)
     assert np.array_equal(h1["aa"]["count"], h4["aa"]["count"])
     np.testing.assert_allclose(h1["aa"]["bin_center"], h4["aa"]["bin_center"], rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# rigid bodies: PartialHistogramManagerMT
+# --------------------------------------------------------------------------
+def test_rigid_transform_closed_forms():
+    rng = np.random.default_rng(3)
+    xyz = rng.normal(size=(40, 3)) * 20
+    w = rng.integers(1, 9, 40).astype(np.float32)
+    ident = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0.0])
+    p = O.rigid_transform(xyz, w, ident)
+    assert np.array_equal(p[:, :3], xyz.astype(np.float32)) and np.array_equal(p[:, 3], w)
+    # quarter turn about z plus a translation: (x, y, z) -> (-y + 1.5, x - 2, z + 0.25),
+    # every product exact, one rounding per sum, then the float cast
+    rz = np.array([0, -1, 0, 1, 0, 0, 0, 0, 1, 1.5, -2.0, 0.25])
+    p = O.rigid_transform(xyz, w, rz)
+    expect = np.stack([-xyz[:, 1] + 1.5, xyz[:, 0] - 2.0, xyz[:, 2] + 0.25], 1).astype(np.float32)
+    assert np.array_equal(p[:, :3], expect)
+    # a general rotation keeps distances up to the float casts
+    th = 0.7
+    R = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    p = O.rigid_transform(xyz, w, np.concatenate([R.reshape(9), [3.0, 4.0, 5.0]]))
+    d0 = np.linalg.norm(xyz[:, None] - xyz[None], axis=2)
+    d1 = np.linalg.norm(p[:, None, :3].astype(np.float64) - p[None, :, :3], axis=2)
+    assert np.max(np.abs(d0 - d1)) < 1e-4
+
+
+def _partial_brute(self_bodies, bodies, waters, bw):
+    """Pair-by-pair restatement with the pinned binning rules: self loops rule by
+    (n-i-1) mod 4 geometry, cross loops by column index; add<2> per pair."""
+    nb = 4000
+    out = {k: np.zeros(nb, O.ENTRY) for k in ("aa", "aw", "ww")}
+
+    def add(dst, d, b, w):
+        out[dst]["count"][b] += 2
+        out[dst]["value"][b] += 2.0 * float(np.float32(w))
+        out[dst]["bin_center"][b] += 2.0 * d
+
+    def self_loop(pts, dst):
+        n = len(pts)
+        for i in range(n):
+            for j in range(i + 1, n):
+                d, b = O.bin_pair(pts[i], pts[j], bw, O.lib.orc_rule_self(n, i, j))
+                add(dst, d, b, np.float32(pts[i, 3]) * np.float32(pts[j, 3]))
+        t = float(np.sum((pts[:, 3] * pts[:, 3]).astype(np.float64)))
+        out[dst]["value"][0] += t
+        out[dst]["count"][0] += int(t)
+
+    def cross(rows, cols, dst):
+        m = len(cols)
+        for i in range(len(rows)):
+            for j in range(m):
+                d, b = O.bin_pair(rows[i], cols[j], bw, O.lib.orc_rule_cross(m, j))
+                add(dst, d, b, np.float32(rows[i, 3]) * np.float32(cols[j, 3]))
+
+    for i in range(len(bodies)):
+        self_loop(self_bodies[i], "aa")
+        for j in range(i):
+            cross(bodies[j], bodies[i], "aa")  # calc_aa(i, j): rows over body j
+        if len(waters):
+            cross(waters, bodies[i], "aw")  # calc_aw(i): rows over the waters
+    if len(waters):
+        self_loop(waters, "ww")
+    return out
+
+
+def test_partial_manager_brute_force():
+    rng = np.random.default_rng(17)
+    sizes = [5, 6, 7, 9]
+    T = []
+    self_b, moved = [], []
+    for k, n in enumerate(sizes):
+        xyz = rng.normal(size=(n, 3)) * 4 + k * 6.0
+        w = rng.integers(1, 5, n).astype(np.float32)
+        self_b.append(O.rigid_transform(xyz, w, [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]))
+        c, s = np.cos(0.3 * k), np.sin(0.3 * k)
+        Tk = [c, -s, 0, s, c, 0, 0, 0, 1, 0.5 * k, -0.25, 1.0]
+        moved.append(O.rigid_transform(xyz, w, Tk))
+    waters = np.zeros((6, 4), np.float32)
+    waters[:, :3] = rng.normal(size=(6, 3)) * 8
+    waters[:, 3] = 1.0
+    h = O.hist_partial(np.concatenate(self_b), np.concatenate(moved), sizes, waters, 0.25, 4000)
+    ref = _partial_brute(self_b, moved, waters, 0.25)
+    mb = h["max_bin"]
+    for key in ("aa", "aw", "ww"):
+        assert np.array_equal(h[key]["count"], ref[key]["count"][:mb]), key
+        np.testing.assert_allclose(h[key]["value"], ref[key]["value"][:mb], rtol=1e-12)
+        np.testing.assert_allclose(h[key]["bin_center"], ref[key]["bin_center"][:mb], rtol=1e-6, atol=1e-9)
+    assert not np.any(ref["aa"]["count"][mb:])
+
+
+def test_partial_equals_full_when_unmoved():
+    # partial_histogram_manager.cpp:70-154: the partial manager reproduces the
+    # full manager on the same molecule
+    rng = np.random.default_rng(23)
+    sizes = [37, 52, 61]
+    pts = np.zeros((sum(sizes), 4), np.float32)
+    pts[:, :3] = rng.normal(size=(sum(sizes), 3)) * 12
+    pts[:, 3] = rng.integers(1, 9, sum(sizes))
+    waters = np.zeros((23, 4), np.float32)
+    waters[:, :3] = rng.normal(size=(23, 3)) * 16
+    waters[:, 3] = 1.0
+    hp = O.hist_partial(pts, pts, sizes, waters, 0.25, 8000)
+    hf = O.hist_simple(pts, waters, 0.25, 8000)
+    assert hp["max_bin"] == hf["max_bin"]
+    for key in ("aa", "aw", "ww", "p_tot"):
+        assert np.array_equal(hp[key]["count"], hf[key]["count"]), key
+        np.testing.assert_allclose(hp[key]["value"], hf[key]["value"], rtol=1e-13)
+        # the partial manager's inter-body blocks are cross loops, whose tail
+        # columns use the scalar (FMA) rule where the full self loop used the
+        # SIMD rule: distances agree to float rounding
+        np.testing.assert_allclose(hp[key]["bin_center"], hf[key]["bin_center"], rtol=1e-6, atol=1e-9)
+
+
+def test_debye_simple_golden(golden_carbon):
+    xyz, q, I_ref = golden_carbon
+    a = np.zeros((len(xyz), 4), np.float32)
+    a[:, :3] = xyz.astype(np.float32)
+    a[:, 3] = 1.0
+    h = O.hist_simple(a, None, 0.1, 8000, nthreads=4)
+    I = O.debye_simple(h, q, 0.1) * np.exp(q * q)
+    ok, i = compare_hist_approx(I, I_ref, 1e-6, 0.01)
+    assert ok, (i, I[i], I_ref[i])

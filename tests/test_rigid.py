@@ -1,0 +1,168 @@
+"""Batched rigid-body scan (PartialHistogramManagerMT on many conformations)
+against the oracle's partial manager, through the C ABI.
+
+Bar: counts bit-exact per conformation; values within 1e-12 relative (FP64
+sums of exact products); bin centres within the fixed-point bound; I(q)
+within 1e-5 relative of the FP64 oracle (stated bound 1e-4).
+"""
+import numpy as np
+import pytest
+
+import ausaxs_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+I0 = {3: 6.0, 7: 7.0, 11: 8.0, 12: 16.0}  # C, N, O, S electron counts as XYZW weights
+
+
+def make_bodies(sizes, seed, spacing=26.0):
+    """Globules of the given sizes (radius ~ N^(1/3)) on a line, FP64, with I0 weights."""
+    bodies, weights = [], []
+    for k, n in enumerate(sizes):
+        r = 20.0 * (n / 2000.0) ** (1 / 3)
+        g = A.synth_globule(seed + k, n, (r, r, r), 2.0)
+        xyz = g[:, :3].astype(np.float64) + np.array([k * spacing, 0.0, 0.0])
+        bodies.append(xyz)
+        weights.append(np.array([I0[int(t)] for t in g[:, 3].view(np.int32)], np.float32))
+    return bodies, weights
+
+
+def random_transforms(rng, bodies, n_conf, static=()):
+    """Rotation about each body's centre + translation in [-10, 10]^3; bodies
+    in `static` keep the identity in every conformation."""
+    T = np.zeros((n_conf, len(bodies), 12))
+    for c in range(n_conf):
+        for b, xyz in enumerate(bodies):
+            if b in static:
+                T[c, b] = A.rigid_transform(np.eye(3), np.zeros(3))
+                continue
+            R = A.random_rotation(rng)
+            cen = xyz.mean(0)
+            t = cen - R @ cen + rng.uniform(-10, 10, 3)
+            T[c, b] = A.rigid_transform(R, t)
+    return T
+
+
+def oracle_conf(bodies, weights, T, waters=None, bw=0.25):
+    self_pts = np.concatenate([O.rigid_transform(b, w, A.rigid_transform(np.eye(3), np.zeros(3)))
+                               for b, w in zip(bodies, weights)])
+    moved = np.concatenate([O.rigid_transform(b, w, T[k]) for k, (b, w) in enumerate(zip(bodies, weights))])
+    sizes = [len(b) for b in bodies]
+    return O.hist_partial(self_pts, moved, sizes, waters, bw, 8000, nthreads=8)
+
+
+def check_hist(h, ref, bw=0.25):
+    assert h.max_bin == ref["max_bin"]
+    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww"), (A.PTOT, "p_tot")):
+        v, c, b = h.get(which)
+        assert np.array_equal(c, ref[key]["count"]), (key, np.argwhere(c != ref[key]["count"])[:5])
+        np.testing.assert_allclose(v, ref[key]["value"], rtol=1e-12, err_msg=key)
+        m = ref[key]["count"] > 0
+        d, dr = b[m], ref[key]["bin_center"][m]
+        assert np.all(np.abs(d - dr) <= 1e-5 * np.abs(dr) + 1e-6), key
+    d = h.d_axis()
+    dr = O.weighted_axis(ref["p_tot"], bw)
+    assert np.all(np.abs(d - dr) <= bw * 2.0 ** -21 + 2.0 ** -22 * np.abs(dr))
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("static", [(), (0,)])
+def test_rigid_scan_parity(static):
+    # sizes with every residue mod 4: tail strips on every block
+    bodies, weights = make_bodies([301, 498, 703], seed=11)
+    rng = np.random.default_rng(5)
+    wx = rng.normal(size=(150, 3)) * 15 + np.array([26.0, 0, 0])
+    waters = np.zeros((150, 4), np.float32)
+    waters[:, :3] = wx.astype(np.float32)
+    waters[:, 3] = 1.0
+    scan = A.RigidScan(bodies, weights, wx, np.ones(150, np.float32))
+    T = random_transforms(rng, bodies, 5, static)
+    q = np.linspace(0.01, 0.5, 60)
+    I = scan.scan(T, q)
+    for c in range(len(T)):
+        ref = oracle_conf(bodies, weights, T[c], waters)
+        check_hist(scan.histogram(c), ref)
+        I_ref = O.debye_simple(ref, q, 0.25)
+        err = relerr(I[c], I_ref)
+        print(f"static {static} conformation {c}: max_bin {ref['max_bin']}, I(q) rel err {err:.2e}")
+        assert err <= 1e-5
+
+
+def test_rigid_scan_single_and_batch_agree():
+    # a conformation scored alone (every block computed once) and inside a batch
+    bodies, weights = make_bodies([250, 333, 417, 120], seed=21)
+    rng = np.random.default_rng(8)
+    T = random_transforms(rng, bodies, 6)
+    q = np.linspace(0.0, 0.6, 40)
+    scan = A.RigidScan(bodies, weights)
+    I_batch = scan.scan(T, q)
+    for c in (0, 3, 5):
+        I_one = scan.scan(T[c:c + 1], q)
+        assert np.array_equal(I_one[0], I_batch[c])
+        ref = oracle_conf(bodies, weights, T[c])
+        check_hist(scan.histogram(0), ref)
+
+
+def test_rigid_graph_replay():
+    # replays of the captured graph with new transforms between them
+    bodies, weights = make_bodies([200, 260, 310], seed=31)
+    rng = np.random.default_rng(9)
+    T1 = random_transforms(rng, bodies, 4)
+    T2 = random_transforms(rng, bodies, 4)
+    q = np.linspace(0.01, 0.4, 30)
+    scan = A.RigidScan(bodies, weights)
+    a1 = scan.scan(T1, q)
+    b1 = scan.scan(T2, q)
+    a2 = scan.scan(T1, q)
+    b2 = scan.scan(T2, q)
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+    assert not np.array_equal(a1, b1)
+    for c in range(4):
+        ref = oracle_conf(bodies, weights, T2[c])
+        assert relerr(b2[c], O.debye_simple(ref, q, 0.25)) <= 1e-5
+
+
+def test_rigid_autotune_matches_fixed_tiles():
+    bodies, weights = make_bodies([400, 600], seed=41)
+    rng = np.random.default_rng(10)
+    T = random_transforms(rng, bodies, 8)
+    q = np.linspace(0.01, 0.5, 50)
+    fixed = A.RigidScan(bodies, weights).scan(T, q)
+    tuned = A.RigidScan(bodies, weights, autotune=True)
+    I = tuned.scan(T, q)
+    info = tuned.info()
+    print("autotuned:", info)
+    assert info["tile_j"] in (1024, 2048, 4096, 8000)
+    assert np.array_equal(I, fixed)
+
+
+def test_rigid_overflow_is_an_error():
+    bodies, weights = make_bodies([100, 100], seed=51)
+    T = random_transforms(np.random.default_rng(1), bodies, 2)
+    with pytest.raises(A.AusaxsError):
+        A.RigidScan(bodies, weights, settings=A.default_settings(bin_count=20)).scan(T, [0.1])
+
+
+def test_rigid_large_invariants():
+    # the benchmark's bodies (3k/5k/8k/12k atoms) on a few conformations:
+    # pair-count identities per conformation and agreement with single scans
+    bodies, weights = make_bodies([3000, 5000, 8000, 12000], seed=4, spacing=45.0)
+    rng = np.random.default_rng(4)
+    T = random_transforms(rng, bodies, 3)
+    scan = A.RigidScan(bodies, weights)
+    q = A.default_q_axis()
+    I = scan.scan(T, q)
+    n = sum(len(b) for b in bodies)
+    sizes = [len(b) for b in bodies]
+    for c in range(3):
+        h = scan.histogram(c)
+        cnt = h.get(A.AA)[1].sum(dtype=np.int64)
+        # every ordered pair once plus the per-body (unsigned) sum w^2 at bin 0
+        self_terms = sum(int(np.sum(w.astype(np.float64) ** 2)) for w in weights)
+        assert cnt == n * (n - 1) + self_terms
+        assert np.all(np.isfinite(I[c]))
+    assert sizes == [3000, 5000, 8000, 12000]
