@@ -81,8 +81,8 @@ def _load():
         "ausaxs_rigid_create": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, C.c_int, vp, P(vp)]),
         "ausaxs_rigid_scan": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]),
         "ausaxs_rigid_histogram": (C.c_int, [vp, C.c_int, P(vp)]),
-        "ausaxs_rigid_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_longlong),
-                                        P(C.c_double)]),
+        "ausaxs_rigid_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int),
+                                        P(C.c_longlong), P(C.c_double)]),
         "ausaxs_rigid_free": (C.c_int, [vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_synth_globule": (
@@ -391,11 +391,11 @@ class RigidScan:
         return CompositeHistogram(h.value)
 
     def info(self):
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         pairs, ms = C.c_longlong(), C.c_double()
         _check(lib.ausaxs_rigid_info(self._r, *map(C.byref, v), C.byref(pairs), C.byref(ms)))
-        return dict(tile_j=v[0].value, grid=v[1].value, batch=v[2].value, n_items=v[3].value, pairs=pairs.value,
-                    scan_ms=ms.value)
+        return dict(tile_j=v[0].value, grid=v[1].value, copies=v[2].value, batch=v[3].value, n_items=v[4].value,
+                    pairs=pairs.value, scan_ms=ms.value)
 
 
 def rigid_transform(R, t):

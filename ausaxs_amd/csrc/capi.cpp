@@ -399,12 +399,13 @@ int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram**
     });
 }
 
-int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* batch, int* n_items, long long* pairs,
-                      double* scan_ms) {
+int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* copies, int* batch, int* n_items,
+                      long long* pairs, double* scan_ms) {
     return guarded([&] {
         require(r && r->r, "bad argument");
         const RigidInfo& i = r->r->info();
         if (tile_j) *tile_j = i.tile_j;
+        if (copies) *copies = i.copies;
         if (grid) *grid = i.grid;
         if (batch) *batch = i.batch;
         if (n_items) *n_items = i.n_items;

@@ -57,7 +57,15 @@ struct PairLaunch {
     int nbins;                // LDS bins per row; slot nbins collects overflow
     int clamp;                // 0 only if every pair's bin is known to be < nbins
     int grid;                 // workgroups; 0 or >= n_items: one per item, else persistent (strided)
+    int rows = 2;             // LDS histogram rows: 2 if any item has kOrient, else 1
+    int copies = 1;           // LDS histogram copies (1, 2, 4 or 8), lane l adds into copy l mod copies
 };
+
+// LDS bytes of one pair_hist_kernel workgroup (static column tile included)
+constexpr size_t kStaticLds = (256 + 4) * 16;
+constexpr size_t pair_lds_bytes(int nbins, int rows, int copies) {
+    return (size_t)copies * rows * (nbins + 2) * sizeof(uint64_t) + kStaticLds;
+}
 
 hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s);
 

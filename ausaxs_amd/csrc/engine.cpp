@@ -55,10 +55,22 @@ void check_tile_j(int tile_j) {
         throw std::runtime_error("ausaxs_hip: tile_j must keep a work item below 2^22 pairs (tile_j <= 8191)");
 }
 
+int item_rows(const std::vector<PairWork>& items) {
+    for (const PairWork& w : items)
+        if (w.flags & kOrient) return 2;
+    return 1;
+}
+
+int pick_copies(int nbins, int rows) {
+    int k = 8;
+    while (k > 1 && pair_lds_bytes(nbins, rows, k) > 40960) k /= 2;
+    return k;
+}
+
 size_t check_pair_lds(int nbins) {
     if (nbins < 1) throw std::runtime_error("ausaxs_hip: nbins must be positive");
-    // dynamic histogram rows + the static column tile (260 float4) of pair_hist_kernel
-    const size_t lds = (size_t)2 * (nbins + 2) * sizeof(uint64_t) + 260 * sizeof(float4);
+    // two histogram rows + the static column tile of pair_hist_kernel
+    const size_t lds = pair_lds_bytes(nbins, 2, 1);
     Context& ctx = Context::get();
     if (lds > ctx.prop.sharedMemPerBlock)
         throw std::runtime_error("ausaxs_hip: " + std::to_string(nbins) + " bins need " + std::to_string(lds) +
@@ -86,6 +98,22 @@ SetLayout layout_set(const PointSet& ps, std::vector<float4>& buf, std::vector<i
         else tail[c].push_back(i);
     }
     for (int c = 0; c < C; ++c) {
+        // Scatter the body part: consecutive points of a structure are spatial
+        // neighbours (chains), so a row tile in input order is a compact blob
+        // whose distances to one column point crowd into few bins, and the
+        // lanes of a wave collide on the same LDS cells. A golden-ratio stride
+        // through the segment spreads every tile over the whole structure.
+        // Order within a segment is free: orientation follows the tags and
+        // diagonal blocks mask by buffer position.
+        std::vector<int>& v = body[c];
+        const size_t nv = v.size();
+        if (nv > 2) {
+            size_t st = (size_t)(0.6180339887498949 * (double)nv);
+            while (std::gcd(st, nv) != 1) ++st;
+            std::vector<int> s(nv);
+            for (size_t k = 0; k < nv; ++k) s[k] = v[(k * st) % nv];
+            v.swap(s);
+        }
         L.seg_start[c] = (int)buf.size();
         for (auto* v : {&body[c], &mid[c], &tail[c]})
             for (int i : *v) {
@@ -238,6 +266,8 @@ void PairEngine::build(int tile_j) {
     for (const PairJob& job : jobs_) append_job(wl_, job, sets_, layout_, n_rows_, tile_j);
     sort_items(wl_.items);
     mp_ = plan_merge(wl_.items, n_rows_);
+    rows_ = item_rows(wl_.items);
+    copies_ = pick_copies(nbins_, rows_);
 
     // ---- device residency -------------------------------------------------
     hipStream_t s = ctx.stream;
@@ -264,7 +294,8 @@ void PairEngine::enqueue(bool timed) {
     Context& ctx = Context::get();
     hipStream_t s = ctx.stream;
     if (timed) check(hipEventRecord(ev_[0], s), "hipEventRecord");
-    PairLaunch pl{d_pts_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_};
+    PairLaunch pl{d_pts_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_,
+                  rows_, copies_};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     if (timed) check(hipEventRecord(ev_[1], s), "hipEventRecord");
     MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),

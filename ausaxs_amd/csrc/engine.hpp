@@ -146,9 +146,14 @@ MergePlan plan_merge(std::vector<PairWork>& items, int n_rows);
 // throws unless a work item of 512 rows x tile_j columns stays below the
 // fixed-point capacity of one LDS cell
 void check_tile_j(int tile_j);
-// LDS bytes per workgroup of pair_hist_kernel for nbins bins; throws if the
-// device offers less.
+// LDS bytes per workgroup of pair_hist_kernel for nbins bins and 2 rows;
+// throws if the device offers less.
 size_t check_pair_lds(int nbins);
+// LDS histogram copies for a launch: the most (up to 8) that keep a
+// workgroup within 40 KB of LDS, so at least four fit on one CU
+int pick_copies(int nbins, int rows);
+// 2 if any item has two output rows (orientation), else 1
+int item_rows(const std::vector<PairWork>& items);
 
 struct EngineTiming {
     float pair_ms = 0;   // pair_hist_kernel on the engine stream (HIP events)
@@ -184,6 +189,7 @@ class PairEngine {
     // workgroups of the pair kernel: 0 = one per item, else a persistent grid
     void set_grid(int g) { grid_ = g; }
     bool clamped() const { return clamp_; }
+    int copies() const { return copies_; }
 
   private:
     std::vector<PointSet> sets_;
@@ -195,6 +201,7 @@ class PairEngine {
     float inv_w_;
     bool clamp_ = true;
     int grid_ = 0;
+    int rows_ = 2, copies_ = 1;
 
     DevBuf<float4> d_pts_;
     std::vector<DevBuf<float4>> d_orig_;

@@ -166,11 +166,18 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
 
+    // the next column tile is loaded into a register while the current one is
+    // evaluated, so the global-memory latency of the tile loads stays hidden
+    // when the point buffer does not fit the caches (large batches)
+    // (an unconditional load of a clamped, always valid index: a conditional
+    // one would be merged through a register copy that waits for it at once)
+    float4 pre = pts[min(w.j_begin + t, w.j_end - 1)];
     for (int jc = w.j_begin; jc < w.j_end; jc += kColTile) {
         const int nc = min(kColTile, w.j_end - jc);
         __syncthreads();  // previous tile consumed
-        if (t < nc) cols[t] = pts[jc + t];
+        if (t < nc) cols[t] = pre;
         __syncthreads();
+        pre = pts[min(jc + kColTile + t, w.j_end - 1)];  // past the item's end: unused
         const int n2 = nc & ~(2 * kColGroup - 1);
         float4 a[kColGroup], bq[kColGroup];
 #pragma unroll
@@ -216,28 +223,37 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 // strides over the item list (grid < items): both visit every item once.
 // clamp = 0 is allowed only when the host has bounded every pair's bin below
 // nbins (bounding-box diagonal); otherwise bins are clamped into slot nbins.
+// The LDS holds `copies` histograms of `rows` rows each; lane l of a wave adds
+// into copy l mod copies, so lanes whose pairs fall into the same bin collide
+// on one LDS cell `copies` times less often (pairs of two bodies crowd into few
+// bins). The copies are summed at the flush.
 __global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const PairWork* __restrict__ items, int n_items,
                                                              uint64_t* __restrict__ partial, float inv_w,
-                                                             int nbins, int clamp) {
+                                                             int nbins, int clamp, int rows, int copies) {
     extern __shared__ uint64_t hist[];
     __shared__ float4 cols[kColTile + kColGroup];
     const int stride = nbins + 2;
+    const int copy_stride = rows * stride;
     const int out_stride = nbins + 1;
+    uint64_t* mine = hist + (threadIdx.x & (copies - 1)) * copy_stride;
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const PairWork w = items[it];
         const bool orient = (w.flags & kOrient) != 0;
-        const int cells = (orient ? 2 : 1) * stride;
         __syncthreads();  // previous item's flush done
-        for (int e = threadIdx.x; e < cells; e += kThreads) hist[e] = 0;
+        for (int e = threadIdx.x; e < copies * copy_stride; e += kThreads) hist[e] = 0;
         __syncthreads();
-        if (clamp) run_any<true>(pts, w, hist, cols, inv_w, nbins);
-        else run_any<false>(pts, w, hist, cols, inv_w, nbins);
+        if (clamp) run_any<true>(pts, w, mine, cols, inv_w, nbins);
+        else run_any<false>(pts, w, mine, cols, inv_w, nbins);
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
         uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
         for (int rr = 0; rr < (orient ? 2 : 1); ++rr)
-            for (int e = threadIdx.x; e < out_stride; e += kThreads) out[rr * out_stride + e] = hist[rr * stride + e];
+            for (int e = threadIdx.x; e < out_stride; e += kThreads) {
+                uint64_t v = hist[rr * stride + e];
+                for (int c = 1; c < copies; ++c) v += hist[c * copy_stride + rr * stride + e];
+                out[rr * out_stride + e] = v;
+            }
     }
 }
 
@@ -370,10 +386,12 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
 
 hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
     if (p.n_items == 0) return hipSuccess;
-    const size_t lds = (size_t)2 * (p.nbins + 2) * sizeof(uint64_t);
+    if ((p.rows != 1 && p.rows != 2) || p.copies < 1 || p.copies > 8 || (p.copies & (p.copies - 1)))
+        return hipErrorInvalidValue;
+    const size_t lds = pair_lds_bytes(p.nbins, p.rows, p.copies) - kStaticLds;
     const int grid = p.grid > 0 ? std::min(p.grid, p.n_items) : p.n_items;
     hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.items, p.n_items, p.partial,
-                       p.inv_w, p.nbins, p.clamp);
+                       p.inv_w, p.nbins, p.clamp, p.rows, p.copies);
     return hipGetLastError();
 }
 

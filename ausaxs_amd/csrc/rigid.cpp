@@ -30,6 +30,7 @@ void sphere_of(const double* p, int n, int stride, double c[3], double& r) {
 
 struct RigidScan::Plan {
     int n_conf = 0, nbins = 0, tile_j = 0, grid = 0;
+    int rows = 1, copies = 1;  // LDS histogram rows and copies of the pair kernel
     std::vector<int> moving;
     int64_t S = 0, M = 0;  // shared points, points per replica
     int n_slots = 0;
@@ -368,6 +369,8 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
             p.items.push_back(w);
         }
     p.mp = plan_merge(p.items, n_rows);
+    p.rows = item_rows(p.items);
+    p.copies = pick_copies(nbins, p.rows);
 
     // device residency
     Context& ctx = Context::get();
@@ -470,7 +473,8 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     TransformLaunch tl{d_xyz_.p, p.T.p, p.slot_src.p, p.slot_body.p, p.slot_dst.p, p.slot_dsto.p, p.slot_tag.p,
                        p.slot_cls.p, p.body_moving.p, p.n_slots, B, p.n_conf, p.M, p.pts.p, p.orig.p};
     check(launch_transform(tl, s), "transform_kernel");
-    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid};
+    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
+                  p.copies};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     MergeLaunch ml{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * R_,
                    p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.cnt.p, p.dsum.p};
@@ -491,9 +495,9 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     }
 }
 
-float RigidScan::time_pair(Plan& p, int grid) {
+float RigidScan::time_pair(Plan& p, int grid, int copies) {
     Context& ctx = Context::get();
-    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid};
+    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid, p.rows, copies};
     check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");  // warm-up
     check(hipEventRecord(p.ev[0], ctx.stream), "hipEventRecord");
     check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");
@@ -527,16 +531,26 @@ RigidScan::Plan& RigidScan::plan_for(int n_conf, const std::vector<int>& moving,
                                p->slot_tag.p, p->slot_cls.p, p->body_moving.p, p->n_slots, n_bodies(), n_conf,
                                p->M, p->pts.p, p->orig.p};
             check(launch_transform(tl, ctx.stream), "transform_kernel");
-            const float ms = time_pair(*p, 0);
+            const float ms = time_pair(*p, 0, p->copies);
             ++cand;
             if (ms < best_ms) {
                 best_ms = ms;
                 best = std::move(p);
             }
         }
+        // LDS histogram copies (up to 64 KB of LDS per workgroup), then the grid
+        for (int k : {1, 2, 4, 8}) {
+            if (k == best->copies || pair_lds_bytes(nbins, best->rows, k) > 65536) continue;
+            const float ms = time_pair(*best, 0, k);
+            ++cand;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best->copies = k;
+            }
+        }
         for (int g : {2 * cus, 4 * cus, 8 * cus}) {
             if (g >= (int)best->items.size()) continue;
-            const float ms = time_pair(*best, g);
+            const float ms = time_pair(*best, g, best->copies);
             ++cand;
             if (ms < best_ms) {
                 best_ms = ms;
@@ -615,6 +629,7 @@ void RigidScan::scan(const double* T, int n_conf, const double* q, int nq, doubl
         run_batch(p, Tc, q, nq, cw, I + (size_t)c0 * nq);
         info_.tile_j = p.tile_j;
         info_.grid = p.grid;
+        info_.copies = p.copies;
         info_.n_items = (int)p.items.size();
         info_.batch = nc;
         info_.pairs += p.pairs;

@@ -31,6 +31,7 @@ struct RigidInfo {
     int n_items = 0;           // work items of one batch
     int batch = 0;             // conformations per graph launch
     int tuned_candidates = 0;
+    int copies = 1;            // LDS histogram copies of the pair kernel
 };
 
 class RigidScan {
@@ -85,7 +86,7 @@ class RigidScan {
     int bound_bins(const double* T, int n_conf) const;
     void run_batch(Plan& p, const double* T, const double* q, int nq, double cw, double* I);
     void enqueue(Plan& p, int nq, double cw, bool copies);
-    float time_pair(Plan& p, int grid);
+    float time_pair(Plan& p, int grid, int copies);
 };
 
 }  // namespace ausaxs::gpu

@@ -1,19 +1,22 @@
 """Benchmark of the MI355X Debye hot path (one GPU).
 
-Workload (default): 250,000-atom ellipsoid (semi-axes 120 x 90 x 70 A, C/N/O/S
-62/17/20/1 %) with 60,000 waters 3 A outside its surface, FF-explicit
-histograms with 0.1 A weighted bins, and I(q) on 1,000 q points up to
-1.5 1/A. Synthetic, deterministic inputs.
+Default workload (--config 3): 250,000-atom ellipsoid (semi-axes 120 x 90 x
+70 A, C/N/O/S 62/17/20/1 %) with 60,000 waters 3 A outside its surface,
+FF-explicit histograms with 0.1 A weighted bins, and I(q) on 1,000 q points
+up to 1.5 1/A. Synthetic, deterministic inputs. One step =
+HistogramManagerMTFFExplicit::calculate_all on coordinates resident in HBM
+(pair kernels, merge, tail strips, histogram read-back and bookkeeping) + the
+Debye transform (sinc table, bf16-split MFMA GEMM, intensity profiles).
 
-One step = HistogramManagerMTFFExplicit::calculate_all on coordinates
-resident in HBM (pair kernels, merge, tail strips, histogram read-back and
-bookkeeping) + the Debye transform (sinc table, bf16-split MFMA GEMM,
-intensity profiles) -> one I(q). The timed region brackets exactly --steps
-steps with device synchronisation on both sides.
+Other workloads: --config 1 (2k-atom globule, FF-avg), --config 2 (20k-atom
+globule + 6k waters + 1 A excluded-volume grid, FF-grid), --config 4 (rigid-
+body scan: bodies of 3k/5k/8k/12k atoms, 512 conformations per step in one
+graph launch).
 
-Prints one JSON line (metric from the run's headline: pair evaluations per
-second), with per-step latency percentiles, the pair kernel's roofline
-against the FP32 VALU peak and the CPU oracle (a port of the reference's
+The timed region brackets exactly --steps steps with device synchronisation
+on both sides. Prints one JSON line: pair evaluations per second, profiles
+per second, per-step latency percentiles, the pair kernel's roofline against
+the FP32 VALU peak, and the CPU oracle (a restatement of the reference's
 multithreaded managers) timed on this host's cores.
 """
 from __future__ import annotations
@@ -32,25 +35,32 @@ sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (AMD published)
 FLOP_PER_PAIR = 11  # 3 sub + 3 mul + 2 add + sqrt + mul(inv_w) + add(sum d)
+METRIC = "Gpair-evals/s + I(q) profiles/s, p50/p99 latency; GPU vs CPU at ≤1e-4 rel err"
 
 CONFIGS = {
-    1: dict(name="2k-atom globule, FF-avg, 0.25 A bins, 100 q in 0.01-0.5", n=2000, axes=(20, 20, 20),
-            r_excl=2.0, waters=0, bw=0.25, mode="ffavg", q=(0.01, 0.5, 100)),
-    3: dict(name="250k-atom ellipsoid + 60k waters, FF-explicit, 0.1 A bins, 1000 q to 1.5", n=250000,
-            axes=(120, 90, 70), r_excl=1.8, waters=60000, bw=0.1, mode="ffexplicit", q=(0.0015, 1.5, 1000)),
+    1: dict(kind="structure", name="2k-atom globule, FF-avg, 0.25 A bins, 100 q in 0.01-0.5", n=2000,
+            axes=(20, 20, 20), r_excl=2.0, waters=0, bw=0.25, mode="ffavg", q=(0.01, 0.5, 100)),
+    2: dict(kind="grid", name="20k-atom globule + 6k waters, FF-grid (1 A excluded-volume grid), 0.25 A bins, "
+            "200 q in 0-0.5", n=20000, axes=(45, 45, 45), r_excl=2.0, waters=6000, bw=0.25, q=(0.0, 0.5, 200)),
+    3: dict(kind="structure", name="250k-atom ellipsoid + 60k waters, FF-explicit, 0.1 A bins, 1000 q to 1.5",
+            n=250000, axes=(120, 90, 70), r_excl=1.8, waters=60000, bw=0.1, mode="ffexplicit",
+            q=(0.0015, 1.5, 1000)),
+    4: dict(kind="rigid", name="rigid-body scan: bodies of 3k/5k/8k/12k atoms, 512 conformations per step, "
+            "XYZW weights, 0.25 A bins, default q axis", sizes=(3000, 5000, 8000, 12000), n_conf=512, bw=0.25),
 }
+I0 = {3: 6.0, 7: 7.0, 11: 8.0, 12: 16.0}  # XYZW weights: electron counts of C, N, O, S
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(cfg, seed):
-    import ausaxs_amd as A
-
-    atoms = A.synth_globule(seed, cfg["n"], cfg["axes"], cfg["r_excl"])
-    waters = A.synth_shell(seed + 1, cfg["waters"], cfg["axes"], 3.0) if cfg["waters"] else None
-    return atoms, waters
+def host_threads():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
 
 
 def traffic_from_profiles():
@@ -63,40 +73,190 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(cfg, atoms, waters, q, budget_s=15.0):
-    """The oracle's multithreaded FF managers (800-row tasks, thread-private
-    24-byte entries, merge) + FP64 Debye on this host, on the full workload if
-    it fits the time budget, else on a prefix sample grown until one run takes
-    at least 40 % of the budget."""
-    import oracle as O
+def pairs_of(n, m):
+    return n * (n - 1) / 2 + n * m + m * (m - 1) / 2
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(cores, 16))
-    mode = O.FFAVG if cfg["mode"] == "ffavg" else O.FFEXPLICIT
-    w = waters if waters is not None else np.zeros((0, 4), np.float32)
 
-    def pairs_of(n, m):
-        return n * (n - 1) / 2 + n * m + m * (m - 1) / 2
-
-    frac = min(1.0, 12000 / max(1, len(atoms)))
+def grow_sample(run, total_units, first_frac, budget_s):
+    """Run run(frac) on growing prefixes of the workload until one run takes
+    at least 40 % of the budget (or the whole workload ran); pair work grows
+    with the square of the prefix."""
+    frac = min(1.0, first_frac)
     while True:
-        ns, ms = int(len(atoms) * frac), int(len(w) * frac)
         t = time.perf_counter()
-        h = O.hist_ff(mode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, threads)
-        O.debye_ff(mode, h, q, cfg["bw"])
+        info = run(frac)
         dt = time.perf_counter() - t
         if frac >= 1.0 or dt >= 0.4 * budget_s:
-            break
-        # pair work grows with the square of the prefix
+            return frac, dt, info
         frac = min(1.0, frac * float(np.sqrt(0.8 * budget_s / max(dt, 1e-3))))
-    pairs = pairs_of(ns, ms)
-    sample = "full workload" if frac >= 1.0 else f"first {ns} atoms + {ms} waters of the workload"
-    return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=threads, kind="port",
-                sample=f"{sample} ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)",
-                seconds=dt, pairs=pairs)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def bench_structure(cfg, args):
+    import ausaxs_amd as A
+
+    atoms = A.synth_globule(args.config, cfg["n"], cfg["axes"], cfg["r_excl"])
+    waters = A.synth_shell(args.config + 1, cfg["waters"], cfg["axes"], 3.0) if cfg["waters"] else None
+    q = np.linspace(*cfg["q"])
+    mode = A.MODE_FFAVG if cfg["mode"] == "ffavg" else A.MODE_FFEXPLICIT
+    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, tile_j=args.tile_j)
+    t0 = time.perf_counter()
+    st = A.Structure(mode, atoms, waters, settings, z_exv=1.0)
+    log(f"resident structure built in {time.perf_counter() - t0:.2f} s")
+    st.set_timing(True)
+
+    def step():
+        return st.calculate().debye_transform(q)
+
+    res = timed_steps(step, args)
+    tm = st.last_timing()
+    res.update(pairs=tm["pairs"], profiles_per_step=1, kernel_ms=tm["pair_ms"], items=tm["n_items"],
+               extra={"atoms": len(atoms), "waters": 0 if waters is None else len(waters), "q_points": len(q),
+                      "debye_terms": args.debye_terms})
+
+    def cpu(budget):
+        import oracle as O
+
+        th = host_threads()
+        omode = O.FFAVG if cfg["mode"] == "ffavg" else O.FFEXPLICIT
+        w = waters if waters is not None else np.zeros((0, 4), np.float32)
+
+        def run(frac):
+            ns, ms = int(len(atoms) * frac), int(len(w) * frac)
+            h = O.hist_ff(omode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, th)
+            O.debye_ff(omode, h, q, cfg["bw"])
+            return ns, ms
+
+        frac, dt, (ns, ms) = grow_sample(run, len(atoms), 12000 / len(atoms), budget)
+        pairs = pairs_of(ns, ms)
+        sample = "full workload" if frac >= 1.0 else f"first {ns} atoms + {ms} waters of the workload"
+        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
+                    sample=f"{sample} ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
+
+    return res, cpu
+
+
+def bench_grid(cfg, args):
+    import ausaxs_amd as A
+
+    g = A.synth_globule(args.config, cfg["n"], cfg["axes"], cfg["r_excl"])
+    w = A.synth_shell(args.config + 1, cfg["waters"], cfg["axes"], 3.0)
+    xyz, ff = g[:, :3].astype(np.float64), g[:, 3].view(np.int32).copy()
+    wxyz = w[:, :3].astype(np.float64)
+    q = np.linspace(*cfg["q"])
+    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, tile_j=args.tile_j)
+    t0 = time.perf_counter()
+    st = A.GridStructure(xyz, ff, wxyz, settings)
+    log(f"resident grid structure built in {time.perf_counter() - t0:.2f} s (dummies from the 1 A grid)")
+    st.set_timing(True)
+
+    def step():
+        return st.calculate().debye_transform(q)
+
+    res = timed_steps(step, args)
+    tm = st.last_timing()
+    res.update(pairs=tm["pairs"], profiles_per_step=1, kernel_ms=tm["pair_ms"], items=tm["n_items"],
+               extra={"atoms": len(xyz), "waters": len(wxyz), "q_points": len(q), "debye_terms": args.debye_terms})
+
+    def cpu(budget):
+        import oracle as O
+
+        th = host_threads()
+        n_x = len(O.grid_exv(xyz, ff, wxyz)[0])
+        t = time.perf_counter()
+        h = O.hist_ffgrid(xyz, ff, wxyz, cfg["bw"], 8000, th)
+        O.debye_ffgrid(h, q, cfg["bw"])
+        dt = time.perf_counter() - t
+        pairs = pairs_of(len(xyz) + len(wxyz) + n_x, 0)
+        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
+                    sample=f"full workload incl. grid ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
+
+    return res, cpu
+
+
+def rigid_inputs(cfg, seed):
+    import ausaxs_amd as A
+
+    bodies, weights, x0 = [], [], 0.0
+    radii = [20.0 * (n / 2000.0) ** (1 / 3) for n in cfg["sizes"]]
+    for k, (n, r) in enumerate(zip(cfg["sizes"], radii)):
+        if k:
+            x0 += radii[k - 1] + r + 8.0
+        gl = A.synth_globule(seed + k, n, (r, r, r), 2.0)
+        bodies.append(gl[:, :3].astype(np.float64) + np.array([x0, 0.0, 0.0]))
+        weights.append(np.array([I0[int(t)] for t in gl[:, 3].view(np.int32)], np.float32))
+    rng = np.random.default_rng(seed)
+    T = np.zeros((cfg["n_conf"], len(bodies), 12))
+    for c in range(cfg["n_conf"]):
+        for b, xyz in enumerate(bodies):
+            R = A.random_rotation(rng)
+            cen = xyz.mean(0)
+            T[c, b] = A.rigid_transform(R, cen - R @ cen + rng.uniform(-10, 10, 3))
+    return bodies, weights, T
+
+
+def bench_rigid(cfg, args):
+    import ausaxs_amd as A
+
+    bodies, weights, T = rigid_inputs(cfg, args.config)
+    q = A.default_q_axis()
+    settings = A.default_settings(bin_width=cfg["bw"])
+    t0 = time.perf_counter()
+    if args.fixed_tiles:
+        settings.tile_j = args.tile_j
+    scan = A.RigidScan(bodies, weights, settings=settings, autotune=not args.fixed_tiles)
+    log(f"bodies resident, self terms cached in {time.perf_counter() - t0:.2f} s")
+    t0 = time.perf_counter()
+    scan.scan(T, q)
+    info = scan.info()
+    log(f"first scan (plan, autotune, graph capture) {time.perf_counter() - t0:.2f} s: {info}")
+
+    def step():
+        return scan.scan(T, q)
+
+    res = timed_steps(step, args)
+    info = scan.info()
+    res.update(pairs=info["pairs"], profiles_per_step=len(T), kernel_ms=None, items=info["n_items"],
+               extra={"bodies": [len(b) for b in bodies], "conformations": len(T), "q_points": len(q),
+                      "tile_j": info["tile_j"], "persistent_grid": info["grid"], "lds_copies": info["copies"],
+                      "graph_ms": info["scan_ms"]})
+
+    def cpu(budget):
+        import oracle as O
+
+        th = host_threads()
+        sizes = [len(b) for b in bodies]
+        ident = A.rigid_transform(np.eye(3), np.zeros(3))
+        self_pts = np.concatenate([O.rigid_transform(b, w, ident) for b, w in zip(bodies, weights)])
+        n = 0
+        t = time.perf_counter()
+        while n < len(T) and (time.perf_counter() - t) < 0.5 * budget:
+            moved = np.concatenate([O.rigid_transform(b, w, T[n, k]) for k, (b, w) in enumerate(zip(bodies, weights))])
+            h = O.hist_partial(self_pts, moved, sizes, None, cfg["bw"], 8000, th)
+            O.debye_simple(h, q, cfg["bw"])
+            n += 1
+        dt = time.perf_counter() - t
+        pairs = n * info["pairs"] / len(T)
+        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
+                    sample=f"first {n} of {len(T)} conformations ({pairs:.3g} inter-body pairs, partial "
+                           f"histograms + I(q), {dt:.2f} s)")
+
+    return res, cpu
+
+
+def timed_steps(step, args):
+    for _ in range(args.warmup):
+        step()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - ts)
+    elapsed = time.perf_counter() - t_start
+    return dict(elapsed=elapsed, lat_ms=np.array(lat) * 1e3)
 
 
 def main():
@@ -106,8 +266,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
     ap.add_argument("--debye-terms", type=int, default=3)
-    ap.add_argument("--tile-j", type=int, default=8000)
+    ap.add_argument("--tile-j", type=int, default=8000, help="columns per work item")
+    ap.add_argument("--fixed-tiles", action="store_true",
+                    help="rigid scan: use --tile-j and one workgroup per item instead of the autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -130,49 +293,39 @@ def main():
 
     cfg = CONFIGS[args.config]
     log("device:", A.device_info())
-    atoms, waters = make_inputs(cfg, args.config)
-    q = np.linspace(*cfg["q"])
-    mode = A.MODE_FFAVG if cfg["mode"] == "ffavg" else A.MODE_FFEXPLICIT
-    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, tile_j=args.tile_j)
-    t0 = time.perf_counter()
-    st = A.Structure(mode, atoms, waters, settings, z_exv=1.0)
-    log(f"resident structure built in {time.perf_counter() - t0:.2f} s")
-    st.set_timing(True)
-
-    def step():
-        h = st.calculate()
-        I = h.debye_transform(q)
-        return h, I
-
-    for _ in range(args.warmup):
-        step()
-    lat, pair_ms, total_ms = [], [], []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        h, I = step()
-        lat.append(time.perf_counter() - ts)
-        tm = st.last_timing()
-        pair_ms.append(tm["pair_ms"])
-        total_ms.append(tm["total_ms"])
-    elapsed = time.perf_counter() - t_start
-    tm = st.last_timing()
-    pairs = tm["pairs"]
+    fn = {"structure": bench_structure, "grid": bench_grid, "rigid": bench_rigid}[cfg["kind"]]
+    res, cpu_fn = fn(cfg, args)
+    elapsed, lat_ms, pairs = res["elapsed"], res["lat_ms"], res["pairs"]
     value = pairs * args.steps / elapsed / 1e9
-    kern_s = float(np.mean(pair_ms)) / 1e3
-    achieved = pairs * FLOP_PER_PAIR / kern_s / 1e12
-    log(f"steps {args.steps}: {elapsed:.3f} s, pair kernel {np.mean(pair_ms):.3f} ms, "
-        f"pairs+merge+strips {np.mean(total_ms):.3f} ms, items {tm['n_items']}")
+    log(f"steps {args.steps}: {elapsed:.3f} s, {pairs:.4g} pairs per step, items {res['items']}")
+
+    roofline = None
+    if res["kernel_ms"]:
+        kern_s = res["kernel_ms"] / 1e3
+        achieved = pairs * FLOP_PER_PAIR / kern_s / 1e12
+        roofline = {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path; 157.3 TF is the FP32 peak)",
+                    "kernel": "pair_hist_kernel", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                    "traffic": traffic_from_profiles() if args.config == 3 else None,
+                    "kernel_ms": round(res["kernel_ms"], 4), "pairs_per_s": round(pairs / kern_s, 1)}
+    else:
+        # whole graph launch (transform, pair, merge, strips, finalize, Debye) per step
+        g_s = res["extra"]["graph_ms"] / 1e3
+        achieved = pairs * FLOP_PER_PAIR / g_s / 1e12
+        roofline = {"bound": "mfma", "engine": "fp32 VALU; timed over the whole graph launch",
+                    "kernel": "pair_hist_kernel (graph)", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                    "graph_ms": round(res["extra"]["graph_ms"], 4), "pairs_per_s": round(pairs / g_s, 1)}
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, atoms, waters if waters is not None else None, q)
-        log(f"cpu baseline: {cpu['value']:.3f} Gpair-evals/s on {cpu['cores']} threads ({cpu['sample']})")
-        cpu = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        cpu = cpu_fn(args.cpu_budget)
+        log(f"cpu baseline: {cpu['value']:.3f} {cpu['unit']} on {cpu['cores']} threads ({cpu['sample']})")
 
-    lat_ms = np.array(lat) * 1e3
+    config = {"workload": cfg["name"], "pairs_per_step": pairs, "parallelism": "single-gpu"}
+    config.update(res["extra"])
     out = {
-        "metric": "Gpair-evals/s",
+        "metric": METRIC,
         "value": round(value, 4),
         "unit": "Gpair-evals/s",
         "n_gpus": 1,
@@ -184,17 +337,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": cfg["name"], "atoms": len(atoms), "waters": 0 if waters is None else len(waters),
-                   "pairs_per_step": pairs, "q_points": len(q), "debye_terms": args.debye_terms,
-                   "parallelism": "single-gpu"},
-        "profiles_per_s": round(args.steps / elapsed, 4),
+        "config": config,
+        "profiles_per_s": round(res["profiles_per_step"] * args.steps / elapsed, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4),
                        "p99": round(float(np.percentile(lat_ms, 99)), 4)},
-        "roofline": {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path; 157.3 TF is the FP32 peak)",
-                     "kernel": "pair_hist_kernel", "achieved": round(achieved, 4),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                     "traffic": traffic_from_profiles(), "kernel_ms": round(kern_s * 1e3, 4),
-                     "pairs_per_s": round(pairs / kern_s, 1)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -127,7 +127,8 @@ typedef struct ausaxs_rigid ausaxs_rigid;
  * body and of the waters is computed here, once, from these coordinates
  * (initialize / calc_self_correlation / calc_ww, :259-294), as the reference
  * caches it while bodies only move rigidly. settings->tile_j = 0 selects the
- * autotuner (tile width and persistent grid timed on the first batch). */
+ * autotuner (tile width, LDS histogram copies and persistent grid, timed on
+ * the first batch of each shape). */
 int ausaxs_rigid_create(const double* xyz, const float* w, const int32_t* body_sizes, int n_bodies,
                         const double* waters_xyz, const float* waters_w, int n_waters, const ausaxs_settings* settings,
                         ausaxs_rigid** out);
@@ -150,10 +151,11 @@ int ausaxs_rigid_scan(ausaxs_rigid* r, const double* transforms, int n_conf, con
 int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram** out);
 
 /* Launch shape and timing of the last scan: tile width, persistent grid
- * (0 = one workgroup per item), conformations per graph launch, work items
- * per launch, pair evaluations, HIP-event time of the graph launches. */
-int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* batch, int* n_items, long long* pairs,
-                      double* scan_ms);
+ * (0 = one workgroup per item), LDS histogram copies, conformations per graph
+ * launch, work items per launch, pair evaluations, HIP-event time of the
+ * graph launches. Any pointer may be NULL. */
+int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* copies, int* batch, int* n_items,
+                      long long* pairs, double* scan_ms);
 int ausaxs_rigid_free(ausaxs_rigid* r);
 
 /* The reference's default q sub-axis (ConstantsAxes.h:14, Axis::sub_axis,
