@@ -84,6 +84,14 @@ def _load():
         "ausaxs_rigid_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int),
                                         P(C.c_longlong), P(C.c_double)]),
         "ausaxs_rigid_free": (C.c_int, [vp]),
+        "ausaxs_server_create": (C.c_int, [C.c_int, C.c_double, P(vp)]),
+        "ausaxs_server_submit": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_double,
+                                           P(C.c_longlong)]),
+        "ausaxs_server_wait": (C.c_int, [vp, C.c_longlong, C.c_double, vp, P(C.c_double), P(C.c_double),
+                                         P(C.c_int)]),
+        "ausaxs_server_stats": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong), P(C.c_longlong), P(C.c_int),
+                                          P(C.c_int), P(C.c_double), P(C.c_longlong)]),
+        "ausaxs_server_free": (C.c_int, [vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_synth_globule": (
             C.c_int,
@@ -412,3 +420,56 @@ def random_rotation(rng):
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+class Server:
+    """Request loop with continuous batching over `slots` device slots and a
+    device-memory budget (GiB). submit() returns an id; wait() returns
+    (I, latency_ms, queue_ms) or None on timeout."""
+
+    def __init__(self, slots=4, budget_gb=64.0):
+        h = C.c_void_p()
+        _check(lib.ausaxs_server_create(slots, budget_gb, C.byref(h)))
+        self._s = h
+        self._nq = {}
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib.ausaxs_server_free(self._s)
+            self._s = None
+
+    def close(self):
+        self.__del__()
+
+    def submit(self, mode, atoms_xyz, ff, waters_xyz=None, q=None, settings=None, z_exv=1.0):
+        a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(ff, np.int32), (len(a),)), np.int32)
+        w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q if q is not None else default_q_axis(), np.float64)
+        s = settings or default_settings()
+        rid = C.c_longlong()
+        _check(lib.ausaxs_server_submit(self._s, mode, _ptr(a), _ptr(f), len(a), _ptr(w), len(w), _ptr(q), len(q),
+                                        C.byref(s), z_exv, C.byref(rid)))
+        self._nq[rid.value] = len(q)
+        return rid.value
+
+    def wait(self, rid, timeout=600.0):
+        I = np.zeros(self._nq[rid])
+        lat, qu, ready = C.c_double(), C.c_double(), C.c_int()
+        try:
+            _check(lib.ausaxs_server_wait(self._s, rid, timeout, _ptr(I), C.byref(lat), C.byref(qu), C.byref(ready)))
+        except AusaxsError:
+            self._nq.pop(rid, None)
+            raise
+        if not ready.value:
+            return None
+        self._nq.pop(rid, None)
+        return I, lat.value, qu.value
+
+    def stats(self):
+        v = [C.c_longlong() for _ in range(3)]
+        fl, pk, gb, pairs = C.c_int(), C.c_int(), C.c_double(), C.c_longlong()
+        _check(lib.ausaxs_server_stats(self._s, *map(C.byref, v), C.byref(fl), C.byref(pk), C.byref(gb),
+                                       C.byref(pairs)))
+        return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
+                    peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value)

@@ -32,8 +32,8 @@ COMMON = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["pair_hist.hip", "debye.hip", "rigid.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "synth.cpp",
-           "capi.cpp"]
+SOURCES = ["pair_hist.hip", "debye.hip", "rigid.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "serve.cpp",
+           "synth.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> Path:

@@ -7,6 +7,7 @@
 #include "grid.hpp"
 #include "hist.hpp"
 #include "rigid.hpp"
+#include "serve.hpp"
 #include "synth.hpp"
 
 #include <cstring>
@@ -26,6 +27,10 @@ struct ausaxs_histogram {
 
 struct ausaxs_rigid {
     std::unique_ptr<RigidScan> r;
+};
+
+struct ausaxs_server {
+    std::unique_ptr<Server> s;
 };
 
 namespace {
@@ -123,24 +128,8 @@ int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff
         require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
         require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
         require(n_waters == 0 || waters_xyz, "waters is NULL");
-        for (int i = 0; i < n_atoms; ++i)
-            require(atom_ff[i] >= 1 && atom_ff[i] < kMaxFF, "atom form factor type outside [1, 14]");
-        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters);
-        // CompactCoordinatesFF: float casts of the FP64 coordinates
-        std::vector<float> a(4 * (size_t)n_atoms), w(4 * (size_t)n_waters);
-        for (int i = 0; i < n_atoms; ++i) {
-            for (int k = 0; k < 3; ++k) a[4 * i + k] = (float)atoms_xyz[3 * i + k];
-            std::memcpy(&a[4 * i + 3], &atom_ff[i], 4);
-        }
-        const int32_t water = kWaterBin;
-        for (int i = 0; i < n_waters; ++i) {
-            for (int k = 0; k < 3; ++k) w[4 * i + k] = (float)waters_xyz[3 * i + k];
-            std::memcpy(&w[4 * i + 3], &water, 4);
-        }
         auto s = std::make_unique<ausaxs_structure>();
-        s->s = std::make_unique<ResidentStructure>(Mode::FFGrid, a.data(), n_atoms, w.data(), n_waters,
-                                                   to_settings(settings), 0.0, g.xyz.data(),
-                                                   (int)(g.xyz.size() / 3), g.volume);
+        s->s = make_grid_structure(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, to_settings(settings));
         *out = s.release();
     });
 }
@@ -300,14 +289,14 @@ int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, con
         Context& ctx = Context::get();
         DevBuf<double> dH, dd, dq, dout;
         DevBuf<char> scratch;
-        dH.upload(H, (size_t)rows * K, ctx.stream);
-        dd.upload(d, K, ctx.stream);
-        dq.upload(q, nq, ctx.stream);
+        dH.upload(H, (size_t)rows * K, ctx.stream());
+        dd.upload(d, K, ctx.stream());
+        dq.upload(q, nq, ctx.stream());
         dout.ensure((size_t)rows * nq);
         scratch.ensure(debye_scratch_bytes(rows, K, nq));
         DebyeLaunch dl{dH.p, dd.p, dq.p, rows, K, nq, terms, dout.p, scratch.p};
-        check(terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
-        check(hipMemcpyAsync(out, dout.p, (size_t)rows * nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+        check(terms > 0 ? launch_debye(dl, ctx.stream()) : launch_debye_f64(dl, ctx.stream()), "debye");
+        check(hipMemcpyAsync(out, dout.p, (size_t)rows * nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()),
               "D2H");
         ctx.sync();
     });
@@ -319,9 +308,9 @@ int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long l
         Context& ctx = Context::get();
         DevBuf<unsigned long long> d;
         d.ensure(3);
-        check(hipMemsetAsync(d.p, 0, 3 * sizeof(unsigned long long), ctx.stream), "memset");
-        check(launch_bin_check(lo, hi, (float)(1.0 / bin_width), d.p, ctx.stream), "bin_check_kernel");
-        check(hipMemcpyAsync(mism, d.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream), "D2H");
+        check(hipMemsetAsync(d.p, 0, 3 * sizeof(unsigned long long), ctx.stream()), "memset");
+        check(launch_bin_check(lo, hi, (float)(1.0 / bin_width), d.p, ctx.stream()), "bin_check_kernel");
+        check(hipMemcpyAsync(mism, d.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
         ctx.sync();
     });
 }
@@ -333,13 +322,13 @@ int ausaxs_copy_probe(size_t bytes, int reps, double* ms) {
         DevBuf<char> a, b;
         a.ensure(bytes);
         b.ensure(bytes);
-        check(hipMemsetAsync(a.p, 1, bytes, ctx.stream), "memset");
+        check(hipMemsetAsync(a.p, 1, bytes, ctx.stream()), "memset");
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
-        check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
-        for (int r = 0; r < reps; ++r) check(launch_copy(a.p, b.p, bytes, ctx.stream), "copy_kernel");
-        check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
+        check(hipEventRecord(e0, ctx.stream()), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) check(launch_copy(a.p, b.p, bytes, ctx.stream()), "copy_kernel");
+        check(hipEventRecord(e1, ctx.stream()), "hipEventRecord");
         ctx.sync();
         float t = 0;
         check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
@@ -416,6 +405,63 @@ int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* copies
 
 int ausaxs_rigid_free(ausaxs_rigid* r) {
     return guarded([&] { delete r; });
+}
+
+int ausaxs_server_create(int slots, double budget_gb, ausaxs_server** out) {
+    return guarded([&] {
+        require(out && budget_gb > 0, "bad argument");
+        auto s = std::make_unique<ausaxs_server>();
+        s->s = std::make_unique<Server>(slots, (size_t)(budget_gb * (1ull << 30)));
+        *out = s.release();
+    });
+}
+
+int ausaxs_server_submit(ausaxs_server* s, int mode, const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                         const double* waters_xyz, int n_waters, const double* q, int nq,
+                         const ausaxs_settings* settings, double z_exv, long long* id) {
+    return guarded([&] {
+        require(s && s->s && id, "bad argument");
+        require(n_atoms >= 0 && n_waters >= 0 && nq >= 0, "negative sizes");
+        require((n_atoms == 0 || (atoms_xyz && atom_ff)) && (n_waters == 0 || waters_xyz) && (nq == 0 || q),
+                "bad argument");
+        require(mode == AUSAXS_MODE_FFAVG || mode == AUSAXS_MODE_FFEXPLICIT || mode == AUSAXS_MODE_FFGRID,
+                "the server scores FFAVG, FFEXPLICIT and FFGRID requests");
+        *id = s->s->submit((Mode)mode, atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, q, nq,
+                           to_settings(settings), z_exv);
+    });
+}
+
+int ausaxs_server_wait(ausaxs_server* s, long long id, double timeout_s, double* I, double* latency_ms,
+                       double* queue_ms, int* ready) {
+    return guarded([&] {
+        require(s && s->s && ready, "bad argument");
+        std::vector<double> v;
+        double lat = 0, qu = 0;
+        *ready = s->s->wait(id, timeout_s, v, lat, qu) ? 1 : 0;
+        if (!*ready) return;
+        if (I) std::copy(v.begin(), v.end(), I);
+        if (latency_ms) *latency_ms = lat;
+        if (queue_ms) *queue_ms = qu;
+    });
+}
+
+int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* completed, long long* failed,
+                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs) {
+    return guarded([&] {
+        require(s && s->s, "bad argument");
+        const ServeStats st = s->s->stats();
+        if (submitted) *submitted = st.submitted;
+        if (completed) *completed = st.completed;
+        if (failed) *failed = st.failed;
+        if (in_flight) *in_flight = st.in_flight;
+        if (peak_in_flight) *peak_in_flight = st.peak_in_flight;
+        if (peak_gb) *peak_gb = (double)st.peak_in_use / (1ull << 30);
+        if (pairs) *pairs = st.pairs;
+    });
+}
+
+int ausaxs_server_free(ausaxs_server* s) {
+    return guarded([&] { delete s; });
 }
 
 }  // extern "C"

@@ -19,13 +19,21 @@ Context::Context() {
         throw std::runtime_error("ausaxs_hip: no HIP device available (the GPU path has no CPU fallback)");
     check(hipGetDevice(&device), "hipGetDevice");
     check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&default_stream, hipStreamNonBlocking), "hipStreamCreate");
 }
 
 Context& Context::get() {
     static Context ctx;
     return ctx;
 }
+
+namespace {
+thread_local hipStream_t t_stream = nullptr;
+}
+
+void Context::set_thread_stream(hipStream_t s) { t_stream = s; }
+
+hipStream_t Context::stream() const { return t_stream ? t_stream : default_stream; }
 
 namespace {
 constexpr int kTileI = 512;  // rows per work item: 256 lanes x 2 (pair_hist.hip)
@@ -270,7 +278,7 @@ void PairEngine::build(int tile_j) {
     copies_ = pick_copies(nbins_, rows_);
 
     // ---- device residency -------------------------------------------------
-    hipStream_t s = ctx.stream;
+    hipStream_t s = ctx.stream();
     d_pts_.upload(buf.data(), buf.size(), s);
     d_orig_.clear();
     for (auto& ps : sets_) {
@@ -292,7 +300,7 @@ void PairEngine::build(int tile_j) {
 
 void PairEngine::enqueue(bool timed) {
     Context& ctx = Context::get();
-    hipStream_t s = ctx.stream;
+    hipStream_t s = ctx.stream();
     if (timed) check(hipEventRecord(ev_[0], s), "hipEventRecord");
     PairLaunch pl{d_pts_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_,
                   rows_, copies_};
@@ -315,8 +323,8 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     const size_t n = (size_t)n_rows_ * (nbins_ + 1);
     cnt.resize(n);
     dsum.resize(n);
-    check(hipMemcpyAsync(cnt.data(), d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream), "D2H");
-    check(hipMemcpyAsync(dsum.data(), d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream), "D2H");
+    check(hipMemcpyAsync(cnt.data(), d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    check(hipMemcpyAsync(dsum.data(), d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     ctx.sync();
 }
 

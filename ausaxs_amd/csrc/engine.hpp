@@ -27,13 +27,18 @@ inline float float_of(int32_t i) {
     return f;
 }
 
-// Process-wide HIP context on the current device: one default-priority stream.
+// Process-wide HIP context on the current device. Work is issued on the
+// calling thread's stream: a serving slot's own stream when the thread has
+// one (set_thread_stream), else the context's default stream. All streams
+// have the default priority.
 struct Context {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t default_stream = nullptr;
     hipDeviceProp_t prop{};
     static Context& get();
-    void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+    static void set_thread_stream(hipStream_t s);
+    hipStream_t stream() const;
+    void sync() const { check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
 
   private:
     Context();

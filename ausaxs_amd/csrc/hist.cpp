@@ -1,6 +1,8 @@
 // Histogram managers and composite histograms (see hist.hpp).
 #include "hist.hpp"
 
+#include "grid.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -579,32 +581,34 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     if (rows == 0 || nq == 0) return cached_sinqd_;
 
     Context& ctx = Context::get();
-    static DevBuf<double> dH, dd, dq, dout;
-    static DevBuf<char> scratch;
+    // per-thread operand buffers, reused across calls (serving slots run
+    // concurrently, each on its own thread and stream)
+    thread_local DevBuf<double> dH, dd, dq, dout;
+    thread_local DevBuf<char> scratch;
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
-    dq.upload(q.data(), q.size(), ctx.stream);
+    dq.upload(q.data(), q.size(), ctx.stream());
     dout.ensure((size_t)rows * nq);
-    check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
+    check(hipEventRecord(e0, ctx.stream()), "hipEventRecord");
     size_t row0 = 0;
     for (int a = 0; a < 3; ++a) {
         const int ra = (int)keys[a].size();
         if (ra == 0) continue;
         if ((int)axes[a]->size() < K) throw std::runtime_error("ausaxs_hip: distance axis shorter than the histogram");
         // the uploads are ordered on the stream before this group's kernels
-        dH.upload(H[a].data(), H[a].size(), ctx.stream);
-        dd.upload(axes[a]->data(), K, ctx.stream);
+        dH.upload(H[a].data(), H[a].size(), ctx.stream());
+        dd.upload(axes[a]->data(), K, ctx.stream());
         scratch.ensure(debye_scratch_bytes(ra, K, nq));
         DebyeLaunch dl{dH.p, dd.p, dq.p, ra, K, nq, debye_terms, dout.p + row0 * nq, scratch.p};
-        check(debye_terms > 0 ? launch_debye(dl, ctx.stream) : launch_debye_f64(dl, ctx.stream), "debye");
+        check(debye_terms > 0 ? launch_debye(dl, ctx.stream()) : launch_debye_f64(dl, ctx.stream()), "debye");
         // dH/dd are reused by the next group: wait before overwriting them
         ctx.sync();
         row0 += ra;
     }
-    check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
+    check(hipEventRecord(e1, ctx.stream()), "hipEventRecord");
     check(hipMemcpyAsync(cached_sinqd_.data(), dout.p, cached_sinqd_.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, ctx.stream),
+                         hipMemcpyDeviceToHost, ctx.stream()),
           "D2H");
     ctx.sync();
     float ms = 0;
@@ -702,6 +706,28 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
 // ---------------------------------------------------------------------------
 // raw pair set
 // ---------------------------------------------------------------------------
+void pack_ff_points(const double* xyz, const int32_t* ff, int n, int32_t fixed_type, std::vector<float>& out) {
+    out.resize(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) out[4 * (size_t)i + k] = (float)xyz[3 * (size_t)i + k];
+        const int32_t t = ff ? ff[i] : fixed_type;
+        std::memcpy(&out[4 * (size_t)i + 3], &t, 4);
+    }
+}
+
+std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                       const double* waters_xyz, int m, const HistSettings& s) {
+    for (int i = 0; i < n; ++i)
+        if (ff[i] < 1 || ff[i] >= kMaxFF)
+            throw std::runtime_error("ausaxs_hip: atom form factor type outside [1, 14]");
+    GridExv g = make_grid_exv(atoms_xyz, ff, n, waters_xyz, m);
+    std::vector<float> a, w;
+    pack_ff_points(atoms_xyz, ff, n, 0, a);
+    pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);
+    return std::make_unique<ResidentStructure>(Mode::FFGrid, a.data(), n, w.data(), m, s, 0.0, g.xyz.data(),
+                                               (int)(g.xyz.size() / 3), g.volume);
+}
+
 void raw_pair_histogram(const float* a, int n, const float* b, int m, const HistSettings& s, uint64_t* count,
                         double* dsum) {
     const float inv_w = inv_width(s);

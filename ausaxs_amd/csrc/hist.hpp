@@ -121,6 +121,15 @@ class ResidentStructure {
     std::vector<int64_t> dsum_;
 };
 
+// HistogramManagerMTFFGrid structure from FP64 atoms (n x 3, form-factor
+// types 1..14) and waters (m x 3): grid dummies (grid.hpp) and float casts
+// of the coordinates as CompactCoordinatesFF holds them.
+std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                       const double* waters_xyz, int m, const HistSettings& s);
+// The FF-manager points of FP64 atoms/waters: {float x, y, z, type bits},
+// waters typed WATER.
+void pack_ff_points(const double* xyz, const int32_t* ff, int n, int32_t fixed_type, std::vector<float>& out);
+
 // single pair set with the reference's loop geometry (test entry)
 void raw_pair_histogram(const float* a, int n, const float* b, int m, const HistSettings& s, uint64_t* count,
                         double* dsum);

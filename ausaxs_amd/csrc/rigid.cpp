@@ -228,10 +228,10 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
         base[4 * base_nb_ + b] = base_ww_[b].count;
         base[5 * base_nb_ + b] = base_ww_[b].bin_center;
     }
-    d_base_.upload(base.data(), base.size(), ctx.stream);
-    d_xyz_.upload(xyz_.data(), xyz_.size(), ctx.stream);
-    d_row_kind_.upload(row_kind_.data(), row_kind_.size(), ctx.stream);
-    d_row_w_.upload(row_w_.data(), row_w_.size(), ctx.stream);
+    d_base_.upload(base.data(), base.size(), ctx.stream());
+    d_xyz_.upload(xyz_.data(), xyz_.size(), ctx.stream());
+    d_row_kind_.upload(row_kind_.data(), row_kind_.size(), ctx.stream());
+    d_row_w_.upload(row_w_.data(), row_w_.size(), ctx.stream());
     ctx.sync();
 }
 
@@ -374,7 +374,7 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
 
     // device residency
     Context& ctx = Context::get();
-    hipStream_t s = ctx.stream;
+    hipStream_t s = ctx.stream();
     p.pts.ensure(total);
     p.orig.ensure(total);
     if (waters_.size()) {
@@ -462,7 +462,7 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
 
 void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     Context& ctx = Context::get();
-    hipStream_t s = ctx.stream;
+    hipStream_t s = ctx.stream();
     const int B = n_bodies();
     if (copies) {
         check(hipMemcpyAsync(p.T.p, p.hT.p, (size_t)p.n_conf * B * 12 * sizeof(double), hipMemcpyHostToDevice, s),
@@ -498,10 +498,10 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
 float RigidScan::time_pair(Plan& p, int grid, int copies) {
     Context& ctx = Context::get();
     PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid, p.rows, copies};
-    check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");  // warm-up
-    check(hipEventRecord(p.ev[0], ctx.stream), "hipEventRecord");
-    check(launch_pair_hist(pl, ctx.stream), "pair_hist_kernel");
-    check(hipEventRecord(p.ev[1], ctx.stream), "hipEventRecord");
+    check(launch_pair_hist(pl, ctx.stream()), "pair_hist_kernel");  // warm-up
+    check(hipEventRecord(p.ev[0], ctx.stream()), "hipEventRecord");
+    check(launch_pair_hist(pl, ctx.stream()), "pair_hist_kernel");
+    check(hipEventRecord(p.ev[1], ctx.stream()), "hipEventRecord");
     check(hipEventSynchronize(p.ev[1]), "hipEventSynchronize");
     float ms = 0;
     check(hipEventElapsedTime(&ms, p.ev[0], p.ev[1]), "hipEventElapsedTime");
@@ -525,12 +525,12 @@ RigidScan::Plan& RigidScan::plan_for(int n_conf, const std::vector<int>& moving,
         for (int tile : {1024, 2048, 4096, 8000}) {
             auto p = build_plan(n_conf, moving, nbins, tile);
             check(hipMemcpyAsync(p->T.p, T, (size_t)n_conf * n_bodies() * 12 * sizeof(double),
-                                 hipMemcpyHostToDevice, ctx.stream),
+                                 hipMemcpyHostToDevice, ctx.stream()),
                   "H2D transforms");
             TransformLaunch tl{d_xyz_.p, p->T.p, p->slot_src.p, p->slot_body.p, p->slot_dst.p, p->slot_dsto.p,
                                p->slot_tag.p, p->slot_cls.p, p->body_moving.p, p->n_slots, n_bodies(), n_conf,
                                p->M, p->pts.p, p->orig.p};
-            check(launch_transform(tl, ctx.stream), "transform_kernel");
+            check(launch_transform(tl, ctx.stream()), "transform_kernel");
             const float ms = time_pair(*p, 0, p->copies);
             ++cand;
             if (ms < best_ms) {
@@ -567,7 +567,7 @@ RigidScan::Plan& RigidScan::plan_for(int n_conf, const std::vector<int>& moving,
 
 void RigidScan::run_batch(Plan& p, const double* T, const double* q, int nq, double cw, double* I) {
     Context& ctx = Context::get();
-    hipStream_t s = ctx.stream;
+    hipStream_t s = ctx.stream();
     const int B = n_bodies();
     std::memcpy(p.hT.p, T, (size_t)p.n_conf * B * 12 * sizeof(double));
     if (nq > p.nq_cap || !p.q.p) {
@@ -653,10 +653,10 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     for (int part = 0; part < 2; ++part) {
         const size_t row0 = part == 0 ? (size_t)lc * R_ : (size_t)p.n_conf * R_;
         check(hipMemcpyAsync(cnt.data() + part * rows_bytes, p.cnt.p + row0 * stride, rows_bytes * sizeof(uint64_t),
-                             hipMemcpyDeviceToHost, ctx.stream),
+                             hipMemcpyDeviceToHost, ctx.stream()),
               "D2H");
         check(hipMemcpyAsync(dsum.data() + part * rows_bytes, p.dsum.p + row0 * stride,
-                             rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream),
+                             rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
               "D2H");
     }
     ctx.sync();

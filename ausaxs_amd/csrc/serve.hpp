@@ -1,0 +1,91 @@
+// Serving loop for profile requests: continuous batching over device slots.
+//
+// Each request is one structure (FP64 atoms with form-factor types, waters)
+// scored by one of the managers (FF-avg, FF-explicit or FF-grid,
+// HistogramManagerMTFF{Avg,Explicit,Grid}::calculate_all) followed by the
+// Debye transform on its own q grid. The server owns `slots` worker threads,
+// each with its own HIP stream. Requests are admitted in arrival order as soon
+// as a slot is idle and their estimated device footprint fits the memory
+// budget; admitted requests run concurrently, so the kernels of requests that
+// arrived while others were already on the GPU overlap with them. Coordinates
+// and partial histograms of a running request stay resident in HBM for its
+// whole evaluation. Every result is computed exactly as for the request alone
+// (integer histograms, deterministic kernels), so it is the same alone and in
+// any mix.
+#pragma once
+
+#include "hist.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace ausaxs::gpu {
+
+struct ServeStats {
+    int64_t submitted = 0, completed = 0, failed = 0;
+    int in_flight = 0, peak_in_flight = 0;
+    size_t in_use = 0, peak_in_use = 0, budget = 0;
+    int64_t pairs = 0;  // pair evaluations of completed requests
+};
+
+class Server {
+  public:
+    using clock = std::chrono::steady_clock;
+
+    Server(int slots, size_t budget_bytes);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    // mode: Mode::FFAvg, FFExplicit or FFGrid. Inputs are copied.
+    int64_t submit(Mode mode, const double* atoms_xyz, const int32_t* ff, int n, const double* waters_xyz, int m,
+                   const double* q, int nq, const HistSettings& s, double z_exv);
+    // Blocks until the request is done or timeout_s passes (false). On
+    // success I receives nq values; latency = done - submit, queue = admit -
+    // submit. A failed request throws its error. The request is forgotten
+    // after a successful wait.
+    bool wait(int64_t id, double timeout_s, std::vector<double>& I, double& latency_ms, double& queue_ms);
+    ServeStats stats();
+
+    // estimated device bytes of a request (admission)
+    static size_t footprint(Mode mode, const double* atoms_xyz, int n, const double* waters_xyz, int m, int nq,
+                            const HistSettings& s);
+
+  private:
+    struct Request {
+        int64_t id = 0;
+        Mode mode = Mode::FFExplicit;
+        std::vector<double> atoms, waters, q;
+        std::vector<int32_t> ff;
+        HistSettings s;
+        double z_exv = 1;
+        size_t bytes = 0;
+        clock::time_point t_submit, t_admit, t_done;
+        std::vector<double> I;
+        std::string error;
+        int64_t pairs = 0;
+        bool done = false;
+    };
+    std::mutex m_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<std::shared_ptr<Request>> pending_;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> requests_;
+    std::vector<std::thread> workers_;
+    std::vector<hipStream_t> streams_;
+    ServeStats st_;
+    int64_t next_id_ = 1;
+    bool stop_ = false;
+
+    void worker(int slot);
+    void run(Request& r);
+};
+
+}  // namespace ausaxs::gpu

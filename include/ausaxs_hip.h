@@ -158,6 +158,42 @@ int ausaxs_rigid_info(const ausaxs_rigid* r, int* tile_j, int* grid, int* copies
                       long long* pairs, double* scan_ms);
 int ausaxs_rigid_free(ausaxs_rigid* r);
 
+/* ---- serving: profile requests with continuous batching ------------------- */
+
+typedef struct ausaxs_server ausaxs_server;
+
+/* A request loop with `slots` device slots (worker threads with their own
+ * streams, 1..16) and a device-memory budget in GiB. Requests are admitted
+ * in arrival order as soon as a slot is idle and their estimated footprint
+ * fits the budget, and run concurrently with those already on the GPU. There
+ * is no reference counterpart; each request is one manager evaluation
+ * (HistogramManagerMTFF{Avg,Explicit,Grid}::calculate_all) plus
+ * debye_transform on its q grid, identical to ausaxs_structure_create[_grid] +
+ * ausaxs_structure_calculate + ausaxs_histogram_debye on the same input. */
+int ausaxs_server_create(int slots, double budget_gb, ausaxs_server** out);
+
+/* Queue one request (inputs are copied): FP64 atoms (n x 3) with form-factor
+ * types (1..14), FP64 waters (m x 3), q grid. mode: AUSAXS_MODE_FFAVG,
+ * _FFEXPLICIT or _FFGRID. A request whose footprint exceeds the whole budget
+ * is refused (status 1). */
+int ausaxs_server_submit(ausaxs_server* s, int mode, const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                         const double* waters_xyz, int n_waters, const double* q, int nq,
+                         const ausaxs_settings* settings, double z_exv, long long* id);
+
+/* Wait up to timeout_s for a request. *ready = 1: I[nq] holds the profile,
+ * latency_ms = completion - submission, queue_ms = admission - submission,
+ * and the request is released. *ready = 0: still running (status 0). A
+ * failed request returns status 1 with its error message. */
+int ausaxs_server_wait(ausaxs_server* s, long long id, double timeout_s, double* I, double* latency_ms,
+                       double* queue_ms, int* ready);
+
+/* Counters since creation; peak_gb = largest sum of admitted footprints. */
+int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* completed, long long* failed,
+                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs);
+
+/* Stops the loop: running requests finish, queued ones fail. */
+int ausaxs_server_free(ausaxs_server* s);
+
 /* The reference's default q sub-axis (ConstantsAxes.h:14, Axis::sub_axis,
  * Axis.cpp:40-59). q must hold 200 doubles; *nq receives the length. */
 int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq);
