@@ -11,7 +11,8 @@ Debye transform (sinc table, bf16-split MFMA GEMM, intensity profiles).
 Other workloads: --config 1 (2k-atom globule, FF-avg), --config 2 (20k-atom
 globule + 6k waters + 1 A excluded-volume grid, FF-grid), --config 4 (rigid-
 body scan: bodies of 3k/5k/8k/12k atoms, 512 conformations per step in one
-graph launch).
+graph launch), --config 5 (serving: Poisson arrivals of mixed requests at 70 %
+of the measured service rate; one step = one request).
 
 The timed region brackets exactly --steps steps with device synchronisation
 on both sides. Prints one JSON line: pair evaluations per second, profiles
@@ -47,6 +48,9 @@ CONFIGS = {
             q=(0.0015, 1.5, 1000)),
     4: dict(kind="rigid", name="rigid-body scan: bodies of 3k/5k/8k/12k atoms, 512 conformations per step, "
             "XYZW weights, 0.25 A bins, default q axis", sizes=(3000, 5000, 8000, 12000), n_conf=512, bw=0.25),
+    5: dict(kind="serve", name="serving: Poisson arrivals at 70 % of the measured service rate, structures "
+            "log-uniform 1k-100k atoms (+30 % waters), modes uniform over FF-avg/FF-explicit/FF-grid, 200 q in 0-0.5",
+            pool=48, n_min=1000, n_max=100000, load=0.7, slots=4, budget_gb=200.0),
 }
 I0 = {3: 6.0, 7: 7.0, 11: 8.0, 12: 16.0}  # XYZW weights: electron counts of C, N, O, S
 
@@ -246,6 +250,126 @@ def bench_rigid(cfg, args):
     return res, cpu
 
 
+def serve_pool(cfg, seed):
+    import ausaxs_amd as A
+
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(cfg["n_min"]), np.log(cfg["n_max"]), cfg["pool"])).astype(int)
+    pool = []
+    for k, n in enumerate(sizes):
+        r = 20.0 * (n / 2000.0) ** (1 / 3)
+        g = A.synth_globule(seed * 1000 + k, int(n), (r, r, r), 2.0)
+        w = A.synth_shell(seed * 1000 + 500 + k, max(1, int(0.3 * n)), (r, r, r), 3.0)
+        pool.append((g[:, :3].astype(np.float64), g[:, 3].view(np.int32).copy(), w[:, :3].astype(np.float64)))
+    return pool
+
+
+def bench_serve(cfg, args):
+    import threading
+
+    import ausaxs_amd as A
+
+    modes = (A.MODE_FFAVG, A.MODE_FFEXPLICIT, A.MODE_FFGRID)
+    t0 = time.perf_counter()
+    pool = serve_pool(cfg, args.config)
+    log(f"request pool of {len(pool)} structures built in {time.perf_counter() - t0:.1f} s")
+    q = np.linspace(0.0, 0.5, 200)
+    rng = np.random.default_rng(args.config + 1)
+    n_req = args.requests
+
+    def draw(count):
+        return [(int(rng.integers(len(pool))), modes[int(rng.integers(3))]) for _ in range(count)]
+
+    srv = A.Server(slots=cfg["slots"], budget_gb=cfg["budget_gb"])
+
+    def submit(req):
+        xyz, ff, w = pool[req[0]]
+        return srv.submit(req[1], xyz, ff, w, q)
+
+    # warm-up, then the closed-loop service rate on a burst
+    for i in [submit(r) for r in draw(max(args.warmup, 1) * 4)]:
+        srv.wait(i)
+    burst = draw(max(30, n_req // 8))
+    s0 = srv.stats()["pairs"]
+    t = time.perf_counter()
+    for i in [submit(r) for r in burst]:
+        srv.wait(i)
+    mu = len(burst) / (time.perf_counter() - t)
+    lam = cfg["load"] * mu
+    log(f"service rate {mu:.2f} requests/s on a burst of {len(burst)} "
+        f"({(srv.stats()['pairs'] - s0) / len(burst):.3g} pairs per request); arrivals at {lam:.2f}/s")
+
+    # open loop: Poisson arrivals at lam
+    reqs = draw(n_req)
+    gaps = rng.exponential(1.0 / lam, n_req)
+    ids = [None] * n_req
+    s0 = srv.stats()["pairs"]
+    t_start = time.perf_counter()
+    t_next = t_start
+
+    def submitter():
+        nonlocal t_next
+        for k, r in enumerate(reqs):
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = submit(r)
+            t_next += gaps[k]
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, queue = [], []
+    for k in range(n_req):
+        while ids[k] is None:
+            time.sleep(1e-4)
+        _, l_ms, q_ms = srv.wait(ids[k])
+        lat.append(l_ms)
+        queue.append(q_ms)
+    th.join()
+    elapsed = time.perf_counter() - t_start
+    st = srv.stats()
+    pairs = st["pairs"] - s0
+    log(f"{n_req} requests in {elapsed:.2f} s; peak in flight {st['peak_in_flight']}, peak admitted "
+        f"{st['peak_gb']:.1f} GiB; queue p50 {np.percentile(queue, 50):.1f} ms")
+    res = dict(elapsed=elapsed, lat_ms=np.array(lat), pairs=pairs / n_req, profiles_per_step=1, kernel_ms=None,
+               items=0, steps_override=n_req,
+               extra={"requests": n_req, "arrival_rate_per_s": round(lam, 3), "service_rate_per_s": round(mu, 3),
+                      "slots": cfg["slots"], "memory_budget_gb": cfg["budget_gb"],
+                      "peak_in_flight": st["peak_in_flight"], "peak_admitted_gb": round(st["peak_gb"], 2),
+                      "queue_ms_p50": round(float(np.percentile(queue, 50)), 3), "q_points": len(q)})
+
+    def cpu(budget):
+        import oracle as O
+
+        th_ = host_threads()
+        done, cpu_pairs = 0, 0.0
+        t = time.perf_counter()
+        for r in reqs:
+            if time.perf_counter() - t > 0.7 * budget:
+                break
+            xyz, ff, w = pool[r[0]]
+            if r[1] == A.MODE_FFGRID and len(xyz) > 20000:
+                continue  # bounded sample: large grid requests take minutes on the host
+            if r[1] == A.MODE_FFGRID:
+                nx = len(O.grid_exv(xyz, ff, w)[0])
+                h = O.hist_ffgrid(xyz, ff, w, 0.25, 8000, th_)
+                O.debye_ffgrid(h, q, 0.25)
+                n = len(xyz) + len(w) + nx
+                cpu_pairs += n * (n - 1) / 2
+            else:
+                om = O.FFAVG if r[1] == A.MODE_FFAVG else O.FFEXPLICIT
+                h = O.hist_ff(om, O.pack_ff(xyz, ff), O.pack_ff(w, np.ones(len(w), np.int32)), 0.25, 8000, 1.0, th_)
+                O.debye_ff(om, h, q, 0.25)
+                cpu_pairs += pairs_of(len(xyz), len(w))
+            done += 1
+        dt = time.perf_counter() - t
+        return dict(value=cpu_pairs / dt / 1e9, unit="Gpair-evals/s", cores=th_, kind="port",
+                    sample=f"{done} requests of the arrival sequence (grid requests above 20k atoms skipped), "
+                           f"{cpu_pairs:.3g} pairs, {dt:.2f} s, {done / dt:.2f} profiles/s")
+
+    return res, cpu
+
+
 def timed_steps(step, args):
     for _ in range(args.warmup):
         step()
@@ -271,6 +395,7 @@ def main():
                     help="rigid scan: use --tile-j and one workgroup per item instead of the autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU work for the baseline sample")
+    ap.add_argument("--requests", type=int, default=400, help="serving: requests in the timed open-loop run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -293,14 +418,16 @@ def main():
 
     cfg = CONFIGS[args.config]
     log("device:", A.device_info())
-    fn = {"structure": bench_structure, "grid": bench_grid, "rigid": bench_rigid}[cfg["kind"]]
+    fn = {"structure": bench_structure, "grid": bench_grid, "rigid": bench_rigid, "serve": bench_serve}[cfg["kind"]]
     res, cpu_fn = fn(cfg, args)
     elapsed, lat_ms, pairs = res["elapsed"], res["lat_ms"], res["pairs"]
+    if "steps_override" in res:  # serving: one step = one request of the open-loop run
+        args.steps = res["steps_override"]
     value = pairs * args.steps / elapsed / 1e9
     log(f"steps {args.steps}: {elapsed:.3f} s, {pairs:.4g} pairs per step, items {res['items']}")
 
     roofline = None
-    if res["kernel_ms"]:
+    if res["kernel_ms"] and cfg["kind"] != "serve":
         kern_s = res["kernel_ms"] / 1e3
         achieved = pairs * FLOP_PER_PAIR / kern_s / 1e12
         roofline = {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path; 157.3 TF is the FP32 peak)",
@@ -308,6 +435,11 @@ def main():
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                     "traffic": traffic_from_profiles() if args.config == 3 else None,
                     "kernel_ms": round(res["kernel_ms"], 4), "pairs_per_s": round(pairs / kern_s, 1)}
+    elif cfg["kind"] == "serve":
+        achieved = value * 1e9 * FLOP_PER_PAIR / 1e12
+        roofline = {"bound": "mfma", "engine": "fp32 VALU; whole serving window at the offered load, idle included",
+                    "kernel": "pair_hist_kernel (serving)", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None}
     else:
         # whole graph launch (transform, pair, merge, strips, finalize, Debye) per step
         g_s = res["extra"]["graph_ms"] / 1e3
