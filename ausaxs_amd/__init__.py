@@ -92,6 +92,8 @@ def _load():
         "ausaxs_server_stats": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong), P(C.c_longlong), P(C.c_int),
                                           P(C.c_int), P(C.c_double), P(C.c_longlong)]),
         "ausaxs_server_free": (C.c_int, [vp]),
+        "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
+        "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_synth_globule": (
             C.c_int,
@@ -473,3 +475,25 @@ class Server:
                                        C.byref(pairs)))
         return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
                     peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value)
+
+
+def lsq(x, y, err=None):
+    """FP64 fits y ~ a x + b on the device for one model curve x[n] or many
+    x[m][n]: rows of (a, b, var a, var b, chi2)."""
+    x = np.ascontiguousarray(x, np.float64)
+    single = x.ndim == 1
+    x = x.reshape(1, -1) if single else x
+    y = np.ascontiguousarray(y, np.float64)
+    e = None if err is None else np.ascontiguousarray(err, np.float64)
+    out = np.zeros((len(x), 5))
+    _check(lib.ausaxs_lsq(_ptr(x), _ptr(y), _ptr(e), x.shape[1], len(x), _ptr(out)))
+    return out[0] if single else out
+
+
+def exact_debye(xyzw, q):
+    """hist::exact_debye_transform on the device (XYZW points)."""
+    p = _pts(xyzw)
+    q = np.ascontiguousarray(q, np.float64)
+    I = np.zeros(len(q))
+    _check(lib.ausaxs_exact_debye(_ptr(p), len(p), _ptr(q), len(q), _ptr(I)))
+    return I

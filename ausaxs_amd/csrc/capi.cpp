@@ -464,4 +464,40 @@ int ausaxs_server_free(ausaxs_server* s) {
     return guarded([&] { delete s; });
 }
 
+int ausaxs_lsq(const double* x, const double* y, const double* err, int n, int n_models, double* out) {
+    return guarded([&] {
+        require(n >= 3 && n_models >= 0 && (n_models == 0 || (x && y && out)), "bad argument (n >= 3 points)");
+        Context& ctx = Context::get();
+        DevBuf<double> dx, dy, de, dout;
+        dx.upload(x, (size_t)n_models * n, ctx.stream());
+        dy.upload(y, n, ctx.stream());
+        if (err) de.upload(err, n, ctx.stream());
+        dout.ensure((size_t)n_models * 5);
+        check(launch_lsq(dx.p, dy.p, err ? de.p : nullptr, n, n_models, dout.p, ctx.stream()), "lsq_kernel");
+        check(hipMemcpyAsync(out, dout.p, (size_t)n_models * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx.stream()),
+              "D2H");
+        ctx.sync();
+    });
+}
+
+int ausaxs_exact_debye(const float* xyzw, int n, const double* q, int nq, double* I) {
+    return guarded([&] {
+        require(n >= 0 && nq >= 0 && (n == 0 || xyzw) && (nq == 0 || (q && I)), "bad argument");
+        if (nq == 0) return;
+        Context& ctx = Context::get();
+        DevBuf<float4> dp;
+        DevBuf<double> dq, dpart, dout;
+        std::vector<float4> pts(n);
+        for (int i = 0; i < n; ++i) pts[i] = make_float4(xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2], xyzw[4 * i + 3]);
+        dp.upload(pts.data(), pts.size(), ctx.stream());
+        dq.upload(q, nq, ctx.stream());
+        dpart.ensure((size_t)nq * std::max(1, exact_debye_blocks(n)));
+        dout.ensure(nq);
+        check(launch_exact_debye(dp.p, n, dq.p, nq, dpart.p, dout.p, ctx.stream()), "exact_debye");
+        check(hipMemcpyAsync(I, dout.p, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+        ctx.sync();
+    });
+}
+
 }  // extern "C"

@@ -133,4 +133,14 @@ hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s);
 // Plain FP64 contraction (same operands), used as the device reference path.
 hipError_t launch_debye_f64(const DebyeLaunch& p, hipStream_t s);
 
+// FP64 weighted least squares y ~ a x + b for n_models curves x[m][n]
+// against one data set y[n] (err may be NULL: unit errors); out[m][5] =
+// a, b, S/delta, Sxx/delta, chi2.
+hipError_t launch_lsq(const double* x, const double* y, const double* err, int n, int n_models, double* out,
+                      hipStream_t s);
+// Exact O(N^2) Debye sum; partial: [nq][exact_debye_blocks(n)] workspace.
+int exact_debye_blocks(int n);
+hipError_t launch_exact_debye(const float4* pts, int n, const double* q, int nq, double* partial, double* out,
+                              hipStream_t s);
+
 }  // namespace ausaxs::gpu

@@ -198,6 +198,21 @@ int ausaxs_server_free(ausaxs_server* s);
  * Axis.cpp:40-59). q must hold 200 doubles; *nq receives the length. */
 int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq);
 
+/* ---- fitting and validation --------------------------------------------- */
+
+/* LinearLeastSquares::fit_params_only (fitter/detail/LinearLeastSquares.cpp:26-44)
+ * and Fitter::chi2 (fitter/Fitter.cpp:10-13) in FP64 on the device, for
+ * n_models model curves x[n_models][n] against one data set y[n] with errors
+ * err[n] (NULL: unit errors): y ~ a x + b. out[n_models][5] = a, b, S/delta,
+ * Sxx/delta, chi2. Scores a whole rigid-body scan against data in one call. */
+int ausaxs_lsq(const double* x, const double* y, const double* err, int n, int n_models, double* out);
+
+/* hist::exact_debye_transform (intensity_calculator/ExactDebyeCalculator.cpp:13-57):
+ * I(q) = exp(-q^2) [sum_i w_i^2 + sum_{i<j} 2 w_i w_j sinc(q d_ij)] over XYZW
+ * points (n x 4 floats), distances with the reference's evaluate_8/4/1 rules,
+ * FP64 sums. */
+int ausaxs_exact_debye(const float* xyzw, int n, const double* q, int nq, double* I);
+
 /* ---- kernel-level entries (tests, tools) -------------------------------- */
 
 /* One pair set with the reference's loop geometry (self if b == NULL, else

@@ -186,7 +186,7 @@ def lsq(x, y, err=None):
     x = np.ascontiguousarray(x, np.float64)
     y = np.ascontiguousarray(y, np.float64)
     e = None if err is None else np.ascontiguousarray(err, np.float64)
-    out = np.zeros(4, np.float64)
+    out = np.zeros(5, np.float64)
     lib.orc_lsq(_p(x), _p(y), _p(e), len(x), _p(out))
     return out
 

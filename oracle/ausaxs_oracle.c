@@ -1043,4 +1043,13 @@ void orc_lsq(const double* x, const double* y, const double* err, int n, double*
     out[1] = (Sxx * Sy - Sx * Sxy) / delta;
     out[2] = S / delta;
     out[3] = Sxx / delta;
+    /* Fitter::chi2 (Fitter.cpp:10-13) over LinearLeastSquares::get_residuals
+     * (LinearLeastSquares.cpp:71-78): sum of ((y - (a x + b)) / err)^2 */
+    double c = 0;
+    for (int i = 0; i < n; ++i) {
+        double is = err ? 1. / err[i] : 1.;
+        double r = (y[i] - (out[0] * x[i] + out[1])) * is;
+        c += r * r;
+    }
+    out[4] = c;
 }

@@ -162,7 +162,8 @@ void orc_total(const double* profiles, int nq, double* out);
 void orc_exact_debye(const float* xyzw, int n, const double* q, int nq, double* out);
 
 /* LinearLeastSquares::fit_params_only (LinearLeastSquares.cpp:26-44):
- * weighted fit y ~ a*x + b with weights 1/err^2. out[0]=a, out[1]=b. */
+ * weighted fit y ~ a*x + b with weights 1/err^2. out[0]=a, out[1]=b,
+ * out[2]=S/delta, out[3]=Sxx/delta, out[4]=chi2 (Fitter.cpp:10-13). */
 void orc_lsq(const double* x, const double* y, const double* err, int n, double* out);
 
 #ifdef __cplusplus

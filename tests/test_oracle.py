@@ -310,8 +310,28 @@ def test_golden_carbon_sphere(golden_carbon):
 def test_lsq_closed_form():
     x = np.linspace(1, 10, 50)
     y = 2.5 * x - 3.0
-    a, b, *_ = O.lsq(x, y, np.full(50, 0.3))
-    assert np.isclose(a, 2.5) and np.isclose(b, -3.0)
+    a, b, va, vb, chi2 = O.lsq(x, y, np.full(50, 0.3))
+    assert np.isclose(a, 2.5) and np.isclose(b, -3.0) and chi2 < 1e-20
+    # unit errors: var(a) = n / (n Sxx - Sx^2)
+    _, _, va1, vb1, _ = O.lsq(x, y)
+    n, sx, sxx = len(x), x.sum(), (x * x).sum()
+    assert np.isclose(va1, n / (n * sxx - sx * sx)) and np.isclose(vb1, sxx / (n * sxx - sx * sx))
+
+
+def test_lsq_matches_weighted_lstsq():
+    # an independent solver: weighted linear least squares by numpy's lstsq;
+    # chi2 is the minimised weighted residual norm
+    rng = np.random.default_rng(2)
+    x = rng.uniform(1, 100, 80)
+    err = rng.uniform(0.5, 2.0, 80)
+    y = 0.7 * x + 4.0 + rng.normal(0, 1, 80) * err
+    A = np.stack([x, np.ones_like(x)], 1) / err[:, None]
+    sol, res, *_ = np.linalg.lstsq(A, y / err, rcond=None)
+    a, b, va, vb, chi2 = O.lsq(x, y, err)
+    assert np.allclose([a, b], sol, rtol=1e-10)
+    assert np.isclose(chi2, res[0], rtol=1e-9)
+    cov = np.linalg.inv(A.T @ A)
+    assert np.isclose(va, cov[0, 0], rtol=1e-9) and np.isclose(vb, cov[1, 1], rtol=1e-9)
 
 
 def test_threads_do_not_change_counts():
