@@ -15,6 +15,8 @@
 // operand (the FP32-level split); terms = 2 is the hi+lo split.
 #include "device_api.hpp"
 
+#include <algorithm>
+
 namespace ausaxs::gpu {
 namespace {
 
@@ -76,12 +78,14 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
     return __builtin_bit_cast(bf16x8, v);
 }
 
-// One wave per 16 (q) x 16 (rows) output tile. Lane l feeds row/column l%16
-// and K sub-block l/16 (8 consecutive bf16); the accumulator holds
-// D[4*(l/16)+i][l%16].
+// One wave per 16 (q) x 16 (rows) output tile and K chunk (blockIdx.z): the
+// chunks of one tile are summed by debye_reduce_kernel in a fixed order, so
+// enough waves run to fill the chip while the result stays deterministic.
+// Lane l feeds row/column l%16 and K sub-block l/16 (8 consecutive bf16);
+// the accumulator holds D[4*(l/16)+i][l%16].
 __global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
                                                         int nq, int rows, int nq_pad, int rows_pad, int K_pad,
-                                                        int terms, double* __restrict__ out) {
+                                                        int terms, int k_steps, double* __restrict__ part) {
     const int l = threadIdx.x;
     const int q0 = blockIdx.x * 16, r0 = blockIdx.y * 16;
     const int m = l & 15, kg = l >> 4;
@@ -89,7 +93,8 @@ __global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restri
     const uint16_t* arow = A + (size_t)(q0 + m) * K_pad + kg * 8;
     const uint16_t* brow = B + (size_t)(r0 + m) * K_pad + kg * 8;
     double acc64[4] = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < K_pad; k0 += 32) {
+    const int kb = blockIdx.z * k_steps * 32, ke = min(K_pad, kb + k_steps * 32);
+    for (int k0 = kb; k0 < ke; k0 += 32) {
         bf16x8 a[kMaxTerms], b[kMaxTerms];
         for (int t = 0; t < terms; ++t) {
             a[t] = load8(arow + t * pa + k0);
@@ -110,12 +115,21 @@ __global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restri
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc64[i] += (double)acc[i];
     }
+    double* dst = part + (size_t)blockIdx.z * rows_pad * nq_pad;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int qk = q0 + 4 * kg + i;
-        const int r = r0 + m;
-        if (qk < nq && r < rows) out[(size_t)r * nq + qk] = acc64[i];
-    }
+    for (int i = 0; i < 4; ++i) dst[(size_t)(r0 + m) * nq_pad + q0 + 4 * kg + i] = acc64[i];
+}
+
+// out[r][k] = sum over K chunks (in chunk order) of the partial tiles
+__global__ __launch_bounds__(256) void debye_reduce_kernel(const double* __restrict__ part, int chunks, int rows,
+                                                           int nq, int rows_pad, int nq_pad,
+                                                           double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * nq) return;
+    const int r = e / nq, k = e % nq;
+    double s = 0;
+    for (int c = 0; c < chunks; ++c) s += part[((size_t)c * rows_pad + r) * nq_pad + k];
+    out[e] = s;
 }
 
 __global__ __launch_bounds__(256) void debye_f64_kernel(const double* __restrict__ H, const double* __restrict__ d,
@@ -143,9 +157,18 @@ inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
+// K chunks per output tile: enough waves for ~8 per CU on 256 CUs
+int k_chunks(int rows, int K, int nq) {
+    const int tiles = (pad_to(nq, 16) / 16) * (pad_to(rows, 16) / 16);
+    const int steps = pad_to(K, 32) / 32;
+    return std::max(1, std::min(steps, (2048 + tiles - 1) / std::max(tiles, 1)));
+}
+
 size_t debye_scratch_bytes(int rows, int K, int nq) {
     const size_t K_pad = pad_to(K, 32), nq_pad = pad_to(nq, 16), rows_pad = pad_to(rows, 16);
-    return (size_t)kMaxTerms * (nq_pad + rows_pad) * K_pad * sizeof(uint16_t);
+    const size_t split = (size_t)kMaxTerms * (nq_pad + rows_pad) * K_pad * sizeof(uint16_t);
+    return (split + 255) / 256 * 256 +
+           (size_t)k_chunks(rows, K, nq) * rows_pad * nq_pad * sizeof(double);
 }
 
 hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s) {
@@ -154,13 +177,22 @@ hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s) {
     const int K_pad = pad_to(p.K, 32), nq_pad = pad_to(p.nq, 16), rows_pad = pad_to(p.rows, 16);
     uint16_t* A = static_cast<uint16_t*>(p.scratch);
     uint16_t* B = A + (size_t)kMaxTerms * nq_pad * K_pad;
+    const size_t split = (size_t)kMaxTerms * (nq_pad + rows_pad) * K_pad * sizeof(uint16_t);
+    double* part = reinterpret_cast<double*>(static_cast<char*>(p.scratch) + (split + 255) / 256 * 256);
     const size_t na = (size_t)nq_pad * K_pad, nb = (size_t)rows_pad * K_pad;
+    const int chunks = k_chunks(p.rows, p.K, p.nq);
+    const int steps = K_pad / 32;
+    const int per = (steps + chunks - 1) / chunks;
+    const int used = (steps + per - 1) / per;  // chunks that hold at least one K step
     hipLaunchKernelGGL(sinc_split_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, p.q, p.d, p.nq, p.K,
                        nq_pad, K_pad, p.terms, A);
     hipLaunchKernelGGL(h_split_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, p.H, p.rows, p.K, rows_pad,
                        K_pad, p.terms, B);
-    hipLaunchKernelGGL(debye_mfma_kernel, dim3(nq_pad / 16, rows_pad / 16), dim3(64), 0, s, A, B, p.nq, p.rows, nq_pad,
-                       rows_pad, K_pad, p.terms, p.out);
+    hipLaunchKernelGGL(debye_mfma_kernel, dim3(nq_pad / 16, rows_pad / 16, used), dim3(64), 0, s, A, B, p.nq, p.rows,
+                       nq_pad, rows_pad, K_pad, p.terms, per, part);
+    const int n = p.rows * p.nq;
+    hipLaunchKernelGGL(debye_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, used, p.rows, p.nq,
+                       rows_pad, nq_pad, p.out);
     return hipGetLastError();
 }
 

@@ -10,20 +10,21 @@
 
 namespace ausaxs::gpu {
 
-// Fixed-point encoding of one LDS/partial histogram cell (one uint64):
-//   bits 42..63  pair count (22 bits)
-//   bits  0..41  sum over pairs of (q + 2^19), q = round(delta * 2^20),
-//                delta = x - rint(x), x = d * inv_width (bin units)
-// A work item holds < 2^22 pairs, so neither field can overflow (each pair
-// adds at most 2^20 to the low field). The mean distance of a bin is then
-// known to 2^-21 bins even when every pair in it shares one delta (lattices,
-// symmetric test bodies). Integer accumulation keeps every histogram
-// bit-reproducible.
+// Fixed-point encoding of one LDS/partial histogram cell (one uint64). Each
+// pair adds kCellUnit + k, with k = round(delta * 2^20) + 2^19 in [0, 2^20],
+// delta = x - rint(x), x = d * inv_width (bin units). The low word of the
+// addend is the bit pattern of the float fma(delta, 2^20, 1.5625 * 2^23),
+// kLowBias + k, so the kernel adds it without a float-to-int conversion. A
+// cell holding n pairs is T = n * (2^42 + kLowBias) + sum k with sum k <= n * 2^20
+// < 2^42 + kLowBias, so n = floor(T / (2^42 + kLowBias)) and T stays below
+// 2^64 while n <= kMaxPairsPerItem. The mean distance of a bin is known to
+// 2^-21 bins even when every pair in it shares one delta (lattices, symmetric
+// test bodies). Integer accumulation keeps every histogram bit-reproducible.
 constexpr int kDeltaBits = 20;
-constexpr uint64_t kCountShift = 42;
-constexpr uint64_t kLowMask = (uint64_t(1) << kCountShift) - 1;
+constexpr uint64_t kLowBias = 0x4B400000u;  // bits of 1.5 * 2^23
+constexpr uint64_t kCellUnit = (uint64_t(1) << 42) + kLowBias;
 constexpr int64_t kDeltaOffset = int64_t(1) << (kDeltaBits - 1);
-constexpr int64_t kMaxPairsPerItem = (int64_t(1) << 22) - 1;
+constexpr int64_t kMaxPairsPerItem = 4193000;  // 4193000 * (kCellUnit + 2^20) < 2^64
 constexpr double kDeltaScale = 1.0 / (double)(int64_t(1) << kDeltaBits);
 
 // One workgroup's rectangle of the pair space.

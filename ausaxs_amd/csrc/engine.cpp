@@ -60,7 +60,7 @@ int bin_bound(const std::vector<const PointSet*>& sets, float inv_w) {
 
 void check_tile_j(int tile_j) {
     if (tile_j < 1 || (int64_t)kTileI * tile_j > kMaxPairsPerItem)
-        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item below 2^22 pairs (tile_j <= 8191)");
+        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item within 4,193,000 pairs (tile_j <= 8189)");
 }
 
 int item_rows(const std::vector<PairWork>& items) {

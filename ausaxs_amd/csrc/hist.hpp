@@ -39,7 +39,7 @@ struct HistSettings {
     uint32_t bin_count = 8000;  // settings::axes::bin_count
     bool weighted_bins = true;  // settings::hist::weighted_bins
     int debye_terms = 3;        // bf16 terms per operand; 0 = FP64 contraction
-    int tile_j = 8000;          // columns per work item (<= 8191)
+    int tile_j = 8000;          // columns per work item (<= 8189)
 };
 // WidthControllers.h:10-27: inverse width as float
 float inv_width(const HistSettings& s);

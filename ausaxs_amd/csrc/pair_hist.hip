@@ -86,13 +86,26 @@ __device__ __forceinline__ void tie_fix(float d2, float inv_w, float& x, float& 
     dl = x - b;
 }
 
-// q + 2^19 with q = round(dl * 2^20); dl in [-0.5, 0.5] -> [0, 2^20]
+// kLowBias + k with k = rne(dl * 2^20) + 2^19: dl * 2^20 + 1.5625 * 2^23 lies in
+// [1.5 * 2^23, 1.625 * 2^23], where floats are the integers, so the FMA rounds
+// dl * 2^20 to an integer once and the bit pattern is 0x4B400000 + k
 __device__ __forceinline__ uint32_t quantize(float dl) {
-    return (uint32_t)__builtin_fmaf(dl, 1048576.0f, 524288.5f);
+    return __float_as_uint(__builtin_fmaf(dl, 1048576.0f, 13107200.0f));
 }
 
 __device__ __forceinline__ uint64_t cell_add(float dl) {
-    return (uint64_t(1) << kCountShift) | (uint64_t)quantize(dl);
+    return (uint64_t(1) << 42) + (uint64_t)quantize(dl);
+}
+
+// count and sum of k of one cell (see device_api.hpp)
+__device__ __forceinline__ void cell_decode(uint64_t v, uint64_t& n, uint64_t& ksum) {
+    const double inv = 1.0 / (double)kCellUnit;
+    uint64_t c = (uint64_t)((double)v * inv);
+    // the estimate is within one of floor(v / kCellUnit)
+    if (c * kCellUnit > v) --c;
+    else if (v - c * kCellUnit >= kCellUnit) ++c;
+    n = c;
+    ksum = v - c * kCellUnit;
 }
 
 __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
@@ -269,9 +282,10 @@ __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __res
     const int e0 = chunk_ptr[c], e1 = chunk_ptr[c + 1];
 #pragma unroll 8
     for (int e = e0; e < e1; ++e) {
-        const uint64_t v = partial[(size_t)row_list[e] * stride + bin];
-        n_sum += v >> kCountShift;
-        low_sum += v & kLowMask;
+        uint64_t n, k;
+        cell_decode(partial[(size_t)row_list[e] * stride + bin], n, k);
+        n_sum += n;
+        low_sum += k;
     }
     tcnt[(size_t)c * stride + bin] = n_sum;
     tdsum[(size_t)c * stride + bin] = (int64_t)low_sum - (int64_t)n_sum * kDeltaOffset;
@@ -329,7 +343,7 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     if (row < 0) return;  // the host guarantees a row for every present class pair
     row += rep * p.out_rep;
     const unsigned bin = min((unsigned)(int)b, (unsigned)p.nbins);
-    const uint32_t low = quantize(dl);
+    const uint32_t low = quantize(dl) - (uint32_t)kLowBias;  // k in [0, 2^20]
     const size_t cell = (size_t)row * p.stride + bin;
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
     atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)low - kDeltaOffset));

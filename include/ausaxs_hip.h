@@ -43,7 +43,7 @@ typedef struct {
     int bin_count;     /* settings::axes::bin_count (default 8000)  */
     int weighted_bins; /* settings::hist::weighted_bins (default 1) */
     int debye_terms;   /* bf16 terms per GEMM operand: 2 = hi+lo, 3 = hi+mid+lo (default); 0 = FP64 */
-    int tile_j;        /* columns per pair work item (default 8000, at most 8191) */
+    int tile_j;        /* columns per pair work item (default 8000, at most 8189) */
 } ausaxs_settings;
 
 typedef struct ausaxs_structure ausaxs_structure; /* resident coordinates + work lists */

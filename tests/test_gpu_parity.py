@@ -180,7 +180,7 @@ def test_cube_known_counts():
 def test_determinism_across_tilings():
     atoms, waters = config1(3000, 500)
     outs = []
-    for tj in (256, 1000, 8191):
+    for tj in (256, 1000, 8189):
         h = A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters, settings=A.default_settings(tile_j=tj))
         outs.append([h.get(w) for w in (A.AA, A.AW, A.WW)])
     for o in outs[1:]:
