@@ -214,9 +214,16 @@ int ausaxs_histogram_get(const ausaxs_histogram* h, int which, double* value, ui
         require(h != nullptr, "NULL histogram");
         const CompositeDistanceHistogramHIP& c = *h->h;
         const std::vector<WeightedEntry>* v = nullptr;
+        std::vector<WeightedEntry> tmp;
         switch (which) {
-            case AUSAXS_AA: v = &c.aa; break;
-            case AUSAXS_AW: v = &c.aw; break;
+            case AUSAXS_AA:
+            case AUSAXS_AW: {
+                const EntryRows& e = which == AUSAXS_AA ? c.aa : c.aw;
+                tmp.resize(e.size());
+                e.dense(tmp.data());
+                v = &tmp;
+                break;
+            }
             case AUSAXS_WW: v = &c.ww; break;
             case AUSAXS_PTOT: v = &c.p_tot; break;
             case AUSAXS_PTOT_AX: v = &c.p_tot_ax; break;

@@ -151,6 +151,51 @@ void entry_add(WeightedEntry& a, const WeightedEntry& b) {
     a.bin_center += b.bin_center;
 }
 
+void EntryRows::reset(int n1, int n2, int width) {
+    n1_ = n1, n2_ = n2, width_ = width;
+    slot_.assign((size_t)n1 * n2, -1);
+    rows_.clear();
+}
+
+const WeightedEntry* EntryRows::row(int a, int b) const {
+    const int32_t k = slot_[(size_t)a * n2_ + b];
+    return k < 0 ? nullptr : rows_[k].data();
+}
+
+WeightedEntry* EntryRows::row(int a, int b) {
+    const int32_t k = slot_[(size_t)a * n2_ + b];
+    return k < 0 ? nullptr : rows_[k].data();
+}
+
+WeightedEntry* EntryRows::touch(int a, int b) {
+    int32_t& k = slot_[(size_t)a * n2_ + b];
+    if (k < 0) {
+        k = (int32_t)rows_.size();
+        rows_.emplace_back((size_t)width_);
+    }
+    return rows_[k].data();
+}
+
+void EntryRows::clear(int a, int b) {
+    int32_t& k = slot_[(size_t)a * n2_ + b];
+    if (k >= 0) std::fill(rows_[k].begin(), rows_[k].end(), WeightedEntry{});
+    k = -1;
+}
+
+void EntryRows::truncate(int width) {
+    width_ = width;
+    for (auto& r : rows_) r.resize((size_t)width);
+}
+
+void EntryRows::dense(WeightedEntry* out) const {
+    for (int a = 0; a < n1_; ++a)
+        for (int b = 0; b < n2_; ++b) {
+            WeightedEntry* dst = out + ((size_t)a * n2_ + b) * width_;
+            if (const WeightedEntry* r = row(a, b)) std::copy_n(r, width_, dst);
+            else std::fill_n(dst, width_, WeightedEntry{});
+        }
+}
+
 namespace {
 
 inline void scale(WeightedEntry& e, double f) {  // WeightedEntry.h operator*
@@ -369,56 +414,83 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         // (HistogramManagerMTFFGrid.cpp:47): FFAvg conventions, always weighted.
         const bool avg = mode_ != Mode::FFExplicit;
         const double z = mode_ == Mode::FFGrid ? (n_atoms_ ? grid_volume_ * 0.334 / n_atoms_ : 0.0) : z_exv_;
-        std::vector<WeightedEntry> aa((size_t)kMaxFF * kMaxFF * nb), aw((size_t)kMaxFF * nb), ww(nb);
-        std::vector<WeightedEntry> xx, ax, wx;
-        if (mode_ == Mode::FFGrid) xx.resize(nb), ax.resize((size_t)kMaxFF * nb), wx.resize(nb);
-        auto A3 = [&](int f1, int f2, int b) -> WeightedEntry& { return aa[((size_t)f1 * kMaxFF + f2) * nb + b]; };
-        auto A2 = [&](int f, int b) -> WeightedEntry& { return aw[(size_t)f * nb + b]; };
+        // working width: bins up to the last non-empty one of any row
+        int nbw = 10;
+        for (size_t r = 0; r < rows_.size(); ++r)
+            for (int b = nbins - 1; b >= nbw; --b)
+                if (cnt_[r * stride + b]) {
+                    nbw = b + 1;
+                    break;
+                }
+        nbw = std::min(nbw, nbins);
+        const size_t nbv = nbw;
+        EntryRows aa, aw, ax;
+        aa.reset(kMaxFF, kMaxFF, nbw);
+        aw.reset(kMaxFF, 1, nbw);
+        ax.reset(kMaxFF, 1, nbw);
+        std::vector<WeightedEntry> ww(nbv), xx, wx;
+        if (mode_ == Mode::FFGrid) xx.resize(nbv), wx.resize(nbv);
         for (size_t r = 0; r < rows_.size(); ++r) {
             const auto& k = rows_[r];
-            for (int b = 0; b < nbins; ++b) {
-                switch (k[0]) {
-                    case 0: A3(k[1], k[2], b) = entry((int)r, b, 2); break;
-                    case 1: A2(k[1], b) = entry((int)r, b, 1); break;
-                    case 2: ww[b] = entry((int)r, b, 2); break;
-                    case 3: xx[b] = entry((int)r, b, 2); break;
-                    case 4: ax[(size_t)k[1] * nb + b] = entry((int)r, b, 1); break;
-                    default: wx[b] = entry((int)r, b, 1); break;
-                }
+            WeightedEntry* dst = nullptr;
+            int factor = 2;
+            switch (k[0]) {
+                case 0: dst = aa.touch(k[1], k[2]); break;
+                case 1: dst = aw.touch(k[1], 0), factor = 1; break;
+                case 2: dst = ww.data(); break;
+                case 3: dst = xx.data(); break;
+                case 4: dst = ax.touch(k[1], 0), factor = 1; break;
+                default: dst = wx.data(), factor = 1; break;
             }
+            for (int b = 0; b < nbw; ++b)
+                if (cnt_[r * stride + b]) dst[b] = entry((int)r, b, factor);
         }
         if (avg) {
             // HistogramManagerMTFFAvg.cpp:27-34,111-118: every aa pair also lands in
             // [ff_i][exv] (ff of the lower original index) and [exv][exv]; every
             // aw pair in aw[exv]. Derived from the ordered cells.
+            std::vector<std::array<int, 2>> keys;
+            aa.for_each([&](int f, int g, const WeightedEntry*) {
+                if (f >= 1 && g >= 1) keys.push_back({f, g});
+            });
+            WeightedEntry* xxrow = aa.touch(kExvBin, kExvBin);
+            for (auto [f, g] : keys) {
+                const WeightedEntry* src = aa.row(f, g);
+                WeightedEntry* fx = aa.touch(f, kExvBin);
+                for (int b = 0; b < nbw; ++b) {
+                    add(fx[b], src[b]);
+                    add(xxrow[b], src[b]);
+                }
+            }
+            WeightedEntry* wexv = aw.touch(kExvBin, 0);
             for (int f = 1; f < kMaxFF; ++f)
-                for (int g = 1; g < kMaxFF; ++g)
-                    for (int b = 0; b < nbins; ++b) {
-                        add(A3(f, kExvBin, b), A3(f, g, b));
-                        add(A3(kExvBin, kExvBin, b), A3(f, g, b));
-                    }
-            for (int f = 1; f < kMaxFF; ++f)
-                for (int b = 0; b < nbins; ++b) add(A2(kExvBin, b), A2(f, b));
+                if (const WeightedEntry* src = aw.row(f, 0))
+                    for (int b = 0; b < nbw; ++b) add(wexv[b], src[b]);
         }
         // self correlations (FFAvg.cpp:308-323, FFExplicit.cpp:331-346)
-        for (int i = 0; i < n_atoms_; ++i) add(A3(atom_ff_[i], atom_ff_[i], 0), WeightedEntry{1, 1, 0});
-        if (avg) add(A3(kExvBin, kExvBin, 0), WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
+        for (int i = 0; i < n_atoms_; ++i) add(aa.touch(atom_ff_[i], atom_ff_[i])[0], WeightedEntry{1, 1, 0});
+        if (avg) add(aa.touch(kExvBin, kExvBin)[0], WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
         add(ww[0], WeightedEntry{(double)n_waters_, (uint32_t)n_waters_, 0});
 
-        std::vector<WeightedEntry> ptot(nb);
-        for (int f1 = 1; f1 < kMaxFF; ++f1)
-            for (int f2 = 1; f2 < kMaxFF; ++f2)
-                for (int b = 0; b < nbins; ++b) add(ptot[b], A3(f1, f2, b));
+        std::vector<WeightedEntry> ptot(nbv);
+        aa.for_each([&](int f1, int f2, const WeightedEntry* src) {
+            if (f1 >= 1 && f2 >= 1)
+                for (int b = 0; b < nbw; ++b) add(ptot[b], src[b]);
+        });
         for (int f = 1; f < kMaxFF; ++f)
-            for (int b = 0; b < nbins; ++b) add(ptot[b], A2(f, b));
-        for (int b = 0; b < nbins; ++b) add(ptot[b], ww[b]);
-        const unsigned base = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
+            if (const WeightedEntry* src = aw.row(f, 0))
+                for (int b = 0; b < nbw; ++b) add(ptot[b], src[b]);
+        for (int b = 0; b < nbw; ++b) add(ptot[b], ww[b]);
+        const unsigned base = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbw);
 
         if (avg) {  // FFAvg.cpp:355-364
             for (int f = 1; f < kMaxFF; ++f)
-                for (unsigned b = 0; b < base; ++b) scale(A3(f, kExvBin, b), z);
-            for (unsigned b = 0; b < base; ++b) scale(A3(kExvBin, kExvBin, b), z * z);
-            for (unsigned b = 0; b < base; ++b) scale(A2(kExvBin, b), z);
+                if (WeightedEntry* r = aa.row(f, kExvBin))
+                    for (unsigned b = 0; b < base; ++b) scale(r[b], z);
+            if (WeightedEntry* r = aa.row(kExvBin, kExvBin))
+                for (unsigned b = 0; b < base; ++b) scale(r[b], z * z);
+            if (WeightedEntry* r = aw.row(kExvBin, 0))
+                for (unsigned b = 0; b < base; ++b) scale(r[b], z);
         }
         unsigned max_bin = base;
         ptot.resize(base);
@@ -426,7 +498,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             // HistogramManagerMTFFGrid.cpp:159-224
             add(xx[0], WeightedEntry{(double)n_exv_, (uint32_t)n_exv_, 0});
             unsigned mbx = 10;
-            for (int i = nbins - 1; i >= 10; --i)
+            for (int i = nbw - 1; i >= 10; --i)
                 if (xx[i].value != 0 || wx[i].value != 0) {
                     mbx = (unsigned)i + 1;
                     break;
@@ -436,15 +508,16 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             const std::vector<double> base_axis = axis_of(ptot, s_.bin_width, true);
             const double q0 = 1e-4;
             std::vector<double> wc(base, 0.0);
-            for (int f1 = 1; f1 < kMaxFF; ++f1)
-                for (int f2 = 1; f2 < kMaxFF; ++f2) {
-                    const double w = ff_atomic(f1, q0) * ff_atomic(f2, q0);
-                    for (unsigned b = 0; b < base; ++b) wc[b] += A3(f1, f2, b).value * w;
+            aa.for_each([&](int f1, int f2, const WeightedEntry* src) {
+                if (f1 < 1 || f2 < 1) return;
+                const double w = ff_atomic(f1, q0) * ff_atomic(f2, q0);
+                for (unsigned b = 0; b < base; ++b) wc[b] += src[b].value * w;
+            });
+            for (int f = 1; f < kMaxFF; ++f)
+                if (const WeightedEntry* src = aw.row(f, 0)) {
+                    const double w = 2 * 1.0 * (ff_atomic(f, q0) * ff_atomic(kWaterBin, q0));
+                    for (unsigned b = 0; b < base; ++b) wc[b] += src[b].value * w;
                 }
-            for (int f = 1; f < kMaxFF; ++f) {
-                const double w = 2 * 1.0 * (ff_atomic(f, q0) * ff_atomic(kWaterBin, q0));
-                for (unsigned b = 0; b < base; ++b) wc[b] += A2(f, b).value * w;
-            }
             {
                 const double w = 1.0 * 1.0 * (ff_atomic(kWaterBin, q0) * ff_atomic(kWaterBin, q0));
                 for (unsigned b = 0; b < base; ++b) wc[b] += ww[b].value * w;
@@ -455,18 +528,23 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             // FFAvg rows beyond the FFAvg size are zero; exv rows become ax / wx / xx
             for (int f1 = 1; f1 < kMaxFF; ++f1) {
                 for (int f2 = 1; f2 < kMaxFF; ++f2)
-                    for (unsigned b = base; b < max_bin; ++b) A3(f1, f2, b) = WeightedEntry{};
-                for (unsigned b = base; b < max_bin; ++b) A2(f1, b) = WeightedEntry{};
+                    if (WeightedEntry* r = aa.row(f1, f2))
+                        for (unsigned b = base; b < max_bin; ++b) r[b] = WeightedEntry{};
+                if (WeightedEntry* r = aw.row(f1, 0))
+                    for (unsigned b = base; b < max_bin; ++b) r[b] = WeightedEntry{};
             }
             for (unsigned b = base; b < max_bin; ++b) ww[b] = WeightedEntry{};
             std::vector<WeightedEntry> pax((size_t)max_bin);
             for (unsigned b = 0; b < max_bin; ++b) add(pax[b], wx[b]);
             for (int f = 0; f < kMaxFF; ++f)
-                for (unsigned b = 0; b < max_bin; ++b) add(pax[b], ax[(size_t)f * nb + b]);
-            for (int f = 0; f < kMaxFF; ++f)
-                for (unsigned b = 0; b < max_bin; ++b) A3(f, kExvBin, b) = ax[(size_t)f * nb + b];
-            for (unsigned b = 0; b < max_bin; ++b) A2(kExvBin, b) = wx[b];
-            for (unsigned b = 0; b < max_bin; ++b) A3(kExvBin, kExvBin, b) = xx[b];
+                if (const WeightedEntry* src = ax.row(f, 0))
+                    for (unsigned b = 0; b < max_bin; ++b) add(pax[b], src[b]);
+            for (int f = 0; f < kMaxFF; ++f) {
+                if (const WeightedEntry* src = ax.row(f, 0)) std::copy_n(src, max_bin, aa.touch(f, kExvBin));
+                else aa.clear(f, kExvBin);
+            }
+            std::copy_n(wx.data(), max_bin, aw.touch(kExvBin, 0));
+            std::copy_n(xx.data(), max_bin, aa.touch(kExvBin, kExvBin));
             // both axes live on histograms of bin_count entries (WeightedDistribution1D
             // of the manager), and neither has its first bin forced to zero
             auto axis_full = [&](const std::vector<WeightedEntry>& p) {
@@ -487,13 +565,10 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             ptot = std::move(grid_ptot);
         }
         res->max_bin = max_bin;
-        res->aa.resize((size_t)kMaxFF * kMaxFF * max_bin);
-        res->aw.resize((size_t)kMaxFF * max_bin);
-        for (int f1 = 0; f1 < kMaxFF; ++f1) {
-            for (int f2 = 0; f2 < kMaxFF; ++f2)
-                std::copy_n(&A3(f1, f2, 0), max_bin, &res->aa[((size_t)f1 * kMaxFF + f2) * max_bin]);
-            std::copy_n(&A2(f1, 0), max_bin, &res->aw[(size_t)f1 * max_bin]);
-        }
+        aa.truncate(max_bin);
+        aw.truncate(max_bin);
+        res->aa = std::move(aa);
+        res->aw = std::move(aw);
         res->ww.assign(ww.begin(), ww.begin() + max_bin);
         res->p_tot = std::move(ptot);
     } else {
@@ -528,8 +603,10 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         }
         const unsigned max_bin = std::min<unsigned>(max_bin_of(ptot), (unsigned)nbins);
         res->max_bin = max_bin;
-        res->aa.assign(aa.begin(), aa.begin() + max_bin);
-        res->aw.assign(aw.begin(), aw.begin() + max_bin);
+        res->aa.reset(1, 1, (int)max_bin);
+        res->aw.reset(1, 1, (int)max_bin);
+        std::copy_n(aa.begin(), max_bin, res->aa.touch(0, 0));
+        std::copy_n(aw.begin(), max_bin, res->aw.touch(0, 0));
         res->ww.assign(ww.begin(), ww.begin() + max_bin);
         res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
     }
@@ -555,18 +632,17 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
         for (int b = 0; b < K; ++b) H[axis].push_back(src[b].value);
     };
     if (mode == Mode::Simple) {
-        push_row(0, 0, 0, 0, aa.data());
-        push_row(0, 1, 0, 0, aw.data());
+        if (const WeightedEntry* r = aa.row(0, 0)) push_row(0, 0, 0, 0, r);
+        if (const WeightedEntry* r = aw.row(0, 0)) push_row(0, 1, 0, 0, r);
         push_row(0, 2, 0, 0, ww.data());
     } else {
         // FFGrid (CompositeDistanceHistogramFFGrid.cpp:77-119): [ff][exv] and
         // aw[exv] on the ax axis, [exv][exv] on the xx axis, the rest on p_tot's
-        for (int f1 = 0; f1 < kMaxFF; ++f1)
-            for (int f2 = 0; f2 < kMaxFF; ++f2) {
-                const int axis = !grid ? 0 : (f1 == kExvBin && f2 == kExvBin) ? 2 : (f2 == kExvBin ? 1 : 0);
-                push_row(axis, 0, f1, f2, &aa[((size_t)f1 * kMaxFF + f2) * K]);
-            }
-        for (int f = 0; f < kMaxFF; ++f) push_row(grid && f == kExvBin ? 1 : 0, 1, f, 0, &aw[(size_t)f * K]);
+        aa.for_each([&](int f1, int f2, const WeightedEntry* r) {
+            const int axis = !grid ? 0 : (f1 == kExvBin && f2 == kExvBin) ? 2 : (f2 == kExvBin ? 1 : 0);
+            push_row(axis, 0, f1, f2, r);
+        });
+        aw.for_each([&](int f, int, const WeightedEntry* r) { push_row(grid && f == kExvBin ? 1 : 0, 1, f, 0, r); });
         push_row(0, 2, 0, 0, ww.data());
     }
     const std::vector<double>* axes[3] = {&d_axis, &d_axis_ax, &d_axis_xx};

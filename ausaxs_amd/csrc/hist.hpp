@@ -28,6 +28,35 @@ struct WeightedEntry {  // distribution/detail/WeightedEntry.h:16-50
 };
 
 void entry_add(WeightedEntry& a, const WeightedEntry& b);
+
+// A [n1][n2][width] table of entries stored by rows, holding only the rows
+// that were touched (form-factor pairs present in a structure); absent rows
+// read as zero. Row storage never moves once created.
+class EntryRows {
+  public:
+    void reset(int n1, int n2, int width);
+    int n1() const { return n1_; }
+    int n2() const { return n2_; }
+    int width() const { return width_; }
+    size_t size() const { return (size_t)n1_ * n2_ * width_; }
+    const WeightedEntry* row(int a, int b) const;  // nullptr if absent
+    WeightedEntry* row(int a, int b);
+    WeightedEntry* touch(int a, int b);  // the row, zero-initialised on first use
+    void clear(int a, int b);            // make a row absent (zero)
+    void truncate(int width);            // keep the first `width` bins
+    void dense(WeightedEntry* out) const;  // [n1][n2][width] with zero rows
+    template <class F>
+    void for_each(F&& f) const {  // f(a, b, row) over present rows in (a, b) order
+        for (int a = 0; a < n1_; ++a)
+            for (int b = 0; b < n2_; ++b)
+                if (const WeightedEntry* r = row(a, b)) f(a, b, r);
+    }
+
+  private:
+    int n1_ = 0, n2_ = 0, width_ = 0;
+    std::vector<int32_t> slot_;
+    std::vector<std::vector<WeightedEntry>> rows_;
+};
 // HistogramManagerMTFFAvg.cpp:340-352: last bin >= 10 with a non-zero value, + 1 (at least 10)
 unsigned max_bin_of(const std::vector<WeightedEntry>& p);
 // WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58) with
@@ -66,9 +95,9 @@ class CompositeDistanceHistogramHIP {
     double bin_width = 0.25;
     bool weighted_bins = true;
     int debye_terms = 3;
-    // FF modes: aa [15][15][max_bin], aw [15][max_bin], ww [max_bin]
-    // Simple:   aa, aw, ww [max_bin]
-    std::vector<WeightedEntry> aa, aw, ww, p_tot;
+    // FF modes: aa [15][15][max_bin], aw [15][1][max_bin]; Simple: [1][1][max_bin]
+    EntryRows aa, aw;
+    std::vector<WeightedEntry> ww, p_tot;  // [max_bin]
     std::vector<double> d_axis;
     // FFGrid only: distance axes of the atom-dummy and dummy-dummy rows
     // (CompositeDistanceHistogramFFGrid.cpp:130-136), first max_bin entries

@@ -691,8 +691,10 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     res->debye_terms = s_.debye_terms;
     const unsigned mb = std::min<unsigned>(max_bin_of(ptot), (unsigned)nb);
     res->max_bin = mb;
-    res->aa.assign(aa.begin(), aa.begin() + mb);
-    res->aw.assign(aw.begin(), aw.begin() + mb);
+    res->aa.reset(1, 1, (int)mb);
+    res->aw.reset(1, 1, (int)mb);
+    std::copy_n(aa.begin(), mb, res->aa.touch(0, 0));
+    std::copy_n(aw.begin(), mb, res->aw.touch(0, 0));
     res->ww.assign(ww.begin(), ww.begin() + mb);
     res->p_tot.assign(ptot.begin(), ptot.begin() + mb);
     res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins);
