@@ -37,6 +37,11 @@ struct PairWork {
 };
 constexpr int32_t kDiag = 1;    // same point set and overlapping ranges: count j > i only
 constexpr int32_t kOrient = 2;  // two output rows chosen by original-index order
+// two output rows chosen by the row point's position: rows before i_begin +
+// (flags >> kSplitShift) use row0, the rest row1 (a row tile spanning two
+// classes of a cross job)
+constexpr int32_t kSplit = 4;
+constexpr int kSplitShift = 8;
 
 // Pairs involving the reference's scalar-tail columns (last <= 3 points of a
 // loop), evaluated with per-pair rule selection and added with global atomics.
@@ -58,7 +63,7 @@ struct PairLaunch {
     int nbins;                // LDS bins per row; slot nbins collects overflow
     int clamp;                // 0 only if every pair's bin is known to be < nbins
     int grid;                 // workgroups; 0 or >= n_items: one per item, else persistent (strided)
-    int rows = 2;             // LDS histogram rows: 2 if any item has kOrient, else 1
+    int rows = 2;             // LDS histogram rows: 2 if any item has kOrient or kSplit, else 1
     int copies = 1;           // LDS histogram copies (1, 2, 4 or 8), lane l adds into copy l mod copies
 };
 

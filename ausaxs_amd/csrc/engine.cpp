@@ -65,7 +65,7 @@ void check_tile_j(int tile_j) {
 
 int item_rows(const std::vector<PairWork>& items) {
     for (const PairWork& w : items)
-        if (w.flags & kOrient) return 2;
+        if (w.flags & (kOrient | kSplit)) return 2;
     return 1;
 }
 
@@ -159,7 +159,7 @@ void add_items(std::vector<PairWork>& items, int i0, int i1, int j0, int j1, int
 }  // namespace
 
 void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
-                const std::vector<SetLayout>& layouts, int n_rows, int tile_j) {
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split_tiles) {
     const bool self = job.col_set < 0;
     const PointSet& R = sets[job.row_set];
     const PointSet& Cs = self ? R : sets[job.col_set];
@@ -207,14 +207,51 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
     } else {
         const int64_t n = R.size(), m = Cs.size();
         wl.pairs += n * m;
-        for (int a = 0; a < CR; ++a) {
-            if (LR.seg_len[a] == 0) continue;
-            const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_len[a];
-            for (int b = 0; b < CC; ++b) {
-                if (LC.seg_colbody[b] == 0) continue;
-                const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
-                const int r = row_of(a, b);
-                add_items(wl.items, s0, s1, t0, t1, r, r, false, tile_j);
+        // Row tiles run through the class segments (contiguous in class order)
+        // and may span two classes: a lane's output row then depends only on
+        // its own row's class, which is constant over the item (kSplit).
+        // Column tiles stay within one class segment.
+        std::vector<int> seg_cls;  // non-empty row classes in buffer order
+        for (int a = 0; a < CR; ++a)
+            if (LR.seg_len[a]) seg_cls.push_back(a);
+        for (int b = 0; b < CC; ++b) {
+            if (LC.seg_colbody[b] == 0) continue;
+            const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
+            size_t k = 0;
+            int i = seg_cls.empty() ? 0 : LR.seg_start[seg_cls[0]];
+            while (k < seg_cls.size()) {
+                const int a = seg_cls[k];
+                const int a_end = LR.seg_start[a] + LR.seg_len[a];
+                int i1 = std::min(i + kTileI, a_end);
+                int split = 0, r1 = row_of(a, b);
+                size_t next = k;
+                if (i1 == a_end) {
+                    next = k + 1;
+                    // fill the tile with the next class (at most two classes per tile)
+                    if (split_tiles && next < seg_cls.size() && i + kTileI > a_end) {
+                        const int a2 = seg_cls[next];
+                        const int a2_end = LR.seg_start[a2] + LR.seg_len[a2];
+                        const int i2 = std::min(i + kTileI, a2_end);
+                        split = a_end - i;
+                        r1 = row_of(a2, b);
+                        i1 = i2;
+                        if (i2 == a2_end) next = k + 2;
+                    }
+                }
+                const int r0 = row_of(a, b);
+                for (int jb = t0; jb < t1; jb += tile_j) {
+                    PairWork w{};
+                    w.i_begin = i;
+                    w.i_end = i1;
+                    w.j_begin = jb;
+                    w.j_end = std::min(jb + tile_j, t1);
+                    w.row0 = r0;
+                    w.row1 = split && r1 != r0 ? r1 : r0;
+                    w.flags = split && r1 != r0 ? (kSplit | (split << kSplitShift)) : 0;
+                    wl.items.push_back(w);
+                }
+                i = i1;
+                k = next;
             }
         }
         const int mm = Cs.size();
@@ -229,12 +266,23 @@ void sort_items(std::vector<PairWork>& items) {
     });
 }
 
+int64_t padded_work(const std::vector<PairWork>& items) {
+    int64_t w = 0;
+    for (const PairWork& x : items) w += (int64_t)kTileI * (x.j_end - x.j_begin);
+    return w;
+}
+
+bool prefer_split(int64_t work_plain, int rows_plain, int64_t work_split, int rows_split) {
+    if (rows_split <= rows_plain) return work_split * 50 <= work_plain * 49;
+    return work_split * 10 <= work_plain * 9;
+}
+
 MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
     std::vector<std::vector<int32_t>> per_row(n_rows);
     for (size_t k = 0; k < items.size(); ++k) {
         items[k].slot = (int32_t)k;
         per_row[items[k].row0].push_back((int32_t)(2 * k));
-        if (items[k].flags & kOrient) per_row[items[k].row1].push_back((int32_t)(2 * k + 1));
+        if (items[k].flags & (kOrient | kSplit)) per_row[items[k].row1].push_back((int32_t)(2 * k + 1));
     }
     // entries grouped by row, cut into merge chunks of at most kMergeChunk
     MergePlan mp;
@@ -270,12 +318,18 @@ void PairEngine::build(int tile_j) {
     std::vector<float4> buf;
     layout_.clear();
     for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf));
-    wl_ = WorkList{};
-    for (const PairJob& job : jobs_) append_job(wl_, job, sets_, layout_, n_rows_, tile_j);
+    WorkList plain, split;
+    for (const PairJob& job : jobs_) {
+        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false);
+        append_job(split, job, sets_, layout_, n_rows_, tile_j, true);
+    }
+    const bool use_split = prefer_split(padded_work(plain.items), item_rows(plain.items), padded_work(split.items),
+                                        item_rows(split.items));
+    wl_ = use_split ? std::move(split) : std::move(plain);
     sort_items(wl_.items);
-    mp_ = plan_merge(wl_.items, n_rows_);
     rows_ = item_rows(wl_.items);
     copies_ = pick_copies(nbins_, rows_);
+    mp_ = plan_merge(wl_.items, n_rows_);
 
     // ---- device residency -------------------------------------------------
     hipStream_t s = ctx.stream();

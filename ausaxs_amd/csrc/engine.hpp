@@ -135,8 +135,18 @@ struct WorkList {
     std::vector<int32_t> rowmaps;
     int64_t pairs = 0;
 };
+// split: row tiles of cross jobs may span two row classes (kSplit items)
 void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
-                const std::vector<SetLayout>& layouts, int n_rows, int tile_j);
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split);
+// Row work the pair kernel spends on a list: kTileI row lanes per item times
+// its columns (padding of partial row tiles included).
+int64_t padded_work(const std::vector<PairWork>& items);
+// Whether the list with split row tiles is worth its cost. kSplit items need a
+// second LDS histogram row, which halves the workgroups a CU holds, so when
+// the plain list needs one row only, split tiles must save at least a tenth
+// of the padded work. Otherwise they must save 2%: on a 250k-atom structure
+// (padding below 0.1%) split tiles measured 5% slower at 8000-column tiles.
+bool prefer_split(int64_t work_plain, int rows_plain, int64_t work_split, int rows_split);
 // largest items first: the tail of a launch is made of short items
 void sort_items(std::vector<PairWork>& items);
 

@@ -112,15 +112,17 @@ __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
     __hip_atomic_fetch_add(cell, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Row points of one lane (kRowsPerLane of them) for the current item.
+// Row points of one lane (kRowsPerLane of them) for the current item; roff:
+// LDS offset of the row each point adds into (split items).
 struct Rows {
     float x[kRowsPerLane], y[kRowsPerLane], z[kRowsPerLane];
     int tag[kRowsPerLane], idx[kRowsPerLane];
     bool valid[kRowsPerLane];
+    int roff[kRowsPerLane];
 };
 
 // All kRowsPerLane pairs of one column point: bin, tie fallback, LDS add.
-template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP>
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT>
 __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, float inv_w,
                                             unsigned ovf, int off1) {
     const int tagj = __float_as_int(pj.w);
@@ -145,8 +147,10 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
             if (DIAG) keep = keep && (j > r.idx[k]);
             bin = keep ? bin : ovf + 1u;
         }
+        // oriented items choose the row per pair, split items by the row point
         int off = 0;
         if (ORIENT) off = r.tag[k] < tagj ? 0 : off1;
+        if (SPLIT) off = r.roff[k];
         lds_add(&hist[off + (int)bin], cell_add(dl[k]));
     }
 }
@@ -156,7 +160,7 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
 // group are issued before the atomics of the other, so a wait for them never
 // waits for atomics (LDS operations retire in order) and no register copies
 // are needed between groups.
-template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP>
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
                                          float4* cols, float inv_w, int nbins) {
     const int t = threadIdx.x;
@@ -175,6 +179,8 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         r.z[k] = p.z;
         r.tag[k] = __float_as_int(p.w);
         r.idx[k] = i;
+        // kSplit: rows from the second class add into the second LDS row
+        r.roff[k] = SPLIT && i >= w.i_begin + (w.flags >> kSplitShift) ? nbins + 2 : 0;
     }
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
@@ -203,16 +209,16 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
             for (int u = 0; u < kColGroup; ++u) bq[u] = cols[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP>(r, a[u], jc + c + u, hist, inv_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jc + c + u, hist, inv_w, ovf, off1);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u) a[u] = cols[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, ovf, off1);
         }
         for (int c = n2; c < nc; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP>(r, cols[c], jc + c, hist, inv_w, ovf, off1);
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cols[c], jc + c, hist, inv_w, ovf, off1);
     }
 }
 
@@ -223,6 +229,10 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
     if (w.flags & kDiag) {
         if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
         else run_item<false, true, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+    } else if (w.flags & kSplit) {
+        // a row tile spanning two classes of a cross job (never oriented)
+        if (w.i_end - w.i_begin >= kTileI) run_item<true, false, false, CLAMP, true>(pts, w, hist, cols, inv_w, nbins);
+        else run_item<false, false, false, CLAMP, true>(pts, w, hist, cols, inv_w, nbins);
     } else if (w.i_end - w.i_begin >= kTileI) {
         if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
         else run_item<true, false, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
@@ -239,8 +249,10 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 // The LDS holds `copies` histograms of `rows` rows each; lane l of a wave adds
 // into copy l mod copies, so lanes whose pairs fall into the same bin collide
 // on one LDS cell `copies` times less often (pairs of two bodies crowd into few
-// bins). The copies are summed at the flush.
-__global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __restrict__ pts,
+// bins). The copies are summed at the flush. The kernel is held to 64 VGPRs
+// (8 waves per SIMD): one register more costs a wave per SIMD and about 5% of
+// the pair rate.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const PairWork* __restrict__ items, int n_items,
                                                              uint64_t* __restrict__ partial, float inv_w,
                                                              int nbins, int clamp, int rows, int copies) {
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __res
     uint64_t* mine = hist + (threadIdx.x & (copies - 1)) * copy_stride;
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const PairWork w = items[it];
-        const bool orient = (w.flags & kOrient) != 0;
+        const bool two = (w.flags & (kOrient | kSplit)) != 0;
         __syncthreads();  // previous item's flush done
         for (int e = threadIdx.x; e < copies * copy_stride; e += kThreads) hist[e] = 0;
         __syncthreads();
@@ -261,7 +273,7 @@ __global__ __launch_bounds__(kThreads) void pair_hist_kernel(const float4* __res
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
         uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
-        for (int rr = 0; rr < (orient ? 2 : 1); ++rr)
+        for (int rr = 0; rr < (two ? 2 : 1); ++rr)
             for (int e = threadIdx.x; e < out_stride; e += kThreads) {
                 uint64_t v = hist[rr * stride + e];
                 for (int c = 1; c < copies; ++c) v += hist[c * copy_stride + rr * stride + e];

@@ -339,16 +339,28 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
             if (r >= 0) r += n_conf * R_;
         return m;
     };
-    WorkList shared, tmpl;
-    for (int i = 0; i < B; ++i) {
-        for (int j = 0; j < i; ++j) {
-            const bool mv = moving[i] || moving[j];
-            append_job(mv ? tmpl : shared, PairJob{1 + j, 1 + i, mv ? aa_map : shifted(aa_map)}, sets, L, n_rows,
-                       tile_j);
+    auto make_lists = [&](bool split, WorkList& shared, WorkList& tmpl) {
+        for (int i = 0; i < B; ++i) {
+            for (int j = 0; j < i; ++j) {
+                const bool mv = moving[i] || moving[j];
+                append_job(mv ? tmpl : shared, PairJob{1 + j, 1 + i, mv ? aa_map : shifted(aa_map)}, sets, L,
+                           n_rows, tile_j, split);
+            }
+            if (waters_.size() > 0)
+                append_job(moving[i] ? tmpl : shared, PairJob{0, 1 + i, moving[i] ? aw_map : shifted(aw_map)}, sets,
+                           L, n_rows, tile_j, split);
         }
-        if (waters_.size() > 0)
-            append_job(moving[i] ? tmpl : shared, PairJob{0, 1 + i, moving[i] ? aw_map : shifted(aw_map)}, sets, L,
-                       n_rows, tile_j);
+    };
+    WorkList shared, tmpl, shared_s, tmpl_s;
+    make_lists(false, shared, tmpl);
+    make_lists(true, shared_s, tmpl_s);
+    const int64_t work = padded_work(shared.items) + (int64_t)n_conf * padded_work(tmpl.items);
+    const int64_t work_s = padded_work(shared_s.items) + (int64_t)n_conf * padded_work(tmpl_s.items);
+    const int rows = std::max(item_rows(shared.items), item_rows(tmpl.items));
+    const int rows_s = std::max(item_rows(shared_s.items), item_rows(tmpl_s.items));
+    if (prefer_split(work, rows, work_s, rows_s)) {
+        shared = std::move(shared_s);
+        tmpl = std::move(tmpl_s);
     }
     p.pairs = shared.pairs + (int64_t)n_conf * tmpl.pairs;
 
