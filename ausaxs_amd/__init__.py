@@ -95,6 +95,14 @@ def _load():
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
+        "ausaxs_symmetry_create": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, P(_Settings),
+                                             P(vp)]),
+        "ausaxs_symmetry_calculate": (C.c_int, [vp, P(vp)]),
+        "ausaxs_symmetry_free": (C.c_int, [vp]),
+        "ausaxs_symmetry_info": (C.c_int, [vp, P(C.c_longlong), P(C.c_int)]),
+        "ausaxs_symmetry_chain_schedule": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, P(C.c_int)]),
+        "ausaxs_symmetry_placement_schedule": (C.c_int, [vp, C.c_int, vp, C.c_int, P(C.c_int)]),
+        "ausaxs_symmetry_cyclic": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, vp, P(C.c_int)]),
         "ausaxs_synth_globule": (
             C.c_int,
             [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp],
@@ -406,6 +414,95 @@ class RigidScan:
         _check(lib.ausaxs_rigid_info(self._r, *map(C.byref, v), C.byref(pairs), C.byref(ms)))
         return dict(tile_j=v[0].value, grid=v[1].value, copies=v[2].value, batch=v[3].value, n_items=v[4].value,
                     pairs=pairs.value, scan_ms=ms.value)
+
+
+class Symmetry:
+    """One symmetry of a body: `transforms` (reps x 12 doubles: rotation
+    row-major, translation) generate copies 1..reps; `schedule` lists the
+    distinct copy pairs (repA, repB, scale), repetition 0 being the original
+    (ISymmetry::internal_pair_schedule)."""
+
+    def __init__(self, transforms, schedule):
+        self.transforms = np.ascontiguousarray(np.asarray(transforms, np.float64).reshape(-1, 12))
+        self.schedule = np.ascontiguousarray(np.asarray(schedule, np.int32).reshape(-1, 3))
+
+    @property
+    def reps(self):
+        return len(self.transforms)
+
+    @classmethod
+    def cyclic(cls, cm, offset, step, axis, angle, reps):
+        """CyclicSymmetry(offset, step, axis, angle, reps) of a body with
+        centre cm, with its chain schedule."""
+        out = np.zeros((reps, 12), np.float64)
+        closed = C.c_int()
+        v = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(3)) for x in (cm, offset, step, axis)]
+        _check(lib.ausaxs_symmetry_cyclic(*[_ptr(x) for x in v], float(angle), int(reps), _ptr(out),
+                                          C.byref(closed)))
+        return cls(out, chain_schedule(reps, bool(closed.value)))
+
+
+def _schedule_out(call):
+    n = C.c_int()
+    _check(call(None, 0, C.byref(n)))
+    out = np.zeros((max(n.value, 1), 3), np.int32)
+    _check(call(_ptr(out), n.value, C.byref(n)))
+    return out[: n.value]
+
+
+def chain_schedule(reps, closed):
+    return _schedule_out(lambda o, c, n: lib.ausaxs_symmetry_chain_schedule(int(reps), int(bool(closed)), o, c, n))
+
+
+def placement_schedule(placements):
+    p = np.ascontiguousarray(np.asarray(placements, np.float64).reshape(-1, 12))
+    return _schedule_out(lambda o, c, n: lib.ausaxs_symmetry_placement_schedule(_ptr(p), len(p), o, c, n))
+
+
+class SymmetryStructure:
+    """Bodies with symmetry copies; calculate() = SymmetryManagerMT::calculate_all.
+
+    bodies: list of (n_b, 4) float arrays {x, y, z, w}; symmetries: list (one
+    per body) of lists of Symmetry; waters: (m, 4) floats or None."""
+
+    def __init__(self, bodies, symmetries, waters=None, settings=None):
+        bodies = [_pts(b) for b in bodies]
+        if len(symmetries) != len(bodies):
+            raise ValueError("one symmetry list per body")
+        self._atoms = np.ascontiguousarray(np.concatenate(bodies) if bodies else np.zeros((0, 4), np.float32))
+        self._sizes = np.array([len(b) for b in bodies], np.int32)
+        syms = [s for per in symmetries for s in per]
+        self._per = np.array([len(per) for per in symmetries], np.int32)
+        self._reps = np.array([s.reps for s in syms], np.int32)
+        self._T = np.ascontiguousarray(np.concatenate([s.transforms for s in syms]) if syms
+                                       else np.zeros((0, 12)), np.float64)
+        self._slen = np.array([len(s.schedule) for s in syms], np.int32)
+        self._sched = np.ascontiguousarray(np.concatenate([s.schedule for s in syms]) if syms
+                                           else np.zeros((0, 3), np.int32), np.int32)
+        self._waters = _pts(waters)
+        s = settings or default_settings()
+        h = C.c_void_p()
+        _check(lib.ausaxs_symmetry_create(
+            _ptr(self._atoms), _ptr(self._sizes), len(self._sizes), _ptr(self._per), _ptr(self._reps),
+            _ptr(self._T), _ptr(self._slen), _ptr(self._sched), _ptr(self._waters), len(self._waters),
+            C.byref(s), C.byref(h)))
+        self._s = h
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.ausaxs_symmetry_free(s)
+            self._s = C.c_void_p(0)
+
+    def calculate(self) -> CompositeHistogram:
+        h = C.c_void_p()
+        _check(lib.ausaxs_symmetry_calculate(self._s, C.byref(h)))
+        return CompositeHistogram(h.value)
+
+    def info(self):
+        pairs, items = C.c_longlong(), C.c_int()
+        _check(lib.ausaxs_symmetry_info(self._s, C.byref(pairs), C.byref(items)))
+        return dict(pairs=pairs.value, n_items=items.value)
 
 
 def rigid_transform(R, t):

@@ -8,6 +8,7 @@
 #include "hist.hpp"
 #include "rigid.hpp"
 #include "serve.hpp"
+#include "symmetry.hpp"
 #include "synth.hpp"
 
 #include <cstring>
@@ -27,6 +28,10 @@ struct ausaxs_histogram {
 
 struct ausaxs_rigid {
     std::unique_ptr<RigidScan> r;
+};
+
+struct ausaxs_symmetry {
+    std::unique_ptr<SymmetryStructure> s;
 };
 
 struct ausaxs_server {
@@ -171,6 +176,107 @@ int ausaxs_structure_calculate(ausaxs_structure* s, ausaxs_histogram** out) {
 int ausaxs_structure_free(ausaxs_structure* s) {
     delete s;
     return 0;
+}
+
+int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                           const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                           const float* waters, int n_waters, const ausaxs_settings* settings,
+                           ausaxs_symmetry** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_bodies >= 0 && n_waters >= 0, "negative sizes");
+        require(n_bodies == 0 || (body_sizes && sym_per_body), "body arrays are NULL");
+        require(n_waters == 0 || waters, "waters is NULL");
+        std::vector<int> sizes(body_sizes, body_sizes + n_bodies);
+        size_t n_atoms = 0;
+        for (int v : sizes) {
+            require(v >= 0, "negative body size");
+            n_atoms += (size_t)v;
+        }
+        require(n_atoms == 0 || atoms, "atoms is NULL");
+        std::vector<std::vector<SymmetrySpec>> syms(n_bodies);
+        size_t k = 0, t = 0, e = 0;
+        for (int b = 0; b < n_bodies; ++b) {
+            require(sym_per_body[b] >= 0, "negative symmetry count");
+            for (int i = 0; i < sym_per_body[b]; ++i, ++k) {
+                require(sym_reps && sched_len, "symmetry arrays are NULL");
+                SymmetrySpec sp;
+                sp.reps = sym_reps[k];
+                require(sp.reps >= 1, "a symmetry needs at least one repetition");
+                require(transforms != nullptr, "transforms is NULL");
+                sp.transforms.assign(transforms + 12 * t, transforms + 12 * (t + sp.reps));
+                t += (size_t)sp.reps;
+                require(sched_len[k] >= 0, "negative schedule length");
+                require(sched_len[k] == 0 || schedule, "schedule is NULL");
+                for (int j = 0; j < sched_len[k]; ++j, ++e)
+                    sp.schedule.push_back(CopyPair{schedule[3 * e], schedule[3 * e + 1], schedule[3 * e + 2]});
+                syms[b].push_back(std::move(sp));
+            }
+        }
+        auto s = std::make_unique<ausaxs_symmetry>();
+        s->s = std::make_unique<SymmetryStructure>(atoms, sizes, std::move(syms), waters, n_waters,
+                                                   to_settings(settings));
+        *out = s.release();
+    });
+}
+
+int ausaxs_symmetry_calculate(ausaxs_symmetry* s, ausaxs_histogram** out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        auto h = std::make_unique<ausaxs_histogram>();
+        h->h = s->s->calculate();
+        *out = h.release();
+    });
+}
+
+int ausaxs_symmetry_free(ausaxs_symmetry* s) {
+    delete s;
+    return 0;
+}
+
+int ausaxs_symmetry_info(const ausaxs_symmetry* s, long long* pairs, int* n_items) {
+    return guarded([&] {
+        require(s != nullptr, "NULL structure");
+        if (pairs) *pairs = s->s->engine().pairs();
+        if (n_items) *n_items = s->s->engine().n_items();
+    });
+}
+
+namespace {
+int put_schedule(const std::vector<CopyPair>& v, int* out, int capacity, int* n_out) {
+    if (n_out) *n_out = (int)v.size();
+    require(capacity >= (int)v.size() || !out, "schedule capacity too small");
+    if (out)
+        for (size_t i = 0; i < v.size(); ++i) {
+            out[3 * i] = v[i].a;
+            out[3 * i + 1] = v[i].b;
+            out[3 * i + 2] = v[i].scale;
+        }
+    return 0;
+}
+}  // namespace
+
+int ausaxs_symmetry_chain_schedule(int reps, int closed, int* out, int capacity, int* n_out) {
+    return guarded([&] { put_schedule(chain_pair_schedule(reps, closed != 0), out, capacity, n_out); });
+}
+
+int ausaxs_symmetry_placement_schedule(const double* placements, int n, int* out, int capacity, int* n_out) {
+    return guarded([&] {
+        require(n >= 0 && (n == 0 || placements), "placements is NULL");
+        put_schedule(placement_pair_schedule(std::vector<double>(placements, placements + 12 * (size_t)n)), out,
+                     capacity, n_out);
+    });
+}
+
+int ausaxs_symmetry_cyclic(const double* cm, const double* offset, const double* step, const double* axis,
+                           double angle, int reps, double* out, int* closed) {
+    return guarded([&] {
+        require(cm && offset && step && axis && out, "NULL argument");
+        bool c = false;
+        const std::vector<double> t = cyclic_transforms(cm, offset, step, axis, angle, reps, &c);
+        std::copy(t.begin(), t.end(), out);
+        if (closed) *closed = c ? 1 : 0;
+    });
 }
 
 int ausaxs_structure_set_timing(ausaxs_structure* s, int on) {

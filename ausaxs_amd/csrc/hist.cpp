@@ -373,7 +373,8 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     }
 
     const int bound = bin_bound({&A, &W, &X}, inv_w);
-    const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, bound);
+    // at least the 10 bins every truncated histogram keeps (max_bin >= 10)
+    const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, std::max(bound, 10));
     std::vector<PointSet> sets;
     sets.push_back(std::move(A));
     sets.push_back(std::move(W));

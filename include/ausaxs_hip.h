@@ -245,6 +245,42 @@ int ausaxs_structure_set_timing(ausaxs_structure* s, int on);
 int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
                        int* n_items);
 
+/* ---- bodies with symmetry copies ------------------------------------------ */
+
+/* Replaces SymmetryManagerMT<weighted_bins, vbw>::calculate_all
+ * (SymmetryManagerMT.cpp:29-170) for XYZW bodies. atoms: sum(body_sizes) x 4
+ * floats {x, y, z, w}, body after body. Body b has sym_per_body[b]
+ * symmetries; symmetry k (counted over all bodies) has sym_reps[k] copies,
+ * each an affine map of 12 doubles in `transforms` (rotation row-major, then
+ * translation; ISymmetry.h:17-25), applied in FP64 to the float coordinates
+ * as generate_transformed_data does (SymmetryHelpers.cpp:16-43), and a pair
+ * schedule of sched_len[k] entries (repA, repB, scale) in `schedule`
+ * (ISymmetry::internal_pair_schedule, ISymmetry.h:26-39; repetition 0 = the
+ * original). Any positive scale is accepted (the reference's switch stops at
+ * 30 and 60, SimpleCalculator.h:286-377). Result: a SIMPLE-mode histogram
+ * (aa, aw, ww, p_tot, max_bin) as the reference's CompositeDistanceHistogram. */
+typedef struct ausaxs_symmetry ausaxs_symmetry;
+int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                           const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                           const float* waters, int n_waters, const ausaxs_settings* settings,
+                           ausaxs_symmetry** out);
+int ausaxs_symmetry_calculate(ausaxs_symmetry* s, ausaxs_histogram** out);
+int ausaxs_symmetry_free(ausaxs_symmetry* s);
+/* Pair evaluations of one calculate. */
+int ausaxs_symmetry_info(const ausaxs_symmetry* s, long long* pairs, int* n_items);
+
+/* ISymmetry::internal_pair_schedule (ISymmetry.cpp:43-66): the cyclic-chain
+ * schedule of a symmetry with reps copies. out: 3 ints per entry; *n_out =
+ * entries (also when capacity is too small: status 1). */
+int ausaxs_symmetry_chain_schedule(int reps, int closed, int* out, int capacity, int* n_out);
+/* compute_pair_schedule (PairScheduleUtil.cpp:33-60): n placements of 12
+ * doubles (placement 0 = the original body). */
+int ausaxs_symmetry_placement_schedule(const double* placements, int n, int* out, int capacity, int* n_out);
+/* CyclicSymmetry::_make_transform / is_closed (CyclicSymmetry.cpp:85-118):
+ * the maps of copies 1..reps (12 doubles each) for a body with centre cm. */
+int ausaxs_symmetry_cyclic(const double* cm, const double* offset, const double* step, const double* axis,
+                           double angle, int reps, double* out, int* closed);
+
 /* ---- synthetic structures (deterministic, xoshiro256** seeded by SplitMix64) */
 
 /* Self-avoiding 3.8 A chain packed in an ellipsoid with semi-axes (ax, ay, az),

@@ -324,6 +324,45 @@ def hist_partial(self_bodies, bodies, sizes, waters=None, bin_width=0.25, bin_co
     return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
 
 
+lib.orc_affine_float.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+lib.orc_hist_symmetry.restype = C.c_int
+lib.orc_hist_symmetry.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4
+
+
+def affine_float(pts, T):
+    """A symmetry copy of float XYZW points (see orc_affine_float)."""
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 4)
+    T = np.ascontiguousarray(T, np.float64).reshape(12)
+    out = np.zeros_like(pts)
+    lib.orc_affine_float(_p(pts), len(pts), _p(T), _p(out))
+    return out
+
+
+def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=8000, nthreads=1):
+    """SymmetryManagerMT::calculate_all (see orc_hist_symmetry). bodies: list of
+    (n, 4) float arrays; symmetries: per body a list of (transforms (reps, 12),
+    schedule (k, 3))."""
+    atoms = np.ascontiguousarray(np.concatenate(bodies) if bodies else np.zeros((0, 4)), np.float32)
+    sizes = np.array([len(b) for b in bodies], np.int32)
+    per = np.array([len(x) for x in symmetries], np.int32)
+    flat = [x for xs in symmetries for x in xs]
+    reps = np.array([len(np.asarray(T).reshape(-1, 12)) for T, _ in flat], np.int32)
+    T = np.ascontiguousarray(np.concatenate([np.asarray(T, np.float64).reshape(-1, 12) for T, _ in flat])
+                             if flat else np.zeros((0, 12)), np.float64)
+    slen = np.array([len(np.asarray(S).reshape(-1, 3)) for _, S in flat], np.int32)
+    sched = np.ascontiguousarray(np.concatenate([np.asarray(S, np.int32).reshape(-1, 3) for _, S in flat])
+                                 if flat else np.zeros((0, 3)), np.int32)
+    waters = np.ascontiguousarray(waters if waters is not None else np.zeros((0, 4), np.float32), np.float32)
+    out = [np.zeros(bin_count, ENTRY) for _ in range(4)]
+    mb = lib.orc_hist_symmetry(_p(atoms), _p(sizes), len(sizes), _p(per), _p(reps), _p(T), _p(slen), _p(sched),
+                               _p(waters), len(waters), C.c_float(inv_width(bin_width)), bin_count, nthreads,
+                               *map(_p, out))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
+
+
 def debye_simple(hist, q, bin_width, cw=1.0):
     """CompositeDistanceHistogram::debye_transform (CompositeDistanceHistogram.cpp:55-70 with
     DistanceHistogram.cpp:54-87): p = aa + cw*aw + cw^2*ww on p_tot's weighted axis, times exp(-q^2)."""

@@ -677,6 +677,7 @@ typedef struct {
     float inv_w;
     int nb;
     int self;
+    int factor; /* template N of add<N>: 2 * scaling */
 } simple_args;
 
 
@@ -695,21 +696,23 @@ static int simple_row(job_ctx* c, void* pv, int i) {
         if (j < tail) rule_simd(pi, pj, A->inv_w, &d, &bin);
         else rule_scalar(pi, pj, A->inv_w, &d, &bin);
         if (bin < 0 || bin >= A->nb) return -2;
-        /* TemplateHelperSimple.h:15-61: add<2>(bin, d, w_i*w_j) (float product) */
-        add_n(&h[bin], 2, d, pi[3] * pj[3]);
+        /* TemplateHelperSimple.h:15-61: add<2 * scaling>(bin, d, w_i*w_j) (float product) */
+        add_n(&h[bin], A->factor, d, pi[3] * pj[3]);
     }
     return 0;
 }
 
-static int run_simple(const float* rows, int nr, const float* cols, int nc, int self, float inv_w, int nb,
-                      int nthreads, orc_entry* out) {
+/* SimpleCalculator::enqueue_calculate_self<scaling> / _cross<scaling>
+ * (SimpleCalculator.h:94-211): rows over data_2, columns over data_1 */
+static int run_simple_scaled(const float* rows, int nr, const float* cols, int nc, int self, int scaling,
+                             float inv_w, int nb, int nthreads, orc_entry* out) {
     simple_priv* privs = (simple_priv*)calloc(nthreads, sizeof(simple_priv));
     void** pp = (void**)malloc(sizeof(void*) * nthreads);
     for (int t = 0; t < nthreads; ++t) {
         privs[t].h = (orc_entry*)calloc(nb, sizeof(orc_entry));
         pp[t] = &privs[t];
     }
-    simple_args A = {rows, nr, cols, nc, inv_w, nb, self};
+    simple_args A = {rows, nr, cols, nc, inv_w, nb, self, 2 * scaling};
     int err = run_rows(nr, nthreads, simple_row, &A, pp);
     memset(out, 0, (size_t)nb * sizeof(orc_entry));
     for (int t = 0; t < nthreads; ++t) {
@@ -719,13 +722,23 @@ static int run_simple(const float* rows, int nr, const float* cols, int nc, int 
     free(privs);
     free(pp);
     if (self) {
-        /* SimpleCalculator.h:144-160: bin 0 += WeightedEntry(sum w^2, (unsigned) sum w^2, 0) */
+        /* SimpleCalculator.h:144-160: bin 0 += WeightedEntry(t, (unsigned) t, 0),
+         * t = scaling * sum (w*w) (float products summed in double) */
         double tw = 0;
-        for (int i = 0; i < nr; ++i) tw += (double)rows[4 * (size_t)i + 3] * rows[4 * (size_t)i + 3];
+        for (int i = 0; i < nr; ++i) {
+            const float w = rows[4 * (size_t)i + 3];
+            tw += (double)(w * w);
+        }
+        tw = scaling * tw;
         orc_entry e = {tw, (uint32_t)tw, 0};
         entry_add(&out[0], &e);
     }
     return err;
+}
+
+static int run_simple(const float* rows, int nr, const float* cols, int nc, int self, float inv_w, int nb,
+                      int nthreads, orc_entry* out) {
+    return run_simple_scaled(rows, nr, cols, nc, self, 1, inv_w, nb, nthreads, out);
 }
 
 int orc_hist_simple(const float* atoms, int n_a, const float* waters, int n_w, float inv_w, int nb, int nthreads,
@@ -789,6 +802,114 @@ int orc_hist_partial(const float* self_bodies, const float* bodies, const int* s
         ptot[b] = aa[b];
         entry_add(&ptot[b], &aw[b]);
         entry_add(&ptot[b], &ww[b]);
+    }
+    return max_bin_of(ptot, nb);
+}
+
+void orc_affine_float(const float* pts, int n, const double* T, float* out) {
+    for (int i = 0; i < n; ++i) {
+        const double x = pts[4 * (size_t)i], y = pts[4 * (size_t)i + 1], z = pts[4 * (size_t)i + 2];
+        for (int k = 0; k < 3; ++k) {
+            const double* r = T + 3 * k;
+            out[4 * (size_t)i + k] = (float)(((r[0] * x + r[1] * y) + r[2] * z) + T[9 + k]);
+        }
+        out[4 * (size_t)i + 3] = pts[4 * (size_t)i + 3];
+    }
+}
+
+int orc_hist_symmetry(const float* atoms, const int* sizes, int n_bodies, const int* sym_per_body,
+                      const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                      const float* waters, int n_w, float inv_w, int nb, int nthreads, orc_entry* aa,
+                      orc_entry* aw, orc_entry* ww, orc_entry* ptot) {
+    if (nthreads < 1) nthreads = 1;
+    int B = n_bodies, n_sym = 0;
+    for (int b = 0; b < B; ++b) n_sym += sym_per_body[b];
+    /* body offsets, first symmetry of each body, copy offsets per symmetry */
+    size_t* off = (size_t*)calloc(B + 1, sizeof(size_t));
+    int* sym0 = (int*)calloc(B + 1, sizeof(int));
+    size_t* t0 = (size_t*)calloc(n_sym + 1, sizeof(size_t));
+    size_t* e0 = (size_t*)calloc(n_sym + 1, sizeof(size_t));
+    for (int b = 0; b < B; ++b) {
+        off[b + 1] = off[b] + (size_t)sizes[b];
+        sym0[b + 1] = sym0[b] + sym_per_body[b];
+    }
+    for (int k = 0; k < n_sym; ++k) {
+        t0[k + 1] = t0[k] + (size_t)sym_reps[k];
+        e0[k + 1] = e0[k] + (size_t)sched_len[k];
+    }
+    /* copies: generate_transformed_data (SymmetryHelpers.cpp:16-43) */
+    float** copy = (float**)calloc(t0[n_sym] + 1, sizeof(float*));
+    for (int b = 0; b < B; ++b)
+        for (int k = sym0[b]; k < sym0[b + 1]; ++k)
+            for (int r = 0; r < sym_reps[k]; ++r) {
+                size_t c = t0[k] + (size_t)r;
+                copy[c] = (float*)malloc(sizeof(float) * 4 * ((size_t)sizes[b] + 1));
+                orc_affine_float(atoms + 4 * off[b], sizes[b], transforms + 12 * c, copy[c]);
+            }
+#define BODY(b) (atoms + 4 * off[b])
+#define AT(b, k, rep) ((rep) == 0 ? BODY(b) : copy[t0[k] + (size_t)(rep)-1])
+    orc_entry* tmp = (orc_entry*)malloc(sizeof(orc_entry) * (size_t)nb);
+    memset(aa, 0, sizeof(orc_entry) * (size_t)nb);
+    memset(aw, 0, sizeof(orc_entry) * (size_t)nb);
+    memset(ww, 0, sizeof(orc_entry) * (size_t)nb);
+    int err = 0;
+#define ACC(dst, expr)                                                  \
+    do {                                                                \
+        err |= (expr);                                                  \
+        for (int q_ = 0; q_ < nb; ++q_) entry_add(&(dst)[q_], &tmp[q_]); \
+    } while (0)
+    /* SymmetryManagerMT.cpp:57-132; cross(d1, d2) = rows d2, columns d1 */
+    for (int b1 = 0; b1 < B; ++b1) {
+        int total = 0;
+        for (int k = sym0[b1]; k < sym0[b1 + 1]; ++k) total += sym_reps[k];
+        ACC(aa, run_simple_scaled(BODY(b1), sizes[b1], NULL, 0, 1, 1 + total, inv_w, nb, nthreads, tmp));
+        if (n_w > 0) ACC(aw, run_simple_scaled(BODY(b1), sizes[b1], waters, n_w, 0, 1, inv_w, nb, nthreads, tmp));
+        for (int k1 = sym0[b1]; k1 < sym0[b1 + 1]; ++k1) {
+            for (size_t e = e0[k1]; e < e0[k1 + 1]; ++e) {
+                const int ra = schedule[3 * e], rb = schedule[3 * e + 1], sc = schedule[3 * e + 2];
+                ACC(aa, run_simple_scaled(AT(b1, k1, rb), sizes[b1], AT(b1, k1, ra), sizes[b1], 0, sc, inv_w, nb,
+                                          nthreads, tmp));
+            }
+            for (int r1 = 1; r1 <= sym_reps[k1]; ++r1) {
+                const float* c1 = AT(b1, k1, r1);
+                if (n_w > 0) ACC(aw, run_simple_scaled(c1, sizes[b1], waters, n_w, 0, 1, inv_w, nb, nthreads, tmp));
+                for (int b2 = b1 + 1; b2 < B; ++b2) {
+                    ACC(aa, run_simple_scaled(c1, sizes[b1], BODY(b2), sizes[b2], 0, 1, inv_w, nb, nthreads, tmp));
+                    for (int k2 = sym0[b2]; k2 < sym0[b2 + 1]; ++k2)
+                        for (int r2 = 1; r2 <= sym_reps[k2]; ++r2)
+                            ACC(aa, run_simple_scaled(AT(b2, k2, r2), sizes[b2], c1, sizes[b1], 0, 1, inv_w, nb,
+                                                      nthreads, tmp));
+                }
+                for (int k2 = k1 + 1; k2 < sym0[b1 + 1]; ++k2)
+                    for (int r2 = 1; r2 <= sym_reps[k2]; ++r2)
+                        ACC(aa, run_simple_scaled(AT(b1, k2, r2), sizes[b1], c1, sizes[b1], 0, 1, inv_w, nb,
+                                                  nthreads, tmp));
+            }
+        }
+        for (int b2 = b1 + 1; b2 < B; ++b2) {
+            ACC(aa, run_simple_scaled(BODY(b2), sizes[b2], BODY(b1), sizes[b1], 0, 1, inv_w, nb, nthreads, tmp));
+            for (int k2 = sym0[b2]; k2 < sym0[b2 + 1]; ++k2)
+                for (int r2 = 1; r2 <= sym_reps[k2]; ++r2)
+                    ACC(aa, run_simple_scaled(AT(b2, k2, r2), sizes[b2], BODY(b1), sizes[b1], 0, 1, inv_w, nb,
+                                              nthreads, tmp));
+        }
+    }
+    if (n_w > 0) ACC(ww, run_simple_scaled(waters, n_w, NULL, 0, 1, 1, inv_w, nb, nthreads, tmp));
+#undef ACC
+#undef AT
+#undef BODY
+    for (size_t c = 0; c < t0[n_sym]; ++c) free(copy[c]);
+    free(copy);
+    free(tmp);
+    free(off);
+    free(sym0);
+    free(t0);
+    free(e0);
+    if (err) return -2;
+    for (int b = 0; b < nb; ++b) {
+        ptot[b] = aa[b];
+        entry_add(&ptot[b], &ww[b]);
+        entry_add(&ptot[b], &aw[b]);
     }
     return max_bin_of(ptot, nb);
 }

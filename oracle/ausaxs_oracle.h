@@ -117,6 +117,22 @@ int orc_hist_partial(const float* self_bodies, const float* bodies, const int* s
                      const float* waters, int n_w, float inv_w, int nb, int nthreads,
                      orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
 
+/* FP64 affine map of float XYZW points as generate_transformed_data applies
+ * a symmetry copy (SymmetryHelpers.cpp:26-36, ISymmetry.h:24, Vector3.h:169-181):
+ *   out_x = (float)(((R00*x + R01*y) + R02*z) + t0) with x = (double) pts_x;
+ * T = {R00..R22, t0, t1, t2}; the weight is copied. */
+void orc_affine_float(const float* pts, int n, const double* T, float* out);
+
+/* SymmetryManagerMT::calculate (SymmetryManagerMT.cpp:39-170) with XYZW
+ * weights: per body a self histogram scaled by 1 + its copies, its pair
+ * schedules (repA, repB, scale), copy x other-body / other-symmetry crosses,
+ * body x water crosses for the original and every copy, water self.
+ * Layout as ausaxs_symmetry_create. Returns max_bin or a negative status. */
+int orc_hist_symmetry(const float* atoms, const int* sizes, int n_bodies, const int* sym_per_body,
+                      const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                      const float* waters, int n_w, float inv_w, int nb, int nthreads, orc_entry* aa,
+                      orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
 /* ---- weighted axis, sinc, Debye ------------------------------------------ */
 /* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
  * followed by DistanceHistogram::initialize (DistanceHistogram.cpp:44-47, d[0]=0). */
