@@ -95,6 +95,11 @@ def _load():
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
+        "ausaxs_fitter_create": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_uint, vp, C.c_int, P(vp)]),
+        "ausaxs_fitter_params": (C.c_int, [vp, P(C.c_int), P(C.c_uint)]),
+        "ausaxs_fitter_evaluate": (C.c_int, [vp, vp, C.c_int, vp]),
+        "ausaxs_fitter_fit": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, P(C.c_int), P(C.c_int), vp, vp]),
+        "ausaxs_fitter_free": (C.c_int, [vp]),
         "ausaxs_symmetry_create": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, P(_Settings),
                                              P(vp)]),
         "ausaxs_symmetry_calculate": (C.c_int, [vp, P(vp)]),
@@ -414,6 +419,56 @@ class RigidScan:
         _check(lib.ausaxs_rigid_info(self._r, *map(C.byref, v), C.byref(pairs), C.byref(ms)))
         return dict(tile_j=v[0].value, grid=v[1].value, copies=v[2].value, batch=v[3].value, n_items=v[4].value,
                     pairs=pairs.value, scan_ms=ms.value)
+
+
+FIT_HYDRATION, FIT_EXCLUDED_VOLUME, FIT_SOLVENT_DENSITY, FIT_ATOMIC_DW, FIT_EXV_DW = 1, 2, 4, 8, 16
+MINI_GOLDEN, MINI_SCAN = 0, 2
+
+
+class Fitter:
+    """SmartFitter(data, histogram): fit of a composite histogram to measured
+    I(q). The histogram must stay alive while the fitter is used."""
+
+    def __init__(self, hist, q, I, err=None, enabled=FIT_HYDRATION, q_model=None):
+        self._hist = hist
+        self.q = np.ascontiguousarray(q, np.float64)
+        self.I = np.ascontiguousarray(I, np.float64)
+        self.err = None if err is None else np.ascontiguousarray(err, np.float64)
+        qm = None if q_model is None else np.ascontiguousarray(q_model, np.float64)
+        h = C.c_void_p()
+        _check(lib.ausaxs_fitter_create(hist._h, _ptr(self.q), _ptr(self.I), _ptr(self.err), len(self.q),
+                                        int(enabled), _ptr(qm), 0 if qm is None else len(qm), C.byref(h)))
+        self._f = h
+        k, en = C.c_int(), C.c_uint()
+        _check(lib.ausaxs_fitter_params(self._f, C.byref(k), C.byref(en)))
+        self.n_params, self.enabled = k.value, en.value
+
+    def __del__(self):
+        f = getattr(self, "_f", None)
+        if f is not None and f.value:
+            lib.ausaxs_fitter_free(f)
+            self._f = C.c_void_p(0)
+
+    def evaluate(self, params):
+        """(n_sets, 5) array of a, b, var a, var b, chi2 for (n_sets, n_params) parameters."""
+        p = np.ascontiguousarray(np.asarray(params, np.float64).reshape(-1, max(self.n_params, 1))
+                                 if self.n_params else np.zeros((np.size(params) or 1, 0)), np.float64)
+        n_sets = len(p)
+        out = np.zeros((n_sets, 5), np.float64)
+        _check(lib.ausaxs_fitter_evaluate(self._f, _ptr(p), n_sets, _ptr(out)))
+        return out
+
+    def fit(self, algorithm=MINI_GOLDEN, scan_evals=100):
+        n = len(self.q)
+        params, perr = np.zeros(max(self.n_params, 1)), np.zeros(max(self.n_params, 1))
+        lin = np.zeros(5)
+        dof, fevals = C.c_int(), C.c_int()
+        I_fit, res = np.zeros(n), np.zeros(n)
+        _check(lib.ausaxs_fitter_fit(self._f, int(algorithm), int(scan_evals), _ptr(params), _ptr(perr), _ptr(lin),
+                                     C.byref(dof), C.byref(fevals), _ptr(I_fit), _ptr(res)))
+        return dict(params=params[: self.n_params], param_err=perr[: self.n_params], a=lin[0], b=lin[1],
+                    a_err=lin[2], b_err=lin[3], chi2=lin[4], dof=dof.value, fevals=fevals.value, I_fit=I_fit,
+                    residuals=res)
 
 
 class Symmetry:

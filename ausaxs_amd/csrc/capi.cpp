@@ -4,6 +4,7 @@
 // source/api/api_helper.h:25-45).
 #include "../../include/ausaxs_hip.h"
 
+#include "fitter.hpp"
 #include "grid.hpp"
 #include "hist.hpp"
 #include "rigid.hpp"
@@ -32,6 +33,10 @@ struct ausaxs_rigid {
 
 struct ausaxs_symmetry {
     std::unique_ptr<SymmetryStructure> s;
+};
+
+struct ausaxs_fitter {
+    std::unique_ptr<HistogramFitter> f;
 };
 
 struct ausaxs_server {
@@ -175,6 +180,60 @@ int ausaxs_structure_calculate(ausaxs_structure* s, ausaxs_histogram** out) {
 
 int ausaxs_structure_free(ausaxs_structure* s) {
     delete s;
+    return 0;
+}
+
+int ausaxs_fitter_create(ausaxs_histogram* h, const double* q, const double* I, const double* err, int n,
+                         unsigned enabled, const double* q_model, int nq_model, ausaxs_fitter** out) {
+    return guarded([&] {
+        require(h && out, "NULL argument");
+        require(n >= 3 && q && I, "a fit needs at least three data points");
+        require(nq_model >= 0 && (nq_model == 0 || q_model), "q_model is NULL");
+        auto f = std::make_unique<ausaxs_fitter>();
+        f->f = std::make_unique<HistogramFitter>(
+            *h->h, std::vector<double>(q, q + n), std::vector<double>(I, I + n),
+            err ? std::vector<double>(err, err + n) : std::vector<double>{}, enabled,
+            std::vector<double>(q_model, q_model + nq_model));
+        *out = f.release();
+    });
+}
+
+int ausaxs_fitter_params(const ausaxs_fitter* f, int* n_params, unsigned* enabled) {
+    return guarded([&] {
+        require(f != nullptr, "NULL fitter");
+        if (n_params) *n_params = f->f->n_params();
+        if (enabled) *enabled = f->f->enabled();
+    });
+}
+
+int ausaxs_fitter_evaluate(ausaxs_fitter* f, const double* params, int n_sets, double* out) {
+    return guarded([&] {
+        require(f && out && n_sets >= 0, "bad argument");
+        require(n_sets == 0 || f->f->n_params() == 0 || params, "params is NULL");
+        f->f->evaluate(params, n_sets, out);
+    });
+}
+
+int ausaxs_fitter_fit(ausaxs_fitter* f, int algorithm, int scan_evals, double* params, double* param_err,
+                      double* lin, int* dof, int* fevals, double* I_fit, double* residuals) {
+    return guarded([&] {
+        require(f != nullptr, "NULL fitter");
+        require(algorithm == 0 || algorithm == 2, "unknown minimizer (0 = golden, 2 = scan)");
+        const FitOutcome r = f->f->fit((FitAlgorithm)algorithm, scan_evals);
+        if (params) std::copy(r.params.begin(), r.params.end(), params);
+        if (param_err) std::copy(r.param_err.begin(), r.param_err.end(), param_err);
+        if (lin) {
+            lin[0] = r.a, lin[1] = r.b, lin[2] = r.a_err, lin[3] = r.b_err, lin[4] = r.chi2;
+        }
+        if (dof) *dof = r.dof;
+        if (fevals) *fevals = r.fevals;
+        if (I_fit) std::copy(r.I_fit.begin(), r.I_fit.end(), I_fit);
+        if (residuals) std::copy(r.residuals.begin(), r.residuals.end(), residuals);
+    });
+}
+
+int ausaxs_fitter_free(ausaxs_fitter* f) {
+    delete f;
     return 0;
 }
 

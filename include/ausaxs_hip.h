@@ -281,6 +281,38 @@ int ausaxs_symmetry_placement_schedule(const double* placements, int n, int* out
 int ausaxs_symmetry_cyclic(const double* cm, const double* offset, const double* step, const double* axis,
                            double angle, int reps, double* out, int* closed);
 
+/* ---- fitting a histogram to measured data ---------------------------------- */
+
+/* Replaces SmartFitter(data, histogram) (SmartFitter.cpp:50-58). The
+ * histogram must outlive the fitter. Data: n points (q, I, err; err NULL =
+ * unit errors). enabled: AUSAXS_FIT_* bits (EnabledFitParameters,
+ * SmartFitter.cpp:11-30); SIMPLE histograms keep only the hydration bit
+ * (validate_model, :32-49). q_model NULL: the default q axis on [1e-4, 0.5]
+ * (DistanceHistogram::get_q_axis). */
+#define AUSAXS_FIT_HYDRATION 1
+#define AUSAXS_FIT_EXCLUDED_VOLUME 2
+#define AUSAXS_FIT_SOLVENT_DENSITY 4
+#define AUSAXS_FIT_ATOMIC_DEBYE_WALLER 8
+#define AUSAXS_FIT_EXV_DEBYE_WALLER 16
+#define AUSAXS_MINI_GOLDEN 0 /* mini::algorithm::GOLDEN */
+#define AUSAXS_MINI_SCAN 2   /* mini::algorithm::SCAN */
+typedef struct ausaxs_fitter ausaxs_fitter;
+int ausaxs_fitter_create(ausaxs_histogram* h, const double* q, const double* I, const double* err, int n,
+                         unsigned enabled, const double* q_model, int nq_model, ausaxs_fitter** out);
+/* Number of free parameters and the enabled mask after validation. */
+int ausaxs_fitter_params(const ausaxs_fitter* f, int* n_params, unsigned* enabled);
+/* SmartFitter::chi2 (Fitter.cpp:10-13) for n_sets parameter vectors
+ * (n_sets x n_params, in bit order) in one batch: out[5*k..] = a, b, var a,
+ * var b, chi2 of the inner linear fit (LinearLeastSquares.cpp:26-44). */
+int ausaxs_fitter_evaluate(ausaxs_fitter* f, const double* params, int n_sets, double* out);
+/* SmartFitter::fit (SmartFitter.cpp:148-170) with mini::Golden or mini::Scan
+ * (Golden.cpp:22-70, Scan.cpp:26-59; one free parameter at most, as the
+ * reference's minimizers without dlib). params / param_err: n_params values;
+ * lin: a, b, a_err, b_err, chi2; I_fit and residuals: n values (optional). */
+int ausaxs_fitter_fit(ausaxs_fitter* f, int algorithm, int scan_evals, double* params, double* param_err,
+                      double* lin, int* dof, int* fevals, double* I_fit, double* residuals);
+int ausaxs_fitter_free(ausaxs_fitter* f);
+
 /* ---- synthetic structures (deterministic, xoshiro256** seeded by SplitMix64) */
 
 /* Self-avoiding 3.8 A chain packed in an ellipsoid with semi-axes (ax, ay, az),

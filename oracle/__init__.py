@@ -363,6 +363,53 @@ def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=800
     return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
 
 
+lib.orc_spline.restype = C.c_int
+lib.orc_spline.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+
+
+def spline(x, y, z):
+    """math::CubicSpline(x, y).spline(z) (see orc_spline)."""
+    x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(np.atleast_1d(z), np.float64)
+    out = np.zeros_like(z)
+    if lib.orc_spline(_p(x), _p(y), len(x), _p(z), len(z), _p(out)):
+        raise ValueError("a spline needs at least four knots")
+    return out
+
+
+def golden(f, lo, hi, tol=1e-4):
+    """mini::Golden::search + minimize_override (Golden.cpp:22-70): the
+    centre of the final interval, its half width and the evaluation count."""
+    invphi = 1 / ((1 + np.sqrt(5.0)) / 2)
+    invphi2 = invphi * invphi
+    evals = [0]
+
+    def F(v):
+        evals[0] += 1
+        return f(v)
+    a, b = min(lo, hi), (lo + hi) - min(lo, hi)
+    diff = b - a
+    if diff >= tol:
+        n = int(np.ceil(np.log(tol / diff) / np.log(invphi)))
+        c, d = a + invphi2 * diff, a + invphi * diff
+        fc, fd = F(c), F(d)
+        for _ in range(n - 1):
+            if fc < fd:
+                b, d, fd = d, c, fc
+                diff = invphi * diff
+                c = a + invphi2 * diff
+                fc = F(c)
+            else:
+                a, c, fc = c, d, fd
+                diff = invphi * diff
+                d = a + invphi * diff
+                fd = F(d)
+        a, b = (a, d) if fc < fd else (c, b)
+    centre = a + (b - a) / 2
+    F(centre)
+    return centre, (b - a) / 2, evals[0]
+
+
 def debye_simple(hist, q, bin_width, cw=1.0):
     """CompositeDistanceHistogram::debye_transform (CompositeDistanceHistogram.cpp:55-70 with
     DistanceHistogram.cpp:54-87): p = aa + cw*aw + cw^2*ww on p_tot's weighted axis, times exp(-q^2)."""

@@ -914,6 +914,58 @@ int orc_hist_symmetry(const float* atoms, const int* sizes, int n_bodies, const 
     return max_bin_of(ptot, nb);
 }
 
+int orc_spline(const double* x, const double* y, int n, const double* z, int m, double* out) {
+    if (n < 4) return -1;
+    double* h = (double*)malloc(sizeof(double) * n);
+    double* p = (double*)malloc(sizeof(double) * n);
+    double* D = (double*)malloc(sizeof(double) * n);
+    double* Q = (double*)malloc(sizeof(double) * n);
+    double* B = (double*)malloc(sizeof(double) * n);
+    double* b = (double*)malloc(sizeof(double) * n);
+    double* c = (double*)malloc(sizeof(double) * n);
+    double* d = (double*)malloc(sizeof(double) * n);
+    for (int i = 0; i < n - 1; ++i) {
+        h[i] = x[i + 1] - x[i];
+        p[i] = (y[i + 1] - y[i]) / h[i];
+    }
+    D[0] = 2;
+    Q[0] = 1;
+    B[0] = 3 * p[0];
+    D[n - 1] = 2;
+    B[n - 1] = 3 * p[n - 2];
+    for (int i = 0; i < n - 2; ++i) {
+        D[i + 1] = 2 * h[i] / h[i + 1] + 2;
+        Q[i + 1] = h[i] / h[i + 1];
+        B[i + 1] = 3 * (p[i] + p[i + 1] * h[i] / h[i + 1]);
+    }
+    for (int i = 1; i < n; ++i) {
+        D[i] -= Q[i - 1] / D[i - 1];
+        B[i] -= B[i - 1] / D[i - 1];
+    }
+    b[n - 1] = B[n - 1] / D[n - 1];
+    for (int i = n - 2; i >= 0; --i) b[i] = (B[i] - Q[i] * b[i + 1]) / D[i];
+    for (int i = 0; i < n - 1; ++i) {
+        c[i] = (-2 * b[i] - b[i + 1] + 3 * p[i]) / h[i];
+        d[i] = (b[i] + b[i + 1] - 2 * p[i]) / pow(h[i], 2);
+    }
+    for (int k = 0; k < m; ++k) {
+        /* CubicSpline::search: binary search for the last knot <= z (0 below
+         * the range), the last interval at or past the end */
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            int mid = lo + (hi - lo) / 2;
+            if (z[k] < x[mid]) hi = mid;
+            else lo = mid + 1;
+        }
+        int i = lo - 1 < 0 ? 0 : lo - 1;
+        if (i == n - 1) --i;
+        double t = z[k] - x[i];
+        out[k] = y[i] + b[i] * t + c[i] * pow(t, 2) + d[i] * pow(t, 3);
+    }
+    free(h), free(p), free(D), free(Q), free(B), free(b), free(c), free(d);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* weighted axis, sinc table, contraction                                   */
 /* ------------------------------------------------------------------------ */

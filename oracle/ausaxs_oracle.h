@@ -133,6 +133,10 @@ int orc_hist_symmetry(const float* atoms, const int* sizes, int n_bodies, const 
                       const float* waters, int n_w, float inv_w, int nb, int nthreads, orc_entry* aa,
                       orc_entry* aw, orc_entry* ww, orc_entry* ptot);
 
+/* math::CubicSpline (CubicSpline.cpp): knots (x, y), n >= 4, evaluated at
+ * m points z. Returns 0, or -1 for fewer than four knots. */
+int orc_spline(const double* x, const double* y, int n, const double* z, int m, double* out);
+
 /* ---- weighted axis, sinc, Debye ------------------------------------------ */
 /* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
  * followed by DistanceHistogram::initialize (DistanceHistogram.cpp:44-47, d[0]=0). */
