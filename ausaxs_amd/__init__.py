@@ -96,6 +96,10 @@ def _load():
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
         "ausaxs_fitter_create": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_uint, vp, C.c_int, P(vp)]),
+        "ausaxs_scalable_create": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
+        "ausaxs_scalable_calculate": (C.c_int, [vp, C.c_double, P(vp)]),
+        "ausaxs_scalable_free": (C.c_int, [vp]),
+        "ausaxs_fitter_create_scalable": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_uint, vp, C.c_int, P(vp)]),
         "ausaxs_fitter_params": (C.c_int, [vp, P(C.c_int), P(C.c_uint)]),
         "ausaxs_fitter_evaluate": (C.c_int, [vp, vp, C.c_int, vp]),
         "ausaxs_fitter_fit": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, P(C.c_int), P(C.c_int), vp, vp]),
@@ -425,9 +429,37 @@ FIT_HYDRATION, FIT_EXCLUDED_VOLUME, FIT_SOLVENT_DENSITY, FIT_ATOMIC_DW, FIT_EXV_
 MINI_GOLDEN, MINI_SCAN = 0, 2
 
 
+class ScalableGridStructure:
+    """HistogramManagerMTFFGridScalableExv: the grid structure whose dummies
+    can be stretched; calculate(k) is the FFGrid histogram at stretch k."""
+
+    def __init__(self, atoms_xyz, ff, waters_xyz=None, settings=None):
+        self._xyz = np.ascontiguousarray(np.asarray(atoms_xyz, np.float64).reshape(-1, 3))
+        self._ff = np.ascontiguousarray(np.asarray(ff, np.int32).reshape(-1))
+        self._w = np.ascontiguousarray(np.zeros((0, 3)) if waters_xyz is None else
+                                       np.asarray(waters_xyz, np.float64).reshape(-1, 3))
+        s = settings or default_settings()
+        h = C.c_void_p()
+        _check(lib.ausaxs_scalable_create(_ptr(self._xyz), _ptr(self._ff), len(self._xyz), _ptr(self._w),
+                                          len(self._w), C.byref(s), C.byref(h)))
+        self._s = h
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.ausaxs_scalable_free(s)
+            self._s = C.c_void_p(0)
+
+    def calculate(self, k=1.0) -> CompositeHistogram:
+        h = C.c_void_p()
+        _check(lib.ausaxs_scalable_calculate(self._s, float(k), C.byref(h)))
+        return CompositeHistogram(h.value)
+
+
 class Fitter:
     """SmartFitter(data, histogram): fit of a composite histogram to measured
-    I(q). The histogram must stay alive while the fitter is used."""
+    I(q). The histogram (or scalable grid structure) must stay alive while the
+    fitter is used."""
 
     def __init__(self, hist, q, I, err=None, enabled=FIT_HYDRATION, q_model=None):
         self._hist = hist
@@ -436,8 +468,11 @@ class Fitter:
         self.err = None if err is None else np.ascontiguousarray(err, np.float64)
         qm = None if q_model is None else np.ascontiguousarray(q_model, np.float64)
         h = C.c_void_p()
-        _check(lib.ausaxs_fitter_create(hist._h, _ptr(self.q), _ptr(self.I), _ptr(self.err), len(self.q),
-                                        int(enabled), _ptr(qm), 0 if qm is None else len(qm), C.byref(h)))
+        make = lib.ausaxs_fitter_create_scalable if isinstance(hist, ScalableGridStructure) else \
+            lib.ausaxs_fitter_create
+        handle = hist._s if isinstance(hist, ScalableGridStructure) else hist._h
+        _check(make(handle, _ptr(self.q), _ptr(self.I), _ptr(self.err), len(self.q),
+                    int(enabled), _ptr(qm), 0 if qm is None else len(qm), C.byref(h)))
         self._f = h
         k, en = C.c_int(), C.c_uint()
         _check(lib.ausaxs_fitter_params(self._f, C.byref(k), C.byref(en)))

@@ -35,6 +35,10 @@ struct ausaxs_symmetry {
     std::unique_ptr<SymmetryStructure> s;
 };
 
+struct ausaxs_scalable {
+    std::unique_ptr<ScalableGridStructure> s;
+};
+
 struct ausaxs_fitter {
     std::unique_ptr<HistogramFitter> f;
 };
@@ -192,6 +196,49 @@ int ausaxs_fitter_create(ausaxs_histogram* h, const double* q, const double* I, 
         auto f = std::make_unique<ausaxs_fitter>();
         f->f = std::make_unique<HistogramFitter>(
             *h->h, std::vector<double>(q, q + n), std::vector<double>(I, I + n),
+            err ? std::vector<double>(err, err + n) : std::vector<double>{}, enabled,
+            std::vector<double>(q_model, q_model + nq_model));
+        *out = f.release();
+    });
+}
+
+int ausaxs_scalable_create(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                          int n_waters, const ausaxs_settings* settings, ausaxs_scalable** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_waters == 0 || waters_xyz, "waters is NULL");
+        auto s = std::make_unique<ausaxs_scalable>();
+        s->s = std::make_unique<ScalableGridStructure>(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters,
+                                                       to_settings(settings));
+        *out = s.release();
+    });
+}
+
+int ausaxs_scalable_calculate(ausaxs_scalable* s, double k, ausaxs_histogram** out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        auto h = std::make_unique<ausaxs_histogram>();
+        h->h = s->s->calculate(k);
+        *out = h.release();
+    });
+}
+
+int ausaxs_scalable_free(ausaxs_scalable* s) {
+    delete s;
+    return 0;
+}
+
+int ausaxs_fitter_create_scalable(ausaxs_scalable* s, const double* q, const double* I, const double* err, int n,
+                                  unsigned enabled, const double* q_model, int nq_model, ausaxs_fitter** out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        require(n >= 3 && q && I, "a fit needs at least three data points");
+        require(nq_model >= 0 && (nq_model == 0 || q_model), "q_model is NULL");
+        auto f = std::make_unique<ausaxs_fitter>();
+        f->f = std::make_unique<HistogramFitter>(
+            *s->s, std::vector<double>(q, q + n), std::vector<double>(I, I + n),
             err ? std::vector<double>(err, err + n) : std::vector<double>{}, enabled,
             std::vector<double>(q_model, q_model + nq_model));
         *out = f.release();

@@ -67,8 +67,21 @@ constexpr ParamSpec kSpecs[5] = {{kFitHydration, 0, 10, 1},
 
 HistogramFitter::HistogramFitter(CompositeDistanceHistogramHIP& h, std::vector<double> q, std::vector<double> I,
                                  std::vector<double> err, unsigned enabled, std::vector<double> q_model)
-    : h_(h), q_(std::move(q)), y_(std::move(I)), err_(std::move(err)), q_model_(std::move(q_model)),
+    : h_(&h), q_(std::move(q)), y_(std::move(I)), err_(std::move(err)), q_model_(std::move(q_model)),
       enabled_(enabled & 31u), base_(h.params) {
+    init();
+}
+
+HistogramFitter::HistogramFitter(ScalableGridStructure& s, std::vector<double> q, std::vector<double> I,
+                                 std::vector<double> err, unsigned enabled, std::vector<double> q_model)
+    : scalable_(&s), owned_(s.calculate(1.0)), q_(std::move(q)), y_(std::move(I)), err_(std::move(err)),
+      q_model_(std::move(q_model)), enabled_(enabled & 31u) {
+    h_ = owned_.get();
+    base_ = h_->params;
+    init();
+}
+
+void HistogramFitter::init() {
     if (q_.size() != y_.size() || (!err_.empty() && err_.size() != q_.size()))
         throw std::runtime_error("ausaxs_hip: data q, I and errors differ in size");
     if (q_.size() < 3) throw std::runtime_error("ausaxs_hip: a fit needs at least three data points");
@@ -76,14 +89,14 @@ HistogramFitter::HistogramFitter(CompositeDistanceHistogramHIP& h, std::vector<d
     if (q_model_.size() < 4) throw std::runtime_error("ausaxs_hip: the model q axis needs at least four points");
     // validate_model: a histogram without excluded-volume terms fits the
     // hydration factor only
-    if (h_.mode == Mode::Simple) enabled_ &= kFitHydration;
+    if (h_->mode == Mode::Simple) enabled_ &= kFitHydration;
     Context& ctx = Context::get();
     d_y_.upload(y_.data(), y_.size(), ctx.stream());
     if (!err_.empty()) d_err_.upload(err_.data(), err_.size(), ctx.stream());
     ctx.sync();
 }
 
-HistogramFitter::~HistogramFitter() { h_.params = base_; }
+HistogramFitter::~HistogramFitter() { h_->params = base_; }
 
 int HistogramFitter::n_params() const {
     int n = 0;
@@ -95,17 +108,27 @@ void HistogramFitter::apply(const double* p) {
     FreeParams f = base_;
     int k = 0;
     if (enabled_ & kFitHydration) f.cw = p[k++];
-    if (enabled_ & kFitExcludedVolume) f.cx = p[k++];
+    if (enabled_ & kFitExcludedVolume) {
+        const double v = p[k++];
+        if (!scalable_) {
+            f.cx = v;
+        } else if (v != owned_k_) {
+            // the stretched dummies' histograms replace the current ones
+            owned_ = scalable_->calculate(v);
+            owned_k_ = v;
+            h_ = owned_.get();
+        }
+    }
     if (enabled_ & kFitSolventDensity) f.crho = p[k++];
     if (enabled_ & kFitAtomicDW) f.dw_atomic = p[k++];
     if (enabled_ & kFitExvDW) f.dw_exv = p[k++];
-    h_.params = f;
+    h_->params = f;
 }
 
 std::vector<double> HistogramFitter::model_curve(const double* params) {
     apply(params);
     // SmartFitter::splice (SmartFitter.cpp:246-252)
-    const CubicSpline s(q_model_, h_.debye_transform(q_model_));
+    const CubicSpline s(q_model_, h_->debye_transform(q_model_));
     std::vector<double> m(q_.size());
     for (size_t i = 0; i < q_.size(); ++i) m[i] = s(q_[i]);
     return m;
@@ -119,7 +142,7 @@ void HistogramFitter::evaluate(const double* params, int P, double* out) {
         const std::vector<double> m = model_curve(params + (size_t)p * k);
         std::copy(m.begin(), m.end(), models.begin() + (size_t)p * n);
     }
-    h_.params = base_;
+    h_->params = base_;
     fevals_ += P;
     Context& ctx = Context::get();
     d_models_.upload(models.data(), models.size(), ctx.stream());
@@ -206,7 +229,7 @@ FitOutcome HistogramFitter::fit(FitAlgorithm alg, int scan_evals) {
     double out[5];
     evaluate(r.params.data(), 1, out);
     r.model = model_curve(r.params.data());
-    h_.params = base_;
+    h_->params = base_;
     r.a = out[0], r.b = out[1], r.a_err = std::sqrt(out[2]), r.b_err = std::sqrt(out[3]), r.chi2 = out[4];
     r.dof = (int)q_.size() - 2 - k;
     r.I_fit.resize(q_.size());

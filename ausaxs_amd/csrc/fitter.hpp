@@ -13,6 +13,7 @@
 
 #include "hist.hpp"
 
+#include <memory>
 #include <vector>
 
 namespace ausaxs::gpu {
@@ -55,6 +56,12 @@ class HistogramFitter {
     // [1e-4, 0.5] (DistanceHistogram::get_q_axis). err empty: unit errors.
     HistogramFitter(CompositeDistanceHistogramHIP& h, std::vector<double> q, std::vector<double> I,
                     std::vector<double> err, unsigned enabled, std::vector<double> q_model = {});
+    // A grid structure with stretchable dummies (must outlive the fitter):
+    // the excluded-volume parameter recomputes the histograms with the
+    // dummies stretched by its value (CompositeDistanceHistogramFFGridScalableExv::
+    // apply_excluded_volume_scaling_factor) instead of scaling the exv terms.
+    HistogramFitter(ScalableGridStructure& s, std::vector<double> q, std::vector<double> I,
+                    std::vector<double> err, unsigned enabled, std::vector<double> q_model = {});
     ~HistogramFitter();
 
     // enabled parameters after validate_model (SmartFitter.cpp:32-49):
@@ -69,13 +76,17 @@ class HistogramFitter {
     FitOutcome fit(FitAlgorithm alg, int scan_evals = 100);
 
   private:
-    CompositeDistanceHistogramHIP& h_;
+    CompositeDistanceHistogramHIP* h_;
+    ScalableGridStructure* scalable_ = nullptr;
+    std::unique_ptr<CompositeDistanceHistogramHIP> owned_;
+    double owned_k_ = 1;
     std::vector<double> q_, y_, err_, q_model_;
     unsigned enabled_;
     FreeParams base_;
     int fevals_ = 0;
     DevBuf<double> d_y_, d_err_, d_models_, d_out_;
 
+    void init();
     void apply(const double* p);
     double chi2_one(double v);  // 1-D objective (mini::Minimizer function)
     struct Interval {

@@ -563,6 +563,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             res->d_axis_ax = axis_full(pax);
             res->d_axis_xx = axis_full(res->p_xx);
             res->grid_volume = grid_volume_;
+            res->grid_exv_volume = grid_exv_volume_;
             ptot = std::move(grid_ptot);
         }
         res->max_bin = max_bin;
@@ -722,6 +723,7 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
         return I;
     }
     const double rm2 = 1.62 * 1.62 / 4;  // FFExplicitBase.cpp:54-57
+    const double gx_q0 = grid_exv_volume * 0.334, gx_exp = std::pow(grid_exv_volume, 2. / 3) / (4 * kPi);
     for (int k = 0; k < nq; ++k) {
         const double qk = q[k];
         const double cx = mode != Mode::FFExplicit ? p.cx : std::pow(p.cx, 3) * std::exp(-rm2 * (p.cx * p.cx - 1) * qk * qk);
@@ -729,8 +731,9 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
         double fa[kMaxFF], fe[kMaxFF];
         for (int f = 0; f < kMaxFF; ++f) fa[f] = ff_atomic(f, qk);
         for (int f = 1; f < kMaxFF; ++f) fe[f] = ff_exv(f, qk);
-        // exv form factor of the FFAvg table, or of the grid dummies (V = 1 A^3)
-        const double fx = mode == Mode::FFGrid ? 0.334 * std::exp(-(1.0 / (4 * kPi)) * qk * qk) : fa[kExvBin];
+        // exv form factor of the FFAvg table, or of the grid dummies
+        // (ExvFormFactor(V), ExvFormFactor.h:23-34; V = 1 A^3 unless stretched)
+        const double fx = mode == Mode::FFGrid ? gx_q0 * std::exp(-gx_exp * qk * qk) : fa[kExvBin];
         for (size_t r = 0; r < row_keys_.size(); ++r) {
             const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
             const double s = S[r * nq + k];
@@ -803,6 +806,34 @@ std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, 
     pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);
     return std::make_unique<ResidentStructure>(Mode::FFGrid, a.data(), n, w.data(), m, s, 0.0, g.xyz.data(),
                                                (int)(g.xyz.size() / 3), g.volume);
+}
+
+ScalableGridStructure::ScalableGridStructure(const double* atoms_xyz, const int32_t* ff, int n,
+                                             const double* waters_xyz, int m, const HistSettings& s)
+    : n_(n), m_(m), s_(s) {
+    for (int i = 0; i < n; ++i)
+        if (ff[i] < 1 || ff[i] >= kMaxFF)
+            throw std::runtime_error("ausaxs_hip: atom form factor type outside [1, 14]");
+    GridExv g = make_grid_exv(atoms_xyz, ff, n, waters_xyz, m);
+    dummies_ = std::move(g.xyz);
+    grid_volume_ = g.volume;
+    pack_ff_points(atoms_xyz, ff, n, 0, atoms_);
+    pack_ff_points(waters_xyz, nullptr, m, kWaterBin, waters_);
+}
+
+std::unique_ptr<CompositeDistanceHistogramHIP> ScalableGridStructure::calculate(double k) {
+    if (!(k > 0) || !std::isfinite(k)) throw std::runtime_error("ausaxs_hip: the excluded-volume scale must be positive");
+    if (!st_ || k != k_) {
+        // eval_scaled_exv: coord.value.pos *= scale on the float dummies
+        std::vector<double> x(dummies_.size());
+        for (size_t i = 0; i < x.size(); ++i) x[i] = (double)(float)dummies_[i] * k;
+        st_ = std::make_unique<ResidentStructure>(Mode::FFGrid, atoms_.data(), n_, waters_.data(), m_, s_, 0.0,
+                                                  x.data(), (int)(x.size() / 3), grid_volume_);
+        // V = (exv.width * k)^3 with the default 1 A width (CompositeDistanceHistogramFFGridScalableExv.cpp:17)
+        st_->set_grid_exv_volume(std::pow(1.0 * k, 3));
+        k_ = k;
+    }
+    return st_->calculate();
 }
 
 void raw_pair_histogram(const float* a, int n, const float* b, int m, const HistSettings& s, uint64_t* count,

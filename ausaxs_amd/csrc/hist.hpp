@@ -104,6 +104,9 @@ class CompositeDistanceHistogramHIP {
     std::vector<double> d_axis_ax, d_axis_xx;
     std::vector<WeightedEntry> p_tot_ax, p_xx;
     double grid_volume = 0;
+    // FFGrid: volume V of the dummies' ExvFormFactor (width^3; (width k)^3
+    // for dummies stretched by k, CompositeDistanceHistogramFFGridScalableExv.cpp:9-19)
+    double grid_exv_volume = 1;
     FreeParams params;
 
     // ICompositeDistanceHistogram(Exv)::debye_transform on an explicit q grid;
@@ -134,13 +137,14 @@ class ResidentStructure {
     std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
     PairEngine& engine() { return *engine_; }
     Mode mode() const { return mode_; }
+    void set_grid_exv_volume(double v) { grid_exv_volume_ = v; }
 
   private:
     Mode mode_;
     HistSettings s_;
     double z_exv_;
     int n_atoms_, n_waters_, n_exv_ = 0;
-    double grid_volume_ = 0;
+    double grid_volume_ = 0, grid_exv_volume_ = 1;
     std::vector<int32_t> atom_ff_;                // FF modes
     std::vector<float> atom_w_, water_w_;         // Simple mode
     std::vector<float> atom_class_w_, water_class_w_;
@@ -155,6 +159,29 @@ class ResidentStructure {
 // of the coordinates as CompactCoordinatesFF holds them.
 std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, const int32_t* ff, int n,
                                                        const double* waters_xyz, int m, const HistSettings& s);
+// HistogramManagerMTFFGridScalableExv (HistogramManagerMTFFGridScalableExv.cpp:16-244):
+// the FFGrid histograms with the dummies stretched by a factor k about the
+// origin (float coordinates times k, as CompactCoordinatesFF stores them) and
+// the dummy form factor of volume k^3. The grid and its dummies are built
+// once; calculate(k) recomputes the pair histograms on the GPU.
+class ScalableGridStructure {
+  public:
+    ScalableGridStructure(const double* atoms_xyz, const int32_t* ff, int n, const double* waters_xyz, int m,
+                          const HistSettings& s);
+    std::unique_ptr<CompositeDistanceHistogramHIP> calculate(double k);
+    int n_dummies() const { return (int)(dummies_.size() / 3); }
+    int64_t pairs() const { return st_ ? st_->engine().pairs() : 0; }
+
+  private:
+    std::vector<float> atoms_, waters_;
+    std::vector<double> dummies_;
+    double grid_volume_ = 0;
+    int n_ = 0, m_ = 0;
+    HistSettings s_;
+    double k_ = 0;
+    std::unique_ptr<ResidentStructure> st_;
+};
+
 // The FF-manager points of FP64 atoms/waters: {float x, y, z, type bits},
 // waters typed WATER.
 void pack_ff_points(const double* xyz, const int32_t* ff, int n, int32_t fixed_type, std::vector<float>& out);

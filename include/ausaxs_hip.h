@@ -281,6 +281,19 @@ int ausaxs_symmetry_placement_schedule(const double* placements, int n, int* out
 int ausaxs_symmetry_cyclic(const double* cm, const double* offset, const double* step, const double* axis,
                            double angle, int reps, double* out, int* closed);
 
+/* ---- grid excluded volume with stretchable dummies -------------------------- */
+
+/* Replaces HistogramManagerMTFFGridScalableExv (HistogramManagerMTFFGridScalableExv.cpp:16-244)
+ * and CompositeDistanceHistogramFFGridScalableExv (:9-19). The grid and its
+ * dummies are built once (as ausaxs_structure_create_grid); calculate(k)
+ * returns the FFGrid histogram with the dummies' float coordinates multiplied
+ * by k and the dummy form factor of volume k^3 (exv width 1 A). */
+typedef struct ausaxs_scalable ausaxs_scalable;
+int ausaxs_scalable_create(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                          int n_waters, const ausaxs_settings* settings, ausaxs_scalable** out);
+int ausaxs_scalable_calculate(ausaxs_scalable* s, double k, ausaxs_histogram** out);
+int ausaxs_scalable_free(ausaxs_scalable* s);
+
 /* ---- fitting a histogram to measured data ---------------------------------- */
 
 /* Replaces SmartFitter(data, histogram) (SmartFitter.cpp:50-58). The
@@ -299,6 +312,11 @@ int ausaxs_symmetry_cyclic(const double* cm, const double* offset, const double*
 typedef struct ausaxs_fitter ausaxs_fitter;
 int ausaxs_fitter_create(ausaxs_histogram* h, const double* q, const double* I, const double* err, int n,
                          unsigned enabled, const double* q_model, int nq_model, ausaxs_fitter** out);
+/* The same fitter on a scalable grid structure: the excluded-volume parameter
+ * stretches the dummies (apply_excluded_volume_scaling_factor of
+ * CompositeDistanceHistogramFFGridScalableExv) and recomputes the histograms. */
+int ausaxs_fitter_create_scalable(ausaxs_scalable* s, const double* q, const double* I, const double* err, int n,
+                                  unsigned enabled, const double* q_model, int nq_model, ausaxs_fitter** out);
 /* Number of free parameters and the enabled mask after validation. */
 int ausaxs_fitter_params(const ausaxs_fitter* f, int* n_params, unsigned* enabled);
 /* SmartFitter::chi2 (Fitter.cpp:10-13) for n_sets parameter vectors

@@ -52,7 +52,7 @@ def _load():
         f.restype, f.argtypes = res, args
 
     class Params(C.Structure):
-        _fields_ = [(k, d) for k in ("cw", "cx", "crho", "dw_atomic", "dw_exv")]
+        _fields_ = [(k, d) for k in ("cw", "cx", "crho", "dw_atomic", "dw_exv", "grid_v")]
 
     lib.orc_profiles.restype = None
     lib.orc_profiles.argtypes = [i, vp, i, vp, vp, vp, Params, vp]
@@ -155,14 +155,16 @@ def ff_exv(t, q):
     return lib.orc_ff_exv(t, q)
 
 
-def profiles(mode, q, sinqd_aa, sinqd_aw, sinqd_ww, cw=1.0, cx=1.0, crho=1.0, dw_atomic=0.0, dw_exv=0.0):
+def profiles(mode, q, sinqd_aa, sinqd_aw, sinqd_ww, cw=1.0, cx=1.0, crho=1.0, dw_atomic=0.0, dw_exv=0.0,
+             grid_exv_volume=1.0):
     q = np.ascontiguousarray(q, np.float64)
     nq = len(q)
     a3 = np.ascontiguousarray(sinqd_aa, np.float64).reshape(MAX_FF * MAX_FF * nq)
     a2 = np.ascontiguousarray(sinqd_aw, np.float64).reshape(MAX_FF * nq)
     a1 = np.ascontiguousarray(sinqd_ww, np.float64).reshape(nq)
     out = np.zeros((6, nq), np.float64)
-    lib.orc_profiles(mode, _p(q), nq, _p(a3), _p(a2), _p(a1), lib.Params(cw, cx, crho, dw_atomic, dw_exv), _p(out))
+    lib.orc_profiles(mode, _p(q), nq, _p(a3), _p(a2), _p(a1),
+                     lib.Params(cw, cx, crho, dw_atomic, dw_exv, grid_exv_volume), _p(out))
     return out
 
 
@@ -246,13 +248,16 @@ def pack_ff(xyz, ff):
     return out
 
 
-def hist_ffgrid(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1):
-    """HistogramManagerMTFFGrid::calculate_all from FP64 coordinates."""
+def hist_ffgrid(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1, exv_scale=1.0):
+    """HistogramManagerMTFFGrid::calculate_all from FP64 coordinates; exv_scale != 1:
+    HistogramManagerMTFFGridScalableExv's eval_scaled_exv (float dummy coordinates times k)."""
     w = np.zeros((0, 3)) if waters_xyz is None else np.asarray(waters_xyz, np.float64)
     dummies, vol = grid_exv(atoms_xyz, ff, w)
     atoms = pack_ff(atoms_xyz, ff)
     waters = pack_ff(w, np.ones(len(w), np.int32))
     xs = pack_ff(dummies, np.zeros(len(dummies), np.int32))
+    if exv_scale != 1.0:
+        xs[:, :3] = (xs[:, :3].astype(np.float64) * exv_scale).astype(np.float32)
     nb = bin_count
     aa = np.zeros(MAX_FF * MAX_FF * nb, ENTRY)
     aw = np.zeros(MAX_FF * nb, ENTRY)

@@ -1130,8 +1130,11 @@ void orc_profiles(int mode, const double* q, int nq, const double* sinqd_aa, con
         double cx = mode != ORC_FFEXPLICIT ? p.cx : pow(p.cx, 3) * exp(-rm2 * (pow(p.cx, 2) - 1) * qk * qk);
         double fw = orc_ff_atomic(ORC_WATER, qk);
         /* exv form factor: FFAvg's raw EXCLUDED_VOLUME entry, or FFGrid's
-         * ExvFormFactor(V = 1 A^3) (CompositeDistanceHistogramFFGrid.cpp:36-75) */
-        double fx = mode == ORC_FFGRID ? 0.334 * exp(-(1.0 / (4 * PI_)) * qk * qk) : orc_ff_atomic(ORC_EXV, qk);
+         * ExvFormFactor(V) (CompositeDistanceHistogramFFGrid.cpp:36-75,
+         * ExvFormFactor.h:23-34): q0 = 0.334 V, exponent V^(2/3) / (4 pi) */
+        double gv = p.grid_v > 0 ? p.grid_v : 1.0;
+        double fx = mode == ORC_FFGRID ? 0.334 * gv * exp(-(pow(gv, 2. / 3) / (4 * PI_)) * qk * qk)
+                                       : orc_ff_atomic(ORC_EXV, qk);
         for (int f1 = 1; f1 < ORC_MAX_FF; ++f1) {
             double a1 = orc_ff_atomic(f1, qk);
             for (int f2 = 1; f2 < ORC_MAX_FF; ++f2) {

@@ -164,6 +164,7 @@ double orc_ff_exv(int type, double q);
 /* Free parameters of ICompositeDistanceHistogramExv (FFAvgBase.h). */
 typedef struct {
     double cw, cx, crho, dw_atomic, dw_exv;
+    double grid_v; /* FFGrid: ExvFormFactor volume of the dummies (0 reads as 1 A^3) */
 } orc_params;
 
 /* Six partial profiles {aa, ax, aw, xx, wx, ww} on an arbitrary q grid:
