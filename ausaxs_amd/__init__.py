@@ -106,6 +106,11 @@ def _load():
         "ausaxs_fitter_free": (C.c_int, [vp]),
         "ausaxs_symmetry_create": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, P(_Settings),
                                              P(vp)]),
+        "ausaxs_symmetry_create_partial": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int,
+                                                     P(_Settings), P(vp)]),
+        "ausaxs_symmetry_set_body": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "ausaxs_symmetry_set_symmetry": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp]),
+        "ausaxs_symmetry_set_waters": (C.c_int, [vp, vp, C.c_int]),
         "ausaxs_symmetry_calculate": (C.c_int, [vp, P(vp)]),
         "ausaxs_symmetry_free": (C.c_int, [vp]),
         "ausaxs_symmetry_info": (C.c_int, [vp, P(C.c_longlong), P(C.c_int)]),
@@ -555,8 +560,11 @@ class SymmetryStructure:
     bodies: list of (n_b, 4) float arrays {x, y, z, w}; symmetries: list (one
     per body) of lists of Symmetry; waters: (m, 4) floats or None."""
 
-    def __init__(self, bodies, symmetries, waters=None, settings=None):
+    def __init__(self, bodies, symmetries, waters=None, settings=None, partial=False):
+        """partial=True: PartialSymmetryManagerMT (its pair roles, self terms
+        kept from the last internal change; see set_body)."""
         bodies = [_pts(b) for b in bodies]
+        self._n = [len(b) for b in bodies]
         if len(symmetries) != len(bodies):
             raise ValueError("one symmetry list per body")
         self._atoms = np.ascontiguousarray(np.concatenate(bodies) if bodies else np.zeros((0, 4), np.float32))
@@ -572,11 +580,28 @@ class SymmetryStructure:
         self._waters = _pts(waters)
         s = settings or default_settings()
         h = C.c_void_p()
-        _check(lib.ausaxs_symmetry_create(
+        create = lib.ausaxs_symmetry_create_partial if partial else lib.ausaxs_symmetry_create
+        _check(create(
             _ptr(self._atoms), _ptr(self._sizes), len(self._sizes), _ptr(self._per), _ptr(self._reps),
             _ptr(self._T), _ptr(self._slen), _ptr(self._sched), _ptr(self._waters), len(self._waters),
             C.byref(s), C.byref(h)))
         self._s = h
+
+    def set_body(self, b, points, internal=False):
+        """New float XYZW coordinates of body b (same size): a rigid move, or
+        with internal=True a change of its own distances or weights."""
+        p = _pts(points)
+        if len(p) != self._n[b]:
+            raise ValueError("a body keeps its size")
+        _check(lib.ausaxs_symmetry_set_body(self._s, int(b), _ptr(p), int(bool(internal))))
+
+    def set_symmetry(self, b, k, sym):
+        _check(lib.ausaxs_symmetry_set_symmetry(self._s, int(b), int(k), sym.reps, _ptr(sym.transforms),
+                                                len(sym.schedule), _ptr(sym.schedule)))
+
+    def set_waters(self, waters):
+        w = _pts(waters)
+        _check(lib.ausaxs_symmetry_set_waters(self._s, _ptr(w), len(w)))
 
     def __del__(self):
         s = getattr(self, "_s", None)

@@ -284,10 +284,11 @@ int ausaxs_fitter_free(ausaxs_fitter* f) {
     return 0;
 }
 
-int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
-                           const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
-                           const float* waters, int n_waters, const ausaxs_settings* settings,
-                           ausaxs_symmetry** out) {
+namespace {
+int symmetry_create(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                    const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                    const float* waters, int n_waters, const ausaxs_settings* settings, bool partial,
+                    ausaxs_symmetry** out) {
     return guarded([&] {
         require(out != nullptr, "out is NULL");
         require(n_bodies >= 0 && n_waters >= 0, "negative sizes");
@@ -321,8 +322,54 @@ int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodi
         }
         auto s = std::make_unique<ausaxs_symmetry>();
         s->s = std::make_unique<SymmetryStructure>(atoms, sizes, std::move(syms), waters, n_waters,
-                                                   to_settings(settings));
+                                                   to_settings(settings), partial);
         *out = s.release();
+    });
+}
+}  // namespace
+
+int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                           const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
+                           const float* waters, int n_waters, const ausaxs_settings* settings,
+                           ausaxs_symmetry** out) {
+    return symmetry_create(atoms, body_sizes, n_bodies, sym_per_body, sym_reps, transforms, sched_len, schedule,
+                           waters, n_waters, settings, false, out);
+}
+
+int ausaxs_symmetry_create_partial(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                                   const int* sym_reps, const double* transforms, const int* sched_len,
+                                   const int* schedule, const float* waters, int n_waters,
+                                   const ausaxs_settings* settings, ausaxs_symmetry** out) {
+    return symmetry_create(atoms, body_sizes, n_bodies, sym_per_body, sym_reps, transforms, sched_len, schedule,
+                           waters, n_waters, settings, true, out);
+}
+
+int ausaxs_symmetry_set_body(ausaxs_symmetry* s, int body, const float* atoms, int internal) {
+    return guarded([&] {
+        require(s && atoms, "NULL argument");
+        s->s->set_body(body, atoms, internal != 0);
+    });
+}
+
+int ausaxs_symmetry_set_symmetry(ausaxs_symmetry* s, int body, int symmetry, int reps, const double* transforms,
+                                 int sched_len, const int* schedule) {
+    return guarded([&] {
+        require(s != nullptr, "NULL structure");
+        require(reps >= 1 && transforms, "a symmetry needs at least one repetition");
+        require(sched_len >= 0 && (sched_len == 0 || schedule), "schedule is NULL");
+        SymmetrySpec sp;
+        sp.reps = reps;
+        sp.transforms.assign(transforms, transforms + 12 * (size_t)reps);
+        for (int j = 0; j < sched_len; ++j)
+            sp.schedule.push_back(CopyPair{schedule[3 * j], schedule[3 * j + 1], schedule[3 * j + 2]});
+        s->s->set_symmetry(body, symmetry, std::move(sp));
+    });
+}
+
+int ausaxs_symmetry_set_waters(ausaxs_symmetry* s, const float* waters, int n_waters) {
+    return guarded([&] {
+        require(s != nullptr && n_waters >= 0 && (n_waters == 0 || waters), "bad argument");
+        s->s->set_waters(waters, n_waters);
     });
 }
 

@@ -258,7 +258,6 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     int n_rows = 0;
 
     if (mode == Mode::FFAvg || mode == Mode::FFExplicit || mode == Mode::FFGrid) {
-        atom_ff_.resize(n_atoms);
         std::set<int> present;
         for (int i = 0; i < n_atoms; ++i) {
             int32_t f;
@@ -267,7 +266,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
             if (f < 1 || f >= kMaxFF)
                 throw std::runtime_error("ausaxs_hip: atom " + std::to_string(i) + " has form factor type " +
                                          std::to_string(f) + " outside [1, 14]");
-            atom_ff_[i] = f;
+            ++ff_count_[f];
             A.orig[i].w = float_of(f);
             present.insert(f);
         }
@@ -342,8 +341,12 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
             if (cw.empty()) cw.push_back(0.f);  // empty set: one (unused) class keeps row tables non-empty
             ps.n_classes = std::max<int>(1, (int)cw.size());
         };
-        classify(atoms, n_atoms, A, atom_w_, atom_class_w_);
-        classify(waters, n_waters, W, water_w_, water_class_w_);
+        std::vector<float> atom_w, water_w;
+        classify(atoms, n_atoms, A, atom_w, atom_class_w_);
+        classify(waters, n_waters, W, water_w, water_class_w_);
+        // SimpleCalculator.h:144-160: sum of w*w (float products) in point order
+        for (float x : atom_w) self_w2_atoms_ += (double)(x * x);
+        for (float x : water_w) self_w2_waters_ += (double)(x * x);
         const int CA = A.n_classes, CW = W.n_classes;
         PairJob aa, ww, aw;
         aa.row_set = 0;
@@ -468,8 +471,11 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
                 if (const WeightedEntry* src = aw.row(f, 0))
                     for (int b = 0; b < nbw; ++b) add(wexv[b], src[b]);
         }
-        // self correlations (FFAvg.cpp:308-323, FFExplicit.cpp:331-346)
-        for (int i = 0; i < n_atoms_; ++i) add(aa.touch(atom_ff_[i], atom_ff_[i])[0], WeightedEntry{1, 1, 0});
+        // self correlations (FFAvg.cpp:308-323, FFExplicit.cpp:331-346): one
+        // (1, 1, 0) per atom of a type; integer sums, so one add per type is
+        // the same
+        for (int f = 0; f < kMaxFF; ++f)
+            if (ff_count_[f]) add(aa.touch(f, f)[0], WeightedEntry{(double)ff_count_[f], ff_count_[f], 0});
         if (avg) add(aa.touch(kExvBin, kExvBin)[0], WeightedEntry{(double)n_atoms_, (uint32_t)n_atoms_, 0});
         add(ww[0], WeightedEntry{(double)n_waters_, (uint32_t)n_waters_, 0});
 
@@ -589,12 +595,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             }
         }
         // SimpleCalculator.h:144-160: bin 0 += (sum w^2, (unsigned) sum w^2, 0)
-        auto self_term = [](const std::vector<float>& w) {
-            double t = 0;
-            for (float x : w) t += (double)(x * x);
-            return t;
-        };
-        const double ta = self_term(atom_w_), tw = self_term(water_w_);
+        const double ta = self_w2_atoms_, tw = self_w2_waters_;
         add(aa[0], WeightedEntry{ta, (uint32_t)ta, 0});
         add(ww[0], WeightedEntry{tw, (uint32_t)tw, 0});
         std::vector<WeightedEntry> ptot(nb);
@@ -724,13 +725,24 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
     }
     const double rm2 = 1.62 * 1.62 / 4;  // FFExplicitBase.cpp:54-57
     const double gx_q0 = grid_exv_volume * 0.334, gx_exp = std::pow(grid_exv_volume, 2. / 3) / (4 * kPi);
+    // the form factors depend on q only: evaluated once per q grid and thread
+    thread_local std::vector<double> ff_q, cached_ff, cached_fe;
+    if (ff_q != q) {
+        cached_ff.assign((size_t)nq * kMaxFF, 0.0);
+        cached_fe.assign((size_t)nq * kMaxFF, 0.0);
+        for (int k = 0; k < nq; ++k)
+            for (int f = 0; f < kMaxFF; ++f) {
+                cached_ff[(size_t)k * kMaxFF + f] = ff_atomic(f, q[k]);
+                if (f >= 1) cached_fe[(size_t)k * kMaxFF + f] = ff_exv(f, q[k]);
+            }
+        ff_q = q;
+    }
     for (int k = 0; k < nq; ++k) {
         const double qk = q[k];
         const double cx = mode != Mode::FFExplicit ? p.cx : std::pow(p.cx, 3) * std::exp(-rm2 * (p.cx * p.cx - 1) * qk * qk);
-        const double fw = ff_atomic(kWaterBin, qk);
-        double fa[kMaxFF], fe[kMaxFF];
-        for (int f = 0; f < kMaxFF; ++f) fa[f] = ff_atomic(f, qk);
-        for (int f = 1; f < kMaxFF; ++f) fe[f] = ff_exv(f, qk);
+        const double* fa = cached_ff.data() + (size_t)k * kMaxFF;
+        const double* fe = cached_fe.data() + (size_t)k * kMaxFF;
+        const double fw = fa[kWaterBin];
         // exv form factor of the FFAvg table, or of the grid dummies
         // (ExvFormFactor(V), ExvFormFactor.h:23-34; V = 1 A^3 unless stretched)
         const double fx = mode == Mode::FFGrid ? gx_q0 * std::exp(-gx_exp * qk * qk) : fa[kExvBin];

@@ -145,8 +145,8 @@ class ResidentStructure {
     double z_exv_;
     int n_atoms_, n_waters_, n_exv_ = 0;
     double grid_volume_ = 0, grid_exv_volume_ = 1;
-    std::vector<int32_t> atom_ff_;                // FF modes
-    std::vector<float> atom_w_, water_w_;         // Simple mode
+    std::array<uint32_t, kMaxFF> ff_count_{};     // FF modes: atoms per form-factor type
+    double self_w2_atoms_ = 0, self_w2_waters_ = 0;  // Simple mode: sum of w*w (float products)
     std::vector<float> atom_class_w_, water_class_w_;
     std::vector<std::array<int, 3>> rows_;        // global row -> (kind, a, b)
     std::unique_ptr<PairEngine> engine_;

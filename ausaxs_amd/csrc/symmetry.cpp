@@ -156,80 +156,143 @@ int tri(int a, int b, int C) {
 }
 }  // namespace
 
+void SymmetryStructure::check_spec(const SymmetrySpec& sp) {
+    if (sp.reps < 1) throw std::runtime_error("ausaxs_hip: a symmetry needs at least one repetition");
+    if (sp.transforms.size() != 12 * (size_t)sp.reps)
+        throw std::runtime_error("ausaxs_hip: a symmetry needs 12 transform values per repetition");
+    for (const CopyPair& p : sp.schedule)
+        if (p.a < 0 || p.b < 0 || p.a > sp.reps || p.b > sp.reps || p.a == p.b || p.scale < 1)
+            throw std::runtime_error("ausaxs_hip: invalid pair schedule entry (" + std::to_string(p.a) + ", " +
+                                     std::to_string(p.b) + ", scale " + std::to_string(p.scale) + ")");
+}
+
+namespace {
+int copies_of(const std::vector<SymmetrySpec>& syms) {
+    int total = 0;
+    for (const SymmetrySpec& sp : syms) total += sp.reps;
+    return total;
+}
+}  // namespace
+
 SymmetryStructure::SymmetryStructure(const float* atoms, const std::vector<int>& sizes,
                                      std::vector<std::vector<SymmetrySpec>> syms, const float* waters, int n_waters,
-                                     const HistSettings& s)
-    : s_(s), sizes_(sizes) {
+                                     const HistSettings& s, bool partial)
+    : s_(s), partial_(partial), syms_(std::move(syms)) {
     const int B = (int)sizes.size();
-    if ((int)syms.size() != B) throw std::runtime_error("ausaxs_hip: one symmetry list per body is required");
+    if ((int)syms_.size() != B) throw std::runtime_error("ausaxs_hip: one symmetry list per body is required");
     if (n_waters < 0) throw std::runtime_error("ausaxs_hip: negative water count");
-    std::vector<size_t> off(B + 1, 0);
+    size_t off = 0;
     for (int b = 0; b < B; ++b) {
         if (sizes[b] < 0) throw std::runtime_error("ausaxs_hip: negative body size");
-        off[b + 1] = off[b] + (size_t)sizes[b];
+        bodies_.emplace_back(atoms + 4 * off, atoms + 4 * (off + sizes[b]));
+        off += (size_t)sizes[b];
+        for (const SymmetrySpec& sp : syms_[b]) check_spec(sp);
     }
-    for (int b = 0; b < B; ++b)
-        for (const SymmetrySpec& sp : syms[b]) {
-            if (sp.reps < 1) throw std::runtime_error("ausaxs_hip: a symmetry needs at least one repetition");
-            if (sp.transforms.size() != 12 * (size_t)sp.reps)
-                throw std::runtime_error("ausaxs_hip: a symmetry needs 12 transform values per repetition");
-            for (const CopyPair& p : sp.schedule)
-                if (p.a < 0 || p.b < 0 || p.a > sp.reps || p.b > sp.reps || p.a == p.b || p.scale < 1)
-                    throw std::runtime_error("ausaxs_hip: invalid pair schedule entry (" + std::to_string(p.a) + ", " +
-                                             std::to_string(p.b) + ", scale " + std::to_string(p.scale) + ")");
-        }
+    waters_.assign(waters, waters + 4 * (size_t)n_waters);
+    self_bodies_ = bodies_;
+    for (int b = 0; b < B; ++b) self_scale_.push_back(1 + copies_of(syms_[b]));
+    rebuild();
+}
+
+void SymmetryStructure::set_body(int b, const float* pts, bool internal) {
+    if (b < 0 || b >= (int)bodies_.size()) throw std::runtime_error("ausaxs_hip: body index out of range");
+    bodies_[b].assign(pts, pts + bodies_[b].size());
+    // calc_aa_self runs on an internal modification only (PartialSymmetryManagerMT.cpp:265-276)
+    if (internal || !partial_) {
+        self_bodies_[b] = bodies_[b];
+        self_scale_[b] = 1 + copies_of(syms_[b]);
+    }
+    dirty_ = true;
+}
+
+void SymmetryStructure::set_symmetry(int b, int sym, SymmetrySpec spec) {
+    if (b < 0 || b >= (int)bodies_.size() || sym < 0 || sym >= (int)syms_[b].size())
+        throw std::runtime_error("ausaxs_hip: symmetry index out of range");
+    check_spec(spec);
+    syms_[b][sym] = std::move(spec);
+    if (!partial_) self_scale_[b] = 1 + copies_of(syms_[b]);
+    dirty_ = true;
+}
+
+void SymmetryStructure::set_waters(const float* pts, int n) {
+    if (n < 0) throw std::runtime_error("ausaxs_hip: negative water count");
+    waters_.assign(pts, pts + 4 * (size_t)n);
+    dirty_ = true;
+}
+
+PairEngine& SymmetryStructure::engine() {
+    if (dirty_) rebuild();
+    return *engine_;
+}
+
+void SymmetryStructure::rebuild() {
+    const int B = (int)bodies_.size();
+    const int n_waters = (int)(waters_.size() / 4);
 
     // weight classes, shared by all body copies (a copy keeps its weights)
     std::map<uint32_t, int> aid, wid;
+    std::vector<float> all;
+    for (const auto& v : bodies_) all.insert(all.end(), v.begin(), v.end());
+    for (const auto& v : self_bodies_) all.insert(all.end(), v.begin(), v.end());
     std::vector<int> acls, wcls;
-    classify(atoms, (int)off[B], aid, atom_class_w_, acls);
-    classify(waters, n_waters, wid, water_class_w_, wcls);
+    atom_class_w_.clear();
+    water_class_w_.clear();
+    classify(all.data(), (int)(all.size() / 4), aid, atom_class_w_, acls);
+    classify(waters_.data(), n_waters, wid, water_class_w_, wcls);
     const int CA = std::max<int>(1, (int)atom_class_w_.size());
     const int CW = std::max<int>(1, (int)water_class_w_.size());
     if (atom_class_w_.empty()) atom_class_w_.push_back(0.f);
     if (water_class_w_.empty()) water_class_w_.push_back(0.f);
+    auto cls_of = [&](float w) {
+        uint32_t key;
+        std::memcpy(&key, &w, 4);
+        return aid.at(key);
+    };
 
-    // point sets: waters, then per body its original and every copy
+    // point sets: waters, then per body its original and every copy (and, in
+    // partial mode, the coordinates of its self term)
     std::vector<PointSet> sets;
-    auto add_set = [&](std::vector<float> pts, const int* cls, int n_classes) {
+    auto add_set = [&](const std::vector<float>& pts, bool water) {
         PointSet ps;
-        ps.n_classes = n_classes;
+        ps.n_classes = water ? CW : CA;
         const int n = (int)(pts.size() / 4);
-        for (int i = 0; i < n; ++i) ps.orig.push_back(make_float4(pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], float_of(cls[i])));
-        set_points_.push_back(std::move(pts));
+        for (int i = 0; i < n; ++i)
+            ps.orig.push_back(make_float4(pts[4 * i], pts[4 * i + 1], pts[4 * i + 2],
+                                          float_of(water ? wcls[i] : cls_of(pts[4 * i + 3]))));
         sets.push_back(std::move(ps));
         return (int)sets.size() - 1;
     };
     const bool has_waters = n_waters > 0;
-    int set_w = -1;
-    if (has_waters) set_w = add_set(std::vector<float>(waters, waters + 4 * (size_t)n_waters), wcls.data(), CW);
-    std::vector<int> set_orig(B);
+    const int set_w = has_waters ? add_set(waters_, true) : -1;
+    std::vector<int> set_orig(B), set_self(B);
     std::vector<std::vector<int>> set_copy0(B);  // first copy set of each symmetry
     for (int b = 0; b < B; ++b) {
-        const float* pb = atoms + 4 * off[b];
-        const int* cb = acls.data() + off[b];
-        set_orig[b] = add_set(std::vector<float>(pb, pb + 4 * (size_t)sizes[b]), cb, CA);
-        for (const SymmetrySpec& sp : syms[b]) {
+        const std::vector<float>& pb = bodies_[b];
+        const int n = (int)(pb.size() / 4);
+        set_orig[b] = add_set(pb, false);
+        set_self[b] = partial_ ? add_set(self_bodies_[b], false) : set_orig[b];
+        for (const SymmetrySpec& sp : syms_[b]) {
             set_copy0[b].push_back((int)sets.size());
             for (int k = 0; k < sp.reps; ++k) {
                 // SymmetryHelpers.cpp:26-36: t(v.pos) in FP64 on the float
                 // coordinates, rotation*v + translation (Vector3.h:169-181),
                 // stored back as float
                 const double* T = sp.transforms.data() + 12 * (size_t)k;
-                std::vector<float> pts(4 * (size_t)sizes[b]);
-                for (int i = 0; i < sizes[b]; ++i) {
+                std::vector<float> pts(4 * (size_t)n);
+                for (int i = 0; i < n; ++i) {
                     const double x = pb[4 * i], y = pb[4 * i + 1], z = pb[4 * i + 2];
                     for (int c = 0; c < 3; ++c)
                         pts[4 * i + c] = (float)((T[3 * c] * x + T[3 * c + 1] * y + T[3 * c + 2] * z) + T[9 + c]);
                     pts[4 * i + 3] = pb[4 * i + 3];
                 }
-                add_set(std::move(pts), cb, CA);
+                add_set(pts, false);
             }
         }
     }
     auto set_at = [&](int b, int sym, int rep) { return rep == 0 ? set_orig[b] : set_copy0[b][sym] + rep - 1; };
 
     // row blocks: aa per multiplicity, aw, ww
+    rows_.clear();
     int n_rows = 0;
     std::map<int, std::vector<int32_t>> aa_maps;
     auto aa_map = [&](int scale) -> const std::vector<int32_t>& {
@@ -245,12 +308,15 @@ SymmetryStructure::SymmetryStructure(const float* atoms, const std::vector<int>&
         n_rows += CA * (CA + 1) / 2;
         return aa_maps.emplace(scale, std::move(m)).first->second;
     };
-    std::vector<int32_t> aw_map, ww_map;
+    // aw rows (atom class, water class); the job's row points are the body
+    // copy (full mode) or the waters (partial mode)
+    std::vector<int32_t> aw_body_rows, aw_water_rows, ww_map;
     if (has_waters) {
-        aw_map.resize((size_t)CA * CW);
+        aw_body_rows.resize((size_t)CA * CW);
+        aw_water_rows.resize((size_t)CW * CA);
         for (int a = 0; a < CA; ++a)
             for (int w = 0; w < CW; ++w) {
-                aw_map[a * CW + w] = n_rows++;
+                aw_body_rows[a * CW + w] = aw_water_rows[w * CA + a] = n_rows++;
                 rows_.push_back({1, a, w, 2});
             }
         ww_map.resize((size_t)CW * CW);
@@ -261,8 +327,7 @@ SymmetryStructure::SymmetryStructure(const float* atoms, const std::vector<int>&
             }
     }
 
-    // the manager's enqueue sequence (SymmetryManagerMT.cpp:57-132); an
-    // enqueue_calculate_cross(d1, d2) runs rows over d2 and columns over d1
+    // an enqueue_calculate_cross(d1, d2) runs rows over d2 and columns over d1
     // (SimpleCalculator.h:188-211)
     std::vector<PairJob> jobs;
     auto cross = [&](int d1, int d2, const std::vector<int32_t>& map) {
@@ -281,54 +346,85 @@ SymmetryStructure::SymmetryStructure(const float* atoms, const std::vector<int>&
         jobs.push_back(std::move(j));
     };
     body_self_w2_.assign(B, 0.0);
-    for (int b1 = 0; b1 < B; ++b1) {
-        int total = 0;
-        for (const SymmetrySpec& sp : syms[b1]) total += sp.reps;
-        // every copy has the original's internal distances: evaluated once
-        self(set_orig[b1], aa_map(1 + total));
-        {
-            double t = 0;
-            const float* pb = atoms + 4 * off[b1];
-            for (int i = 0; i < sizes[b1]; ++i) t += (double)(pb[4 * i + 3] * pb[4 * i + 3]);
-            body_self_w2_[b1] = (1 + total) * t;
-        }
-        if (has_waters) cross(set_w, set_orig[b1], aw_map);
-        for (int s1 = 0; s1 < (int)syms[b1].size(); ++s1) {
-            const SymmetrySpec& sp = syms[b1][s1];
-            for (const CopyPair& p : sp.schedule) cross(set_at(b1, s1, p.a), set_at(b1, s1, p.b), aa_map(p.scale));
-            for (int r1 = 1; r1 <= sp.reps; ++r1) {
-                const int c1 = set_at(b1, s1, r1);
-                if (has_waters) cross(set_w, c1, aw_map);
-                for (int b2 = b1 + 1; b2 < B; ++b2) {
-                    cross(set_orig[b2], c1, aa_map(1));
-                    for (int s2 = 0; s2 < (int)syms[b2].size(); ++s2)
-                        for (int r2 = 1; r2 <= syms[b2][s2].reps; ++r2) cross(c1, set_at(b2, s2, r2), aa_map(1));
+    for (int b = 0; b < B; ++b) {
+        // every copy has the original's internal distances: evaluated once and
+        // scaled by 1 + copies; bin 0 gets scale * sum w^2
+        self(set_self[b], aa_map(self_scale_[b]));
+        double t = 0;
+        const std::vector<float>& pb = self_bodies_[b];
+        for (size_t i = 0; i < pb.size() / 4; ++i) t += (double)(pb[4 * i + 3] * pb[4 * i + 3]);
+        body_self_w2_[b] = self_scale_[b] * t;
+    }
+    if (!partial_) {
+        // SymmetryManagerMT.cpp:57-132
+        for (int b1 = 0; b1 < B; ++b1) {
+            if (has_waters) cross(set_w, set_orig[b1], aw_body_rows);
+            for (int s1 = 0; s1 < (int)syms_[b1].size(); ++s1) {
+                const SymmetrySpec& sp = syms_[b1][s1];
+                for (const CopyPair& p : sp.schedule) cross(set_at(b1, s1, p.a), set_at(b1, s1, p.b), aa_map(p.scale));
+                for (int r1 = 1; r1 <= sp.reps; ++r1) {
+                    const int c1 = set_at(b1, s1, r1);
+                    if (has_waters) cross(set_w, c1, aw_body_rows);
+                    for (int b2 = b1 + 1; b2 < B; ++b2) {
+                        cross(set_orig[b2], c1, aa_map(1));
+                        for (int s2 = 0; s2 < (int)syms_[b2].size(); ++s2)
+                            for (int r2 = 1; r2 <= syms_[b2][s2].reps; ++r2) cross(c1, set_at(b2, s2, r2), aa_map(1));
+                    }
+                    for (int s2 = s1 + 1; s2 < (int)syms_[b1].size(); ++s2)
+                        for (int r2 = 1; r2 <= syms_[b1][s2].reps; ++r2) cross(c1, set_at(b1, s2, r2), aa_map(1));
                 }
-                for (int s2 = s1 + 1; s2 < (int)syms[b1].size(); ++s2)
-                    for (int r2 = 1; r2 <= syms[b1][s2].reps; ++r2) cross(c1, set_at(b1, s2, r2), aa_map(1));
+            }
+            for (int b2 = b1 + 1; b2 < B; ++b2) {
+                cross(set_orig[b1], set_orig[b2], aa_map(1));
+                for (int s2 = 0; s2 < (int)syms_[b2].size(); ++s2)
+                    for (int r2 = 1; r2 <= syms_[b2][s2].reps; ++r2) cross(set_orig[b1], set_at(b2, s2, r2), aa_map(1));
             }
         }
-        for (int b2 = b1 + 1; b2 < B; ++b2) {
-            cross(set_orig[b1], set_orig[b2], aa_map(1));
-            for (int s2 = 0; s2 < (int)syms[b2].size(); ++s2)
-                for (int r2 = 1; r2 <= syms[b2][s2].reps; ++r2) cross(set_orig[b1], set_at(b2, s2, r2), aa_map(1));
+    } else {
+        // PartialSymmetryManagerMT::calc_aa / calc_aw (:596-706): between two
+        // bodies (b1 > b2) rows run over b2's coordinates, within a body over
+        // the lower symmetry, and body x water over the waters
+        auto reps_of = [&](int b, int sym) { return sym == 0 ? 1 : syms_[b][sym - 1].reps; };
+        auto coords = [&](int b, int sym, int rep) { return sym == 0 ? set_orig[b] : set_copy0[b][sym - 1] + rep; };
+        for (int b1 = 0; b1 < B; ++b1) {
+            const int ns1 = (int)syms_[b1].size();
+            for (int b2 = 0; b2 < b1; ++b2)
+                for (int s1 = 0; s1 <= ns1; ++s1)
+                    for (int s2 = 0; s2 <= (int)syms_[b2].size(); ++s2)
+                        for (int r1 = 0; r1 < reps_of(b1, s1); ++r1)
+                            for (int r2 = 0; r2 < reps_of(b2, s2); ++r2)
+                                cross(coords(b1, s1, r1), coords(b2, s2, r2), aa_map(1));
+            for (int s1 = 0; s1 < ns1; ++s1) {
+                for (const CopyPair& p : syms_[b1][s1].schedule)
+                    cross(set_at(b1, s1, p.a), set_at(b1, s1, p.b), aa_map(p.scale));
+                for (int s2 = 0; s2 < s1; ++s2)
+                    for (int r1 = 0; r1 < syms_[b1][s1].reps; ++r1)
+                        for (int r2 = 0; r2 < syms_[b1][s2].reps; ++r2)
+                            cross(coords(b1, s1 + 1, r1), coords(b1, s2 + 1, r2), aa_map(1));
+            }
+            if (has_waters)
+                for (int s1 = 0; s1 <= ns1; ++s1)
+                    for (int r1 = 0; r1 < reps_of(b1, s1); ++r1) cross(coords(b1, s1, r1), set_w, aw_water_rows);
         }
     }
+    water_self_w2_ = 0;
     if (has_waters) {
         self(set_w, ww_map);
-        for (int i = 0; i < n_waters; ++i) water_self_w2_ += (double)(waters[4 * i + 3] * waters[4 * i + 3]);
+        for (int i = 0; i < n_waters; ++i) water_self_w2_ += (double)(waters_[4 * i + 3] * waters_[4 * i + 3]);
     }
     if (n_rows == 0) aa_map(1);  // keeps the row table non-empty
 
-    const float inv_w = inv_width(s);
+    const float inv_w = inv_width(s_);
     std::vector<const PointSet*> ps;
     for (const auto& x : sets) ps.push_back(&x);
     // at least the 10 bins every truncated histogram keeps (max_bin >= 10)
-    const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, std::max(bin_bound(ps, inv_w), 10));
-    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j);
+    const int nbins = (int)std::min<int64_t>((int64_t)s_.bin_count, std::max(bin_bound(ps, inv_w), 10));
+    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s_.tile_j);
+    dirty_ = false;
 }
 
 std::unique_ptr<CompositeDistanceHistogramHIP> SymmetryStructure::calculate(bool timed) {
+    if (dirty_) rebuild();
     engine_->run(cnt_, dsum_, timed);
     const int nbins = engine_->nbins();
     const int stride = nbins + 1;

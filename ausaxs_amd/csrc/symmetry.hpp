@@ -47,31 +47,54 @@ std::vector<CopyPair> placement_pair_schedule(const std::vector<double>& placeme
 std::vector<double> cyclic_transforms(const double* cm, const double* offset, const double* step, const double* axis,
                                       double angle, int reps, bool* closed);
 
+// Histograms of bodies with symmetry copies, on resident point sets.
+//
+// Full mode restates SymmetryManagerMT::calculate (SymmetryManagerMT.cpp:39-170).
+// Partial mode restates PartialSymmetryManagerMT (PartialSymmetryManagerMT.cpp:78-450):
+// the same pairs with its own row/column roles (calc_aa, calc_aw:
+// rows over the lower body, over the lower symmetry, over the waters), and the
+// self term of a body kept from the coordinates and copy count of its last
+// internal modification (calc_aa_self runs only then). Changing bodies,
+// symmetries or waters marks the structure for a rebuild; calculate() then
+// evaluates every block on the current coordinates, which gives what the
+// reference's recomputed and cached blocks hold.
 class SymmetryStructure {
   public:
     // atoms: sum(sizes) x 4 floats {x, y, z, w}, body after body; syms[b]: the
     // symmetries of body b; waters: n_waters x 4 floats.
     SymmetryStructure(const float* atoms, const std::vector<int>& sizes, std::vector<std::vector<SymmetrySpec>> syms,
-                      const float* waters, int n_waters, const HistSettings& s);
+                      const float* waters, int n_waters, const HistSettings& s, bool partial = false);
     std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
-    PairEngine& engine() { return *engine_; }
-    // float coordinates of every point set (debugging and tests): body copies
-    // in (body, symmetry, repetition) order after the originals
-    int n_sets() const { return (int)set_points_.size(); }
-    const std::vector<float>& set_points(int k) const { return set_points_[k]; }
+    PairEngine& engine();
+    bool partial() const { return partial_; }
+
+    // state changes (partial mode: IPartialHistogramManager's state manager).
+    // internal: the body's own distances changed (weights or a non-rigid
+    // move): its self term is recomputed too.
+    void set_body(int b, const float* pts, bool internal);
+    void set_symmetry(int b, int s, SymmetrySpec spec);
+    void set_waters(const float* pts, int n);
 
   private:
     HistSettings s_;
-    std::vector<int> sizes_;
-    std::vector<double> body_self_w2_;          // per body: (1 + copies) * sum w^2 (SimpleCalculator.h:144-160)
+    bool partial_;
+    std::vector<std::vector<float>> bodies_, self_bodies_;  // current and self-term coordinates (x 4)
+    std::vector<int> self_scale_;                           // 1 + copies when the self term was taken
+    std::vector<std::vector<SymmetrySpec>> syms_;
+    std::vector<float> waters_;
+    bool dirty_ = true;
+
+    std::vector<double> body_self_w2_;  // per body: scale * sum w^2 (SimpleCalculator.h:144-160)
     double water_self_w2_ = 0;
     std::vector<float> atom_class_w_, water_class_w_;
     // global row -> (kind 0 aa / 1 aw / 2 ww, class a, class b, factor)
     std::vector<std::array<int, 4>> rows_;
-    std::vector<std::vector<float>> set_points_;
     std::unique_ptr<PairEngine> engine_;
     std::vector<uint64_t> cnt_;
     std::vector<int64_t> dsum_;
+
+    void rebuild();
+    static void check_spec(const SymmetrySpec& sp);
 };
 
 }  // namespace ausaxs::gpu

@@ -264,6 +264,22 @@ int ausaxs_symmetry_create(const float* atoms, const int* body_sizes, int n_bodi
                            const int* sym_reps, const double* transforms, const int* sched_len, const int* schedule,
                            const float* waters, int n_waters, const ausaxs_settings* settings,
                            ausaxs_symmetry** out);
+/* Replaces PartialSymmetryManagerMT<weighted_bins, vbw> (PartialSymmetryManagerMT.cpp:78-450):
+ * same arguments; the pairs are taken with that manager's row/column roles
+ * (calc_aa, calc_aw, :596-706) and a body's self term keeps the coordinates
+ * and copy count of its last internal modification (:265-276). */
+int ausaxs_symmetry_create_partial(const float* atoms, const int* body_sizes, int n_bodies, const int* sym_per_body,
+                                   const int* sym_reps, const double* transforms, const int* sched_len,
+                                   const int* schedule, const float* waters, int n_waters,
+                                   const ausaxs_settings* settings, ausaxs_symmetry** out);
+/* State changes between calculations (the state manager's signals,
+ * StateManager.h:16-128): new float coordinates of one body (an external,
+ * rigid move; internal = 1 for a change of its own distances or weights), one
+ * symmetry's copies and schedule, or the waters. */
+int ausaxs_symmetry_set_body(ausaxs_symmetry* s, int body, const float* atoms, int internal);
+int ausaxs_symmetry_set_symmetry(ausaxs_symmetry* s, int body, int symmetry, int reps, const double* transforms,
+                                 int sched_len, const int* schedule);
+int ausaxs_symmetry_set_waters(ausaxs_symmetry* s, const float* waters, int n_waters);
 int ausaxs_symmetry_calculate(ausaxs_symmetry* s, ausaxs_histogram** out);
 int ausaxs_symmetry_free(ausaxs_symmetry* s);
 /* Pair evaluations of one calculate. */

@@ -344,10 +344,18 @@ def affine_float(pts, T):
     return out
 
 
-def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=8000, nthreads=1):
-    """SymmetryManagerMT::calculate_all (see orc_hist_symmetry). bodies: list of
-    (n, 4) float arrays; symmetries: per body a list of (transforms (reps, 12),
-    schedule (k, 3))."""
+lib.orc_hist_symmetry_partial.restype = C.c_int
+lib.orc_hist_symmetry_partial.argtypes = [C.c_void_p] * 3 + [C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [
+    C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4
+
+
+def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=8000, nthreads=1, partial=False,
+                  self_bodies=None, self_scale=None):
+    """SymmetryManagerMT::calculate_all (see orc_hist_symmetry), or with
+    partial=True PartialSymmetryManagerMT's (orc_hist_symmetry_partial; self
+    terms from self_bodies with factors self_scale, default: the bodies and
+    1 + their copies). bodies: list of (n, 4) float arrays; symmetries: per
+    body a list of (transforms (reps, 12), schedule (k, 3))."""
     atoms = np.ascontiguousarray(np.concatenate(bodies) if bodies else np.zeros((0, 4)), np.float32)
     sizes = np.array([len(b) for b in bodies], np.int32)
     per = np.array([len(x) for x in symmetries], np.int32)
@@ -360,9 +368,18 @@ def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=800
                                  if flat else np.zeros((0, 3)), np.int32)
     waters = np.ascontiguousarray(waters if waters is not None else np.zeros((0, 4), np.float32), np.float32)
     out = [np.zeros(bin_count, ENTRY) for _ in range(4)]
-    mb = lib.orc_hist_symmetry(_p(atoms), _p(sizes), len(sizes), _p(per), _p(reps), _p(T), _p(slen), _p(sched),
-                               _p(waters), len(waters), C.c_float(inv_width(bin_width)), bin_count, nthreads,
-                               *map(_p, out))
+    if partial:
+        sb = np.ascontiguousarray(np.concatenate(self_bodies) if self_bodies is not None else atoms, np.float32)
+        if self_scale is None:
+            self_scale = [1 + sum(len(np.asarray(T).reshape(-1, 12)) for T, _ in xs) for xs in symmetries]
+        sc = np.ascontiguousarray(self_scale, np.int32)
+        mb = lib.orc_hist_symmetry_partial(_p(atoms), _p(sb), _p(sc), _p(sizes), len(sizes), _p(per), _p(reps),
+                                           _p(T), _p(slen), _p(sched), _p(waters), len(waters),
+                                           C.c_float(inv_width(bin_width)), bin_count, nthreads, *map(_p, out))
+    else:
+        mb = lib.orc_hist_symmetry(_p(atoms), _p(sizes), len(sizes), _p(per), _p(reps), _p(T), _p(slen), _p(sched),
+                                   _p(waters), len(waters), C.c_float(inv_width(bin_width)), bin_count, nthreads,
+                                   *map(_p, out))
     if mb < 0:
         raise RuntimeError(f"oracle status {mb}")
     return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])

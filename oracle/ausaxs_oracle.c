@@ -966,6 +966,104 @@ int orc_spline(const double* x, const double* y, int n, const double* z, int m, 
     return 0;
 }
 
+int orc_hist_symmetry_partial(const float* atoms, const float* self_atoms, const int* self_scale, const int* sizes,
+                              int n_bodies, const int* sym_per_body, const int* sym_reps, const double* transforms,
+                              const int* sched_len, const int* schedule, const float* waters, int n_w, float inv_w,
+                              int nb, int nthreads, orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot) {
+    if (nthreads < 1) nthreads = 1;
+    int B = n_bodies, n_sym = 0;
+    for (int b = 0; b < B; ++b) n_sym += sym_per_body[b];
+    size_t* off = (size_t*)calloc(B + 1, sizeof(size_t));
+    int* sym0 = (int*)calloc(B + 1, sizeof(int));
+    size_t* t0 = (size_t*)calloc(n_sym + 1, sizeof(size_t));
+    size_t* e0 = (size_t*)calloc(n_sym + 1, sizeof(size_t));
+    for (int b = 0; b < B; ++b) {
+        off[b + 1] = off[b] + (size_t)sizes[b];
+        sym0[b + 1] = sym0[b] + sym_per_body[b];
+    }
+    for (int k = 0; k < n_sym; ++k) {
+        t0[k + 1] = t0[k] + (size_t)sym_reps[k];
+        e0[k + 1] = e0[k] + (size_t)sched_len[k];
+    }
+    float** copy = (float**)calloc(t0[n_sym] + 1, sizeof(float*));
+    for (int b = 0; b < B; ++b)
+        for (int k = sym0[b]; k < sym0[b + 1]; ++k)
+            for (int r = 0; r < sym_reps[k]; ++r) {
+                size_t c = t0[k] + (size_t)r;
+                copy[c] = (float*)malloc(sizeof(float) * 4 * ((size_t)sizes[b] + 1));
+                orc_affine_float(atoms + 4 * off[b], sizes[b], transforms + 12 * c, copy[c]);
+            }
+#define BODY(b) (atoms + 4 * off[b])
+/* symmetry slot s of body b (0 = the body, s >= 1 = symmetry s-1), repetition r */
+#define COORD(b, s, r) ((s) == 0 ? BODY(b) : copy[t0[sym0[b] + (s)-1] + (size_t)(r)])
+#define REPS(b, s) ((s) == 0 ? 1 : sym_reps[sym0[b] + (s)-1])
+    orc_entry* tmp = (orc_entry*)malloc(sizeof(orc_entry) * (size_t)nb);
+    memset(aa, 0, sizeof(orc_entry) * (size_t)nb);
+    memset(aw, 0, sizeof(orc_entry) * (size_t)nb);
+    memset(ww, 0, sizeof(orc_entry) * (size_t)nb);
+    int err = 0;
+#define ACC(dst, expr)                                                  \
+    do {                                                                \
+        err |= (expr);                                                  \
+        for (int q_ = 0; q_ < nb; ++q_) entry_add(&(dst)[q_], &tmp[q_]); \
+    } while (0)
+    for (int b1 = 0; b1 < B; ++b1) {
+        const int ns1 = sym_per_body[b1];
+        /* calc_aa_self: the self-term coordinates and scale */
+        ACC(aa, run_simple_scaled(self_atoms + 4 * off[b1], sizes[b1], NULL, 0, 1, self_scale[b1], inv_w, nb,
+                                  nthreads, tmp));
+        /* calc_aa(b1, s1, b2, s2), b2 < b1: rows over b2, columns over b1 (:620-678) */
+        for (int b2 = 0; b2 < b1; ++b2)
+            for (int s1 = 0; s1 <= ns1; ++s1)
+                for (int s2 = 0; s2 <= sym_per_body[b2]; ++s2)
+                    for (int r1 = 0; r1 < REPS(b1, s1); ++r1)
+                        for (int r2 = 0; r2 < REPS(b2, s2); ++r2)
+                            ACC(aa, run_simple_scaled(COORD(b2, s2, r2), sizes[b2], COORD(b1, s1, r1), sizes[b1], 0,
+                                                      1, inv_w, nb, nthreads, tmp));
+        for (int s1 = 1; s1 <= ns1; ++s1) {
+            /* calc_aa(b1, s1, b1, 0): the pair schedule (:604-618) */
+            const int k1 = sym0[b1] + s1 - 1;
+            for (size_t e = e0[k1]; e < e0[k1 + 1]; ++e) {
+                const int ra = schedule[3 * e], rb = schedule[3 * e + 1], sc = schedule[3 * e + 2];
+                const float* A = ra == 0 ? BODY(b1) : copy[t0[k1] + (size_t)ra - 1];
+                const float* Bp = rb == 0 ? BODY(b1) : copy[t0[k1] + (size_t)rb - 1];
+                ACC(aa, run_simple_scaled(Bp, sizes[b1], A, sizes[b1], 0, sc, inv_w, nb, nthreads, tmp));
+            }
+            /* calc_aa(b1, s1, b1, s2), s2 < s1: rows over symmetry s2 */
+            for (int s2 = 1; s2 < s1; ++s2)
+                for (int r1 = 0; r1 < REPS(b1, s1); ++r1)
+                    for (int r2 = 0; r2 < REPS(b1, s2); ++r2)
+                        ACC(aa, run_simple_scaled(COORD(b1, s2, r2), sizes[b1], COORD(b1, s1, r1), sizes[b1], 0, 1,
+                                                  inv_w, nb, nthreads, tmp));
+        }
+        /* calc_aw(b1, s1): rows over the waters (:685-706) */
+        if (n_w > 0)
+            for (int s1 = 0; s1 <= ns1; ++s1)
+                for (int r1 = 0; r1 < REPS(b1, s1); ++r1)
+                    ACC(aw, run_simple_scaled(waters, n_w, COORD(b1, s1, r1), sizes[b1], 0, 1, inv_w, nb, nthreads,
+                                              tmp));
+    }
+    if (n_w > 0) ACC(ww, run_simple_scaled(waters, n_w, NULL, 0, 1, 1, inv_w, nb, nthreads, tmp));
+#undef ACC
+#undef REPS
+#undef COORD
+#undef BODY
+    for (size_t c = 0; c < t0[n_sym]; ++c) free(copy[c]);
+    free(copy);
+    free(tmp);
+    free(off);
+    free(sym0);
+    free(t0);
+    free(e0);
+    if (err) return -2;
+    for (int b = 0; b < nb; ++b) {
+        ptot[b] = aa[b];
+        entry_add(&ptot[b], &ww[b]);
+        entry_add(&ptot[b], &aw[b]);
+    }
+    return max_bin_of(ptot, nb);
+}
+
 /* ------------------------------------------------------------------------ */
 /* weighted axis, sinc table, contraction                                   */
 /* ------------------------------------------------------------------------ */

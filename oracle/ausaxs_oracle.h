@@ -137,6 +137,16 @@ int orc_hist_symmetry(const float* atoms, const int* sizes, int n_bodies, const 
  * m points z. Returns 0, or -1 for fewer than four knots. */
 int orc_spline(const double* x, const double* y, int n, const double* z, int m, double* out);
 
+/* PartialSymmetryManagerMT::calculate_all (PartialSymmetryManagerMT.cpp:78-450)
+ * in the state after the given changes: self terms from self_atoms with
+ * factors self_scale[b] (calc_aa_self at the last internal change), every
+ * other block from atoms with the manager's roles (calc_aa: rows over the
+ * lower body / the lower symmetry; calc_aw: rows over the waters). */
+int orc_hist_symmetry_partial(const float* atoms, const float* self_atoms, const int* self_scale, const int* sizes,
+                              int n_bodies, const int* sym_per_body, const int* sym_reps, const double* transforms,
+                              const int* sched_len, const int* schedule, const float* waters, int n_w, float inv_w,
+                              int nb, int nthreads, orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
 /* ---- weighted axis, sinc, Debye ------------------------------------------ */
 /* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
  * followed by DistanceHistogram::initialize (DistanceHistogram.cpp:44-47, d[0]=0). */

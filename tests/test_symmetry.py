@@ -95,10 +95,12 @@ def oracle_of(bodies, syms, waters, **kw):
     return O.hist_symmetry(bodies, [[(s.transforms, s.schedule) for s in per] for per in syms], waters, **kw)
 
 
+# the reference runs every analytic case through both managers
+@pytest.mark.parametrize("partial", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
-def test_oracle_reference_cases(name):
+def test_oracle_reference_cases(name, partial):
     bodies, syms, waters = build(CASES[name])
-    h = oracle_of(bodies, syms, waters)
+    h = oracle_of(bodies, syms, waters, partial=partial)
     v = h["p_tot"]["value"]
     np.testing.assert_array_equal(v, expected_bins(CASES[name][3], len(v)))
     if name in WATER_SPLIT:
@@ -242,6 +244,22 @@ def test_oracle_matches_explicit_structure():
     np.testing.assert_allclose(h["p_tot"]["value"].sum(), ref["p_tot"]["value"].sum(), rtol=1e-12)
 
 
+def test_oracle_partial_matches_full():
+    """PartialSymmetryManagerMT against SymmetryManagerMT on one state
+    (partial_symmetry_manager.cpp compares them approximately): the same
+    pairs in other row/column roles, so only the bins of tie pairs of the
+    scalar-tail columns may differ."""
+    bodies, syms, waters = random_case(9, n=(37, 22), n_waters=13)
+    full = oracle_of(bodies, syms, waters)
+    part = oracle_of(bodies, syms, waters, partial=True)
+    c0, c1 = full["p_tot"]["count"].astype(np.int64), part["p_tot"]["count"].astype(np.int64)
+    n = min(len(c0), len(c1))
+    assert c0[1:].sum() == c1[1:].sum()
+    assert np.abs(c0[:n] - c1[:n]).sum() <= 1e-3 * c0.sum()
+    for key in ("aa", "aw", "ww"):
+        assert full[key]["count"].astype(np.int64)[1:].sum() == part[key]["count"].astype(np.int64)[1:].sum()
+
+
 def test_symmetry_input_errors():
     b = A.points_w(np.zeros((3, 3)), 1.0)
     T = np.r_[np.eye(3).ravel(), [1, 0, 0]]
@@ -266,13 +284,52 @@ def compare(h, ref):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("partial", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
-def test_gpu_reference_cases(name):
+def test_gpu_reference_cases(name, partial):
     bodies, syms, waters = build(CASES[name])
-    h = A.SymmetryStructure(bodies, syms, waters).calculate()
+    h = A.SymmetryStructure(bodies, syms, waters, partial=partial).calculate()
     v, _, _ = h.get(A.PTOT)
     np.testing.assert_array_equal(v, expected_bins(CASES[name][3], len(v)))
-    compare(h, oracle_of(bodies, syms, waters))
+    compare(h, oracle_of(bodies, syms, waters, partial=partial))
+
+
+@pytest.mark.gpu
+def test_gpu_partial_state_changes():
+    """PartialSymmetryManagerMT across state changes (partial_symmetry_manager.cpp,
+    test): a symmetry change, new waters, a rigid move of a body (its self term
+    stays from the old coordinates), then an internal change (self term
+    refreshed); each state against the oracle, counts bit-exact."""
+    bodies, syms, waters = random_case(4, n=(300, 200), n_waters=120)
+    st = A.SymmetryStructure(bodies, syms, waters, partial=True)
+    compare(st.calculate(), oracle_of(bodies, syms, waters, partial=True, nthreads=4))
+    self_bodies = [b.copy() for b in bodies]
+    # modify a symmetry
+    b0 = bodies[0]
+    syms[0][1] = A.Symmetry.cyclic(b0[:, :3].mean(0), np.zeros(3), (0, 1.5, 9.0), Z, 0.5, 2)
+    st.set_symmetry(0, 1, syms[0][1])
+    compare(st.calculate(), oracle_of(bodies, syms, waters, partial=True, nthreads=4))
+    # new waters
+    waters = A.points_w(np.random.default_rng(6).uniform(-30, 30, (90, 3)), 1.0)
+    st.set_waters(waters)
+    compare(st.calculate(), oracle_of(bodies, syms, waters, partial=True, nthreads=4))
+    # rigid move of body 1 (external): self term from the old coordinates
+    moved = bodies[1].copy()
+    moved[:, :3] = (moved[:, :3].astype(np.float64) @ rot(X, 0.4).T + [2.0, -1.0, 0.5]).astype(F32)
+    bodies[1] = moved
+    st.set_body(1, moved)
+    compare(st.calculate(), oracle_of(bodies, syms, waters, partial=True, self_bodies=self_bodies, nthreads=4))
+    # internal change of body 0 (weights): its self term is taken again
+    b0 = bodies[0].copy()
+    b0[:5, 3] = 3.0
+    bodies[0] = b0
+    self_bodies[0] = b0
+    st.set_body(0, b0, internal=True)
+    compare(st.calculate(), oracle_of(bodies, syms, waters, partial=True, self_bodies=self_bodies, nthreads=4))
+    # the full manager recomputes the self term on any move
+    full = A.SymmetryStructure(bodies, syms, waters)
+    full.set_body(1, moved)
+    compare(full.calculate(), oracle_of(bodies, syms, waters, nthreads=4))
 
 
 @pytest.mark.gpu
