@@ -746,6 +746,7 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
         // exv form factor of the FFAvg table, or of the grid dummies
         // (ExvFormFactor(V), ExvFormFactor.h:23-34; V = 1 A^3 unless stretched)
         const double fx = mode == Mode::FFGrid ? gx_q0 * std::exp(-gx_exp * qk * qk) : fa[kExvBin];
+        const double cr = cx * p.crho, cr2 = cr * cr;  // pow(cx * crho, 2): the product rounded once
         for (size_t r = 0; r < row_keys_.size(); ++r) {
             const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
             const double s = S[r * nq + k];
@@ -756,7 +757,7 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
                 if (kind == 0) {
                     if (f1 >= 1 && f2 >= 1) aa_[k] += s * fa[f1] * fa[f2];
                     else if (f1 >= 1 && f2 == kExvBin) ax_[k] += 2 * p.crho * cx * s * (fa[f1] * fx);
-                    else if (f1 == kExvBin && f2 == kExvBin) xx_[k] += std::pow(cx * p.crho, 2) * s * (fx * fx);
+                    else if (f1 == kExvBin && f2 == kExvBin) xx_[k] += cr2 * s * (fx * fx);
                 } else if (kind == 1) {
                     if (f1 >= 1) aw_[k] += 2 * p.cw * s * (fa[f1] * fw);
                     else wx_[k] += 2 * p.crho * cx * p.cw * s * (fx * fw);
@@ -769,7 +770,7 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
                     if (f1 < 1 || f2 < 1) continue;
                     aa_[k] += s * fa[f1] * fa[f2];
                     ax_[k] += p.crho * cx * s * (fa[f1] * fe[f2] + fa[f2] * fe[f1]);
-                    xx_[k] += std::pow(cx * p.crho, 2) * s * fe[f1] * fe[f2];
+                    xx_[k] += cr2 * s * fe[f1] * fe[f2];
                 } else if (kind == 1) {
                     if (f1 < 1) continue;
                     aw_[k] += 2 * p.cw * s * fa[f1] * fw;
