@@ -25,6 +25,8 @@ LIB_PATH = _HERE / "libausaxs_hip.so"
 MODE_FFAVG = 0
 MODE_FFEXPLICIT = 1
 MODE_SIMPLE = 2
+MODE_FFGRID_SURFACE = 4
+XX_INTERIOR, XX_SURFACE, XX_CROSS, WX_INTERIOR, WX_SURFACE, AX_INTERIOR, AX_SURFACE = 6, 7, 8, 9, 10, 11, 12
 AA, AW, WW, PTOT = 0, 1, 2, 3
 MAX_FF = 15
 
@@ -64,6 +66,9 @@ def _load():
             [C.c_int, vp, C.c_int, vp, C.c_int, P(_Settings), C.c_double, P(vp)],
         ),
         "ausaxs_structure_calculate": (C.c_int, [vp, P(vp)]),
+        "ausaxs_structure_create_grid_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
+        "ausaxs_grid_exv_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, P(C.c_int), vp, C.c_int,
+                                              P(C.c_int)]),
         "ausaxs_structure_free": (C.c_int, [vp]),
         "ausaxs_hist_calculate": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int, P(_Settings), C.c_double, P(vp)]),
         "ausaxs_histogram_info": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
@@ -208,7 +213,7 @@ class CompositeHistogram:
 
     def _shape(self, which):
         mb = self.max_bin
-        if self.mode == MODE_SIMPLE or which in (WW, PTOT, 4, 5):  # 4, 5: FFGrid p_tot_ax, xx
+        if self.mode == MODE_SIMPLE or which in (WW, PTOT, 4, 5, 6, 7, 8, 9, 10):  # FFGrid / surface 1-D profiles
             return (mb,)
         return (MAX_FF, MAX_FF, mb) if which == AA else (MAX_FF, mb)
 
@@ -368,6 +373,36 @@ class GridStructure(Structure):
         _check(lib.ausaxs_structure_create_grid(_ptr(self._a), _ptr(self._f), len(self._a), _ptr(self._w),
                                                 len(self._w), C.byref(s), C.byref(h)))
         self._s = h
+
+
+class GridSurfaceStructure(Structure):
+    """HistogramManagerMTFFGridSurface on resident data: the grid dummies split
+    into interior and surface ones; set_params(cx=...) scales the surface."""
+
+    def __init__(self, atoms_xyz, ff, waters_xyz=None, settings=None):
+        self._a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+        self._f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+        self._w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)),
+                                       np.float64).reshape(-1, 3)
+        s = settings or default_settings()
+        h = C.c_void_p()
+        _check(lib.ausaxs_structure_create_grid_surface(_ptr(self._a), _ptr(self._f), len(self._a), _ptr(self._w),
+                                                        len(self._w), C.byref(s), C.byref(h)))
+        self._s = h
+
+
+def grid_exv_surface(atoms_xyz, ff, waters_xyz=None):
+    """Interior and surface dummies (RawGridWithSurfaceExv, reference order); host-only."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+    ni, ns = C.c_int(), C.c_int()
+    _check(lib.ausaxs_grid_exv_surface(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), None, 0, C.byref(ni), None, 0,
+                                       C.byref(ns)))
+    xi, xs = np.zeros((max(1, ni.value), 3)), np.zeros((max(1, ns.value), 3))
+    _check(lib.ausaxs_grid_exv_surface(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), _ptr(xi), ni.value, C.byref(ni),
+                                       _ptr(xs), ns.value, C.byref(ns)))
+    return xi[: ni.value].copy(), xs[: ns.value].copy()
 
 
 def _grid_axes(self):

@@ -148,6 +148,37 @@ int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff
     });
 }
 
+int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                         const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                         ausaxs_structure** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_waters == 0 || waters_xyz, "waters is NULL");
+        auto s = std::make_unique<ausaxs_structure>();
+        s->s = make_grid_surface_structure(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, to_settings(settings));
+        *out = s.release();
+    });
+}
+
+int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                            int n_waters, double* interior_xyz, int capacity_interior, int* n_interior,
+                            double* surface_xyz, int capacity_surface, int* n_surface) {
+    return guarded([&] {
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, true);
+        const int ni = (int)(g.xyz.size() / 3), ns = (int)(g.surface.size() / 3);
+        if (n_interior) *n_interior = ni;
+        if (n_surface) *n_surface = ns;
+        require((!interior_xyz || capacity_interior >= ni) && (!surface_xyz || capacity_surface >= ns),
+                "capacity too small");
+        if (interior_xyz) std::copy(g.xyz.begin(), g.xyz.end(), interior_xyz);
+        if (surface_xyz) std::copy(g.surface.begin(), g.surface.end(), surface_xyz);
+    });
+}
+
 int ausaxs_grid_exv(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
                     int n_waters, double* out_xyz, int capacity, int* n_out, double* volume) {
     return guarded([&] {
@@ -166,7 +197,7 @@ int ausaxs_grid_exv(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms
 int ausaxs_histogram_grid_axes(const ausaxs_histogram* h, double* d_ax, double* d_xx) {
     return guarded([&] {
         require(h && d_ax && d_xx, "NULL argument");
-        require(h->h->mode == Mode::FFGrid, "not an FFGrid histogram");
+        require(h->h->mode == Mode::FFGrid || h->h->mode == Mode::FFGridSurface, "not an FFGrid histogram");
         std::memcpy(d_ax, h->h->d_axis_ax.data(), h->h->d_axis_ax.size() * sizeof(double));
         std::memcpy(d_xx, h->h->d_axis_xx.data(), h->h->d_axis_xx.size() * sizeof(double));
     });
@@ -487,6 +518,20 @@ int ausaxs_histogram_get(const ausaxs_histogram* h, int which, double* value, ui
             case AUSAXS_PTOT: v = &c.p_tot; break;
             case AUSAXS_PTOT_AX: v = &c.p_tot_ax; break;
             case AUSAXS_XX: v = &c.p_xx; break;
+            case AUSAXS_XX_INTERIOR: v = &c.xx_interior; break;
+            case AUSAXS_XX_SURFACE: v = &c.xx_surface; break;
+            case AUSAXS_XX_CROSS: v = &c.xx_cross; break;
+            case AUSAXS_WX_INTERIOR: v = &c.wx_interior; break;
+            case AUSAXS_WX_SURFACE: v = &c.wx_surface; break;
+            case AUSAXS_AX_INTERIOR:
+            case AUSAXS_AX_SURFACE: {
+                const EntryRows& e = which == AUSAXS_AX_INTERIOR ? c.ax_interior : c.ax_surface;
+                require(c.mode == Mode::FFGridSurface, "not an FFGridSurface histogram");
+                tmp.resize(e.size());
+                e.dense(tmp.data());
+                v = &tmp;
+                break;
+            }
             default: throw std::runtime_error("unknown distribution id");
         }
         for (size_t i = 0; i < v->size(); ++i) {

@@ -200,6 +200,8 @@ FitOutcome HistogramFitter::fit(FitAlgorithm alg, int scan_evals) {
         for (const auto& s : kSpecs)
             if (enabled_ & s.bit) sp = &s;
         Interval iv{sp->lo, sp->hi};
+        // CompositeDistanceHistogramFFGridSurface::get_excluded_volume_scaling_factor_limits (:47-49)
+        if (sp->bit == kFitExcludedVolume && h_->mode == Mode::FFGridSurface) iv = Interval{0, 2};
         if (alg == FitAlgorithm::Scan) {
             // Scan::landscape (Scan.cpp:26-36) as one batch, its minimum
             // (Dataset::find_minimum: first smallest), then Golden on one

@@ -2,6 +2,7 @@
 #include "grid.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <limits>
 #include <stdexcept>
@@ -28,7 +29,56 @@ constexpr double kRvol = 2.15;    // settings::grid::min_exv_radius
 constexpr uint8_t kACenter = 1, kWCenter = 2, kExv = 4;
 }  // namespace
 
-GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m) {
+namespace {
+// RadialLineGenerator::generate (RadialLineGenerator.cpp:15-63) for cell width
+// 1 and the collision radii of RawGridWithSurfaceExv: the bin offsets of the
+// points at radius r_k along each direction of an octant-reflected sphere
+struct RadialLines {
+    std::vector<std::array<int, 3>> b1, b2, b3;
+};
+RadialLines radial_lines() {
+    const double pi = 3.14159265358979323846;
+    const double r[3] = {std::sqrt(1.0) + 1e-3, 2.0, 3.0};
+    const double ang = 2 * pi / 8;
+    std::vector<std::array<double, 3>> sphere;
+    for (double theta = 0; theta <= pi * 0.5; theta += ang)
+        for (double phi = 0; phi <= pi * 0.5; phi += ang) {
+            const double x = std::cos(phi) * std::sin(theta), y = std::sin(phi) * std::sin(theta),
+                         z = std::cos(theta);
+            for (int sz : {1, -1})
+                for (int sy : {1, -1})
+                    for (int sx : {1, -1}) sphere.push_back({sx * x, sy * y, sz * z});
+        }
+    // the reflections in the reference's order: (x,y,z), (-x,y,z), (x,-y,z), (-x,-y,z), then -z
+    std::vector<std::array<double, 3>> rots;
+    for (auto p : sphere) {
+        for (double& c : p)
+            if (std::abs(c) < 1e-5) c = 0;
+        bool present = false;
+        for (const auto& q : rots) {
+            const double d = std::sqrt(std::pow(q[0] - p[0], 2) + std::pow(q[1] - p[1], 2) + std::pow(q[2] - p[2], 2));
+            if (d < 1e-5) {
+                present = true;
+                break;
+            }
+        }
+        if (!present) rots.push_back(p);
+    }
+    RadialLines out;
+    for (const auto& p : rots) {
+        std::array<int, 3> v[3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) v[k][c] = (int)(std::round(r[k] * p[c]) * 1.0);
+        out.b1.push_back(v[0]);
+        out.b2.push_back(v[1]);
+        out.b3.push_back(v[2]);
+    }
+    return out;
+}
+}  // namespace
+
+GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                      bool detect_surface) {
     GridExv out;
     if (n <= 0) return out;
     // grid minimum as in Grid(bodies) (Grid.cpp:34-104): bounding box grown from
@@ -41,9 +91,11 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
     double scaling = 0.25;  // settings::grid::scaling; small systems switch to 1 (Grid.cpp:87-90)
     if (hi[0] - lo[0] < 50 || hi[1] - lo[1] < 50 || hi[2] - lo[2] < 50) scaling = 1;
     const double margin = 2.14 + 1.71 + (-0.35) + 3 * kCell;  // vdw P + vdw O + shell_correction + 3 cells
+    int bins[3];  // Axis3D(nmin, nmax, width).bins
     for (int k = 0; k < 3; ++k) {
         const double expand = std::max(0.5 * (hi[k] - lo[k]) * scaling, margin);
         out.origin[k] = (int)std::floor(lo[k] - expand - kCell);
+        bins[k] = (int)std::ceil(hi[k] + expand + kCell) - out.origin[k];
     }
 
     // dense occupancy box around the atoms (one voxel = one lattice point)
@@ -103,16 +155,61 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
             }
         }
     }
-    // RawGridExv::create: lexicographic voxel order
+    // GridObj::is_empty_or_water: nothing, or a water centre (water areas
+    // only ever fill empty voxels); voxels outside the box hold no atom
+    auto empty_or_water = [&](int i, int j, int k) {
+        if (!inside(i, j, k)) return true;
+        const uint8_t s = at(i, j, k);
+        return s == 0 || (s & kWCenter);
+    };
+    auto out_of_bounds = [&](int i, int j, int k) {
+        return i < 0 || i >= bins[0] || j < 0 || j >= bins[1] || k < 0 || k >= bins[2];
+    };
+    RadialLines lines;
+    if (detect_surface) lines = radial_lines();
+    // collision_check (RawGridWithSurfaceExv.cpp:7-61): true for interior dummies
+    auto interior = [&](int i, int j, int k) {
+        int score = 0;
+        for (size_t d = 0; d < lines.b1.size(); ++d) {
+            const auto& a = lines.b1[d];
+            if (out_of_bounds(i + a[0], j + a[1], k + a[2])) {
+                score += 7;
+                continue;
+            }
+            if (!empty_or_water(i + a[0], j + a[1], k + a[2])) continue;
+            const auto& b = lines.b2[d];
+            if (out_of_bounds(i + b[0], j + b[1], k + b[2])) {
+                score += 7;
+                continue;
+            }
+            if (!empty_or_water(i + b[0], j + b[1], k + b[2])) {
+                score += 3;
+                continue;
+            }
+            const auto& c = lines.b3[d];
+            if (out_of_bounds(i + c[0], j + c[1], k + c[2])) {
+                score += 7;
+                continue;
+            }
+            if (!empty_or_water(i + c[0], j + c[1], k + c[2])) {
+                score += 5;
+                continue;
+            }
+            score += 7;
+        }
+        return score < 42;
+    };
+    // RawGridExv::create / RawGridWithSurfaceExv::create: lexicographic voxel order
     for (int i = bmin[0]; i <= bmax[0]; ++i)
         for (int j = bmin[1]; j <= bmax[1]; ++j)
             for (int k = bmin[2]; k <= bmax[2]; ++k)
                 if (at(i, j, k) & (kACenter | kExv)) {
-                    out.xyz.push_back(out.origin[0] + i * kCell);
-                    out.xyz.push_back(out.origin[1] + j * kCell);
-                    out.xyz.push_back(out.origin[2] + k * kCell);
+                    std::vector<double>& dst = !detect_surface || interior(i, j, k) ? out.xyz : out.surface;
+                    dst.push_back(out.origin[0] + i * kCell);
+                    dst.push_back(out.origin[1] + j * kCell);
+                    dst.push_back(out.origin[2] + k * kCell);
                 }
-    out.volume = (double)(out.xyz.size() / 3) * kCell * kCell * kCell;
+    out.volume = (double)((out.xyz.size() + out.surface.size()) / 3) * kCell * kCell * kCell;
     return out;
 }
 

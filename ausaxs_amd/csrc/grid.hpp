@@ -21,13 +21,20 @@
 namespace ausaxs::gpu {
 
 struct GridExv {
-    std::vector<double> xyz;  // n x 3, integer-valued lattice points
-    double volume = 0;        // Grid::get_volume(): count * cell_width^3
-    int origin[3]{};          // grid minimum (integer)
+    std::vector<double> xyz;      // n x 3, integer-valued lattice points (interior ones if split)
+    std::vector<double> surface;  // surface lattice points (detect_surface only)
+    double volume = 0;            // Grid::get_volume(): count * cell_width^3 (both lists)
+    int origin[3]{};              // grid minimum (integer)
 };
 
 // atoms: n x 3 FP64 coordinates + form-factor types; waters: m x 3 FP64.
-GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m);
+// detect_surface: RawGridWithSurfaceExv::create (RawGridWithSurfaceExv.cpp:7-155)
+// with surface_thickness 1: a dummy is on the surface when the radial lines
+// from it (RadialLineGenerator, radii 1.001, 2, 3 A, 8 divisions) score at
+// least 42 (7 per line that leaves the molecule, 3 or 5 for one blocked at 2
+// or 3 A, 0 at 1 A), every other dummy is interior.
+GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                      bool detect_surface = false);
 
 // constants::radius::vdw per form-factor type (FormFactorType.h:174-193)
 double vdw_radius(int ff_type);

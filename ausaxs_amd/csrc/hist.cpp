@@ -244,20 +244,24 @@ std::vector<float4> to_points(const float* xyzw, int n) {
 // ---------------------------------------------------------------------------
 ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
                                      const HistSettings& s, double z_exv, const double* exv_xyz, int n_exv,
-                                     double grid_volume)
-    : mode_(mode), s_(s), z_exv_(z_exv), n_atoms_(n_atoms), n_waters_(n_waters), n_exv_(n_exv),
+                                     double grid_volume, const double* surf_xyz, int n_surf)
+    : mode_(mode), s_(s), z_exv_(z_exv), n_atoms_(n_atoms), n_waters_(n_waters), n_exv_(n_exv), n_surf_(n_surf),
       grid_volume_(grid_volume) {
-    if (n_atoms < 0 || n_waters < 0 || n_exv < 0) throw std::runtime_error("ausaxs_hip: negative point count");
+    if (n_atoms < 0 || n_waters < 0 || n_exv < 0 || n_surf < 0)
+        throw std::runtime_error("ausaxs_hip: negative point count");
     if (!(s.bin_width > 0) || s.bin_count < 10) throw std::runtime_error("ausaxs_hip: invalid bin settings");
-    if (mode == Mode::FFGrid && n_exv > 0 && !exv_xyz) throw std::runtime_error("ausaxs_hip: FFGrid needs dummies");
+    const bool grid = mode == Mode::FFGrid || mode == Mode::FFGridSurface;
+    if (grid && n_exv > 0 && !exv_xyz) throw std::runtime_error("ausaxs_hip: FFGrid needs dummies");
+    if (mode == Mode::FFGridSurface && n_surf > 0 && !surf_xyz)
+        throw std::runtime_error("ausaxs_hip: FFGridSurface needs surface dummies");
     const float inv_w = inv_width(s);
-    PointSet A, W, X;
+    PointSet A, W, X, XS;
     A.orig = to_points(atoms, n_atoms);
     W.orig = to_points(waters, n_waters);
     std::vector<PairJob> jobs;
     int n_rows = 0;
 
-    if (mode == Mode::FFAvg || mode == Mode::FFExplicit || mode == Mode::FFGrid) {
+    if (mode != Mode::Simple) {
         std::set<int> present;
         for (int i = 0; i < n_atoms; ++i) {
             int32_t f;
@@ -292,6 +296,43 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
         ww.rowmap[0] = n_rows++;
         rows_.push_back({2, 0, 0});
         jobs = {aa, aw, ww};
+        if (mode == Mode::FFGridSurface) {
+            // interior (set 2) and surface (set 3) dummies (HistogramManagerMTFFGridSurface.cpp:30-47):
+            // xx interior, surface and their cross (rows interior), ax and wx
+            // against each (:52-196)
+            X.orig.resize(n_exv);
+            for (int i = 0; i < n_exv; ++i)
+                X.orig[i] = make_float4((float)exv_xyz[3 * i], (float)exv_xyz[3 * i + 1], (float)exv_xyz[3 * i + 2], 0.f);
+            XS.orig.resize(n_surf);
+            for (int i = 0; i < n_surf; ++i)
+                XS.orig[i] =
+                    make_float4((float)surf_xyz[3 * i], (float)surf_xyz[3 * i + 1], (float)surf_xyz[3 * i + 2], 0.f);
+            X.n_classes = XS.n_classes = 1;
+            auto one = [&](int row_set, int col_set, int kind) {
+                PairJob j;
+                j.row_set = row_set;
+                j.col_set = col_set;
+                j.rowmap = {n_rows++};
+                rows_.push_back({kind, 0, 0});
+                jobs.push_back(std::move(j));
+            };
+            one(2, -1, 3);  // xx interior
+            one(3, -1, 6);  // xx surface
+            one(2, 3, 7);   // xx interior x surface
+            for (int set : {2, 3}) {
+                PairJob ax;
+                ax.row_set = 0;
+                ax.col_set = set;
+                ax.rowmap.assign(kMaxFF, -1);
+                for (int f : present) {
+                    ax.rowmap[f] = n_rows++;
+                    rows_.push_back({set == 2 ? 4 : 8, f, 0});
+                }
+                jobs.push_back(std::move(ax));
+            }
+            one(1, 2, 5);  // wx interior
+            one(1, 3, 9);  // wx surface
+        }
         if (mode == Mode::FFGrid) {
             // dummies (CompactCoordinatesFF of EXCLUDED_VOLUME atoms, HistogramManagerMTFFGrid.cpp:49-57)
             X.orig.resize(n_exv);
@@ -375,13 +416,14 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
         jobs = {aa, ww, aw};
     }
 
-    const int bound = bin_bound({&A, &W, &X}, inv_w);
+    const int bound = bin_bound({&A, &W, &X, &XS}, inv_w);
     // at least the 10 bins every truncated histogram keeps (max_bin >= 10)
     const int nbins = (int)std::min<int64_t>((int64_t)s.bin_count, std::max(bound, 10));
     std::vector<PointSet> sets;
     sets.push_back(std::move(A));
     sets.push_back(std::move(W));
-    if (mode == Mode::FFGrid) sets.push_back(std::move(X));
+    if (grid) sets.push_back(std::move(X));
+    if (mode == Mode::FFGridSurface) sets.push_back(std::move(XS));
     engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j);
 }
 
@@ -417,7 +459,9 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         // HistogramManagerMTFFGrid starts from HistogramManagerMTFFAvg<true, vbw>
         // (HistogramManagerMTFFGrid.cpp:47): FFAvg conventions, always weighted.
         const bool avg = mode_ != Mode::FFExplicit;
-        const double z = mode_ == Mode::FFGrid ? (n_atoms_ ? grid_volume_ * 0.334 / n_atoms_ : 0.0) : z_exv_;
+        const bool grid = mode_ == Mode::FFGrid || mode_ == Mode::FFGridSurface;
+        const bool surf = mode_ == Mode::FFGridSurface;
+        const double z = grid ? (n_atoms_ ? grid_volume_ * 0.334 / n_atoms_ : 0.0) : z_exv_;
         // working width: bins up to the last non-empty one of any row
         int nbw = 10;
         for (size_t r = 0; r < rows_.size(); ++r)
@@ -428,12 +472,15 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
                 }
         nbw = std::min(nbw, nbins);
         const size_t nbv = nbw;
-        EntryRows aa, aw, ax;
+        EntryRows aa, aw, ax, axs;
         aa.reset(kMaxFF, kMaxFF, nbw);
         aw.reset(kMaxFF, 1, nbw);
         ax.reset(kMaxFF, 1, nbw);
-        std::vector<WeightedEntry> ww(nbv), xx, wx;
-        if (mode_ == Mode::FFGrid) xx.resize(nbv), wx.resize(nbv);
+        axs.reset(kMaxFF, 1, nbw);
+        // xx / ax / wx: the (interior) dummies; xxs, xxc, axs, wxs: surface ones
+        std::vector<WeightedEntry> ww(nbv), xx, wx, xxs, xxc, wxs;
+        if (grid) xx.resize(nbv), wx.resize(nbv);
+        if (surf) xxs.resize(nbv), xxc.resize(nbv), wxs.resize(nbv);
         for (size_t r = 0; r < rows_.size(); ++r) {
             const auto& k = rows_[r];
             WeightedEntry* dst = nullptr;
@@ -444,7 +491,11 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
                 case 2: dst = ww.data(); break;
                 case 3: dst = xx.data(); break;
                 case 4: dst = ax.touch(k[1], 0), factor = 1; break;
-                default: dst = wx.data(), factor = 1; break;
+                case 5: dst = wx.data(), factor = 1; break;
+                case 6: dst = xxs.data(); break;
+                case 7: dst = xxc.data(); break;
+                case 8: dst = axs.touch(k[1], 0), factor = 1; break;
+                default: dst = wxs.data(), factor = 1; break;
             }
             for (int b = 0; b < nbw; ++b)
                 if (cnt_[r * stride + b]) dst[b] = entry((int)r, b, factor);
@@ -501,12 +552,13 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         }
         unsigned max_bin = base;
         ptot.resize(base);
-        if (mode_ == Mode::FFGrid) {
-            // HistogramManagerMTFFGrid.cpp:159-224
+        if (grid) {
+            // HistogramManagerMTFFGrid.cpp:159-224; HistogramManagerMTFFGridSurface.cpp:203-272
             add(xx[0], WeightedEntry{(double)n_exv_, (uint32_t)n_exv_, 0});
+            if (surf) add(xxs[0], WeightedEntry{(double)n_surf_, (uint32_t)n_surf_, 0});
             unsigned mbx = 10;
             for (int i = nbw - 1; i >= 10; --i)
-                if (xx[i].value != 0 || wx[i].value != 0) {
+                if (xx[i].value != 0 || wx[i].value != 0 || (surf && (xxs[i].value != 0 || wxs[i].value != 0))) {
                     mbx = (unsigned)i + 1;
                     break;
                 }
@@ -541,17 +593,59 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
                     for (unsigned b = base; b < max_bin; ++b) r[b] = WeightedEntry{};
             }
             for (unsigned b = base; b < max_bin; ++b) ww[b] = WeightedEntry{};
-            std::vector<WeightedEntry> pax((size_t)max_bin);
-            for (unsigned b = 0; b < max_bin; ++b) add(pax[b], wx[b]);
-            for (int f = 0; f < kMaxFF; ++f)
-                if (const WeightedEntry* src = ax.row(f, 0))
-                    for (unsigned b = 0; b < max_bin; ++b) add(pax[b], src[b]);
-            for (int f = 0; f < kMaxFF; ++f) {
-                if (const WeightedEntry* src = ax.row(f, 0)) std::copy_n(src, max_bin, aa.touch(f, kExvBin));
-                else aa.clear(f, kExvBin);
+            std::vector<WeightedEntry> pax((size_t)max_bin), pxx((size_t)max_bin);
+            for (unsigned b = 0; b < max_bin; ++b) {
+                add(pax[b], wx[b]);
+                if (surf) add(pax[b], wxs[b]);
             }
-            std::copy_n(wx.data(), max_bin, aw.touch(kExvBin, 0));
-            std::copy_n(xx.data(), max_bin, aa.touch(kExvBin, kExvBin));
+            for (int f = 0; f < kMaxFF; ++f) {
+                const WeightedEntry* src = ax.row(f, 0);
+                const WeightedEntry* srs = surf ? axs.row(f, 0) : nullptr;
+                for (unsigned b = 0; b < max_bin; ++b) {
+                    if (src) add(pax[b], src[b]);
+                    if (srs) add(pax[b], srs[b]);
+                }
+            }
+            for (unsigned b = 0; b < max_bin; ++b) {
+                pxx[b] = xx[b];
+                if (surf) add(pxx[b], xxs[b]), add(pxx[b], xxc[b]);
+            }
+            // the exv rows become the dummy histograms (FFGrid), or their
+            // sums at unit surface scaling (FFGridSurface's initialize)
+            for (int f = 0; f < kMaxFF; ++f) {
+                const WeightedEntry* src = ax.row(f, 0);
+                const WeightedEntry* srs = surf ? axs.row(f, 0) : nullptr;
+                if (!src && !srs) {
+                    aa.clear(f, kExvBin);
+                    continue;
+                }
+                WeightedEntry* dst = aa.touch(f, kExvBin);
+                for (unsigned b = 0; b < max_bin; ++b) {
+                    dst[b] = src ? src[b] : WeightedEntry{};
+                    if (srs) add(dst[b], srs[b]);
+                }
+            }
+            {
+                WeightedEntry* dst = aw.touch(kExvBin, 0);
+                for (unsigned b = 0; b < max_bin; ++b) {
+                    dst[b] = wx[b];
+                    if (surf) add(dst[b], wxs[b]);
+                }
+            }
+            std::copy_n(pxx.data(), max_bin, aa.touch(kExvBin, kExvBin));
+            if (surf) {
+                res->ax_interior.reset(kMaxFF, 1, (int)max_bin);
+                res->ax_surface.reset(kMaxFF, 1, (int)max_bin);
+                for (int f = 1; f < kMaxFF; ++f) {
+                    if (const WeightedEntry* src = ax.row(f, 0)) std::copy_n(src, max_bin, res->ax_interior.touch(f, 0));
+                    if (const WeightedEntry* src = axs.row(f, 0)) std::copy_n(src, max_bin, res->ax_surface.touch(f, 0));
+                }
+                res->xx_interior.assign(xx.begin(), xx.begin() + max_bin);
+                res->xx_surface.assign(xxs.begin(), xxs.begin() + max_bin);
+                res->xx_cross.assign(xxc.begin(), xxc.begin() + max_bin);
+                res->wx_interior.assign(wx.begin(), wx.begin() + max_bin);
+                res->wx_surface.assign(wxs.begin(), wxs.begin() + max_bin);
+            }
             // both axes live on histograms of bin_count entries (WeightedDistribution1D
             // of the manager), and neither has its first bin forced to zero
             auto axis_full = [&](const std::vector<WeightedEntry>& p) {
@@ -565,7 +659,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
                 return d;
             };
             res->p_tot_ax = pax;
-            res->p_xx.assign(xx.begin(), xx.begin() + max_bin);
+            res->p_xx = pxx;
             res->d_axis_ax = axis_full(pax);
             res->d_axis_xx = axis_full(res->p_xx);
             res->grid_volume = grid_volume_;
@@ -613,7 +707,8 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         res->ww.assign(ww.begin(), ww.begin() + max_bin);
         res->p_tot.assign(ptot.begin(), ptot.begin() + max_bin);
     }
-    res->d_axis = axis_of(res->p_tot, s_.bin_width, s_.weighted_bins || mode_ == Mode::FFGrid);
+    res->d_axis = axis_of(res->p_tot, s_.bin_width,
+                          s_.weighted_bins || mode_ == Mode::FFGrid || mode_ == Mode::FFGridSurface);
     return res;
 }
 
@@ -638,6 +733,24 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
         if (const WeightedEntry* r = aa.row(0, 0)) push_row(0, 0, 0, 0, r);
         if (const WeightedEntry* r = aw.row(0, 0)) push_row(0, 1, 0, 0, r);
         push_row(0, 2, 0, 0, ww.data());
+    } else if (mode == Mode::FFGridSurface) {
+        // CompositeDistanceHistogramFFGridSurface.cpp:104-266: atomic rows on
+        // p_tot's axis; the interior and surface dummy profiles on the ax / xx
+        // axes, kept apart so the surface scaling applies per q
+        aa.for_each([&](int f1, int f2, const WeightedEntry* r) {
+            if (f1 >= 1 && f2 >= 1) push_row(0, 0, f1, f2, r);
+        });
+        aw.for_each([&](int f, int, const WeightedEntry* r) {
+            if (f >= 1) push_row(0, 1, f, 0, r);
+        });
+        push_row(0, 2, 0, 0, ww.data());
+        ax_interior.for_each([&](int f, int, const WeightedEntry* r) { push_row(1, 10, f, 0, r); });
+        ax_surface.for_each([&](int f, int, const WeightedEntry* r) { push_row(1, 11, f, 0, r); });
+        push_row(1, 12, 0, 0, wx_interior.data());
+        push_row(1, 13, 0, 0, wx_surface.data());
+        push_row(2, 14, 0, 0, xx_interior.data());
+        push_row(2, 15, 0, 0, xx_surface.data());
+        push_row(2, 16, 0, 0, xx_cross.data());
     } else {
         // FFGrid (CompositeDistanceHistogramFFGrid.cpp:77-119): [ff][exv] and
         // aw[exv] on the ax axis, [exv][exv] on the xx axis, the rest on p_tot's
@@ -725,6 +838,7 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
     }
     const double rm2 = 1.62 * 1.62 / 4;  // FFExplicitBase.cpp:54-57
     const double gx_q0 = grid_exv_volume * 0.334, gx_exp = std::pow(grid_exv_volume, 2. / 3) / (4 * kPi);
+    const bool surface = mode == Mode::FFGridSurface;
     // the form factors depend on q only: evaluated once per q grid and thread
     thread_local std::vector<double> ff_q, cached_ff, cached_fe;
     if (ff_q != q) {
@@ -745,8 +859,34 @@ std::vector<double> CompositeDistanceHistogramHIP::debye_transform(const std::ve
         const double fw = fa[kWaterBin];
         // exv form factor of the FFAvg table, or of the grid dummies
         // (ExvFormFactor(V), ExvFormFactor.h:23-34; V = 1 A^3 unless stretched)
-        const double fx = mode == Mode::FFGrid ? gx_q0 * std::exp(-gx_exp * qk * qk) : fa[kExvBin];
+        const double fx =
+            mode == Mode::FFGrid || surface ? gx_q0 * std::exp(-gx_exp * qk * qk) : fa[kExvBin];
         const double cr = cx * p.crho, cr2 = cr * cr;  // pow(cx * crho, 2): the product rounded once
+        if (surface) {
+            // CompositeDistanceHistogramFFGridSurface::cache_refresh_intensity_profiles
+            // (:137-266): the surface dummies scale by exv_factor(q, cx) =
+            // cx^3 exp(-rm2 (cx^2 - 1) q^2), xx = xx_i + c^2 xx_s + c xx_c
+            const double c = std::pow(p.cx, 3) * std::exp(-rm2 * (std::pow(p.cx, 2) - 1) * qk * qk);
+            const double c2 = std::pow(c, 2);
+            for (size_t r = 0; r < row_keys_.size(); ++r) {
+                const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
+                const double sv = S[r * nq + k];
+                switch (kind) {
+                    case 0: aa_[k] += sv * fa[f1] * fa[f2]; break;
+                    case 1: aw_[k] += 2 * p.cw * sv * (fa[f1] * fw); break;
+                    case 2: ww_[k] += p.cw * p.cw * sv * (fw * fw); break;
+                    case 10: ax_[k] += 2 * p.crho * sv * (fa[f1] * fx); break;
+                    case 11: ax_[k] += 2 * p.crho * (c * sv) * (fa[f1] * fx); break;
+                    case 12: wx_[k] += 2 * p.crho * sv * p.cw * (fw * fx); break;
+                    case 13: wx_[k] += 2 * p.crho * (c * sv) * p.cw * (fw * fx); break;
+                    case 14: xx_[k] += p.crho * p.crho * sv * (fx * fx); break;
+                    case 15: xx_[k] += p.crho * p.crho * (c2 * sv) * (fx * fx); break;
+                    default: xx_[k] += p.crho * p.crho * (c * sv) * (fx * fx); break;
+                }
+                (void)f2;
+            }
+            continue;
+        }
         for (size_t r = 0; r < row_keys_.size(); ++r) {
             const int kind = row_keys_[r][0], f1 = row_keys_[r][1], f2 = row_keys_[r][2];
             const double s = S[r * nq + k];
@@ -819,6 +959,21 @@ std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, 
     pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);
     return std::make_unique<ResidentStructure>(Mode::FFGrid, a.data(), n, w.data(), m, s, 0.0, g.xyz.data(),
                                                (int)(g.xyz.size() / 3), g.volume);
+}
+
+std::unique_ptr<ResidentStructure> make_grid_surface_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                               const double* waters_xyz, int m,
+                                                               const HistSettings& s) {
+    for (int i = 0; i < n; ++i)
+        if (ff[i] < 1 || ff[i] >= kMaxFF)
+            throw std::runtime_error("ausaxs_hip: atom form factor type outside [1, 14]");
+    GridExv g = make_grid_exv(atoms_xyz, ff, n, waters_xyz, m, true);
+    std::vector<float> a, w;
+    pack_ff_points(atoms_xyz, ff, n, 0, a);
+    pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);
+    return std::make_unique<ResidentStructure>(Mode::FFGridSurface, a.data(), n, w.data(), m, s, 0.0, g.xyz.data(),
+                                               (int)(g.xyz.size() / 3), g.volume, g.surface.data(),
+                                               (int)(g.surface.size() / 3));
 }
 
 ScalableGridStructure::ScalableGridStructure(const double* atoms_xyz, const int32_t* ff, int n,

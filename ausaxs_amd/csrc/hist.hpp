@@ -77,7 +77,7 @@ struct FreeParams {  // CompositeDistanceHistogramFFAvgBase free parameters
     double cw = 1, cx = 1, crho = 1, dw_atomic = 0, dw_exv = 0;
 };
 
-enum class Mode : int { FFAvg = 0, FFExplicit = 1, Simple = 2, FFGrid = 3 };
+enum class Mode : int { FFAvg = 0, FFExplicit = 1, Simple = 2, FFGrid = 3, FFGridSurface = 4 };
 
 // form factors restated from FormFactor.h:33-39 / FormFactorTable.h:28-120 /
 // ExvFormFactor.h:16-42 (default MinimumFluctuation_implicit_H volumes)
@@ -103,6 +103,11 @@ class CompositeDistanceHistogramHIP {
     // (CompositeDistanceHistogramFFGrid.cpp:130-136), first max_bin entries
     std::vector<double> d_axis_ax, d_axis_xx;
     std::vector<WeightedEntry> p_tot_ax, p_xx;
+    // FFGridSurface only: the dummy profiles of interior and surface dummies
+    // (CompositeDistanceHistogramFFGridSurface.cpp:33-46), first max_bin entries;
+    // the surface terms scale with the excluded-volume parameter
+    EntryRows ax_interior, ax_surface;  // [15][1][max_bin]
+    std::vector<WeightedEntry> xx_interior, xx_surface, xx_cross, wx_interior, wx_surface;
     double grid_volume = 0;
     // FFGrid: volume V of the dummies' ExvFormFactor (width^3; (width k)^3
     // for dummies stretched by k, CompositeDistanceHistogramFFGridScalableExv.cpp:9-19)
@@ -131,9 +136,10 @@ class ResidentStructure {
     // atoms: n x 4 floats {x,y,z, ff (int bits) | weight}; waters likewise.
     // FFGrid: exv_xyz holds the n_exv dummy lattice points (grid.hpp) and
     // grid_volume the grid volume used for the FFAvg base (Z_exv).
+    // FFGridSurface: exv_xyz the interior dummies, surf_xyz the surface ones
     ResidentStructure(Mode mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
                       const HistSettings& s, double z_exv, const double* exv_xyz = nullptr, int n_exv = 0,
-                      double grid_volume = 0);
+                      double grid_volume = 0, const double* surf_xyz = nullptr, int n_surf = 0);
     std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
     PairEngine& engine() { return *engine_; }
     Mode mode() const { return mode_; }
@@ -143,7 +149,7 @@ class ResidentStructure {
     Mode mode_;
     HistSettings s_;
     double z_exv_;
-    int n_atoms_, n_waters_, n_exv_ = 0;
+    int n_atoms_, n_waters_, n_exv_ = 0, n_surf_ = 0;
     double grid_volume_ = 0, grid_exv_volume_ = 1;
     std::array<uint32_t, kMaxFF> ff_count_{};     // FF modes: atoms per form-factor type
     double self_w2_atoms_ = 0, self_w2_waters_ = 0;  // Simple mode: sum of w*w (float products)
@@ -159,6 +165,11 @@ class ResidentStructure {
 // of the coordinates as CompactCoordinatesFF holds them.
 std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, const int32_t* ff, int n,
                                                        const double* waters_xyz, int m, const HistSettings& s);
+// HistogramManagerMTFFGridSurface (HistogramManagerMTFFGridSurface.cpp:16-274):
+// the dummies split into interior and surface ones (RawGridWithSurfaceExv)
+std::unique_ptr<ResidentStructure> make_grid_surface_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                               const double* waters_xyz, int m,
+                                                               const HistSettings& s);
 // HistogramManagerMTFFGridScalableExv (HistogramManagerMTFFGridScalableExv.cpp:16-244):
 // the FFGrid histograms with the dummies stretched by a factor k about the
 // origin (float coordinates times k, as CompactCoordinatesFF stores them) and

@@ -29,6 +29,7 @@ extern "C" {
 #define AUSAXS_MODE_FFEXPLICIT 1 /* HistogramManagerMTFFExplicit  */
 #define AUSAXS_MODE_SIMPLE 2     /* HistogramManagerMT (XYZW)     */
 #define AUSAXS_MODE_FFGRID 3     /* HistogramManagerMTFFGrid      */
+#define AUSAXS_MODE_FFGRID_SURFACE 4 /* HistogramManagerMTFFGridSurface */
 
 /* which distribution ausaxs_histogram_get returns */
 #define AUSAXS_AA 0
@@ -37,6 +38,14 @@ extern "C" {
 #define AUSAXS_PTOT 3
 #define AUSAXS_PTOT_AX 4 /* FFGrid: weighted atom/water-dummy totals [max_bin] */
 #define AUSAXS_XX 5      /* FFGrid: weighted dummy-dummy histogram [max_bin]   */
+/* FFGridSurface: interior / surface dummy profiles (CompositeDistanceHistogramFFGridSurface.cpp:33-46) */
+#define AUSAXS_XX_INTERIOR 6 /* [max_bin] */
+#define AUSAXS_XX_SURFACE 7  /* [max_bin] */
+#define AUSAXS_XX_CROSS 8    /* [max_bin], interior x surface */
+#define AUSAXS_WX_INTERIOR 9 /* [max_bin] */
+#define AUSAXS_WX_SURFACE 10 /* [max_bin] */
+#define AUSAXS_AX_INTERIOR 11 /* [15][max_bin] */
+#define AUSAXS_AX_SURFACE 12  /* [15][max_bin] */
 
 typedef struct {
     double bin_width;  /* settings::axes::bin_width (default 0.25)  */
@@ -67,6 +76,20 @@ int ausaxs_structure_create(int mode, const float* atoms, int n_atoms, const flo
 int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
                                  const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
                                  ausaxs_structure** out);
+
+/* Grid excluded volume with a surface layer (HistogramManagerMTFFGridSurface,
+ * HistogramManagerMTFFGridSurface.cpp:16-274; CompositeDistanceHistogramFFGridSurface.cpp).
+ * The dummies split into interior and surface ones (RawGridWithSurfaceExv::create,
+ * RawGridWithSurfaceExv.cpp:7-155); the excluded-volume parameter (cx of
+ * ausaxs_histogram_set_params) scales the surface dummies only. */
+int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                         const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                         ausaxs_structure** out);
+/* The interior and surface dummies of RawGridWithSurfaceExv::create, each in the
+ * reference's voxel order; counts are returned also when a capacity is too small. */
+int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
+                            int n_waters, double* interior_xyz, int capacity_interior, int* n_interior,
+                            double* surface_xyz, int capacity_surface, int* n_surface);
 
 /* Excluded-volume dummy atoms of the default grid (RawGridExv::create,
  * RawGridExv.cpp:10-59; Grid.cpp:34-110; SphericalExpander.h:29-90) in the

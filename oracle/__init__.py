@@ -296,6 +296,102 @@ def debye_ffgrid(hist, q, bin_width, long_double=False, **params):
 
 
 # ---------------------------------------------------------------------------
+# grid excluded volume with a surface layer (FFGridSurface)
+# ---------------------------------------------------------------------------
+lib.orc_grid_exv_surface.restype = C.c_int
+lib.orc_grid_exv_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+lib.orc_hist_ffgrid_surface.restype = C.c_int
+lib.orc_hist_ffgrid_surface.argtypes = ([C.c_void_p, C.c_int] * 4 + [C.c_double, C.c_float, C.c_int, C.c_int]
+                                        + [C.c_void_p] * 13)
+
+
+def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=None):
+    """Interior and surface dummies (each n x 3 FP64, reference order) and the grid volume.
+    rvol: the minimum excluded-volume radius; atomic_radius: one radius for every atom in
+    place of the van der Waals radii."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+    cap = max(1, len(a) * (2 * int(np.ceil(max(rvol, 2.15))) + 1) ** 3)
+    xi = np.zeros((cap, 3), np.float64)
+    xs = np.zeros((cap, 3), np.float64)
+    ns, vol = C.c_int(), C.c_double()
+    n = lib.orc_grid_exv_surface(_p(a), _p(f), len(a), _p(w), len(w), _p(xi), _p(xs), cap, C.byref(ns),
+                                 C.byref(vol), rvol, -1.0 if atomic_radius is None else atomic_radius)
+    if n < 0:
+        raise RuntimeError("grid_exv_surface capacity")
+    return xi[:n].copy(), xs[:ns.value].copy(), vol.value
+
+
+def hist_ffgrid_surface(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1):
+    """HistogramManagerMTFFGridSurface::calculate_all from FP64 coordinates (see orc_hist_ffgrid_surface)."""
+    w = np.zeros((0, 3)) if waters_xyz is None else np.asarray(waters_xyz, np.float64)
+    xi_xyz, xs_xyz, vol = grid_exv_surface(atoms_xyz, ff, w)
+    atoms = pack_ff(atoms_xyz, ff)
+    waters = pack_ff(w, np.ones(len(w), np.int32))
+    xi = pack_ff(xi_xyz, np.zeros(len(xi_xyz), np.int32))
+    xs = pack_ff(xs_xyz, np.zeros(len(xs_xyz), np.int32))
+    nb = bin_count
+    aa = np.zeros(MAX_FF * MAX_FF * nb, ENTRY)
+    aw, axi, axs = (np.zeros(MAX_FF * nb, ENTRY) for _ in range(3))
+    ww, pt, pax, pxx, xxi, xxs, xxc, wxi, wxs = (np.zeros(nb, ENTRY) for _ in range(9))
+    mb = lib.orc_hist_ffgrid_surface(_p(atoms), len(atoms), _p(waters), len(waters), _p(xi), len(xi), _p(xs),
+                                     len(xs), vol, C.c_float(inv_width(bin_width)), nb, nthreads, _p(aa), _p(aw),
+                                     _p(ww), _p(pt), _p(pax), _p(pxx), _p(xxi), _p(xxs), _p(xxc), _p(wxi), _p(wxs),
+                                     _p(axi), _p(axs))
+    if mb < 0:
+        raise RuntimeError(f"oracle status {mb}")
+    d_ax = np.zeros(mb)
+    d_xx = np.zeros(mb)
+    lib.orc_weighted_axis_n(_p(pax), mb, nb, bin_width, _p(d_ax))
+    lib.orc_weighted_axis_n(_p(pxx), mb, nb, bin_width, _p(d_xx))
+    return dict(max_bin=mb, aa=aa.reshape(MAX_FF, MAX_FF, nb)[:, :, :mb], aw=aw.reshape(MAX_FF, nb)[:, :mb],
+                ww=ww[:mb], p_tot=pt[:mb], p_tot_ax=pax[:mb], p_xx=pxx[:mb], d_ax=d_ax, d_xx=d_xx, volume=vol,
+                xx_interior=xxi[:mb], xx_surface=xxs[:mb], xx_cross=xxc[:mb], wx_interior=wxi[:mb],
+                wx_surface=wxs[:mb], ax_interior=axi.reshape(MAX_FF, nb)[:, :mb],
+                ax_surface=axs.reshape(MAX_FF, nb)[:, :mb], interior=xi_xyz, surface=xs_xyz)
+
+
+def exv_factor(q, cx):
+    """CompositeDistanceHistogramFFGridSurface::exv_factor (.cpp:63-66), r_m = 1.62 A."""
+    rm2 = 1.62 * 1.62 / 4
+    return cx ** 3 * np.exp(-rm2 * (cx ** 2 - 1) * np.asarray(q, np.float64) ** 2)
+
+
+def debye_ffgrid_surface(hist, q, bin_width, cw=1.0, cx=1.0, crho=1.0, grid_exv_volume=1.0):
+    """I(q) and the six profiles {aa, ax, aw, xx, wx, ww} of an oracle FFGridSurface histogram
+    (CompositeDistanceHistogramFFGridSurface::cache_refresh_intensity_profiles, .cpp:174-266):
+    the surface profiles scale by exv_factor(q, cx) per q; atomic terms on the p_tot axis, the
+    ax / wx sums on the p_tot_ax axis and xx on the p_xx axis. No Debye-Waller factors."""
+    q = np.asarray(q, np.float64)
+    nq = len(q)
+    mb = hist["max_bin"]
+    S_aa = sinc_table(q, weighted_axis(hist["p_tot"], bin_width))
+    S_ax, S_xx = sinc_table(q, hist["d_ax"]), sinc_table(q, hist["d_xx"])
+    fa = np.array([[ff_atomic(f, qk) for f in range(MAX_FF)] for qk in q])  # nq x MAX_FF
+    V = grid_exv_volume
+    fx = 0.334 * V * np.exp(-(V ** (2 / 3) / (4 * np.pi)) * q * q)  # ExvFormFactor.h:23-34
+    fw = fa[:, 1]
+    c = exv_factor(q, cx)
+    prof = np.zeros((6, nq))
+    aa_v = hist["aa"]["value"]
+    sa = contract(aa_v[1:, 1:].reshape(-1, mb), S_aa).reshape(MAX_FF - 1, MAX_FF - 1, nq)
+    prof[0] = np.einsum("abq,qa,qb->q", sa, fa[:, 1:], fa[:, 1:])
+    si = contract(hist["ax_interior"]["value"][1:], S_ax)  # (MAX_FF - 1) x nq
+    ss = contract(hist["ax_surface"]["value"][1:], S_ax)
+    prof[1] = 2 * crho * np.sum((si + c[None, :] * ss) * fa[:, 1:].T, axis=0) * fx
+    xi, xs, xc = (contract(hist[k]["value"][None, :], S_xx)[0] for k in ("xx_interior", "xx_surface", "xx_cross"))
+    prof[3] = crho * crho * (xi + c * c * xs + c * xc) * fx * fx
+    sw = contract(hist["aw"]["value"][1:], S_aa)
+    prof[2] = 2 * cw * np.sum(sw * fa[:, 1:].T, axis=0) * fw
+    wi, ws = (contract(hist[k]["value"][None, :], S_ax)[0] for k in ("wx_interior", "wx_surface"))
+    prof[4] = 2 * crho * cw * (wi + c * ws) * fw * fx
+    prof[5] = cw * cw * contract(hist["ww"]["value"][None, :], S_aa)[0] * fw * fw
+    return total(prof), prof
+
+
+# ---------------------------------------------------------------------------
 # rigid bodies (PartialHistogramManagerMT) and the plain composite I(q)
 # ---------------------------------------------------------------------------
 lib.orc_rigid_transform.restype = None

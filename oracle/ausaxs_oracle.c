@@ -387,9 +387,47 @@ static double orc_vdw(int t) {
     }
 }
 
-int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
-                 int cap, double* volume) {
-    const double width = 1.0, rvol = 2.15;
+/* RadialLineGenerator::generate (RadialLineGenerator.cpp:15-63) at width 1
+ * with the radii of collision_check (RawGridWithSurfaceExv.cpp:15-16):
+ * 8 divisions of one octant reflected to the full sphere, duplicates (points
+ * within 1e-5, components below 1e-5 set to 0) removed in first-seen order;
+ * off[d][r] = bin offset of direction d at radius r. Returns the count. */
+static int radial_offsets(int off[][3][3], int cap) {
+    const double pi = 3.14159265358979323846, ang = 2 * pi / 8;
+    const double radius[3] = {sqrt(1.0) + 1e-3, 2.0, 3.0};
+    double dirs[256][3];
+    int nd = 0;
+    for (double theta = 0; theta <= pi * 0.5; theta += ang)
+        for (double phi = 0; phi <= pi * 0.5; phi += ang) {
+            const double base[3] = {cos(phi) * sin(theta), sin(phi) * sin(theta), cos(theta)};
+            /* reflections in the order +++, -++, +-+, --+, ++-, -+-, +--, --- */
+            for (int r = 0; r < 8; ++r) {
+                double p[3] = {(r & 1) ? -base[0] : base[0], (r & 2) ? -base[1] : base[1],
+                               (r & 4) ? -base[2] : base[2]};
+                for (int c = 0; c < 3; ++c)
+                    if (fabs(p[c]) < 1e-5) p[c] = 0;
+                int dup = 0;
+                for (int e = 0; e < nd && !dup; ++e) {
+                    double dx = dirs[e][0] - p[0], dy = dirs[e][1] - p[1], dz = dirs[e][2] - p[2];
+                    dup = sqrt(dx * dx + dy * dy + dz * dz) < 1e-5;
+                }
+                if (!dup && nd < 256) memcpy(dirs[nd++], p, sizeof p);
+            }
+        }
+    if (nd > cap) return -1;
+    for (int d = 0; d < nd; ++d)
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) off[d][r][c] = (int)(round(radius[r] * dirs[d][c]) * 1.0);
+    return nd;
+}
+
+/* rvol: settings::grid::min_exv_radius; ra < 0: the van der Waals radius of
+ * each atom's type, else this radius for every atom (the reference's tests
+ * override Grid::get_atomic_radius this way) */
+static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                         double* out_xyz, int cap, double* surf_xyz, int* n_surf, double* volume, double rvol,
+                         double ra) {
+    const double width = 1.0;
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
@@ -432,7 +470,7 @@ int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* wa
     }
     /* expand_volume: atoms first (waters afterwards only add W_AREA marks) */
     for (int a = 0; a < n; ++a) {
-        double rvdw = orc_vdw(ff[a]);
+        double rvdw = ra < 0 ? orc_vdw(ff[a]) : ra;
         double rvdw2 = pow(rvdw, 2), rvol2 = pow(rvol, 2);
         double br = ceil((rvdw > rvol ? rvdw : rvol) / width);
         int bx = loc[3 * a], by = loc[3 * a + 1], bz = loc[3 * a + 2];
@@ -471,26 +509,76 @@ int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* wa
         }
     if (n == 0)
         for (int k = 0; k < 3; ++k) imin[k] = imax[k] = 0;
-    const int buffer = 2; /* max(1, round(max(2.15, 2)/1)) */
-    int count = 0, ok = 1;
+    /* max(1, round(max(min_exv_radius, 2) / cell_width)) (RawGridWithSurfaceExv.cpp:107) */
+    const int buffer = (int)fmax(1.0, round(fmax(rvol, 2.0) / width));
+    int off[64][3][3];
+    int n_dirs = surf_xyz ? radial_offsets(off, 64) : 0;
+    if (n_dirs < 0) {
+        free(g), free(loc);
+        return -1;
+    }
+#define OOB(a, b, c) ((a) < 0 || (a) >= bins[0] || (b) < 0 || (b) >= bins[1] || (c) < 0 || (c) >= bins[2])
+#define EMPTY_OR_WATER(a, b, c) (G(a, b, c) == G_EMPTY || (G(a, b, c) & (G_WAREA | G_WCENTER)))
+    int count = 0, ns = 0, ok = 1;
     for (int i = imin[0] - buffer > 0 ? imin[0] - buffer : 0; i < (imax[0] + buffer + 1 < bins[0] ? imax[0] + buffer + 1 : bins[0]); ++i)
         for (int j = imin[1] - buffer > 0 ? imin[1] - buffer : 0; j < (imax[1] + buffer + 1 < bins[1] ? imax[1] + buffer + 1 : bins[1]); ++j)
             for (int k = imin[2] - buffer > 0 ? imin[2] - buffer : 0; k < (imax[2] + buffer + 1 < bins[2] ? imax[2] + buffer + 1 : bins[2]); ++k) {
                 if (!(G(i, j, k) & (G_VOLUME | G_ACENTER | G_AAREA))) continue;
-                if (count < cap) {
-                    out_xyz[3 * count] = gmin[0] + i * width;
-                    out_xyz[3 * count + 1] = gmin[1] + j * width;
-                    out_xyz[3 * count + 2] = gmin[2] + k * width;
+                int surface = 0;
+                if (surf_xyz) {
+                    /* collision_check (RawGridWithSurfaceExv.cpp:14-80): per
+                     * direction, the first of r1, r2, r3 that is out of
+                     * bounds (+7) or occupied (+0, +3, +5) ends it; +7 if all
+                     * three are empty or water. Interior iff score >= 42. */
+                    int score = 0;
+                    for (int d = 0; d < n_dirs; ++d) {
+                        static const int gain[3] = {0, 3, 5};
+                        int r = 0;
+                        for (; r < 3; ++r) {
+                            int a = i + off[d][r][0], b = j + off[d][r][1], c = k + off[d][r][2];
+                            if (OOB(a, b, c)) {
+                                score += 7;
+                                break;
+                            }
+                            if (!EMPTY_OR_WATER(a, b, c)) {
+                                score += gain[r];
+                                break;
+                            }
+                        }
+                        if (r == 3) score += 7;
+                    }
+                    surface = score >= 42;
+                }
+                double* dst = surface ? surf_xyz : out_xyz;
+                int* cnt = surface ? &ns : &count;
+                if (*cnt < cap) {
+                    dst[3 * *cnt] = gmin[0] + i * width;
+                    dst[3 * *cnt + 1] = gmin[1] + j * width;
+                    dst[3 * *cnt + 2] = gmin[2] + k * width;
                 } else {
                     ok = 0;
                 }
-                ++count;
+                ++*cnt;
             }
+#undef OOB
+#undef EMPTY_OR_WATER
 #undef G
     free(g);
     free(loc);
     if (volume) *volume = vol * width * width * width;
+    if (n_surf) *n_surf = ns;
     return ok ? count : -1;
+}
+
+int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
+                 int cap, double* volume) {
+    return grid_exv_core(atoms, ff, n, waters, m, out_xyz, cap, NULL, NULL, volume, 2.15, -1);
+}
+
+int orc_grid_exv_surface(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                         double* interior_xyz, double* surface_xyz, int cap, int* n_surface, double* volume,
+                         double rvol, double ra) {
+    return grid_exv_core(atoms, ff, n, waters, m, interior_xyz, cap, surface_xyz, n_surface, volume, rvol, ra);
 }
 
 /* grid pair loops: xx self (factor 2, 1D), ax atoms x dummies (factor 1,
@@ -503,8 +591,9 @@ typedef struct {
     int nc;
     float inv_w;
     int nb;
-    int self;  /* xx */
-    int typed; /* ax: 2D by row ff */
+    int self;   /* xx */
+    int typed;  /* ax: 2D by row ff */
+    int factor; /* template N: 2 for dummy-dummy, 1 for atom/water-dummy */
 } grid_args;
 
 static int grid_row(job_ctx* c, void* pv, int i) {
@@ -515,7 +604,7 @@ static int grid_row(job_ctx* c, void* pv, int i) {
     const float* cols = A->self ? A->rows : A->cols;
     const int j0 = A->self ? i + 1 : 0;
     const int tail = A->self ? m - ((m - i - 1) % 4) : m - (m % 4);
-    const int factor = A->self ? 2 : 1;
+    const int factor = A->factor;
     const size_t off = A->typed ? (size_t)ff_of(pi) * A->nb : 0;
     for (int j = j0; j < m; ++j) {
         float d;
@@ -528,8 +617,8 @@ static int grid_row(job_ctx* c, void* pv, int i) {
     return 0;
 }
 
-static int run_grid(const float* rows, int nr, const float* cols, int nc, int self, int typed, float inv_w, int nb,
-                    int nthreads, orc_entry* out) {
+static int run_grid_f(const float* rows, int nr, const float* cols, int nc, int self, int typed, int factor,
+                      float inv_w, int nb, int nthreads, orc_entry* out) {
     const size_t n = (size_t)(typed ? ORC_MAX_FF : 1) * nb;
     simple_priv* privs = (simple_priv*)calloc(nthreads, sizeof(simple_priv));
     void** pp = (void**)malloc(sizeof(void*) * nthreads);
@@ -537,7 +626,7 @@ static int run_grid(const float* rows, int nr, const float* cols, int nc, int se
         privs[t].h = (orc_entry*)calloc(n, sizeof(orc_entry));
         pp[t] = &privs[t];
     }
-    grid_args A = {rows, nr, cols, nc, inv_w, nb, self, typed};
+    grid_args A = {rows, nr, cols, nc, inv_w, nb, self, typed, factor};
     int err = run_rows(nr, nthreads, grid_row, &A, pp);
     memset(out, 0, n * sizeof(orc_entry));
     for (int t = 0; t < nthreads; ++t) {
@@ -547,6 +636,11 @@ static int run_grid(const float* rows, int nr, const float* cols, int nc, int se
     free(privs);
     free(pp);
     return err;
+}
+
+static int run_grid(const float* rows, int nr, const float* cols, int nc, int self, int typed, float inv_w, int nb,
+                    int nthreads, orc_entry* out) {
+    return run_grid_f(rows, nr, cols, nc, self, typed, self ? 2 : 1, inv_w, nb, nthreads, out);
 }
 
 static void weighted_axis_n(const orc_entry* p, int n_entries, int n_axis, double bin_width, double* d_out) {
@@ -629,6 +723,105 @@ int orc_hist_ffgrid(const float* atoms, int n_a, const float* waters, int n_w, c
     for (int b = 0; b < max_bin; ++b) aa[IDX3(ORC_EXV, ORC_EXV, b, nb)] = xx[b];
     memcpy(pxx, xx, (size_t)nb * sizeof(orc_entry));
     free(xx), free(ax), free(wx), free(dax), free(wc);
+    return max_bin;
+}
+
+int orc_hist_ffgrid_surface(const float* atoms, int n_a, const float* waters, int n_w, const float* xi, int n_i,
+                            const float* xs, int n_s, double volume, float inv_w, int nb, int nthreads,
+                            orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot, orc_entry* ptot_ax,
+                            orc_entry* pxx, orc_entry* xxi, orc_entry* xxs, orc_entry* xxc, orc_entry* wxi,
+                            orc_entry* wxs, orc_entry* axi, orc_entry* axs) {
+    if (nthreads < 1) nthreads = 1;
+    /* the FFAvg base with Z from the grid volume (HistogramManagerMTFFGridSurface.cpp:28) */
+    double z = n_a == 0 ? 0 : volume * 0.334 / n_a;
+    int base = orc_hist_ff(ORC_FFAVG, atoms, n_a, waters, n_w, inv_w, nb, z, nthreads, aa, aw, ww, ptot);
+    if (base < 0) return base;
+    int err = 0;
+    /* :52-196: xx interior/surface self, interior x surface (factor 2), ax and wx against each */
+    err |= run_grid_f(xi, n_i, NULL, 0, 1, 0, 2, inv_w, nb, nthreads, xxi);
+    err |= run_grid_f(xs, n_s, NULL, 0, 1, 0, 2, inv_w, nb, nthreads, xxs);
+    err |= run_grid_f(xi, n_i, xs, n_s, 0, 0, 2, inv_w, nb, nthreads, xxc);
+    err |= run_grid_f(atoms, n_a, xi, n_i, 0, 1, 1, inv_w, nb, nthreads, axi);
+    err |= run_grid_f(atoms, n_a, xs, n_s, 0, 1, 1, inv_w, nb, nthreads, axs);
+    err |= run_grid_f(waters, n_w, xi, n_i, 0, 0, 1, inv_w, nb, nthreads, wxi);
+    err |= run_grid_f(waters, n_w, xs, n_s, 0, 0, 1, inv_w, nb, nthreads, wxs);
+    if (err) return -2;
+    {
+        orc_entry e = {(double)n_i, (uint32_t)n_i, 0};
+        entry_add(&xxi[0], &e);
+        orc_entry f = {(double)n_s, (uint32_t)n_s, 0};
+        entry_add(&xxs[0], &f);
+    }
+    int mbx = 10;
+    for (int i = nb - 1; i >= 10; --i)
+        if (xxs[i].value != 0 || xxi[i].value != 0 || wxs[i].value != 0 || wxi[i].value != 0) {
+            mbx = i + 1;
+            break;
+        }
+    /* p_tot: the FFAvg weighted counts at the first default q, centres = FFAvg d axis */
+    double* dax = (double*)calloc(base, sizeof(double));
+    orc_weighted_axis(ptot, base, 1.0 / inv_w, dax);
+    double* wc = (double*)calloc(base, sizeof(double));
+    const double q0 = 1e-4;
+    for (int f1 = 1; f1 < ORC_MAX_FF; ++f1)
+        for (int f2 = 1; f2 < ORC_MAX_FF; ++f2) {
+            double w = orc_ff_atomic(f1, q0) * orc_ff_atomic(f2, q0);
+            for (int b = 0; b < base; ++b) wc[b] += aa[IDX3(f1, f2, b, nb)].value * w;
+        }
+    for (int f = 1; f < ORC_MAX_FF; ++f) {
+        double w = 2 * 1.0 * (orc_ff_atomic(f, q0) * orc_ff_atomic(ORC_WATER, q0));
+        for (int b = 0; b < base; ++b) wc[b] += aw[IDX2(f, b, nb)].value * w;
+    }
+    {
+        double w = 1.0 * 1.0 * (orc_ff_atomic(ORC_WATER, q0) * orc_ff_atomic(ORC_WATER, q0));
+        for (int b = 0; b < base; ++b) wc[b] += ww[b].value * w;
+    }
+    int max_bin = mbx > base ? mbx : base;
+    memset(ptot, 0, (size_t)nb * sizeof(orc_entry));
+    for (int b = 0; b < base; ++b) {
+        ptot[b].value = wc[b];
+        ptot[b].count = 0;
+        ptot[b].bin_center = dax[b];
+    }
+    for (int f1 = 0; f1 < ORC_MAX_FF; ++f1) {
+        for (int f2 = 0; f2 < ORC_MAX_FF; ++f2)
+            for (int b = base; b < nb; ++b) memset(&aa[IDX3(f1, f2, b, nb)], 0, sizeof(orc_entry));
+        for (int b = base; b < nb; ++b) memset(&aw[IDX2(f1, b, nb)], 0, sizeof(orc_entry));
+    }
+    for (int b = base; b < nb; ++b) memset(&ww[b], 0, sizeof(orc_entry));
+    /* :233-247: the weighted totals that give the ax and xx distance axes */
+    memset(ptot_ax, 0, (size_t)nb * sizeof(orc_entry));
+    memset(pxx, 0, (size_t)nb * sizeof(orc_entry));
+    for (int b = 0; b < max_bin; ++b) {
+        entry_add(&ptot_ax[b], &wxi[b]);
+        entry_add(&ptot_ax[b], &wxs[b]);
+    }
+    for (int f = 0; f < ORC_MAX_FF; ++f)
+        for (int b = 0; b < max_bin; ++b) {
+            entry_add(&ptot_ax[b], &axi[IDX2(f, b, nb)]);
+            entry_add(&ptot_ax[b], &axs[IDX2(f, b, nb)]);
+        }
+    for (int b = 0; b < max_bin; ++b) {
+        entry_add(&pxx[b], &xxi[b]);
+        entry_add(&pxx[b], &xxs[b]);
+        entry_add(&pxx[b], &xxc[b]);
+    }
+    /* :248-256 and CompositeDistanceHistogramFFGridSurface::initialize (:88-102):
+     * the exv rows are cleared, then hold the dummy profiles at unit scaling */
+    for (int f = 0; f < ORC_MAX_FF; ++f)
+        for (int b = 0; b < nb; ++b) memset(&aa[IDX3(f, ORC_EXV, b, nb)], 0, sizeof(orc_entry));
+    for (int b = 0; b < nb; ++b) memset(&aw[IDX2(ORC_EXV, b, nb)], 0, sizeof(orc_entry));
+    for (int f = 1; f < ORC_MAX_FF; ++f)
+        for (int b = 0; b < max_bin; ++b) {
+            entry_add(&aa[IDX3(f, ORC_EXV, b, nb)], &axi[IDX2(f, b, nb)]);
+            entry_add(&aa[IDX3(f, ORC_EXV, b, nb)], &axs[IDX2(f, b, nb)]);
+        }
+    for (int b = 0; b < max_bin; ++b) {
+        aa[IDX3(ORC_EXV, ORC_EXV, b, nb)] = pxx[b];
+        entry_add(&aw[IDX2(ORC_EXV, b, nb)], &wxi[b]);
+        entry_add(&aw[IDX2(ORC_EXV, b, nb)], &wxs[b]);
+    }
+    free(dax), free(wc);
     return max_bin;
 }
 

@@ -69,6 +69,18 @@ int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int 
 int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
                  int cap, double* volume);
 
+/* RawGridWithSurfaceExv::create with surface detection at unit cell width
+ * (RawGridWithSurfaceExv.cpp:82-127): the dummies of orc_grid_exv split by
+ * collision_check (:14-80) over the radial lines of RadialLineGenerator
+ * (RadialLineGenerator.cpp:15-63). interior_xyz / surface_xyz: cap x 3 each.
+ * rvol: settings::grid::min_exv_radius (2.15 by default); ra < 0: each
+ * atom's van der Waals radius, else ra for every atom (GridDebug of the
+ * reference's grid tests). Returns the interior count (or -1 if cap is too
+ * small); *n_surface. */
+int orc_grid_exv_surface(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                         double* interior_xyz, double* surface_xyz, int cap, int* n_surface, double* volume,
+                         double rvol, double ra);
+
 /* HistogramManagerMTFFGrid::calculate_all (HistogramManagerMTFFGrid.cpp:43-234)
  * on top of the FFAvg manager. atoms/waters: n x 4 floats {x,y,z,ff} as for
  * orc_hist_ff; dummies: nx x 4 floats (integer lattice points); volume: grid
@@ -146,6 +158,18 @@ int orc_hist_symmetry_partial(const float* atoms, const float* self_atoms, const
                               int n_bodies, const int* sym_per_body, const int* sym_reps, const double* transforms,
                               const int* sched_len, const int* schedule, const float* waters, int n_w, float inv_w,
                               int nb, int nthreads, orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot);
+
+/* HistogramManagerMTFFGridSurface::calculate_all (HistogramManagerMTFFGridSurface.cpp:16-274)
+ * with interior dummies xi and surface dummies xs (n x 4 floats, type EXV):
+ * the FFAvg base with Z from the grid volume, then xx interior / surface /
+ * interior x surface (factor 2), ax and wx against each (factor 1); exv rows
+ * of aa / aw set to the profiles at unit surface scaling. Profiles: [nb]
+ * (xx*, wx*), [15][nb] (ax*). Returns max_bin or a negative status. */
+int orc_hist_ffgrid_surface(const float* atoms, int n_a, const float* waters, int n_w, const float* xi, int n_i,
+                            const float* xs, int n_s, double volume, float inv_w, int nb, int nthreads,
+                            orc_entry* aa, orc_entry* aw, orc_entry* ww, orc_entry* ptot, orc_entry* ptot_ax,
+                            orc_entry* pxx, orc_entry* xxi, orc_entry* xxs, orc_entry* xxc, orc_entry* wxi,
+                            orc_entry* wxs, orc_entry* axi, orc_entry* axs);
 
 /* ---- weighted axis, sinc, Debye ------------------------------------------ */
 /* WeightedDistribution1D::get_weighted_axis (WeightedDistribution1D.cpp:51-58)
