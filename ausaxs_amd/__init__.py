@@ -45,6 +45,7 @@ class _Settings(C.Structure):
         ("weighted_bins", C.c_int),
         ("debye_terms", C.c_int),
         ("tile_j", C.c_int),
+        ("lattice", C.c_int),
     ]
 
 
@@ -99,7 +100,8 @@ def _load():
         "ausaxs_server_free": (C.c_int, [vp]),
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
-        "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int)]),
+        "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int),
+                                         P(C.c_longlong), P(C.c_longlong)]),
         "ausaxs_fitter_create": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_uint, vp, C.c_int, P(vp)]),
         "ausaxs_scalable_create": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
         "ausaxs_scalable_calculate": (C.c_int, [vp, C.c_double, P(vp)]),
@@ -274,8 +276,11 @@ class Structure:
 
     def last_timing(self):
         pm, tm, pr, ni = C.c_double(), C.c_double(), C.c_longlong(), C.c_int()
-        _check(lib.ausaxs_last_timing(self._s, C.byref(pm), C.byref(tm), C.byref(pr), C.byref(ni)))
-        return dict(pair_ms=pm.value, total_ms=tm.value, pairs=pr.value, n_items=ni.value)
+        kp, lp = C.c_longlong(), C.c_longlong()
+        _check(lib.ausaxs_last_timing(self._s, C.byref(pm), C.byref(tm), C.byref(pr), C.byref(ni), C.byref(kp),
+                                      C.byref(lp)))
+        return dict(pair_ms=pm.value, total_ms=tm.value, pairs=pr.value, n_items=ni.value, kernel_pairs=kp.value,
+                    lattice_pairs=lp.value)
 
 
 def calculate_all(mode, atoms, waters=None, settings=None, z_exv=1.0) -> CompositeHistogram:

@@ -76,7 +76,9 @@ HistSettings to_settings(const ausaxs_settings* s) {
         h.weighted_bins = s->weighted_bins != 0;
         h.debye_terms = s->debye_terms;
         h.tile_j = s->tile_j;
+        h.lattice = s->lattice;
     }
+    require(h.lattice >= 0 && h.lattice <= 2, "lattice must be 0, 1 or 2");
     require(h.bin_width > 0, "bin_width must be positive");
     require(h.debye_terms >= 0 && h.debye_terms <= 3, "debye_terms must be 0, 1, 2 or 3");
     return h;
@@ -99,6 +101,7 @@ void ausaxs_default_settings(ausaxs_settings* s) {
     s->weighted_bins = h.weighted_bins ? 1 : 0;
     s->debye_terms = h.debye_terms;
     s->tile_j = h.tile_j;
+    s->lattice = h.lattice;
 }
 
 int ausaxs_get_last_error(char* buf, int len) {
@@ -471,13 +474,15 @@ int ausaxs_structure_set_timing(ausaxs_structure* s, int on) {
 }
 
 int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
-                       int* n_items) {
+                       int* n_items, long long* kernel_pairs, long long* lattice_pairs) {
     return guarded([&] {
         require(s != nullptr, "NULL structure");
         if (pair_ms) *pair_ms = s->last.pair_ms;
         if (total_ms) *total_ms = s->last.total_ms;
         if (pairs) *pairs = s->last.pairs;
         if (n_items) *n_items = s->last.n_items;
+        if (kernel_pairs) *kernel_pairs = s->last.kernel_pairs;
+        if (lattice_pairs) *lattice_pairs = s->last.lattice_pairs;
     });
 }
 

@@ -159,7 +159,7 @@ void add_items(std::vector<PairWork>& items, int i0, int i1, int j0, int j1, int
 }  // namespace
 
 void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
-                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split_tiles) {
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split_tiles, bool body_items) {
     const bool self = job.col_set < 0;
     const PointSet& R = sets[job.row_set];
     const PointSet& Cs = self ? R : sets[job.col_set];
@@ -186,7 +186,7 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
     if (self) {
         const int64_t n = R.size();
         wl.pairs += n * (n - 1) / 2;
-        for (int a = 0; a < CR; ++a) {
+        for (int a = 0; a < CR && body_items; ++a) {
             if (LR.seg_body[a] == 0) continue;
             const int s0 = LR.seg_start[a], s1 = s0 + LR.seg_body[a];
             // triangle: diagonal blocks masked to j > i, the rest plain
@@ -203,7 +203,10 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
             }
         }
         const int nn = R.size();
-        if (nn >= 2) add_strip(StripWork{nn, nn - std::min(3, nn), nn, nn, 1, CC});
+        if (nn >= 2) {
+            add_strip(StripWork{nn, nn - std::min(3, nn), nn, nn, 1, CC});
+            for (int t = nn - std::min(3, nn); t < nn; ++t) wl.strip_pairs += t;  // rows i < t
+        }
     } else {
         const int64_t n = R.size(), m = Cs.size();
         wl.pairs += n * m;
@@ -214,7 +217,7 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
         std::vector<int> seg_cls;  // non-empty row classes in buffer order
         for (int a = 0; a < CR; ++a)
             if (LR.seg_len[a]) seg_cls.push_back(a);
-        for (int b = 0; b < CC; ++b) {
+        for (int b = 0; b < CC && body_items; ++b) {
             if (LC.seg_colbody[b] == 0) continue;
             const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
             size_t k = 0;
@@ -255,8 +258,75 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
             }
         }
         const int mm = Cs.size();
-        if (mm % 4 && R.size() > 0) add_strip(StripWork{R.size(), mm - (mm % 4), mm, mm, 0, CC});
+        if (mm % 4 && R.size() > 0) {
+            add_strip(StripWork{R.size(), mm - (mm % 4), mm, mm, 0, CC});
+            wl.strip_pairs += (int64_t)R.size() * (mm % 4);
+        }
     }
+}
+
+bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mode, LatticeJob& out) {
+    if (mode == 0 || !job.lattice) return false;
+    const bool self = job.col_set < 0;
+    const PointSet& R = sets[job.row_set];
+    const PointSet& Cs = self ? R : sets[job.col_set];
+    if (R.n_classes != 1 || Cs.n_classes != 1 || job.rowmap.size() != 1 || job.rowmap[0] < 0) return false;
+    LatticeJob j;
+    j.row_set = job.row_set;
+    j.col_set = job.col_set;
+    j.out_row = job.rowmap[0];
+    // the body pairs: the strips take the self-loop tail candidates (last 3
+    // points) and the cross loop's scalar columns (last m mod 4)
+    const int n = R.size(), m = Cs.size();
+    j.n_rows = self ? n - std::min(3, n) : n;
+    j.n_cols = self ? j.n_rows : m - (m % 4);
+    j.pairs = self ? (int64_t)j.n_rows * (j.n_rows - 1) / 2 : (int64_t)j.n_rows * j.n_cols;
+    if (j.pairs == 0) return false;
+    // integer coordinates well inside float's exact range, and their span
+    constexpr float kLim = 1 << 20;
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    auto scan = [&](const PointSet& ps, int count) {
+        for (int i = 0; i < count; ++i) {
+            const float v[3] = {ps.orig[i].x, ps.orig[i].y, ps.orig[i].z};
+            for (int k = 0; k < 3; ++k) {
+                if (!(std::fabs(v[k]) <= kLim) || v[k] != std::rint(v[k])) return false;
+                lo[k] = std::min(lo[k], (int)v[k]);
+                hi[k] = std::max(hi[k], (int)v[k]);
+            }
+        }
+        return true;
+    };
+    if (!scan(R, j.n_rows) || (!self && !scan(Cs, j.n_cols))) return false;
+    int64_t cells = 1, s_true = 0;
+    for (int k = 0; k < 3; ++k) {
+        const int64_t E = (int64_t)hi[k] - lo[k] + 1;
+        int64_t L = 1;
+        while (L < 2 * E - 1) L *= 2;
+        if (L > kLatticeMaxL) return false;
+        j.g.origin[k] = lo[k];
+        j.g.L[k] = (int)L;
+        cells *= L;
+        s_true += (E - 1) * (E - 1);
+        j.smax += (int)((L / 2) * (L / 2));
+    }
+    // d2 = s exact in float: every squared displacement below 2^24
+    if (s_true >= (int64_t(1) << 24) || cells > (int64_t(1) << 28)) return false;
+    // a transform costs about as much as 50 pair evaluations per cell
+    if (mode == 1 && j.pairs < 64 * cells * (self ? 1 : 2)) return false;
+    out = j;
+    return true;
+}
+
+size_t lattice_bytes(const std::vector<LatticeJob>& jobs) {
+    size_t cells = 0, cross = 0, smax = 0, tw = 0;
+    for (const LatticeJob& j : jobs) {
+        const size_t c = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
+        cells = std::max(cells, c);
+        if (j.col_set >= 0) cross = std::max(cross, c);
+        smax = std::max(smax, (size_t)j.smax + 1);
+        tw += (size_t)(j.g.L[0] + j.g.L[1] + j.g.L[2]) / 2 + 3;
+    }
+    return (cells + cross + tw) * sizeof(double2) + smax * 8 + 64;
 }
 
 void sort_items(std::vector<PairWork>& items) {
@@ -300,8 +370,10 @@ MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
 }
 
 PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-                       int tile_j)
-    : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w) {
+                       int tile_j, int lattice)
+    : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w),
+      lattice_mode_(lattice) {
+    if (lattice < 0 || lattice > 2) throw std::runtime_error("ausaxs_hip: lattice must be 0, 1 or 2");
     check_tile_j(tile_j);
     check_pair_lds(nbins);
     // bins need no clamp in the pair kernel when the bounding-box diagonal
@@ -319,9 +391,17 @@ void PairEngine::build(int tile_j) {
     layout_.clear();
     for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf));
     WorkList plain, split;
+    lat_.clear();
     for (const PairJob& job : jobs_) {
-        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false);
-        append_job(split, job, sets_, layout_, n_rows_, tile_j, true);
+        LatticeJob lj;
+        const bool lattice = plan_lattice(job, sets_, lattice_mode_, lj);
+        if (lattice) {
+            plain.lattice_pairs += lj.pairs;
+            split.lattice_pairs += lj.pairs;
+            lat_.push_back(lj);
+        }
+        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false, !lattice);
+        append_job(split, job, sets_, layout_, n_rows_, tile_j, true, !lattice);
     }
     const bool use_split = prefer_split(padded_work(plain.items), item_rows(plain.items), padded_work(split.items),
                                         item_rows(split.items));
@@ -349,6 +429,31 @@ void PairEngine::build(int tile_j) {
     d_partial_.ensure(std::max<size_t>(wl_.items.size(), 1) * 2 * (nbins_ + 1));
     d_cnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
     d_dsum_.ensure((size_t)n_rows_ * (nbins_ + 1));
+    if (!lat_.empty()) {
+        // twiddle tables exp(-2 pi i k / L), k < L / 2, per job and axis
+        std::vector<double2> tw;
+        size_t cells = 0, cross = 0;
+        int smax = 0;
+        for (LatticeJob& j : lat_) {
+            for (int a = 0; a < 3; ++a) {
+                j.tw_off[a] = (int)tw.size();
+                const int L = j.g.L[a];
+                for (int k = 0; k < std::max(1, L / 2); ++k) {
+                    const double ang = -2.0 * 3.14159265358979323846 * (double)k / (double)L;
+                    tw.push_back(make_double2(std::cos(ang), std::sin(ang)));
+                }
+            }
+            const size_t c = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
+            cells = std::max(cells, c);
+            if (j.col_set >= 0) cross = std::max(cross, c);
+            smax = std::max(smax, j.smax);
+        }
+        d_tw_.upload(tw.data(), tw.size(), s);
+        d_lat_a_.ensure(cells);
+        if (cross) d_lat_b_.ensure(cross);
+        d_scount_.ensure((size_t)smax + 1);
+        d_lat_flag_.ensure(2);
+    }
     ctx.sync();
 }
 
@@ -363,6 +468,31 @@ void PairEngine::enqueue(bool timed) {
     MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
                    n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_cnt_.p, d_dsum_.p};
     check(launch_merge(ml, s), "merge kernels");
+    // lattice jobs add into the merged rows
+    // the flag buffer holds two ints, cleared as one 8-byte word
+    if (!lat_.empty())
+        check(launch_lattice_zero_u64((unsigned long long*)d_lat_flag_.p, 1, s), "lattice_zero_u64_kernel");
+    for (const LatticeJob& j : lat_) {
+        const size_t cells = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
+        const double2* tw[3] = {d_tw_.p + j.tw_off[0], d_tw_.p + j.tw_off[1], d_tw_.p + j.tw_off[2]};
+        const bool self = j.col_set < 0;
+        check(launch_lattice_zero(d_lat_a_.p, cells, s), "lattice_zero_kernel");
+        check(launch_lattice_fill(d_orig_[j.row_set].p, j.n_rows, j.g, d_lat_a_.p, s), "lattice_fill_kernel");
+        check(launch_fft3(d_lat_a_.p, j.g, tw, false, s), "fft_lines_kernel");
+        if (!self) {
+            check(launch_lattice_zero(d_lat_b_.p, cells, s), "lattice_zero_kernel");
+            check(launch_lattice_fill(d_orig_[j.col_set].p, j.n_cols, j.g, d_lat_b_.p, s), "lattice_fill_kernel");
+            check(launch_fft3(d_lat_b_.p, j.g, tw, false, s), "fft_lines_kernel");
+        }
+        check(launch_lattice_spectrum(d_lat_a_.p, self ? nullptr : d_lat_b_.p, cells, s), "lattice_spectrum_kernel");
+        check(launch_fft3(d_lat_a_.p, j.g, tw, true, s), "fft_lines_kernel");
+        check(launch_lattice_zero_u64(d_scount_.p, (size_t)j.smax + 1, s), "lattice_zero_u64_kernel");
+        check(launch_lattice_shist(d_lat_a_.p, j.g, self, self ? j.n_rows : 0, d_scount_.p, j.smax, d_lat_flag_.p, s),
+              "lattice_shist_kernel");
+        const size_t row = (size_t)j.out_row * (nbins_ + 1);
+        check(launch_lattice_bins(d_scount_.p, j.smax, inv_w_, nbins_, d_cnt_.p + row, d_dsum_.p + row, s),
+              "lattice_bins_kernel");
+    }
     for (const StripSpec& st : wl_.strips) {
         StripLaunch sl{d_orig_[st.set_rows].p, d_orig_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
                        d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};
@@ -379,7 +509,13 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     dsum.resize(n);
     check(hipMemcpyAsync(cnt.data(), d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     check(hipMemcpyAsync(dsum.data(), d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    int flag = 0;
+    if (!lat_.empty())
+        check(hipMemcpyAsync(&flag, d_lat_flag_.p, sizeof(int), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     ctx.sync();
+    if (flag)
+        throw std::runtime_error("ausaxs_hip: lattice correlation left a count off an integer (flag " +
+                                 std::to_string(flag) + ")");
 }
 
 void PairEngine::run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, bool timed) {
@@ -390,7 +526,10 @@ void PairEngine::run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, boo
 EngineTiming PairEngine::timing() {
     EngineTiming t;
     t.pairs = wl_.pairs;
+    t.lattice_pairs = wl_.lattice_pairs;
+    t.kernel_pairs = wl_.pairs - wl_.lattice_pairs - wl_.strip_pairs;
     t.n_items = (int)wl_.items.size();
+    t.n_lattice = (int)lat_.size();
     if (have_timing_) {
         check(hipEventSynchronize(ev_[2]), "hipEventSynchronize");
         check(hipEventElapsedTime(&t.pair_ms, ev_[0], ev_[1]), "hipEventElapsedTime");

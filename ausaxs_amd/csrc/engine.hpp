@@ -106,6 +106,10 @@ struct PairJob {
     int row_set = 0;
     int col_set = -1;             // -1: self job
     std::vector<int32_t> rowmap;  // [n_row_classes][n_col_classes]
+    // one class on each side and integer-lattice points (grid dummies): the
+    // body pairs may be histogrammed by lattice correlation instead of the
+    // pair kernel (the tail pairs keep their strips)
+    bool lattice = false;
 };
 
 // Class-sorted layout of one point set inside a point buffer. Within each
@@ -133,11 +137,33 @@ struct WorkList {
     std::vector<PairWork> items;
     std::vector<StripSpec> strips;
     std::vector<int32_t> rowmaps;
+    int64_t pairs = 0;          // all pairs of the jobs
+    int64_t strip_pairs = 0;    // of which in strips
+    int64_t lattice_pairs = 0;  // of which by lattice correlation
+};
+// split: row tiles of cross jobs may span two row classes (kSplit items);
+// body_items false: the body pairs get no work items (lattice path)
+void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
+                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split, bool body_items = true);
+
+// A job whose body pairs are histogrammed on the integer lattice: rows are
+// the first n_rows points of row_set (original order), columns the first
+// n_cols of col_set (self: col_set = -1, the row points themselves).
+struct LatticeJob {
+    int row_set = 0, col_set = -1;
+    int n_rows = 0, n_cols = 0;
+    int out_row = 0;
+    LatticeGeom g{};
+    int smax = 0;
+    int tw_off[3] = {0, 0, 0};  // twiddle tables in the engine's table buffer
     int64_t pairs = 0;
 };
-// split: row tiles of cross jobs may span two row classes (kSplit items)
-void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
-                const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split);
+// The lattice job of a one-class job whose body points are all integers with
+// a small enough span, or false. mode: 0 never, 1 when the body pairs
+// outweigh the transforms, 2 whenever possible.
+bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mode, LatticeJob& out);
+// device bytes of the lattice buffers for a set of jobs
+size_t lattice_bytes(const std::vector<LatticeJob>& jobs);
 // Row work the pair kernel spends on a list: kTileI row lanes per item times
 // its columns (padding of partial row tiles included).
 int64_t padded_work(const std::vector<PairWork>& items);
@@ -171,18 +197,22 @@ int pick_copies(int nbins, int rows);
 int item_rows(const std::vector<PairWork>& items);
 
 struct EngineTiming {
-    float pair_ms = 0;   // pair_hist_kernel on the engine stream (HIP events)
-    float total_ms = 0;  // pair + merge + strips
-    int64_t pairs = 0;   // algorithmic pair evaluations
+    float pair_ms = 0;          // pair_hist_kernel on the engine stream (HIP events)
+    float total_ms = 0;         // pair + merge + lattice + strips
+    int64_t pairs = 0;          // pairs in the histograms
+    int64_t kernel_pairs = 0;   // of which evaluated by pair_hist_kernel
+    int64_t lattice_pairs = 0;  // of which counted by lattice correlation
     int n_items = 0;
+    int n_lattice = 0;          // lattice jobs
 };
 
 // Coordinates resident on the device with their work lists built once; run()
 // recomputes all jobs into [n_rows][nbins+1] (count, fixed-point delta sum).
 class PairEngine {
   public:
+    // lattice: lattice-path mode for PairJob::lattice jobs (see plan_lattice)
     PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-               int tile_j = 8000);
+               int tile_j = 8000, int lattice = 1);
     PairEngine(const PairEngine&) = delete;
     PairEngine& operator=(const PairEngine&) = delete;
 
@@ -197,6 +227,7 @@ class PairEngine {
     int nbins() const { return nbins_; }
     float inv_w() const { return inv_w_; }
     int64_t pairs() const { return wl_.pairs; }
+    int n_lattice() const { return (int)lat_.size(); }
     int n_items() const { return (int)wl_.items.size(); }
     const uint64_t* device_cnt() const { return d_cnt_.p; }
     const int64_t* device_dsum() const { return d_dsum_.p; }
@@ -217,6 +248,11 @@ class PairEngine {
     bool clamp_ = true;
     int grid_ = 0;
     int rows_ = 2, copies_ = 1;
+    int lattice_mode_ = 1;
+    std::vector<LatticeJob> lat_;
+    DevBuf<double2> d_lat_a_, d_lat_b_, d_tw_;
+    DevBuf<unsigned long long> d_scount_;
+    DevBuf<int> d_lat_flag_;
 
     DevBuf<float4> d_pts_;
     std::vector<DevBuf<float4>> d_orig_;

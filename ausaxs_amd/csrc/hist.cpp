@@ -312,6 +312,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
                 PairJob j;
                 j.row_set = row_set;
                 j.col_set = col_set;
+                j.lattice = row_set >= 2 && col_set != 1;  // dummies only
                 j.rowmap = {n_rows++};
                 rows_.push_back({kind, 0, 0});
                 jobs.push_back(std::move(j));
@@ -342,6 +343,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
             // xx (self, :68-89), ax (atoms x dummies by atom type, :92-113), wx (waters x dummies, :116-137)
             PairJob xx, ax, wx;
             xx.row_set = 2;
+            xx.lattice = true;
             xx.rowmap = {n_rows++};
             rows_.push_back({3, 0, 0});
             ax.row_set = 0;
@@ -424,7 +426,8 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     sets.push_back(std::move(W));
     if (grid) sets.push_back(std::move(X));
     if (mode == Mode::FFGridSurface) sets.push_back(std::move(XS));
-    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j);
+    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j,
+                                           s.lattice);
 }
 
 std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool timed) {

@@ -69,6 +69,7 @@ struct HistSettings {
     bool weighted_bins = true;  // settings::hist::weighted_bins
     int debye_terms = 3;        // bf16 terms per operand; 0 = FP64 contraction
     int tile_j = 8000;          // columns per work item (<= 8189)
+    int lattice = 1;            // grid dummy-dummy pairs by lattice correlation: 0 off, 1 auto, 2 always
 };
 // WidthControllers.h:10-27: inverse width as float
 float inv_width(const HistSettings& s);

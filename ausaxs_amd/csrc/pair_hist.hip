@@ -21,19 +21,24 @@
 // square root (sqrtf: v_sqrt_f32 plus an FMA-residual correction; note that
 // __fsqrt_rn lowers to the uncorrected instruction) decides.
 // launch_bin_check verifies both exhaustively on device.
+#include "binning.hpp"
 #include "device_api.hpp"
 
 namespace ausaxs::gpu {
 namespace {
+
+using binning::near_tie;
+using binning::quantize;
+using binning::simd_rule_exact;
+using binning::sqrt_fix;
+using binning::sqrt_rn;
+using binning::tie_fix;
 
 constexpr int kThreads = 256;
 constexpr int kRowsPerLane = 2;
 constexpr int kTileI = kThreads * kRowsPerLane;  // row points per work item
 constexpr int kColTile = kThreads;               // column points per LDS tile
 constexpr int kColGroup = 4;                     // column points per double-buffered group
-
-// correctly rounded square root (IEEE); see the note above
-__device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
 
 __device__ __forceinline__ void simd_rule_fast(float dx, float dy, float dz, float inv_w, float& x, float& b,
                                                float& dl, float& d2) {
@@ -47,51 +52,6 @@ __device__ __forceinline__ void simd_rule_fast(float dx, float dy, float dz, flo
     dl = x - b;
 }
 
-__device__ __forceinline__ bool near_tie(float x, float dl) {
-    // The raw sqrt is within 1 ulp (<= 2^-23 relative) of the correctly rounded
-    // one, and each of the two products rounds by <= 2^-24, so x is within
-    // 2^-22 * x of the correctly rounded x. Outside a window of 1.25 * 2^-22 * x
-    // around a .5 tie both round to the same bin (the margin covers the
-    // rounding of the threshold itself).
-    return fabsf(dl) > __builtin_fmaf(x, -2.98023223876953125e-7f, 0.5f);
-}
-
-__device__ __forceinline__ void simd_rule_exact(float d2, float inv_w, float& x, float& b, float& dl) {
-    const float r = sqrt_rn(d2);
-    x = r * inv_w;
-    b = __builtin_rintf(x);
-    dl = x - b;
-}
-
-// Correctly rounded sqrt(d2) from a raw v_sqrt_f32 result s (within 1 ulp),
-// for the tie fallback, where d2 >= (bin_width/2)^2 is normal. With s- and s+
-// the neighbours of s, s- s and s s+ are exact multiples of ulp(s)^2, as is
-// d2, so d2 > s- s implies d2 >= s- s + ulp^2 > ((s- + s)/2)^2 and d2 <= s- s
-// implies sqrt(d2) < (s- + s)/2 (likewise for s+). The FMA residuals'
-// signs are exact, so they pick the correctly rounded neighbour.
-__device__ __forceinline__ float sqrt_fix(float d2, float s) {
-    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
-    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
-    const float r_dn = __builtin_fmaf(-s_dn, s, d2);
-    const float r_up = __builtin_fmaf(-s_up, s, d2);
-    s = r_dn <= 0.f ? s_dn : s;
-    return r_up > 0.f ? s_up : s;
-}
-
-// fast-path result near a tie: redo the bin with the correctly rounded sqrt
-__device__ __forceinline__ void tie_fix(float d2, float inv_w, float& x, float& b, float& dl) {
-    const float r = sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2));
-    x = r * inv_w;
-    b = __builtin_rintf(x);
-    dl = x - b;
-}
-
-// kLowBias + k with k = rne(dl * 2^20) + 2^19: dl * 2^20 + 1.5625 * 2^23 lies in
-// [1.5 * 2^23, 1.625 * 2^23], where floats are the integers, so the FMA rounds
-// dl * 2^20 to an integer once and the bit pattern is 0x4B400000 + k
-__device__ __forceinline__ uint32_t quantize(float dl) {
-    return __float_as_uint(__builtin_fmaf(dl, 1048576.0f, 13107200.0f));
-}
 
 __device__ __forceinline__ uint64_t cell_add(float dl) {
     return (uint64_t(1) << 42) + (uint64_t)quantize(dl);

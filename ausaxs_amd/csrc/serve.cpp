@@ -63,6 +63,19 @@ size_t Server::footprint(Mode mode, const double* atoms_xyz, int n, const double
                    + (items / 16 + rows) * 2 * cell    // merge chunk sums
                    + rows * 2 * cell                   // counts + delta sums
                    + rows * nb * (8 + 6) + nq * nb * 6 + rows * nq * 8 + nq * 8;  // Debye operands
+    if (mode == Mode::FFGrid && s.lattice != 0 && n > 0) {
+        // lattice correlation of the dummies: one transform grid of at least
+        // twice the padded atom box per axis, and the squared-distance counts
+        double cells = 1, smax = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double E = std::ceil(ahi[k] - alo[k]) + 2 * 3 + 1;
+            double L = 1;
+            while (L < 2 * E - 1) L *= 2;
+            cells *= L;
+            smax += (L / 2) * (L / 2);
+        }
+        bytes += cells * 16 + smax * 8;
+    }
     return (size_t)(bytes * 1.1) + (1u << 20);
 }
 

@@ -53,6 +53,8 @@ typedef struct {
     int weighted_bins; /* settings::hist::weighted_bins (default 1) */
     int debye_terms;   /* bf16 terms per GEMM operand: 2 = hi+lo, 3 = hi+mid+lo (default); 0 = FP64 */
     int tile_j;        /* columns per pair work item (default 8000, at most 8189) */
+    int lattice;       /* grid dummy-dummy pairs by lattice correlation instead of pair by pair
+                          (identical histograms): 0 off, 1 when cheaper (default), 2 always */
 } ausaxs_settings;
 
 typedef struct ausaxs_structure ausaxs_structure; /* resident coordinates + work lists */
@@ -263,10 +265,12 @@ int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsig
  * known size (no counterpart in the reference). */
 int ausaxs_copy_probe(size_t bytes, int reps, double* ms);
 
-/* HIP-event timings of the last ausaxs_structure_calculate with timing on. */
+/* HIP-event timings of the last ausaxs_structure_calculate with timing on.
+ * pairs: all pairs in the histograms; kernel_pairs: of which pair_hist_kernel
+ * evaluated; lattice_pairs: of which lattice correlation counted. */
 int ausaxs_structure_set_timing(ausaxs_structure* s, int on);
 int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
-                       int* n_items);
+                       int* n_items, long long* kernel_pairs, long long* lattice_pairs);
 
 /* ---- bodies with symmetry copies ------------------------------------------ */
 
