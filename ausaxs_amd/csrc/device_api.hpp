@@ -110,6 +110,9 @@ struct StripLaunch {
     int n_rep = 1;
     int64_t rows_rep = 0, cols_rep = 0;
     int out_rep = 0;
+    // >= 0: every pair of the strip goes to this histogram row (one class on
+    // each side), which lets long strips accumulate in LDS first
+    int fixed_row = -1;
 };
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
 

@@ -496,6 +496,8 @@ void PairEngine::enqueue(bool timed) {
     for (const StripSpec& st : wl_.strips) {
         StripLaunch sl{d_orig_[st.set_rows].p, d_orig_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
                        d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};
+        if (sets_[st.set_rows].n_classes == 1 && sets_[st.set_cols].n_classes == 1)
+            sl.fixed_row = wl_.rowmaps[st.rowmap_off];
         check(launch_strip(sl, s), "strip_kernel");
     }
     if (timed) check(hipEventRecord(ev_[2], s), "hipEventRecord");

@@ -16,8 +16,9 @@
 // error). Counts and delta sums come out identical to evaluating every pair.
 //
 // FFT: radix-2 decimation in time on lines of length L (a power of two, at
-// most 1024) held in LDS, G lines per workgroup (G * L <= 4096 complex values,
-// 64 KB); loads run along the lines' common contiguous direction.
+// most 1024) held in LDS, G lines per workgroup (G * L <= 2048 complex values,
+// 32 KB, so that several workgroups share a CU); loads run along the lines'
+// common contiguous direction.
 #include "binning.hpp"
 #include "device_api.hpp"
 
@@ -199,7 +200,7 @@ hipError_t launch_fft3(double2* buf, const LatticeGeom& g, const double2* const 
             if (L > kLatticeMaxL || (L & (L - 1))) return hipErrorInvalidValue;
             int logL = 0;
             while ((1 << logL) < L) ++logL;
-            const int G = std::max(1, std::min(16, 4096 / L));
+            const int G = std::max(1, std::min(16, 2048 / L));
             const long long nlines = total / L;
             const unsigned blocks = (unsigned)((nlines + G - 1) / G);
             hipLaunchKernelGGL(fft_lines_kernel, dim3(blocks), dim3(256), (size_t)G * L * sizeof(double2), s, buf, L,

@@ -284,18 +284,11 @@ __global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* __restr
     dsum[(size_t)row * stride + bin] = s;
 }
 
-// Strip kernel: rows x tail columns with the reference's per-pair rule.
-__global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
+// One strip pair with the reference's per-pair rule: bin (clamped into the
+// overflow slot nbins) and offset k - 2^19 of pair (row i, column t).
+__device__ __forceinline__ void strip_pair(const StripLaunch& p, const float4& a, const float4& c, int i, int t,
+                                           unsigned& bin, int64_t& koff) {
     const StripWork& w = p.w;
-    const int nt = w.t_end - w.t_begin;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (long long)w.n_rows * nt) return;
-    const int i = (int)(tid / nt);
-    const int t = w.t_begin + (int)(tid % nt);
-    if (w.self && i >= t) return;
-    const int rep = blockIdx.y;
-    const float4 a = p.rows[p.rows_rep * rep + i];
-    const float4 c = p.cols[p.cols_rep * rep + t];
     const float dx = a.x - c.x, dy = a.y - c.y, dz = a.z - c.z;
     bool scalar_rule = true;
     if (w.self) scalar_rule = t >= w.n_cols_total - ((w.n_cols_total - i - 1) % 4);
@@ -311,14 +304,68 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
         const float d2 = (dx * dx + dy * dy) + dz * dz;
         simd_rule_exact(d2, p.inv_w, x, b, dl);
     }
+    bin = min((unsigned)(int)b, (unsigned)p.nbins);
+    koff = (int64_t)(quantize(dl) - (uint32_t)kLowBias) - kDeltaOffset;  // k in [0, 2^20]
+}
+
+// Strip kernel: rows x tail columns, one thread per pair, global atomics.
+__global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
+    const StripWork& w = p.w;
+    const int nt = w.t_end - w.t_begin;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long long)w.n_rows * nt) return;
+    const int i = (int)(tid / nt);
+    const int t = w.t_begin + (int)(tid % nt);
+    if (w.self && i >= t) return;
+    const int rep = blockIdx.y;
+    const float4 a = p.rows[p.rows_rep * rep + i];
+    const float4 c = p.cols[p.cols_rep * rep + t];
     int row = p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
     if (row < 0) return;  // the host guarantees a row for every present class pair
     row += rep * p.out_rep;
-    const unsigned bin = min((unsigned)(int)b, (unsigned)p.nbins);
-    const uint32_t low = quantize(dl) - (uint32_t)kLowBias;  // k in [0, 2^20]
+    unsigned bin;
+    int64_t koff;
+    strip_pair(p, a, c, i, t, bin, koff);
     const size_t cell = (size_t)row * p.stride + bin;
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
-    atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)low - kDeltaOffset));
+    atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)koff);
+}
+
+// Strip kernel for a strip with one output row (fixed_row) and no replicas:
+// kStripBlockPairs pairs per workgroup accumulate in an LDS histogram of
+// packed cells (as in the pair kernel), flushed with two global atomics per
+// occupied bin. Long strips (a dummy set's rows against its last columns)
+// crowd into few bins, where one global atomic per pair serialises.
+constexpr int kStripBlockPairs = 4096;
+__global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
+    extern __shared__ uint64_t cells[];  // [nbins + 1]
+    const StripWork& w = p.w;
+    const int nt = w.t_end - w.t_begin;
+    for (int b = threadIdx.x; b <= p.nbins; b += blockDim.x) cells[b] = 0;
+    __syncthreads();
+    const long long total = (long long)w.n_rows * nt;
+    const long long start = (long long)blockIdx.x * kStripBlockPairs;
+    const long long end = min(start + kStripBlockPairs, total);
+    for (long long tid = start + threadIdx.x; tid < end; tid += blockDim.x) {
+        const int i = (int)(tid / nt);
+        const int t = w.t_begin + (int)(tid % nt);
+        if (w.self && i >= t) continue;
+        unsigned bin;
+        int64_t koff;
+        strip_pair(p, p.rows[i], p.cols[t], i, t, bin, koff);
+        lds_add(&cells[bin], (uint64_t(1) << 42) + (uint64_t)(koff + kDeltaOffset) + kLowBias);
+    }
+    __syncthreads();
+    const size_t row = (size_t)p.fixed_row * p.stride;
+    for (int b = threadIdx.x; b <= p.nbins; b += blockDim.x) {
+        const uint64_t v = cells[b];
+        if (!v) continue;
+        uint64_t n, ksum;
+        cell_decode(v, n, ksum);
+        atomicAdd((unsigned long long*)&p.cnt[row + b], (unsigned long long)n);
+        atomicAdd((unsigned long long*)&p.dsum[row + b],
+                  (unsigned long long)((int64_t)ksum - (int64_t)n * kDeltaOffset));
+    }
 }
 
 __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi, float inv_w,
@@ -398,6 +445,13 @@ hipError_t launch_strip(const StripLaunch& p, hipStream_t s) {
     const long long n = (long long)p.w.n_rows * (p.w.t_end - p.w.t_begin);
     if (n <= 0) return hipSuccess;
     if (p.n_rep <= 0) return hipSuccess;
+    if (p.fixed_row >= 0 && p.n_rep == 1 && (size_t)(p.nbins + 1) * sizeof(uint64_t) <= 65536 &&
+        n > kStripBlockPairs) {
+        const unsigned blocks = (unsigned)((n + kStripBlockPairs - 1) / kStripBlockPairs);
+        hipLaunchKernelGGL(strip_lds_kernel, dim3(blocks), dim3(256), (size_t)(p.nbins + 1) * sizeof(uint64_t), s,
+                           p);
+        return hipGetLastError();
+    }
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(strip_kernel, dim3(blocks, p.n_rep), dim3(256), 0, s, p);
     return hipGetLastError();
