@@ -68,6 +68,7 @@ def _load():
         ),
         "ausaxs_structure_calculate": (C.c_int, [vp, P(vp)]),
         "ausaxs_structure_create_grid_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
+        "ausaxs_hydrate_radial": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, P(C.c_int), vp]),
         "ausaxs_grid_exv_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, P(C.c_int), vp, C.c_int,
                                               P(C.c_int)]),
         "ausaxs_structure_free": (C.c_int, [vp]),
@@ -394,6 +395,28 @@ class GridSurfaceStructure(Structure):
         _check(lib.ausaxs_structure_create_grid_surface(_ptr(self._a), _ptr(self._f), len(self._a), _ptr(self._w),
                                                         len(self._w), C.byref(s), C.byref(h)))
         self._s = h
+
+
+def hydrate(atoms_xyz, ff, bodies=None, noise=None):
+    """Radial hydration shell (RadialHydration with NoCulling); host-only.
+
+    bodies: body sizes (None: one body). noise: None (no displacement, as the
+    reference's own tests use), an int seed (N(0, 0.75) per component, the
+    reference's default distribution, from numpy), or an (n, 26, 3) array.
+    Returns (waters n x 3 FP64 in placement order, waters per body)."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    n = len(a)
+    b = np.ascontiguousarray([n] if bodies is None else bodies, np.int32)
+    if isinstance(noise, (int, np.integer)) and not isinstance(noise, bool):
+        noise = np.random.default_rng(int(noise)).normal(0.0, 0.75, (n, 26, 3))
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float64).reshape(n, 26, 3)
+    out = np.zeros((max(1, 26 * n), 3), np.float64)
+    m = C.c_int()
+    per = np.zeros(len(b), np.int32)
+    _check(lib.ausaxs_hydrate_radial(_ptr(a), _ptr(f), n, _ptr(b), len(b), None if nz is None else _ptr(nz),
+                                     _ptr(out), len(out), C.byref(m), _ptr(per)))
+    return out[: m.value].copy(), per
 
 
 def grid_exv_surface(atoms_xyz, ff, waters_xyz=None):

@@ -6,6 +6,7 @@
 
 #include "fitter.hpp"
 #include "grid.hpp"
+#include "hydrate.hpp"
 #include "hist.hpp"
 #include "rigid.hpp"
 #include "serve.hpp"
@@ -179,6 +180,23 @@ int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int
                 "capacity too small");
         if (interior_xyz) std::copy(g.xyz.begin(), g.xyz.end(), interior_xyz);
         if (surface_xyz) std::copy(g.surface.begin(), g.surface.end(), surface_xyz);
+    });
+}
+
+int ausaxs_hydrate_radial(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const int* body_sizes,
+                          int n_bodies, const double* noise, double* waters_xyz, int capacity, int* n_waters,
+                          int* per_body) {
+    return guarded([&] {
+        require(n_atoms >= 0 && n_bodies >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_bodies == 0 || body_sizes, "body_sizes is NULL");
+        std::vector<int> bodies(body_sizes, body_sizes + n_bodies);
+        Hydration h = hydrate_radial(atoms_xyz, atom_ff, n_atoms, bodies, noise);
+        const int m = (int)(h.waters.size() / 3);
+        if (n_waters) *n_waters = m;
+        if (per_body) std::copy(h.body_count.begin(), h.body_count.end(), per_body);
+        require(!waters_xyz || capacity >= m, "waters capacity too small");
+        if (waters_xyz) std::copy(h.waters.begin(), h.waters.end(), waters_xyz);
     });
 }
 

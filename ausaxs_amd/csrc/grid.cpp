@@ -37,9 +37,23 @@ struct RadialLines {
     std::vector<std::array<int, 3>> b1, b2, b3;
 };
 RadialLines radial_lines() {
-    const double pi = 3.14159265358979323846;
     const double r[3] = {std::sqrt(1.0) + 1e-3, 2.0, 3.0};
-    const double ang = 2 * pi / 8;
+    RadialLines out;
+    for (const auto& p : sphere_directions(8)) {
+        std::array<int, 3> v[3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) v[k][c] = (int)(std::round(r[k] * p[c]) * 1.0);
+        out.b1.push_back(v[0]);
+        out.b2.push_back(v[1]);
+        out.b3.push_back(v[2]);
+    }
+    return out;
+}
+}  // namespace
+
+std::vector<std::array<double, 3>> sphere_directions(int divisions) {
+    const double pi = 3.14159265358979323846;
+    const double ang = 2 * pi / divisions;
     std::vector<std::array<double, 3>> sphere;
     for (double theta = 0; theta <= pi * 0.5; theta += ang)
         for (double phi = 0; phi <= pi * 0.5; phi += ang) {
@@ -64,25 +78,13 @@ RadialLines radial_lines() {
         }
         if (!present) rots.push_back(p);
     }
-    RadialLines out;
-    for (const auto& p : rots) {
-        std::array<int, 3> v[3];
-        for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 3; ++c) v[k][c] = (int)(std::round(r[k] * p[c]) * 1.0);
-        out.b1.push_back(v[0]);
-        out.b2.push_back(v[1]);
-        out.b3.push_back(v[2]);
-    }
-    return out;
+    return rots;
 }
-}  // namespace
 
-GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
-                      bool detect_surface) {
-    GridExv out;
-    if (n <= 0) return out;
-    // grid minimum as in Grid(bodies) (Grid.cpp:34-104): bounding box grown from
-    // the origin, expanded by max(0.5*diff*scaling, margin) and a cell, floored.
+GridGeometry grid_geometry(const double* atoms, int n, const double* waters, int m) {
+    // Grid(bodies) (Grid.cpp:34-104): bounding box grown from the origin,
+    // expanded by max(0.5*diff*scaling, margin) and a cell, floored / ceiled
+    GridGeometry geo;
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], atoms[3 * i + k]), hi[k] = std::max(hi[k], atoms[3 * i + k]);
@@ -91,12 +93,21 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
     double scaling = 0.25;  // settings::grid::scaling; small systems switch to 1 (Grid.cpp:87-90)
     if (hi[0] - lo[0] < 50 || hi[1] - lo[1] < 50 || hi[2] - lo[2] < 50) scaling = 1;
     const double margin = 2.14 + 1.71 + (-0.35) + 3 * kCell;  // vdw P + vdw O + shell_correction + 3 cells
-    int bins[3];  // Axis3D(nmin, nmax, width).bins
     for (int k = 0; k < 3; ++k) {
         const double expand = std::max(0.5 * (hi[k] - lo[k]) * scaling, margin);
-        out.origin[k] = (int)std::floor(lo[k] - expand - kCell);
-        bins[k] = (int)std::ceil(hi[k] + expand + kCell) - out.origin[k];
+        geo.origin[k] = (int)std::floor(lo[k] - expand - kCell);
+        geo.bins[k] = (int)std::ceil(hi[k] + expand + kCell) - geo.origin[k];  // Axis3D(nmin, nmax, width).bins
     }
+    return geo;
+}
+
+GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                      bool detect_surface) {
+    GridExv out;
+    if (n <= 0) return out;
+    const GridGeometry geo = grid_geometry(atoms, n, waters, m);
+    int bins[3];
+    for (int k = 0; k < 3; ++k) out.origin[k] = geo.origin[k], bins[k] = geo.bins[k];
 
     // dense occupancy box around the atoms (one voxel = one lattice point)
     int bmin[3], bmax[3];

@@ -15,6 +15,7 @@
 // listed in lexicographic (x, y, z) voxel order; the grid volume is their count.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -38,5 +39,19 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
 
 // constants::radius::vdw per form-factor type (FormFactorType.h:174-193)
 double vdw_radius(int ff_type);
+
+// The grid box of Grid(bodies) (Grid.cpp:34-110) at cell width 1: integer
+// minimum and bin counts, from FP64 atoms and waters.
+struct GridGeometry {
+    int origin[3]{};
+    int bins[3]{};
+};
+GridGeometry grid_geometry(const double* atoms, int n, const double* waters, int m);
+
+// Unit directions of an octant of a sphere cut into `divisions` angles and
+// reflected into all eight, duplicates removed, in the reference's order
+// (RadialLineGenerator::generate, RadialLineGenerator.cpp:15-63, and
+// RadialHydration::prepare_rotations, RadialHydration.cpp:82-139): 26 for 8.
+std::vector<std::array<double, 3>> sphere_directions(int divisions);
 
 }  // namespace ausaxs::gpu

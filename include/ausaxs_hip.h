@@ -87,6 +87,20 @@ int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff
 int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
                                          const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
                                          ausaxs_structure** out);
+/* Radial hydration shell: RadialHydration::generate_explicit_hydration
+ * (RadialHydration.cpp:40-80) per body on the molecule's grid, as
+ * GridBasedHydration::hydrate runs it (GridBasedHydration.cpp:36-79) with the
+ * default NoCulling; candidates at vdw + 1.36 A along 26 directions from each
+ * atom, accepted by RadialHydration::collision_check (:141-206), each accepted
+ * water marking the grid before the next candidate. atoms: n x 3 FP64 with
+ * form-factor types; body_sizes: n_bodies sizes summing to n (NULL / 0: one
+ * body); noise: NULL (none; the reference's tests) or n x 26 x 3 displacements
+ * (the reference draws N(0, 0.75) per component). waters: capacity x 3, in
+ * placement order (capacity 26 n always suffices; NULL: count only);
+ * per_body: n_bodies counts or NULL. Host only (no device work). */
+int ausaxs_hydrate_radial(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const int* body_sizes,
+                          int n_bodies, const double* noise, double* waters_xyz, int capacity, int* n_waters,
+                          int* per_body);
 /* The interior and surface dummies of RawGridWithSurfaceExv::create, each in the
  * reference's voxel order; counts are returned also when a capacity is too small. */
 int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,

@@ -324,6 +324,26 @@ def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=No
     return xi[:n].copy(), xs[:ns.value].copy(), vol.value
 
 
+lib.orc_hydrate_radial.restype = C.c_int
+lib.orc_hydrate_radial.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_void_p]
+
+
+def hydrate_radial(atoms_xyz, ff, bodies=None, noise=None):
+    """Radial hydration shell (see orc_hydrate_radial): waters (m x 3) and per-body counts."""
+    a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+    b = np.ascontiguousarray([len(a)] if bodies is None else bodies, np.int32)
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float64).reshape(len(a), 26, 3)
+    cap = max(1, 26 * len(a))
+    out = np.zeros((cap, 3), np.float64)
+    per = np.zeros(len(b), np.int32)
+    m = lib.orc_hydrate_radial(_p(a), _p(f), len(a), _p(b), len(b), _p(nz), _p(out), cap, _p(per))
+    if m < 0:
+        raise RuntimeError("hydrate_radial capacity")
+    return out[:m].copy(), per
+
+
 def hist_ffgrid_surface(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1):
     """HistogramManagerMTFFGridSurface::calculate_all from FP64 coordinates (see orc_hist_ffgrid_surface)."""
     w = np.zeros((0, 3)) if waters_xyz is None else np.asarray(waters_xyz, np.float64)

@@ -387,20 +387,18 @@ static double orc_vdw(int t) {
     }
 }
 
-/* RadialLineGenerator::generate (RadialLineGenerator.cpp:15-63) at width 1
- * with the radii of collision_check (RawGridWithSurfaceExv.cpp:15-16):
- * 8 divisions of one octant reflected to the full sphere, duplicates (points
- * within 1e-5, components below 1e-5 set to 0) removed in first-seen order;
- * off[d][r] = bin offset of direction d at radius r. Returns the count. */
-static int radial_offsets(int off[][3][3], int cap) {
+/* One octant of a sphere in 8 angular divisions, reflected to the full sphere
+ * in the order +++, -++, +-+, --+, ++-, -+-, +--, ---, components below 1e-5
+ * set to 0 and points within 1e-5 of an earlier one dropped
+ * (RadialLineGenerator::generate, RadialLineGenerator.cpp:15-63, and
+ * RadialHydration::prepare_rotations, RadialHydration.cpp:82-139). Returns
+ * the count (26). */
+static int sphere_dirs(double dirs[][3], int cap) {
     const double pi = 3.14159265358979323846, ang = 2 * pi / 8;
-    const double radius[3] = {sqrt(1.0) + 1e-3, 2.0, 3.0};
-    double dirs[256][3];
     int nd = 0;
     for (double theta = 0; theta <= pi * 0.5; theta += ang)
         for (double phi = 0; phi <= pi * 0.5; phi += ang) {
             const double base[3] = {cos(phi) * sin(theta), sin(phi) * sin(theta), cos(theta)};
-            /* reflections in the order +++, -++, +-+, --+, ++-, -+-, +--, --- */
             for (int r = 0; r < 8; ++r) {
                 double p[3] = {(r & 1) ? -base[0] : base[0], (r & 2) ? -base[1] : base[1],
                                (r & 4) ? -base[2] : base[2]};
@@ -411,22 +409,36 @@ static int radial_offsets(int off[][3][3], int cap) {
                     double dx = dirs[e][0] - p[0], dy = dirs[e][1] - p[1], dz = dirs[e][2] - p[2];
                     dup = sqrt(dx * dx + dy * dy + dz * dz) < 1e-5;
                 }
-                if (!dup && nd < 256) memcpy(dirs[nd++], p, sizeof p);
+                if (!dup) {
+                    if (nd >= cap) return -1;
+                    memcpy(dirs[nd++], p, sizeof p);
+                }
             }
         }
-    if (nd > cap) return -1;
+    return nd;
+}
+
+/* The bin offsets of the collision radii of collision_check
+ * (RawGridWithSurfaceExv.cpp:15-16) at width 1: off[d][r] for direction d at
+ * radius r. Returns the count. */
+static int radial_offsets(int off[][3][3], int cap) {
+    const double radius[3] = {sqrt(1.0) + 1e-3, 2.0, 3.0};
+    double dirs[64][3];
+    int nd = sphere_dirs(dirs, 64);
+    if (nd < 0 || nd > cap) return -1;
     for (int d = 0; d < nd; ++d)
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) off[d][r][c] = (int)(round(radius[r] * dirs[d][c]) * 1.0);
     return nd;
 }
 
-/* rvol: settings::grid::min_exv_radius; ra < 0: the van der Waals radius of
- * each atom's type, else this radius for every atom (the reference's tests
- * override Grid::get_atomic_radius this way) */
-static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
-                         double* out_xyz, int cap, double* surf_xyz, int* n_surf, double* volume, double rvol,
-                         double ra) {
+/* Grid(bodies) (Grid.cpp:34-110, to_bins :414-419) with Grid::add (:300-352)
+ * marking atom and water centres, then SphericalExpander<true,false>::
+ * expand_volume for the atoms (SphericalExpander.h:29-90); rvol and ra as for
+ * grid_exv_core. Returns the state grid (calloc'd) with its minimum, bin
+ * counts, the atoms' bins (malloc'd) and Grid::get_volume's count. */
+static uint8_t* grid_build(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double rvol,
+                           double ra, double gmin[3], int bins[3], int** loc_out, long* vol_out) {
     const double width = 1.0;
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i)
@@ -442,8 +454,6 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
     double scaling = 0.25;
     if (mx[0] - mn[0] < 50 || mx[1] - mn[1] < 50 || mx[2] - mn[2] < 50) scaling = 1;
     const double margin = 2.14 + 1.71 + (-0.35) + 3 * width; /* vdw P + vdw O + shell_correction + 3 cells */
-    double gmin[3];
-    int bins[3];
     for (int k = 0; k < 3; ++k) {
         double diff = mx[k] - mn[k];
         double expand = 0.5 * diff * scaling > margin ? 0.5 * diff * scaling : margin;
@@ -500,6 +510,21 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
             }
         }
     }
+#undef G
+    *loc_out = loc;
+    *vol_out = vol;
+    return g;
+}
+
+static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
+                         double* out_xyz, int cap, double* surf_xyz, int* n_surf, double* volume, double rvol,
+                         double ra) {
+    const double width = 1.0;
+    double gmin[3];
+    int bins[3], *loc = NULL;
+    long vol = 0;
+    uint8_t* g = grid_build(atoms, ff, n, waters, m, rvol, ra, gmin, bins, &loc, &vol);
+#define G(i, j, k) g[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
     /* RawGridExv::create over the atoms' bounding_box_index +- buffer */
     int imin[3] = {1 << 30, 1 << 30, 1 << 30}, imax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
     for (int a = 0; a < n; ++a)
@@ -567,6 +592,104 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
     free(loc);
     if (volume) *volume = vol * width * width * width;
     if (n_surf) *n_surf = ns;
+    return ok ? count : -1;
+}
+
+/* RadialHydration::generate_explicit_hydration (RadialHydration.cpp:40-80)
+ * per body on the molecule's grid (GridBasedHydration::hydrate,
+ * GridBasedHydration.cpp:36-79, NoCulling), with collision_check (:141-206). */
+int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
+                       const double* noise, double* out, int cap, int* per_body) {
+    if (n <= 0) return 0;
+    double gmin[3];
+    int bins[3], *loc = NULL;
+    long vol = 0;
+    uint8_t* g = grid_build(atoms, ff, n, NULL, 0, 2.15, -1, gmin, bins, &loc, &vol);
+#define G(i, j, k) g[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
+#define OOB(a, b, c) ((a) < 0 || (a) >= bins[0] || (b) < 0 || (b) >= bins[1] || (c) < 0 || (c) >= bins[2])
+#define ONLY_EMPTY_OR_VOLUME(v) ((v) == G_EMPTY || (v) == G_VOLUME)
+    double dirs[64][3];
+    const int nd = sphere_dirs(dirs, 64);
+    const double rh = 1.71 + (-0.35); /* vdw O (Grid::get_hydration_radius) + shell_correction */
+    int p1[64][3], p3[64][3], p5[64][3];
+    for (int d = 0; d < nd; ++d)
+        for (int c = 0; c < 3; ++c) {
+            p1[d][c] = (int)(round(rh * dirs[d][c]) / 1.0);
+            p3[d][c] = (int)(round(3 * rh * dirs[d][c]) / 1.0);
+            p5[d][c] = (int)(round(5 * rh * dirs[d][c]) / 1.0);
+        }
+    int count = 0, ok = 1, one_body = n;
+    if (n_bodies <= 0) body_sizes = &one_body, n_bodies = 1;
+    int a = 0;
+    for (int b = 0; b < n_bodies; ++b) {
+        int placed = 0;
+        for (int e = 0; e < body_sizes[b]; ++e, ++a) {
+            const double reff = orc_vdw(ff[a]) + rh;
+            for (int d = 0; d < nd; ++d) {
+                double w[3];
+                int wb[3];
+                for (int c = 0; c < 3; ++c) {
+                    double nz = noise ? noise[((size_t)a * nd + d) * 3 + c] : 0.0;
+                    w[c] = atoms[3 * a + c] + dirs[d][c] * reff + nz;
+                    wb[c] = (int)round((w[c] - gmin[c]) / 1.0);
+                }
+                if (OOB(wb[0], wb[1], wb[2])) continue;
+                if (!ONLY_EMPTY_OR_VOLUME(G(wb[0], wb[1], wb[2]))) continue;
+                /* collision_check */
+                int score = 0, inside = 0, reject = 0;
+                for (int d2 = 0; d2 < nd && !reject; ++d2) {
+                    int x = wb[0] + p1[d2][0], y = wb[1] + p1[d2][1], z = wb[2] + p1[d2][2];
+                    x = x < 0 ? 0 : x >= bins[0] ? bins[0] - 1 : x;
+                    y = y < 0 ? 0 : y >= bins[1] ? bins[1] - 1 : y;
+                    z = z < 0 ? 0 : z >= bins[2] ? bins[2] - 1 : z;
+                    if (!ONLY_EMPTY_OR_VOLUME(G(x, y, z))) {
+                        if (++inside > 2) reject = 1;
+                        continue;
+                    }
+                    x = wb[0] + p3[d2][0], y = wb[1] + p3[d2][1], z = wb[2] + p3[d2][2];
+                    if (OOB(x, y, z)) {
+                        score += 2;
+                        continue;
+                    }
+                    if (!ONLY_EMPTY_OR_VOLUME(G(x, y, z))) {
+                        score -= 2;
+                        continue;
+                    }
+                    score += 2;
+                    x = wb[0] + p5[d2][0], y = wb[1] + p5[d2][1], z = wb[2] + p5[d2][2];
+                    if (OOB(x, y, z)) {
+                        score += 1;
+                        continue;
+                    }
+                    if (!ONLY_EMPTY_OR_VOLUME(G(x, y, z))) score -= 1;
+                }
+                if (reject || score <= 0.25 * (3.0 * nd)) continue;
+                /* Grid::add(water, true): centre, then W_AREA on empty voxels
+                 * within the hydration radius of the water (SphericalExpander.h:92-140) */
+                G(wb[0], wb[1], wb[2]) |= G_WCENTER;
+                const double rw2 = pow(1.71, 2);
+                const int br = (int)ceil(1.71 / 1.0);
+                for (int i = wb[0] - br > 0 ? wb[0] - br : 0; i < (wb[0] + br + 1 < bins[0] ? wb[0] + br + 1 : bins[0]); ++i) {
+                    double x2 = pow(gmin[0] + i - w[0], 2);
+                    for (int j = wb[1] - br > 0 ? wb[1] - br : 0; j < (wb[1] + br + 1 < bins[1] ? wb[1] + br + 1 : bins[1]); ++j) {
+                        double x2y2 = x2 + pow(gmin[1] + j - w[1], 2);
+                        for (int k = wb[2] - br > 0 ? wb[2] - br : 0; k < (wb[2] + br + 1 < bins[2] ? wb[2] + br + 1 : bins[2]); ++k) {
+                            double dist = x2y2 + pow(gmin[2] + k - w[2], 2);
+                            if (dist <= rw2 && G(i, j, k) == G_EMPTY) G(i, j, k) |= G_WAREA;
+                        }
+                    }
+                }
+                if (count < cap) memcpy(out + 3 * (size_t)count, w, sizeof w);
+                else ok = 0;
+                ++count, ++placed;
+            }
+        }
+        if (per_body) per_body[b] = placed;
+    }
+#undef G
+#undef OOB
+#undef ONLY_EMPTY_OR_VOLUME
+    free(g), free(loc);
     return ok ? count : -1;
 }
 

@@ -69,6 +69,15 @@ int orc_hist_ff(int mode, const float* atoms, int n_a, const float* waters, int 
 int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
                  int cap, double* volume);
 
+/* RadialHydration with NoCulling (RadialHydration.cpp:40-206,
+ * GridBasedHydration.cpp:36-79): the molecule's grid from the atoms, expanded,
+ * then per body per atom per direction a candidate at vdw + 1.36 A (+ noise
+ * [n][26][3], NULL for none), accepted when its voxel is empty or volume only
+ * and collision_check passes, each acceptance marking the grid. out: cap x 3.
+ * Returns the water count (-1 if cap is too small); per_body (optional). */
+int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
+                       const double* noise, double* out, int cap, int* per_body);
+
 /* RawGridWithSurfaceExv::create with surface detection at unit cell width
  * (RawGridWithSurfaceExv.cpp:82-127): the dummies of orc_grid_exv split by
  * collision_check (:14-80) over the radial lines of RadialLineGenerator
