@@ -129,28 +129,32 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // Pair histograms of integer-lattice point sets by FFT correlation
 // (lattice.hip). Grid index (i, j, k) holds the point origin + (i, j, k);
-// L: power-of-two lengths (x slowest, z fastest), each at most kLatticeMaxL.
+// L: power-of-two lengths (x slowest, z fastest), each in [2, kLatticeMaxL];
+// E: extent of the occupied box (points at indices [0, E) per axis),
+// 2E - 1 <= L so that no displacement wraps.
 constexpr int kLatticeMaxL = 1024;
 struct LatticeGeom {
     int origin[3];
     int L[3];
+    int E[3];
 };
 hipError_t launch_lattice_zero(double2* buf, size_t n, hipStream_t s);
 hipError_t launch_lattice_zero_u64(unsigned long long* buf, size_t n, hipStream_t s);
-// occupancy += 1 at every point (integer coordinates inside the grid)
+// occupancy += 1 at every point (integer coordinates inside the occupied box)
 hipError_t launch_lattice_fill(const float4* pts, int n, const LatticeGeom& g, double2* buf, hipStream_t s);
-// in-place 3D FFT, forward exp(-2 pi i k x / L) or inverse (unnormalised);
-// tw[a]: exp(-2 pi i k / L[a]) for k < L[a] / 2
-hipError_t launch_fft3(double2* buf, const LatticeGeom& g, const double2* const tw[3], bool inverse, hipStream_t s);
-// b == nullptr: a = |a|^2; else a = conj(a) * b
-hipError_t launch_lattice_spectrum(double2* a, const double2* b, size_t n, hipStream_t s);
-// inverse-transformed correlation -> pair counts per squared displacement
-// scount[s] (s <= smax); self: each pair once; flag |= 1 on a non-integer
-// count, 2 on s > smax
-hipError_t launch_lattice_shist(const double2* c, const LatticeGeom& g, bool self, long long n_self,
-                                unsigned long long* scount, int smax, int* flag, hipStream_t s);
-// scount -> counts and delta sums of one histogram row (SIMD-body rule),
-// added to cnt / dsum [nbins + 1] with atomics
+// in-place forward 3D FFT, exp(-2 pi i k x / L), of an occupancy grid;
+// tw[a]: exp(-2 pi i k / L[a]) for k < L[a] / 2. store: 0 the transform,
+// 1 its power spectrum |F|^2, 2 conj(F) * other (other: a finished transform)
+hipError_t launch_fft3_forward(double2* buf, const LatticeGeom& g, const double2* const tw[3], int store,
+                               const double2* other, hipStream_t s);
+// inverse 3D FFT of a (power or cross) spectrum into pair counts per squared
+// displacement, scount[s] (s <= smax), for displacements |D| < E per axis;
+// self: each pair once, n_self points; flag |= 1 on a non-integer count,
+// 2 on s > smax. The buffer is left partly transformed.
+hipError_t launch_fft3_counts(double2* buf, const LatticeGeom& g, const double2* const tw[3], bool self,
+                              long long n_self, unsigned long long* scount, int smax, int* flag, hipStream_t s);
+// scount -> counts and delta sums of one histogram row (the pair kernel's
+// binning), added to cnt / dsum [nbins + 1] with atomics
 hipError_t launch_lattice_bins(const unsigned long long* scount, int smax, float inv_w, int nbins, uint64_t* cnt,
                                int64_t* dsum, hipStream_t s);
 

@@ -300,11 +300,12 @@ bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mod
     int64_t cells = 1, s_true = 0;
     for (int k = 0; k < 3; ++k) {
         const int64_t E = (int64_t)hi[k] - lo[k] + 1;
-        int64_t L = 1;
+        int64_t L = 2;
         while (L < 2 * E - 1) L *= 2;
         if (L > kLatticeMaxL) return false;
         j.g.origin[k] = lo[k];
         j.g.L[k] = (int)L;
+        j.g.E[k] = (int)E;
         cells *= L;
         s_true += (E - 1) * (E - 1);
         j.smax += (int)((L / 2) * (L / 2));
@@ -476,19 +477,19 @@ void PairEngine::enqueue(bool timed) {
         const size_t cells = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
         const double2* tw[3] = {d_tw_.p + j.tw_off[0], d_tw_.p + j.tw_off[1], d_tw_.p + j.tw_off[2]};
         const bool self = j.col_set < 0;
-        check(launch_lattice_zero(d_lat_a_.p, cells, s), "lattice_zero_kernel");
-        check(launch_lattice_fill(d_orig_[j.row_set].p, j.n_rows, j.g, d_lat_a_.p, s), "lattice_fill_kernel");
-        check(launch_fft3(d_lat_a_.p, j.g, tw, false, s), "fft_lines_kernel");
         if (!self) {
+            // the column set's transform first: the row set's last pass multiplies by it
             check(launch_lattice_zero(d_lat_b_.p, cells, s), "lattice_zero_kernel");
             check(launch_lattice_fill(d_orig_[j.col_set].p, j.n_cols, j.g, d_lat_b_.p, s), "lattice_fill_kernel");
-            check(launch_fft3(d_lat_b_.p, j.g, tw, false, s), "fft_lines_kernel");
+            check(launch_fft3_forward(d_lat_b_.p, j.g, tw, 0, nullptr, s), "fft_lines_kernel");
         }
-        check(launch_lattice_spectrum(d_lat_a_.p, self ? nullptr : d_lat_b_.p, cells, s), "lattice_spectrum_kernel");
-        check(launch_fft3(d_lat_a_.p, j.g, tw, true, s), "fft_lines_kernel");
+        check(launch_lattice_zero(d_lat_a_.p, cells, s), "lattice_zero_kernel");
+        check(launch_lattice_fill(d_orig_[j.row_set].p, j.n_rows, j.g, d_lat_a_.p, s), "lattice_fill_kernel");
+        check(launch_fft3_forward(d_lat_a_.p, j.g, tw, self ? 1 : 2, self ? nullptr : d_lat_b_.p, s),
+              "fft_lines_kernel");
         check(launch_lattice_zero_u64(d_scount_.p, (size_t)j.smax + 1, s), "lattice_zero_u64_kernel");
-        check(launch_lattice_shist(d_lat_a_.p, j.g, self, self ? j.n_rows : 0, d_scount_.p, j.smax, d_lat_flag_.p, s),
-              "lattice_shist_kernel");
+        check(launch_fft3_counts(d_lat_a_.p, j.g, tw, self, self ? j.n_rows : 0, d_scount_.p, j.smax, d_lat_flag_.p, s),
+              "fft_lines_kernel");
         const size_t row = (size_t)j.out_row * (nbins_ + 1);
         check(launch_lattice_bins(d_scount_.p, j.smax, inv_w_, nbins_, d_cnt_.p + row, d_dsum_.p + row, s),
               "lattice_bins_kernel");

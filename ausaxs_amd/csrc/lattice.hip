@@ -11,14 +11,18 @@
 // IFFT(|FFT(occ)|^2) in FP64 on a grid padded to at least 2E - 1 per axis
 // (E = extent), so that no displacement wraps. Its entries are pair counts;
 // the FP64 transforms leave them within a small multiple of 2^-52 * n * log L
-// of those integers, far below 1/2 for the sizes allowed, and the s-histogram
-// kernel checks it (|A - rint(A)| <= 0.25, else a flag the host turns into an
+// of those integers, far below 1/2 for the sizes allowed, and the count pass
+// checks it (|A - rint(A)| <= 0.25, else a flag the host turns into an
 // error). Counts and delta sums come out identical to evaluating every pair.
 //
 // FFT: radix-2 decimation in time on lines of length L (a power of two, at
 // most 1024) held in LDS, G lines per workgroup (G * L <= 2048 complex values,
 // 32 KB, so that several workgroups share a CU); loads run along the lines'
-// common contiguous direction.
+// common contiguous direction. Lines known to be zero (forward: outside the
+// occupied box) or not needed (inverse: leading only to displacements beyond
+// the extents) are skipped; the forward transform's last pass stores the
+// power (or cross) spectrum and the inverse transform's last pass turns its
+// entries into pair counts, so neither needs a pass of its own.
 #include "binning.hpp"
 #include "device_api.hpp"
 
@@ -52,33 +56,65 @@ __global__ __launch_bounds__(256) void lattice_fill_kernel(const float4* __restr
     atomicAdd(&buf[idx].x, 1.0);
 }
 
-// One FFT pass along an axis: lines of L elements at element stride S; line l
-// starts at (l / S) * S * L + (l % S). tw[k] = exp(-2 pi i k / L), k < L / 2.
-__global__ __launch_bounds__(256) void fft_lines_kernel(double2* __restrict__ buf, int L, int logL, long long S,
-                                                        long long nlines, int G,
-                                                        const double2* __restrict__ tw, int inverse) {
+// One FFT pass over the lines of one axis and what its store does.
+struct FftPass {
+    double2* buf;
+    const double2* tw;     // exp(-2 pi i k / L), k < L / 2
+    const double2* other;  // store 2: the other transform
+    int L, logL, G;        // line length, log2 L, lines per workgroup
+    long long S, nlines;   // element stride; line l starts at (l / S) * S * L + (l % S)
+    int inverse;
+    // a line is computed iff keep(p, pe, pL) and keep(q, qe, qL), with
+    // outer = l / S, p = outer / qdim, q = outer % qdim and
+    // keep(v, e, Lv) = v < e or (Lv > 0 and v > Lv - e)
+    int qdim, pe, pL, qe, qL;
+    int store;  // 0: the transform; 1: |F|^2; 2: conj(F) * other; 3: pair counts
+    // store 3 (the z pass): line l = x * Ly + y, element k = z
+    int Lx, Ly, Ez;
+    int self;
+    long long n_self;
+    double scale;
+    unsigned long long* scount;
+    int smax;
+    int* flag;
+};
+
+__device__ __forceinline__ bool keep_coord(int v, int e, int Lv) { return v < e || (Lv > 0 && v > Lv - e); }
+
+__global__ __launch_bounds__(256) void fft_lines_kernel(FftPass p) {
     extern __shared__ double2 sh[];  // [G][L]
+    const int L = p.L, G = p.G;
+    const long long S = p.S;
     const long long l0 = (long long)blockIdx.x * G;
+    auto kept = [&](long long l) {
+        if (l >= p.nlines) return false;
+        const long long outer = l / S;
+        const int pc = (int)(outer / p.qdim), qc = (int)(outer % p.qdim);
+        return keep_coord(pc, p.pe, p.pL) && keep_coord(qc, p.qe, p.qL);
+    };
+    bool any = false;
+    for (int g = 0; g < G; ++g) any = any || kept(l0 + g);
+    if (!any) return;  // the same for every thread of the workgroup, before any barrier
     const int total = G * L;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         const int g = S == 1 ? e / L : e % G;
         const int k = S == 1 ? e % L : e / G;
         const long long l = l0 + g;
-        if (l >= nlines) continue;
+        if (!kept(l)) continue;
         const long long base = (l / S) * S * L + (l % S);
-        const int r = (int)(__brev((unsigned)k) >> (32 - logL));
-        sh[g * L + r] = buf[base + (long long)k * S];
+        const int r = (int)(__brev((unsigned)k) >> (32 - p.logL));
+        sh[g * L + r] = p.buf[base + (long long)k * S];
     }
     __syncthreads();
     const int half = L / 2;
     for (int h = 1; h < L; h <<= 1) {
         const int tstride = L / (2 * h);
-        for (int p = threadIdx.x; p < G * half; p += blockDim.x) {
-            const int g = p / half, q = p % half;
+        for (int t = threadIdx.x; t < G * half; t += blockDim.x) {
+            const int g = t / half, q = t % half;
             const int j = q & (h - 1);
             const int i0 = g * L + (q / h) * 2 * h + j, i1 = i0 + h;
-            double2 w = tw[j * tstride];
-            if (inverse) w.y = -w.y;
+            double2 w = p.tw[j * tstride];
+            if (p.inverse) w.y = -w.y;
             const double2 a = sh[i0], b = sh[i1];
             const double2 bw = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
             sh[i0] = make_double2(a.x + bw.x, a.y + bw.y);
@@ -90,65 +126,48 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(double2* __restrict__ bu
         const int g = S == 1 ? e / L : e % G;
         const int k = S == 1 ? e % L : e / G;
         const long long l = l0 + g;
-        if (l >= nlines) continue;
-        const long long base = (l / S) * S * L + (l % S);
-        buf[base + (long long)k * S] = sh[g * L + k];
-    }
-}
-
-// self: a = |a|^2; cross: a = conj(a) * b (the transform of sum_v a(v) b(v + D))
-__global__ __launch_bounds__(256) void lattice_spectrum_kernel(double2* __restrict__ a, const double2* __restrict__ b,
-                                                               size_t n) {
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const double2 x = a[e];
-        if (b) {
-            const double2 y = b[e];
-            a[e] = make_double2(x.x * y.x + x.y * y.y, x.x * y.y - x.y * y.x);
+        if (!kept(l)) continue;
+        const long long idx = (l / S) * S * L + (l % S) + (long long)k * S;
+        const double2 v = sh[g * L + k];
+        if (p.store == 0) {
+            p.buf[idx] = v;
+        } else if (p.store == 1) {
+            p.buf[idx] = make_double2(v.x * v.x + v.y * v.y, 0.0);
+        } else if (p.store == 2) {
+            // the transform of sum_v a(v) b(v + D): conj(F_a) * F_b
+            const double2 o = p.other[idx];
+            p.buf[idx] = make_double2(v.x * o.x + v.y * o.y, v.x * o.y - v.y * o.x);
         } else {
-            a[e] = make_double2(x.x * x.x + x.y * x.y, 0.0);
-        }
-    }
-}
-
-// Correlation entries -> pair counts per squared displacement s. Self jobs
-// keep one of D and -D (the pair once) and, at D = 0, the pairs of coincident
-// points, (A(0) - n) / 2.
-__global__ __launch_bounds__(256) void lattice_shist_kernel(const double2* __restrict__ c, LatticeGeom g, double scale,
-                                                            int self, long long n_self,
-                                                            unsigned long long* __restrict__ scount, int smax,
-                                                            int* __restrict__ flag) {
-    const size_t total = (size_t)g.L[0] * g.L[1] * g.L[2];
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const double v = c[idx].x * scale;
-        const double r = rint(v);
-        if (fabs(v - r) > 0.25 || r < 0.0) {
-            atomicOr(flag, 1);
-            continue;
-        }
-        long long cnt = (long long)r;
-        if (cnt == 0) continue;
-        const int k = (int)(idx % g.L[2]);
-        const size_t t = idx / g.L[2];
-        const int j = (int)(t % g.L[1]);
-        const int i = (int)(t / g.L[1]);
-        const int dx = i <= g.L[0] / 2 ? i : i - g.L[0];
-        const int dy = j <= g.L[1] / 2 ? j : j - g.L[1];
-        const int dz = k <= g.L[2] / 2 ? k : k - g.L[2];
-        if (self) {
-            if (dx == 0 && dy == 0 && dz == 0) {
-                cnt = (cnt - n_self) / 2;
-                if (cnt <= 0) continue;
-            } else if (!(dx > 0 || (dx == 0 && (dy > 0 || (dy == 0 && dz > 0))))) {
+            if (!keep_coord(k, p.Ez, L)) continue;
+            const double c = v.x * p.scale;
+            const double r = rint(c);
+            if (fabs(c - r) > 0.25 || r < 0.0) {
+                atomicOr(p.flag, 1);
                 continue;
             }
+            long long cnt = (long long)r;
+            if (cnt == 0) continue;
+            const int xi = (int)(l / p.Ly), yi = (int)(l % p.Ly);
+            const int dx = xi <= p.Lx / 2 ? xi : xi - p.Lx;
+            const int dy = yi <= p.Ly / 2 ? yi : yi - p.Ly;
+            const int dz = k <= L / 2 ? k : k - L;
+            if (p.self) {
+                // one of D and -D (the pair once); at D = 0 the pairs of
+                // coincident points, (A(0) - n) / 2
+                if (dx == 0 && dy == 0 && dz == 0) {
+                    cnt = (cnt - p.n_self) / 2;
+                    if (cnt <= 0) continue;
+                } else if (!(dx > 0 || (dx == 0 && (dy > 0 || (dy == 0 && dz > 0))))) {
+                    continue;
+                }
+            }
+            const int s2 = dx * dx + dy * dy + dz * dz;
+            if (s2 > p.smax) {
+                atomicOr(p.flag, 2);
+                continue;
+            }
+            atomicAdd(&p.scount[s2], (unsigned long long)cnt);
         }
-        const int s = dx * dx + dy * dy + dz * dz;
-        if (s > smax) {
-            atomicOr(flag, 2);
-            continue;
-        }
-        atomicAdd(&scount[s], (unsigned long long)cnt);
     }
 }
 
@@ -171,6 +190,30 @@ __global__ __launch_bounds__(256) void lattice_bins_kernel(const unsigned long l
 
 unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 65536); }
 
+// one pass along axis a of a grid with element stride S on that axis
+hipError_t fft_pass(FftPass p, const LatticeGeom& g, int a, long long S, hipStream_t s) {
+    const long long total = (long long)g.L[0] * g.L[1] * g.L[2];
+    const int L = g.L[a];
+    if (L < 2 || L > kLatticeMaxL || (L & (L - 1))) return hipErrorInvalidValue;
+    p.L = L;
+    p.logL = 0;
+    while ((1 << p.logL) < L) ++p.logL;
+    p.G = std::max(1, std::min(16, 2048 / L));
+    p.S = S;
+    p.nlines = total / L;
+    const unsigned blocks = (unsigned)((p.nlines + p.G - 1) / p.G);
+    hipLaunchKernelGGL(fft_lines_kernel, dim3(blocks), dim3(256), (size_t)p.G * L * sizeof(double2), s, p);
+    return hipGetLastError();
+}
+
+FftPass keep_all(double2* buf, const double2* tw) {
+    FftPass p{};
+    p.buf = buf;
+    p.tw = tw;
+    p.qdim = 1, p.pe = 1 << 30, p.pL = 0, p.qe = 1, p.qL = 0;
+    return p;
+}
+
 }  // namespace
 
 hipError_t launch_lattice_zero(double2* buf, size_t n, hipStream_t s) {
@@ -191,40 +234,54 @@ hipError_t launch_lattice_fill(const float4* pts, int n, const LatticeGeom& g, d
     return hipGetLastError();
 }
 
-hipError_t launch_fft3(double2* buf, const LatticeGeom& g, const double2* const tw[3], bool inverse, hipStream_t s) {
-    const long long total = (long long)g.L[0] * g.L[1] * g.L[2];
-    long long S = 1;
-    for (int a = 2; a >= 0; --a) {
-        const int L = g.L[a];
-        if (L > 1) {
-            if (L > kLatticeMaxL || (L & (L - 1))) return hipErrorInvalidValue;
-            int logL = 0;
-            while ((1 << logL) < L) ++logL;
-            const int G = std::max(1, std::min(16, 2048 / L));
-            const long long nlines = total / L;
-            const unsigned blocks = (unsigned)((nlines + G - 1) / G);
-            hipLaunchKernelGGL(fft_lines_kernel, dim3(blocks), dim3(256), (size_t)G * L * sizeof(double2), s, buf, L,
-                               logL, S, nlines, G, tw[a], inverse ? 1 : 0);
-            if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-        }
-        S *= L;
-    }
-    return hipSuccess;
+hipError_t launch_fft3_forward(double2* buf, const LatticeGeom& g, const double2* const tw[3], int store,
+                               const double2* other, hipStream_t s) {
+    for (int a = 0; a < 3; ++a)
+        if (g.L[a] < 2) return hipErrorInvalidValue;
+    if (store < 0 || store > 2 || (store == 2 && !other)) return hipErrorInvalidValue;
+    const long long S[3] = {(long long)g.L[1] * g.L[2], g.L[2], 1};
+    // z pass: the occupancy is zero outside x < Ex, y < Ey
+    FftPass pz = keep_all(buf, tw[2]);
+    pz.qdim = g.L[1], pz.pe = g.E[0], pz.qe = g.E[1];
+    if (hipError_t e = fft_pass(pz, g, 2, S[2], s); e != hipSuccess) return e;
+    // y pass: zero outside x < Ex
+    FftPass py = keep_all(buf, tw[1]);
+    py.pe = g.E[0];
+    if (hipError_t e = fft_pass(py, g, 1, S[1], s); e != hipSuccess) return e;
+    // x pass: every line, with the requested store
+    FftPass px = keep_all(buf, tw[0]);
+    px.store = store;
+    px.other = other;
+    return fft_pass(px, g, 0, S[0], s);
 }
 
-hipError_t launch_lattice_spectrum(double2* a, const double2* b, size_t n, hipStream_t s) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(lattice_spectrum_kernel, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_lattice_shist(const double2* c, const LatticeGeom& g, bool self, long long n_self,
-                                unsigned long long* scount, int smax, int* flag, hipStream_t s) {
-    const size_t total = (size_t)g.L[0] * g.L[1] * g.L[2];
-    const double scale = 1.0 / (double)total;
-    hipLaunchKernelGGL(lattice_shist_kernel, dim3(grid_for(total)), dim3(256), 0, s, c, g, scale, self ? 1 : 0,
-                       n_self, scount, smax, flag);
-    return hipGetLastError();
+hipError_t launch_fft3_counts(double2* buf, const LatticeGeom& g, const double2* const tw[3], bool self,
+                              long long n_self, unsigned long long* scount, int smax, int* flag, hipStream_t s) {
+    for (int a = 0; a < 3; ++a)
+        if (g.L[a] < 2) return hipErrorInvalidValue;
+    const long long S[3] = {(long long)g.L[1] * g.L[2], g.L[2], 1};
+    // x pass: every line
+    FftPass px = keep_all(buf, tw[0]);
+    px.inverse = 1;
+    if (hipError_t e = fft_pass(px, g, 0, S[0], s); e != hipSuccess) return e;
+    // y pass: the lines whose x leads to |Dx| < Ex
+    FftPass py = keep_all(buf, tw[1]);
+    py.inverse = 1;
+    py.pe = g.E[0], py.pL = g.L[0];
+    if (hipError_t e = fft_pass(py, g, 1, S[1], s); e != hipSuccess) return e;
+    // z pass: the lines with |Dx| < Ex and |Dy| < Ey, entries with |Dz| < Ez to counts
+    FftPass pz = keep_all(buf, tw[2]);
+    pz.inverse = 1;
+    pz.qdim = g.L[1], pz.pe = g.E[0], pz.pL = g.L[0], pz.qe = g.E[1], pz.qL = g.L[1];
+    pz.store = 3;
+    pz.Lx = g.L[0], pz.Ly = g.L[1], pz.Ez = g.E[2];
+    pz.self = self ? 1 : 0;
+    pz.n_self = n_self;
+    pz.scale = 1.0 / ((double)g.L[0] * g.L[1] * g.L[2]);
+    pz.scount = scount;
+    pz.smax = smax;
+    pz.flag = flag;
+    return fft_pass(pz, g, 2, S[2], s);
 }
 
 hipError_t launch_lattice_bins(const unsigned long long* scount, int smax, float inv_w, int nbins, uint64_t* cnt,
