@@ -235,7 +235,6 @@ class PairEngine {
     // workgroups of the pair kernel: 0 = one per item, else a persistent grid
     void set_grid(int g) { grid_ = g; }
     bool clamped() const { return clamp_; }
-    int copies() const { return copies_; }
 
   private:
     std::vector<PointSet> sets_;

@@ -209,7 +209,8 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 // The LDS holds `copies` histograms of `rows` rows each; lane l of a wave adds
 // into copy l mod copies, so lanes whose pairs fall into the same bin collide
 // on one LDS cell `copies` times less often (pairs of two bodies crowd into few
-// bins). The copies are summed at the flush. The kernel is held to 64 VGPRs
+// bins). An item with one output row in a two-row launch uses the same LDS as
+// 2 * copies one-row histograms. The copies are summed at the flush. The kernel is held to 64 VGPRs
 // (8 waves per SIMD): one register more costs a wave per SIMD and about 5% of
 // the pair rate.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
@@ -219,24 +220,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     extern __shared__ uint64_t hist[];
     __shared__ float4 cols[kColTile + kColGroup];
     const int stride = nbins + 2;
-    const int copy_stride = rows * stride;
     const int out_stride = nbins + 1;
-    uint64_t* mine = hist + (threadIdx.x & (copies - 1)) * copy_stride;
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const PairWork w = items[it];
         const bool two = (w.flags & (kOrient | kSplit)) != 0;
+        // this item's histograms: n_copies of item_rows rows each
+        const int item_rows = two ? 2 : 1;
+        const int n_copies = two ? copies : copies * rows;
+        const int copy_stride = item_rows * stride;
+        uint64_t* mine = hist + (threadIdx.x & (n_copies - 1)) * copy_stride;
         __syncthreads();  // previous item's flush done
-        for (int e = threadIdx.x; e < copies * copy_stride; e += kThreads) hist[e] = 0;
+        for (int e = threadIdx.x; e < copies * rows * stride; e += kThreads) hist[e] = 0;
         __syncthreads();
         if (clamp) run_any<true>(pts, w, mine, cols, inv_w, nbins);
         else run_any<false>(pts, w, mine, cols, inv_w, nbins);
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
         uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
-        for (int rr = 0; rr < (two ? 2 : 1); ++rr)
+        for (int rr = 0; rr < item_rows; ++rr)
             for (int e = threadIdx.x; e < out_stride; e += kThreads) {
                 uint64_t v = hist[rr * stride + e];
-                for (int c = 1; c < copies; ++c) v += hist[c * copy_stride + rr * stride + e];
+                for (int c = 1; c < n_copies; ++c) v += hist[c * copy_stride + rr * stride + e];
                 out[rr * out_stride + e] = v;
             }
     }
