@@ -71,7 +71,7 @@ int item_rows(const std::vector<PairWork>& items) {
 
 int pick_copies(int nbins, int rows) {
     int k = 8;
-    while (k > 1 && pair_lds_bytes(nbins, rows, k) > 40960) k /= 2;
+    while (k > 1 && pair_lds_bytes(nbins, rows, k) > 65536) k /= 2;
     return k;
 }
 

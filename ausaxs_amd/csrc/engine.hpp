@@ -191,7 +191,9 @@ void check_tile_j(int tile_j);
 // throws if the device offers less.
 size_t check_pair_lds(int nbins);
 // LDS histogram copies for a launch: the most (up to 8) that keep a
-// workgroup within 40 KB of LDS, so at least four fit on one CU
+// workgroup within 64 KB of LDS, so at least two fit on one CU (measured on
+// the 20k-atom grid workload: 40 KB gave 5.79 ms per pair launch, 64 KB 5.44 ms,
+// 96 KB 9.88 ms; the 250k-atom workload's single copy is unchanged)
 int pick_copies(int nbins, int rows);
 // 2 if any item has two output rows (orientation), else 1
 int item_rows(const std::vector<PairWork>& items);
