@@ -97,7 +97,7 @@ def _load():
         "ausaxs_server_wait": (C.c_int, [vp, C.c_longlong, C.c_double, vp, P(C.c_double), P(C.c_double),
                                          P(C.c_int)]),
         "ausaxs_server_stats": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong), P(C.c_longlong), P(C.c_int),
-                                          P(C.c_int), P(C.c_double), P(C.c_longlong)]),
+                                          P(C.c_int), P(C.c_double), P(C.c_longlong), P(C.c_longlong)]),
         "ausaxs_server_free": (C.c_int, [vp]),
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
@@ -745,11 +745,11 @@ class Server:
 
     def stats(self):
         v = [C.c_longlong() for _ in range(3)]
-        fl, pk, gb, pairs = C.c_int(), C.c_int(), C.c_double(), C.c_longlong()
+        fl, pk, gb, pairs, kp = C.c_int(), C.c_int(), C.c_double(), C.c_longlong(), C.c_longlong()
         _check(lib.ausaxs_server_stats(self._s, *map(C.byref, v), C.byref(fl), C.byref(pk), C.byref(gb),
-                                       C.byref(pairs)))
+                                       C.byref(pairs), C.byref(kp)))
         return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
-                    peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value)
+                    peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value, kernel_pairs=kp.value)
 
 
 def lsq(x, y, err=None):

@@ -780,7 +780,8 @@ int ausaxs_server_wait(ausaxs_server* s, long long id, double timeout_s, double*
 }
 
 int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* completed, long long* failed,
-                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs) {
+                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs,
+                        long long* kernel_pairs) {
     return guarded([&] {
         require(s && s->s, "bad argument");
         const ServeStats st = s->s->stats();
@@ -791,6 +792,7 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
         if (peak_in_flight) *peak_in_flight = st.peak_in_flight;
         if (peak_gb) *peak_gb = (double)st.peak_in_use / (1ull << 30);
         if (pairs) *pairs = st.pairs;
+        if (kernel_pairs) *kernel_pairs = st.kernel_pairs;
     });
 }
 

@@ -530,7 +530,7 @@ EngineTiming PairEngine::timing() {
     EngineTiming t;
     t.pairs = wl_.pairs;
     t.lattice_pairs = wl_.lattice_pairs;
-    t.kernel_pairs = wl_.pairs - wl_.lattice_pairs - wl_.strip_pairs;
+    t.kernel_pairs = kernel_pairs();
     t.n_items = (int)wl_.items.size();
     t.n_lattice = (int)lat_.size();
     if (have_timing_) {

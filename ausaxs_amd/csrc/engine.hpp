@@ -229,6 +229,8 @@ class PairEngine {
     int nbins() const { return nbins_; }
     float inv_w() const { return inv_w_; }
     int64_t pairs() const { return wl_.pairs; }
+    // pairs the pair kernel evaluates (without strips and lattice jobs)
+    int64_t kernel_pairs() const { return wl_.pairs - wl_.lattice_pairs - wl_.strip_pairs; }
     int n_lattice() const { return (int)lat_.size(); }
     int n_items() const { return (int)wl_.items.size(); }
     const uint64_t* device_cnt() const { return d_cnt_.p; }

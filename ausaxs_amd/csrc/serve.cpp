@@ -122,6 +122,7 @@ void Server::run(Request& r) {
     }
     auto h = st->calculate();
     r.pairs = st->engine().pairs();
+    r.kernel_pairs = st->engine().kernel_pairs();
     r.I = h->debye_transform(r.q);
 }
 
@@ -169,6 +170,7 @@ void Server::worker(int slot) {
             if (r->error.empty()) {
                 ++st_.completed;
                 st_.pairs += r->pairs;
+                st_.kernel_pairs += r->kernel_pairs;
             } else {
                 ++st_.failed;
             }

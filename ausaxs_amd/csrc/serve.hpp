@@ -33,7 +33,8 @@ struct ServeStats {
     int64_t submitted = 0, completed = 0, failed = 0;
     int in_flight = 0, peak_in_flight = 0;
     size_t in_use = 0, peak_in_use = 0, budget = 0;
-    int64_t pairs = 0;  // pair evaluations of completed requests
+    int64_t pairs = 0;         // pairs in the histograms of completed requests
+    int64_t kernel_pairs = 0;  // of which evaluated by the pair kernel
 };
 
 class Server {
@@ -71,7 +72,7 @@ class Server {
         clock::time_point t_submit, t_admit, t_done;
         std::vector<double> I;
         std::string error;
-        int64_t pairs = 0;
+        int64_t pairs = 0, kernel_pairs = 0;
         bool done = false;
     };
     std::mutex m_;

@@ -313,6 +313,7 @@ def bench_serve(cfg, args):
     gaps = rng.exponential(1.0 / lam, n_req)
     ids = [None] * n_req
     s0 = srv.stats()["pairs"]
+    k0 = srv.stats()["kernel_pairs"]
     t_start = time.perf_counter()
     t_next = t_start
 
@@ -338,10 +339,11 @@ def bench_serve(cfg, args):
     elapsed = time.perf_counter() - t_start
     st = srv.stats()
     pairs = st["pairs"] - s0
+    kernel_pairs = st["kernel_pairs"] - k0
     log(f"{n_req} requests in {elapsed:.2f} s; peak in flight {st['peak_in_flight']}, peak admitted "
         f"{st['peak_gb']:.1f} GiB; queue p50 {np.percentile(queue, 50):.1f} ms")
-    res = dict(elapsed=elapsed, lat_ms=np.array(lat), pairs=pairs / n_req, profiles_per_step=1, kernel_ms=None,
-               items=0, steps_override=n_req,
+    res = dict(elapsed=elapsed, lat_ms=np.array(lat), pairs=pairs / n_req, kernel_pairs=kernel_pairs / n_req,
+               profiles_per_step=1, kernel_ms=None, items=0, steps_override=n_req,
                extra={"requests": n_req, "arrival_rate_per_s": round(lam, 3), "service_rate_per_s": round(mu, 3),
                       "slots": cfg["slots"], "memory_budget_gb": cfg["budget_gb"],
                       "peak_in_flight": st["peak_in_flight"], "peak_admitted_gb": round(st["peak_gb"], 2),
@@ -449,7 +451,9 @@ def main():
                     "kernel_ms": round(res["kernel_ms"], 4), "kernel_pairs": kpairs,
                     "pairs_per_s": round(kpairs / kern_s, 1)}
     elif cfg["kind"] == "serve":
-        achieved = value * 1e9 * FLOP_PER_PAIR / 1e12
+        # pairs the pair kernel evaluated over the window (grid requests count
+        # their dummy-dummy pairs through lattice correlation, not here)
+        achieved = res["kernel_pairs"] * args.steps / elapsed * FLOP_PER_PAIR / 1e12
         roofline = {"bound": "mfma", "engine": "fp32 VALU; whole serving window at the offered load, idle included",
                     "kernel": "pair_hist_kernel (serving)", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None}

@@ -226,9 +226,12 @@ int ausaxs_server_submit(ausaxs_server* s, int mode, const double* atoms_xyz, co
 int ausaxs_server_wait(ausaxs_server* s, long long id, double timeout_s, double* I, double* latency_ms,
                        double* queue_ms, int* ready);
 
-/* Counters since creation; peak_gb = largest sum of admitted footprints. */
+/* Counters since creation; peak_gb = largest sum of admitted footprints;
+ * pairs: pairs in the histograms of completed requests, kernel_pairs: of
+ * which the pair kernel evaluated (grid dummy pairs go by lattice correlation). */
 int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* completed, long long* failed,
-                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs);
+                        int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs,
+                        long long* kernel_pairs);
 
 /* Stops the loop: running requests finish, queued ones fail. */
 int ausaxs_server_free(ausaxs_server* s);
