@@ -7,6 +7,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace ausaxs::gpu {
 
@@ -146,26 +147,48 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
         for (int k = 0; k < 3; ++k) b[k] = (int)std::round((waters[3 * w + k] - out.origin[k]) / kCell);
         if (inside(b[0], b[1], b[2])) at(b[0], b[1], b[2]) |= kWCenter;
     }
+    // The marking below has an order-free result, so it runs on slabs of x in
+    // parallel: each thread marks and later collects the voxels of its own
+    // slab (no two threads write one voxel) and the slabs are concatenated in
+    // x order, which keeps the lexicographic voxel order.
+    const int n_thr = (int)std::clamp<unsigned>(std::thread::hardware_concurrency(), 1u,
+                                                (unsigned)std::clamp(dim[0] / 8, 1, 16));
+    auto slab = [&](int t, int& x0, int& x1) {
+        x0 = bmin[0] + (int)((int64_t)dim[0] * t / n_thr);
+        x1 = bmin[0] + (int)((int64_t)dim[0] * (t + 1) / n_thr);  // exclusive
+    };
+    auto parallel = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < n_thr; ++t) pool.emplace_back(body, t);
+        body(0);
+        for (auto& th : pool) th.join();
+    };
     // atomic expansion (SphericalExpander.h:29-90): a voxel holding only a water
     // centre is never claimed by an atom
-    for (int a = 0; a < n; ++a) {
-        const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
-        const int br = (int)std::ceil(std::sqrt(r2[a]) / kCell - 1e-12);
-        const int bx = cen[3 * a], by = cen[3 * a + 1], bz = cen[3 * a + 2];
-        for (int i = bx - br; i <= bx + br; ++i) {
-            const double x2 = std::pow(out.origin[0] + i * kCell - ax, 2);
-            for (int j = by - br; j <= by + br; ++j) {
-                const double x2y2 = x2 + std::pow(out.origin[1] + j * kCell - ay, 2);
-                for (int k = bz - br; k <= bz + br; ++k) {
-                    const double dist = x2y2 + std::pow(out.origin[2] + k * kCell - az, 2);
-                    if (dist > r2[a]) continue;
-                    uint8_t& s = at(i, j, k);
-                    if ((s & kWCenter) && !(s & kACenter)) continue;
-                    s |= kExv;
+    parallel([&](int t) {
+        int x0, x1;
+        slab(t, x0, x1);
+        for (int a = 0; a < n; ++a) {
+            const int br = (int)std::ceil(std::sqrt(r2[a]) / kCell - 1e-12);
+            const int bx = cen[3 * a], by = cen[3 * a + 1], bz = cen[3 * a + 2];
+            const int i0 = std::max(bx - br, x0), i1 = std::min(bx + br, x1 - 1);
+            if (i0 > i1) continue;
+            const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+            for (int i = i0; i <= i1; ++i) {
+                const double x2 = std::pow(out.origin[0] + i * kCell - ax, 2);
+                for (int j = by - br; j <= by + br; ++j) {
+                    const double x2y2 = x2 + std::pow(out.origin[1] + j * kCell - ay, 2);
+                    for (int k = bz - br; k <= bz + br; ++k) {
+                        const double dist = x2y2 + std::pow(out.origin[2] + k * kCell - az, 2);
+                        if (dist > r2[a]) continue;
+                        uint8_t& s = at(i, j, k);
+                        if ((s & kWCenter) && !(s & kACenter)) continue;
+                        s |= kExv;
+                    }
                 }
             }
         }
-    }
+    });
     // GridObj::is_empty_or_water: nothing, or a water centre (water areas
     // only ever fill empty voxels); voxels outside the box hold no atom
     auto empty_or_water = [&](int i, int j, int k) {
@@ -210,16 +233,47 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
         }
         return score < 42;
     };
-    // RawGridExv::create / RawGridWithSurfaceExv::create: lexicographic voxel order
-    for (int i = bmin[0]; i <= bmax[0]; ++i)
-        for (int j = bmin[1]; j <= bmax[1]; ++j)
-            for (int k = bmin[2]; k <= bmax[2]; ++k)
-                if (at(i, j, k) & (kACenter | kExv)) {
-                    std::vector<double>& dst = !detect_surface || interior(i, j, k) ? out.xyz : out.surface;
-                    dst.push_back(out.origin[0] + i * kCell);
-                    dst.push_back(out.origin[1] + j * kCell);
-                    dst.push_back(out.origin[2] + k * kCell);
+    // RawGridExv::create / RawGridWithSurfaceExv::create: lexicographic voxel
+    // order. Pass 1 classifies each slab's dummies and counts them, pass 2
+    // writes them at their slab's offset into the output.
+    std::vector<uint8_t> cls(g.size(), 0);  // 1 interior, 2 surface
+    std::vector<int64_t> cnt_in(n_thr + 1, 0), cnt_sf(n_thr + 1, 0);
+    auto idx = [&](int i, int j, int k) { return ((size_t)(i - bmin[0]) * dim[1] + (j - bmin[1])) * dim[2] + (k - bmin[2]); };
+    parallel([&](int t) {
+        int x0, x1;
+        slab(t, x0, x1);
+        int64_t ci = 0, cs = 0;
+        for (int i = x0; i < x1; ++i)
+            for (int j = bmin[1]; j <= bmax[1]; ++j)
+                for (int k = bmin[2]; k <= bmax[2]; ++k)
+                    if (at(i, j, k) & (kACenter | kExv)) {
+                        const bool in = !detect_surface || interior(i, j, k);
+                        cls[idx(i, j, k)] = in ? 1 : 2;
+                        (in ? ci : cs) += 1;
+                    }
+        cnt_in[t + 1] = ci;
+        cnt_sf[t + 1] = cs;
+    });
+    for (int t = 0; t < n_thr; ++t) cnt_in[t + 1] += cnt_in[t], cnt_sf[t + 1] += cnt_sf[t];
+    out.xyz.resize(3 * (size_t)cnt_in[n_thr]);
+    out.surface.resize(3 * (size_t)cnt_sf[n_thr]);
+    parallel([&](int t) {
+        int x0, x1;
+        slab(t, x0, x1);
+        double* pi = out.xyz.data() + 3 * cnt_in[t];
+        double* ps = out.surface.data() + 3 * cnt_sf[t];
+        for (int i = x0; i < x1; ++i)
+            for (int j = bmin[1]; j <= bmax[1]; ++j)
+                for (int k = bmin[2]; k <= bmax[2]; ++k) {
+                    const uint8_t c = cls[idx(i, j, k)];
+                    if (!c) continue;
+                    double*& p = c == 1 ? pi : ps;
+                    p[0] = out.origin[0] + i * kCell;
+                    p[1] = out.origin[1] + j * kCell;
+                    p[2] = out.origin[2] + k * kCell;
+                    p += 3;
                 }
+    });
     out.volume = (double)((out.xyz.size() + out.surface.size()) / 3) * kCell * kCell * kCell;
     return out;
 }
