@@ -1,6 +1,8 @@
 // Host side of the pair-histogram engine (see engine.hpp).
 #include "engine.hpp"
 
+#include "host_parallel.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -105,6 +107,10 @@ SetLayout layout_set(const PointSet& ps, std::vector<float4>& buf, std::vector<i
         else if (i < t_col) mid[c].push_back(i);
         else tail[c].push_back(i);
     }
+    const size_t base = buf.size(), pbase = perm ? perm->size() : 0;
+    buf.resize(base + (size_t)n);
+    if (perm) perm->resize(pbase + (size_t)n);
+    size_t pos = base;
     for (int c = 0; c < C; ++c) {
         // Scatter the body part: consecutive points of a structure are spatial
         // neighbours (chains), so a row tile in input order is a compact blob
@@ -119,17 +125,31 @@ SetLayout layout_set(const PointSet& ps, std::vector<float4>& buf, std::vector<i
             size_t st = (size_t)(0.6180339887498949 * (double)nv);
             while (std::gcd(st, nv) != 1) ++st;
             std::vector<int> s(nv);
-            for (size_t k = 0; k < nv; ++k) s[k] = v[(k * st) % nv];
+            // the gathers jump through memory: chunks on several threads
+            parallel_for((int64_t)nv, 1 << 16, [&](int64_t k0, int64_t k1) {
+                size_t j = (size_t)(((unsigned __int128)k0 * st) % nv);
+                for (int64_t k = k0; k < k1; ++k) {
+                    s[k] = v[j];
+                    j += st;
+                    if (j >= nv) j -= nv;
+                }
+            });
             v.swap(s);
         }
-        L.seg_start[c] = (int)buf.size();
-        for (auto* v : {&body[c], &mid[c], &tail[c]})
-            for (int i : *v) {
-                float4 p = ps.orig[i];
-                p.w = float_of(i);
-                buf.push_back(p);
-                if (perm) perm->push_back(i);
-            }
+        L.seg_start[c] = (int)pos;
+        for (auto* v : {&body[c], &mid[c], &tail[c]}) {
+            const size_t at = pos;
+            parallel_for((int64_t)v->size(), 1 << 16, [&](int64_t k0, int64_t k1) {
+                for (int64_t k = k0; k < k1; ++k) {
+                    const int i = (*v)[k];
+                    float4 p = ps.orig[i];
+                    p.w = float_of(i);
+                    buf[at + k] = p;
+                    if (perm) (*perm)[pbase + (at - base) + k] = i;
+                }
+            });
+            pos += v->size();
+        }
         L.seg_len[c] = (int)(body[c].size() + mid[c].size() + tail[c].size());
         L.seg_body[c] = (int)body[c].size();
         L.seg_colbody[c] = (int)(body[c].size() + mid[c].size());
