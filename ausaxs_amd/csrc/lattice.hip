@@ -15,10 +15,11 @@
 // checks it (|A - rint(A)| <= 0.25, else a flag the host turns into an
 // error). Counts and delta sums come out identical to evaluating every pair.
 //
-// FFT: radix-2 decimation in time on lines of length L (a power of two, at
-// most 1024) held in LDS, G lines per workgroup (G * L <= 2048 complex values,
-// 32 KB, so that several workgroups share a CU); loads run along the lines'
-// common contiguous direction. Lines known to be zero (forward: outside the
+// FFT: decimation in time (radix-4 stages, one radix-2 stage when log2 L is
+// odd) on lines of length L (a power of two, at most 1024) held in LDS, G
+// lines per workgroup (G * L <= 2048 complex values, 32 KB, so that several
+// workgroups share a CU); loads run along the lines' common contiguous
+// direction. Lines known to be zero (forward: outside the
 // occupied box) or not needed (inverse: leading only to displacements beyond
 // the extents) are skipped; the forward transform's last pass stores the
 // power (or cross) spectrum and the inverse transform's last pass turns its
@@ -106,19 +107,43 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(FftPass p) {
         sh[g * L + r] = p.buf[base + (long long)k * S];
     }
     __syncthreads();
-    const int half = L / 2;
-    for (int h = 1; h < L; h <<= 1) {
-        const int tstride = L / (2 * h);
+    auto cmul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    int h = 1;
+    if (p.logL & 1) {
+        // one radix-2 stage first when log2 L is odd
+        const int half = L / 2;
         for (int t = threadIdx.x; t < G * half; t += blockDim.x) {
             const int g = t / half, q = t % half;
-            const int j = q & (h - 1);
-            const int i0 = g * L + (q / h) * 2 * h + j, i1 = i0 + h;
-            double2 w = p.tw[j * tstride];
-            if (p.inverse) w.y = -w.y;
+            const int i0 = g * L + 2 * q, i1 = i0 + 1;
             const double2 a = sh[i0], b = sh[i1];
-            const double2 bw = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
-            sh[i0] = make_double2(a.x + bw.x, a.y + bw.y);
-            sh[i1] = make_double2(a.x - bw.x, a.y - bw.y);
+            sh[i0] = make_double2(a.x + b.x, a.y + b.y);
+            sh[i1] = make_double2(a.x - b.x, a.y - b.y);
+        }
+        __syncthreads();
+        h = 2;
+    }
+    // radix-4: the radix-2 stages of half-sizes h and 2h in registers, one LDS
+    // round trip and one barrier for both. Stage h pairs (i0, i1), (i2, i3)
+    // with W_2h^j; stage 2h pairs (i0, i2) with W_4h^j and (i1, i3) with
+    // W_4h^(j+h) = W_4h^j * (-i) (conjugates for the inverse).
+    const int quarter = L / 4;
+    for (; h < L; h *= 4) {
+        for (int t = threadIdx.x; t < G * quarter; t += blockDim.x) {
+            const int g = t / quarter, q = t % quarter;
+            const int j = q & (h - 1);
+            const int i0 = g * L + (q / h) * 4 * h + j, i1 = i0 + h, i2 = i0 + 2 * h, i3 = i0 + 3 * h;
+            double2 w1 = p.tw[j * (L / (2 * h))], w2 = p.tw[j * (L / (4 * h))];
+            if (p.inverse) w1.y = -w1.y, w2.y = -w2.y;
+            const double2 w3 = p.inverse ? make_double2(-w2.y, w2.x) : make_double2(w2.y, -w2.x);
+            const double2 a0 = sh[i0], a2 = sh[i2];
+            const double2 a1 = cmul(sh[i1], w1), a3 = cmul(sh[i3], w1);
+            const double2 b0 = make_double2(a0.x + a1.x, a0.y + a1.y), b1 = make_double2(a0.x - a1.x, a0.y - a1.y);
+            const double2 b2 = cmul(make_double2(a2.x + a3.x, a2.y + a3.y), w2);
+            const double2 b3 = cmul(make_double2(a2.x - a3.x, a2.y - a3.y), w3);
+            sh[i0] = make_double2(b0.x + b2.x, b0.y + b2.y);
+            sh[i2] = make_double2(b0.x - b2.x, b0.y - b2.y);
+            sh[i1] = make_double2(b1.x + b3.x, b1.y + b3.y);
+            sh[i3] = make_double2(b1.x - b3.x, b1.y - b3.y);
         }
         __syncthreads();
     }
