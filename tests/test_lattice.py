@@ -93,3 +93,19 @@ def test_lattice_setting_is_checked():
     xyz, ff, w = structure(5, 50, 10)
     with pytest.raises(A.AusaxsError):
         A.GridStructure(xyz, ff, w, settings=A.default_settings(lattice=3))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_lattice_path_on_small_random_structures(seed):
+    # small, ragged cases: dummy sets whose sizes leave every tail length, few
+    # surface dummies (short cross columns), both bin widths
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 120))
+    m = int(rng.integers(0, 40))
+    xyz, ff, w = structure(200 + seed, max(n, 2), max(m, 1))
+    w = w[:m]
+    bw = (0.25, 0.1)[seed % 2]
+    for cls, dists in ((A.GridStructure, GRID_DISTS), (A.GridSurfaceStructure, SURFACE_DISTS)):
+        h2, t2 = run(cls, xyz, ff, w, 2, bin_width=bw)
+        h0, _ = run(cls, xyz, ff, w, 0, bin_width=bw)
+        same(h2, h0, dists)
