@@ -338,17 +338,6 @@ bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mod
     return true;
 }
 
-size_t lattice_bytes(const std::vector<LatticeJob>& jobs) {
-    size_t cells = 0, cross = 0, smax = 0, tw = 0;
-    for (const LatticeJob& j : jobs) {
-        const size_t c = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
-        cells = std::max(cells, c);
-        if (j.col_set >= 0) cross = std::max(cross, c);
-        smax = std::max(smax, (size_t)j.smax + 1);
-        tw += (size_t)(j.g.L[0] + j.g.L[1] + j.g.L[2]) / 2 + 3;
-    }
-    return (cells + cross + tw) * sizeof(double2) + smax * 8 + 64;
-}
 
 void sort_items(std::vector<PairWork>& items) {
     std::stable_sort(items.begin(), items.end(), [](const PairWork& x, const PairWork& y) {

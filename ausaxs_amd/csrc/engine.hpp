@@ -162,8 +162,7 @@ struct LatticeJob {
 // a small enough span, or false. mode: 0 never, 1 when the body pairs
 // outweigh the transforms, 2 whenever possible.
 bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mode, LatticeJob& out);
-// device bytes of the lattice buffers for a set of jobs
-size_t lattice_bytes(const std::vector<LatticeJob>& jobs);
+
 // Row work the pair kernel spends on a list: kTileI row lanes per item times
 // its columns (padding of partial row tiles included).
 int64_t padded_work(const std::vector<PairWork>& items);

@@ -186,3 +186,29 @@ def test_fit_surface_scaling_outside_the_plain_limits():
     h.set_params()
     r = A.Fitter(h, q, y, 0.01 * np.abs(y), enabled=A.FIT_EXCLUDED_VOLUME).fit(A.MINI_GOLDEN)
     assert abs(r["params"][0] - 1.3) < 2e-3 and r["chi2"] < 1e-2
+
+
+@pytest.mark.gpu
+def test_surface_large_invariants():
+    # 20k-atom globule (radius 45 A) + 6k waters, dummy pairs by lattice
+    # correlation (automatic at this size): exact pair-count identities, and
+    # the split histograms add up to the plain grid's bin for bin
+    g = A.synth_globule(2, 20000, (45, 45, 45), 2.0)
+    xyz, ff = g[:, :3].astype(np.float64), g.view(np.int32)[:, 3].copy()
+    w = A.synth_shell(3, 6000, (45, 45, 45), 3.0)[:, :3].astype(np.float64)
+    xi, xs = A.grid_exv_surface(xyz, ff, w)
+    ni, ns, n, m = len(xi), len(xs), len(xyz), len(w)
+    st = A.GridSurfaceStructure(xyz, ff, w)
+    st.set_timing(True)
+    h = st.calculate()
+    assert st.last_timing()["lattice_pairs"] > 0
+    c = lambda which: h.get(which)[1].astype(np.int64)
+    assert c(A.XX_INTERIOR).sum() == ni * ni and c(A.XX_SURFACE).sum() == ns * ns
+    assert c(A.XX_CROSS).sum() == 2 * ni * ns
+    assert c(A.AX_INTERIOR).sum() == n * ni and c(A.AX_SURFACE).sum() == n * ns
+    assert c(A.WX_INTERIOR).sum() == m * ni and c(A.WX_SURFACE).sum() == m * ns
+    plain = A.GridStructure(xyz, ff, w).calculate()
+    assert plain.max_bin == h.max_bin
+    xx = c(A.XX_INTERIOR) + c(A.XX_SURFACE) + c(A.XX_CROSS)
+    assert np.array_equal(xx, plain.get(A.XX)[1].astype(np.int64))
+    assert np.array_equal(c(A.WX_INTERIOR) + c(A.WX_SURFACE), plain.get(A.AW)[1][0].astype(np.int64))
