@@ -22,13 +22,31 @@ __device__ __forceinline__ void simd_rule_exact(float d2, float inv_w, float& x,
     dl = x - b;
 }
 
-__device__ __forceinline__ bool near_tie(float x, float dl) {
-    // The raw sqrt is within 1 ulp (<= 2^-23 relative) of the correctly rounded
-    // one, and each of the two products rounds by <= 2^-24, so x is within
-    // 2^-22 * x of the correctly rounded x. Outside a window of 1.25 * 2^-22 * x
-    // around a .5 tie both round to the same bin (the margin covers the
-    // rounding of the threshold itself).
-    return fabsf(dl) > __builtin_fmaf(x, -2.98023223876953125e-7f, 0.5f);
+// The pair kernel's fast path bins through t = fma(s, inv_w, kMagic): the
+// product s * inv_w is exact inside the FMA and the sum lands in [2^23, 2^24),
+// where floats are the integers, so t = kMagic + rint(s * inv_w) (ties to even)
+// and the bin is the bit pattern of t minus kMagicBits (no conversion).
+// dl = fma(s, inv_w, -(t - kMagic)) is the offset from the bin centre,
+// rounded once.
+constexpr float kMagic = 12582912.0f;  // 1.5 * 2^23
+constexpr uint32_t kMagicBits = 0x4B400000u;
+// Tie window slope per bin unit: 1.25 * 2^-22 (see bin_fast)
+constexpr float kTieSlope = -2.98023223876953125e-7f;
+
+__device__ __forceinline__ unsigned bin_of(float t) { return __float_as_uint(t) - kMagicBits; }
+
+// Fast binning of one d2 with the raw v_sqrt_f32 root s (within 1 ulp, 2^-23
+// relative, of the correctly rounded root). The reference's x is the
+// correctly rounded root times inv_w, rounded; the exact product s * inv_w
+// lies within 1.5 * 2^-23 * x of it, so both round to the same bin unless x is
+// that close to a .5 tie. Returns true inside a window of 1.25 * 2^-22 * x
+// around a tie (the margin covers the rounding of dl and of the threshold);
+// there the caller redoes the bin with bin_tie. slope_w = kTieSlope * inv_w.
+__device__ __forceinline__ bool bin_fast(float d2, float inv_w, float slope_w, float& t, float& dl) {
+    const float s = __builtin_amdgcn_sqrtf(d2);
+    t = __builtin_fmaf(s, inv_w, kMagic);
+    dl = __builtin_fmaf(s, inv_w, kMagic - t);
+    return fabsf(dl) > __builtin_fmaf(s, slope_w, 0.5f);
 }
 
 // Correctly rounded sqrt(d2) from a raw v_sqrt_f32 result s (within 1 ulp),
@@ -46,23 +64,20 @@ __device__ __forceinline__ float sqrt_fix(float d2, float s) {
     return r_up > 0.f ? s_up : s;
 }
 
-// fast-path result near a tie: redo the bin with the correctly rounded sqrt
-__device__ __forceinline__ void tie_fix(float d2, float inv_w, float& x, float& b, float& dl) {
-    const float r = sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2));
-    x = r * inv_w;
-    b = __builtin_rintf(x);
+// Inside the tie window: the reference's rule with the correctly rounded root
+// (x = r * inv_w rounded, bin = rint(x), dl = x - bin exactly).
+__device__ __forceinline__ void bin_tie(float d2, float inv_w, float& t, float& dl) {
+    const float x = sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2)) * inv_w;
+    const float b = __builtin_rintf(x);
     dl = x - b;
+    t = b + kMagic;
 }
 
-// The pair kernel's binning of one d2: the raw v_sqrt_f32 result, redone
-// with the correctly rounded root within the tie window. The bin is the
-// correctly rounded one; dl is the raw root's (within 1 ulp of x), so every
-// path that bins pairs this way produces the same fixed-point offsets.
-__device__ __forceinline__ void bin_fast(float d2, float inv_w, float& x, float& b, float& dl) {
-    x = __builtin_amdgcn_sqrtf(d2) * inv_w;
-    b = __builtin_rintf(x);
-    dl = x - b;
-    if (near_tie(x, dl)) tie_fix(d2, inv_w, x, b, dl);
+// One d2 binned as the pair kernel bins it (fast path, tie fallback): the bin
+// is the reference's; every path that bins pairs this way produces the same
+// fixed-point offsets.
+__device__ __forceinline__ void bin_pair(float d2, float inv_w, float slope_w, float& t, float& dl) {
+    if (bin_fast(d2, inv_w, slope_w, t, dl)) bin_tie(d2, inv_w, t, dl);
 }
 
 // kLowBias + k with k = rne(dl * 2^20) + 2^19: dl * 2^20 + 1.5625 * 2^23 lies in

@@ -218,6 +218,9 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
             }
             for (int b = a + 1; b < CR; ++b) {
                 if (LR.seg_body[b] == 0) continue;
+                // oriented items compare original indices as floats
+                if (row_of(a, b) != row_of(b, a) && n > (int64_t(1) << 24))
+                    throw std::runtime_error("ausaxs_hip: a point set with per-orientation rows holds at most 2^24 points");
                 const int t0 = LR.seg_start[b], t1 = t0 + LR.seg_body[b];
                 add_items(wl.items, s0, s1, t0, t1, row_of(a, b), row_of(b, a), false, tile_j);
             }

@@ -32,7 +32,9 @@
 namespace ausaxs::gpu {
 namespace {
 
-using binning::bin_fast;
+using binning::bin_of;
+using binning::bin_pair;
+using binning::kTieSlope;
 using binning::quantize;
 
 __global__ __launch_bounds__(256) void lattice_zero_kernel(double2* __restrict__ buf, size_t n) {
@@ -204,9 +206,9 @@ __global__ __launch_bounds__(256) void lattice_bins_kernel(const unsigned long l
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s <= smax; s += gridDim.x * blockDim.x) {
         const unsigned long long c = scount[s];
         if (!c) continue;
-        float x, b, dl;
-        bin_fast((float)s, inv_w, x, b, dl);
-        const unsigned bin = min((unsigned)(int)b, (unsigned)nbins);
+        float t, dl;
+        bin_pair((float)s, inv_w, kTieSlope * inv_w, t, dl);
+        const unsigned bin = min(bin_of(t), (unsigned)nbins);
         const int64_t k = (int64_t)(quantize(dl) - (uint32_t)kLowBias) - kDeltaOffset;
         atomicAdd((unsigned long long*)&cnt[bin], c);
         atomicAdd((unsigned long long*)&dsum[bin], (unsigned long long)((int64_t)c * k));

@@ -16,10 +16,12 @@
 //   d2 = (dx*dx + dy*dy) + dz*dz   separately rounded (file built with
 //                                  -ffp-contract=off)
 //   d  = sqrt_rn(d2), x = d*inv_w, bin = rint(x)  (round half to even)
-// v_sqrt_f32 is accurate to 1 ulp but not correctly rounded; it is used when
-// x is more than 1.25 * 2^-22 * x away from a .5 tie, otherwise the correctly rounded
-// square root (sqrtf: v_sqrt_f32 plus an FMA-residual correction; note that
-// __fsqrt_rn lowers to the uncorrected instruction) decides.
+// v_sqrt_f32 is accurate to 1 ulp but not correctly rounded; its root s is
+// used when x is more than 1.25 * 2^-22 * x away from a .5 tie, and the bin
+// then comes from one FMA, t = s * inv_w + 1.5 * 2^23, whose bit pattern is the
+// bin plus a constant (folded into the LDS base address). Inside the window the
+// correctly rounded square root (v_sqrt_f32 plus an FMA-residual correction;
+// note that __fsqrt_rn lowers to the uncorrected instruction) decides.
 // launch_bin_check verifies both exhaustively on device.
 #include "binning.hpp"
 #include "device_api.hpp"
@@ -27,12 +29,17 @@
 namespace ausaxs::gpu {
 namespace {
 
-using binning::near_tie;
+using binning::bin_fast;
+using binning::bin_of;
+using binning::bin_pair;
+using binning::bin_tie;
+using binning::kMagic;
+using binning::kMagicBits;
+using binning::kTieSlope;
 using binning::quantize;
 using binning::simd_rule_exact;
 using binning::sqrt_fix;
 using binning::sqrt_rn;
-using binning::tie_fix;
 
 constexpr int kThreads = 256;
 constexpr int kRowsPerLane = 2;
@@ -40,18 +47,13 @@ constexpr int kTileI = kThreads * kRowsPerLane;  // row points per work item
 constexpr int kColTile = kThreads;               // column points per LDS tile
 constexpr int kColGroup = 4;                     // column points per double-buffered group
 
-__device__ __forceinline__ void simd_rule_fast(float dx, float dy, float dz, float inv_w, float& x, float& b,
-                                               float& dl, float& d2) {
+// d2 of the reference's SIMD body: separately rounded squares and sums
+__device__ __forceinline__ float simd_d2(float dx, float dy, float dz) {
     const float sx = dx * dx;
     const float sy = dy * dy;
     const float sz = dz * dz;
-    d2 = (sx + sy) + sz;
-    const float r = __builtin_amdgcn_sqrtf(d2);
-    x = r * inv_w;
-    b = __builtin_rintf(x);
-    dl = x - b;
+    return (sx + sy) + sz;
 }
-
 
 __device__ __forceinline__ uint64_t cell_add(float dl) {
     return (uint64_t(1) << 42) + (uint64_t)quantize(dl);
@@ -72,46 +74,67 @@ __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
     __hip_atomic_fetch_add(cell, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// LDS byte addresses are 32-bit. The pair loop addresses a cell as
+// base + 8 * bits(t), with the magic constant's bits folded into base (mod
+// 2^32), so the address of a pair is one shift-and-add.
+using lds_u64 = __attribute__((address_space(3))) uint64_t;
+__device__ __forceinline__ uint32_t lds_byte_addr(uint64_t* p) { return (uint32_t)(uintptr_t)(lds_u64*)p; }
+__device__ __forceinline__ void lds_add_at(uint32_t addr, uint64_t v) {
+    __hip_atomic_fetch_add((lds_u64*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Row points of one lane (kRowsPerLane of them) for the current item; roff:
 // LDS offset of the row each point adds into (split items).
 struct Rows {
     float x[kRowsPerLane], y[kRowsPerLane], z[kRowsPerLane];
-    int tag[kRowsPerLane], idx[kRowsPerLane];
+    float tagf[kRowsPerLane];  // original index as a float (exact below 2^24)
+    int idx[kRowsPerLane];
     bool valid[kRowsPerLane];
-    int roff[kRowsPerLane];
+    uint32_t base[kRowsPerLane];  // LDS byte address of the row's cell 0, minus 8 * kMagicBits (mod 2^32)
 };
 
 // All kRowsPerLane pairs of one column point: bin, tie fallback, LDS add.
+// Oriented items hold the column's original index in pj.w as a float (the
+// staging converts it) and add into row 1 when the row point's index is the
+// larger: sat(tag_i - tag_j) is 1 exactly then (distinct integers differ by
+// at least 1), and t + sat * off1 stays an exact integer-valued float, so the
+// row choice costs two VALU operations per pair and no compare or select.
 template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT>
 __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, float inv_w,
-                                            unsigned ovf, int off1) {
-    const int tagj = __float_as_int(pj.w);
-    float x[kRowsPerLane], b[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane];
+                                            float slope_w, unsigned ovf, int off1) {
+    float t[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane];
+    bool near[kRowsPerLane];
     bool tie = false;
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
-        simd_rule_fast(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z, inv_w, x[k], b[k], dl[k], d2[k]);
-        tie |= near_tie(x[k], dl[k]);
+        d2[k] = simd_d2(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z);
+        near[k] = bin_fast(d2[k], inv_w, slope_w, t[k], dl[k]);
+        tie |= near[k];
     }
     if (tie) {
 #pragma unroll
         for (int k = 0; k < kRowsPerLane; ++k)
-            if (near_tie(x[k], dl[k])) tie_fix(d2[k], inv_w, x[k], b[k], dl[k]);
+            if (near[k]) bin_tie(d2[k], inv_w, t[k], dl[k]);
     }
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
-        unsigned bin = (unsigned)(int)b[k];
-        if (CLAMP) bin = min(bin, ovf);
+        // u = kMagicBits + bin (+ off1 for row 1 of oriented items)
+        uint32_t u;
+        if (ORIENT) {
+            float tt = t[k];
+            if (CLAMP) tt = fminf(tt, kMagic + (float)ovf);
+            const float sel = __builtin_amdgcn_fmed3f(r.tagf[k] - pj.w, 0.f, 1.f);
+            u = __float_as_uint(__builtin_fmaf(sel, (float)off1, tt));
+        } else {
+            u = __float_as_uint(t[k]);
+            if (CLAMP) u = min(u, kMagicBits + ovf);
+        }
         if (!FULL || DIAG) {
             bool keep = r.valid[k];
             if (DIAG) keep = keep && (j > r.idx[k]);
-            bin = keep ? bin : ovf + 1u;
+            u = keep ? u : kMagicBits + ovf + 1u;
         }
-        // oriented items choose the row per pair, split items by the row point
-        int off = 0;
-        if (ORIENT) off = r.tag[k] < tagj ? 0 : off1;
-        if (SPLIT) off = r.roff[k];
-        lds_add(&hist[off + (int)bin], cell_add(dl[k]));
+        lds_add_at(r.base[k] + (u << 3), cell_add(dl[k]));
     }
 }
 
@@ -137,13 +160,15 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         r.x[k] = p.x;
         r.y[k] = p.y;
         r.z[k] = p.z;
-        r.tag[k] = __float_as_int(p.w);
+        r.tagf[k] = (float)__float_as_int(p.w);
         r.idx[k] = i;
         // kSplit: rows from the second class add into the second LDS row
-        r.roff[k] = SPLIT && i >= w.i_begin + (w.flags >> kSplitShift) ? nbins + 2 : 0;
+        const int roff = SPLIT && i >= w.i_begin + (w.flags >> kSplitShift) ? nbins + 2 : 0;
+        r.base[k] = lds_byte_addr(hist + roff) - kMagicBits * 8u;
     }
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
+    const float slope_w = kTieSlope * inv_w;
 
     // the next column tile is loaded into a register while the current one is
     // evaluated, so the global-memory latency of the tile loads stays hidden
@@ -154,7 +179,7 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     for (int jc = w.j_begin; jc < w.j_end; jc += kColTile) {
         const int nc = min(kColTile, w.j_end - jc);
         __syncthreads();  // previous tile consumed
-        if (t < nc) cols[t] = pre;
+        if (t < nc) cols[t] = ORIENT ? make_float4(pre.x, pre.y, pre.z, (float)__float_as_int(pre.w)) : pre;
         __syncthreads();
         pre = pts[min(jc + kColTile + t, w.j_end - 1)];  // past the item's end: unused
         const int n2 = nc & ~(2 * kColGroup - 1);
@@ -169,16 +194,16 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
             for (int u = 0; u < kColGroup; ++u) bq[u] = cols[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jc + c + u, hist, inv_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jc + c + u, hist, inv_w, slope_w, ovf, off1);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u) a[u] = cols[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, slope_w, ovf, off1);
         }
         for (int c = n2; c < nc; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cols[c], jc + c, hist, inv_w, ovf, off1);
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cols[c], jc + c, hist, inv_w, slope_w, ovf, off1);
     }
 }
 
@@ -393,11 +418,9 @@ __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi
         if (!ok && d2 > 0.f) ++bad_cr;
         const float xr = r * inv_w;
         const float br = __builtin_rintf(xr);
-        float x = __builtin_amdgcn_sqrtf(d2) * inv_w;
-        float b = __builtin_rintf(x);
-        float dl = x - b;
-        if (near_tie(x, dl)) tie_fix(d2, inv_w, x, b, dl);
-        if (b != br) ++bad_fast;
+        float t, dl;
+        bin_pair(d2, inv_w, kTieSlope * inv_w, t, dl);
+        if (t - kMagic != br) ++bad_fast;
         // the residual correction of the tie fallback, on every normal d2
         if (d2 >= 1e-30f && sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2)) != r) ++bad_fix;
     }
