@@ -2,9 +2,10 @@
 //
 // Restates the reference's fused evaluate+increment (CompactCoordinatesXYZFF.h
 // evaluate_{16,8,4,1}, HistogramManagerMTFFAvg.cpp:214-280) as:
-//   * one workgroup per rectangle of the pair space (PairWork);
+//   * one workgroup per rectangle of the pair space (PairWork): 512 lanes in
+//     two parts that hold the same 512 row points and split each column tile;
 //   * row points in VGPRs (2 per lane); column points staged through LDS in
-//     tiles of 256 and read back as wave-wide broadcasts in double-buffered
+//     tiles of 512 and read back as wave-wide broadcasts in double-buffered
 //     groups of 4, so each lane evaluates 2 pairs per column point (LDS reads
 //     and LDS atomics retire in order, so the broadcast reads of the next
 //     group never wait on the atomics of the current one);
@@ -41,11 +42,20 @@ using binning::simd_rule_exact;
 using binning::sqrt_fix;
 using binning::sqrt_rn;
 
-constexpr int kThreads = 256;
+// Parts: groups of 256 lanes that hold the same row points, take separate
+// shares of each column tile and add into one LDS histogram. Two parts put 6
+// waves on each SIMD where the histogram of a 0.1 A workload (two rows of
+// ~2,500 bins, 40 KB) allows three workgroups per CU; one part left the SIMDs
+// with 3 waves and 12 % of their issue slots idle, four parts measure the same
+// as two.
+constexpr int kParts = 2;
+constexpr int kRowLanes = 256;                   // lanes holding distinct row points
+constexpr int kThreads = kRowLanes * kParts;
 constexpr int kRowsPerLane = 2;
-constexpr int kTileI = kThreads * kRowsPerLane;  // row points per work item
-constexpr int kColTile = kThreads;               // column points per LDS tile
+constexpr int kTileI = kRowLanes * kRowsPerLane;  // row points per work item
+constexpr int kColTile = 512;                    // column points per LDS tile (each part takes a share)
 constexpr int kColGroup = 4;                     // column points per double-buffered group
+static_assert(kColTile + kColGroup == kStaticLds / sizeof(float4), "static LDS size");
 
 // d2 of the reference's SIMD body: separately rounded squares and sums
 __device__ __forceinline__ float simd_d2(float dx, float dy, float dz) {
@@ -138,8 +148,9 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
     }
 }
 
-// One work item. Column points pass through LDS 256 at a time; within a tile
-// they are consumed in double-buffered groups of kColGroup: the reads of one
+// One work item. Column points pass through LDS 512 at a time, each part
+// taking a share of the tile; within its share a part consumes them in
+// double-buffered groups of kColGroup: the reads of one
 // group are issued before the atomics of the other, so a wait for them never
 // waits for atomics (LDS operations retire in order) and no register copies
 // are needed between groups.
@@ -147,13 +158,14 @@ template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
                                          float4* cols, float inv_w, int nbins) {
     const int t = threadIdx.x;
+    const int part = t / kRowLanes;  // every part holds the same row points
     Rows r;
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
         // lanes of one wave take points 8 apart: spatially spread rows give
         // spread distances to a common column point, so fewer lanes collide
         // on one LDS cell.
-        const int li = k * kThreads + t;
+        const int li = k * kRowLanes + (t & (kRowLanes - 1));
         const int i = w.i_begin + ((li & 63) * 8 + (li >> 6));
         r.valid[k] = FULL || i < w.i_end;
         const float4 p = r.valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -182,28 +194,36 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         if (t < nc) cols[t] = ORIENT ? make_float4(pre.x, pre.y, pre.z, (float)__float_as_int(pre.w)) : pre;
         __syncthreads();
         pre = pts[min(jc + kColTile + t, w.j_end - 1)];  // past the item's end: unused
-        const int n2 = nc & ~(2 * kColGroup - 1);
+        // the tile's columns split between the parts: part p takes
+        // [p m, (p + 1) m) clipped to nc, m a multiple of 2 * kColGroup (the
+        // last parts may be empty)
+        const int m = (nc + 2 * kColGroup * kParts - 1) / (2 * kColGroup * kParts) * (2 * kColGroup);
+        const int c0 = min(part * m, nc);
+        const int n = min(c0 + m, nc) - c0;
+        const float4* cc = cols + c0;
+        const int jh = jc + c0;
+        const int n2 = n & ~(2 * kColGroup - 1);
         float4 a[kColGroup], bq[kColGroup];
 #pragma unroll
-        for (int u = 0; u < kColGroup; ++u) a[u] = cols[u];
+        for (int u = 0; u < kColGroup; ++u) a[u] = cc[u];
         // completes the first reads before the loop, so the loop needs no
         // wait that would also drain atomics in flight
         __syncthreads();
         for (int c = 0; c < n2; c += 2 * kColGroup) {
 #pragma unroll
-            for (int u = 0; u < kColGroup; ++u) bq[u] = cols[c + kColGroup + u];
+            for (int u = 0; u < kColGroup; ++u) bq[u] = cc[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jc + c + u, hist, inv_w, slope_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jh + c + u, hist, inv_w, slope_w, ovf, off1);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
-            for (int u = 0; u < kColGroup; ++u) a[u] = cols[c + 2 * kColGroup + u];
+            for (int u = 0; u < kColGroup; ++u) a[u] = cc[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jc + c + kColGroup + u, hist, inv_w, slope_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jh + c + kColGroup + u, hist, inv_w, slope_w, ovf, off1);
         }
-        for (int c = n2; c < nc; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cols[c], jc + c, hist, inv_w, slope_w, ovf, off1);
+        for (int c = n2; c < n; ++c)
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cc[c], jh + c, hist, inv_w, slope_w, ovf, off1);
     }
 }
 
