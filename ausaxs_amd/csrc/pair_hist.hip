@@ -255,10 +255,12 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 // into copy l mod copies, so lanes whose pairs fall into the same bin collide
 // on one LDS cell `copies` times less often (pairs of two bodies crowd into few
 // bins). An item with one output row in a two-row launch uses the same LDS as
-// 2 * copies one-row histograms. The copies are summed at the flush. The kernel is held to 64 VGPRs
-// (8 waves per SIMD): one register more costs a wave per SIMD and about 5% of
-// the pair rate.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
+// 2 * copies one-row histograms. The copies are summed at the flush. The
+// kernel may use up to 80 VGPRs (6 waves per SIMD, what the LDS of a 0.1 A
+// workload allows anyway): held to 64, it spilled loop-invariant values to
+// scratch in every workgroup, ~200 MB of scratch writes per launch on the
+// 250k-atom workload, at the same pair rate.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const PairWork* __restrict__ items, int n_items,
                                                              uint64_t* __restrict__ partial, float inv_w,
                                                              int nbins, int clamp, int rows, int copies) {
