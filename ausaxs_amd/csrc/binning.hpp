@@ -64,18 +64,20 @@ __device__ __forceinline__ float sqrt_fix(float d2, float s) {
     return r_up > 0.f ? s_up : s;
 }
 
-// Inside the tie window: the reference's rule with the correctly rounded root
-// (x = r * inv_w rounded, bin = rint(x), dl = x - bin exactly).
+// Inside the tie window: the bin of the reference's rule with the correctly
+// rounded root (x = r * inv_w rounded, bin = rint(x)). The offset keeps the
+// fast path's formula, dl = s * inv_w - bin with the raw root s (clamped to
+// [-0.5, 0.5] when the bin differs from the raw root's), so dl is a function
+// of d2 alone, whichever path bins the pair.
 __device__ __forceinline__ void bin_tie(float d2, float inv_w, float& t, float& dl) {
-    const float x = sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2)) * inv_w;
-    const float b = __builtin_rintf(x);
-    dl = x - b;
+    const float s = __builtin_amdgcn_sqrtf(d2);
+    const float b = __builtin_rintf(sqrt_fix(d2, s) * inv_w);
     t = b + kMagic;
+    dl = __builtin_amdgcn_fmed3f(__builtin_fmaf(s, inv_w, -b), -0.5f, 0.5f);
 }
 
 // One d2 binned as the pair kernel bins it (fast path, tie fallback): the bin
-// is the reference's; every path that bins pairs this way produces the same
-// fixed-point offsets.
+// is the reference's, the offset a function of d2 alone.
 __device__ __forceinline__ void bin_pair(float d2, float inv_w, float slope_w, float& t, float& dl) {
     if (bin_fast(d2, inv_w, slope_w, t, dl)) bin_tie(d2, inv_w, t, dl);
 }
