@@ -46,6 +46,7 @@ class _Settings(C.Structure):
         ("debye_terms", C.c_int),
         ("tile_j", C.c_int),
         ("lattice", C.c_int),
+        ("surface_thickness", C.c_double),
     ]
 
 
@@ -70,7 +71,7 @@ def _load():
         "ausaxs_structure_create_grid_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
         "ausaxs_hydrate_radial": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, P(C.c_int), vp]),
         "ausaxs_grid_exv_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, P(C.c_int), vp, C.c_int,
-                                              P(C.c_int)]),
+                                              P(C.c_int), C.c_double]),
         "ausaxs_structure_free": (C.c_int, [vp]),
         "ausaxs_hist_calculate": (C.c_int, [C.c_int, vp, C.c_int, vp, C.c_int, P(_Settings), C.c_double, P(vp)]),
         "ausaxs_histogram_info": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
@@ -419,17 +420,19 @@ def hydrate(atoms_xyz, ff, bodies=None, noise=None):
     return out[: m.value].copy(), per
 
 
-def grid_exv_surface(atoms_xyz, ff, waters_xyz=None):
-    """Interior and surface dummies (RawGridWithSurfaceExv, reference order); host-only."""
+def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, surface_thickness=1.0):
+    """Interior and surface dummies (RawGridWithSurfaceExv, reference order); host-only.
+    surface_thickness: settings::grid::exv::surface_thickness in A (cell width 1)."""
     a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(ff, np.int32).reshape(-1)
     w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
     ni, ns = C.c_int(), C.c_int()
+    t = C.c_double(surface_thickness)
     _check(lib.ausaxs_grid_exv_surface(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), None, 0, C.byref(ni), None, 0,
-                                       C.byref(ns)))
+                                       C.byref(ns), t))
     xi, xs = np.zeros((max(1, ni.value), 3)), np.zeros((max(1, ns.value), 3))
     _check(lib.ausaxs_grid_exv_surface(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), _ptr(xi), ni.value, C.byref(ni),
-                                       _ptr(xs), ns.value, C.byref(ns)))
+                                       _ptr(xs), ns.value, C.byref(ns), t))
     return xi[: ni.value].copy(), xs[: ns.value].copy()
 
 

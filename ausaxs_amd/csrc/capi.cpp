@@ -78,7 +78,9 @@ HistSettings to_settings(const ausaxs_settings* s) {
         h.debye_terms = s->debye_terms;
         h.tile_j = s->tile_j;
         h.lattice = s->lattice;
+        h.surface_thickness = s->surface_thickness;
     }
+    require(h.surface_thickness >= 0 && h.surface_thickness <= 64, "surface_thickness must lie in [0, 64] A");
     require(h.lattice >= 0 && h.lattice <= 2, "lattice must be 0, 1 or 2");
     require(h.bin_width > 0, "bin_width must be positive");
     require(h.debye_terms >= 0 && h.debye_terms <= 3, "debye_terms must be 0, 1, 2 or 3");
@@ -103,6 +105,7 @@ void ausaxs_default_settings(ausaxs_settings* s) {
     s->debye_terms = h.debye_terms;
     s->tile_j = h.tile_j;
     s->lattice = h.lattice;
+    s->surface_thickness = h.surface_thickness;
 }
 
 int ausaxs_get_last_error(char* buf, int len) {
@@ -168,11 +171,11 @@ int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t*
 
 int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
                             int n_waters, double* interior_xyz, int capacity_interior, int* n_interior,
-                            double* surface_xyz, int capacity_surface, int* n_surface) {
+                            double* surface_xyz, int capacity_surface, int* n_surface, double surface_thickness) {
     return guarded([&] {
         require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
         require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
-        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, true);
+        GridExv g = make_grid_exv(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, true, surface_thickness);
         const int ni = (int)(g.xyz.size() / 3), ns = (int)(g.surface.size() / 3);
         if (n_interior) *n_interior = ni;
         if (n_surface) *n_surface = ns;

@@ -103,7 +103,9 @@ GridGeometry grid_geometry(const double* atoms, int n, const double* waters, int
 }
 
 GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
-                      bool detect_surface) {
+                      bool detect_surface, double surface_thickness) {
+    if (!(surface_thickness >= 0) || surface_thickness > 64)
+        throw std::runtime_error("ausaxs_hip: surface_thickness must lie in [0, 64] A");
     GridExv out;
     if (n <= 0) return out;
     const GridGeometry geo = grid_geometry(atoms, n, waters, m);
@@ -254,6 +256,43 @@ GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const doubl
         cnt_in[t + 1] = ci;
         cnt_sf[t + 1] = cs;
     });
+    // A thicker surface (RawGridWithSurfaceExv.cpp:137-190): the reference
+    // marks every dummy within expand cells of each first-pass surface dummy;
+    // the same set is the dummies with a first-pass surface dummy within
+    // expand cells of them (the relation is symmetric), gathered per slab.
+    const int thick = (int)std::round(surface_thickness / kCell);
+    if (detect_surface && thick != 1) {
+        const int e = (int)(thick * 0.5), e2 = e * e;
+        std::vector<uint8_t> grown(cls);
+        parallel([&](int t) {
+            int x0, x1;
+            slab(t, x0, x1);
+            int64_t ci = 0, cs = 0;
+            for (int i = x0; i < x1; ++i)
+                for (int j = bmin[1]; j <= bmax[1]; ++j)
+                    for (int k = bmin[2]; k <= bmax[2]; ++k) {
+                        uint8_t& c = grown[idx(i, j, k)];
+                        if (c == 1) {
+                            for (int l = -e; l <= e && c == 1; ++l)
+                                for (int mm = -e; mm <= e && c == 1; ++mm)
+                                    for (int nn = -e; nn <= e; ++nn) {
+                                        const int a = i + l, b = j + mm, d = k + nn;
+                                        if (l * l + mm * mm + nn * nn > e2 || a < bmin[0] || a > bmax[0] ||
+                                            b < bmin[1] || b > bmax[1] || d < bmin[2] || d > bmax[2])
+                                            continue;
+                                        if (cls[idx(a, b, d)] == 2) {
+                                            c = 2;
+                                            break;
+                                        }
+                                    }
+                        }
+                        if (c) (c == 1 ? ci : cs) += 1;
+                    }
+            cnt_in[t + 1] = ci;
+            cnt_sf[t + 1] = cs;
+        });
+        cls.swap(grown);
+    }
     for (int t = 0; t < n_thr; ++t) cnt_in[t + 1] += cnt_in[t], cnt_sf[t + 1] += cnt_sf[t];
     out.xyz.resize(3 * (size_t)cnt_in[n_thr]);
     out.surface.resize(3 * (size_t)cnt_sf[n_thr]);

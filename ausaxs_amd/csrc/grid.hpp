@@ -34,8 +34,12 @@ struct GridExv {
 // from it (RadialLineGenerator, radii 1.001, 2, 3 A, 8 divisions) score at
 // least 42 (7 per line that leaves the molecule, 3 or 5 for one blocked at 2
 // or 3 A, 0 at 1 A), every other dummy is interior.
+// surface_thickness (settings::grid::exv::surface_thickness, A; cell width 1):
+// other than one cell, every dummy within int(round(thickness) * 0.5) cells
+// (squared integer distance) of a surface dummy is a surface dummy too
+// (RawGridWithSurfaceExv.cpp:137-190, 196-204).
 GridExv make_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
-                      bool detect_surface = false);
+                      bool detect_surface = false, double surface_thickness = 1.0);
 
 // constants::radius::vdw per form-factor type (FormFactorType.h:174-193)
 double vdw_radius(int ff_type);

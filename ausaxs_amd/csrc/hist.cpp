@@ -970,7 +970,7 @@ std::unique_ptr<ResidentStructure> make_grid_surface_structure(const double* ato
     for (int i = 0; i < n; ++i)
         if (ff[i] < 1 || ff[i] >= kMaxFF)
             throw std::runtime_error("ausaxs_hip: atom form factor type outside [1, 14]");
-    GridExv g = make_grid_exv(atoms_xyz, ff, n, waters_xyz, m, true);
+    GridExv g = make_grid_exv(atoms_xyz, ff, n, waters_xyz, m, true, s.surface_thickness);
     std::vector<float> a, w;
     pack_ff_points(atoms_xyz, ff, n, 0, a);
     pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);

@@ -70,6 +70,7 @@ struct HistSettings {
     int debye_terms = 3;        // bf16 terms per operand; 0 = FP64 contraction
     int tile_j = 8000;          // columns per work item (<= 8189)
     int lattice = 1;            // grid dummy-dummy pairs by lattice correlation: 0 off, 1 auto, 2 always
+    double surface_thickness = 1.0;  // grid-surface excluded volume: surface layer thickness (A)
 };
 // WidthControllers.h:10-27: inverse width as float
 float inv_width(const HistSettings& s);

@@ -55,6 +55,8 @@ typedef struct {
     int tile_j;        /* columns per pair work item (default 8000, at most 8189) */
     int lattice;       /* grid dummy-dummy pairs by lattice correlation instead of pair by pair
                           (identical histograms): 0 off, 1 when cheaper (default), 2 always */
+    double surface_thickness; /* grid-surface excluded volume: settings::grid::exv::surface_thickness
+                                 (GridSettings.cpp:17; A, default 1, in [0, 64]) */
 } ausaxs_settings;
 
 typedef struct ausaxs_structure ausaxs_structure; /* resident coordinates + work lists */
@@ -102,10 +104,11 @@ int ausaxs_hydrate_radial(const double* atoms_xyz, const int32_t* atom_ff, int n
                           int n_bodies, const double* noise, double* waters_xyz, int capacity, int* n_waters,
                           int* per_body);
 /* The interior and surface dummies of RawGridWithSurfaceExv::create, each in the
- * reference's voxel order; counts are returned also when a capacity is too small. */
+ * reference's voxel order; counts are returned also when a capacity is too small.
+ * surface_thickness: settings::grid::exv::surface_thickness (A; 1 is the default). */
 int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const double* waters_xyz,
                             int n_waters, double* interior_xyz, int capacity_interior, int* n_interior,
-                            double* surface_xyz, int capacity_surface, int* n_surface);
+                            double* surface_xyz, int capacity_surface, int* n_surface, double surface_thickness);
 
 /* Excluded-volume dummy atoms of the default grid (RawGridExv::create,
  * RawGridExv.cpp:10-59; Grid.cpp:34-110; SphericalExpander.h:29-90) in the

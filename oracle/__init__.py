@@ -300,16 +300,16 @@ def debye_ffgrid(hist, q, bin_width, long_double=False, **params):
 # ---------------------------------------------------------------------------
 lib.orc_grid_exv_surface.restype = C.c_int
 lib.orc_grid_exv_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                                     C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double]
 lib.orc_hist_ffgrid_surface.restype = C.c_int
 lib.orc_hist_ffgrid_surface.argtypes = ([C.c_void_p, C.c_int] * 4 + [C.c_double, C.c_float, C.c_int, C.c_int]
                                         + [C.c_void_p] * 13)
 
 
-def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=None):
+def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=None, surface_thickness=1.0):
     """Interior and surface dummies (each n x 3 FP64, reference order) and the grid volume.
     rvol: the minimum excluded-volume radius; atomic_radius: one radius for every atom in
-    place of the van der Waals radii."""
+    place of the van der Waals radii; surface_thickness in A (cell width 1)."""
     a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(ff, np.int32).reshape(-1)
     w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
@@ -318,7 +318,8 @@ def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=No
     xs = np.zeros((cap, 3), np.float64)
     ns, vol = C.c_int(), C.c_double()
     n = lib.orc_grid_exv_surface(_p(a), _p(f), len(a), _p(w), len(w), _p(xi), _p(xs), cap, C.byref(ns),
-                                 C.byref(vol), rvol, -1.0 if atomic_radius is None else atomic_radius)
+                                 C.byref(vol), rvol, -1.0 if atomic_radius is None else atomic_radius,
+                                 float(surface_thickness))
     if n < 0:
         raise RuntimeError("grid_exv_surface capacity")
     return xi[:n].copy(), xs[:ns.value].copy(), vol.value
@@ -344,10 +345,11 @@ def hydrate_radial(atoms_xyz, ff, bodies=None, noise=None):
     return out[:m].copy(), per
 
 
-def hist_ffgrid_surface(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1):
+def hist_ffgrid_surface(atoms_xyz, ff, waters_xyz=None, bin_width=0.25, bin_count=8000, nthreads=1,
+                        surface_thickness=1.0):
     """HistogramManagerMTFFGridSurface::calculate_all from FP64 coordinates (see orc_hist_ffgrid_surface)."""
     w = np.zeros((0, 3)) if waters_xyz is None else np.asarray(waters_xyz, np.float64)
-    xi_xyz, xs_xyz, vol = grid_exv_surface(atoms_xyz, ff, w)
+    xi_xyz, xs_xyz, vol = grid_exv_surface(atoms_xyz, ff, w, surface_thickness=surface_thickness)
     atoms = pack_ff(atoms_xyz, ff)
     waters = pack_ff(w, np.ones(len(w), np.int32))
     xi = pack_ff(xi_xyz, np.zeros(len(xi_xyz), np.int32))

@@ -518,7 +518,7 @@ static uint8_t* grid_build(const double* atoms, const int32_t* ff, int n, const 
 
 static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
                          double* out_xyz, int cap, double* surf_xyz, int* n_surf, double* volume, double rvol,
-                         double ra) {
+                         double ra, double thickness) {
     const double width = 1.0;
     double gmin[3];
     int bins[3], *loc = NULL;
@@ -544,6 +544,19 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
     }
 #define OOB(a, b, c) ((a) < 0 || (a) >= bins[0] || (b) < 0 || (b) >= bins[1] || (c) < 0 || (c) >= bins[2])
 #define EMPTY_OR_WATER(a, b, c) (G(a, b, c) == G_EMPTY || (G(a, b, c) & (G_WAREA | G_WCENTER)))
+    /* RawGridWithSurfaceExv::create (:196-204): a surface thickness other than
+     * one cell takes the helper's non-unity branch, whose first pass only marks
+     * the surface dummies (RESERVED_1, here mark bit 1) */
+    const int thick = (int)round(thickness / width);
+    uint8_t* mark = NULL;
+    if (surf_xyz && thick != 1) {
+        mark = calloc((size_t)bins[0] * bins[1] * bins[2], 1);
+        if (!mark) {
+            free(g), free(loc);
+            return -1;
+        }
+    }
+#define MARK(i, j, k) mark[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
     int count = 0, ns = 0, ok = 1;
     for (int i = imin[0] - buffer > 0 ? imin[0] - buffer : 0; i < (imax[0] + buffer + 1 < bins[0] ? imax[0] + buffer + 1 : bins[0]); ++i)
         for (int j = imin[1] - buffer > 0 ? imin[1] - buffer : 0; j < (imax[1] + buffer + 1 < bins[1] ? imax[1] + buffer + 1 : bins[1]); ++j)
@@ -573,6 +586,10 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
                         if (r == 3) score += 7;
                     }
                     surface = score >= 42;
+                    if (mark) {
+                        if (surface) MARK(i, j, k) |= 1;
+                        continue;
+                    }
                 }
                 double* dst = surface ? surf_xyz : out_xyz;
                 int* cnt = surface ? &ns : &count;
@@ -585,6 +602,48 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
                 }
                 ++*cnt;
             }
+    if (mark) {
+        /* expand = round(thickness / cell_width) * 0.5 as an int (:140-155):
+         * every exv-state voxel within squared integer distance expand^2 of a
+         * marked voxel gets RESERVED_2 (mark bit 2). The reference indexes the
+         * neighbours without a bounds check; here voxels off the grid are
+         * skipped (they hold no dummies). */
+        const int e = (int)(thick * 0.5), e2 = e * e;
+        for (int i = imin[0] - buffer > 0 ? imin[0] - buffer : 0; i < (imax[0] + buffer + 1 < bins[0] ? imax[0] + buffer + 1 : bins[0]); ++i)
+            for (int j = imin[1] - buffer > 0 ? imin[1] - buffer : 0; j < (imax[1] + buffer + 1 < bins[1] ? imax[1] + buffer + 1 : bins[1]); ++j)
+                for (int k = imin[2] - buffer > 0 ? imin[2] - buffer : 0; k < (imax[2] + buffer + 1 < bins[2] ? imax[2] + buffer + 1 : bins[2]); ++k) {
+                    if (!(MARK(i, j, k) & 1)) continue;
+                    for (int l = -e; l <= e; ++l)
+                        for (int mm = -e; mm <= e; ++mm)
+                            for (int nn = -e; nn <= e; ++nn) {
+                                const int a = i + l, b = j + mm, c = k + nn;
+                                if (OOB(a, b, c)) continue;
+                                if ((G(a, b, c) & (G_VOLUME | G_ACENTER | G_AAREA)) && l * l + mm * mm + nn * nn <= e2)
+                                    MARK(a, b, c) |= 2;
+                            }
+                }
+        /* collection (:158-180): marked voxels are surface, other exv voxels interior */
+        for (int i = imin[0] - buffer > 0 ? imin[0] - buffer : 0; i < (imax[0] + buffer + 1 < bins[0] ? imax[0] + buffer + 1 : bins[0]); ++i)
+            for (int j = imin[1] - buffer > 0 ? imin[1] - buffer : 0; j < (imax[1] + buffer + 1 < bins[1] ? imax[1] + buffer + 1 : bins[1]); ++j)
+                for (int k = imin[2] - buffer > 0 ? imin[2] - buffer : 0; k < (imax[2] + buffer + 1 < bins[2] ? imax[2] + buffer + 1 : bins[2]); ++k) {
+                    int surface;
+                    if (MARK(i, j, k)) surface = 1;
+                    else if (G(i, j, k) & (G_VOLUME | G_ACENTER | G_AAREA)) surface = 0;
+                    else continue;
+                    double* dst = surface ? surf_xyz : out_xyz;
+                    int* cnt = surface ? &ns : &count;
+                    if (*cnt < cap) {
+                        dst[3 * *cnt] = gmin[0] + i * width;
+                        dst[3 * *cnt + 1] = gmin[1] + j * width;
+                        dst[3 * *cnt + 2] = gmin[2] + k * width;
+                    } else {
+                        ok = 0;
+                    }
+                    ++*cnt;
+                }
+        free(mark);
+    }
+#undef MARK
 #undef OOB
 #undef EMPTY_OR_WATER
 #undef G
@@ -695,13 +754,14 @@ int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int*
 
 int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double* out_xyz,
                  int cap, double* volume) {
-    return grid_exv_core(atoms, ff, n, waters, m, out_xyz, cap, NULL, NULL, volume, 2.15, -1);
+    return grid_exv_core(atoms, ff, n, waters, m, out_xyz, cap, NULL, NULL, volume, 2.15, -1, 1.0);
 }
 
 int orc_grid_exv_surface(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
                          double* interior_xyz, double* surface_xyz, int cap, int* n_surface, double* volume,
-                         double rvol, double ra) {
-    return grid_exv_core(atoms, ff, n, waters, m, interior_xyz, cap, surface_xyz, n_surface, volume, rvol, ra);
+                         double rvol, double ra, double surface_thickness) {
+    return grid_exv_core(atoms, ff, n, waters, m, interior_xyz, cap, surface_xyz, n_surface, volume, rvol, ra,
+                         surface_thickness);
 }
 
 /* grid pair loops: xx self (factor 2, 1D), ax atoms x dummies (factor 1,

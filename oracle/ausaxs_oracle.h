@@ -84,11 +84,14 @@ int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int*
  * (RadialLineGenerator.cpp:15-63). interior_xyz / surface_xyz: cap x 3 each.
  * rvol: settings::grid::min_exv_radius (2.15 by default); ra < 0: each
  * atom's van der Waals radius, else ra for every atom (GridDebug of the
- * reference's grid tests). Returns the interior count (or -1 if cap is too
- * small); *n_surface. */
+ * reference's grid tests). surface_thickness: settings::grid::exv::
+ * surface_thickness (A, cell width 1); other than one cell, the non-unity
+ * branch (RawGridWithSurfaceExv.cpp:137-190) grows the surface by the
+ * dummies within int(round(thickness) * 0.5) cells of it. Returns the
+ * interior count (or -1 if cap is too small); *n_surface. */
 int orc_grid_exv_surface(const double* atoms, const int32_t* ff, int n, const double* waters, int m,
                          double* interior_xyz, double* surface_xyz, int cap, int* n_surface, double* volume,
-                         double rvol, double ra);
+                         double rvol, double ra, double surface_thickness);
 
 /* HistogramManagerMTFFGrid::calculate_all (HistogramManagerMTFFGrid.cpp:43-234)
  * on top of the FFAvg manager. atoms/waters: n x 4 floats {x,y,z,ff} as for
