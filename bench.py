@@ -116,9 +116,12 @@ def bench_structure(cfg, args):
     st = A.Structure(mode, atoms, waters, settings, z_exv=1.0)
     log(f"resident structure built in {time.perf_counter() - t0:.2f} s")
     st.set_timing(True)
+    last = {}
 
     def step():
-        return st.calculate().debye_transform(q)
+        h = st.calculate()
+        last["h"] = h
+        return h.debye_transform(q)
 
     res = timed_steps(step, args)
     tm = st.last_timing()
@@ -137,14 +140,29 @@ def bench_structure(cfg, args):
         def run(frac):
             ns, ms = int(len(atoms) * frac), int(len(w) * frac)
             h = O.hist_ff(omode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, th)
-            O.debye_ff(omode, h, q, cfg["bw"])
-            return ns, ms
+            I, _ = O.debye_ff(omode, h, q, cfg["bw"])
+            return ns, ms, h, I
 
-        frac, dt, (ns, ms) = grow_sample(run, len(atoms), 12000 / len(atoms), budget)
+        frac, dt, (ns, ms, h_cpu, I_cpu) = grow_sample(run, len(atoms), 12000 / len(atoms), budget)
         pairs = pairs_of(ns, ms)
         sample = "full workload" if frac >= 1.0 else f"first {ns} atoms + {ms} waters of the workload"
-        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
-                    sample=f"{sample} ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
+        out = dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
+                   sample=f"{sample} ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
+        if frac >= 1.0:
+            # the GPU step's result against the CPU restatement on the same input:
+            # pair counts per bin and form-factor row, and I(q) (bf16-split Debye)
+            try:
+                h = last["h"]
+                same = h.max_bin == h_cpu["max_bin"] and all(
+                    np.array_equal(h.get(which)[1], h_cpu[key]["count"])
+                    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww")))
+                I_gpu = h.debye_transform(q)
+                err = float(np.max(np.abs(I_gpu - I_cpu) / np.abs(I_cpu)))
+                out["parity"] = {"counts_identical": bool(same), "iq_max_rel_err": err, "iq_tolerance": 1e-4}
+                log(f"GPU vs CPU: counts identical {same}, I(q) max relative error {err:.2e}")
+            except Exception as e:  # the baseline line must still be printed
+                out["parity"] = {"error": str(e)}
+        return out
 
     return res, cpu
 
