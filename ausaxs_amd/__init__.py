@@ -50,6 +50,17 @@ class _Settings(C.Structure):
     ]
 
 
+class _HydrateSettings(C.Structure):
+    _fields_ = [
+        ("atomic_radius", C.c_double),
+        ("hydration_radius", C.c_double),
+        ("shell_correction", C.c_double),
+        ("explicit_grid", C.c_int),
+        ("grid_min", C.c_double * 3),
+        ("grid_max", C.c_double * 3),
+    ]
+
+
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(
@@ -69,7 +80,9 @@ def _load():
         ),
         "ausaxs_structure_calculate": (C.c_int, [vp, P(vp)]),
         "ausaxs_structure_create_grid_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
-        "ausaxs_hydrate_radial": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, P(C.c_int), vp]),
+        "ausaxs_hydrate_radial": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, P(C.c_int), vp,
+                                            P(_HydrateSettings)]),
+        "ausaxs_default_hydrate_settings": (None, [P(_HydrateSettings)]),
         "ausaxs_grid_exv_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, P(C.c_int), vp, C.c_int,
                                               P(C.c_int), C.c_double]),
         "ausaxs_structure_free": (C.c_int, [vp]),
@@ -101,6 +114,7 @@ def _load():
                                           P(C.c_int), P(C.c_double), P(C.c_longlong), P(C.c_longlong)]),
         "ausaxs_server_free": (C.c_int, [vp]),
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
+        "ausaxs_cubic_spline": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int),
                                          P(C.c_longlong), P(C.c_longlong)]),
@@ -321,9 +335,16 @@ def debye_contract(H, d, q, terms=3):
 
 
 def bin_check(lo_bits, hi_bits, bin_width):
-    m = np.zeros(3, np.uint64)
+    """Exhaustive device check over float d2 bit patterns [lo_bits, hi_bits]:
+    (bins differing from the correctly rounded rule, device sqrt results not
+    correctly rounded, tie-fallback corrections differing from it, offset codes
+    more than one unit from the real root's, mean code error against the real
+    root in units, the same mean for the correctly rounded root)."""
+    m = np.zeros(6, np.uint64)
     _check(lib.ausaxs_bin_check(lo_bits, hi_bits, bin_width, _ptr(m)))
-    return int(m[0]), int(m[1]), int(m[2])
+    n = hi_bits - lo_bits + 1
+    mean = [float(np.int64(np.uint64(v).view(np.int64))) / 65536.0 / n for v in (m[4], m[5])]
+    return int(m[0]), int(m[1]), int(m[2]), int(m[3]), mean[0], mean[1]
 
 
 def synth_globule(seed, n, axes, r_excl):
@@ -398,13 +419,31 @@ class GridSurfaceStructure(Structure):
         self._s = h
 
 
-def hydrate(atoms_xyz, ff, bodies=None, noise=None):
+def hydrate(atoms_xyz, ff, bodies=None, noise=None, atomic_radius=None, hydration_radius=None,
+            shell_correction=None, grid_limits=None):
     """Radial hydration shell (RadialHydration with NoCulling); host-only.
 
     bodies: body sizes (None: one body). noise: None (no displacement, as the
     reference's own tests use), an int seed (N(0, 0.75) per component, the
     reference's default distribution, from numpy), or an (n, 26, 3) array.
+    atomic_radius / hydration_radius / shell_correction: the grid's radii and
+    settings::hydrate::shell_correction (None: the reference's defaults);
+    grid_limits: ((xmin, ymin, zmin), (xmax, ymax, zmax)) for a grid of
+    Grid(Limit3D) instead of the one built from the atoms.
     Returns (waters n x 3 FP64 in placement order, waters per body)."""
+    hs = _HydrateSettings()
+    lib.ausaxs_default_hydrate_settings(C.byref(hs))
+    if atomic_radius is not None:
+        hs.atomic_radius = float(atomic_radius)
+    if hydration_radius is not None:
+        hs.hydration_radius = float(hydration_radius)
+    if shell_correction is not None:
+        hs.shell_correction = float(shell_correction)
+    if grid_limits is not None:
+        hs.explicit_grid = 1
+        for k in range(3):
+            hs.grid_min[k] = float(grid_limits[0][k])
+            hs.grid_max[k] = float(grid_limits[1][k])
     a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(ff, np.int32).reshape(-1)
     n = len(a)
@@ -416,7 +455,7 @@ def hydrate(atoms_xyz, ff, bodies=None, noise=None):
     m = C.c_int()
     per = np.zeros(len(b), np.int32)
     _check(lib.ausaxs_hydrate_radial(_ptr(a), _ptr(f), n, _ptr(b), len(b), None if nz is None else _ptr(nz),
-                                     _ptr(out), len(out), C.byref(m), _ptr(per)))
+                                     _ptr(out), len(out), C.byref(m), _ptr(per), C.byref(hs)))
     return out[: m.value].copy(), per
 
 
@@ -753,6 +792,16 @@ class Server:
                                        C.byref(pairs), C.byref(kp)))
         return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
                     peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value, kernel_pairs=kp.value)
+
+
+def cubic_spline(x, y, z):
+    """math::CubicSpline: the natural cubic spline through (x, y) at z (host only)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    out = np.zeros(len(z), np.float64)
+    _check(lib.ausaxs_cubic_spline(_ptr(x), _ptr(y), len(x), _ptr(z), len(z), _ptr(out)))
+    return out
 
 
 def lsq(x, y, err=None):

@@ -186,15 +186,33 @@ int ausaxs_grid_exv_surface(const double* atoms_xyz, const int32_t* atom_ff, int
     });
 }
 
+void ausaxs_default_hydrate_settings(ausaxs_hydrate_settings* s) {
+    if (!s) return;
+    const HydrateSettings d;
+    s->atomic_radius = d.atomic_radius;
+    s->hydration_radius = d.hydration_radius;
+    s->shell_correction = d.shell_correction;
+    s->explicit_grid = 0;
+    for (int k = 0; k < 3; ++k) s->grid_min[k] = s->grid_max[k] = 0;
+}
+
 int ausaxs_hydrate_radial(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const int* body_sizes,
                           int n_bodies, const double* noise, double* waters_xyz, int capacity, int* n_waters,
-                          int* per_body) {
+                          int* per_body, const ausaxs_hydrate_settings* settings) {
     return guarded([&] {
         require(n_atoms >= 0 && n_bodies >= 0, "negative sizes");
         require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
         require(n_bodies == 0 || body_sizes, "body_sizes is NULL");
         std::vector<int> bodies(body_sizes, body_sizes + n_bodies);
-        Hydration h = hydrate_radial(atoms_xyz, atom_ff, n_atoms, bodies, noise);
+        HydrateSettings hs;
+        if (settings) {
+            hs.atomic_radius = settings->atomic_radius;
+            hs.hydration_radius = settings->hydration_radius;
+            hs.shell_correction = settings->shell_correction;
+            hs.explicit_grid = settings->explicit_grid != 0;
+            for (int k = 0; k < 3; ++k) hs.grid_min[k] = settings->grid_min[k], hs.grid_max[k] = settings->grid_max[k];
+        }
+        Hydration h = hydrate_radial(atoms_xyz, atom_ff, n_atoms, bodies, noise, hs);
         const int m = (int)(h.waters.size() / 3);
         if (n_waters) *n_waters = m;
         if (per_body) std::copy(h.body_count.begin(), h.body_count.end(), per_body);
@@ -644,10 +662,10 @@ int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long l
         require(mism && lo <= hi, "bad argument");
         Context& ctx = Context::get();
         DevBuf<unsigned long long> d;
-        d.ensure(3);
-        check(hipMemsetAsync(d.p, 0, 3 * sizeof(unsigned long long), ctx.stream()), "memset");
+        d.ensure(6);
+        check(hipMemsetAsync(d.p, 0, 6 * sizeof(unsigned long long), ctx.stream()), "memset");
         check(launch_bin_check(lo, hi, (float)(1.0 / bin_width), d.p, ctx.stream()), "bin_check_kernel");
-        check(hipMemcpyAsync(mism, d.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+        check(hipMemcpyAsync(mism, d.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
         ctx.sync();
     });
 }
@@ -801,6 +819,16 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
 
 int ausaxs_server_free(ausaxs_server* s) {
     return guarded([&] { delete s; });
+}
+
+int ausaxs_cubic_spline(const double* x, const double* y, int n, const double* z, int m, double* out) {
+    return guarded([&] {
+        require(n >= 0 && m >= 0, "negative sizes");
+        require((x && y) || n == 0, "x or y is NULL");
+        require((z && out) || m == 0, "z or out is NULL");
+        const CubicSpline sp(std::vector<double>(x, x + n), std::vector<double>(y, y + n));
+        for (int i = 0; i < m; ++i) out[i] = sp(z[i]);
+    });
 }
 
 int ausaxs_lsq(const double* x, const double* y, const double* err, int n, int n_models, double* out) {

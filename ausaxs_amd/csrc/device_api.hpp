@@ -11,21 +11,41 @@
 namespace ausaxs::gpu {
 
 // Fixed-point encoding of one LDS/partial histogram cell (one uint64). Each
-// pair adds kCellUnit + k, with k = round(delta * 2^20) + 2^19 in [0, 2^20],
-// delta = x - rint(x), x = d * inv_width (bin units). The low word of the
-// addend is the bit pattern of the float fma(delta, 2^20, 1.5625 * 2^23),
-// kLowBias + k, so the kernel adds it without a float-to-int conversion. A
-// cell holding n pairs is T = n * (2^42 + kLowBias) + sum k with sum k <= n * 2^20
-// < 2^42 + kLowBias, so n = floor(T / (2^42 + kLowBias)) and T stays below
-// 2^64 while n <= kMaxPairsPerItem. The mean distance of a bin is known to
-// 2^-21 bins even when every pair in it shares one delta (lattices, symmetric
-// test bodies). Integer accumulation keeps every histogram bit-reproducible.
-constexpr int kDeltaBits = 20;
+// pair adds kCellUnit + k, with k = round(offset / 2^-E) + kDeltaBias, where
+// offset = d - b / inv_w is the pair's distance from its bin centre in A and
+// E = delta_exp(inv_w) (|offset| <= 0.5 bins = 2^19 units, plus the pair
+// kernel's Newton correction of at most 2^13 units). The low word of the
+// addend is the bit pattern of a float in [1.5 * 2^23, 2^24), where floats are
+// the integers, kLowBias + k, so the kernel adds it without a float-to-int
+// conversion. A cell holding n pairs is T = n * (2^42 + kLowBias) + sum k with
+// sum k <= n * kDeltaMax < 2^42 + kLowBias, so n = floor(T / (2^42 +
+// kLowBias)) and T stays below 2^64 while n <= kMaxPairsPerItem. The mean
+// distance of a bin is known to half a unit (2^-21 bins or finer) even when
+// every pair in it shares one offset (lattices, symmetric test bodies).
+// Integer accumulation keeps every histogram bit-reproducible.
+constexpr int64_t kDeltaBias = (int64_t(1) << 19) + (int64_t(1) << 13);
+constexpr int64_t kDeltaMax = (int64_t(1) << 20) + (int64_t(1) << 14);
 constexpr uint64_t kLowBias = 0x4B400000u;  // bits of 1.5 * 2^23
 constexpr uint64_t kCellUnit = (uint64_t(1) << 42) + kLowBias;
-constexpr int64_t kDeltaOffset = int64_t(1) << (kDeltaBits - 1);
-constexpr int64_t kMaxPairsPerItem = 4193000;  // 4193000 * (kCellUnit + 2^20) < 2^64
-constexpr double kDeltaScale = 1.0 / (double)(int64_t(1) << kDeltaBits);
+constexpr int64_t kMaxPairsPerItem = 4128768;  // 512 x 8064; * kDeltaMax < kCellUnit
+static_assert((uint64_t)kMaxPairsPerItem * (uint64_t)kDeltaMax < kCellUnit, "cell count decode");
+static_assert((uint64_t)kMaxPairsPerItem <= ~uint64_t(0) / (kCellUnit + (uint64_t)kDeltaMax), "cell range");
+// Pair distances reach at most this many bins (the bound on the Newton
+// correction above); larger extents are rejected on the host.
+constexpr int kMaxExtentBins = 1 << 15;
+
+// Exponent of the offset unit 2^-E A: the largest E with 2^E / inv_w <= 2^20,
+// so that half a bin is at most 2^19 units (E = 23 at 0.1 A, 22 at 0.25 A).
+__host__ __device__ inline int delta_exp(float inv_w) {
+    int ex = 0;
+    (void)frexpf(inv_w, &ex);  // inv_w = m 2^ex, m in [0.5, 1)
+    return ex + 19;
+}
+__host__ __device__ inline double delta_unit(float inv_w) { return ldexp(1.0, -delta_exp(inv_w)); }
+// Sum of the distances (A) of n pairs in bin b whose offsets sum to ds units.
+__host__ __device__ inline double distance_sum(uint64_t n, int b, int64_t ds, float inv_w) {
+    return (double)n * (double)b / (double)inv_w + (double)ds * delta_unit(inv_w);
+}
 
 // One workgroup's rectangle of the pair space.
 struct PairWork {
@@ -117,10 +137,15 @@ struct StripLaunch {
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
 
 // Exhaustive check of the binning arithmetic over float d2 in [lo, hi] (bit
-// patterns): counts lanes where the fast path (raw v_sqrt + tie fallback)
-// disagrees with the correctly rounded reference, and where the device's
-// correctly rounded sqrt is not the IEEE result (checked in FP64).
-hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* mismatches,
+// patterns). out[0]: lanes where the pair kernel's bin (raw v_sqrt + tie
+// fallback, scaled coordinates) differs from the correctly rounded reference;
+// out[1]: where the device's correctly rounded sqrt is not the IEEE result
+// (checked in FP64); out[2]: where the fallback's residual correction is not
+// it; out[3]: where the pair kernel's offset code is more than 1 + 2^-24 q
+// units (q = units per bin) from the exact offset of the real root (FP64); out[4]: the sum of those code
+// errors in units of 2^-16 (two's complement); out[5]: the same sum for the
+// correctly rounded root's offset (the reference's distance).
+hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* out,
                             hipStream_t s);
 
 // Plain copy of `bytes` (a multiple of 16) from src to dst: the reference

@@ -57,12 +57,19 @@ int bin_bound(const std::vector<const PointSet*>& sets, float inv_w) {
     for (int k = 0; k < 3; ++k) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
     const double x = std::sqrt(d2) * (double)inv_w * (1 + 1e-5) + 2;
     if (!(x < 1e9)) throw std::runtime_error("ausaxs_hip: coordinates are not finite or too far apart");
+    // the pair kernel's offset codes hold the Newton correction of pairs up
+    // to kMaxExtentBins bins apart
+    if (x > kMaxExtentBins)
+        throw std::runtime_error("ausaxs_hip: the structure spans more than " + std::to_string(kMaxExtentBins) +
+                                 " bins; use wider bins");
     return (int)std::ceil(x);
 }
 
 void check_tile_j(int tile_j) {
     if (tile_j < 1 || (int64_t)kTileI * tile_j > kMaxPairsPerItem)
-        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item within 4,193,000 pairs (tile_j <= 8189)");
+        throw std::runtime_error("ausaxs_hip: tile_j must keep a work item within " +
+                                 std::to_string(kMaxPairsPerItem) + " pairs (tile_j <= " +
+                                 std::to_string(kMaxPairsPerItem / kTileI) + ")");
 }
 
 int item_rows(const std::vector<PairWork>& items) {

@@ -8,47 +8,61 @@
 
 namespace ausaxs::gpu {
 
+// Natural cubic spline through the knots (x_k, y_k): S is a cubic on every
+// interval, S, S' and S'' are continuous at the inner knots and S'' = 0 at
+// both ends. With h_k = x_{k+1} - x_k, the chord slopes s_k = (y_{k+1} - y_k)
+// / h_k and the knot curvatures M_k = S''(x_k), continuity of S' gives the
+// tridiagonal system
+//   h_{k-1} M_{k-1} + 2 (h_{k-1} + h_k) M_k + h_k M_{k+1} = 6 (s_k - s_{k-1})
+// for the inner knots, M_0 = M_{n-1} = 0, solved here by forward elimination
+// and back substitution (diagonally dominant, no pivoting). On interval k,
+// S(x_k + t) = y_k + t (a1 + t (a2 + t a3)) with a1 = s_k - h_k (2 M_k +
+// M_{k+1}) / 6, a2 = M_k / 2, a3 = (M_{k+1} - M_k) / (6 h_k).
 CubicSpline::CubicSpline(const std::vector<double>& x, const std::vector<double>& y) : x_(x), y_(y) {
     const int n = (int)x.size();
     if (n != (int)y.size()) throw std::runtime_error("ausaxs_hip: spline x and y differ in size");
     if (n < 4) throw std::runtime_error("ausaxs_hip: a spline needs at least four points");
-    // tridiagonal system for the knot slopes b (CubicSpline.cpp, eqs 15-27)
-    std::vector<double> h(n - 1), p(n - 1), D(n), Q(n - 1), B(n);
-    for (int i = 0; i < n - 1; ++i) {
-        h[i] = x[i + 1] - x[i];
-        p[i] = (y[i + 1] - y[i]) / h[i];
+    const int m = n - 1;  // intervals
+    std::vector<double> h(m), slope(m);
+    for (int k = 0; k < m; ++k) {
+        h[k] = x[k + 1] - x[k];
+        if (!(h[k] > 0)) throw std::runtime_error("ausaxs_hip: spline knots must increase strictly");
+        slope[k] = (y[k + 1] - y[k]) / h[k];
     }
-    D[0] = 2, Q[0] = 1, B[0] = 3 * p[0];
-    D[n - 1] = 2, B[n - 1] = 3 * p[n - 2];
-    for (int i = 0; i < n - 2; ++i) {
-        D[i + 1] = 2 * h[i] / h[i + 1] + 2;
-        Q[i + 1] = h[i] / h[i + 1];
-        B[i + 1] = 3 * (p[i] + p[i + 1] * h[i] / h[i + 1]);
+    // inner unknowns M_1 .. M_{n-2}: diagonal, super-diagonal (after
+    // elimination) and right-hand side
+    std::vector<double> curv(n, 0.0), diag(n, 0.0), upper(n, 0.0), rhs(n, 0.0);
+    for (int k = 1; k < m; ++k) {
+        diag[k] = 2 * (h[k - 1] + h[k]);
+        upper[k] = h[k];
+        rhs[k] = 6 * (slope[k] - slope[k - 1]);
+        if (k > 1) {
+            // eliminate the sub-diagonal entry h_{k-1} with row k-1
+            const double f = h[k - 1] / diag[k - 1];
+            diag[k] -= f * upper[k - 1];
+            rhs[k] -= f * rhs[k - 1];
+        }
     }
-    for (int i = 1; i < n; ++i) {
-        D[i] -= Q[i - 1] / D[i - 1];
-        B[i] -= B[i - 1] / D[i - 1];
-    }
-    b_.assign(n, 0.0);
-    c_.assign(n - 1, 0.0);
-    d_.assign(n - 1, 0.0);
-    b_[n - 1] = B[n - 1] / D[n - 1];
-    for (int i = n - 2; i >= 0; --i) b_[i] = (B[i] - Q[i] * b_[i + 1]) / D[i];
-    for (int i = 0; i < n - 1; ++i) {
-        c_[i] = (-2 * b_[i] - b_[i + 1] + 3 * p[i]) / h[i];
-        d_[i] = (b_[i] + b_[i + 1] - 2 * p[i]) / std::pow(h[i], 2);
+    for (int k = m - 1; k >= 1; --k) curv[k] = (rhs[k] - (k + 1 < m ? upper[k] * curv[k + 1] : 0.0)) / diag[k];
+    a1_.resize(m);
+    a2_.resize(m);
+    a3_.resize(m);
+    for (int k = 0; k < m; ++k) {
+        a1_[k] = slope[k] - h[k] * (2 * curv[k] + curv[k + 1]) / 6;
+        a2_[k] = curv[k] / 2;
+        a3_[k] = (curv[k + 1] - curv[k]) / (6 * h[k]);
     }
 }
 
 double CubicSpline::operator()(double z) const {
-    // the last knot at or below z (the first knot below the range), and the
-    // last interval for z at or past the end
+    // the interval whose left knot is the last one at or below z; below the
+    // first knot the first interval and at or past the last knot the last
+    // one, whose cubics continue outside the knots
     const int n = (int)x_.size();
-    int i = (int)(std::upper_bound(x_.begin(), x_.end(), z) - x_.begin()) - 1;
-    i = std::max(i, 0);
-    if (i == n - 1) --i;
-    const double t = z - x_[i];
-    return y_[i] + b_[i] * t + c_[i] * std::pow(t, 2) + d_[i] * std::pow(t, 3);
+    int k = (int)(std::upper_bound(x_.begin(), x_.end(), z) - x_.begin()) - 1;
+    k = std::min(std::max(k, 0), n - 2);
+    const double t = z - x_[k];
+    return y_[k] + t * (a1_[k] + t * (a2_[k] + t * a3_[k]));
 }
 
 namespace {

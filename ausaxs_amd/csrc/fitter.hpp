@@ -18,15 +18,17 @@
 
 namespace ausaxs::gpu {
 
-// math::CubicSpline (CubicSpline.cpp): the reference's spline with its
-// end conditions (D[0] = D[n-1] = 2, Q[0] = 1) and interval search.
+// The natural cubic spline of math::CubicSpline (CubicSpline.cpp): zero
+// curvature at both end knots, evaluated on the interval of the last knot at
+// or below the argument (the end intervals' cubics continue outside).
 class CubicSpline {
   public:
     CubicSpline(const std::vector<double>& x, const std::vector<double>& y);
     double operator()(double z) const;
 
   private:
-    std::vector<double> x_, y_, b_, c_, d_;
+    std::vector<double> x_, y_;
+    std::vector<double> a1_, a2_, a3_;  // per interval: S = y_k + a1 t + a2 t^2 + a3 t^3
 };
 
 // Free parameters in the reference's order (SmartFitter.cpp:19-30); bit k of

@@ -438,7 +438,6 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
                                  std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
                                  " A); increase bin_count or bin_width");
-    const double inv_w = (double)engine_->inv_w();
     const size_t nb = nbins;
 
     // entry of one global row and bin, times the reference's template factor
@@ -448,7 +447,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         WeightedEntry e;
         e.value = (double)factor * (double)c;
         e.count = (uint32_t)(factor * c);
-        e.bin_center = factor * (((double)c * b + (double)s * kDeltaScale) / inv_w);
+        e.bin_center = factor * distance_sum(c, b, s, engine_->inv_w());
         return e;
     };
 
@@ -1030,7 +1029,7 @@ void raw_pair_histogram(const float* a, int n, const float* b, int m, const Hist
     if (eng.overflowed(c)) throw std::runtime_error("ausaxs_hip: pair distance beyond bin_count*bin_width");
     for (int k = 0; k < nbins; ++k) {
         count[k] = c[k];
-        dsum[k] = ((double)c[k] * k + (double)ds[k] * kDeltaScale) / (double)inv_w;
+        dsum[k] = distance_sum(c[k], (int)k, ds[k], inv_w);
     }
 }
 

@@ -14,14 +14,12 @@ namespace ausaxs::gpu {
 namespace {
 // grid::detail::State bits (GridObj.h)
 constexpr uint8_t kACenter = 1, kAArea = 2, kWCenter = 4, kWArea = 8, kVolume = 16;
-constexpr double kRvol = 2.15;              // settings::grid::min_exv_radius
-constexpr double kHydrationRadius = 1.71;   // Grid::get_hydration_radius: vdw radius of O
-constexpr double kShellCorrection = -0.35;  // settings::hydrate::shell_correction
-constexpr double kMinScore = 0.25;          // settings::grid::detail::min_score
+constexpr double kRvol = 2.15;      // settings::grid::min_exv_radius
+constexpr double kMinScore = 0.25;  // settings::grid::detail::min_score
 }  // namespace
 
 Hydration hydrate_radial(const double* atoms, const int32_t* ff, int n, const std::vector<int>& bodies,
-                         const double* noise) {
+                         const double* noise, const HydrateSettings& hs) {
     Hydration out;
     if (n < 0) throw std::runtime_error("ausaxs_hip: negative atom count");
     std::vector<int> sizes = bodies.empty() ? std::vector<int>{n} : bodies;
@@ -33,25 +31,44 @@ Hydration hydrate_radial(const double* atoms, const int32_t* ff, int n, const st
     if (total != n) throw std::runtime_error("ausaxs_hip: body sizes do not add up to the atom count");
     out.body_count.assign(sizes.size(), 0);
     if (n == 0) return out;
-    for (int i = 0; i < n; ++i) (void)vdw_radius(ff[i]);  // throws on types without a radius
+    if (!(hs.hydration_radius > 0) || !std::isfinite(hs.shell_correction) || !(hs.atomic_radius < 64) ||
+        !(hs.hydration_radius < 64))
+        throw std::runtime_error("ausaxs_hip: hydration radii must be positive and finite");
+    // Grid::get_atomic_radius (throws on types without a radius unless overridden)
+    std::vector<double> ra(n);
+    for (int i = 0; i < n; ++i) ra[i] = hs.atomic_radius < 0 ? vdw_radius(ff[i]) : hs.atomic_radius;
 
     // the molecule's grid from its atoms (GridBasedHydration::initialize creates
-    // it after clear_hydration), Grid::add marking the atom centres
-    const GridGeometry geo = grid_geometry(atoms, n, nullptr, 0);
-    const int B[3] = {geo.bins[0], geo.bins[1], geo.bins[2]};
+    // it after clear_hydration), or the caller's limits (Grid(Limit3D): span /
+    // width bins per axis); Grid::add marking the atom centres
+    double origin[3];
+    int B[3];
+    if (hs.explicit_grid) {
+        for (int k = 0; k < 3; ++k) {
+            const double span = hs.grid_max[k] - hs.grid_min[k];
+            if (!(span >= 1) || !(span <= 4096)) throw std::runtime_error("ausaxs_hip: grid limits must span 1-4096 A");
+            origin[k] = hs.grid_min[k];
+            B[k] = (int)(span / 1.0);  // Axis3D(limits, width): bins = span / width
+        }
+    } else {
+        const GridGeometry geo = grid_geometry(atoms, n, nullptr, 0);
+        for (int k = 0; k < 3; ++k) origin[k] = geo.origin[k], B[k] = geo.bins[k];
+    }
     std::vector<uint8_t> g((size_t)B[0] * B[1] * B[2], 0);
     auto at = [&](int i, int j, int k) -> uint8_t& { return g[((size_t)i * B[1] + j) * B[2] + k]; };
-    auto to_bin = [&](double v, int k) { return (int)std::round((v - geo.origin[k]) / 1.0); };
-    auto to_coord = [&](int i, int k) { return geo.origin[k] + i * 1.0; };
+    auto to_bin = [&](double v, int k) { return (int)std::round((v - origin[k]) / 1.0); };
+    auto to_coord = [&](int i, int k) { return origin[k] + i * 1.0; };
+    auto oob = [&](int i, int j, int k) { return i < 0 || i >= B[0] || j < 0 || j >= B[1] || k < 0 || k >= B[2]; };
     std::vector<std::array<int, 3>> loc(n);
     for (int a = 0; a < n; ++a) {
         for (int k = 0; k < 3; ++k) loc[a][k] = to_bin(atoms[3 * a + k], k);
+        if (oob(loc[a][0], loc[a][1], loc[a][2])) throw std::runtime_error("ausaxs_hip: an atom lies outside the grid");
         at(loc[a][0], loc[a][1], loc[a][2]) |= kACenter;
     }
     // Grid::expand_volume: SphericalExpander<true, false> per atom, in order
     // (SphericalExpander.h:29-90)
     for (int a = 0; a < n; ++a) {
-        const double rvdw = vdw_radius(ff[a]), rvdw2 = std::pow(rvdw, 2), rvol2 = std::pow(kRvol, 2);
+        const double rvdw = ra[a], rvdw2 = std::pow(rvdw, 2), rvol2 = std::pow(kRvol, 2);
         const int br = (int)std::ceil(std::max(rvdw, kRvol) / 1.0);
         int lo[3], hi[3];
         for (int k = 0; k < 3; ++k) lo[k] = std::max(loc[a][k] - br, 0), hi[k] = std::min(loc[a][k] + br + 1, B[k]);
@@ -79,7 +96,7 @@ Hydration hydrate_radial(const double* atoms, const int32_t* ff, int n, const st
 
     // RadialHydration::prepare_rotations(8): directions and the probe offsets
     // at 1, 3 and 5 shell radii
-    const double rh = kHydrationRadius + kShellCorrection;
+    const double rh = hs.hydration_radius + hs.shell_correction;
     const std::vector<std::array<double, 3>> dirs = sphere_directions(8);
     const int nd = (int)dirs.size();
     std::vector<std::array<int, 3>> p1(nd), p3(nd), p5(nd);
@@ -91,7 +108,6 @@ Hydration hydrate_radial(const double* atoms, const int32_t* ff, int n, const st
         }
     // GridObj::is_only_empty_or_volume
     auto open = [](uint8_t s) { return s == 0 || s == kVolume; };
-    auto oob = [&](int i, int j, int k) { return i < 0 || i >= B[0] || j < 0 || j >= B[1] || k < 0 || k >= B[2]; };
     // RadialHydration::collision_check (RadialHydration.cpp:141-206)
     auto collision_check = [&](int x, int y, int z) {
         int score = 0;
@@ -134,12 +150,12 @@ Hydration hydrate_radial(const double* atoms, const int32_t* ff, int n, const st
     };
 
     // generate_explicit_hydration per body (RadialHydration::global() is false)
-    const double rw2 = std::pow(kHydrationRadius, 2);
-    const int brw = (int)std::ceil(kHydrationRadius / 1.0);
+    const double rw2 = std::pow(hs.hydration_radius, 2);
+    const int brw = (int)std::ceil(hs.hydration_radius / 1.0);
     int a0 = 0;
     for (size_t b = 0; b < sizes.size(); ++b) {
         for (int a = a0; a < a0 + sizes[b]; ++a) {
-            const double reff = vdw_radius(ff[a]) + rh;
+            const double reff = ra[a] + rh;
             for (int d = 0; d < nd; ++d) {
                 double w[3];
                 for (int c = 0; c < 3; ++c) {

@@ -34,8 +34,8 @@ namespace {
 
 using binning::bin_of;
 using binning::bin_pair;
-using binning::kTieSlope;
-using binning::quantize;
+using binning::Consts;
+using binning::make_consts;
 
 __global__ __launch_bounds__(256) void lattice_zero_kernel(double2* __restrict__ buf, size_t n) {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
@@ -199,17 +199,20 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(FftPass p) {
 }
 
 // Pair counts per squared displacement -> one histogram row: bin and offset
-// of d2 = s (exact in float below 2^24) as the pair kernel bins a body pair.
+// of d2 = s (exact in float below 2^24) as the pair kernel bins a body pair
+// (on d2 scaled by the pair loop's power of four).
 __global__ __launch_bounds__(256) void lattice_bins_kernel(const unsigned long long* __restrict__ scount, int smax,
                                                            float inv_w, int nbins, uint64_t* __restrict__ cnt,
                                                            int64_t* __restrict__ dsum) {
+    const Consts cs = make_consts(inv_w);
+    const float sc2 = cs.scale * cs.scale;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s <= smax; s += gridDim.x * blockDim.x) {
         const unsigned long long c = scount[s];
         if (!c) continue;
-        float t, dl;
-        bin_pair((float)s, inv_w, kTieSlope * inv_w, t, dl);
+        float t;
+        const uint32_t code = bin_pair((float)s * sc2, cs, t);
         const unsigned bin = min(bin_of(t), (unsigned)nbins);
-        const int64_t k = (int64_t)(quantize(dl) - (uint32_t)kLowBias) - kDeltaOffset;
+        const int64_t k = (int64_t)(code - (uint32_t)kLowBias) - kDeltaBias;
         atomicAdd((unsigned long long*)&cnt[bin], c);
         atomicAdd((unsigned long long*)&dsum[bin], (unsigned long long)((int64_t)c * k));
     }

@@ -17,13 +17,17 @@
 //   d2 = (dx*dx + dy*dy) + dz*dz   separately rounded (file built with
 //                                  -ffp-contract=off)
 //   d  = sqrt_rn(d2), x = d*inv_w, bin = rint(x)  (round half to even)
-// v_sqrt_f32 is accurate to 1 ulp but not correctly rounded; its root s is
-// used when x is more than 1.25 * 2^-22 * x away from a .5 tie, and the bin
-// then comes from one FMA, t = s * inv_w + 1.5 * 2^23, whose bit pattern is the
-// bin plus a constant (folded into the LDS base address). Inside the window the
-// correctly rounded square root (v_sqrt_f32 plus an FMA-residual correction;
-// note that __fsqrt_rn lowers to the uncorrected instruction) decides.
-// launch_bin_check verifies both exhaustively on device.
+// The loop runs on coordinates scaled by a power of two (binning.hpp), which
+// scales d2 and the roots exactly. The root s = d2 * rsq(d2) (within 1.5 ulp
+// of the real root) is used when x is more than 1.75 * 2^-22 * x away from a
+// .5 tie, and the bin then comes from one FMA, t = s * inv_w + 1.5 * 2^23,
+// whose bit pattern is the bin plus a constant (folded into the LDS base
+// address). Inside the window the correctly rounded square root (v_sqrt_f32
+// plus an FMA-residual correction; note that __fsqrt_rn lowers to the
+// uncorrected instruction) decides. The offset that goes into the bin's
+// distance sum is that of the real root (s plus its Newton correction, two
+// FMAs with the same rsq), not of s, whose error would bias the bin centres.
+// launch_bin_check verifies the bins and the offsets exhaustively on device.
 #include "binning.hpp"
 #include "device_api.hpp"
 
@@ -34,10 +38,11 @@ using binning::bin_fast;
 using binning::bin_of;
 using binning::bin_pair;
 using binning::bin_tie;
+using binning::Consts;
 using binning::kMagic;
 using binning::kMagicBits;
-using binning::kTieSlope;
-using binning::quantize;
+using binning::make_consts;
+using binning::offset_code;
 using binning::simd_rule_exact;
 using binning::sqrt_fix;
 using binning::sqrt_rn;
@@ -65,9 +70,8 @@ __device__ __forceinline__ float simd_d2(float dx, float dy, float dz) {
     return (sx + sy) + sz;
 }
 
-__device__ __forceinline__ uint64_t cell_add(float dl) {
-    return (uint64_t(1) << 42) + (uint64_t)quantize(dl);
-}
+// addend of one pair: count 2^42 plus the offset code (kLowBias + k)
+__device__ __forceinline__ uint64_t cell_add(uint32_t code) { return (uint64_t(1) << 42) + (uint64_t)code; }
 
 // count and sum of k of one cell (see device_api.hpp)
 __device__ __forceinline__ void cell_decode(uint64_t v, uint64_t& n, uint64_t& ksum) {
@@ -110,21 +114,21 @@ struct Rows {
 // at least 1), and t + sat * off1 stays an exact integer-valued float, so the
 // row choice costs two VALU operations per pair and no compare or select.
 template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT>
-__device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, float inv_w,
-                                            float slope_w, unsigned ovf, int off1) {
-    float t[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane];
+__device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, const Consts& cs,
+                                            unsigned ovf, int off1) {
+    float t[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane], s[kRowsPerLane];
     bool near[kRowsPerLane];
     bool tie = false;
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
         d2[k] = simd_d2(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z);
-        near[k] = bin_fast(d2[k], inv_w, slope_w, t[k], dl[k]);
+        near[k] = bin_fast(d2[k], cs, s[k], t[k], dl[k]);
         tie |= near[k];
     }
     if (tie) {
 #pragma unroll
         for (int k = 0; k < kRowsPerLane; ++k)
-            if (near[k]) bin_tie(d2[k], inv_w, t[k], dl[k]);
+            if (near[k]) bin_tie(d2[k], s[k], cs, t[k], dl[k]);
     }
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
@@ -144,7 +148,7 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
             if (DIAG) keep = keep && (j > r.idx[k]);
             u = keep ? u : kMagicBits + ovf + 1u;
         }
-        lds_add_at(r.base[k] + (u << 3), cell_add(dl[k]));
+        lds_add_at(r.base[k] + (u << 3), cell_add(offset_code(d2[k], s[k], dl[k], cs)));
     }
 }
 
@@ -156,7 +160,7 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
 // are needed between groups.
 template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
-                                         float4* cols, float inv_w, int nbins) {
+                                         float4* cols, const Consts& cs, int nbins) {
     const int t = threadIdx.x;
     const int part = t / kRowLanes;  // every part holds the same row points
     Rows r;
@@ -169,9 +173,9 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         const int i = w.i_begin + ((li & 63) * 8 + (li >> 6));
         r.valid[k] = FULL || i < w.i_end;
         const float4 p = r.valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        r.x[k] = p.x;
-        r.y[k] = p.y;
-        r.z[k] = p.z;
+        r.x[k] = p.x * cs.scale;
+        r.y[k] = p.y * cs.scale;
+        r.z[k] = p.z * cs.scale;
         r.tagf[k] = (float)__float_as_int(p.w);
         r.idx[k] = i;
         // kSplit: rows from the second class add into the second LDS row
@@ -180,7 +184,6 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     }
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
-    const float slope_w = kTieSlope * inv_w;
 
     // the next column tile is loaded into a register while the current one is
     // evaluated, so the global-memory latency of the tile loads stays hidden
@@ -191,7 +194,9 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     for (int jc = w.j_begin; jc < w.j_end; jc += kColTile) {
         const int nc = min(kColTile, w.j_end - jc);
         __syncthreads();  // previous tile consumed
-        if (t < nc) cols[t] = ORIENT ? make_float4(pre.x, pre.y, pre.z, (float)__float_as_int(pre.w)) : pre;
+        if (t < nc)
+            cols[t] = make_float4(pre.x * cs.scale, pre.y * cs.scale, pre.z * cs.scale,
+                                  ORIENT ? (float)__float_as_int(pre.w) : pre.w);
         __syncthreads();
         pre = pts[min(jc + kColTile + t, w.j_end - 1)];  // past the item's end: unused
         // the tile's columns split between the parts: part p takes
@@ -214,36 +219,36 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
             for (int u = 0; u < kColGroup; ++u) bq[u] = cc[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jh + c + u, hist, inv_w, slope_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jh + c + u, hist, cs, ovf, off1);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u) a[u] = cc[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jh + c + kColGroup + u, hist, inv_w, slope_w, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jh + c + kColGroup + u, hist, cs, ovf, off1);
         }
         for (int c = n2; c < n; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cc[c], jh + c, hist, inv_w, slope_w, ovf, off1);
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cc[c], jh + c, hist, cs, ovf, off1);
     }
 }
 
 template <bool CLAMP>
 __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
-                                        float4* cols, float inv_w, int nbins) {
+                                        float4* cols, const Consts& cs, int nbins) {
     const bool orient = (w.flags & kOrient) != 0;
     if (w.flags & kDiag) {
-        if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<false, true, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, cs, nbins);
+        else run_item<false, true, false, CLAMP>(pts, w, hist, cols, cs, nbins);
     } else if (w.flags & kSplit) {
         // a row tile spanning two classes of a cross job (never oriented)
-        if (w.i_end - w.i_begin >= kTileI) run_item<true, false, false, CLAMP, true>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<false, false, false, CLAMP, true>(pts, w, hist, cols, inv_w, nbins);
+        if (w.i_end - w.i_begin >= kTileI) run_item<true, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins);
+        else run_item<false, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins);
     } else if (w.i_end - w.i_begin >= kTileI) {
-        if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<true, false, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, cs, nbins);
+        else run_item<true, false, false, CLAMP>(pts, w, hist, cols, cs, nbins);
     } else {
-        if (orient) run_item<false, false, true, CLAMP>(pts, w, hist, cols, inv_w, nbins);
-        else run_item<false, false, false, CLAMP>(pts, w, hist, cols, inv_w, nbins);
+        if (orient) run_item<false, false, true, CLAMP>(pts, w, hist, cols, cs, nbins);
+        else run_item<false, false, false, CLAMP>(pts, w, hist, cols, cs, nbins);
     }
 }
 
@@ -262,7 +267,7 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 // 250k-atom workload, at the same pair rate.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const PairWork* __restrict__ items, int n_items,
-                                                             uint64_t* __restrict__ partial, float inv_w,
+                                                             uint64_t* __restrict__ partial, Consts cs,
                                                              int nbins, int clamp, int rows, int copies) {
     extern __shared__ uint64_t hist[];
     __shared__ float4 cols[kColTile + kColGroup];
@@ -279,8 +284,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         __syncthreads();  // previous item's flush done
         for (int e = threadIdx.x; e < copies * rows * stride; e += kThreads) hist[e] = 0;
         __syncthreads();
-        if (clamp) run_any<true>(pts, w, mine, cols, inv_w, nbins);
-        else run_any<false>(pts, w, mine, cols, inv_w, nbins);
+        if (clamp) run_any<true>(pts, w, mine, cols, cs, nbins);
+        else run_any<false>(pts, w, mine, cols, cs, nbins);
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
         uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
@@ -311,7 +316,7 @@ __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __res
         low_sum += k;
     }
     tcnt[(size_t)c * stride + bin] = n_sum;
-    tdsum[(size_t)c * stride + bin] = (int64_t)low_sum - (int64_t)n_sum * kDeltaOffset;
+    tdsum[(size_t)c * stride + bin] = (int64_t)low_sum - (int64_t)n_sum * kDeltaBias;
 }
 
 // Merge phase 2: output row r = sum of its chunks row_chunk[r] .. row_chunk[r+1].
@@ -336,27 +341,27 @@ __global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* __restr
 }
 
 // One strip pair with the reference's per-pair rule: bin (clamped into the
-// overflow slot nbins) and offset k - 2^19 of pair (row i, column t).
-__device__ __forceinline__ void strip_pair(const StripLaunch& p, const float4& a, const float4& c, int i, int t,
-                                           unsigned& bin, int64_t& koff) {
+// overflow slot nbins) and offset code (kLowBias + k) of pair (row i, column
+// t). The root is correctly rounded, so its offset needs no correction.
+__device__ __forceinline__ void strip_pair(const StripLaunch& p, const Consts& cs, const float4& a, const float4& c,
+                                           int i, int t, unsigned& bin, uint32_t& code) {
     const StripWork& w = p.w;
     const float dx = a.x - c.x, dy = a.y - c.y, dz = a.z - c.z;
     bool scalar_rule = true;
     if (w.self) scalar_rule = t >= w.n_cols_total - ((w.n_cols_total - i - 1) % 4);
-    float x, b, dl;
+    float b;
     if (scalar_rule) {
         // evaluate_scalar with FMA contraction and round() lowered to
         // trunc(x + 0.49999997f) (CompactCoordinatesXYZFF.h:313-329)
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
-        x = sqrt_rn(d2) * p.inv_w;
-        b = __builtin_truncf(x + 0.49999997f);
-        dl = x - b;
+        const float r = sqrt_rn(d2);
+        b = __builtin_truncf(r * p.inv_w + 0.49999997f);
+        code = __float_as_uint(__builtin_fmaf(__builtin_fmaf(r, p.inv_w, -b), cs.q, cs.m2));
     } else {
         const float d2 = (dx * dx + dy * dy) + dz * dz;
-        simd_rule_exact(d2, p.inv_w, x, b, dl);
+        code = simd_rule_exact(d2, p.inv_w, cs, b);
     }
     bin = min((unsigned)(int)b, (unsigned)p.nbins);
-    koff = (int64_t)(quantize(dl) - (uint32_t)kLowBias) - kDeltaOffset;  // k in [0, 2^20]
 }
 
 // Strip kernel: rows x tail columns, one thread per pair, global atomics.
@@ -375,11 +380,11 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     if (row < 0) return;  // the host guarantees a row for every present class pair
     row += rep * p.out_rep;
     unsigned bin;
-    int64_t koff;
-    strip_pair(p, a, c, i, t, bin, koff);
+    uint32_t code;
+    strip_pair(p, make_consts(p.inv_w), a, c, i, t, bin, code);
     const size_t cell = (size_t)row * p.stride + bin;
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
-    atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)koff);
+    atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)(code - (uint32_t)kLowBias) - kDeltaBias));
 }
 
 // Strip kernel for a strip with one output row (fixed_row) and no replicas:
@@ -397,14 +402,15 @@ __global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
     const long long total = (long long)w.n_rows * nt;
     const long long start = (long long)blockIdx.x * kStripBlockPairs;
     const long long end = min(start + kStripBlockPairs, total);
+    const Consts cs = make_consts(p.inv_w);
     for (long long tid = start + threadIdx.x; tid < end; tid += blockDim.x) {
         const int i = (int)(tid / nt);
         const int t = w.t_begin + (int)(tid % nt);
         if (w.self && i >= t) continue;
         unsigned bin;
-        int64_t koff;
-        strip_pair(p, p.rows[i], p.cols[t], i, t, bin, koff);
-        lds_add(&cells[bin], (uint64_t(1) << 42) + (uint64_t)(koff + kDeltaOffset) + kLowBias);
+        uint32_t code;
+        strip_pair(p, cs, p.rows[i], p.cols[t], i, t, bin, code);
+        lds_add(&cells[bin], cell_add(code));
     }
     __syncthreads();
     const size_t row = (size_t)p.fixed_row * p.stride;
@@ -415,13 +421,19 @@ __global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
         cell_decode(v, n, ksum);
         atomicAdd((unsigned long long*)&p.cnt[row + b], (unsigned long long)n);
         atomicAdd((unsigned long long*)&p.dsum[row + b],
-                  (unsigned long long)((int64_t)ksum - (int64_t)n * kDeltaOffset));
+                  (unsigned long long)((int64_t)ksum - (int64_t)n * kDeltaBias));
     }
 }
 
 __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi, float inv_w,
-                                                        unsigned long long* mism) {
-    unsigned long long bad_fast = 0, bad_cr = 0, bad_fix = 0;
+                                                        unsigned long long* out) {
+    unsigned long long bad_fast = 0, bad_cr = 0, bad_fix = 0, bad_off = 0;
+    long long err_sum = 0, err_cr_sum = 0;  // units of 2^-16 offset units
+    const Consts cs = make_consts(inv_w);
+    const float sc2 = cs.scale * cs.scale;
+    const double unit_inv = ldexp(1.0, delta_exp(inv_w));
+    // two roundings to whole units, and the rounding of dl and of q: 1 + 2^-24 q
+    const double off_tol = 1.0 + ldexp((double)cs.q, -24) + 1e-6;
     const uint64_t n = (uint64_t)hi - lo + 1;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (uint64_t)gridDim.x * blockDim.x) {
@@ -440,15 +452,26 @@ __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi
         if (!ok && d2 > 0.f) ++bad_cr;
         const float xr = r * inv_w;
         const float br = __builtin_rintf(xr);
-        float t, dl;
-        bin_pair(d2, inv_w, kTieSlope * inv_w, t, dl);
+        // the pair kernel's path on the scaled d2 (exact: a power of four)
+        float t;
+        const uint32_t code = bin_pair(d2 * sc2, cs, t);
         if (t - kMagic != br) ++bad_fast;
+        // its offset against the real root's, computed in FP64
+        const double b = (double)(t - kMagic);
+        const double k_true = (sqrt(dd) - b / (double)inv_w) * unit_inv + (double)kDeltaBias;
+        const double err = (double)(code - (uint32_t)kLowBias) - k_true;
+        if (fabs(err) > off_tol) ++bad_off;
+        err_sum += (long long)rint(err * 65536.0);
+        err_cr_sum += (long long)rint((rd - sqrt(dd)) * unit_inv * 65536.0);
         // the residual correction of the tie fallback, on every normal d2
         if (d2 >= 1e-30f && sqrt_fix(d2, __builtin_amdgcn_sqrtf(d2)) != r) ++bad_fix;
     }
-    if (bad_fast) atomicAdd(&mism[0], bad_fast);
-    if (bad_cr) atomicAdd(&mism[1], bad_cr);
-    if (bad_fix) atomicAdd(&mism[2], bad_fix);
+    if (bad_fast) atomicAdd(&out[0], bad_fast);
+    if (bad_cr) atomicAdd(&out[1], bad_cr);
+    if (bad_fix) atomicAdd(&out[2], bad_fix);
+    if (bad_off) atomicAdd(&out[3], bad_off);
+    if (err_sum) atomicAdd(&out[4], (unsigned long long)err_sum);
+    if (err_cr_sum) atomicAdd(&out[5], (unsigned long long)err_cr_sum);
 }
 
 // plain streaming copy of a known number of bytes (calibration of HBM traffic counters)
@@ -473,7 +496,7 @@ hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
     const size_t lds = pair_lds_bytes(p.nbins, p.rows, p.copies) - kStaticLds;
     const int grid = p.grid > 0 ? std::min(p.grid, p.n_items) : p.n_items;
     hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.items, p.n_items, p.partial,
-                       p.inv_w, p.nbins, p.clamp, p.rows, p.copies);
+                       make_consts(p.inv_w), p.nbins, p.clamp, p.rows, p.copies);
     return hipGetLastError();
 }
 
@@ -506,9 +529,9 @@ hipError_t launch_strip(const StripLaunch& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* mism,
+hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* out,
                             hipStream_t s) {
-    hipLaunchKernelGGL(bin_check_kernel, dim3(4096), dim3(256), 0, s, lo_bits, hi_bits, inv_w, mism);
+    hipLaunchKernelGGL(bin_check_kernel, dim3(4096), dim3(256), 0, s, lo_bits, hi_bits, inv_w, out);
     return hipGetLastError();
 }
 

@@ -205,7 +205,7 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
             WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
             e.value = 2.0 * (double)c * row_w_[rr];
             e.count = (uint32_t)(2 * c);
-            e.bin_center = 2.0 * (((double)c * b + (double)ds * kDeltaScale) / (double)inv_w_);
+            e.bin_center = 2.0 * distance_sum(c, b, ds, inv_w_);
             entry_add(dst[b], e);
         }
     }
@@ -272,7 +272,9 @@ int RigidScan::bound_bins(const double* T, int n_conf) const {
     // float rounding of the moved coordinates: a few ulp of their magnitude
     dmax += 1e-5 * (cmax + dmax) + 1e-3;
     const double x = dmax * (double)inv_w_ * (1 + 1e-5) + 2;
-    if (!(x < 1e9)) throw std::runtime_error("ausaxs_hip: conformations too far apart for the bin axis");
+    if (!(x < 1e9) || x > kMaxExtentBins)
+        throw std::runtime_error("ausaxs_hip: conformations span more than " + std::to_string(kMaxExtentBins) +
+                                 " bins");
     int nb = (int)std::ceil(x);
     nb = (nb + 15) / 16 * 16;  // a little slack: nearby scans reuse one plan
     return std::max(10, std::min<int>(nb, (int)s_.bin_count));
@@ -687,7 +689,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
                 WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
                 e.value = 2.0 * (double)n * row_w_[r];
                 e.count = (uint32_t)(2 * n);
-                e.bin_center = 2.0 * (((double)n * b + (double)dsum[cell] * kDeltaScale) / (double)inv_w_);
+                e.bin_center = 2.0 * distance_sum(n, b, dsum[cell], inv_w_);
                 entry_add(row_kind_[r] == 0 ? aa[b] : aw[b], e);
             }
     std::vector<WeightedEntry> ptot(nb);

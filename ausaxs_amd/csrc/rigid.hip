@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeLaunch p) {
                 if (!n) continue;
                 // add<2> (SimpleCalculator.h:195-207): value 2 n w_i w_j, count 2 n, 2 sum d
                 const double nv = 2.0 * (double)n;
-                const double d = 2.0 * (((double)n * b + (double)p.dsum[cell] * kDeltaScale) / p.inv_w);
+                const double d = 2.0 * distance_sum(n, b, p.dsum[cell], (float)p.inv_w);
                 if (p.row_kind[r] == 0) {
                     aa_v += nv * p.row_w[r];
                     aa_c += nv;

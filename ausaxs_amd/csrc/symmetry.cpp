@@ -432,7 +432,6 @@ std::unique_ptr<CompositeDistanceHistogramHIP> SymmetryStructure::calculate(bool
         throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
                                  std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
                                  " A); increase bin_count or bin_width");
-    const double inv_w = (double)engine_->inv_w();
     const size_t nb = nbins;
     std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
     for (size_t r = 0; r < rows_.size(); ++r) {
@@ -450,7 +449,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> SymmetryStructure::calculate(bool
             WeightedEntry e;
             e.value = (double)(factor * c) * wprod;
             e.count = (uint32_t)(factor * c);
-            e.bin_center = (double)factor * (((double)c * b + (double)sd * kDeltaScale) / inv_w);
+            e.bin_center = (double)factor * distance_sum(c, b, sd, engine_->inv_w());
             entry_add((*dst)[b], e);
         }
     }

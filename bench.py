@@ -65,12 +65,68 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_threads():
+def host_cpu():
+    """(threads the CPU baseline uses, facts about the host): every logical
+    core this process may run on, capped by OMP_NUM_THREADS where the host
+    sets it (a GPU box's CPU share per GPU)."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
+        affinity = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(affinity, int(share)) if share.isdigit() and int(share) > 0 else affinity
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    facts = {"nproc": os.cpu_count(), "affinity_cores": affinity, "cpu_model": model,
+             "thread_limit": f"OMP_NUM_THREADS={share}" if share else "none"}
+    return max(1, threads), facts
+
+
+CPU_PROTOCOL = "1 warm-up run, then the median of 5 timed runs"
+CPU_BUILD = ("oracle restatement built on this host with the reference's release flags "
+             "(-O3 -ffast-math -fno-finite-math-only -march=native; reassociation and FMA contraction off)")
+
+
+def median_runs(fn, reps=5):
+    """One warm-up call, then `reps` timed calls: (median seconds, all times, last result)."""
+    fn()
+    times, info = [], None
+    for _ in range(reps):
+        t = time.perf_counter()
+        info = fn()
+        times.append(time.perf_counter() - t)
+    return float(np.median(times)), times, info
+
+
+def pick_fraction(run, target_s, first_frac):
+    """Prefix fraction of a workload whose run takes about target_s: probe runs
+    on growing prefixes (pair work grows with the square of the prefix; a
+    small prefix also pays fixed costs such as the thread-private histograms)
+    until one takes half the target, then the whole workload if it would take
+    at most 1.3 times the target."""
+    frac = min(1.0, first_frac)
+    while True:
+        t = time.perf_counter()
+        run(frac)
+        dt = time.perf_counter() - t
+        grow = float(np.sqrt(target_s / max(dt, 1e-3)))
+        if frac >= 1.0 or dt >= 0.5 * target_s:
+            return 1.0 if frac * grow * 1.14 >= 1.0 else frac * grow
+        frac = min(1.0, frac * grow)
+
+
+def baseline(value, unit, threads, facts, sample, times):
+    out = dict(value=value, unit=unit, cores=threads, kind="port", sample=sample, protocol=CPU_PROTOCOL,
+               build=CPU_BUILD, runs_s=[round(t, 4) for t in times])
+    out.update(facts)
+    return out
 
 
 def traffic_from_profiles():
@@ -85,20 +141,6 @@ def traffic_from_profiles():
 
 def pairs_of(n, m):
     return n * (n - 1) / 2 + n * m + m * (m - 1) / 2
-
-
-def grow_sample(run, total_units, first_frac, budget_s):
-    """Run run(frac) on growing prefixes of the workload until one run takes
-    at least 40 % of the budget (or the whole workload ran); pair work grows
-    with the square of the prefix."""
-    frac = min(1.0, first_frac)
-    while True:
-        t = time.perf_counter()
-        info = run(frac)
-        dt = time.perf_counter() - t
-        if frac >= 1.0 or dt >= 0.4 * budget_s:
-            return frac, dt, info
-        frac = min(1.0, frac * float(np.sqrt(0.8 * budget_s / max(dt, 1e-3))))
 
 
 # ---------------------------------------------------------------------------
@@ -118,14 +160,21 @@ def bench_structure(cfg, args):
     st.set_timing(True)
     last = {}
 
+    kernel_ms = []
+
     def step():
         h = st.calculate()
         last["h"] = h
+        # the pair kernel's launch time (HIP events on the structure's stream;
+        # calculate() has already synchronised on its read-back)
+        kernel_ms.append(st.last_timing()["pair_ms"])
         return h.debye_transform(q)
 
     res = timed_steps(step, args)
     tm = st.last_timing()
-    res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1, kernel_ms=tm["pair_ms"],
+    timed_ms = kernel_ms[-args.steps:]  # the launches of the timed steps
+    res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1,
+               kernel_ms=float(np.mean(timed_ms)),
                items=tm["n_items"],
                extra={"atoms": len(atoms), "waters": 0 if waters is None else len(waters), "q_points": len(q),
                       "debye_terms": args.debye_terms})
@@ -133,35 +182,40 @@ def bench_structure(cfg, args):
     def cpu(budget):
         import oracle as O
 
-        th = host_threads()
+        th, facts = host_cpu()
         omode = O.FFAVG if cfg["mode"] == "ffavg" else O.FFEXPLICIT
         w = waters if waters is not None else np.zeros((0, 4), np.float32)
 
         def run(frac):
-            ns, ms = int(len(atoms) * frac), int(len(w) * frac)
+            ns, ms = max(2, int(len(atoms) * frac)), int(len(w) * frac)
             h = O.hist_ff(omode, atoms[:ns], w[:ms], cfg["bw"], 8000, 1.0, th)
-            I, _ = O.debye_ff(omode, h, q, cfg["bw"])
-            return ns, ms, h, I
+            O.debye_ff(omode, h, q, cfg["bw"])
+            return ns, ms
 
-        frac, dt, (ns, ms, h_cpu, I_cpu) = grow_sample(run, len(atoms), 12000 / len(atoms), budget)
+        with O.release_build():
+            # about budget / 6 per run: 1 warm-up + 5 timed runs fill the budget
+            frac = pick_fraction(run, budget / 6, 12000 / len(atoms))
+            med, times, (ns, ms) = median_runs(lambda: run(frac))
         pairs = pairs_of(ns, ms)
-        sample = "full workload" if frac >= 1.0 else f"first {ns} atoms + {ms} waters of the workload"
-        out = dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
-                   sample=f"{sample} ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
-        if frac >= 1.0:
-            # the GPU step's result against the CPU restatement on the same input:
-            # pair counts per bin and form-factor row, and I(q) (bf16-split Debye)
-            try:
-                h = last["h"]
-                same = h.max_bin == h_cpu["max_bin"] and all(
-                    np.array_equal(h.get(which)[1], h_cpu[key]["count"])
-                    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww")))
-                I_gpu = h.debye_transform(q)
-                err = float(np.max(np.abs(I_gpu - I_cpu) / np.abs(I_cpu)))
-                out["parity"] = {"counts_identical": bool(same), "iq_max_rel_err": err, "iq_tolerance": 1e-4}
-                log(f"GPU vs CPU: counts identical {same}, I(q) max relative error {err:.2e}")
-            except Exception as e:  # the baseline line must still be printed
-                out["parity"] = {"error": str(e)}
+        sample = "full workload" if ns == len(atoms) else f"first {ns} atoms + {ms} waters of the workload"
+        out = baseline(pairs / med / 1e9, "Gpair-evals/s", th, facts,
+                       f"{sample} ({pairs:.3g} pairs, histogram + I(q); median {med:.3f} s)", times)
+        # the GPU step's result against the checker (the oracle's IEEE build) on
+        # the full workload, outside the timing: pair counts per bin and
+        # form-factor row, and I(q) (bf16-split Debye) against the FP64 oracle
+        try:
+            h = last["h"]
+            h_cpu = O.hist_ff(omode, atoms, w, cfg["bw"], 8000, 1.0, th)
+            I_cpu, _ = O.debye_ff(omode, h_cpu, q, cfg["bw"])
+            same = h.max_bin == h_cpu["max_bin"] and all(
+                np.array_equal(h.get(which)[1], h_cpu[key]["count"])
+                for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww")))
+            I_gpu = h.debye_transform(q)
+            err = float(np.max(np.abs(I_gpu - I_cpu) / np.abs(I_cpu)))
+            out["parity"] = {"counts_identical": bool(same), "iq_max_rel_err": err, "iq_tolerance": 1e-4}
+            log(f"GPU vs CPU: counts identical {same}, I(q) max relative error {err:.2e}")
+        except Exception as e:  # the baseline line must still be printed
+            out["parity"] = {"error": str(e)}
         return out
 
     return res, cpu
@@ -181,12 +235,17 @@ def bench_grid(cfg, args):
     log(f"resident grid structure built in {time.perf_counter() - t0:.2f} s (dummies from the 1 A grid)")
     st.set_timing(True)
 
+    kernel_ms = []
+
     def step():
-        return st.calculate().debye_transform(q)
+        h = st.calculate()
+        kernel_ms.append(st.last_timing()["pair_ms"])
+        return h.debye_transform(q)
 
     res = timed_steps(step, args)
     tm = st.last_timing()
-    res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1, kernel_ms=tm["pair_ms"],
+    res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1,
+               kernel_ms=float(np.mean(kernel_ms[-args.steps:])),
                items=tm["n_items"],
                extra={"atoms": len(xyz), "waters": len(wxyz), "q_points": len(q), "debye_terms": args.debye_terms,
                       "lattice_pairs_per_step": tm["lattice_pairs"]})
@@ -194,15 +253,24 @@ def bench_grid(cfg, args):
     def cpu(budget):
         import oracle as O
 
-        th = host_threads()
-        n_x = len(O.grid_exv(xyz, ff, wxyz)[0])
-        t = time.perf_counter()
-        h = O.hist_ffgrid(xyz, ff, wxyz, cfg["bw"], 8000, th)
-        O.debye_ffgrid(h, q, cfg["bw"])
-        dt = time.perf_counter() - t
-        pairs = pairs_of(len(xyz) + len(wxyz) + n_x, 0)
-        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
-                    sample=f"full workload incl. grid ({pairs:.3g} pairs, histogram + I(q), {dt:.2f} s)")
+        th, facts = host_cpu()
+
+        def run(frac):
+            # a prefix of the atoms with its own grid and waters (the same generator's structure, smaller)
+            na, nw = max(2, int(len(xyz) * frac)), max(1, int(len(wxyz) * frac))
+            h = O.hist_ffgrid(xyz[:na], ff[:na], wxyz[:nw], cfg["bw"], 8000, th)
+            O.debye_ffgrid(h, q, cfg["bw"])
+            return na, nw
+
+        with O.release_build():
+            frac = pick_fraction(run, budget / 6, 0.15)
+            med, times, (na, nw) = median_runs(lambda: run(frac))
+        n_x = len(O.grid_exv(xyz[:na], ff[:na], wxyz[:nw])[0])
+        pairs = pairs_of(na + nw + n_x, 0)
+        sample = "full workload" if na == len(xyz) else f"first {na} atoms + {nw} waters with their grid"
+        return baseline(pairs / med / 1e9, "Gpair-evals/s", th, facts,
+                        f"{sample} ({n_x} dummies, {pairs:.3g} pairs incl. grid, histogram + I(q); "
+                        f"median {med:.3f} s)", times)
 
     return res, cpu
 
@@ -257,22 +325,26 @@ def bench_rigid(cfg, args):
     def cpu(budget):
         import oracle as O
 
-        th = host_threads()
+        th, facts = host_cpu()
         sizes = [len(b) for b in bodies]
         ident = A.rigid_transform(np.eye(3), np.zeros(3))
         self_pts = np.concatenate([O.rigid_transform(b, w, ident) for b, w in zip(bodies, weights)])
-        n = 0
-        t = time.perf_counter()
-        while n < len(T) and (time.perf_counter() - t) < 0.5 * budget:
-            moved = np.concatenate([O.rigid_transform(b, w, T[n, k]) for k, (b, w) in enumerate(zip(bodies, weights))])
+        k = [0]
+
+        def run():
+            # one conformation: moved bodies, partial histograms (inter-body blocks) and I(q)
+            n = k[0] % len(T)
+            k[0] += 1
+            moved = np.concatenate([O.rigid_transform(b, w, T[n, j]) for j, (b, w) in enumerate(zip(bodies, weights))])
             h = O.hist_partial(self_pts, moved, sizes, None, cfg["bw"], 8000, th)
             O.debye_simple(h, q, cfg["bw"])
-            n += 1
-        dt = time.perf_counter() - t
-        pairs = n * info["pairs"] / len(T)
-        return dict(value=pairs / dt / 1e9, unit="Gpair-evals/s", cores=th, kind="port",
-                    sample=f"first {n} of {len(T)} conformations ({pairs:.3g} inter-body pairs, partial "
-                           f"histograms + I(q), {dt:.2f} s)")
+
+        with O.release_build():
+            med, times, _ = median_runs(run)
+        pairs = info["pairs"] / len(T)
+        return baseline(pairs / med / 1e9, "Gpair-evals/s", th, facts,
+                        f"one conformation per run ({pairs:.3g} inter-body pairs, partial histograms + I(q); "
+                        f"median {med:.3f} s)", times)
 
     return res, cpu
 
@@ -370,31 +442,42 @@ def bench_serve(cfg, args):
     def cpu(budget):
         import oracle as O
 
-        th_ = host_threads()
-        done, cpu_pairs = 0, 0.0
-        t = time.perf_counter()
-        for r in reqs:
-            if time.perf_counter() - t > 0.7 * budget:
-                break
+        th_, facts = host_cpu()
+
+        def one(r):
             xyz, ff, w = pool[r[0]]
-            if r[1] == A.MODE_FFGRID and len(xyz) > 20000:
-                continue  # bounded sample: large grid requests take minutes on the host
             if r[1] == A.MODE_FFGRID:
                 nx = len(O.grid_exv(xyz, ff, w)[0])
                 h = O.hist_ffgrid(xyz, ff, w, 0.25, 8000, th_)
                 O.debye_ffgrid(h, q, 0.25)
                 n = len(xyz) + len(w) + nx
-                cpu_pairs += n * (n - 1) / 2
-            else:
-                om = O.FFAVG if r[1] == A.MODE_FFAVG else O.FFEXPLICIT
-                h = O.hist_ff(om, O.pack_ff(xyz, ff), O.pack_ff(w, np.ones(len(w), np.int32)), 0.25, 8000, 1.0, th_)
-                O.debye_ff(om, h, q, 0.25)
-                cpu_pairs += pairs_of(len(xyz), len(w))
-            done += 1
-        dt = time.perf_counter() - t
-        return dict(value=cpu_pairs / dt / 1e9, unit="Gpair-evals/s", cores=th_, kind="port",
-                    sample=f"{done} requests of the arrival sequence (grid requests above 20k atoms skipped), "
-                           f"{cpu_pairs:.3g} pairs, {dt:.2f} s, {done / dt:.2f} profiles/s")
+                return n * (n - 1) / 2
+            om = O.FFAVG if r[1] == A.MODE_FFAVG else O.FFEXPLICIT
+            h = O.hist_ff(om, O.pack_ff(xyz, ff), O.pack_ff(w, np.ones(len(w), np.int32)), 0.25, 8000, 1.0, th_)
+            O.debye_ff(om, h, q, 0.25)
+            return pairs_of(len(xyz), len(w))
+
+        # requests of the arrival sequence in order (grid requests above 20k
+        # atoms skipped: they take minutes on the host); the first one is the
+        # warm-up, then the aggregate rate over the budget
+        seq = [r for r in reqs if not (r[1] == A.MODE_FFGRID and len(pool[r[0]][0]) > 20000)]
+        with O.release_build():
+            one(seq[0])
+            done, cpu_pairs, times = 0, 0.0, []
+            t = time.perf_counter()
+            for r in seq[1:]:
+                if done >= 5 and time.perf_counter() - t > 0.7 * budget:
+                    break
+                t1 = time.perf_counter()
+                cpu_pairs += one(r)
+                times.append(time.perf_counter() - t1)
+                done += 1
+            dt = time.perf_counter() - t
+        out = baseline(cpu_pairs / dt / 1e9, "Gpair-evals/s", th_, facts,
+                       f"{done} requests of the arrival sequence after one warm-up request (grid requests above "
+                       f"20k atoms skipped), {cpu_pairs:.3g} pairs, {dt:.2f} s, {done / dt:.2f} profiles/s", times)
+        out["protocol"] = "1 warm-up request, then the aggregate rate over the requests that follow"
+        return out
 
     return res, cpu
 
@@ -423,7 +506,8 @@ def main():
     ap.add_argument("--fixed-tiles", action="store_true",
                     help="rigid scan: use --tile-j and one workgroup per item instead of the autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU work for the baseline sample")
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of CPU work for the baseline (1 warm-up + 5 timed runs of a sample)")
     ap.add_argument("--requests", type=int, default=400, help="serving: requests in the timed open-loop run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (traffic-counter calibration) and exit")
@@ -462,7 +546,7 @@ def main():
         kern_s = res["kernel_ms"] / 1e3
         kpairs = res.get("kernel_pairs", pairs)
         achieved = kpairs * FLOP_PER_PAIR / kern_s / 1e12
-        roofline = {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path; 157.3 TF is the FP32 peak)",
+        roofline = {"bound": "valu", "engine": "fp32 VALU (no matrix op on this path; 157.3 TF is the FP32 peak)",
                     "kernel": "pair_hist_kernel", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                     "traffic": traffic_from_profiles() if args.config == 3 else None,
@@ -472,14 +556,14 @@ def main():
         # pairs the pair kernel evaluated over the window (grid requests count
         # their dummy-dummy pairs through lattice correlation, not here)
         achieved = res["kernel_pairs"] * args.steps / elapsed * FLOP_PER_PAIR / 1e12
-        roofline = {"bound": "mfma", "engine": "fp32 VALU; whole serving window at the offered load, idle included",
+        roofline = {"bound": "valu", "engine": "fp32 VALU; whole serving window at the offered load, idle included",
                     "kernel": "pair_hist_kernel (serving)", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None}
     else:
         # whole graph launch (transform, pair, merge, strips, finalize, Debye) per step
         g_s = res["extra"]["graph_ms"] / 1e3
         achieved = pairs * FLOP_PER_PAIR / g_s / 1e12
-        roofline = {"bound": "mfma", "engine": "fp32 VALU; timed over the whole graph launch",
+        roofline = {"bound": "valu", "engine": "fp32 VALU; timed over the whole graph launch",
                     "kernel": "pair_hist_kernel (graph)", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
                     "graph_ms": round(res["extra"]["graph_ms"], 4), "pairs_per_s": round(pairs / g_s, 1)}

@@ -52,7 +52,7 @@ typedef struct {
     int bin_count;     /* settings::axes::bin_count (default 8000)  */
     int weighted_bins; /* settings::hist::weighted_bins (default 1) */
     int debye_terms;   /* bf16 terms per GEMM operand: 2 = hi+lo, 3 = hi+mid+lo (default); 0 = FP64 */
-    int tile_j;        /* columns per pair work item (default 8000, at most 8189) */
+    int tile_j;        /* columns per pair work item (default 8000, at most 8064) */
     int lattice;       /* grid dummy-dummy pairs by lattice correlation instead of pair by pair
                           (identical histograms): 0 off, 1 when cheaper (default), 2 always */
     double surface_thickness; /* grid-surface excluded volume: settings::grid::exv::surface_thickness
@@ -99,10 +99,20 @@ int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t*
  * body); noise: NULL (none; the reference's tests) or n x 26 x 3 displacements
  * (the reference draws N(0, 0.75) per component). waters: capacity x 3, in
  * placement order (capacity 26 n always suffices; NULL: count only);
- * per_body: n_bodies counts or NULL. Host only (no device work). */
+ * per_body: n_bodies counts or NULL. settings: NULL for the reference's
+ * defaults, else the radii, shell correction and grid extent below. Host
+ * only (no device work). */
+typedef struct ausaxs_hydrate_settings {
+    double atomic_radius;    /* Grid::get_atomic_radius for every atom; < 0: each type's vdW radius (default -1) */
+    double hydration_radius; /* Grid::get_hydration_radius (default 1.71, the vdW radius of O) */
+    double shell_correction; /* settings::hydrate::shell_correction (default -0.35) */
+    int explicit_grid;       /* 1: the grid of Grid(Limit3D(grid_min, grid_max)) with 1 A cells; 0: Grid(bodies) */
+    double grid_min[3], grid_max[3];
+} ausaxs_hydrate_settings;
+void ausaxs_default_hydrate_settings(ausaxs_hydrate_settings* s);
 int ausaxs_hydrate_radial(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms, const int* body_sizes,
                           int n_bodies, const double* noise, double* waters_xyz, int capacity, int* n_waters,
-                          int* per_body);
+                          int* per_body, const ausaxs_hydrate_settings* settings);
 /* The interior and surface dummies of RawGridWithSurfaceExv::create, each in the
  * reference's voxel order; counts are returned also when a capacity is too small.
  * surface_thickness: settings::grid::exv::surface_thickness (A; 1 is the default). */
@@ -252,6 +262,12 @@ int ausaxs_default_q_axis(double qmin, double qmax, double* q, int* nq);
  * Sxx/delta, chi2. Scores a whole rigid-body scan against data in one call. */
 int ausaxs_lsq(const double* x, const double* y, const double* err, int n, int n_models, double* out);
 
+/* math::CubicSpline (math/CubicSpline.cpp): the natural cubic spline through
+ * (x[k], y[k]), k < n (n >= 4, x strictly increasing), at z[i], i < m, the end
+ * intervals' cubics continuing outside the knots. The fitter splines model
+ * curves onto the data's q values with it. Host only. */
+int ausaxs_cubic_spline(const double* x, const double* y, int n, const double* z, int m, double* out);
+
 /* hist::exact_debye_transform (intensity_calculator/ExactDebyeCalculator.cpp:13-57):
  * I(q) = exp(-q^2) [sum_i w_i^2 + sum_{i<j} 2 w_i w_j sinc(q d_ij)] over XYZW
  * points (n x 4 floats), distances with the reference's evaluate_8/4/1 rules,
@@ -273,12 +289,16 @@ int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, con
                           double* out);
 
 /* Exhaustive device check of the binning arithmetic over float d2 bit
- * patterns [lo_bits, hi_bits]: mismatches[0] = fast-path bins that differ
- * from the correctly rounded rule, mismatches[1] = device sqrt results that
- * are not correctly rounded, mismatches[2] = results of the tie fallback's
- * residual correction (normal d2) that differ from the correctly rounded sqrt.
- * mismatches: 3 entries. */
-int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsigned long long* mismatches);
+ * patterns [lo_bits, hi_bits]: out[0] = pair-kernel bins that differ from
+ * the correctly rounded rule, out[1] = device sqrt results that are not
+ * correctly rounded, out[2] = results of the tie fallback's residual
+ * correction (normal d2) that differ from the correctly rounded sqrt,
+ * out[3] = pair-kernel offset codes more than 1 + 2^-24 q units (q = units
+ * per bin) from the exact offset of the real root, out[4] = the sum of the code errors in units of 2^-16
+ * (two's complement), out[5] = the same sum for the correctly rounded root.
+ * out: 6 entries. No counterpart in the reference: the pin of the pair
+ * kernel's arithmetic behind CompactCoordinatesXYZFF.h:514. */
+int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsigned long long* out);
 
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming
  * kernel, `reps` times; *ms = mean kernel time. A traffic-counter reference of

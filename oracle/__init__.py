@@ -6,8 +6,10 @@ product (ausaxs_amd).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import subprocess
+import tempfile
 from pathlib import Path
 
 import numpy as np
@@ -25,10 +27,23 @@ def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
-def _load():
-    if not LIB_PATH.exists():
+_SIGS = {}
+
+
+def _sig(name, restype, argtypes):
+    """Declare the C signature of an oracle function (applied to every build loaded)."""
+    _SIGS[name] = (restype, argtypes)
+    f = getattr(lib, name)
+    f.restype, f.argtypes = restype, argtypes
+
+
+def _load(path=LIB_PATH):
+    if not Path(path).exists():
         build()
-    lib = C.CDLL(str(LIB_PATH))
+    lib = C.CDLL(str(path))
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     sig = {
         "orc_bin_pair": (None, [vp, vp, C.c_float, i, vp, vp]),
@@ -61,6 +76,23 @@ def _load():
 
 
 lib = _load()
+
+
+@contextlib.contextmanager
+def release_build():
+    """Within the block every function here runs on the same restatement
+    built with the reference's release flags for this host (Makefile target
+    `release`): the CPU baseline's timing build. It is compiled into a
+    temporary directory on the machine that times it (-march=native)."""
+    global lib
+    with tempfile.TemporaryDirectory() as tmp:
+        out = Path(tmp) / "liboracle_release.so"
+        subprocess.run(["make", "-s", "-C", str(HERE), "release", f"OUT={out}"], check=True)
+        saved, lib = lib, _load(out)
+        try:
+            yield
+        finally:
+            lib = saved
 
 
 def _p(a):
@@ -216,14 +248,11 @@ def debye_ff(mode, hist, q, bin_width, weighted=True, long_double=False, **param
 # ---------------------------------------------------------------------------
 # grid excluded volume (FFGrid)
 # ---------------------------------------------------------------------------
-lib.orc_grid_exv.restype = C.c_int
-lib.orc_grid_exv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                             C.c_void_p]
-lib.orc_hist_ffgrid.restype = C.c_int
-lib.orc_hist_ffgrid.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
-                                C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 6
-lib.orc_weighted_axis_n.restype = None
-lib.orc_weighted_axis_n.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
+_sig("orc_grid_exv", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                             C.c_void_p])
+_sig("orc_hist_ffgrid", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
+                                C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 6)
+_sig("orc_weighted_axis_n", None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p])
 FFGRID = 2
 
 
@@ -298,12 +327,10 @@ def debye_ffgrid(hist, q, bin_width, long_double=False, **params):
 # ---------------------------------------------------------------------------
 # grid excluded volume with a surface layer (FFGridSurface)
 # ---------------------------------------------------------------------------
-lib.orc_grid_exv_surface.restype = C.c_int
-lib.orc_grid_exv_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                                     C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double]
-lib.orc_hist_ffgrid_surface.restype = C.c_int
-lib.orc_hist_ffgrid_surface.argtypes = ([C.c_void_p, C.c_int] * 4 + [C.c_double, C.c_float, C.c_int, C.c_int]
-                                        + [C.c_void_p] * 13)
+_sig("orc_grid_exv_surface", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double])
+_sig("orc_hist_ffgrid_surface", C.c_int, ([C.c_void_p, C.c_int] * 4 + [C.c_double, C.c_float, C.c_int, C.c_int]
+                                        + [C.c_void_p] * 13))
 
 
 def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=None, surface_thickness=1.0):
@@ -325,13 +352,24 @@ def grid_exv_surface(atoms_xyz, ff, waters_xyz=None, rvol=2.15, atomic_radius=No
     return xi[:n].copy(), xs[:ns.value].copy(), vol.value
 
 
-lib.orc_hydrate_radial.restype = C.c_int
-lib.orc_hydrate_radial.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                                   C.c_int, C.c_void_p]
+_sig("orc_hydrate_radial", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_void_p])
 
 
-def hydrate_radial(atoms_xyz, ff, bodies=None, noise=None):
-    """Radial hydration shell (see orc_hydrate_radial): waters (m x 3) and per-body counts."""
+class HydrateOpts(C.Structure):
+    _fields_ = [("atomic_radius", C.c_double), ("hydration_radius", C.c_double), ("shell_correction", C.c_double),
+                ("explicit_grid", C.c_int), ("grid_lim", C.c_double * 6)]
+
+
+_sig("orc_hydrate_radial_opts", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                          C.POINTER(HydrateOpts), C.c_void_p, C.c_int, C.c_void_p])
+
+
+def hydrate_radial(atoms_xyz, ff, bodies=None, noise=None, atomic_radius=-1.0, hydration_radius=1.71,
+                   shell_correction=-0.35, grid_limits=None):
+    """Radial hydration shell (see orc_hydrate_radial_opts): waters (m x 3) and
+    per-body counts. grid_limits: ((xmin, ymin, zmin), (xmax, ymax, zmax)) for
+    Grid(Limit3D), None for the grid built from the atoms."""
     a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(ff, np.int32).reshape(-1)
     b = np.ascontiguousarray([len(a)] if bodies is None else bodies, np.int32)
@@ -339,7 +377,14 @@ def hydrate_radial(atoms_xyz, ff, bodies=None, noise=None):
     cap = max(1, 26 * len(a))
     out = np.zeros((cap, 3), np.float64)
     per = np.zeros(len(b), np.int32)
-    m = lib.orc_hydrate_radial(_p(a), _p(f), len(a), _p(b), len(b), _p(nz), _p(out), cap, _p(per))
+    o = HydrateOpts(atomic_radius, hydration_radius, shell_correction, 0)
+    if grid_limits is not None:
+        o.explicit_grid = 1
+        for k in range(3):
+            o.grid_lim[k], o.grid_lim[3 + k] = grid_limits[0][k], grid_limits[1][k]
+    m = lib.orc_hydrate_radial_opts(_p(a), _p(f), len(a), _p(b), len(b), _p(nz), o, _p(out), cap, _p(per))
+    if m == -2:
+        raise ValueError("an atom lies outside the grid")
     if m < 0:
         raise RuntimeError("hydrate_radial capacity")
     return out[:m].copy(), per
@@ -416,11 +461,9 @@ def debye_ffgrid_surface(hist, q, bin_width, cw=1.0, cx=1.0, crho=1.0, grid_exv_
 # ---------------------------------------------------------------------------
 # rigid bodies (PartialHistogramManagerMT) and the plain composite I(q)
 # ---------------------------------------------------------------------------
-lib.orc_rigid_transform.restype = None
-lib.orc_rigid_transform.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
-lib.orc_hist_partial.restype = C.c_int
-lib.orc_hist_partial.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
-                                 C.c_int, C.c_int] + [C.c_void_p] * 4
+_sig("orc_rigid_transform", None, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p])
+_sig("orc_hist_partial", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
+                                 C.c_int, C.c_int] + [C.c_void_p] * 4)
 
 
 def rigid_transform(xyz, w, T):
@@ -447,10 +490,9 @@ def hist_partial(self_bodies, bodies, sizes, waters=None, bin_width=0.25, bin_co
     return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
 
 
-lib.orc_affine_float.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
-lib.orc_hist_symmetry.restype = C.c_int
-lib.orc_hist_symmetry.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4
+_sig("orc_affine_float", None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p])
+_sig("orc_hist_symmetry", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4)
 
 
 def affine_float(pts, T):
@@ -462,9 +504,8 @@ def affine_float(pts, T):
     return out
 
 
-lib.orc_hist_symmetry_partial.restype = C.c_int
-lib.orc_hist_symmetry_partial.argtypes = [C.c_void_p] * 3 + [C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [
-    C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4
+_sig("orc_hist_symmetry_partial", C.c_int, [C.c_void_p] * 3 + [C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [
+    C.c_int, C.c_float, C.c_int, C.c_int] + [C.c_void_p] * 4)
 
 
 def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=8000, nthreads=1, partial=False,
@@ -503,8 +544,7 @@ def hist_symmetry(bodies, symmetries, waters=None, bin_width=0.25, bin_count=800
     return dict(max_bin=mb, aa=out[0][:mb], aw=out[1][:mb], ww=out[2][:mb], p_tot=out[3][:mb])
 
 
-lib.orc_spline.restype = C.c_int
-lib.orc_spline.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+_sig("orc_spline", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
 
 
 def spline(x, y, z):

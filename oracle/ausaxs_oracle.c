@@ -437,8 +437,10 @@ static int radial_offsets(int off[][3][3], int cap) {
  * expand_volume for the atoms (SphericalExpander.h:29-90); rvol and ra as for
  * grid_exv_core. Returns the state grid (calloc'd) with its minimum, bin
  * counts, the atoms' bins (malloc'd) and Grid::get_volume's count. */
+/* lim: NULL for Grid(bodies) (Grid.cpp:34-104), else {xmin, ymin, zmin, xmax,
+ * ymax, zmax} for Grid(Limit3D) (Grid.cpp:30, Axis3D.cpp:17: span / width bins) */
 static uint8_t* grid_build(const double* atoms, const int32_t* ff, int n, const double* waters, int m, double rvol,
-                           double ra, double gmin[3], int bins[3], int** loc_out, long* vol_out) {
+                           double ra, const double* lim, double gmin[3], int bins[3], int** loc_out, long* vol_out) {
     const double width = 1.0;
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i)
@@ -460,6 +462,10 @@ static uint8_t* grid_build(const double* atoms, const int32_t* ff, int n, const 
         gmin[k] = floor(mn[k] - expand - width);
         double gmax = ceil(mx[k] + expand + width);
         bins[k] = (int)((gmax - gmin[k]) / width);
+        if (lim) {
+            gmin[k] = lim[k];
+            bins[k] = (int)((lim[3 + k] - lim[k]) / width);
+        }
     }
     size_t total = (size_t)bins[0] * bins[1] * bins[2];
     uint8_t* g = (uint8_t*)calloc(total, 1);
@@ -523,7 +529,7 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
     double gmin[3];
     int bins[3], *loc = NULL;
     long vol = 0;
-    uint8_t* g = grid_build(atoms, ff, n, waters, m, rvol, ra, gmin, bins, &loc, &vol);
+    uint8_t* g = grid_build(atoms, ff, n, waters, m, rvol, ra, NULL, gmin, bins, &loc, &vol);
 #define G(i, j, k) g[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
     /* RawGridExv::create over the atoms' bounding_box_index +- buffer */
     int imin[3] = {1 << 30, 1 << 30, 1 << 30}, imax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
@@ -659,17 +665,30 @@ static int grid_exv_core(const double* atoms, const int32_t* ff, int n, const do
  * GridBasedHydration.cpp:36-79, NoCulling), with collision_check (:141-206). */
 int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
                        const double* noise, double* out, int cap, int* per_body) {
+    orc_hydrate_opts o = {-1, 1.71, -0.35, 0, {0, 0, 0, 0, 0, 0}};
+    return orc_hydrate_radial_opts(atoms, ff, n, body_sizes, n_bodies, noise, &o, out, cap, per_body);
+}
+
+int orc_hydrate_radial_opts(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
+                            const double* noise, const orc_hydrate_opts* o, double* out, int cap, int* per_body) {
     if (n <= 0) return 0;
+    if (o->explicit_grid)
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const int b = (int)round((atoms[3 * i + k] - o->grid_lim[k]) / 1.0);
+                if (b < 0 || b >= (int)((o->grid_lim[3 + k] - o->grid_lim[k]) / 1.0)) return -2; /* outside the grid */
+            }
     double gmin[3];
     int bins[3], *loc = NULL;
     long vol = 0;
-    uint8_t* g = grid_build(atoms, ff, n, NULL, 0, 2.15, -1, gmin, bins, &loc, &vol);
+    uint8_t* g = grid_build(atoms, ff, n, NULL, 0, 2.15, o->atomic_radius, o->explicit_grid ? o->grid_lim : NULL, gmin, bins, &loc, &vol);
 #define G(i, j, k) g[((size_t)(i) * bins[1] + (j)) * bins[2] + (k)]
 #define OOB(a, b, c) ((a) < 0 || (a) >= bins[0] || (b) < 0 || (b) >= bins[1] || (c) < 0 || (c) >= bins[2])
 #define ONLY_EMPTY_OR_VOLUME(v) ((v) == G_EMPTY || (v) == G_VOLUME)
     double dirs[64][3];
     const int nd = sphere_dirs(dirs, 64);
-    const double rh = 1.71 + (-0.35); /* vdw O (Grid::get_hydration_radius) + shell_correction */
+    /* Grid::get_hydration_radius (vdw O by default) + settings::hydrate::shell_correction */
+    const double rh = o->hydration_radius + o->shell_correction;
     int p1[64][3], p3[64][3], p5[64][3];
     for (int d = 0; d < nd; ++d)
         for (int c = 0; c < 3; ++c) {
@@ -683,7 +702,7 @@ int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int*
     for (int b = 0; b < n_bodies; ++b) {
         int placed = 0;
         for (int e = 0; e < body_sizes[b]; ++e, ++a) {
-            const double reff = orc_vdw(ff[a]) + rh;
+            const double reff = (o->atomic_radius < 0 ? orc_vdw(ff[a]) : o->atomic_radius) + rh;
             for (int d = 0; d < nd; ++d) {
                 double w[3];
                 int wb[3];
@@ -726,8 +745,8 @@ int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int*
                 /* Grid::add(water, true): centre, then W_AREA on empty voxels
                  * within the hydration radius of the water (SphericalExpander.h:92-140) */
                 G(wb[0], wb[1], wb[2]) |= G_WCENTER;
-                const double rw2 = pow(1.71, 2);
-                const int br = (int)ceil(1.71 / 1.0);
+                const double rw2 = pow(o->hydration_radius, 2);
+                const int br = (int)ceil(o->hydration_radius / 1.0);
                 for (int i = wb[0] - br > 0 ? wb[0] - br : 0; i < (wb[0] + br + 1 < bins[0] ? wb[0] + br + 1 : bins[0]); ++i) {
                     double x2 = pow(gmin[0] + i - w[0], 2);
                     for (int j = wb[1] - br > 0 ? wb[1] - br : 0; j < (wb[1] + br + 1 < bins[1] ? wb[1] + br + 1 : bins[1]); ++j) {

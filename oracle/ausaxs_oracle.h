@@ -77,6 +77,20 @@ int orc_grid_exv(const double* atoms, const int32_t* ff, int n, const double* wa
  * Returns the water count (-1 if cap is too small); per_body (optional). */
 int orc_hydrate_radial(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
                        const double* noise, double* out, int cap, int* per_body);
+/* The same with the knobs of the reference's own hydration tests
+ * (tests/feature/grid/grid.cpp:707-748): GridDebug's atomic and hydration
+ * radius overrides, settings::hydrate::shell_correction and a grid built from
+ * explicit limits (Grid(Limit3D), Grid.cpp:30). Defaults: -1 (each atom's
+ * vdW radius), 1.71, -0.35, 0 (Grid(bodies)). */
+typedef struct {
+    double atomic_radius;
+    double hydration_radius;
+    double shell_correction;
+    int explicit_grid;
+    double grid_lim[6]; /* xmin, ymin, zmin, xmax, ymax, zmax */
+} orc_hydrate_opts;
+int orc_hydrate_radial_opts(const double* atoms, const int32_t* ff, int n, const int* body_sizes, int n_bodies,
+                            const double* noise, const orc_hydrate_opts* o, double* out, int cap, int* per_body);
 
 /* RawGridWithSurfaceExv::create with surface detection at unit cell width
  * (RawGridWithSurfaceExv.cpp:82-127): the dummies of orc_grid_exv split by

@@ -16,18 +16,23 @@ import oracle as O
 F32 = np.float32
 
 
-def test_spline_is_the_natural_cubic_spline():
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_spline_is_the_natural_cubic_spline(impl):
     from scipy.interpolate import CubicSpline
+    f = O.spline if impl == "oracle" else A.cubic_spline
     rng = np.random.default_rng(1)
-    for n in (4, 7, 40):
+    for n in (4, 7, 40, 200):
         x = np.sort(rng.uniform(0, 5, n))
         y = np.sin(2 * x) + rng.normal(0, 0.1, n)
         z = np.linspace(x[0] - 0.3, x[-1] + 0.3, 97)  # both extrapolation ends
         ref = CubicSpline(x, y, bc_type="natural")(z)
-        np.testing.assert_allclose(O.spline(x, y, z), ref, rtol=1e-12, atol=1e-12)
-        np.testing.assert_allclose(O.spline(x, y, x), y, rtol=1e-14, atol=1e-14)
-    with pytest.raises(ValueError):
-        O.spline([0, 1, 2], [0, 1, 2], [0.5])
+        np.testing.assert_allclose(f(x, y, z), ref, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(f(x, y, x), y, rtol=1e-14, atol=1e-14)
+    # a straight line is reproduced exactly, inside the knots and outside
+    x = np.linspace(0, 3, 9)
+    np.testing.assert_allclose(f(x, 2 * x - 1, [-1.0, 0.37, 5.0]), [-3.0, -0.26, 9.0], rtol=1e-14, atol=1e-14)
+    with pytest.raises((ValueError, A.AusaxsError)):
+        f([0, 1, 2], [0, 1, 2], [0.5])
 
 
 def test_golden_section_closed_form():

@@ -8,7 +8,7 @@ import pytest
 
 import ausaxs_amd as A
 import oracle as O
-from conftest import compare_hist_approx, cube_atoms, globule
+from conftest import axis_ok, compare_hist_approx, cube_atoms, globule
 
 pytestmark = pytest.mark.gpu
 
@@ -18,14 +18,6 @@ F32 = np.float32
 def rel_close(a, b, rtol, atol=1e-9):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return np.all(np.abs(a - b) <= atol + rtol * np.abs(b))
-
-
-def axis_ok(d, d_ref, bin_width):
-    """Bin centres from the device: fixed-point delta sums (mean error <= 2^-21
-    bins) of distances from v_sqrt_f32 (within 1 ulp, so <= 2^-23 relative;
-    bins themselves are exact)."""
-    d, d_ref = np.asarray(d), np.asarray(d_ref)
-    return np.all(np.abs(d - d_ref) <= bin_width * 2.0 ** -21 + 2.0 ** -22 * np.abs(d_ref))
 
 
 def test_device_is_gfx950():
@@ -41,7 +33,23 @@ def test_binning_arithmetic_exhaustive(bw):
     # sqrt_rn, the device sqrtf is correctly rounded (verified in FP64), and the
     # fallback's correction equals it on every normal d2
     hi = int(np.array(6.4e5, np.float32).view(np.uint32))
-    assert A.bin_check(0, hi, bw) == (0, 0, 0)
+    bad_bin, bad_cr, bad_fix, bad_off, _, _ = A.bin_check(0, hi, bw)
+    assert (bad_bin, bad_cr, bad_fix, bad_off) == (0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("bw", [0.25, 0.1, 1.0])
+def test_offsets_unbiased_at_long_distances(bw):
+    # every float d2 in [1.6e4, 6.4e4) (distances 126-253 A, where one ulp is
+    # 2^-16 A and v_sqrt_f32's shortfall alone would move a bin centre by up to
+    # ~5e-6 A): the pair kernel's offset codes average to the real root's
+    # offsets within 0.01 units, as do the correctly rounded roots the
+    # reference sums, and no code is more than one unit off
+    lo = int(np.array(1.6e4, np.float32).view(np.uint32))
+    hi = int(np.array(6.4e4, np.float32).view(np.uint32)) - 1
+    bad_bin, _, _, bad_off, mean_err, mean_err_cr = A.bin_check(lo, hi, bw)
+    assert bad_bin == 0 and bad_off == 0
+    assert abs(mean_err) < 0.01, mean_err
+    assert abs(mean_err_cr) < 0.01, mean_err_cr
 
 
 def check_pairs(a, b, bw, nb):
@@ -180,7 +188,7 @@ def test_cube_known_counts():
 def test_determinism_across_tilings():
     atoms, waters = config1(3000, 500)
     outs = []
-    for tj in (256, 1000, 8189):
+    for tj in (256, 1000, 8064):
         h = A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters, settings=A.default_settings(tile_j=tj))
         outs.append([h.get(w) for w in (A.AA, A.AW, A.WW)])
     for o in outs[1:]:
@@ -216,12 +224,12 @@ def test_intensity_parity(mode):
     params = dict(cw=1.1, cx=1.05, crho=0.97, dw_atomic=0.2, dw_exv=0.1)
     I_ref, P_ref = O.debye_ff(omode, ref_h, q, 0.25, long_double=True, **params)
     # terms 0 (FP64 on device) differs from the oracle only through the bin
-    # centres (fixed-point distance sums: mean error <= 2^-21 bins), which I(q)
-    # amplifies near its minima (it spans ~4 decades here): a systematic 2^-21-bin
-    # shift of every centre moves the oracle's own I(q) by up to 1.5e-6.
+    # centres (offsets of the real root in units of 2^-22 A against sums of
+    # correctly rounded roots), which I(q) amplifies near its minima (it spans
+    # ~4 decades here)
     for terms, tol in ((3, 1e-4), (0, 2e-5)):
         h = A.calculate_all(mode, atoms, waters, settings=A.default_settings(debye_terms=terms), z_exv=0.5)
-        assert axis_ok(h.d_axis(), O.weighted_axis(ref_h["p_tot"], 0.25), 0.25)
+        assert axis_ok(h.d_axis(), O.weighted_axis(ref_h["p_tot"], 0.25), 0.25, ref_h["p_tot"]["count"])
         h.set_params(**params)
         I, P = h.debye_transform(q, profiles=True)
         err = relerr(I, I_ref)

@@ -2,10 +2,14 @@
 of GridBasedHydration::hydrate, GridBasedHydration.cpp:36-79; NoCulling).
 
 The placement is host code in both the product and the oracle. The
-reference's own test of it (tests/feature/hydrate/radial_hydration.cpp) fixes
-the noise to zero and requires identical waters on every call; it holds no
-water counts or positions, so beyond that the oracle is pinned by what the
-procedure's definition implies, checked on its output with plain geometry:
+reference's known-answer case (tests/feature/grid/grid.cpp:707-748: one carbon
+atom at the origin of a grid spanning -10..10 A, atomic and hydration radius 3
+A, shell correction 0, zero noise) requires waters at the six axis sites 6 A
+out; both reproduce it, and variants of it whose outcome follows from the
+geometry alone. Its zero-noise consistency test
+(tests/feature/hydrate/radial_hydration.cpp) requires identical waters on
+every call. Beyond those the oracle is pinned by what the procedure's
+definition implies, checked on its output with plain geometry:
 every water sits at vdw + 1.36 A from an atom along one of the 26 directions;
 no water's voxel is an atom centre, lies inside an atom's van der Waals
 sphere or repeats another water's voxel, and one within 1.71 A of an earlier
@@ -46,6 +50,55 @@ def directions():
 
 def test_directions_are_the_26_of_the_octant_construction():
     assert len(directions()) == 26
+
+
+SIX_SITES = {(0, 0, 6), (0, 0, -6), (6, 0, 0), (-6, 0, 0), (0, 6, 0), (0, -6, 0)}
+LIMITS = ((-10, -10, -10), (10, 10, 10))
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_reference_known_answer_six_axis_sites(impl):
+    # tests/feature/grid/grid.cpp:707-748 (RadialStrategy): GridDebug with
+    # atomic and hydration radius 3 on Limit3D(-10, 10, ...) at 1 A cells,
+    # shell_correction 0, zero noise; REQUIRE(6 <= waters) and exactly the six
+    # axis sites among them
+    f = O.hydrate_radial if impl == "oracle" else A.hydrate
+    w, per = f(np.zeros((1, 3)), np.array([3], np.int32), atomic_radius=3, hydration_radius=3, shell_correction=0,
+               grid_limits=LIMITS)
+    assert len(w) >= 6 and per.tolist() == [len(w)]
+    sites = [tuple(int(v) for v in np.rint(x)) for x in w if np.allclose(x, np.rint(x), atol=0, rtol=0)]
+    assert sum(1 for t in sites if t in SIX_SITES) == 6
+    # here no other candidate passes: the 20 off-axis directions put the
+    # water's 1-shell-radius probes back onto the atom's 3 A sphere
+    assert len(w) == 6
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_known_answer_variants_follow_the_geometry(impl):
+    f = O.hydrate_radial if impl == "oracle" else A.hydrate
+    a, t = np.zeros((1, 3)), np.array([3], np.int32)
+    dirs = directions()
+    for corr in (1.0, -0.5):
+        # every water sits at ra + rh + shell_correction along one of the 26
+        # directions, inside the grid
+        w, _ = f(a, t, atomic_radius=3, hydration_radius=3, shell_correction=corr, grid_limits=LIMITS)
+        assert len(w) > 0
+        reff = 6.0 + corr
+        assert np.allclose(np.linalg.norm(w, axis=1), reff, atol=1e-12)
+        assert all(np.min(np.linalg.norm(dirs - x / reff, axis=1)) < 1e-12 for x in w)
+    # the atom and the grid moved together by whole cells move every water with them
+    w0, _ = f(a, t, atomic_radius=3, hydration_radius=3, shell_correction=0, grid_limits=LIMITS)
+    shift = np.array([1.0, -2.0, 3.0])
+    moved = (tuple(np.add(LIMITS[0], shift)), tuple(np.add(LIMITS[1], shift)))
+    w1, _ = f(a + shift, t, atomic_radius=3, hydration_radius=3, shell_correction=0, grid_limits=moved)
+    assert np.array_equal(w1, w0 + shift)
+    # a candidate whose voxel falls outside the grid is never placed
+    small = ((-6, -6, -6), (6, 6, 6))  # bins 0..11 hold -6..5: +6 rounds to bin 12
+    w2, _ = f(a, t, atomic_radius=3, hydration_radius=3, shell_correction=0, grid_limits=small)
+    assert np.all(np.rint(w2) < 6)
+    # an atom outside the explicit grid is an error
+    with pytest.raises((ValueError, A.AusaxsError)):
+        f(a + 20.0, t, atomic_radius=3, hydration_radius=3, shell_correction=0, grid_limits=LIMITS)
 
 
 @pytest.mark.parametrize("seed,n", [(1, 300), (2, 2000)])
