@@ -96,6 +96,7 @@ def _load():
         "ausaxs_default_q_axis": (C.c_int, [C.c_double, C.c_double, vp, P(C.c_int)]),
         "ausaxs_pair_histogram": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp, vp]),
         "ausaxs_debye_contract": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp]),
+        "ausaxs_debye_contract_noalloc": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp]),
         "ausaxs_bin_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, vp]),
         "ausaxs_structure_set_timing": (C.c_int, [vp, C.c_int]),
         "ausaxs_copy_probe": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),

@@ -657,6 +657,28 @@ int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, con
     });
 }
 
+int ausaxs_debye_contract_noalloc(const double* H, int rows, int K, const double* d, const double* q, int nq,
+                                  int terms, double* out, void* stream) {
+    return guarded([&] {
+        require(H && d && q && out && rows >= 0 && K >= 0 && nq >= 0, "bad argument");
+        require(rows <= (1 << 20) && K <= (1 << 24) && nq <= (1 << 24), "rows, K or nq out of range");
+        require(terms >= 0 && terms <= 3, "terms must be 0..3");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        DebyeLaunch dl{H, d, q, rows, K, nq, terms, out, nullptr};
+        if (terms == 0) {
+            check(launch_debye_f64(dl, s), "debye_f64");
+            return;
+        }
+        if (debye_scratch_bytes(rows, K, nq) > kDebyeNoAllocBytes)
+            throw std::runtime_error("ausaxs_hip: this contraction needs " +
+                                     std::to_string(debye_scratch_bytes(rows, K, nq)) +
+                                     " B of work space; the allocation-free entry point holds " +
+                                     std::to_string(kDebyeNoAllocBytes) + " B (use ausaxs_debye_contract)");
+        check(debye_noalloc_scratch(&dl.scratch), "hipGetSymbolAddress");
+        check(launch_debye(dl, s), "debye");
+    });
+}
+
 int ausaxs_bin_check(uint32_t lo, uint32_t hi, double bin_width, unsigned long long* mism) {
     return guarded([&] {
         require(mism && lo <= hi, "bad argument");

@@ -16,6 +16,7 @@
 #include "device_api.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace ausaxs::gpu {
 namespace {
@@ -162,6 +163,37 @@ int k_chunks(int rows, int K, int nq) {
     const int tiles = (pad_to(nq, 16) / 16) * (pad_to(rows, 16) / 16);
     const int steps = pad_to(K, 32) / 32;
     return std::max(1, std::min(steps, (2048 + tiles - 1) / std::max(tiles, 1)));
+}
+
+// Work space of the allocation-free contraction: split operands and partial
+// tiles of one call. Every location a call reads it writes first in the same
+// call (sinc_split and h_split fill their padded planes, debye_mfma every
+// tile of the chunks debye_reduce sums), so nothing needs clearing between
+// calls; calls that overlap on one GPU race on it.
+static __device__ __align__(256) unsigned char scratch[kDebyeNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // per device: the device address of `scratch`
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in; its address on the
+// current device comes from hipGetSymbolAddress, once per device, and only
+// that address is passed to kernels.
+hipError_t debye_noalloc_scratch(void** ptr) {
+    int dev = 0;
+    if (hipError_t e = hipGetDevice(&dev); e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)); e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 size_t debye_scratch_bytes(int rows, int K, int nq) {

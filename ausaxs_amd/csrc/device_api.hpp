@@ -194,6 +194,11 @@ struct DebyeLaunch {
     void* scratch;     // workspace, see debye_scratch_bytes
 };
 size_t debye_scratch_bytes(int rows, int K, int nq);
+// The allocation-free contraction's work space: a static device array of
+// kDebyeNoAllocBytes on every GPU the library runs on (debye.hip), whose
+// address on the current device is looked up once per device.
+constexpr size_t kDebyeNoAllocBytes = size_t(32) << 20;
+hipError_t debye_noalloc_scratch(void** ptr);
 hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s);
 // Plain FP64 contraction (same operands), used as the device reference path.
 hipError_t launch_debye_f64(const DebyeLaunch& p, hipStream_t s);

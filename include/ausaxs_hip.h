@@ -288,6 +288,20 @@ int ausaxs_pair_histogram(const float* a, int n, const float* b, int m, double b
 int ausaxs_debye_contract(const double* H, int rows, int K, const double* d, const double* q, int nq, int terms,
                           double* out);
 
+/* No counterpart in the reference: ausaxs_debye_contract for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). H (rows x K), d (K), q (nq) and out
+ * (rows x nq) are device pointers; the work is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting.
+ * It allocates nothing: its work space is a static device array of
+ * AUSAXS_DEBYE_NOALLOC_SCRATCH_BYTES on every GPU the library runs on, held
+ * until the process ends. Contractions that need more (bf16-split work space
+ * 6 (nq + rows) K bytes plus the partial tiles) fail with an error. Calls
+ * that overlap on one GPU share that array and race; serialise them. */
+#define AUSAXS_DEBYE_NOALLOC_SCRATCH_BYTES (32u << 20)
+int ausaxs_debye_contract_noalloc(const double* H, int rows, int K, const double* d, const double* q, int nq,
+                                  int terms, double* out, void* stream);
+
 /* Exhaustive device check of the binning arithmetic over float d2 bit
  * patterns [lo_bits, hi_bits]: out[0] = pair-kernel bins that differ from
  * the correctly rounded rule, out[1] = device sqrt results that are not
