@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdint>
 
 namespace ausaxs::gpu {
@@ -85,7 +86,46 @@ struct PairLaunch {
     int grid;                 // workgroups; 0 or >= n_items: one per item, else persistent (strided)
     int rows = 2;             // LDS histogram rows: 2 if any item has kOrient or kSplit, else 1
     int copies = 1;           // LDS histogram copies (1, 2, 4 or 8), lane l adds into copy l mod copies
+    // Weight pass (XYZW sets of any weights): pts holds (x, y, z, weight) and
+    // every pair adds weight_code(w_i * w_j) instead of a count cell (no
+    // oriented or split items); the merge then runs with raw = 1.
+    int wmode = 0;
+    double wscale = 1;        // 2^s: fixed-point scale of the weight products
 };
+
+// Weight products as integers: w_i * w_j (float, as the reference's XYZW
+// evaluate forms it) times wscale, rounded to nearest, is k = weight_code
+// (two's complement in a uint64): the double kWeightMagic + w_i w_j wscale
+// lies in [2^52, 2^53), where doubles are the integers, and its bit pattern
+// minus kWeightMagicBits is k. With |w_i w_j| wscale <= 2^40 each product is
+// kept to 2^-41 of the largest one, and the k of one work item (at most
+// kMaxPairsPerItem < 2^22 pairs) or one strip cell sum exactly in 64 bits;
+// the merge adds the items' sums in FP64 (MergeLaunch::raw) and the strips'
+// integer sums are added to that on the host side.
+constexpr double kWeightMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr uint64_t kWeightMagicBits = 0x4338000000000000ull;
+constexpr int kWeightBits = 40;
+__host__ __device__ inline uint64_t weight_code(float product, double wscale) {
+    const double t = __builtin_fma((double)product, wscale, kWeightMagic);
+    uint64_t u;
+    __builtin_memcpy(&u, &t, 8);
+    return u - kWeightMagicBits;
+}
+// wscale for weights of magnitude up to wmax: a power of two with
+// wmax^2 * wscale <= 2^kWeightBits
+inline double weight_scale(double wmax) {
+    if (!(wmax > 0)) return 1.0;
+    int ex = 0;
+    (void)std::frexp(wmax, &ex);  // wmax < 2^ex
+    return std::ldexp(1.0, kWeightBits - 2 * ex);
+}
+// The summed weight products of a histogram cell: items' FP64 sum (bits
+// of a double, as the raw merge writes it) plus the strips' integer sum.
+__host__ __device__ inline double weight_value(int64_t item_bits, int64_t strip_k, double wscale) {
+    double items;
+    __builtin_memcpy(&items, &item_bits, 8);
+    return (items + (double)strip_k) / wscale;
+}
 
 // LDS bytes of one pair_hist_kernel workgroup (static column tile included)
 constexpr size_t kStaticLds = (512 + 4) * 16;
@@ -112,6 +152,8 @@ struct MergeLaunch {
     int64_t* tmp_dsum;
     uint64_t* cnt;
     int64_t* dsum;
+    int raw = 0;  // 1: partials of a weight pass (integer k sums per item), summed in FP64 into dsum as
+                  //    the bits of doubles (cnt zero)
 };
 hipError_t launch_merge(const MergeLaunch& m, hipStream_t s);
 
@@ -133,6 +175,11 @@ struct StripLaunch {
     // >= 0: every pair of the strip goes to this histogram row (one class on
     // each side), which lets long strips accumulate in LDS first
     int fixed_row = -1;
+    // weight pass: points hold (x, y, z, weight), every pair goes to rowmap[0]
+    // and adds weight_code(w_i * w_j) to dsum, an integer sum that must start
+    // at zero (cnt untouched)
+    int wmode = 0;
+    double wscale = 1;
 };
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
 

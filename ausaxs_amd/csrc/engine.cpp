@@ -390,10 +390,28 @@ MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
 }
 
 PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-                       int tile_j, int lattice)
+                       int tile_j, int lattice, std::vector<std::vector<float>> weights)
     : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w),
-      lattice_mode_(lattice) {
+      lattice_mode_(lattice), weights_(std::move(weights)) {
     if (lattice < 0 || lattice > 2) throw std::runtime_error("ausaxs_hip: lattice must be 0, 1 or 2");
+    weighted_ = !weights_.empty();
+    if (weighted_) {
+        // the weight pass bins like the count pass and adds weight products:
+        // one class per set (no oriented or split items), no lattice jobs
+        if (weights_.size() != sets_.size()) throw std::runtime_error("ausaxs_hip: one weight vector per point set");
+        double wmax = 0;
+        for (size_t k = 0; k < sets_.size(); ++k) {
+            if (sets_[k].n_classes != 1 || weights_[k].size() != sets_[k].orig.size())
+                throw std::runtime_error("ausaxs_hip: weighted point sets hold one class and one weight per point");
+            for (float w : weights_[k]) {
+                if (!std::isfinite(w)) throw std::runtime_error("ausaxs_hip: weights must be finite");
+                wmax = std::max(wmax, (double)std::fabs(w));
+            }
+        }
+        for (const PairJob& j : jobs_)
+            if (j.lattice) throw std::runtime_error("ausaxs_hip: weighted point sets take no lattice jobs");
+        wscale_ = weight_scale(wmax);
+    }
     check_tile_j(tile_j);
     check_pair_lds(nbins);
     // bins need no clamp in the pair kernel when the bounding-box diagonal
@@ -408,8 +426,9 @@ PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, in
 void PairEngine::build(int tile_j) {
     Context& ctx = Context::get();
     std::vector<float4> buf;
+    std::vector<int32_t> perm;
     layout_.clear();
-    for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf));
+    for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf, weighted_ ? &perm : nullptr));
     WorkList plain, split;
     lat_.clear();
     for (const PairJob& job : jobs_) {
@@ -446,6 +465,30 @@ void PairEngine::build(int tile_j) {
     d_tmp_cnt_.ensure((size_t)mp_.n_chunks() * (nbins_ + 1));
     d_tmp_dsum_.ensure((size_t)mp_.n_chunks() * (nbins_ + 1));
     d_rowmaps_.upload(wl_.rowmaps.data(), wl_.rowmaps.size(), s);
+    if (weighted_) {
+        // weight-pass copies of the point buffers: the same positions with each
+        // point's weight in the fourth slot
+        std::vector<float4> wbuf(buf.size());
+        for (size_t k = 0; k < sets_.size(); ++k) {
+            const int base = layout_[k].base, n = sets_[k].size();
+            for (int i = 0; i < n; ++i) {
+                float4 v = buf[(size_t)base + i];
+                v.w = weights_[k][perm[(size_t)base + i]];
+                wbuf[(size_t)base + i] = v;
+            }
+        }
+        d_pts_w_.upload(wbuf.data(), wbuf.size(), s);
+        d_orig_w_.clear();
+        for (size_t k = 0; k < sets_.size(); ++k) {
+            std::vector<float4> o(sets_[k].orig);
+            for (size_t i = 0; i < o.size(); ++i) o[i].w = weights_[k][i];
+            d_orig_w_.emplace_back();
+            d_orig_w_.back().upload(o.data(), o.size(), s);
+        }
+        d_wcnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
+        d_wsum_.ensure((size_t)n_rows_ * (nbins_ + 1));
+        d_wstrip_.ensure((size_t)n_rows_ * (nbins_ + 1));
+    }
     d_partial_.ensure(std::max<size_t>(wl_.items.size(), 1) * 2 * (nbins_ + 1));
     d_cnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
     d_dsum_.ensure((size_t)n_rows_ * (nbins_ + 1));
@@ -520,17 +563,46 @@ void PairEngine::enqueue(bool timed) {
             sl.fixed_row = wl_.rowmaps[st.rowmap_off];
         check(launch_strip(sl, s), "strip_kernel");
     }
+    if (weighted_) {
+        // the weight pass over the same work lists (wmode), its own outputs
+        PairLaunch wl{d_pts_w_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0,
+                      grid_, rows_, copies_, 1, wscale_};
+        check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
+        MergeLaunch wm{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
+                       n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_wcnt_.p, d_wsum_.p, 1};
+        check(launch_merge(wm, s), "merge kernels (weights)");
+        check(launch_lattice_zero_u64((unsigned long long*)d_wstrip_.p, (size_t)n_rows_ * (nbins_ + 1), s),
+              "lattice_zero_u64_kernel");
+        for (const StripSpec& st : wl_.strips) {
+            StripLaunch sl{d_orig_w_[st.set_rows].p, d_orig_w_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
+                           d_wcnt_.p, d_wstrip_.p, inv_w_, nbins_, nbins_ + 1};
+            sl.fixed_row = wl_.rowmaps[st.rowmap_off];
+            sl.wmode = 1;
+            sl.wscale = wscale_;
+            check(launch_strip(sl, s), "strip_kernel (weights)");
+        }
+    }
     if (timed) check(hipEventRecord(ev_[2], s), "hipEventRecord");
     have_timing_ = timed;
 }
 
-void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum) {
+void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, WeightSums* wsum) {
     Context& ctx = Context::get();
     const size_t n = (size_t)n_rows_ * (nbins_ + 1);
     cnt.resize(n);
     dsum.resize(n);
     check(hipMemcpyAsync(cnt.data(), d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     check(hipMemcpyAsync(dsum.data(), d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    if (wsum) {
+        if (!weighted_) throw std::runtime_error("ausaxs_hip: weight sums of an engine without weights");
+        wsum->items.resize(n);
+        wsum->strips.resize(n);
+        check(hipMemcpyAsync(wsum->items.data(), d_wsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
+              "D2H");
+        check(hipMemcpyAsync(wsum->strips.data(), d_wstrip_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost,
+                             ctx.stream()),
+              "D2H");
+    }
     int flag = 0;
     if (!lat_.empty())
         check(hipMemcpyAsync(&flag, d_lat_flag_.p, sizeof(int), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
@@ -540,9 +612,9 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
                                  std::to_string(flag) + ")");
 }
 
-void PairEngine::run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, bool timed) {
+void PairEngine::run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, bool timed, WeightSums* wsum) {
     enqueue(timed);
-    download(cnt, dsum);
+    download(cnt, dsum, wsum);
 }
 
 EngineTiming PairEngine::timing() {

@@ -207,21 +207,33 @@ struct EngineTiming {
     int n_lattice = 0;          // lattice jobs
 };
 
+// Weight-pass results of a weighted engine, [n_rows][nbins+1] each: the work
+// items' FP64 sums (bits of doubles) and the tail strips' integer sums of
+// weight_code (device_api.hpp); weight_value combines them.
+struct WeightSums {
+    std::vector<int64_t> items, strips;
+};
+
 // Coordinates resident on the device with their work lists built once; run()
 // recomputes all jobs into [n_rows][nbins+1] (count, fixed-point delta sum).
+// With per-point weights (one vector per set, all sets single-class) a second
+// pass over the same work lists sums the pairs' weight products into
+// [n_rows][nbins+1] (weight_code sums, see device_api.hpp).
 class PairEngine {
   public:
     // lattice: lattice-path mode for PairJob::lattice jobs (see plan_lattice)
     PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-               int tile_j = 8000, int lattice = 1);
+               int tile_j = 8000, int lattice = 1, std::vector<std::vector<float>> weights = {});
     PairEngine(const PairEngine&) = delete;
     PairEngine& operator=(const PairEngine&) = delete;
 
     // enqueue all kernels on the context stream; results stay on device
     void enqueue(bool timed);
-    // copy results to host (synchronizes)
-    void download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum);
-    void run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, bool timed = false);
+    // copy results to host (synchronizes); weighted engines also fill wsum
+    void download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, WeightSums* wsum = nullptr);
+    void run(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, bool timed = false, WeightSums* wsum = nullptr);
+    bool weighted() const { return weighted_; }
+    double wscale() const { return wscale_; }
     EngineTiming timing();
 
     int n_rows() const { return n_rows_; }
@@ -258,6 +270,14 @@ class PairEngine {
 
     DevBuf<float4> d_pts_;
     std::vector<DevBuf<float4>> d_orig_;
+    // weight pass: the same buffers with weights in the fourth slot, and its sums
+    std::vector<std::vector<float>> weights_;
+    bool weighted_ = false;
+    double wscale_ = 1;
+    DevBuf<float4> d_pts_w_;
+    std::vector<DevBuf<float4>> d_orig_w_;
+    DevBuf<uint64_t> d_wcnt_;
+    DevBuf<int64_t> d_wsum_, d_wstrip_;
     DevBuf<PairWork> d_items_;
     DevBuf<int32_t> d_row_list_, d_chunk_ptr_, d_row_chunk_, d_rowmaps_;
     DevBuf<uint64_t> d_partial_, d_cnt_, d_tmp_cnt_;

@@ -259,6 +259,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     A.orig = to_points(atoms, n_atoms);
     W.orig = to_points(waters, n_waters);
     std::vector<PairJob> jobs;
+    std::vector<std::vector<float>> weights;  // weighted engine only
     int n_rows = 0;
 
     if (mode != Mode::Simple) {
@@ -363,7 +364,9 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
         }
     } else {
         // XYZW weights: one class per distinct weight value; w_i*w_j of a pair
-        // is then the float product of its two class weights.
+        // is then the float product of its two class weights. Sets with more
+        // than 64 distinct weights take the weighted engine instead: one class
+        // each, and a second pass that sums every pair's weight product.
         auto classify = [](const float* p, int n, PointSet& ps, std::vector<float>& w, std::vector<float>& cw) {
             std::map<uint32_t, int> id;
             w.resize(n);
@@ -379,14 +382,23 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
                 w[i] = wi;
                 ps.orig[i].w = float_of(it->second);
             }
-            if (cw.size() > 64)
-                throw std::runtime_error("ausaxs_hip: more than 64 distinct weights in one point set");
             if (cw.empty()) cw.push_back(0.f);  // empty set: one (unused) class keeps row tables non-empty
             ps.n_classes = std::max<int>(1, (int)cw.size());
         };
         std::vector<float> atom_w, water_w;
         classify(atoms, n_atoms, A, atom_w, atom_class_w_);
         classify(waters, n_waters, W, water_w, water_class_w_);
+        if (atom_class_w_.size() > kMaxWeightClasses || water_class_w_.size() > kMaxWeightClasses) {
+            weighted_ = true;
+            for (auto* ps : {&A, &W}) {
+                ps->n_classes = 1;
+                for (auto& v : ps->orig) v.w = float_of(0);
+            }
+            atom_class_w_.assign(1, 0.f);
+            water_class_w_.assign(1, 0.f);
+            weights.push_back(atom_w);
+            weights.push_back(water_w);
+        }
         // SimpleCalculator.h:144-160: sum of w*w (float products) in point order
         for (float x : atom_w) self_w2_atoms_ += (double)(x * x);
         for (float x : water_w) self_w2_waters_ += (double)(x * x);
@@ -427,11 +439,11 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     if (grid) sets.push_back(std::move(X));
     if (mode == Mode::FFGridSurface) sets.push_back(std::move(XS));
     engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j,
-                                           s.lattice);
+                                           s.lattice, std::move(weights));
 }
 
 std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool timed) {
-    engine_->run(cnt_, dsum_, timed);
+    engine_->run(cnt_, dsum_, timed, weighted_ ? &wsum_ : nullptr);
     const int nbins = engine_->nbins();
     const int stride = nbins + 1;
     if (engine_->overflowed(cnt_))
@@ -686,7 +698,13 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
             else wprod = (double)(water_class_w_[k[1]] * atom_class_w_[k[2]]), dst = &aw;
             for (int b = 0; b < nbins; ++b) {
                 WeightedEntry e = entry((int)r, b, 2);  // add<2> (TemplateHelperSimple.h; SimpleCalculator.h:195-207)
-                e.value *= wprod;
+                if (weighted_) {
+                    // the summed weight products of the bin's pairs
+                    const size_t cell = (size_t)r * stride + b;
+                    e.value = 2.0 * weight_value(wsum_.items[cell], wsum_.strips[cell], engine_->wscale());
+                } else {
+                    e.value *= wprod;
+                }
                 add((*dst)[b], e);
             }
         }

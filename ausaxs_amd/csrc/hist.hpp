@@ -20,6 +20,9 @@ namespace ausaxs::gpu {
 constexpr int kMaxFF = 15;  // settings/FormFactorSettings.h:12
 constexpr int kExvBin = 0;  // form_factor/FormFactorType.h:35
 constexpr int kWaterBin = 1;
+// XYZW sets with at most this many distinct weights keep one histogram row
+// per weight-class pair; more take the weighted engine (engine.hpp)
+constexpr size_t kMaxWeightClasses = 64;
 
 struct WeightedEntry {  // distribution/detail/WeightedEntry.h:16-50
     double value = 0;
@@ -160,6 +163,8 @@ class ResidentStructure {
     std::unique_ptr<PairEngine> engine_;
     std::vector<uint64_t> cnt_;
     std::vector<int64_t> dsum_;
+    bool weighted_ = false;       // Simple mode with more than kMaxWeightClasses distinct weights
+    WeightSums wsum_;             // weight-pass sums (weighted_)
 };
 
 // HistogramManagerMTFFGrid structure from FP64 atoms (n x 3, form-factor

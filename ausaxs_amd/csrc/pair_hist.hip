@@ -101,7 +101,7 @@ __device__ __forceinline__ void lds_add_at(uint32_t addr, uint64_t v) {
 // LDS offset of the row each point adds into (split items).
 struct Rows {
     float x[kRowsPerLane], y[kRowsPerLane], z[kRowsPerLane];
-    float tagf[kRowsPerLane];  // original index as a float (exact below 2^24)
+    float tagf[kRowsPerLane];  // original index as a float (exact below 2^24); weight pass: the weight
     int idx[kRowsPerLane];
     bool valid[kRowsPerLane];
     uint32_t base[kRowsPerLane];  // LDS byte address of the row's cell 0, minus 8 * kMagicBits (mod 2^32)
@@ -113,9 +113,11 @@ struct Rows {
 // larger: sat(tag_i - tag_j) is 1 exactly then (distinct integers differ by
 // at least 1), and t + sat * off1 stays an exact integer-valued float, so the
 // row choice costs two VALU operations per pair and no compare or select.
-template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT>
+// The weight pass (WEIGHT) bins the same way and adds the pair's weight
+// product (weight_code) instead of its count cell.
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT, bool WEIGHT>
 __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, const Consts& cs,
-                                            unsigned ovf, int off1) {
+                                            unsigned ovf, int off1, double wscale) {
     float t[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane], s[kRowsPerLane];
     bool near[kRowsPerLane];
     bool tie = false;
@@ -148,7 +150,8 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
             if (DIAG) keep = keep && (j > r.idx[k]);
             u = keep ? u : kMagicBits + ovf + 1u;
         }
-        lds_add_at(r.base[k] + (u << 3), cell_add(offset_code(d2[k], s[k], dl[k], cs)));
+        if (WEIGHT) lds_add_at(r.base[k] + (u << 3), weight_code(r.tagf[k] * pj.w, wscale));
+        else lds_add_at(r.base[k] + (u << 3), cell_add(offset_code(d2[k], s[k], dl[k], cs)));
     }
 }
 
@@ -158,9 +161,9 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
 // group are issued before the atomics of the other, so a wait for them never
 // waits for atomics (LDS operations retire in order) and no register copies
 // are needed between groups.
-template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT = false>
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT = false, bool WEIGHT = false>
 __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
-                                         float4* cols, const Consts& cs, int nbins) {
+                                         float4* cols, const Consts& cs, int nbins, double wscale) {
     const int t = threadIdx.x;
     const int part = t / kRowLanes;  // every part holds the same row points
     Rows r;
@@ -176,7 +179,7 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
         r.x[k] = p.x * cs.scale;
         r.y[k] = p.y * cs.scale;
         r.z[k] = p.z * cs.scale;
-        r.tagf[k] = (float)__float_as_int(p.w);
+        r.tagf[k] = WEIGHT ? p.w : (float)__float_as_int(p.w);
         r.idx[k] = i;
         // kSplit: rows from the second class add into the second LDS row
         const int roff = SPLIT && i >= w.i_begin + (w.flags >> kSplitShift) ? nbins + 2 : 0;
@@ -219,16 +222,17 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
             for (int u = 0; u < kColGroup; ++u) bq[u] = cc[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, a[u], jh + c + u, hist, cs, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, a[u], jh + c + u, hist, cs, ovf, off1, wscale);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u) a[u] = cc[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, bq[u], jh + c + kColGroup + u, hist, cs, ovf, off1);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, bq[u], jh + c + kColGroup + u, hist, cs, ovf, off1,
+                                                                     wscale);
         }
         for (int c = n2; c < n; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT>(r, cc[c], jh + c, hist, cs, ovf, off1);
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, cc[c], jh + c, hist, cs, ovf, off1, wscale);
     }
 }
 
@@ -237,19 +241,29 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
                                         float4* cols, const Consts& cs, int nbins) {
     const bool orient = (w.flags & kOrient) != 0;
     if (w.flags & kDiag) {
-        if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, cs, nbins);
-        else run_item<false, true, false, CLAMP>(pts, w, hist, cols, cs, nbins);
+        if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
+        else run_item<false, true, false, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
     } else if (w.flags & kSplit) {
         // a row tile spanning two classes of a cross job (never oriented)
-        if (w.i_end - w.i_begin >= kTileI) run_item<true, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins);
-        else run_item<false, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins);
+        if (w.i_end - w.i_begin >= kTileI) run_item<true, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins, 1.0);
+        else run_item<false, false, false, CLAMP, true>(pts, w, hist, cols, cs, nbins, 1.0);
     } else if (w.i_end - w.i_begin >= kTileI) {
-        if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, cs, nbins);
-        else run_item<true, false, false, CLAMP>(pts, w, hist, cols, cs, nbins);
+        if (orient) run_item<true, false, true, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
+        else run_item<true, false, false, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
     } else {
-        if (orient) run_item<false, false, true, CLAMP>(pts, w, hist, cols, cs, nbins);
-        else run_item<false, false, false, CLAMP>(pts, w, hist, cols, cs, nbins);
+        if (orient) run_item<false, false, true, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
+        else run_item<false, false, false, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
     }
+}
+
+// The weight pass: single-class point sets (no oriented or split items).
+template <bool CLAMP>
+__device__ __forceinline__ void run_weighted(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
+                                             float4* cols, const Consts& cs, int nbins, double wscale) {
+    if (w.flags & kDiag) run_item<false, true, false, CLAMP, false, true>(pts, w, hist, cols, cs, nbins, wscale);
+    else if (w.i_end - w.i_begin >= kTileI)
+        run_item<true, false, false, CLAMP, false, true>(pts, w, hist, cols, cs, nbins, wscale);
+    else run_item<false, false, false, CLAMP, false, true>(pts, w, hist, cols, cs, nbins, wscale);
 }
 
 // One workgroup per work item (grid = items), or a persistent grid that
@@ -268,7 +282,8 @@ __device__ __forceinline__ void run_any(const float4* __restrict__ pts, const Pa
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const PairWork* __restrict__ items, int n_items,
                                                              uint64_t* __restrict__ partial, Consts cs,
-                                                             int nbins, int clamp, int rows, int copies) {
+                                                             int nbins, int clamp, int rows, int copies,
+                                                             int wmode, double wscale) {
     extern __shared__ uint64_t hist[];
     __shared__ float4 cols[kColTile + kColGroup];
     const int stride = nbins + 2;
@@ -284,8 +299,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         __syncthreads();  // previous item's flush done
         for (int e = threadIdx.x; e < copies * rows * stride; e += kThreads) hist[e] = 0;
         __syncthreads();
-        if (clamp) run_any<true>(pts, w, mine, cols, cs, nbins);
-        else run_any<false>(pts, w, mine, cols, cs, nbins);
+        if (wmode) {
+            if (clamp) run_weighted<true>(pts, w, mine, cols, cs, nbins, wscale);
+            else run_weighted<false>(pts, w, mine, cols, cs, nbins, wscale);
+        } else if (clamp) {
+            run_any<true>(pts, w, mine, cols, cs, nbins);
+        } else {
+            run_any<false>(pts, w, mine, cols, cs, nbins);
+        }
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
         uint64_t* out = partial + (size_t)w.slot * 2 * out_stride;
@@ -301,13 +322,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
 __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __restrict__ partial,
                                                            const int32_t* __restrict__ row_list,
                                                            const int32_t* __restrict__ chunk_ptr, int nbins,
-                                                           uint64_t* __restrict__ tcnt, int64_t* __restrict__ tdsum) {
+                                                           uint64_t* __restrict__ tcnt, int64_t* __restrict__ tdsum,
+                                                           int raw) {
     const int c = blockIdx.y;
     const int stride = nbins + 1;
     const int bin = blockIdx.x * blockDim.x + threadIdx.x;
     if (bin >= stride) return;
     uint64_t n_sum = 0, low_sum = 0;
     const int e0 = chunk_ptr[c], e1 = chunk_ptr[c + 1];
+    if (raw) {
+        // weight pass: the items' exact integer sums, added in FP64 in a fixed order
+        double w = 0;
+        for (int e = e0; e < e1; ++e) w += (double)(int64_t)partial[(size_t)row_list[e] * stride + bin];
+        tcnt[(size_t)c * stride + bin] = 0;
+        tdsum[(size_t)c * stride + bin] = __double_as_longlong(w);
+        return;
+    }
 #pragma unroll 8
     for (int e = e0; e < e1; ++e) {
         uint64_t n, k;
@@ -323,7 +353,8 @@ __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __res
 __global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* __restrict__ tcnt,
                                                          const int64_t* __restrict__ tdsum,
                                                          const int32_t* __restrict__ row_chunk, int nbins,
-                                                         uint64_t* __restrict__ cnt, int64_t* __restrict__ dsum) {
+                                                         uint64_t* __restrict__ cnt, int64_t* __restrict__ dsum,
+                                                         int raw) {
     const int row = blockIdx.y;
     const int stride = nbins + 1;
     const int bin = blockIdx.x * blockDim.x + threadIdx.x;
@@ -331,6 +362,13 @@ __global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* __restr
     uint64_t c = 0;
     int64_t s = 0;
     const int c0 = row_chunk[row], c1 = row_chunk[row + 1];
+    if (raw) {  // weight pass: FP64 chunk sums (bits of doubles)
+        double w = 0;
+        for (int k = c0; k < c1; ++k) w += __longlong_as_double(tdsum[(size_t)k * stride + bin]);
+        cnt[(size_t)row * stride + bin] = 0;
+        dsum[(size_t)row * stride + bin] = __double_as_longlong(w);
+        return;
+    }
 #pragma unroll 4
     for (int k = c0; k < c1; ++k) {
         c += tcnt[(size_t)k * stride + bin];
@@ -376,13 +414,17 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     const int rep = blockIdx.y;
     const float4 a = p.rows[p.rows_rep * rep + i];
     const float4 c = p.cols[p.cols_rep * rep + t];
-    int row = p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
+    int row = p.wmode ? p.rowmap[0] : p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
     if (row < 0) return;  // the host guarantees a row for every present class pair
     row += rep * p.out_rep;
     unsigned bin;
     uint32_t code;
     strip_pair(p, make_consts(p.inv_w), a, c, i, t, bin, code);
     const size_t cell = (size_t)row * p.stride + bin;
+    if (p.wmode) {
+        atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)weight_code(a.w * c.w, p.wscale));
+        return;
+    }
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
     atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)(code - (uint32_t)kLowBias) - kDeltaBias));
 }
@@ -409,14 +451,19 @@ __global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
         if (w.self && i >= t) continue;
         unsigned bin;
         uint32_t code;
-        strip_pair(p, cs, p.rows[i], p.cols[t], i, t, bin, code);
-        lds_add(&cells[bin], cell_add(code));
+        const float4 a = p.rows[i], c = p.cols[t];
+        strip_pair(p, cs, a, c, i, t, bin, code);
+        lds_add(&cells[bin], p.wmode ? weight_code(a.w * c.w, p.wscale) : cell_add(code));
     }
     __syncthreads();
     const size_t row = (size_t)p.fixed_row * p.stride;
     for (int b = threadIdx.x; b <= p.nbins; b += blockDim.x) {
         const uint64_t v = cells[b];
         if (!v) continue;
+        if (p.wmode) {  // weight pass: the cell's integer sum
+            atomicAdd((unsigned long long*)&p.dsum[row + b], (unsigned long long)v);
+            continue;
+        }
         uint64_t n, ksum;
         cell_decode(v, n, ksum);
         atomicAdd((unsigned long long*)&p.cnt[row + b], (unsigned long long)n);
@@ -496,7 +543,7 @@ hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
     const size_t lds = pair_lds_bytes(p.nbins, p.rows, p.copies) - kStaticLds;
     const int grid = p.grid > 0 ? std::min(p.grid, p.n_items) : p.n_items;
     hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.items, p.n_items, p.partial,
-                       make_consts(p.inv_w), p.nbins, p.clamp, p.rows, p.copies);
+                       make_consts(p.inv_w), p.nbins, p.clamp, p.rows, p.copies, p.wmode, p.wscale);
     return hipGetLastError();
 }
 
@@ -505,11 +552,11 @@ hipError_t launch_merge(const MergeLaunch& m, hipStream_t s) {
     const unsigned bx = (unsigned)((m.nbins + 1 + 255) / 256);
     if (m.n_chunks > 0) {
         hipLaunchKernelGGL(merge_chunks_kernel, dim3(bx, m.n_chunks), dim3(256), 0, s, m.partial, m.row_list,
-                           m.chunk_ptr, m.nbins, m.tmp_cnt, m.tmp_dsum);
+                           m.chunk_ptr, m.nbins, m.tmp_cnt, m.tmp_dsum, m.raw);
         if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(merge_rows_kernel, dim3(bx, m.n_rows), dim3(256), 0, s, m.tmp_cnt, m.tmp_dsum, m.row_chunk,
-                       m.nbins, m.cnt, m.dsum);
+                       m.nbins, m.cnt, m.dsum, m.raw);
     return hipGetLastError();
 }
 

@@ -10,7 +10,6 @@ namespace ausaxs::gpu {
 
 namespace {
 constexpr int kMaxBatch = 1024;  // conformations per graph launch
-constexpr int kMaxClasses = 64;
 
 // bounding sphere (centre, radius) of n points given as x,y,z with stride
 void sphere_of(const double* p, int n, int stride, double c[3], double& r) {
@@ -37,12 +36,17 @@ struct RigidScan::Plan {
     std::vector<PairWork> items;
     MergePlan mp;
     std::vector<StripLaunch> strips;  // device pointers filled at build
+    std::vector<StripLaunch> wstrips;  // weighted scans: the same strips of the weight pass
     int64_t pairs = 0;
     int nq_cap = 0;
     double graph_cw = 0;
     int graph_nq = -1;
 
     DevBuf<float4> pts, orig;
+    DevBuf<float4> pts_w, orig_w;  // weighted scans: weights in the fourth slot
+    DevBuf<float> slot_w;
+    DevBuf<uint64_t> wcnt;
+    DevBuf<int64_t> wsum, wstrip;
     DevBuf<PairWork> d_items;
     DevBuf<int32_t> row_list, chunk_ptr, row_chunk, rowmaps;
     DevBuf<uint64_t> partial, cnt, tmp_cnt;
@@ -126,11 +130,27 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
         if ((int)in_water.size() <= c) in_water.resize(c + 1, false);
         in_water[c] = true;
     }
-    if (class_w_.size() > (size_t)kMaxClasses)
-        throw std::runtime_error("ausaxs_hip: more than 64 distinct weights in a rigid-body scan");
+    if (class_w_.size() > kMaxWeightClasses) {
+        // more distinct weights than class-pair rows can hold: one class, and
+        // a weight pass that sums every pair's weight product
+        weighted_ = true;
+        class_w_.assign(1, 0.f);
+        std::fill(point_cls_.begin(), point_cls_.end(), 0);
+        for (auto& v : waters_.orig) v.w = float_of(0);
+        in_water.assign(n_waters ? 1 : 0, true);
+    }
     C_ = std::max<int>(1, (int)class_w_.size());
     if (class_w_.empty()) class_w_.push_back(0.f);
     waters_.n_classes = C_;
+    water_w_ = water_w;
+    if (weighted_) {
+        double wmax = 0;
+        for (float v : w_) wmax = std::max(wmax, (double)std::fabs(v));
+        for (float v : water_w_) wmax = std::max(wmax, (double)std::fabs(v));
+        for (float v : w_)
+            if (!std::isfinite(v)) throw std::runtime_error("ausaxs_hip: weights must be finite");
+        wscale_ = weight_scale(wmax);
+    }
     for (int c = 0; c < (int)in_water.size(); ++c)
         if (in_water[c]) water_classes_.push_back(c);
 
@@ -186,11 +206,19 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
     base_nb_ = std::max(10, std::min<int>((int)s_.bin_count, bin_bound(ptrs, inv_w_)));
     std::vector<uint64_t> cnt;
     std::vector<int64_t> dsum;
+    WeightSums wsum;
+    double wscale = 1;
     {
-        PairEngine eng(sets, jobs, 2 * R_aa_, inv_w_, base_nb_, s_.tile_j ? s_.tile_j : 8000);
-        eng.run(cnt, dsum);
+        std::vector<std::vector<float>> weights;
+        if (weighted_) {
+            for (int b = 0; b < n_bodies; ++b) weights.emplace_back(w_.begin() + body_off_[b], w_.begin() + body_off_[b + 1]);
+            weights.push_back(water_w_);
+        }
+        PairEngine eng(sets, jobs, 2 * R_aa_, inv_w_, base_nb_, s_.tile_j ? s_.tile_j : 8000, 1, std::move(weights));
+        eng.run(cnt, dsum, false, weighted_ ? &wsum : nullptr);
         if (eng.overflowed(cnt))
             throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width");
+        wscale = eng.wscale();
     }
     const int stride = base_nb_ + 1;
     base_aa_.assign(base_nb_, WeightedEntry{});
@@ -203,7 +231,9 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
             if (!c) continue;
             const int64_t ds = dsum[(size_t)r * stride + b];
             WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
-            e.value = 2.0 * (double)c * row_w_[rr];
+            const size_t cell = (size_t)r * stride + b;
+            e.value = weighted_ ? 2.0 * weight_value(wsum.items[cell], wsum.strips[cell], wscale)
+                                : 2.0 * (double)c * row_w_[rr];
             e.count = (uint32_t)(2 * c);
             e.bin_center = 2.0 * distance_sum(c, b, ds, inv_w_);
             entry_add(dst[b], e);
@@ -305,7 +335,8 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     std::vector<SetLayout> L(1 + B);
     std::vector<std::vector<int32_t>> perm(B);
     std::vector<float4> buf;
-    L[0] = layout_set(sets[0], buf);
+    std::vector<int32_t> wperm;
+    L[0] = layout_set(sets[0], buf, weighted_ ? &wperm : nullptr);
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) p.S = (int64_t)buf.size();
         for (int b = 0; b < B; ++b)
@@ -318,9 +349,11 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
 
     // slots of the transform kernel
     std::vector<int32_t> src, body, dst, dsto, tag, cls;
+    std::vector<float> sw;
     for (int b = 0; b < B; ++b)
         for (int k = 0; k < sizes_[b]; ++k) {
             const int o = perm[b][k];
+            sw.push_back(w_[body_off_[b] + o]);
             src.push_back(body_off_[b] + o);
             body.push_back(b);
             dst.push_back(L[1 + b].base + k);
@@ -417,6 +450,23 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     p.slot_tag.upload(tag.data(), tag.size(), s);
     p.slot_cls.upload(cls.data(), cls.size(), s);
     p.body_moving.upload(moving.data(), moving.size(), s);
+    if (weighted_) {
+        // the weight pass's point copies: waters here, bodies by the transform kernel
+        p.pts_w.ensure(total);
+        p.orig_w.ensure(total);
+        p.slot_w.upload(sw.data(), sw.size(), s);
+        p.wcnt.ensure((size_t)n_rows * (nbins + 1));
+        p.wsum.ensure((size_t)n_rows * (nbins + 1));
+        p.wstrip.ensure((size_t)n_rows * (nbins + 1));
+        const int nw = waters_.size();
+        if (nw) {
+            std::vector<float4> a(buf.begin(), buf.begin() + nw), o(waters_.orig);
+            for (int i = 0; i < nw; ++i) a[i].w = water_w_[wperm[i]], o[i].w = water_w_[i];
+            check(hipMemcpyAsync(p.pts_w.p, a.data(), nw * sizeof(float4), hipMemcpyHostToDevice, s), "H2D waters");
+            check(hipMemcpyAsync(p.orig_w.p, o.data(), nw * sizeof(float4), hipMemcpyHostToDevice, s), "H2D waters");
+            ctx.sync();  // the host vectors go out of scope
+        }
+    }
     p.T.ensure((size_t)n_conf * B * 12);
     p.hT.ensure((size_t)n_conf * B * 12);
     p.H.ensure((size_t)n_conf * nbins);
@@ -439,6 +489,17 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     };
     for (const StripSpec& st : tmpl.strips) strip(st, st.rowmap_off, true);
     for (const StripSpec& st : shared.strips) strip(st, (int)tmpl.rowmaps.size() + st.rowmap_off, false);
+    if (weighted_)
+        for (StripLaunch sl : p.strips) {
+            // the same strip on the weight copies, into the weight sums (one row per job)
+            sl.rows = p.orig_w.p + (sl.rows - p.orig.p);
+            sl.cols = p.orig_w.p + (sl.cols - p.orig.p);
+            sl.cnt = p.wcnt.p;
+            sl.dsum = p.wstrip.p;
+            sl.wmode = 1;
+            sl.wscale = wscale_;
+            p.wstrips.push_back(sl);
+        }
 
     // every index the kernels will touch, checked on the host before any launch
     auto fail = [](const char* what) {
@@ -486,6 +547,7 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     }
     TransformLaunch tl{d_xyz_.p, p.T.p, p.slot_src.p, p.slot_body.p, p.slot_dst.p, p.slot_dsto.p, p.slot_tag.p,
                        p.slot_cls.p, p.body_moving.p, p.n_slots, B, p.n_conf, p.M, p.pts.p, p.orig.p};
+    if (weighted_) tl.slot_w = p.slot_w.p, tl.pts_w = p.pts_w.p, tl.orig_w = p.orig_w.p;
     check(launch_transform(tl, s), "transform_kernel");
     PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
                   p.copies};
@@ -494,9 +556,21 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
                    p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.cnt.p, p.dsum.p};
     check(launch_merge(ml, s), "merge kernels");
     for (const StripLaunch& sl : p.strips) check(launch_strip(sl, s), "strip_kernel");
+    if (weighted_) {
+        PairLaunch wl{p.pts_w.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
+                      p.copies, 1, wscale_};
+        check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
+        MergeLaunch wm{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * R_,
+                       p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.wcnt.p, p.wsum.p, 1};
+        check(launch_merge(wm, s), "merge kernels (weights)");
+        check(launch_lattice_zero_u64((unsigned long long*)p.wstrip.p, (size_t)(p.n_conf + 1) * R_ * (p.nbins + 1), s),
+              "lattice_zero_u64_kernel");
+        for (const StripLaunch& sl : p.wstrips) check(launch_strip(sl, s), "strip_kernel (weights)");
+    }
     FinalizeLaunch fl{p.cnt.p, p.dsum.p, R_, p.nbins, p.n_conf, d_row_kind_.p, d_row_w_.p, d_base_.p, base_nb_,
                       (double)inv_w_, (s_.bin_count * s_.bin_width) / s_.bin_count, cw, s_.weighted_bins ? 1 : 0,
                       p.H.p, p.dbar.p, p.max_bin.p, p.flags.p};
+    if (weighted_) fl.wsum = p.wsum.p, fl.wstrip = p.wstrip.p, fl.wscale = wscale_;
     check(launch_finalize(fl, s), "finalize_kernel");
     DebyeBatchLaunch dl{p.H.p, p.dbar.p, p.max_bin.p, p.q.p, p.nbins, p.n_conf, nq, p.I.p};
     check(launch_debye_batch(dl, s), "debye_batch_kernel");
@@ -662,7 +736,8 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     const int stride = p.nbins + 1;
     const int lc = c - last_conf0_;
     std::vector<uint64_t> cnt((size_t)2 * R_ * stride);
-    std::vector<int64_t> dsum((size_t)2 * R_ * stride);
+    std::vector<int64_t> dsum((size_t)2 * R_ * stride), wsum(weighted_ ? (size_t)2 * R_ * stride : 0),
+        wstrip(wsum.size());
     const size_t rows_bytes = (size_t)R_ * stride;
     for (int part = 0; part < 2; ++part) {
         const size_t row0 = part == 0 ? (size_t)lc * R_ : (size_t)p.n_conf * R_;
@@ -672,6 +747,14 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
         check(hipMemcpyAsync(dsum.data() + part * rows_bytes, p.dsum.p + row0 * stride,
                              rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
               "D2H");
+        if (weighted_) {
+            check(hipMemcpyAsync(wsum.data() + part * rows_bytes, p.wsum.p + row0 * stride,
+                                 rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
+                  "D2H");
+            check(hipMemcpyAsync(wstrip.data() + part * rows_bytes, p.wstrip.p + row0 * stride,
+                                 rows_bytes * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
+                  "D2H");
+        }
     }
     ctx.sync();
     const int nb = p.nbins;
@@ -687,7 +770,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
                 const uint64_t n = cnt[cell];
                 if (!n) continue;
                 WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
-                e.value = 2.0 * (double)n * row_w_[r];
+                e.value = weighted_ ? 2.0 * weight_value(wsum[cell], wstrip[cell], wscale_) : 2.0 * (double)n * row_w_[r];
                 e.count = (uint32_t)(2 * n);
                 e.bin_center = 2.0 * distance_sum(n, b, dsum[cell], inv_w_);
                 entry_add(row_kind_[r] == 0 ? aa[b] : aw[b], e);

@@ -26,6 +26,10 @@ __global__ __launch_bounds__(256) void transform_kernel(TransformLaunch p) {
     const size_t off = moving ? (size_t)c * p.rep_stride : 0;
     p.pts[p.slot_dst[k] + off] = make_float4(vx, vy, vz, __int_as_float(p.slot_tag[k]));
     p.orig[p.slot_dsto[k] + off] = make_float4(vx, vy, vz, __int_as_float(p.slot_cls[k]));
+    if (p.pts_w) {
+        p.pts_w[p.slot_dst[k] + off] = make_float4(vx, vy, vz, p.slot_w[k]);
+        p.orig_w[p.slot_dsto[k] + off] = make_float4(vx, vy, vz, p.slot_w[k]);
+    }
 }
 
 // One workgroup per conformation: aa/aw/ww/p_tot entries per bin from the
@@ -63,12 +67,13 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeLaunch p) {
                 // add<2> (SimpleCalculator.h:195-207): value 2 n w_i w_j, count 2 n, 2 sum d
                 const double nv = 2.0 * (double)n;
                 const double d = 2.0 * distance_sum(n, b, p.dsum[cell], (float)p.inv_w);
+                const double v = p.wsum ? 2.0 * weight_value(p.wsum[cell], p.wstrip[cell], p.wscale) : nv * p.row_w[r];
                 if (p.row_kind[r] == 0) {
-                    aa_v += nv * p.row_w[r];
+                    aa_v += v;
                     aa_c += nv;
                     aa_d += d;
                 } else {
-                    aw_v += nv * p.row_w[r];
+                    aw_v += v;
                     aw_c += nv;
                     aw_d += d;
                 }

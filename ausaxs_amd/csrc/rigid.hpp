@@ -61,6 +61,9 @@ class RigidScan {
     std::vector<float> class_w_;        // weight of each class
     std::vector<int32_t> point_cls_;    // class of each body point
     PointSet waters_;                   // float waters (class ids)
+    std::vector<float> water_w_;        // water weights
+    bool weighted_ = false;             // more than kMaxWeightClasses distinct weights: weight pass
+    double wscale_ = 1;                 // its fixed-point scale
     std::vector<int> water_classes_;    // classes present in the waters
     int C_ = 1;                         // classes
     int R_ = 0, R_aa_ = 0;              // rows per conformation

@@ -20,6 +20,11 @@ struct TransformLaunch {
     int64_t rep_stride;         // points per replica
     float4* pts;
     float4* orig;
+    // weighted scans: the weight pass's copies of pts and orig, with the
+    // point's weight (slot_w) in the fourth slot
+    const float* slot_w = nullptr;
+    float4* pts_w = nullptr;
+    float4* orig_w = nullptr;
 };
 hipError_t launch_transform(const TransformLaunch& p, hipStream_t s);
 
@@ -39,6 +44,11 @@ struct FinalizeLaunch {
     double* dbar;             // [n_conf][nbins]
     int* max_bin;             // [n_conf]
     int* flags;               // [n_conf] 1: a pair fell beyond nbins
+    // weighted scans (more than 64 distinct weights): values from the weight
+    // pass's sums instead of count * row_w (see weight_value)
+    const int64_t* wsum = nullptr;    // same layout as cnt: items' FP64 sums (bits)
+    const int64_t* wstrip = nullptr;  // strips' integer sums
+    double wscale = 1;
 };
 hipError_t launch_finalize(const FinalizeLaunch& p, hipStream_t s);
 

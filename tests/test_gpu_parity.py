@@ -173,6 +173,37 @@ def test_simple_manager_parity():
     compare_ff_hist(h, ref, A.MODE_SIMPLE)
 
 
+@pytest.mark.parametrize("n,m,lo,hi", [(1200, 203, 0.5, 16.0), (777, 0, -3.0, 9.0), (3001, 1499, 0.0, 1.0)])
+def test_simple_manager_any_weights(n, m, lo, hi):
+    # one weight per point (occupancies, effective charges: more than 64
+    # distinct values, zeros and negative ones included), as
+    # HistogramManagerMT / SimpleCalculator add<2> take them
+    # (SimpleCalculator.h:195-207): the weighted engine's values are the
+    # oracle's FP64 sums of float products to 1e-9
+    rng = np.random.default_rng(n + m)
+    atoms = globule(n, 16.0 * (n / 1200) ** (1 / 3), 9)
+    atoms[:, 3] = rng.uniform(lo, hi, n).astype(F32)
+    atoms[::17, 3] = 0.0
+    waters = A.synth_shell(7, max(m, 1), (16, 16, 16), 3.0)[:m]
+    waters[:, 3] = rng.uniform(0.2, 1.0, m).astype(F32)
+    h = A.calculate_all(A.MODE_SIMPLE, atoms, waters)
+    ref = O.hist_simple(atoms, waters, nthreads=4)
+    assert h.max_bin == ref["max_bin"]
+    for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww"), (A.PTOT, "p_tot")):
+        v, c, b = h.get(which)
+        rv = ref[key]
+        assert np.array_equal(c, rv["count"]), key
+        scale = np.max(np.abs(rv["value"])) or 1.0
+        np.testing.assert_allclose(v, rv["value"], rtol=1e-9, atol=1e-12 * scale, err_msg=key)
+    assert axis_ok(h.d_axis(), O.weighted_axis(ref["p_tot"], 0.25), 0.25, ref["p_tot"]["count"])
+    q = np.linspace(0.0, 0.5, 100)
+    I = h.debye_transform(q)
+    I_ref = O.debye_simple(ref, q, 0.25)
+    err = np.max(np.abs(I - I_ref)) / np.max(np.abs(I_ref))
+    print(f"weights in [{lo}, {hi}]: I(q) max error {err:.2e} of max |I|")
+    assert err <= 1e-6
+
+
 def test_cube_known_counts():
     atoms = A.points_ff(cube_atoms(), 3)
     waters = A.points_ff(cube_atoms(), 1)

@@ -103,6 +103,57 @@ def test_serve_admission_by_memory():
         tiny.submit(big["mode"], big["xyz"], big["ff"], big["wxyz"], big["q"])
 
 
+def oracle_intensity(mode, xyz, ff, wxyz, q, nthreads=8):
+    if mode == A.MODE_FFGRID:
+        h = O.hist_ffgrid(xyz, ff, wxyz, 0.25, 8000, nthreads=nthreads)
+        return O.debye_ffgrid(h, q, 0.25)[0]
+    om = O.FFAVG if mode == A.MODE_FFAVG else O.FFEXPLICIT
+    h = O.hist_ff(om, O.pack_ff(xyz, ff), O.pack_ff(wxyz, np.ones(len(wxyz), np.int32)), 0.25, 8000, 1.0, nthreads)
+    return O.debye_ff(om, h, q, 0.25)[0]
+
+
+def test_serve_benchmark_requests():
+    # the benchmark's request stream: structures of its pool (log-uniform
+    # 1k-100k atoms, 30 % waters), modes uniform over FF-avg / FF-explicit /
+    # FF-grid, 200 q, Poisson arrivals while earlier requests run; each result
+    # equals the same request served alone, and the oracle's for requests up to
+    # 20k atoms (6k for grid requests); the larger ones are finite (the
+    # contrast profiles' sum may change sign in the tail)
+    import time
+
+    import bench
+
+    cfg = bench.CONFIGS[5]
+    pool = bench.serve_pool(cfg, 5)
+    rng = np.random.default_rng(6)
+    q = np.linspace(0.0, 0.5, 200)
+    reqs = [(int(rng.integers(len(pool))), MODES[int(rng.integers(3))]) for _ in range(40)]
+    gaps = rng.exponential(0.01, len(reqs))
+    srv = A.Server(slots=cfg["slots"], budget_gb=cfg["budget_gb"])
+    ids = []
+    for (k, mode), g in zip(reqs, gaps):
+        xyz, ff, w = pool[k]
+        ids.append(srv.submit(mode, xyz, ff, w, q))
+        time.sleep(g)
+    results = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    print("stats:", st)
+    assert st["failed"] == 0 and st["completed"] == len(reqs)
+    alone = A.Server(slots=1, budget_gb=cfg["budget_gb"])
+    checked = 0
+    for (k, mode), I in zip(reqs, results):
+        xyz, ff, w = pool[k]
+        assert np.array_equal(I, alone.wait(alone.submit(mode, xyz, ff, w, q))[0]), (k, mode)
+        assert np.all(np.isfinite(I))
+        if len(xyz) <= (6000 if mode == A.MODE_FFGRID else 20000):
+            I_ref = oracle_intensity(mode, xyz, ff, w, q)
+            err = float(np.max(np.abs(I - I_ref) / np.abs(I_ref)))
+            print(f"request of {len(xyz)} atoms, mode {mode}: rel err {err:.2e}")
+            assert err <= 1e-4
+            checked += 1
+    assert checked >= 5
+
+
 def test_serve_errors():
     srv = A.Server(slots=2, budget_gb=1.0)
     r = make_request(5, 100, A.MODE_FFEXPLICIT)
