@@ -80,6 +80,8 @@ def _load():
         ),
         "ausaxs_structure_calculate": (C.c_int, [vp, P(vp)]),
         "ausaxs_structure_create_grid_surface": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
+        "ausaxs_structure_create_ffavg": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(C.c_double),
+                                                    P(vp)]),
         "ausaxs_hydrate_radial": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, P(C.c_int), vp,
                                             P(_HydrateSettings)]),
         "ausaxs_default_hydrate_settings": (None, [P(_HydrateSettings)]),
@@ -387,6 +389,22 @@ def grid_exv(atoms_xyz, ff, waters_xyz=None):
     _check(lib.ausaxs_grid_exv(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), _ptr(out), n.value, C.byref(n),
                                C.byref(vol)))
     return out[: n.value].copy(), vol.value
+
+
+class FFAvgStructure(Structure):
+    """HistogramManagerMTFFAvg from FP64 atoms and waters, with Z_exv from the
+    molecule's grid volume as the reference manager computes it."""
+
+    def __init__(self, atoms_xyz, ff, waters_xyz=None, settings=None):
+        a = np.ascontiguousarray(atoms_xyz, np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(ff, np.int32).reshape(-1)
+        w = np.ascontiguousarray(waters_xyz if waters_xyz is not None else np.zeros((0, 3)), np.float64).reshape(-1, 3)
+        s = settings or default_settings()
+        h, z = C.c_void_p(), C.c_double()
+        _check(lib.ausaxs_structure_create_ffavg(_ptr(a), _ptr(f), len(a), _ptr(w), len(w), C.byref(s), C.byref(z),
+                                                 C.byref(h)))
+        self._s = h
+        self.z_exv = z.value
 
 
 class GridStructure(Structure):

@@ -155,6 +155,21 @@ int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff
     });
 }
 
+int ausaxs_structure_create_ffavg(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                  const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                  double* z_exv, ausaxs_structure** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_atoms >= 0 && n_waters >= 0, "negative sizes");
+        require(n_atoms == 0 || (atoms_xyz && atom_ff), "atoms is NULL");
+        require(n_waters == 0 || waters_xyz, "waters is NULL");
+        auto s = std::make_unique<ausaxs_structure>();
+        s->s = make_ffavg_structure(atoms_xyz, atom_ff, n_atoms, waters_xyz, n_waters, to_settings(settings));
+        if (z_exv) *z_exv = s->s->z_exv();
+        *out = s.release();
+    });
+}
+
 int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
                                          const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
                                          ausaxs_structure** out) {

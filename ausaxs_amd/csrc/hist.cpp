@@ -981,6 +981,23 @@ std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, 
                                                (int)(g.xyz.size() / 3), g.volume);
 }
 
+std::unique_ptr<ResidentStructure> make_ffavg_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                        const double* waters_xyz, int m, const HistSettings& s) {
+    for (int i = 0; i < n; ++i)
+        if (ff[i] < 1 || ff[i] >= kMaxFF)
+            throw std::runtime_error("ausaxs_hip: atom form factor type outside [1, 14]");
+    // HistogramManagerMTFFAvg.cpp:355-359: Z_exv = V_grid * 0.334 e/A^3 / N,
+    // V_grid = Molecule::get_volume_grid() (Molecule.cpp:147-150): the
+    // volume of the molecule's grid (Grid(bodies), waters included) after
+    // expand_volume, which the excluded-volume grid builder counts
+    const double v_grid = n ? make_grid_exv(atoms_xyz, ff, n, waters_xyz, m).volume : 0.0;
+    const double z_exv = n ? v_grid * kWaterChargeDensity / n : 0.0;
+    std::vector<float> a, w;
+    pack_ff_points(atoms_xyz, ff, n, 0, a);
+    pack_ff_points(waters_xyz, nullptr, m, kWaterBin, w);
+    return std::make_unique<ResidentStructure>(Mode::FFAvg, a.data(), n, w.data(), m, s, z_exv);
+}
+
 std::unique_ptr<ResidentStructure> make_grid_surface_structure(const double* atoms_xyz, const int32_t* ff, int n,
                                                                const double* waters_xyz, int m,
                                                                const HistSettings& s) {

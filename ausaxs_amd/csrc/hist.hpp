@@ -149,6 +149,7 @@ class ResidentStructure {
     PairEngine& engine() { return *engine_; }
     Mode mode() const { return mode_; }
     void set_grid_exv_volume(double v) { grid_exv_volume_ = v; }
+    double z_exv() const { return z_exv_; }
 
   private:
     Mode mode_;
@@ -172,6 +173,12 @@ class ResidentStructure {
 // of the coordinates as CompactCoordinatesFF holds them.
 std::unique_ptr<ResidentStructure> make_grid_structure(const double* atoms_xyz, const int32_t* ff, int n,
                                                        const double* waters_xyz, int m, const HistSettings& s);
+// HistogramManagerMTFFAvg from FP64 atoms and waters with the reference's own
+// excluded-volume charge Z_exv = V_grid * 0.334 / N (HistogramManagerMTFFAvg.cpp:
+// 355-359; V_grid of the molecule's grid, Molecule.cpp:147-150)
+constexpr double kWaterChargeDensity = 0.334;  // constants::charge::density::water, e/A^3
+std::unique_ptr<ResidentStructure> make_ffavg_structure(const double* atoms_xyz, const int32_t* ff, int n,
+                                                        const double* waters_xyz, int m, const HistSettings& s);
 // HistogramManagerMTFFGridSurface (HistogramManagerMTFFGridSurface.cpp:16-274):
 // the dummies split into interior and surface ones (RawGridWithSurfaceExv)
 std::unique_ptr<ResidentStructure> make_grid_surface_structure(const double* atoms_xyz, const int32_t* ff, int n,

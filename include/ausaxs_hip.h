@@ -89,6 +89,16 @@ int ausaxs_structure_create_grid(const double* atoms_xyz, const int32_t* atom_ff
 int ausaxs_structure_create_grid_surface(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
                                          const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
                                          ausaxs_structure** out);
+/* HistogramManagerMTFFAvg from FP64 atoms (n x 3, form-factor types 1..14)
+ * and waters (m x 3) with the manager's own excluded-volume charge:
+ * Z_exv = V_grid * 0.334 / N (HistogramManagerMTFFAvg.cpp:355-359), V_grid the
+ * volume of the molecule's grid (Molecule::get_volume_grid, Molecule.cpp:
+ * 147-150; the same grid as ausaxs_grid_exv, whose volume it is). *z_exv
+ * (optional) receives the value used. The atoms and waters are cast to float
+ * as CompactCoordinatesFF holds them. */
+int ausaxs_structure_create_ffavg(const double* atoms_xyz, const int32_t* atom_ff, int n_atoms,
+                                  const double* waters_xyz, int n_waters, const ausaxs_settings* settings,
+                                  double* z_exv, ausaxs_structure** out);
 /* Radial hydration shell: RadialHydration::generate_explicit_hydration
  * (RadialHydration.cpp:40-80) per body on the molecule's grid, as
  * GridBasedHydration::hydrate runs it (GridBasedHydration.cpp:36-79) with the

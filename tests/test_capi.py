@@ -86,3 +86,38 @@ def test_gpu_entry_fails_loudly_without_device():
     atoms = A.points_ff(np.zeros((4, 3)), 3)
     with pytest.raises(A.AusaxsError, match="no HIP device"):
         A.calculate_all(A.MODE_FFEXPLICIT, atoms)
+
+
+def _compile(cmd, cwd):
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_header_is_plain_c(tmp_path):
+    # the header compiles as strict C99 and as C++20 with every warning an error
+    src = tmp_path / "h.c"
+    src.write_text('#include "ausaxs_hip.h"\nint main(void) { ausaxs_settings s; ausaxs_default_settings(&s);'
+                   " return s.bin_count == 8000 ? 0 : 1; }\n")
+    inc = str(ROOT / "include")
+    _compile(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", inc, "-fsyntax-only",
+              str(src)], tmp_path)
+    _compile(["g++", "-std=c++20", "-pedantic", "-Wall", "-Wextra", "-Werror", "-x", "c++", "-I", inc,
+              "-fsyntax-only", str(src)], tmp_path)
+
+
+def test_cpp_consumer_builds_against_header(tmp_path):
+    # a caller that binds the histogram managers through the header alone
+    # (tests/capi_consumer.cpp): it compiles warning-free, links, and on a
+    # host without a GPU stops at the first compute call with the library's
+    # message instead of computing anything on the CPU
+    import json
+
+    exe = tmp_path / "capi_consumer"
+    _compile(["g++", "-std=c++20", "-O1", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"),
+              str(ROOT / "tests" / "capi_consumer.cpp"), "-o", str(exe), "-L", str(A.LIB_PATH.parent),
+              "-lausaxs_hip", f"-Wl,-rpath,{A.LIB_PATH.parent}"], tmp_path)
+    if _has_gpu():
+        pytest.skip("a HIP device is present (the GPU test runs the consumer)")
+    r = subprocess.run([str(exe), "200"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3, r.stdout + r.stderr
+    assert "no HIP device" in json.loads(r.stdout)["error"]

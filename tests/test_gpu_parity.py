@@ -148,6 +148,25 @@ def test_ff_manager_parity(mode):
     compare_ff_hist(h, ref, mode)
 
 
+def test_ffavg_excluded_volume_charge_from_grid():
+    # HistogramManagerMTFFAvg.cpp:355-359: Z_exv = V_grid * 0.334 / N with the
+    # molecule's grid volume, here from FP64 atoms and waters; the oracle's
+    # grid gives the same volume, and the histograms and I(q) follow
+    a = globule(1500, 18.0, 12)
+    xyz, ff = a[:, :3].astype(np.float64), a[:, 3].view(np.int32).copy()
+    w = A.synth_shell(13, 400, (18, 18, 18), 3.0)[:, :3].astype(np.float64)
+    st = A.FFAvgStructure(xyz, ff, w)
+    vol = O.grid_exv(xyz, ff, w)[1]
+    z = vol * 0.334 / len(xyz)
+    assert st.z_exv == z, (st.z_exv, z)
+    h = st.calculate()
+    ref = O.hist_ff(O.FFAVG, O.pack_ff(xyz, ff), O.pack_ff(w, np.ones(len(w), np.int32)), z_exv=z, nthreads=4)
+    compare_ff_hist(h, ref, A.MODE_FFAVG)
+    q = np.linspace(0.0, 0.5, 100)
+    I_ref, _ = O.debye_ff(O.FFAVG, ref, q, 0.25)
+    assert relerr(h.debye_transform(q), I_ref) <= 1e-4
+
+
 def test_ff_manager_types_and_tails():
     # every active atom type, sizes with all residues mod 4, types interleaved
     rng = np.random.default_rng(4)
@@ -355,3 +374,48 @@ def test_ffgrid_large_invariants():
     assert c[0, 0].sum(dtype=np.int64) == nx * (nx - 1) + nx
     assert c[1:, 0].sum(dtype=np.int64) == n * nx
     assert h.get(A.AW)[1][0].sum(dtype=np.int64) == m * nx
+
+
+def test_cpp_consumer_matches_python_api():
+    # the C++ caller of the header (tests/capi_consumer.cpp, built by the
+    # package build) copies the FF-explicit and FF-avg distributions into the
+    # reference's layouts and runs I(q); its figures equal those of the same
+    # structures through the Python binding, sums taken in the same order
+    import json
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parent / "_bin" / "capi_consumer"
+    assert exe.exists(), f"{exe} is not built; run __graft_entry__.build()"
+    n = 1500
+    print("capi_consumer", n, flush=True)
+    r = subprocess.run([str(exe), str(n)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    m = n // 4
+    atoms = A.synth_globule(7, n, (18, 16, 14), 2.0)
+    waters = A.synth_shell(8, m, (18, 16, 14), 3.0)
+    q = A.default_q_axis(1e-4, 0.5)
+    assert out["nq"] == len(q)
+
+    def seqsum(x):
+        return float(np.cumsum(np.asarray(x, np.float64).ravel())[-1])
+
+    def check(key, h):
+        o = out[key]
+        assert o["max_bin"] == h.max_bin
+        assert o["aa_sum"] == seqsum(h.get(A.AA)[0])
+        assert o["aw_sum"] == seqsum(h.get(A.AW)[0])
+        assert o["ww_sum"] == seqsum(h.get(A.WW)[0])
+        v, c, b = h.get(A.PTOT)
+        assert o["ptot_value_sum"] == seqsum(v)
+        assert o["ptot_count_sum"] == int(c.astype(np.int64).sum())
+        assert o["ptot_center_sum"] == seqsum(b)
+        assert o["d_sum"] == seqsum(h.d_axis())
+        assert np.array_equal(np.array(o["I"]), h.debye_transform(q))
+
+    check("ffexplicit", A.calculate_all(A.MODE_FFEXPLICIT, atoms, waters, z_exv=0.0))
+    xyz = atoms[:, :3].astype(np.float64)
+    st = A.FFAvgStructure(xyz, atoms[:, 3].view(np.int32).copy(), waters[:, :3].astype(np.float64))
+    assert out["z_exv"] == st.z_exv
+    check("ffavg", st.calculate())
