@@ -104,6 +104,7 @@ def _load():
         "ausaxs_copy_probe": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
         "ausaxs_rigid_create": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, C.c_int, vp, P(vp)]),
         "ausaxs_rigid_scan": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]),
+        "ausaxs_rigid_scan_hydrated": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_double, vp]),
         "ausaxs_rigid_histogram": (C.c_int, [vp, C.c_int, P(vp)]),
         "ausaxs_rigid_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int),
                                         P(C.c_longlong), P(C.c_double)]),
@@ -539,6 +540,23 @@ class RigidScan:
         q = np.ascontiguousarray(q, np.float64)
         I = np.zeros((len(T), len(q)))
         _check(lib.ausaxs_rigid_scan(self._r, _ptr(T), len(T), _ptr(q), len(q), cw, _ptr(I)))
+        return I
+
+    def scan_hydrated(self, T, waters, q, cw=1.0):
+        """Every conformation with its own waters: waters[c] = (xyz (m x 3), weights (m or scalar))."""
+        T = np.ascontiguousarray(T, np.float64).reshape(-1, len(self.sizes), 12)
+        if len(waters) != len(T):
+            raise ValueError("one water set per conformation")
+        counts = np.array([len(np.asarray(x).reshape(-1, 3)) for x, _ in waters], np.int32)
+        wx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x, _ in waters])
+                                  if len(waters) else np.zeros((0, 3)), np.float64)
+        ww = np.ascontiguousarray(np.concatenate([np.broadcast_to(np.asarray(w, np.float32), (n,))
+                                                  for (_, w), n in zip(waters, counts)])
+                                  if len(waters) else np.zeros(0), np.float32)
+        q = np.ascontiguousarray(q, np.float64)
+        I = np.zeros((len(T), len(q)))
+        _check(lib.ausaxs_rigid_scan_hydrated(self._r, _ptr(T), len(T), _ptr(wx), _ptr(ww), _ptr(counts), _ptr(q),
+                                              len(q), cw, _ptr(I)))
         return I
 
     def histogram(self, c) -> CompositeHistogram:

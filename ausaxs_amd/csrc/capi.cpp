@@ -771,6 +771,17 @@ int ausaxs_rigid_scan(ausaxs_rigid* r, const double* transforms, int n_conf, con
     });
 }
 
+int ausaxs_rigid_scan_hydrated(ausaxs_rigid* r, const double* transforms, int n_conf, const double* waters_xyz,
+                               const float* waters_w, const int32_t* water_counts, const double* q, int nq,
+                               double cw, double* I) {
+    return guarded([&] {
+        require(r && r->r && n_conf >= 0 && nq >= 0, "bad argument");
+        require((n_conf == 0 || (transforms && water_counts)) && (nq == 0 || q) && (n_conf == 0 || nq == 0 || I),
+                "bad argument");
+        r->r->scan_hydrated(transforms, n_conf, waters_xyz, waters_w, water_counts, q, nq, cw, I);
+    });
+}
+
 int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram** out) {
     return guarded([&] {
         require(r && r->r && out, "bad argument");

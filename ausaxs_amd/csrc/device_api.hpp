@@ -182,6 +182,16 @@ struct StripLaunch {
     double wscale = 1;
 };
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
+// Strips table[0..n) (device) in one launch, each with n_rep = 1: strip k
+// takes blocks [first[k], first[k+1]) of 256 pairs, first[n] = blocks
+// (device, int64). See strip_table_blocks for the host-side prefix.
+hipError_t launch_strip_table(const StripLaunch* table, const int64_t* first, int n, int64_t blocks,
+                              hipStream_t s);
+// blocks of 256 pairs a strip takes in launch_strip_table
+inline int64_t strip_table_blocks(const StripLaunch& p) {
+    const int64_t n = (int64_t)p.w.n_rows * (p.w.t_end - p.w.t_begin);
+    return n > 0 ? (n + 255) / 256 : 0;
+}
 
 // Exhaustive check of the binning arithmetic over float d2 in [lo, hi] (bit
 // patterns). out[0]: lanes where the pair kernel's bin (raw v_sqrt + tie

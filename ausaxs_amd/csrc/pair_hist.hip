@@ -402,16 +402,14 @@ __device__ __forceinline__ void strip_pair(const StripLaunch& p, const Consts& c
     bin = min((unsigned)(int)b, (unsigned)p.nbins);
 }
 
-// Strip kernel: rows x tail columns, one thread per pair, global atomics.
-__global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
+// One strip pair per thread (pair tid of replica rep), added with global atomics.
+__device__ __forceinline__ void strip_one(const StripLaunch& p, long long tid, int rep) {
     const StripWork& w = p.w;
     const int nt = w.t_end - w.t_begin;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (long long)w.n_rows * nt) return;
     const int i = (int)(tid / nt);
     const int t = w.t_begin + (int)(tid % nt);
     if (w.self && i >= t) return;
-    const int rep = blockIdx.y;
     const float4 a = p.rows[p.rows_rep * rep + i];
     const float4 c = p.cols[p.cols_rep * rep + t];
     int row = p.wmode ? p.rowmap[0] : p.rowmap[__float_as_int(a.w) * w.n_col_classes + __float_as_int(c.w)];
@@ -427,6 +425,27 @@ __global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
     }
     atomicAdd((unsigned long long*)&p.cnt[cell], 1ull);
     atomicAdd((unsigned long long*)&p.dsum[cell], (unsigned long long)((int64_t)(code - (uint32_t)kLowBias) - kDeltaBias));
+}
+
+// Strip kernel: rows x tail columns, one thread per pair, global atomics.
+__global__ __launch_bounds__(256) void strip_kernel(StripLaunch p) {
+    strip_one(p, (long long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// Many strips in one launch: strip k owns blocks [first[k], first[k+1]) of a
+// one-dimensional grid, 256 pairs per block (replica 0 only). A block finds
+// its strip by binary search over first; the descriptor is read from global
+// memory (every thread of the block reads the same bytes).
+__global__ __launch_bounds__(256) void strip_table_kernel(const StripLaunch* table, const int64_t* first, int n) {
+    const int64_t blk = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const StripLaunch p = table[lo];
+    strip_one(p, (blk - first[lo]) * 256 + threadIdx.x, 0);
 }
 
 // Strip kernel for a strip with one output row (fixed_row) and no replicas:
@@ -573,6 +592,14 @@ hipError_t launch_strip(const StripLaunch& p, hipStream_t s) {
     }
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(strip_kernel, dim3(blocks, p.n_rep), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_table(const StripLaunch* table, const int64_t* first, int n, int64_t blocks,
+                              hipStream_t s) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(strip_table_kernel, dim3((unsigned)blocks), dim3(256), 0, s, table, first, n);
     return hipGetLastError();
 }
 

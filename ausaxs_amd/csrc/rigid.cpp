@@ -25,7 +25,19 @@ void sphere_of(const double* p, int n, int stride, double c[3], double& r) {
         r = std::max(r, std::sqrt(d2));
     }
 }
+
+// a set's layout moved to another place of the point buffer
+void shift_layout(SetLayout& L, int64_t off) {
+    L.base = (int)(L.base + off);
+    for (int& v : L.seg_start) v = (int)(v + off);
+}
 }  // namespace
+
+struct RigidScan::Hydration {
+    std::vector<PointSet> sets;         // per conformation: float waters, weight class ids
+    std::vector<std::vector<float>> w;  // per conformation: weights, original order
+    std::vector<double> sphere;         // per conformation: bounding-sphere centre (3) and radius
+};
 
 struct RigidScan::Plan {
     int n_conf = 0, nbins = 0, tile_j = 0, grid = 0;
@@ -59,6 +71,22 @@ struct RigidScan::Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4]{};
+
+    // rows per conformation: [aa class pairs | aw water class x class] or, in a
+    // hydration-modified batch, [aa | aw, every water class x class | ww class pairs]
+    int R = 0;
+    bool hydrated = false;
+    std::vector<int32_t> row_kind;  // hydration-modified batches (else the scan's own)
+    std::vector<double> row_w;
+    DevBuf<int32_t> d_row_kind;
+    DevBuf<double> d_row_w;
+    std::vector<double> ww_self;    // [n_conf][2]: each water set's self term
+    DevBuf<double> d_ww_self;
+    // tail strips of the per-conformation water jobs, one launch for all
+    int n_table = 0;
+    int64_t table_blocks = 0;
+    DevBuf<StripLaunch> d_table, d_wtable;
+    DevBuf<int64_t> d_first;
 
     Plan() {
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
@@ -265,12 +293,17 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
     ctx.sync();
 }
 
-int RigidScan::bound_bins(const double* T, int n_conf) const {
+int RigidScan::bound_bins(const double* T, int n_conf, const Hydration* hy) const {
     // bounding spheres moved by each conformation; ||R x|| <= ||R||_F ||x||
     const int B = n_bodies();
-    double dmax = 2 * std::max(wradius_, 0.0), cmax = 0;
+    double dmax = hy ? 0.0 : 2 * std::max(wradius_, 0.0), cmax = 0;
     std::vector<double> c(3 * (size_t)B), r(B);
     for (int k = 0; k < n_conf; ++k) {
+        // the waters of this conformation
+        const double* wc = hy ? &hy->sphere[4 * (size_t)k] : wcenter_;
+        const double wr = hy ? hy->sphere[4 * (size_t)k + 3] : wradius_;
+        dmax = std::max(dmax, 2 * wr);
+        for (int e = 0; e < 3 && wr >= 0; ++e) cmax = std::max(cmax, std::fabs(wc[e]) + wr);
         for (int b = 0; b < B; ++b) {
             const double* t = T + ((size_t)k * B + b) * 12;
             double f = 0;
@@ -293,9 +326,9 @@ int RigidScan::bound_bins(const double* T, int n_conf) const {
                 const double dx = c[3 * i] - c[3 * j], dy = c[3 * i + 1] - c[3 * j + 1], dz = c[3 * i + 2] - c[3 * j + 2];
                 dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz) + r[i] + r[j]);
             }
-            if (wradius_ >= 0) {
-                const double dx = c[3 * i] - wcenter_[0], dy = c[3 * i + 1] - wcenter_[1], dz = c[3 * i + 2] - wcenter_[2];
-                dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz) + r[i] + wradius_);
+            if (wr >= 0) {
+                const double dx = c[3 * i] - wc[0], dy = c[3 * i + 1] - wc[1], dz = c[3 * i + 2] - wc[2];
+                dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz) + r[i] + wr);
             }
         }
     }
@@ -311,7 +344,7 @@ int RigidScan::bound_bins(const double* T, int n_conf) const {
 }
 
 std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::vector<int>& moving, int nbins,
-                                                       int tile_j) {
+                                                       int tile_j, const Hydration* hy) {
     check_pair_lds(nbins);
     check_tile_j(tile_j);
     const int B = n_bodies();
@@ -321,11 +354,18 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     p.nbins = nbins;
     p.tile_j = tile_j;
     p.moving = moving;
-    const int n_rows = (n_conf + 1) * R_;
+    p.hydrated = hy != nullptr;
+    const int R = hy ? R_aa_ + C_ * C_ + R_aa_ : R_;
+    p.R = R;
+    if ((int64_t)(n_conf + 1) * R >= std::numeric_limits<int32_t>::max() / (nbins + 1))
+        throw std::runtime_error("ausaxs_hip: too many histogram rows in one rigid-body batch");
+    const int n_rows = (n_conf + 1) * R;
 
-    // set 0 = waters, 1 + b = body b; layout: waters, static bodies (shared), moving bodies (replica 0)
+    // set 0 = waters (none shared in a hydration-modified batch), 1 + b = body b;
+    // layout: waters, static bodies (shared), moving bodies (replica 0)
     std::vector<PointSet> sets(1 + B);
-    sets[0] = waters_;
+    if (hy) sets[0].n_classes = C_;
+    else sets[0] = waters_;
     for (int b = 0; b < B; ++b) {
         sets[1 + b].n_classes = C_;
         sets[1 + b].orig.resize(sizes_[b]);
@@ -343,9 +383,26 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
             if ((moving[b] != 0) == (pass == 1)) L[1 + b] = layout_set(sets[1 + b], buf, &perm[b]);
     }
     p.M = (int64_t)buf.size() - p.S;
-    const int64_t total = p.S + (int64_t)n_conf * p.M;
+    const int64_t body_total = p.S + (int64_t)n_conf * p.M;
+    int64_t water_total = 0;
+    if (hy)
+        for (const PointSet& w : hy->sets) water_total += w.size();
+    const int64_t total = body_total + water_total;
     if (total >= std::numeric_limits<int32_t>::max())
         throw std::runtime_error("ausaxs_hip: too many points in one rigid-body batch");
+    // hydration-modified batches: each conformation's waters after the body replicas
+    std::vector<SetLayout> LW;
+    std::vector<float4> wbuf, worig;
+    std::vector<int32_t> wperm_h;
+    if (hy) {
+        for (int c = 0; c < n_conf; ++c) {
+            LW.push_back(layout_set(hy->sets[c], wbuf, weighted_ ? &wperm_h : nullptr));
+            shift_layout(LW.back(), body_total);
+        }
+        worig.resize(wbuf.size());
+        for (int c = 0; c < n_conf; ++c)
+            std::copy(hy->sets[c].orig.begin(), hy->sets[c].orig.end(), worig.begin() + (LW[c].base - body_total));
+    }
 
     // slots of the transform kernel
     std::vector<int32_t> src, body, dst, dsto, tag, cls;
@@ -371,7 +428,7 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
         for (int b = 0; b < C_; ++b) aw_map[water_classes_[k] * C_ + b] = R_aa_ + (int)k * C_ + b;
     auto shifted = [&](std::vector<int32_t> m) {
         for (auto& r : m)
-            if (r >= 0) r += n_conf * R_;
+            if (r >= 0) r += n_conf * R;
         return m;
     };
     auto make_lists = [&](bool split, WorkList& shared, WorkList& tmpl) {
@@ -381,7 +438,7 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
                 append_job(mv ? tmpl : shared, PairJob{1 + j, 1 + i, mv ? aa_map : shifted(aa_map)}, sets, L,
                            n_rows, tile_j, split);
             }
-            if (waters_.size() > 0)
+            if (!hy && waters_.size() > 0)
                 append_job(moving[i] ? tmpl : shared, PairJob{0, 1 + i, moving[i] ? aw_map : shifted(aw_map)}, sets,
                            L, n_rows, tile_j, split);
         }
@@ -393,11 +450,43 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     const int64_t work_s = padded_work(shared_s.items) + (int64_t)n_conf * padded_work(tmpl_s.items);
     const int rows = std::max(item_rows(shared.items), item_rows(tmpl.items));
     const int rows_s = std::max(item_rows(shared_s.items), item_rows(tmpl_s.items));
-    if (prefer_split(work, rows, work_s, rows_s)) {
+    const bool use_split = prefer_split(work, rows, work_s, rows_s);
+    if (use_split) {
         shared = std::move(shared_s);
         tmpl = std::move(tmpl_s);
     }
     p.pairs = shared.pairs + (int64_t)n_conf * tmpl.pairs;
+
+    // hydration-modified batches: per conformation, every body against its
+    // waters (calc_aw) and the waters' self-correlation (calc_ww), rows absolute
+    WorkList hyd;
+    std::vector<std::pair<int64_t, int64_t>> hyd_base;  // per strip of hyd: row and column set bases
+    if (hy) {
+        std::vector<PointSet> hs(1 + B);
+        for (int b = 0; b < B; ++b) hs[1 + b] = sets[1 + b];
+        std::vector<SetLayout> hl(1 + B);
+        std::vector<int32_t> awm((size_t)C_ * C_), wwm((size_t)C_ * C_);
+        for (int c = 0; c < n_conf; ++c) {
+            for (int a = 0; a < C_; ++a)
+                for (int b = 0; b < C_; ++b) {
+                    awm[a * C_ + b] = c * R + R_aa_ + a * C_ + b;
+                    wwm[a * C_ + b] = c * R + R_aa_ + C_ * C_ + tri(a, b);
+                }
+            hs[0] = hy->sets[c];
+            hl[0] = LW[c];
+            for (int b = 0; b < B; ++b) {
+                hl[1 + b] = L[1 + b];
+                if (moving[b]) shift_layout(hl[1 + b], (int64_t)c * p.M);
+            }
+            const size_t s0 = hyd.strips.size();
+            if (hs[0].size() > 0)
+                for (int i = 0; i < B; ++i) append_job(hyd, PairJob{0, 1 + i, awm}, hs, hl, n_rows, tile_j, use_split);
+            append_job(hyd, PairJob{0, -1, wwm}, hs, hl, n_rows, tile_j, use_split);
+            for (size_t k = s0; k < hyd.strips.size(); ++k)
+                hyd_base.emplace_back(hl[hyd.strips[k].set_rows].base, hl[hyd.strips[k].set_cols].base);
+        }
+        p.pairs += hyd.pairs;
+    }
 
     // items: shared ones, then each template item for every conformation
     sort_items(shared.items);
@@ -411,10 +500,12 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
             w.i_end = t.i_begin >= p.S ? w.i_begin + (t.i_end - t.i_begin) : t.i_end;
             w.j_begin = shift(t.j_begin, c);
             w.j_end = t.j_begin >= p.S ? w.j_begin + (t.j_end - t.j_begin) : t.j_end;
-            w.row0 += c * R_;
-            w.row1 += c * R_;
+            w.row0 += c * R;
+            w.row1 += c * R;
             p.items.push_back(w);
         }
+    p.items.insert(p.items.end(), hyd.items.begin(), hyd.items.end());
+    if (hy) sort_items(p.items);
     p.mp = plan_merge(p.items, n_rows);
     p.rows = item_rows(p.items);
     p.copies = pick_copies(nbins, p.rows);
@@ -424,7 +515,15 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     hipStream_t s = ctx.stream();
     p.pts.ensure(total);
     p.orig.ensure(total);
-    if (waters_.size()) {
+    if (hy && !wbuf.empty()) {
+        check(hipMemcpyAsync(p.pts.p + body_total, wbuf.data(), wbuf.size() * sizeof(float4), hipMemcpyHostToDevice,
+                             s),
+              "H2D waters");
+        check(hipMemcpyAsync(p.orig.p + body_total, worig.data(), worig.size() * sizeof(float4),
+                             hipMemcpyHostToDevice, s),
+              "H2D waters");
+    }
+    if (!hy && waters_.size()) {
         check(hipMemcpyAsync(p.pts.p, buf.data(), waters_.size() * sizeof(float4), hipMemcpyHostToDevice, s),
               "H2D waters");
         check(hipMemcpyAsync(p.orig.p, waters_.orig.data(), waters_.size() * sizeof(float4), hipMemcpyHostToDevice,
@@ -437,6 +536,8 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     p.row_chunk.upload(p.mp.row_chunk.data(), p.mp.row_chunk.size(), s);
     std::vector<int32_t> rowmaps = tmpl.rowmaps;
     rowmaps.insert(rowmaps.end(), shared.rowmaps.begin(), shared.rowmaps.end());
+    const int hyd_rowmap_off = (int)rowmaps.size();
+    rowmaps.insert(rowmaps.end(), hyd.rowmaps.begin(), hyd.rowmaps.end());
     p.rowmaps.upload(rowmaps.data(), rowmaps.size(), s);
     p.partial.ensure(std::max<size_t>(p.items.size(), 1) * 2 * (nbins + 1));
     p.tmp_cnt.ensure((size_t)p.mp.n_chunks() * (nbins + 1));
@@ -458,7 +559,26 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
         p.wcnt.ensure((size_t)n_rows * (nbins + 1));
         p.wsum.ensure((size_t)n_rows * (nbins + 1));
         p.wstrip.ensure((size_t)n_rows * (nbins + 1));
-        const int nw = waters_.size();
+        std::vector<float4> hw_pts, hw_orig;
+        if (hy && !wbuf.empty()) {
+            hw_pts = wbuf;
+            hw_orig = worig;
+            for (int c = 0; c < n_conf; ++c) {
+                const int64_t b0 = LW[c].base - body_total;
+                for (int i = 0; i < hy->sets[c].size(); ++i) {
+                    hw_pts[b0 + i].w = hy->w[c][wperm_h[b0 + i]];
+                    hw_orig[b0 + i].w = hy->w[c][i];
+                }
+            }
+            check(hipMemcpyAsync(p.pts_w.p + body_total, hw_pts.data(), hw_pts.size() * sizeof(float4),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D waters");
+            check(hipMemcpyAsync(p.orig_w.p + body_total, hw_orig.data(), hw_orig.size() * sizeof(float4),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D waters");
+            ctx.sync();  // the host vectors go out of scope
+        }
+        const int nw = hy ? 0 : waters_.size();
         if (nw) {
             std::vector<float4> a(buf.begin(), buf.begin() + nw), o(waters_.orig);
             for (int i = 0; i < nw; ++i) a[i].w = water_w_[wperm[i]], o[i].w = water_w_[i];
@@ -484,11 +604,75 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
         sl.n_rep = replicated ? n_conf : 1;
         sl.rows_rep = replicated && rm ? p.M : 0;
         sl.cols_rep = replicated && cm ? p.M : 0;
-        sl.out_rep = replicated ? R_ : 0;
+        sl.out_rep = replicated ? R : 0;
         p.strips.push_back(sl);
     };
     for (const StripSpec& st : tmpl.strips) strip(st, st.rowmap_off, true);
     for (const StripSpec& st : shared.strips) strip(st, (int)tmpl.rowmaps.size() + st.rowmap_off, false);
+    // the water jobs' strips: one table, launched at once
+    std::vector<StripLaunch> table, wtable;
+    std::vector<int64_t> first;
+    for (size_t k = 0; k < hyd.strips.size(); ++k) {
+        const StripSpec& st = hyd.strips[k];
+        const int64_t rb = hyd_base[k].first, cb = hyd_base[k].second;
+        StripLaunch sl{p.orig.p + rb, p.orig.p + cb, st.w, p.rowmaps.p + hyd_rowmap_off + st.rowmap_off, p.cnt.p,
+                       p.dsum.p, inv_w_, nbins, nbins + 1};
+        const int64_t nb = strip_table_blocks(sl);
+        if (!nb) continue;
+        // bounds of every strip, checked here before any launch
+        if (rb < 0 || rb + st.w.n_rows > total || cb + st.w.t_begin < 0 || cb + st.w.t_end > total)
+            throw std::runtime_error("ausaxs_hip: rigid-body plan out of bounds: water strip");
+        const size_t off = (size_t)hyd_rowmap_off + st.rowmap_off, nmap = (size_t)C_ * st.w.n_col_classes;
+        if (off + nmap > rowmaps.size())
+            throw std::runtime_error("ausaxs_hip: rigid-body plan out of bounds: water strip rowmap");
+        for (size_t e = 0; e < nmap; ++e)
+            if (rowmaps[off + e] >= n_rows)
+                throw std::runtime_error("ausaxs_hip: rigid-body plan out of bounds: water strip row");
+        first.push_back(p.table_blocks);
+        p.table_blocks += nb;
+        table.push_back(sl);
+        if (weighted_) {
+            sl.rows = p.orig_w.p + rb;
+            sl.cols = p.orig_w.p + cb;
+            sl.cnt = p.wcnt.p;
+            sl.dsum = p.wstrip.p;
+            sl.wmode = 1;
+            sl.wscale = wscale_;
+            wtable.push_back(sl);
+        }
+    }
+    if (p.table_blocks > 0x7fffffff) throw std::runtime_error("ausaxs_hip: too many water strip pairs in one batch");
+    p.n_table = (int)table.size();
+    if (p.n_table) {
+        first.push_back(p.table_blocks);
+        p.d_table.upload(table.data(), table.size(), s);
+        p.d_first.upload(first.data(), first.size(), s);
+        if (weighted_) p.d_wtable.upload(wtable.data(), wtable.size(), s);
+    }
+    if (hy) {
+        // row kinds and weights of this batch's rows
+        p.row_kind.assign(R, 0);
+        p.row_w.assign(R, 0.0);
+        for (int a = 0; a < C_; ++a)
+            for (int b = 0; b < C_; ++b) {
+                const double w = (double)(class_w_[a] * class_w_[b]);
+                p.row_w[tri(a, b)] = w;
+                p.row_kind[R_aa_ + a * C_ + b] = 1;
+                p.row_w[R_aa_ + a * C_ + b] = w;
+                p.row_kind[R_aa_ + C_ * C_ + tri(a, b)] = 2;
+                p.row_w[R_aa_ + C_ * C_ + tri(a, b)] = w;
+            }
+        p.d_row_kind.upload(p.row_kind.data(), p.row_kind.size(), s);
+        p.d_row_w.upload(p.row_w.data(), p.row_w.size(), s);
+        p.ww_self.assign(2 * (size_t)n_conf, 0.0);
+        for (int c = 0; c < n_conf; ++c) {
+            double t = 0;  // SimpleCalculator.h:144-160: sum of w^2 at bin 0
+            for (float v : hy->w[c]) t += (double)(v * v);
+            p.ww_self[2 * c] = t;
+            p.ww_self[2 * c + 1] = (double)(uint32_t)t;
+        }
+        p.d_ww_self.upload(p.ww_self.data(), p.ww_self.size(), s);
+    }
     if (weighted_)
         for (StripLaunch sl : p.strips) {
             // the same strip on the weight copies, into the weight sums (one row per job)
@@ -527,7 +711,7 @@ std::unique_ptr<RigidScan::Plan> RigidScan::build_plan(int n_conf, const std::ve
     }
     for (int k = 0; k < p.n_slots; ++k) {
         const int64_t reach = moving[body[k]] ? (int64_t)(n_conf - 1) * p.M : 0;
-        const int64_t lim = moving[body[k]] ? total : p.S;
+        const int64_t lim = moving[body[k]] ? body_total : p.S;
         if (dst[k] < 0 || dst[k] + reach >= lim || dsto[k] < 0 || dsto[k] + reach >= lim) fail("transform slot");
         if (src[k] < 0 || src[k] >= body_off_.back()) fail("transform source");
     }
@@ -552,25 +736,31 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
                   p.copies};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
-    MergeLaunch ml{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * R_,
+    MergeLaunch ml{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * p.R,
                    p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.cnt.p, p.dsum.p};
     check(launch_merge(ml, s), "merge kernels");
     for (const StripLaunch& sl : p.strips) check(launch_strip(sl, s), "strip_kernel");
+    check(launch_strip_table(p.d_table.p, p.d_first.p, p.n_table, p.table_blocks, s), "strip_table_kernel");
     if (weighted_) {
         PairLaunch wl{p.pts_w.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
                       p.copies, 1, wscale_};
         check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
-        MergeLaunch wm{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * R_,
+        MergeLaunch wm{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * p.R,
                        p.nbins, p.tmp_cnt.p, p.tmp_dsum.p, p.wcnt.p, p.wsum.p, 1};
         check(launch_merge(wm, s), "merge kernels (weights)");
-        check(launch_lattice_zero_u64((unsigned long long*)p.wstrip.p, (size_t)(p.n_conf + 1) * R_ * (p.nbins + 1), s),
+        check(launch_lattice_zero_u64((unsigned long long*)p.wstrip.p, (size_t)(p.n_conf + 1) * p.R * (p.nbins + 1), s),
               "lattice_zero_u64_kernel");
         for (const StripLaunch& sl : p.wstrips) check(launch_strip(sl, s), "strip_kernel (weights)");
+        check(launch_strip_table(p.d_wtable.p, p.d_first.p, p.n_table, p.table_blocks, s),
+              "strip_table_kernel (weights)");
     }
-    FinalizeLaunch fl{p.cnt.p, p.dsum.p, R_, p.nbins, p.n_conf, d_row_kind_.p, d_row_w_.p, d_base_.p, base_nb_,
+    FinalizeLaunch fl{p.cnt.p, p.dsum.p, p.R, p.nbins, p.n_conf,
+                      p.hydrated ? p.d_row_kind.p : d_row_kind_.p, p.hydrated ? p.d_row_w.p : d_row_w_.p,
+                      p.hydrated ? d_base_aa_.p : d_base_.p, base_nb_,
                       (double)inv_w_, (s_.bin_count * s_.bin_width) / s_.bin_count, cw, s_.weighted_bins ? 1 : 0,
                       p.H.p, p.dbar.p, p.max_bin.p, p.flags.p};
     if (weighted_) fl.wsum = p.wsum.p, fl.wstrip = p.wstrip.p, fl.wscale = wscale_;
+    if (p.hydrated) fl.ww_self = p.d_ww_self.p;
     check(launch_finalize(fl, s), "finalize_kernel");
     DebyeBatchLaunch dl{p.H.p, p.dbar.p, p.max_bin.p, p.q.p, p.nbins, p.n_conf, nq, p.I.p};
     check(launch_debye_batch(dl, s), "debye_batch_kernel");
@@ -667,7 +857,12 @@ void RigidScan::run_batch(Plan& p, const double* T, const double* q, int nq, dou
         p.nq_cap = nq;
     }
     std::memcpy(p.hq.p, q, (size_t)nq * sizeof(double));
-    if (!p.exec || p.graph_nq != nq || p.graph_cw != cw) {
+    if (p.hydrated) {
+        // a plan of one call: launched directly, no graph to capture
+        check(hipEventRecord(p.ev[2], s), "hipEventRecord");
+        enqueue(p, nq, cw, true);
+        check(hipEventRecord(p.ev[3], s), "hipEventRecord");
+    } else if (!p.exec || p.graph_nq != nq || p.graph_cw != cw) {
         p.drop_graph();
         // the whole batch as one graph: copies in, kernels, copies out (no
         // memset nodes: every output the graph reads back is written by a kernel)
@@ -685,9 +880,11 @@ void RigidScan::run_batch(Plan& p, const double* T, const double* q, int nq, dou
         p.graph_nq = nq;
         p.graph_cw = cw;
     }
-    check(hipEventRecord(p.ev[2], s), "hipEventRecord");
-    check(hipGraphLaunch(p.exec, s), "hipGraphLaunch");
-    check(hipEventRecord(p.ev[3], s), "hipEventRecord");
+    if (!p.hydrated) {
+        check(hipEventRecord(p.ev[2], s), "hipEventRecord");
+        check(hipGraphLaunch(p.exec, s), "hipGraphLaunch");
+        check(hipEventRecord(p.ev[3], s), "hipEventRecord");
+    }
     ctx.sync();
     float ms = 0;
     check(hipEventElapsedTime(&ms, p.ev[2], p.ev[3]), "hipEventElapsedTime");
@@ -699,6 +896,28 @@ void RigidScan::run_batch(Plan& p, const double* T, const double* q, int nq, dou
     std::memcpy(I, p.hI.p, (size_t)p.n_conf * nq * sizeof(double));
 }
 
+std::vector<int> RigidScan::moving_bodies(const double* T, int n_conf) const {
+    // bodies whose transform is not the same in every conformation of the batch
+    const int B = n_bodies();
+    std::vector<int> moving(B, 0);
+    for (int b = 0; b < B; ++b)
+        for (int c = 1; c < n_conf && !moving[b]; ++c)
+            moving[b] = std::memcmp(T + ((size_t)c * B + b) * 12, T + (size_t)b * 12, 12 * sizeof(double)) != 0;
+    return moving;
+}
+
+void RigidScan::record(Plan& p, int c0, int nc) {
+    info_.tile_j = p.tile_j;
+    info_.grid = p.grid;
+    info_.copies = p.copies;
+    info_.n_items = (int)p.items.size();
+    info_.batch = nc;
+    info_.pairs += p.pairs;
+    last_ = &p;
+    last_conf0_ = c0;
+    last_n_conf_ = nc;
+}
+
 void RigidScan::scan(const double* T, int n_conf, const double* q, int nq, double cw, double* I) {
     if (n_conf < 0 || nq < 0) throw std::runtime_error("ausaxs_hip: negative conformation or q count");
     const int B = n_bodies();
@@ -707,23 +926,98 @@ void RigidScan::scan(const double* T, int n_conf, const double* q, int nq, doubl
     for (int c0 = 0; c0 < n_conf; c0 += kMaxBatch) {
         const int nc = std::min(kMaxBatch, n_conf - c0);
         const double* Tc = T + (size_t)c0 * B * 12;
-        // bodies whose transform is the same in every conformation of the batch
-        std::vector<int> moving(B, 0);
-        for (int b = 0; b < B; ++b)
-            for (int c = 1; c < nc && !moving[b]; ++c)
-                moving[b] = std::memcmp(Tc + ((size_t)c * B + b) * 12, Tc + (size_t)b * 12, 12 * sizeof(double)) != 0;
+        const std::vector<int> moving = moving_bodies(Tc, nc);
         const int nbins = bound_bins(Tc, nc);
         Plan& p = plan_for(nc, moving, nbins, Tc);
         run_batch(p, Tc, q, nq, cw, I + (size_t)c0 * nq);
-        info_.tile_j = p.tile_j;
-        info_.grid = p.grid;
-        info_.copies = p.copies;
-        info_.n_items = (int)p.items.size();
-        info_.batch = nc;
-        info_.pairs += p.pairs;
-        last_ = &p;
-        last_conf0_ = c0;
-        last_n_conf_ = nc;
+        record(p, c0, nc);
+    }
+}
+
+void RigidScan::scan_hydrated(const double* T, int n_conf, const double* wxyz, const float* ww,
+                              const int32_t* counts, const double* q, int nq, double cw, double* I) {
+    if (n_conf < 0 || nq < 0) throw std::runtime_error("ausaxs_hip: negative conformation or q count");
+    if (n_conf > 0 && !counts) throw std::runtime_error("ausaxs_hip: NULL water counts");
+    int64_t n_w = 0;
+    for (int c = 0; c < n_conf; ++c) {
+        if (counts[c] < 0) throw std::runtime_error("ausaxs_hip: negative water count");
+        n_w += counts[c];
+    }
+    if (n_w > 0 && (!wxyz || !ww)) throw std::runtime_error("ausaxs_hip: NULL waters");
+    for (int64_t i = 0; i < 3 * n_w; ++i)
+        if (!std::isfinite(wxyz[i])) throw std::runtime_error("ausaxs_hip: water coordinates must be finite");
+    for (int64_t i = 0; i < n_w; ++i)
+        if (!std::isfinite(ww[i])) throw std::runtime_error("ausaxs_hip: weights must be finite");
+    const int B = n_bodies();
+    // weight classes of the waters: those of the scan (unless it runs a weight pass)
+    std::map<uint32_t, int> cls;
+    for (int c = 0; c < (int)class_w_.size(); ++c) {
+        uint32_t key;
+        std::memcpy(&key, &class_w_[c], 4);
+        cls.emplace(key, c);
+    }
+    double wmax = 0;
+    for (int64_t i = 0; i < n_w; ++i) wmax = std::max(wmax, (double)std::fabs(ww[i]));
+    if (weighted_ && weight_scale(wmax) < wscale_)
+        throw std::runtime_error("ausaxs_hip: a water weight is larger in magnitude than every weight given at "
+                                 "creation");
+    if (!d_base_aa_.p) {
+        // the cached self terms of the bodies only: the creation waters' ww is replaced
+        std::vector<double> base(6 * (size_t)base_nb_, 0.0);
+        for (int b = 0; b < base_nb_; ++b) {
+            base[b] = base_aa_[b].value;
+            base[base_nb_ + b] = base_aa_[b].count;
+            base[2 * base_nb_ + b] = base_aa_[b].bin_center;
+        }
+        Context& ctx = Context::get();
+        d_base_aa_.upload(base.data(), base.size(), ctx.stream());
+        ctx.sync();
+    }
+    info_.total_ms = 0;
+    info_.pairs = 0;
+    int64_t off = 0;
+    for (int c0 = 0; c0 < n_conf; c0 += kMaxBatch) {
+        const int nc = std::min(kMaxBatch, n_conf - c0);
+        const double* Tc = T + (size_t)c0 * B * 12;
+        Hydration hy;
+        hy.sets.resize(nc);
+        hy.w.resize(nc);
+        hy.sphere.assign(4 * (size_t)nc, 0.0);
+        for (int c = 0; c < nc; ++c) {
+            const int m = counts[c0 + c];
+            PointSet& ps = hy.sets[c];
+            ps.n_classes = C_;
+            ps.orig.resize(m);
+            hy.w[c].assign(ww + off, ww + off + m);
+            std::vector<double> pd(3 * (size_t)m);
+            for (int i = 0; i < m; ++i) {
+                const double* x = wxyz + 3 * (off + i);
+                int k = 0;
+                if (!weighted_) {
+                    uint32_t key;
+                    std::memcpy(&key, &ww[off + i], 4);
+                    auto it = cls.find(key);
+                    if (it == cls.end())
+                        throw std::runtime_error("ausaxs_hip: a water weight that no point of the scan's creation "
+                                                 "has (its weight classes are fixed then)");
+                    k = it->second;
+                }
+                // CompactCoordinates: float cast of the FP64 coordinates
+                ps.orig[i] = make_float4((float)x[0], (float)x[1], (float)x[2], float_of(k));
+                for (int e = 0; e < 3; ++e) pd[3 * i + e] = (&ps.orig[i].x)[e];
+            }
+            sphere_of(pd.data(), m, 3, &hy.sphere[4 * (size_t)c], hy.sphere[4 * (size_t)c + 3]);
+            off += m;
+        }
+        const std::vector<int> moving = moving_bodies(Tc, nc);
+        const int nbins = bound_bins(Tc, nc, &hy);
+        // the tile width of the scan's tuned plans, else the default
+        const int tile = s_.tile_j ? s_.tile_j : (info_.tile_j ? info_.tile_j : 8000);
+        if (last_ == hplan_.get()) last_ = nullptr;
+        hplan_.reset();
+        hplan_ = build_plan(nc, moving, nbins, tile, &hy);
+        run_batch(*hplan_, Tc, q, nq, cw, I + (size_t)c0 * nq);
+        record(*hplan_, c0, nc);
     }
 }
 
@@ -735,12 +1029,15 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     Context& ctx = Context::get();
     const int stride = p.nbins + 1;
     const int lc = c - last_conf0_;
-    std::vector<uint64_t> cnt((size_t)2 * R_ * stride);
-    std::vector<int64_t> dsum((size_t)2 * R_ * stride), wsum(weighted_ ? (size_t)2 * R_ * stride : 0),
+    const int R = p.R;
+    const std::vector<int32_t>& row_kind = p.hydrated ? p.row_kind : row_kind_;
+    const std::vector<double>& row_w = p.hydrated ? p.row_w : row_w_;
+    std::vector<uint64_t> cnt((size_t)2 * R * stride);
+    std::vector<int64_t> dsum((size_t)2 * R * stride), wsum(weighted_ ? (size_t)2 * R * stride : 0),
         wstrip(wsum.size());
-    const size_t rows_bytes = (size_t)R_ * stride;
+    const size_t rows_bytes = (size_t)R * stride;
     for (int part = 0; part < 2; ++part) {
-        const size_t row0 = part == 0 ? (size_t)lc * R_ : (size_t)p.n_conf * R_;
+        const size_t row0 = part == 0 ? (size_t)lc * R : (size_t)p.n_conf * R;
         check(hipMemcpyAsync(cnt.data() + part * rows_bytes, p.cnt.p + row0 * stride, rows_bytes * sizeof(uint64_t),
                              hipMemcpyDeviceToHost, ctx.stream()),
               "D2H");
@@ -761,20 +1058,22 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
     for (int b = 0; b < std::min(nb, base_nb_); ++b) {
         aa[b] = base_aa_[b];
-        ww[b] = base_ww_[b];
+        if (!p.hydrated) ww[b] = base_ww_[b];
     }
     for (int part = 0; part < 2; ++part)
-        for (int r = 0; r < R_; ++r)
+        for (int r = 0; r < R; ++r)
             for (int b = 0; b < nb; ++b) {
-                const size_t cell = (size_t)(part * R_ + r) * stride + b;
+                const size_t cell = (size_t)(part * R + r) * stride + b;
                 const uint64_t n = cnt[cell];
                 if (!n) continue;
                 WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
-                e.value = weighted_ ? 2.0 * weight_value(wsum[cell], wstrip[cell], wscale_) : 2.0 * (double)n * row_w_[r];
+                e.value = weighted_ ? 2.0 * weight_value(wsum[cell], wstrip[cell], wscale_) : 2.0 * (double)n * row_w[r];
                 e.count = (uint32_t)(2 * n);
                 e.bin_center = 2.0 * distance_sum(n, b, dsum[cell], inv_w_);
-                entry_add(row_kind_[r] == 0 ? aa[b] : aw[b], e);
+                entry_add(row_kind[r] == 0 ? aa[b] : row_kind[r] == 1 ? aw[b] : ww[b], e);
             }
+    if (p.hydrated)  // this conformation's water set's self term (SimpleCalculator.h:144-160)
+        entry_add(ww[0], WeightedEntry{p.ww_self[2 * lc], (uint32_t)p.ww_self[2 * lc + 1], 0});
     std::vector<WeightedEntry> ptot(nb);
     for (int b = 0; b < nb; ++b) {
         ptot[b] = aa[b];

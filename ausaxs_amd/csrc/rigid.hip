@@ -33,7 +33,8 @@ __global__ __launch_bounds__(256) void transform_kernel(TransformLaunch p) {
 }
 
 // One workgroup per conformation: aa/aw/ww/p_tot entries per bin from the
-// conformation's rows, the shared rows and the cached self terms; the
+// conformation's rows, the shared rows and the cached self terms (and, with
+// per-conformation waters, the water set's own self term); the
 // weighted axis, H = aa + cw aw + cw^2 ww, max_bin and the overflow flag.
 __global__ __launch_bounds__(256) void finalize_kernel(FinalizeLaunch p) {
     const int c = blockIdx.x;
@@ -68,16 +69,25 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeLaunch p) {
                 const double nv = 2.0 * (double)n;
                 const double d = 2.0 * distance_sum(n, b, p.dsum[cell], (float)p.inv_w);
                 const double v = p.wsum ? 2.0 * weight_value(p.wsum[cell], p.wstrip[cell], p.wscale) : nv * p.row_w[r];
-                if (p.row_kind[r] == 0) {
+                const int kind = p.row_kind[r];
+                if (kind == 0) {
                     aa_v += v;
                     aa_c += nv;
                     aa_d += d;
-                } else {
+                } else if (kind == 1) {
                     aw_v += v;
                     aw_c += nv;
                     aw_d += d;
+                } else {
+                    ww_v += v;
+                    ww_c += nv;
+                    ww_d += d;
                 }
             }
+        }
+        if (b == 0 && p.ww_self) {
+            ww_v += p.ww_self[2 * c];
+            ww_c += p.ww_self[2 * c + 1];
         }
         const double tv = (aa_v + aw_v) + ww_v, tc = (aa_c + aw_c) + ww_c, td = (aa_d + aw_d) + ww_d;
         double dbar = b * p.width;

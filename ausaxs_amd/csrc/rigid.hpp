@@ -46,6 +46,13 @@ class RigidScan {
 
     // T: [n_conf][n_bodies][12] (R row-major, t); I: [n_conf][nq]
     void scan(const double* T, int n_conf, const double* q, int nq, double cw, double* I);
+    // The hydration-modified step (PartialHistogramManagerMT.cpp:83-105, after
+    // the re-hydration of SimpleController.cpp:78): conformation c has its own
+    // waters, counts[c] rows of wxyz (FP64 n x 3, fixed in space, in
+    // conformation order) with weights ww, which replace the waters given at
+    // creation; aw of every body and ww are recomputed for each conformation.
+    void scan_hydrated(const double* T, int n_conf, const double* wxyz, const float* ww, const int32_t* counts,
+                       const double* q, int nq, double cw, double* I);
     // composite histogram of conformation c of the last scan
     std::unique_ptr<CompositeDistanceHistogramHIP> histogram(int c);
     const RigidInfo& info() const { return info_; }
@@ -53,6 +60,7 @@ class RigidScan {
 
   private:
     struct Plan;
+    struct Hydration;  // per-conformation waters of one batch
     HistSettings s_;
     float inv_w_;
     std::vector<int32_t> sizes_, body_off_;
@@ -78,6 +86,8 @@ class RigidScan {
     std::vector<double> center_;        // per body: bounding-sphere centre (3) and radius
     double wcenter_[3] = {0, 0, 0}, wradius_ = -1;
     std::map<std::tuple<int, std::vector<int>, int>, std::unique_ptr<Plan>> plans_;
+    std::unique_ptr<Plan> hplan_;       // plan of the last hydration-modified batch
+    DevBuf<double> d_base_aa_;          // the cached self terms without the creation waters' ww
     Plan* last_ = nullptr;
     int last_conf0_ = 0;                // first conformation of the last batch
     int last_n_conf_ = 0;
@@ -85,8 +95,11 @@ class RigidScan {
 
     int tri(int a, int b) const;  // unordered class-pair row
     Plan& plan_for(int n_conf, const std::vector<int>& moving, int nbins, const double* T);
-    std::unique_ptr<Plan> build_plan(int n_conf, const std::vector<int>& moving, int nbins, int tile_j);
-    int bound_bins(const double* T, int n_conf) const;
+    std::unique_ptr<Plan> build_plan(int n_conf, const std::vector<int>& moving, int nbins, int tile_j,
+                                     const Hydration* hy = nullptr);
+    int bound_bins(const double* T, int n_conf, const Hydration* hy = nullptr) const;
+    std::vector<int> moving_bodies(const double* T, int n_conf) const;
+    void record(Plan& p, int c0, int nc);
     void run_batch(Plan& p, const double* T, const double* q, int nq, double cw, double* I);
     void enqueue(Plan& p, int nq, double cw, bool copies);
     float time_pair(Plan& p, int grid, int copies);

@@ -32,7 +32,7 @@ struct FinalizeLaunch {
     const uint64_t* cnt;   // [(n_conf + 1) * R][nbins + 1]: conformation rows, then the shared rows
     const int64_t* dsum;
     int R, nbins, n_conf;
-    const int32_t* row_kind;  // [R] 0 aa, 1 aw
+    const int32_t* row_kind;  // [R] 0 aa, 1 aw, 2 ww (waters that change per conformation)
     const double* row_w;      // [R] float(w_a * w_b) of the row
     const double* base;       // [6][base_nb]: aa value, count, sum d; ww value, count, sum d
     int base_nb;
@@ -49,6 +49,10 @@ struct FinalizeLaunch {
     const int64_t* wsum = nullptr;    // same layout as cnt: items' FP64 sums (bits)
     const int64_t* wstrip = nullptr;  // strips' integer sums
     double wscale = 1;
+    // per-conformation waters: [n_conf][2] self term of each water set (sum of
+    // w^2 and its unsigned cast, SimpleCalculator.h:144-160), added at bin 0
+    // after the ww rows; the base then holds no water terms
+    const double* ww_self = nullptr;
 };
 hipError_t launch_finalize(const FinalizeLaunch& p, hipStream_t s);
 

@@ -208,6 +208,20 @@ int ausaxs_rigid_create(const double* xyz, const float* w, const int32_t* body_s
 int ausaxs_rigid_scan(ausaxs_rigid* r, const double* transforms, int n_conf, const double* q, int nq, double cw,
                       double* I);
 
+/* The hydration-modified step of PartialHistogramManagerMT::calculate_all
+ * (PartialHistogramManagerMT.cpp:83-105: calc_ww, and calc_aw for every body),
+ * as after SimpleController::step re-hydrates the moved molecule
+ * (source/rigidbody/controller/SimpleController.cpp:78). Conformation c has its
+ * own waters: water_counts[c] consecutive rows of waters_xyz (FP64 x, y, z in
+ * the frame of the moved bodies; cast to float as CompactCoordinates does) and
+ * waters_w, in conformation order. They replace the waters given at creation;
+ * each weight must be one that a body point or creation water has, unless the
+ * scan has more than 64 distinct weights. The bodies' self terms stay cached.
+ * ausaxs_rigid_histogram then returns this scan's conformations. */
+int ausaxs_rigid_scan_hydrated(ausaxs_rigid* r, const double* transforms, int n_conf, const double* waters_xyz,
+                               const float* waters_w, const int32_t* water_counts, const double* q, int nq,
+                               double cw, double* I);
+
 /* The composite histogram (SIMPLE layout) of one conformation of the last
  * batch of the last scan (batches hold up to 1024 conformations). */
 int ausaxs_rigid_histogram(ausaxs_rigid* r, int conformation, ausaxs_histogram** out);

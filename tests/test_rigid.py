@@ -225,3 +225,140 @@ def test_rigid_benchmark_scan_512_conformations():
         print(f"conformation {c}: I(q) rel err {err:.2e}")
         assert err <= 1e-5
     assert np.array_equal(scan.scan(T, q), I)
+
+
+def water_shell(rng, centre, m, r0, r1):
+    """m waters with radii uniform in [r0, r1] around centre (FP64 m x 3)."""
+    v = rng.normal(size=(m, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    return centre + v * rng.uniform(r0, r1, (m, 1))
+
+
+def as_xyzw(xyz, w):
+    out = np.zeros((len(xyz), 4), np.float32)
+    out[:, :3] = np.asarray(xyz).astype(np.float32)
+    out[:, 3] = w
+    return out
+
+
+@pytest.mark.parametrize("static", [(), (1,)])
+def test_rigid_scan_hydrated_parity(static):
+    # waters that change with every conformation (the re-hydrated step): aw of
+    # every body and ww recomputed per conformation; water counts with every
+    # residue mod 4, one conformation without waters
+    bodies, weights = make_bodies([301, 498, 703], seed=71)
+    rng = np.random.default_rng(72)
+    w0 = water_shell(rng, np.array([26.0, 0, 0]), 150, 20, 30)
+    scan = A.RigidScan(bodies, weights, w0, np.ones(150, np.float32))
+    T = random_transforms(rng, bodies, 5, static)
+    counts = [150, 97, 0, 203, 62]
+    waters = [(water_shell(rng, np.array([26.0, 0, 0]) + rng.uniform(-3, 3, 3), m, 18, 32), 1.0) for m in counts]
+    q = np.linspace(0.01, 0.5, 60)
+    I = scan.scan_hydrated(T, waters, q)
+    for c in range(len(T)):
+        ref = oracle_conf(bodies, weights, T[c], as_xyzw(waters[c][0], 1.0))
+        check_hist(scan.histogram(c), ref)
+        err = relerr(I[c], O.debye_simple(ref, q, 0.25))
+        print(f"static {static} conformation {c} ({counts[c]} waters): I(q) rel err {err:.2e}")
+        assert err <= 1e-5
+
+
+def test_rigid_scan_hydrated_same_waters_as_fixed():
+    # the creation waters passed to every conformation give the fixed-water scan
+    bodies, weights = make_bodies([250, 333, 417], seed=81)
+    rng = np.random.default_rng(82)
+    w0 = water_shell(rng, np.array([26.0, 0, 0]), 121, 20, 30)
+    scan = A.RigidScan(bodies, weights, w0, np.ones(121, np.float32))
+    T = random_transforms(rng, bodies, 4)
+    q = np.linspace(0.0, 0.6, 50)
+    I_fixed = scan.scan(T, q)
+    h_fixed = [scan.histogram(c) for c in range(len(T))]
+    I_hyd = scan.scan_hydrated(T, [(w0, 1.0)] * len(T), q)
+    for c in range(len(T)):
+        h = scan.histogram(c)
+        for which in (A.AA, A.AW, A.WW, A.PTOT):
+            a, b = h.get(which), h_fixed[c].get(which)
+            assert np.array_equal(a[1], b[1])
+            np.testing.assert_allclose(a[0], b[0], rtol=1e-14)
+            np.testing.assert_allclose(a[2], b[2], rtol=1e-14)
+    np.testing.assert_allclose(I_hyd, I_fixed, rtol=1e-13)
+    # and the fixed-water scan is unchanged after a hydrated one
+    assert np.array_equal(scan.scan(T, q), I_fixed)
+
+
+def test_rigid_scan_hydrated_any_weights():
+    # more than 64 distinct weights (weight pass) with per-conformation waters
+    bodies, weights = make_bodies([301, 257], seed=91)
+    rng = np.random.default_rng(92)
+    weights = [rng.uniform(0.5, 12.0, len(w)).astype(np.float32) for w in weights]
+    w0 = water_shell(rng, np.array([13.0, 0, 0]), 80, 18, 25)
+    scan = A.RigidScan(bodies, weights, w0, rng.uniform(0.5, 1.5, 80).astype(np.float32))
+    T = random_transforms(rng, bodies, 3)
+    waters = []
+    for m in (77, 0, 130):
+        waters.append((water_shell(rng, np.array([13.0, 0, 0]), m, 17, 26), rng.uniform(0.5, 1.5, m).astype(np.float32)))
+    q = np.linspace(0.01, 0.5, 60)
+    I = scan.scan_hydrated(T, waters, q)
+    for c in range(len(T)):
+        ref = oracle_conf(bodies, weights, T[c], as_xyzw(*waters[c]))
+        h = scan.histogram(c)
+        assert h.max_bin == ref["max_bin"]
+        for which, key in ((A.AA, "aa"), (A.AW, "aw"), (A.WW, "ww"), (A.PTOT, "p_tot")):
+            v, cnt, _ = h.get(which)
+            assert np.array_equal(cnt, ref[key]["count"]), key
+            np.testing.assert_allclose(v, ref[key]["value"], rtol=1e-9, atol=1e-9 * np.max(np.abs(ref[key]["value"])),
+                                       err_msg=key)
+        assert axis_ok(h.d_axis(), O.weighted_axis(ref["p_tot"], 0.25), 0.25, ref["p_tot"]["count"])
+        err = relerr(I[c], O.debye_simple(ref, q, 0.25))
+        print(f"conformation {c}: I(q) rel err {err:.2e}")
+        assert err <= 1e-5
+
+
+def test_rigid_scan_hydrated_errors():
+    bodies, weights = make_bodies([120, 130], seed=95)
+    rng = np.random.default_rng(96)
+    scan = A.RigidScan(bodies, weights)
+    T = random_transforms(rng, bodies, 2)
+    w = water_shell(rng, np.array([13.0, 0, 0]), 10, 15, 20)
+    with pytest.raises(A.AusaxsError, match="water weight"):
+        scan.scan_hydrated(T, [(w, 6.0), (w, 2.5)], [0.1])  # 2.5 is no weight of the scan
+    with pytest.raises(A.AusaxsError, match="finite"):
+        scan.scan_hydrated(T, [(w, 6.0), (np.full((3, 3), np.nan), 6.0)], [0.1])
+    I = scan.scan_hydrated(T, [(w, 6.0), (w[:0], 6.0)], [0.1])
+    assert np.all(np.isfinite(I))
+
+
+def test_rigid_benchmark_bodies_hydrated():
+    # the benchmark's bodies (3k/5k/8k/12k atoms) with a new hydration shell of
+    # 3,000-4,000 waters in each of 64 conformations: pair-count identities on
+    # every conformation, the oracle's partial manager on two
+    import bench
+
+    cfg = bench.CONFIGS[4]
+    bodies, weights, T = bench.rigid_inputs(cfg, 4)
+    T = T[:64]
+    rng = np.random.default_rng(44)
+    centre = np.concatenate(bodies).mean(0)
+    w0 = water_shell(rng, centre, 3000, 40, 60)
+    scan = A.RigidScan(bodies, weights, w0, np.ones(3000, np.float32),
+                       settings=A.default_settings(bin_width=cfg["bw"]))
+    counts = rng.integers(3000, 4001, len(T))
+    waters = [(water_shell(rng, centre + rng.uniform(-5, 5, 3), int(m), 38, 62), 1.0) for m in counts]
+    q = A.default_q_axis()
+    I = scan.scan_hydrated(T, waters, q)
+    print("scan:", scan.info())
+    n = sum(len(b) for b in bodies)
+    self_terms = sum(int(np.sum(w.astype(np.float64) ** 2)) for w in weights)
+    for c in range(len(T)):
+        h = scan.histogram(c)
+        m = int(counts[c])
+        assert h.get(A.AA)[1].sum(dtype=np.int64) == n * (n - 1) + self_terms, c
+        assert h.get(A.AW)[1].sum(dtype=np.int64) == 2 * n * m, c
+        assert h.get(A.WW)[1].sum(dtype=np.int64) == m * (m - 1) + m, c
+    assert np.all(np.isfinite(I))
+    for c in (0, 63):
+        ref = oracle_conf(bodies, weights, T[c], as_xyzw(waters[c][0], 1.0), bw=cfg["bw"])
+        check_hist(scan.histogram(c), ref, cfg["bw"])
+        err = relerr(I[c], O.debye_simple(ref, q, cfg["bw"]))
+        print(f"conformation {c}: I(q) rel err {err:.2e}")
+        assert err <= 1e-5
