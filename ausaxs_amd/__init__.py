@@ -117,6 +117,10 @@ def _load():
         "ausaxs_server_stats": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong), P(C.c_longlong), P(C.c_int),
                                           P(C.c_int), P(C.c_double), P(C.c_longlong), P(C.c_longlong)]),
         "ausaxs_server_free": (C.c_int, [vp]),
+        "ausaxs_server_batch_stats": (C.c_int, [vp] + [P(C.c_longlong)] * 5 + [P(C.c_double)]),
+        "ausaxs_pool_stats": (C.c_int, [P(C.c_size_t), P(C.c_size_t), P(C.c_longlong), P(C.c_longlong),
+                                        P(C.c_longlong)]),
+        "ausaxs_pool_trim": (C.c_int, []),
         "ausaxs_lsq": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "ausaxs_cubic_spline": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
@@ -827,8 +831,26 @@ class Server:
         fl, pk, gb, pairs, kp = C.c_int(), C.c_int(), C.c_double(), C.c_longlong(), C.c_longlong()
         _check(lib.ausaxs_server_stats(self._s, *map(C.byref, v), C.byref(fl), C.byref(pk), C.byref(gb),
                                        C.byref(pairs), C.byref(kp)))
+        b = [C.c_longlong() for _ in range(5)]
+        cgb = C.c_double()
+        _check(lib.ausaxs_server_batch_stats(self._s, *map(C.byref, b), C.byref(cgb)))
         return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
-                    peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value, kernel_pairs=kp.value)
+                    peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value, kernel_pairs=kp.value,
+                    cache_hits=b[0].value, cache_misses=b[1].value, batches=b[2].value, batched=b[3].value,
+                    evictions=b[4].value, cached_gb=cgb.value)
+
+
+def pool_stats():
+    """The caching allocator's counters (device bytes held / in use, hipMalloc and hipHostMalloc calls, reuses)."""
+    held, used = C.c_size_t(), C.c_size_t()
+    dm, hm, ru = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    _check(lib.ausaxs_pool_stats(C.byref(held), C.byref(used), C.byref(dm), C.byref(hm), C.byref(ru)))
+    return dict(device_held=held.value, device_in_use=used.value, device_mallocs=dm.value, host_mallocs=hm.value,
+                reuses=ru.value)
+
+
+def pool_trim():
+    _check(lib.ausaxs_pool_trim())
 
 
 def cubic_spline(x, y, z):

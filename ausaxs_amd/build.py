@@ -33,7 +33,7 @@ COMMON = [
 ]
 
 SOURCES = ["pair_hist.hip", "lattice.hip", "debye.hip", "rigid.hip", "fit.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "serve.cpp", "symmetry.cpp", "fitter.cpp", "hydrate.cpp",
-           "synth.cpp", "capi.cpp"]
+           "synth.cpp", "pool.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> Path:

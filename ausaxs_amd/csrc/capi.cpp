@@ -865,6 +865,39 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
     });
 }
 
+int ausaxs_server_batch_stats(ausaxs_server* s, long long* cache_hits, long long* cache_misses, long long* batches,
+                              long long* batched, long long* evictions, double* cached_gb) {
+    return guarded([&] {
+        require(s && s->s, "bad argument");
+        const ServeStats st = s->s->stats();
+        if (cache_hits) *cache_hits = st.cache_hits;
+        if (cache_misses) *cache_misses = st.cache_misses;
+        if (batches) *batches = st.batches;
+        if (batched) *batched = st.batched;
+        if (evictions) *evictions = st.evictions;
+        if (cached_gb) *cached_gb = (double)st.cached_bytes / (1ull << 30);
+    });
+}
+
+int ausaxs_pool_stats(size_t* device_held, size_t* device_in_use, long long* device_mallocs,
+                      long long* host_mallocs, long long* reuses) {
+    return guarded([&] {
+        const PoolStats p = pool_stats();
+        if (device_held) *device_held = p.device_held;
+        if (device_in_use) *device_in_use = p.device_in_use;
+        if (device_mallocs) *device_mallocs = p.device_mallocs;
+        if (host_mallocs) *host_mallocs = p.host_mallocs;
+        if (reuses) *reuses = p.reuses;
+    });
+}
+
+int ausaxs_pool_trim(void) {
+    return guarded([&] {
+        Context::get();  // a device must be present
+        pool_trim();
+    });
+}
+
 int ausaxs_server_free(ausaxs_server* s) {
     return guarded([&] { delete s; });
 }

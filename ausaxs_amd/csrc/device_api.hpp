@@ -59,10 +59,22 @@ struct PairWork {
 constexpr int32_t kDiag = 1;    // same point set and overlapping ranges: count j > i only
 constexpr int32_t kOrient = 2;  // two output rows chosen by original-index order
 // two output rows chosen by the row point's position: rows before i_begin +
-// (flags >> kSplitShift) use row0, the rest row1 (a row tile spanning two
-// classes of a cross job)
+// split_rows(flags) use row0, the rest row1 (a row tile spanning two classes
+// of a cross job)
 constexpr int32_t kSplit = 4;
 constexpr int kSplitShift = 8;
+// Columns of a lattice set given as z-runs (PairLaunch::runs): [j_begin,
+// j_end) are run indices, and the item's column count is run_cols(flags).
+// A run is {x, y, z0, length as int bits}: the lattice points (x, y, z0 + t),
+// t < length, in the set's original order.
+constexpr int32_t kRuns = 8;
+constexpr int kRunColsShift = 18;
+__host__ __device__ inline int split_rows(int32_t flags) { return (flags >> kSplitShift) & 0x3FF; }
+__host__ __device__ inline int run_cols(int32_t flags) { return (flags >> kRunColsShift) & 0x1FFF; }
+// columns of an item
+__host__ __device__ inline int item_cols(const PairWork& w) {
+    return (w.flags & kRuns) ? run_cols(w.flags) : w.j_end - w.j_begin;
+}
 
 // Pairs involving the reference's scalar-tail columns (last <= 3 points of a
 // loop), evaluated with per-pair rule selection and added with global atomics.
@@ -77,6 +89,7 @@ struct StripWork {
 
 struct PairLaunch {
     const float4* pts;        // point buffer (x, y, z, tag bits)
+    const float4* runs;       // z-runs of lattice column sets (kRuns items; may be null otherwise)
     const PairWork* items;    // device
     int n_items;
     uint64_t* partial;        // [n_items][2][nbins+1]

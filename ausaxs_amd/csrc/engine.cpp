@@ -185,6 +185,37 @@ void add_items(std::vector<PairWork>& items, int i0, int i1, int j0, int j1, int
 }
 }  // namespace
 
+bool lattice_runs(const PointSet& ps, int mb, int max_cols, std::vector<float4>& runs) {
+    // integer lattice points only (the run kernel steps z by exactly one):
+    // |coordinates| below 2^22 that a cast to int and back leaves unchanged
+    const float lim = (float)(1 << 22);
+    auto lattice = [&](const float4& p) {
+        return std::fabs(p.x) < lim && std::fabs(p.y) < lim && std::fabs(p.z) < lim && p.x == (float)(int)p.x &&
+               p.y == (float)(int)p.y && p.z == (float)(int)p.z;
+    };
+    const size_t r0 = runs.size();
+    int i = 0;
+    while (i < mb) {
+        const float4& p = ps.orig[i];
+        if (!lattice(p)) {
+            runs.resize(r0);
+            return false;
+        }
+        int len = 1;
+        while (i + len < mb && len < max_cols && ps.orig[i + len].x == p.x && ps.orig[i + len].y == p.y &&
+               ps.orig[i + len].z == p.z + (float)len)
+            ++len;  // the points of the run are lattice points too (same x, y; z integer)
+        runs.push_back(make_float4(p.x, p.y, p.z, float_of(len)));
+        i += len;
+    }
+    // short runs would cost more per pair than plain columns
+    if (mb > 0 && (double)mb / (double)(runs.size() - r0) < 4.0) {
+        runs.resize(r0);
+        return false;
+    }
+    return true;
+}
+
 void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& sets,
                 const std::vector<SetLayout>& layouts, int n_rows, int tile_j, bool split_tiles, bool body_items) {
     const bool self = job.col_set < 0;
@@ -250,6 +281,41 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
         for (int b = 0; b < CC && body_items; ++b) {
             if (LC.seg_colbody[b] == 0) continue;
             const int t0 = LC.seg_start[b], t1 = t0 + LC.seg_colbody[b];
+            // column chunks of the items: point ranges of the buffer, or, for
+            // lattice columns, ranges of z-runs (kRuns) over the same points
+            struct Chunk {
+                int jb, je, flags;
+            };
+            std::vector<Chunk> chunks;
+            // the runs of a column set, built once per list
+            int run_lo = 0, run_hi = -1;
+            if (job.col_runs && CC == 1) {
+                auto it = std::find_if(wl.set_runs.begin(), wl.set_runs.end(),
+                                       [&](const std::array<int, 3>& e) { return e[0] == job.col_set; });
+                if (it == wl.set_runs.end()) {
+                    const int r0 = (int)wl.runs.size();
+                    const bool ok = lattice_runs(Cs, LC.seg_colbody[0], tile_j, wl.runs);
+                    wl.set_runs.push_back({job.col_set, r0, ok ? (int)wl.runs.size() : -1});
+                    it = wl.set_runs.end() - 1;
+                }
+                run_lo = (*it)[1];
+                run_hi = (*it)[2];
+            }
+            if (run_hi >= 0) {
+                int cols = 0, first = run_lo;
+                for (int q = run_lo; q < run_hi; ++q) {
+                    const int len = bits_of(wl.runs[q].w);
+                    if (cols + len > tile_j) {
+                        chunks.push_back({first, q, kRuns | (cols << kRunColsShift)});
+                        first = q;
+                        cols = 0;
+                    }
+                    cols += len;
+                }
+                if (cols) chunks.push_back({first, run_hi, kRuns | (cols << kRunColsShift)});
+            } else {
+                for (int jb = t0; jb < t1; jb += tile_j) chunks.push_back({jb, std::min(jb + tile_j, t1), 0});
+            }
             size_t k = 0;
             int i = seg_cls.empty() ? 0 : LR.seg_start[seg_cls[0]];
             while (k < seg_cls.size()) {
@@ -272,15 +338,15 @@ void append_job(WorkList& wl, const PairJob& job, const std::vector<PointSet>& s
                     }
                 }
                 const int r0 = row_of(a, b);
-                for (int jb = t0; jb < t1; jb += tile_j) {
+                for (const Chunk& ch : chunks) {
                     PairWork w{};
                     w.i_begin = i;
                     w.i_end = i1;
-                    w.j_begin = jb;
-                    w.j_end = std::min(jb + tile_j, t1);
+                    w.j_begin = ch.jb;
+                    w.j_end = ch.je;
                     w.row0 = r0;
                     w.row1 = split && r1 != r0 ? r1 : r0;
-                    w.flags = split && r1 != r0 ? (kSplit | (split << kSplitShift)) : 0;
+                    w.flags = ch.flags | (split && r1 != r0 ? (kSplit | (split << kSplitShift)) : 0);
                     wl.items.push_back(w);
                 }
                 i = i1;
@@ -351,14 +417,13 @@ bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mod
 
 void sort_items(std::vector<PairWork>& items) {
     std::stable_sort(items.begin(), items.end(), [](const PairWork& x, const PairWork& y) {
-        return (int64_t)(x.i_end - x.i_begin) * (x.j_end - x.j_begin) >
-               (int64_t)(y.i_end - y.i_begin) * (y.j_end - y.j_begin);
+        return (int64_t)(x.i_end - x.i_begin) * item_cols(x) > (int64_t)(y.i_end - y.i_begin) * item_cols(y);
     });
 }
 
 int64_t padded_work(const std::vector<PairWork>& items) {
     int64_t w = 0;
-    for (const PairWork& x : items) w += (int64_t)kTileI * (x.j_end - x.j_begin);
+    for (const PairWork& x : items) w += (int64_t)kTileI * item_cols(x);
     return w;
 }
 
@@ -391,8 +456,62 @@ MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
 
 PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
                        int tile_j, int lattice, std::vector<std::vector<float>> weights)
-    : sets_(std::move(sets)), jobs_(std::move(jobs)), n_rows_(n_rows), nbins_(nbins), inv_w_(inv_w),
-      lattice_mode_(lattice), weights_(std::move(weights)) {
+    : PairEngine(EngineSpec{std::move(sets), std::move(jobs), n_rows, inv_w, nbins, tile_j, lattice,
+                            std::move(weights), -1}) {}
+
+EngineSpec merge_specs(const std::vector<SpecView>& specs, std::vector<int>& row0) {
+    if (specs.empty()) throw std::runtime_error("ausaxs_hip: nothing to merge");
+    EngineSpec m;
+    m.inv_w = specs[0].inv_w;
+    m.tile_j = specs[0].tile_j;
+    m.lattice = specs[0].lattice;
+    m.nbins = 0;
+    m.clamp = 0;
+    row0.clear();
+    for (const SpecView& v : specs) {
+        const SpecView* sp = &v;
+        if (sp->inv_w != m.inv_w || sp->tile_j != m.tile_j || sp->lattice != m.lattice || sp->weighted)
+            throw std::runtime_error("ausaxs_hip: merged structures must share bin width, tile width and lattice "
+                                     "mode and carry no per-point weights");
+        const int set0 = (int)m.sets.size();
+        row0.push_back(m.n_rows);
+        for (const PointSet& ps : *sp->sets) m.sets.push_back(ps);
+        for (PairJob j : *sp->jobs) {
+            j.row_set += set0;
+            if (j.col_set >= 0) j.col_set += set0;
+            for (int32_t& r : j.rowmap)
+                if (r >= 0) r += m.n_rows;
+            m.jobs.push_back(std::move(j));
+        }
+        m.n_rows += sp->n_rows;
+        m.nbins = std::max(m.nbins, sp->nbins);
+        // a spec without a known clamp flag is taken as clamped
+        m.clamp = m.clamp || sp->clamp != 0;
+    }
+    return m;
+}
+
+PairEngine::~PairEngine() {
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+size_t PairEngine::device_bytes() const {
+    size_t b = d_pts_.bytes() + d_runs_.bytes() + d_pts_w_.bytes() + d_wcnt_.bytes() + d_wsum_.bytes() + d_wstrip_.bytes() +
+               d_items_.bytes() + d_row_list_.bytes() + d_chunk_ptr_.bytes() + d_row_chunk_.bytes() +
+               d_rowmaps_.bytes() + d_partial_.bytes() + d_cnt_.bytes() + d_tmp_cnt_.bytes() + d_dsum_.bytes() +
+               d_tmp_dsum_.bytes() + d_lat_a_.bytes() + d_lat_b_.bytes() + d_tw_.bytes() + d_scount_.bytes() +
+               d_lat_flag_.bytes();
+    for (const auto& o : d_orig_) b += o.bytes();
+    for (const auto& o : d_orig_w_) b += o.bytes();
+    return b;
+}
+
+PairEngine::PairEngine(EngineSpec spec)
+    : sets_(std::move(spec.sets)), jobs_(std::move(spec.jobs)), n_rows_(spec.n_rows), nbins_(spec.nbins),
+      inv_w_(spec.inv_w), lattice_mode_(spec.lattice), weights_(std::move(spec.weights)) {
+    const int lattice = spec.lattice, tile_j = spec.tile_j, nbins = spec.nbins;
+    tile_j_ = tile_j;
     if (lattice < 0 || lattice > 2) throw std::runtime_error("ausaxs_hip: lattice must be 0, 1 or 2");
     weighted_ = !weights_.empty();
     if (weighted_) {
@@ -409,38 +528,103 @@ PairEngine::PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, in
             }
         }
         for (const PairJob& j : jobs_)
-            if (j.lattice) throw std::runtime_error("ausaxs_hip: weighted point sets take no lattice jobs");
+            if (j.lattice || j.col_runs)
+                throw std::runtime_error("ausaxs_hip: weighted point sets take no lattice jobs");
         wscale_ = weight_scale(wmax);
     }
     check_tile_j(tile_j);
     check_pair_lds(nbins);
     // bins need no clamp in the pair kernel when the bounding-box diagonal
     // already keeps every pair below nbins
-    std::vector<const PointSet*> ps;
-    for (const auto& x : sets_) ps.push_back(&x);
-    clamp_ = nbins_ < bin_bound(ps, inv_w_);
+    if (spec.clamp < 0) {
+        std::vector<const PointSet*> ps;
+        for (const auto& x : sets_) ps.push_back(&x);
+        clamp_ = nbins_ < bin_bound(ps, inv_w_);
+    } else {
+        clamp_ = spec.clamp != 0;
+    }
     build(tile_j);
     for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
 }
 
+namespace {
+// The class segments' sizes of a set that is not placed in the point buffer
+// (base -1): its points are read only in original order (lattice jobs, tail
+// strips) or as z-runs.
+SetLayout layout_counts(const PointSet& ps) {
+    const int n = ps.size(), C = ps.n_classes;
+    const int t_self = n - std::min(3, n), t_col = n - (n % 4);
+    SetLayout L;
+    L.base = -1;
+    L.seg_start.assign(C, -1);
+    L.seg_len.assign(C, 0);
+    L.seg_body.assign(C, 0);
+    L.seg_colbody.assign(C, 0);
+    for (int i = 0; i < n; ++i) {
+        const int c = bits_of(ps.orig[i].w);
+        if (c < 0 || c >= C) throw std::runtime_error("ausaxs_hip: point class out of range");
+        ++L.seg_len[c];
+        if (i < t_self) ++L.seg_body[c];
+        if (i < t_col) ++L.seg_colbody[c];
+    }
+    return L;
+}
+}  // namespace
+
 void PairEngine::build(int tile_j) {
     Context& ctx = Context::get();
+    // lattice jobs first: their body pairs need no work items
+    lat_.clear();
+    std::vector<char> on_lattice(jobs_.size(), 0);
+    int64_t lattice_pairs = 0;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        LatticeJob lj;
+        if (plan_lattice(jobs_[k], sets_, lattice_mode_, lj)) {
+            on_lattice[k] = 1;
+            lattice_pairs += lj.pairs;
+            lat_.push_back(lj);
+        }
+    }
+    // z-runs of the lattice column sets, once for both work lists
+    std::vector<float4> runs;
+    std::vector<std::array<int, 3>> set_runs;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        const PairJob& job = jobs_[k];
+        if (on_lattice[k] || !job.col_runs || job.col_set < 0 || sets_[job.col_set].n_classes != 1) continue;
+        bool seen = false;
+        for (const auto& e : set_runs) seen = seen || e[0] == job.col_set;
+        if (seen) continue;
+        const PointSet& cs = sets_[job.col_set];
+        const int r0 = (int)runs.size();
+        const bool ok = lattice_runs(cs, cs.size() - cs.size() % 4, tile_j, runs);
+        set_runs.push_back({job.col_set, r0, ok ? (int)runs.size() : -1});
+    }
+    auto runs_of = [&](int set) {
+        for (const auto& e : set_runs)
+            if (e[0] == set) return e[2] >= 0;
+        return false;
+    };
+    // sets whose points some work item reads from the class-sorted buffer
+    std::vector<char> placed(sets_.size(), weighted_ ? 1 : 0);
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        const PairJob& job = jobs_[k];
+        if (on_lattice[k]) continue;
+        placed[job.row_set] = 1;
+        if (job.col_set >= 0 && !(job.col_runs && runs_of(job.col_set))) placed[job.col_set] = 1;
+    }
     std::vector<float4> buf;
     std::vector<int32_t> perm;
     layout_.clear();
-    for (const PointSet& ps : sets_) layout_.push_back(layout_set(ps, buf, weighted_ ? &perm : nullptr));
+    for (size_t k = 0; k < sets_.size(); ++k)
+        layout_.push_back(placed[k] ? layout_set(sets_[k], buf, weighted_ ? &perm : nullptr) : layout_counts(sets_[k]));
     WorkList plain, split;
-    lat_.clear();
-    for (const PairJob& job : jobs_) {
-        LatticeJob lj;
-        const bool lattice = plan_lattice(job, sets_, lattice_mode_, lj);
-        if (lattice) {
-            plain.lattice_pairs += lj.pairs;
-            split.lattice_pairs += lj.pairs;
-            lat_.push_back(lj);
-        }
-        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false, !lattice);
-        append_job(split, job, sets_, layout_, n_rows_, tile_j, true, !lattice);
+    plain.runs = split.runs = runs;
+    plain.set_runs = split.set_runs = set_runs;
+    plain.lattice_pairs = split.lattice_pairs = lattice_pairs;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+        const PairJob& job = jobs_[k];
+        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false, !on_lattice[k]);
+        append_job(split, job, sets_, layout_, n_rows_, tile_j, true, !on_lattice[k]);
     }
     const bool use_split = prefer_split(padded_work(plain.items), item_rows(plain.items), padded_work(split.items),
                                         item_rows(split.items));
@@ -459,6 +643,7 @@ void PairEngine::build(int tile_j) {
         d_orig_.back().upload(ps.orig.data(), ps.orig.size(), s);
     }
     d_items_.upload(wl_.items.data(), wl_.items.size(), s);
+    if (!wl_.runs.empty()) d_runs_.upload(wl_.runs.data(), wl_.runs.size(), s);
     d_row_list_.upload(mp_.row_list.data(), mp_.row_list.size(), s);
     d_chunk_ptr_.upload(mp_.chunk_ptr.data(), mp_.chunk_ptr.size(), s);
     d_row_chunk_.upload(mp_.row_chunk.data(), mp_.row_chunk.size(), s);
@@ -524,7 +709,7 @@ void PairEngine::enqueue(bool timed) {
     Context& ctx = Context::get();
     hipStream_t s = ctx.stream();
     if (timed) check(hipEventRecord(ev_[0], s), "hipEventRecord");
-    PairLaunch pl{d_pts_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_,
+    PairLaunch pl{d_pts_.p, d_runs_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0, grid_,
                   rows_, copies_};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     if (timed) check(hipEventRecord(ev_[1], s), "hipEventRecord");
@@ -565,7 +750,7 @@ void PairEngine::enqueue(bool timed) {
     }
     if (weighted_) {
         // the weight pass over the same work lists (wmode), its own outputs
-        PairLaunch wl{d_pts_w_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0,
+        PairLaunch wl{d_pts_w_.p, d_runs_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0,
                       grid_, rows_, copies_, 1, wscale_};
         check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
         MergeLaunch wm{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),

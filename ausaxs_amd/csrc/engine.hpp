@@ -3,7 +3,9 @@
 #pragma once
 
 #include "device_api.hpp"
+#include "pool.hpp"
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -44,6 +46,8 @@ struct Context {
     Context();
 };
 
+// Device buffer from the caching pool (pool.hpp): growing it or dropping it
+// hands the old block back to the pool, not to HIP.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -52,24 +56,23 @@ struct DevBuf {
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
+    ~DevBuf() { pool_free_device(p); }
     void ensure(size_t count) {
         if (count <= n && p) return;
-        if (p) check(hipFree(p), "hipFree");
+        pool_free_device(p);
         p = nullptr;
         n = 0;
-        check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        p = static_cast<T*>(pool_alloc_device(std::max<size_t>(count, 1) * sizeof(T)));
         n = std::max<size_t>(count, 1);
     }
+    size_t bytes() const { return p ? pool_size_class(n * sizeof(T)) : 0; }
     void upload(const T* src, size_t count, hipStream_t s) {
         ensure(count);
         if (count) check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
     }
 };
 
-// Pinned host buffer (hipHostMalloc), for copies captured in graphs.
+// Pinned host buffer (from the caching pool), for copies captured in graphs.
 template <class T>
 struct HostBuf {
     T* p = nullptr;
@@ -77,16 +80,13 @@ struct HostBuf {
     HostBuf() = default;
     HostBuf(const HostBuf&) = delete;
     HostBuf& operator=(const HostBuf&) = delete;
-    ~HostBuf() {
-        if (p) (void)hipHostFree(p);
-    }
+    ~HostBuf() { pool_free_host(p); }
     void ensure(size_t count) {
         if (count <= n && p) return;
-        if (p) check(hipHostFree(p), "hipHostFree");
+        pool_free_host(p);
         p = nullptr;
         n = 0;
-        check(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault),
-              "hipHostMalloc");
+        p = static_cast<T*>(pool_alloc_host(std::max<size_t>(count, 1) * sizeof(T)));
         n = std::max<size_t>(count, 1);
     }
 };
@@ -110,6 +110,10 @@ struct PairJob {
     // body pairs may be histogrammed by lattice correlation instead of the
     // pair kernel (the tail pairs keep their strips)
     bool lattice = false;
+    // a cross job whose one-class column set holds integer lattice points in
+    // z-runs (grid dummies in voxel order): its body columns go to the pair
+    // kernel as runs (kRuns items), when the points allow it
+    bool col_runs = false;
 };
 
 // Class-sorted layout of one point set inside a point buffer. Within each
@@ -132,8 +136,16 @@ struct StripSpec {
     int rowmap_off;  // into WorkList::rowmaps
 };
 
+// The z-runs of the first mb points of ps (original order; consecutive points
+// (x, y, z), (x, y, z + 1), ...), each at most max_cols long, appended to
+// runs as {x, y, z0, length bits}; false (nothing appended) unless every
+// point is an integer lattice point and the runs average at least 4 points.
+bool lattice_runs(const PointSet& ps, int mb, int max_cols, std::vector<float4>& runs);
+
 // Work items, strips and pair count of a list of jobs.
 struct WorkList {
+    std::vector<float4> runs;  // z-runs of kRuns items (lattice_runs)
+    std::vector<std::array<int, 3>> set_runs;  // {column set, first run, end run} (end -1: not runs)
     std::vector<PairWork> items;
     std::vector<StripSpec> strips;
     std::vector<int32_t> rowmaps;
@@ -214,6 +226,41 @@ struct WeightSums {
     std::vector<int64_t> items, strips;
 };
 
+// What a PairEngine is built from. Structures keep theirs, so that the work
+// of several small structures can run as one engine (one pair launch): their
+// sets side by side, jobs renumbered, rows offset (merge_specs).
+struct EngineSpec {
+    std::vector<PointSet> sets;
+    std::vector<PairJob> jobs;
+    int n_rows = 0;
+    float inv_w = 1;
+    int nbins = 10;
+    int tile_j = 8000;
+    int lattice = 1;
+    std::vector<std::vector<float>> weights;
+    int clamp = -1;  // -1: from the sets' bounding box; else whether bins must be clamped
+};
+// A read-only view of an engine's inputs (an EngineSpec, or the sets and jobs
+// a PairEngine already holds).
+struct SpecView {
+    const std::vector<PointSet>* sets = nullptr;
+    const std::vector<PairJob>* jobs = nullptr;
+    int n_rows = 0;
+    float inv_w = 1;
+    int nbins = 10;
+    int tile_j = 8000;
+    int lattice = 1;
+    bool weighted = false;
+    int clamp = -1;
+};
+inline SpecView view_of(const EngineSpec& s) {
+    return SpecView{&s.sets, &s.jobs, s.n_rows, s.inv_w, s.nbins, s.tile_j, s.lattice, !s.weights.empty(), s.clamp};
+}
+// The specs side by side: spec k's sets, jobs and rows follow those of
+// specs 0..k-1 (row0[k] = its first row); nbins is the largest, clamp the
+// OR. All must share inv_w, tile_j and lattice and have no weights.
+EngineSpec merge_specs(const std::vector<SpecView>& specs, std::vector<int>& row0);
+
 // Coordinates resident on the device with their work lists built once; run()
 // recomputes all jobs into [n_rows][nbins+1] (count, fixed-point delta sum).
 // With per-point weights (one vector per set, all sets single-class) a second
@@ -224,6 +271,8 @@ class PairEngine {
     // lattice: lattice-path mode for PairJob::lattice jobs (see plan_lattice)
     PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
                int tile_j = 8000, int lattice = 1, std::vector<std::vector<float>> weights = {});
+    explicit PairEngine(EngineSpec spec);
+    ~PairEngine();
     PairEngine(const PairEngine&) = delete;
     PairEngine& operator=(const PairEngine&) = delete;
 
@@ -250,6 +299,12 @@ class PairEngine {
     // workgroups of the pair kernel: 0 = one per item, else a persistent grid
     void set_grid(int g) { grid_ = g; }
     bool clamped() const { return clamp_; }
+    // device bytes the engine holds (pool size classes)
+    size_t device_bytes() const;
+    // the engine's inputs
+    SpecView view() const {
+        return SpecView{&sets_, &jobs_, n_rows_, inv_w_, nbins_, tile_j_, lattice_mode_, weighted_, clamp_ ? 1 : 0};
+    }
 
   private:
     std::vector<PointSet> sets_;
@@ -259,6 +314,7 @@ class PairEngine {
     MergePlan mp_;
     int n_rows_, nbins_;
     float inv_w_;
+    int tile_j_ = 8000;
     bool clamp_ = true;
     int grid_ = 0;
     int rows_ = 2, copies_ = 1;
@@ -269,6 +325,7 @@ class PairEngine {
     DevBuf<int> d_lat_flag_;
 
     DevBuf<float4> d_pts_;
+    DevBuf<float4> d_runs_;  // z-runs of the lattice column sets
     std::vector<DevBuf<float4>> d_orig_;
     // weight pass: the same buffers with weights in the fourth slot, and its sums
     std::vector<std::vector<float>> weights_;

@@ -314,6 +314,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
                 j.row_set = row_set;
                 j.col_set = col_set;
                 j.lattice = row_set >= 2 && col_set != 1;  // dummies only
+                j.col_runs = row_set < 2 && col_set >= 2;   // waters against dummies
                 j.rowmap = {n_rows++};
                 rows_.push_back({kind, 0, 0});
                 jobs.push_back(std::move(j));
@@ -325,6 +326,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
                 PairJob ax;
                 ax.row_set = 0;
                 ax.col_set = set;
+                ax.col_runs = true;
                 ax.rowmap.assign(kMaxFF, -1);
                 for (int f : present) {
                     ax.rowmap[f] = n_rows++;
@@ -356,6 +358,7 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
             }
             wx.row_set = 1;
             wx.col_set = 2;
+            ax.col_runs = wx.col_runs = true;  // dummies in voxel order: z-runs
             wx.rowmap = {n_rows++};
             rows_.push_back({5, 0, 0});
             jobs.push_back(xx);
@@ -438,19 +441,76 @@ ResidentStructure::ResidentStructure(Mode mode, const float* atoms, int n_atoms,
     sets.push_back(std::move(W));
     if (grid) sets.push_back(std::move(X));
     if (mode == Mode::FFGridSurface) sets.push_back(std::move(XS));
-    engine_ = std::make_unique<PairEngine>(std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j,
-                                           s.lattice, std::move(weights));
+    for (const PairJob& j : jobs) {
+        const int64_t a = sets[j.row_set].size();
+        pairs_ += j.col_set < 0 ? a * (a - 1) / 2 : a * sets[j.col_set].size();
+    }
+    spec_ = EngineSpec{std::move(sets), std::move(jobs), n_rows, inv_w, nbins, s.tile_j, s.lattice,
+                       std::move(weights), nbins < bound ? 1 : 0};
+    nbins_ = nbins;
+    n_rows_ = n_rows;
+    inv_w_ = inv_w;
+    if (weighted_) engine();  // the weight pass checks its weights at construction
+}
+
+PairEngine& ResidentStructure::engine() {
+    // the engine takes the spec's sets and jobs (spec_view reads them there)
+    if (!engine_) engine_ = std::make_unique<PairEngine>(std::move(spec_));
+    return *engine_;
 }
 
 std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool timed) {
-    engine_->run(cnt_, dsum_, timed, weighted_ ? &wsum_ : nullptr);
-    const int nbins = engine_->nbins();
+    engine().run(cnt_, dsum_, timed, weighted_ ? &wsum_ : nullptr);
+    return finalize();
+}
+
+std::vector<std::unique_ptr<CompositeDistanceHistogramHIP>> ResidentStructure::calculate_merged(
+    const std::vector<ResidentStructure*>& members, int64_t* pairs, int64_t* kernel_pairs) {
+    std::vector<SpecView> specs;
+    for (ResidentStructure* m : members) {
+        if (m->weighted_) throw std::runtime_error("ausaxs_hip: structures with per-point weights run alone");
+        specs.push_back(m->spec_view());
+    }
+    std::vector<int> row0;
+    PairEngine eng(merge_specs(specs, row0));
+    std::vector<uint64_t> cnt;
+    std::vector<int64_t> dsum;
+    eng.run(cnt, dsum);
+    if (pairs) *pairs = eng.pairs();
+    if (kernel_pairs) *kernel_pairs = eng.kernel_pairs();
+    // each member's rows at its own width; pairs at or beyond it go to its
+    // overflow slot (nbins), as its own engine would put them
+    const size_t ms = (size_t)eng.nbins() + 1;
+    std::vector<std::unique_ptr<CompositeDistanceHistogramHIP>> out;
+    for (size_t k = 0; k < members.size(); ++k) {
+        ResidentStructure& m = *members[k];
+        const int nb = m.nbins_;
+        const size_t st = (size_t)nb + 1;
+        m.cnt_.assign((size_t)m.n_rows_ * st, 0);
+        m.dsum_.assign(m.cnt_.size(), 0);
+        for (int r = 0; r < m.n_rows_; ++r) {
+            const size_t src = (size_t)(row0[k] + r) * ms, dst = (size_t)r * st;
+            for (int b = 0; b < nb; ++b) {
+                m.cnt_[dst + b] = cnt[src + b];
+                m.dsum_[dst + b] = dsum[src + b];
+            }
+            for (size_t b = nb; b < ms; ++b) m.cnt_[dst + nb] += cnt[src + b];
+        }
+        out.push_back(m.finalize());
+    }
+    return out;
+}
+
+std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::finalize() {
+    const int nbins = nbins_;
     const int stride = nbins + 1;
-    if (engine_->overflowed(cnt_))
-        throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
-                                 std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
-                                 " A); increase bin_count or bin_width");
+    for (int r = 0; r < n_rows_; ++r)
+        if (cnt_[(size_t)r * stride + nbins])
+            throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width (" +
+                                     std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
+                                     " A); increase bin_count or bin_width");
     const size_t nb = nbins;
+    const float inv_w = inv_w_;
 
     // entry of one global row and bin, times the reference's template factor
     auto entry = [&](int row, int b, int factor) {
@@ -459,7 +519,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::calculate(bool
         WeightedEntry e;
         e.value = (double)factor * (double)c;
         e.count = (uint32_t)(factor * c);
-        e.bin_center = factor * distance_sum(c, b, s, engine_->inv_w());
+        e.bin_center = factor * distance_sum(c, b, s, inv_w);
         return e;
     };
 

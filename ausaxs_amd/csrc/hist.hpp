@@ -146,10 +146,23 @@ class ResidentStructure {
                       const HistSettings& s, double z_exv, const double* exv_xyz = nullptr, int n_exv = 0,
                       double grid_volume = 0, const double* surf_xyz = nullptr, int n_surf = 0);
     std::unique_ptr<CompositeDistanceHistogramHIP> calculate(bool timed = false);
-    PairEngine& engine() { return *engine_; }
+    // the structure's own engine, built (and its points uploaded) on first use
+    PairEngine& engine();
+    bool has_engine() const { return engine_ != nullptr; }
+    // the engine's inputs (before or after the engine was built)
+    SpecView spec_view() const { return engine_ ? engine_->view() : view_of(spec_); }
     Mode mode() const { return mode_; }
     void set_grid_exv_volume(double v) { grid_exv_volume_ = v; }
     double z_exv() const { return z_exv_; }
+    bool weighted() const { return weighted_; }
+    // pairs of all jobs (the histograms' pair count)
+    int64_t pairs() const { return pairs_; }
+    // Several structures' pair work in one engine (one pair launch, one
+    // merge): each result equals its calculate() alone (integer histograms).
+    // The structures must share bin width, tile width and lattice mode and
+    // have no per-point weights. *pairs / *kernel_pairs: the merged engine's.
+    static std::vector<std::unique_ptr<CompositeDistanceHistogramHIP>> calculate_merged(
+        const std::vector<ResidentStructure*>& members, int64_t* pairs = nullptr, int64_t* kernel_pairs = nullptr);
 
   private:
     Mode mode_;
@@ -161,11 +174,17 @@ class ResidentStructure {
     double self_w2_atoms_ = 0, self_w2_waters_ = 0;  // Simple mode: sum of w*w (float products)
     std::vector<float> atom_class_w_, water_class_w_;
     std::vector<std::array<int, 3>> rows_;        // global row -> (kind, a, b)
+    EngineSpec spec_;                             // what the engine is built from (moved into it)
+    int nbins_ = 10, n_rows_ = 0;
+    float inv_w_ = 1;
+    int64_t pairs_ = 0;
     std::unique_ptr<PairEngine> engine_;
     std::vector<uint64_t> cnt_;
     std::vector<int64_t> dsum_;
     bool weighted_ = false;       // Simple mode with more than kMaxWeightClasses distinct weights
     WeightSums wsum_;             // weight-pass sums (weighted_)
+    // the composite histogram from cnt_ / dsum_ ([rows][nbins_ + 1])
+    std::unique_ptr<CompositeDistanceHistogramHIP> finalize();
 };
 
 // HistogramManagerMTFFGrid structure from FP64 atoms (n x 3, form-factor

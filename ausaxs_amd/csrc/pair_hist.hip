@@ -107,23 +107,23 @@ struct Rows {
     uint32_t base[kRowsPerLane];  // LDS byte address of the row's cell 0, minus 8 * kMagicBits (mod 2^32)
 };
 
-// All kRowsPerLane pairs of one column point: bin, tie fallback, LDS add.
-// Oriented items hold the column's original index in pj.w as a float (the
+// Bins and adds the kRowsPerLane pairs of one column whose squared distances
+// (scaled) are d2: fast path, tie fallback, LDS add.
+// Oriented items hold the column's original index in tag_j as a float (the
 // staging converts it) and add into row 1 when the row point's index is the
 // larger: sat(tag_i - tag_j) is 1 exactly then (distinct integers differ by
 // at least 1), and t + sat * off1 stays an exact integer-valued float, so the
 // row choice costs two VALU operations per pair and no compare or select.
 // The weight pass (WEIGHT) bins the same way and adds the pair's weight
-// product (weight_code) instead of its count cell.
-template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT, bool WEIGHT>
-__device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, const Consts& cs,
-                                            unsigned ovf, int off1, double wscale) {
-    float t[kRowsPerLane], dl[kRowsPerLane], d2[kRowsPerLane], s[kRowsPerLane];
+// product (weight_code of w_i * tag_j) instead of its count cell.
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool WEIGHT>
+__device__ __forceinline__ void add_pairs(const Rows& r, const float (&d2)[kRowsPerLane], float tag_j, int j,
+                                          const Consts& cs, unsigned ovf, int off1, double wscale) {
+    float t[kRowsPerLane], dl[kRowsPerLane], s[kRowsPerLane];
     bool near[kRowsPerLane];
     bool tie = false;
 #pragma unroll
     for (int k = 0; k < kRowsPerLane; ++k) {
-        d2[k] = simd_d2(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z);
         near[k] = bin_fast(d2[k], cs, s[k], t[k], dl[k]);
         tie |= near[k];
     }
@@ -139,7 +139,7 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
         if (ORIENT) {
             float tt = t[k];
             if (CLAMP) tt = fminf(tt, kMagic + (float)ovf);
-            const float sel = __builtin_amdgcn_fmed3f(r.tagf[k] - pj.w, 0.f, 1.f);
+            const float sel = __builtin_amdgcn_fmed3f(r.tagf[k] - tag_j, 0.f, 1.f);
             u = __float_as_uint(__builtin_fmaf(sel, (float)off1, tt));
         } else {
             u = __float_as_uint(t[k]);
@@ -150,8 +150,94 @@ __device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int 
             if (DIAG) keep = keep && (j > r.idx[k]);
             u = keep ? u : kMagicBits + ovf + 1u;
         }
-        if (WEIGHT) lds_add_at(r.base[k] + (u << 3), weight_code(r.tagf[k] * pj.w, wscale));
+        if (WEIGHT) lds_add_at(r.base[k] + (u << 3), weight_code(r.tagf[k] * tag_j, wscale));
         else lds_add_at(r.base[k] + (u << 3), cell_add(offset_code(d2[k], s[k], dl[k], cs)));
+    }
+}
+
+// All kRowsPerLane pairs of one column point.
+template <bool FULL, bool DIAG, bool ORIENT, bool CLAMP, bool SPLIT, bool WEIGHT>
+__device__ __forceinline__ void pair_column(const Rows& r, const float4 pj, int j, uint64_t* hist, const Consts& cs,
+                                            unsigned ovf, int off1, double wscale) {
+    float d2[kRowsPerLane];
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) d2[k] = simd_d2(r.x[k] - pj.x, r.y[k] - pj.y, r.z[k] - pj.z);
+    add_pairs<FULL, DIAG, ORIENT, CLAMP, WEIGHT>(r, d2, pj.w, j, cs, ovf, off1, wscale);
+}
+
+// The row points of this lane for item w (scaled coordinates, tags, LDS row
+// bases). Lanes of one wave take points 8 apart: spatially spread rows give
+// spread distances to a common column point, so fewer lanes collide on one
+// LDS cell.
+template <bool FULL, bool SPLIT, bool WEIGHT>
+__device__ __forceinline__ void load_rows(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
+                                          const Consts& cs, int nbins, Rows& r) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) {
+        const int li = k * kRowLanes + (t & (kRowLanes - 1));
+        const int i = w.i_begin + ((li & 63) * 8 + (li >> 6));
+        r.valid[k] = FULL || i < w.i_end;
+        const float4 p = r.valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        r.x[k] = p.x * cs.scale;
+        r.y[k] = p.y * cs.scale;
+        r.z[k] = p.z * cs.scale;
+        r.tagf[k] = WEIGHT ? p.w : (float)__float_as_int(p.w);
+        r.idx[k] = i;
+        // kSplit: rows from the second class add into the second LDS row
+        const int roff = SPLIT && i >= w.i_begin + split_rows(w.flags) ? nbins + 2 : 0;
+        r.base[k] = lds_byte_addr(hist + roff) - kMagicBits * 8u;
+    }
+}
+
+// One work item whose columns are z-runs of lattice points (kRuns). For each
+// run the lanes form dx^2 + dy^2 once per row; along the run only dz changes,
+// so a pair costs dz, dz^2 and one add before the binning (the reference's
+// ((dx dx + dy dy) + dz dz), separately rounded, as for any other pair). The
+// run's z advances by the scaled lattice step, exactly. Run descriptors pass
+// through LDS 512 at a time; the parts take alternate runs.
+template <bool FULL, bool CLAMP, bool SPLIT>
+__device__ __forceinline__ void run_item_runs(const float4* __restrict__ pts, const float4* __restrict__ runs,
+                                              const PairWork& w, uint64_t* hist, float4* stage, const Consts& cs,
+                                              int nbins) {
+    const int t = threadIdx.x;
+    const int part = t / kRowLanes;
+    Rows r;
+    load_rows<FULL, SPLIT, false>(pts, w, hist, cs, nbins, r);
+    const unsigned ovf = (unsigned)nbins;
+    for (int rb = w.j_begin; rb < w.j_end; rb += kColTile) {
+        const int n = min(kColTile, w.j_end - rb);
+        __syncthreads();  // previous descriptors consumed
+        if (t < n) {
+            const float4 d = runs[rb + t];
+            stage[t] = make_float4(d.x * cs.scale, d.y * cs.scale, d.z * cs.scale, d.w);
+        }
+        __syncthreads();
+        for (int u = part; u < n; u += kParts) {
+            // the run is the same for every lane of the wave: scalar registers
+            const float4 dv = stage[u];
+            const float dxr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dv.x)));
+            const float dyr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dv.y)));
+            const int len = __builtin_amdgcn_readfirstlane(__float_as_int(dv.w));
+            float sxy[kRowsPerLane];
+#pragma unroll
+            for (int k = 0; k < kRowsPerLane; ++k) {
+                const float dx = r.x[k] - dxr, dy = r.y[k] - dyr;
+                sxy[k] = dx * dx + dy * dy;
+            }
+            // (read into a scalar register first: the loop then waits for no LDS load)
+            float z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dv.z)));
+            for (int c = 0; c < len; ++c) {
+                float d2[kRowsPerLane];
+#pragma unroll
+                for (int k = 0; k < kRowsPerLane; ++k) {
+                    const float dz = r.z[k] - z;
+                    d2[k] = sxy[k] + dz * dz;
+                }
+                add_pairs<FULL, false, false, CLAMP, false>(r, d2, 0.f, 0, cs, ovf, 0, 1.0);
+                z += cs.scale;
+            }
+        }
     }
 }
 
@@ -167,24 +253,7 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     const int t = threadIdx.x;
     const int part = t / kRowLanes;  // every part holds the same row points
     Rows r;
-#pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k) {
-        // lanes of one wave take points 8 apart: spatially spread rows give
-        // spread distances to a common column point, so fewer lanes collide
-        // on one LDS cell.
-        const int li = k * kRowLanes + (t & (kRowLanes - 1));
-        const int i = w.i_begin + ((li & 63) * 8 + (li >> 6));
-        r.valid[k] = FULL || i < w.i_end;
-        const float4 p = r.valid[k] ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        r.x[k] = p.x * cs.scale;
-        r.y[k] = p.y * cs.scale;
-        r.z[k] = p.z * cs.scale;
-        r.tagf[k] = WEIGHT ? p.w : (float)__float_as_int(p.w);
-        r.idx[k] = i;
-        // kSplit: rows from the second class add into the second LDS row
-        const int roff = SPLIT && i >= w.i_begin + (w.flags >> kSplitShift) ? nbins + 2 : 0;
-        r.base[k] = lds_byte_addr(hist + roff) - kMagicBits * 8u;
-    }
+    load_rows<FULL, SPLIT, WEIGHT>(pts, w, hist, cs, nbins, r);
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
 
@@ -237,10 +306,21 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
 }
 
 template <bool CLAMP>
-__device__ __forceinline__ void run_any(const float4* __restrict__ pts, const PairWork& w, uint64_t* hist,
-                                        float4* cols, const Consts& cs, int nbins) {
+__device__ __forceinline__ void run_any(const float4* __restrict__ pts, const float4* __restrict__ runs,
+                                        const PairWork& w, uint64_t* hist, float4* cols, const Consts& cs,
+                                        int nbins) {
     const bool orient = (w.flags & kOrient) != 0;
-    if (w.flags & kDiag) {
+    if (w.flags & kRuns) {
+        // a cross job against lattice columns (never oriented)
+        const bool full = w.i_end - w.i_begin >= kTileI;
+        if (w.flags & kSplit) {
+            if (full) run_item_runs<true, CLAMP, true>(pts, runs, w, hist, cols, cs, nbins);
+            else run_item_runs<false, CLAMP, true>(pts, runs, w, hist, cols, cs, nbins);
+        } else {
+            if (full) run_item_runs<true, CLAMP, false>(pts, runs, w, hist, cols, cs, nbins);
+            else run_item_runs<false, CLAMP, false>(pts, runs, w, hist, cols, cs, nbins);
+        }
+    } else if (w.flags & kDiag) {
         if (orient) run_item<false, true, true, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
         else run_item<false, true, false, CLAMP>(pts, w, hist, cols, cs, nbins, 1.0);
     } else if (w.flags & kSplit) {
@@ -280,6 +360,7 @@ __device__ __forceinline__ void run_weighted(const float4* __restrict__ pts, con
 // scratch in every workgroup, ~200 MB of scratch writes per launch on the
 // 250k-atom workload, at the same pair rate.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
+                                                             const float4* __restrict__ runs,
                                                              const PairWork* __restrict__ items, int n_items,
                                                              uint64_t* __restrict__ partial, Consts cs,
                                                              int nbins, int clamp, int rows, int copies,
@@ -303,9 +384,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
             if (clamp) run_weighted<true>(pts, w, mine, cols, cs, nbins, wscale);
             else run_weighted<false>(pts, w, mine, cols, cs, nbins, wscale);
         } else if (clamp) {
-            run_any<true>(pts, w, mine, cols, cs, nbins);
+            run_any<true>(pts, runs, w, mine, cols, cs, nbins);
         } else {
-            run_any<false>(pts, w, mine, cols, cs, nbins);
+            run_any<false>(pts, runs, w, mine, cols, cs, nbins);
         }
         __syncthreads();
         // partial slot layout: [2][nbins+1] (the masked-lane slot is dropped)
@@ -561,7 +642,7 @@ hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
         return hipErrorInvalidValue;
     const size_t lds = pair_lds_bytes(p.nbins, p.rows, p.copies) - kStaticLds;
     const int grid = p.grid > 0 ? std::min(p.grid, p.n_items) : p.n_items;
-    hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.items, p.n_items, p.partial,
+    hipLaunchKernelGGL(pair_hist_kernel, dim3(grid), dim3(kThreads), lds, s, p.pts, p.runs, p.items, p.n_items, p.partial,
                        make_consts(p.inv_w), p.nbins, p.clamp, p.rows, p.copies, p.wmode, p.wscale);
     return hipGetLastError();
 }

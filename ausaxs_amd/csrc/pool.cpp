@@ -1,0 +1,193 @@
+// Caching allocator (see pool.hpp).
+#include "pool.hpp"
+
+#include "engine.hpp"
+
+#include <algorithm>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace ausaxs::gpu {
+
+size_t pool_size_class(size_t bytes) {
+    size_t b = std::max<size_t>(bytes, 256);
+    size_t p = 1;
+    while (p < b) p <<= 1;  // smallest power of two >= b
+    if (p <= (size_t(1) << 20)) return p;
+    const size_t step = p / 8;  // b lies in (p/2, p]: quarter steps of p/2
+    return (b + step - 1) / step * step;
+}
+
+namespace {
+
+struct Block {
+    void* p = nullptr;
+    hipEvent_t ev = nullptr;  // recorded at release; null: nothing to wait for
+};
+
+struct Arena {
+    bool host = false;
+    std::unordered_map<void*, size_t> live;              // block -> size class
+    std::unordered_map<size_t, std::vector<Block>> free;  // size class -> free blocks
+    std::vector<void*> parked;                            // released during graph capture
+    size_t held = 0, in_use = 0;
+    int64_t mallocs = 0;
+};
+
+class Pool {
+  public:
+    static Pool& get() {
+        // never destroyed: blocks and events stay valid until the process ends
+        static Pool* pool = new Pool;
+        return *pool;
+    }
+
+    void* alloc(Arena& a, size_t bytes) {
+        const size_t cls = pool_size_class(bytes);
+        Block blk;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            auto it = a.free.find(cls);
+            if (it != a.free.end() && !it->second.empty()) {
+                blk = it->second.back();
+                it->second.pop_back();
+                a.live[blk.p] = cls;
+                a.in_use += cls;
+                ++reuses_;
+            }
+        }
+        if (blk.p) {
+            if (blk.ev) {
+                if (a.host) check(hipEventSynchronize(blk.ev), "hipEventSynchronize");
+                else check(hipStreamWaitEvent(Context::get().stream(), blk.ev, 0), "hipStreamWaitEvent");
+                give_event(blk.ev);
+            }
+            return blk.p;
+        }
+        void* p = raw_alloc(a, cls);
+        if (!p) {
+            trim();
+            p = raw_alloc(a, cls);
+        }
+        if (!p)
+            throw std::runtime_error(std::string("ausaxs_hip: out of ") + (a.host ? "pinned host" : "device") +
+                                     " memory for " + std::to_string(cls) + " bytes");
+        std::lock_guard<std::mutex> lk(m_);
+        a.live[p] = cls;
+        a.held += cls;
+        a.in_use += cls;
+        ++a.mallocs;
+        return p;
+    }
+
+    void release(Arena& a, void* p) {
+        if (!p) return;
+        hipStream_t s = Context::get().stream();
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        const bool capturing = hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone;
+        hipEvent_t ev = nullptr;
+        if (!capturing) {
+            ev = take_event();
+            if (hipEventRecord(ev, s) != hipSuccess) {
+                give_event(ev);
+                ev = nullptr;
+                (void)hipStreamSynchronize(s);
+            }
+        }
+        std::lock_guard<std::mutex> lk(m_);
+        auto it = a.live.find(p);
+        if (it == a.live.end()) return;  // not a pool block
+        const size_t cls = it->second;
+        a.live.erase(it);
+        a.in_use -= cls;
+        if (capturing) {
+            // an event record would join the captured graph: keep the block
+            // out of circulation until the next trim
+            a.parked.push_back(p);
+            return;
+        }
+        a.free[cls].push_back(Block{p, ev});
+    }
+
+    void trim() {
+        // every free block's last work must be done before HIP takes it back
+        (void)hipDeviceSynchronize();
+        std::lock_guard<std::mutex> lk(m_);
+        for (Arena* a : {&dev_, &host_}) {
+            for (auto& [cls, blocks] : a->free) {
+                for (Block& b : blocks) {
+                    if (a->host) (void)hipHostFree(b.p);
+                    else (void)hipFree(b.p);
+                    a->held -= cls;
+                    if (b.ev) events_.push_back(b.ev);
+                }
+                blocks.clear();
+            }
+            for (void* p : a->parked) {
+                if (a->host) (void)hipHostFree(p);
+                else (void)hipFree(p);
+            }
+            a->parked.clear();
+        }
+    }
+
+    PoolStats stats() {
+        std::lock_guard<std::mutex> lk(m_);
+        PoolStats s;
+        s.device_held = dev_.held;
+        s.device_in_use = dev_.in_use;
+        s.host_held = host_.held;
+        s.host_in_use = host_.in_use;
+        s.device_mallocs = dev_.mallocs;
+        s.host_mallocs = host_.mallocs;
+        s.reuses = reuses_;
+        return s;
+    }
+
+    Arena dev_, host_;
+
+  private:
+    Pool() { host_.host = true; }
+    std::mutex m_;
+    std::vector<hipEvent_t> events_;
+    int64_t reuses_ = 0;
+
+    void* raw_alloc(Arena& a, size_t cls) {
+        void* p = nullptr;
+        const hipError_t e = a.host ? hipHostMalloc(&p, cls, hipHostMallocDefault) : hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // clear the sticky error of the failed allocation
+            return nullptr;
+        }
+        return p;
+    }
+    hipEvent_t take_event() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (!events_.empty()) {
+                hipEvent_t e = events_.back();
+                events_.pop_back();
+                return e;
+            }
+        }
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        return e;
+    }
+    void give_event(hipEvent_t e) {
+        std::lock_guard<std::mutex> lk(m_);
+        events_.push_back(e);
+    }
+};
+
+}  // namespace
+
+void* pool_alloc_device(size_t bytes) { return Pool::get().alloc(Pool::get().dev_, bytes); }
+void pool_free_device(void* p) { Pool::get().release(Pool::get().dev_, p); }
+void* pool_alloc_host(size_t bytes) { return Pool::get().alloc(Pool::get().host_, bytes); }
+void pool_free_host(void* p) { Pool::get().release(Pool::get().host_, p); }
+void pool_trim() { Pool::get().trim(); }
+PoolStats pool_stats() { return Pool::get().stats(); }
+
+}  // namespace ausaxs::gpu

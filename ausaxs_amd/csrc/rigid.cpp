@@ -733,7 +733,7 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
                        p.slot_cls.p, p.body_moving.p, p.n_slots, B, p.n_conf, p.M, p.pts.p, p.orig.p};
     if (weighted_) tl.slot_w = p.slot_w.p, tl.pts_w = p.pts_w.p, tl.orig_w = p.orig_w.p;
     check(launch_transform(tl, s), "transform_kernel");
-    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
+    PairLaunch pl{p.pts.p, nullptr, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
                   p.copies};
     check(launch_pair_hist(pl, s), "pair_hist_kernel");
     MergeLaunch ml{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * p.R,
@@ -742,7 +742,7 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
     for (const StripLaunch& sl : p.strips) check(launch_strip(sl, s), "strip_kernel");
     check(launch_strip_table(p.d_table.p, p.d_first.p, p.n_table, p.table_blocks, s), "strip_table_kernel");
     if (weighted_) {
-        PairLaunch wl{p.pts_w.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
+        PairLaunch wl{p.pts_w.p, nullptr, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, p.grid, p.rows,
                       p.copies, 1, wscale_};
         check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
         MergeLaunch wm{p.partial.p, p.row_list.p, p.chunk_ptr.p, p.row_chunk.p, p.mp.n_chunks(), (p.n_conf + 1) * p.R,
@@ -775,7 +775,7 @@ void RigidScan::enqueue(Plan& p, int nq, double cw, bool copies) {
 
 float RigidScan::time_pair(Plan& p, int grid, int copies) {
     Context& ctx = Context::get();
-    PairLaunch pl{p.pts.p, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid, p.rows, copies};
+    PairLaunch pl{p.pts.p, nullptr, p.d_items.p, (int)p.items.size(), p.partial.p, inv_w_, p.nbins, 1, grid, p.rows, copies};
     check(launch_pair_hist(pl, ctx.stream()), "pair_hist_kernel");  // warm-up
     check(hipEventRecord(p.ev[0], ctx.stream()), "hipEventRecord");
     check(launch_pair_hist(pl, ctx.stream()), "pair_hist_kernel");

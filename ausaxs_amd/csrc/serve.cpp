@@ -3,9 +3,50 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 
 namespace ausaxs::gpu {
+
+struct Server::Entry {
+    uint64_t key = 0;
+    Mode mode = Mode::FFExplicit;
+    HistSettings s;
+    double z_exv = 1;
+    std::vector<double> atoms, waters;  // the exact content, compared on every hit
+    std::vector<int32_t> ff;
+    std::unique_ptr<ResidentStructure> st;
+    bool busy = false;
+    bool in_cache = false;
+    size_t bytes = 0;  // device bytes of its own engine (idle entries count them in cached_bytes)
+    std::list<std::shared_ptr<Entry>>::iterator lru;
+};
+
+namespace {
+
+// FNV-1a over the bytes of a value sequence
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ull;
+    return h;
+}
+
+// estimated grid dummies of n atoms in a box (1 A^3 per lattice point)
+double dummies_estimate(const double* atoms_xyz, int n) {
+    if (n <= 0) return 0;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) lo[k] = std::numeric_limits<double>::max(), hi[k] = -lo[k];
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double v = atoms_xyz[3 * (size_t)i + k];
+            lo[k] = std::min(lo[k], v), hi[k] = std::max(hi[k], v);
+        }
+    double vol = 1;
+    for (int k = 0; k < 3; ++k) vol *= (hi[k] - lo[k]) + 2 * 2.15 + 2;
+    return std::min(vol, 82.0 * n);
+}
+
+}  // namespace
 
 Server::Server(int slots, size_t budget_bytes) {
     if (slots < 1 || slots > 16) throw std::runtime_error("ausaxs_hip: server slots must be in [1, 16]");
@@ -25,6 +66,10 @@ Server::~Server() {
     }
     cv_work_.notify_all();
     for (auto& t : workers_) t.join();
+    // the resident structures' buffers go back to the pool before the streams end
+    (void)hipDeviceSynchronize();
+    lru_.clear();
+    cache_.clear();
     for (auto& s : streams_) (void)hipStreamDestroy(s);
 }
 
@@ -44,14 +89,11 @@ size_t Server::footprint(Mode mode, const double* atoms_xyz, int n, const double
             const double v = waters_xyz[3 * (size_t)i + k];
             lo[k] = std::min(lo[k], v), hi[k] = std::max(hi[k], v);
         }
-    double diag2 = 0, vol = 1;
-    for (int k = 0; k < 3; ++k) {
+    double diag2 = 0;
+    for (int k = 0; k < 3; ++k)
         if (n + m > 0) diag2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
-        vol *= n > 0 ? (ahi[k] - alo[k]) + 2 * 2.15 + 2 : 0;
-    }
     const double nb = std::min<double>(s.bin_count, std::sqrt(diag2) / s.bin_width + 4);
-    // dummies: at most the padded atom box (1 A^3 per lattice point)
-    const double x = mode == Mode::FFGrid ? std::min(vol, 82.0 * n) : 0.0;
+    const double x = mode == Mode::FFGrid ? dummies_estimate(atoms_xyz, n) : 0.0;
     const double P = n + m + x;
     const double C = 16;
     const double tile = s.tile_j > 0 ? s.tile_j : 8000;
@@ -95,6 +137,19 @@ int64_t Server::submit(Mode mode, const double* atoms_xyz, const int32_t* ff, in
     r->s = s;
     r->z_exv = z_exv;
     r->bytes = footprint(mode, atoms_xyz, n, waters_xyz, m, nq, s);
+    // pair-kernel pairs: everything but the grid's dummy-dummy block (lattice)
+    const double pw = (double)n + m, x = mode == Mode::FFGrid ? dummies_estimate(atoms_xyz, n) : 0.0;
+    r->est_pairs = pw * (pw - 1) / 2 + pw * x;
+    uint64_t h = 0xcbf29ce484222325ull;
+    const int md = (int)mode;
+    h = fnv(h, &md, sizeof md);
+    h = fnv(h, &s.bin_width, sizeof s.bin_width);
+    h = fnv(h, &s.bin_count, sizeof s.bin_count);
+    h = fnv(h, &z_exv, sizeof z_exv);
+    h = fnv(h, r->atoms.data(), r->atoms.size() * sizeof(double));
+    h = fnv(h, r->ff.data(), r->ff.size() * sizeof(int32_t));
+    h = fnv(h, r->waters.data(), r->waters.size() * sizeof(double));
+    r->key = h;
     std::lock_guard<std::mutex> lk(m_);
     if (r->bytes > st_.budget)
         throw std::runtime_error("ausaxs_hip: request needs an estimated " + std::to_string(r->bytes >> 20) +
@@ -109,32 +164,190 @@ int64_t Server::submit(Mode mode, const double* atoms_xyz, const int32_t* ff, in
     return r->id;
 }
 
-void Server::run(Request& r) {
+bool Server::mergeable(const Request& a, const Request& b) const {
+    return a.s.bin_width == b.s.bin_width && a.s.tile_j == b.s.tile_j && a.s.lattice == b.s.lattice;
+}
+
+void Server::make_room(size_t need) {
+    while (st_.in_use + st_.cached_bytes + need > st_.budget) {
+        // least recently used idle entry
+        auto it = lru_.end();
+        bool found = false;
+        while (it != lru_.begin()) {
+            --it;
+            if (!(*it)->busy) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) return;
+        std::shared_ptr<Entry> e = *it;
+        lru_.erase(it);
+        auto range = cache_.equal_range(e->key);
+        for (auto c = range.first; c != range.second; ++c)
+            if (c->second == e) {
+                cache_.erase(c);
+                break;
+            }
+        e->in_cache = false;
+        st_.cached_bytes -= e->bytes;
+        ++st_.evictions;
+    }
+}
+
+std::shared_ptr<Server::Entry> Server::acquire(const Request& r) {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        auto range = cache_.equal_range(r.key);
+        for (auto c = range.first; c != range.second; ++c) {
+            Entry& e = *c->second;
+            if (e.busy || e.mode != r.mode || e.s.bin_width != r.s.bin_width || e.s.bin_count != r.s.bin_count ||
+                e.s.weighted_bins != r.s.weighted_bins || e.s.tile_j != r.s.tile_j || e.s.lattice != r.s.lattice ||
+                e.z_exv != r.z_exv || e.atoms != r.atoms || e.ff != r.ff || e.waters != r.waters)
+                continue;
+            e.busy = true;
+            st_.cached_bytes -= e.bytes;
+            lru_.splice(lru_.begin(), lru_, e.lru);
+            ++st_.cache_hits;
+            return c->second;
+        }
+        ++st_.cache_misses;
+    }
+    // build the structure outside the lock (host set-up: packing, grid dummies, work lists)
+    auto e = std::make_shared<Entry>();
+    e->key = r.key;
+    e->mode = r.mode;
+    e->s = r.s;
+    e->z_exv = r.z_exv;
+    e->atoms = r.atoms;
+    e->ff = r.ff;
+    e->waters = r.waters;
     const int n = (int)r.ff.size(), m = (int)(r.waters.size() / 3);
-    std::unique_ptr<ResidentStructure> st;
     if (r.mode == Mode::FFGrid) {
-        st = make_grid_structure(r.atoms.data(), r.ff.data(), n, r.waters.data(), m, r.s);
+        e->st = make_grid_structure(r.atoms.data(), r.ff.data(), n, r.waters.data(), m, r.s);
     } else {
         std::vector<float> a, w;
         pack_ff_points(r.atoms.data(), r.ff.data(), n, 0, a);
         pack_ff_points(r.waters.data(), nullptr, m, kWaterBin, w);
-        st = std::make_unique<ResidentStructure>(r.mode, a.data(), n, w.data(), m, r.s, r.z_exv);
+        e->st = std::make_unique<ResidentStructure>(r.mode, a.data(), n, w.data(), m, r.s, r.z_exv);
     }
-    auto h = st->calculate();
-    r.pairs = st->engine().pairs();
-    r.kernel_pairs = st->engine().kernel_pairs();
-    r.I = h->debye_transform(r.q);
+    e->busy = true;
+    std::lock_guard<std::mutex> lk(m_);
+    cache_.emplace(e->key, e);
+    lru_.push_front(e);
+    e->lru = lru_.begin();
+    e->in_cache = true;
+    // beyond the entry cap: the least recently used idle entries go
+    while (cache_.size() > kCacheEntries) {
+        auto it = lru_.end();
+        bool found = false;
+        while (it != lru_.begin()) {
+            --it;
+            if (!(*it)->busy) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+        std::shared_ptr<Entry> old = *it;
+        lru_.erase(it);
+        auto range = cache_.equal_range(old->key);
+        for (auto c = range.first; c != range.second; ++c)
+            if (c->second == old) {
+                cache_.erase(c);
+                break;
+            }
+        old->in_cache = false;
+        st_.cached_bytes -= old->bytes;
+        ++st_.evictions;
+    }
+    return e;
+}
+
+void Server::release(const std::shared_ptr<Entry>& e) {
+    // the entry's device memory now counts as cached (its last work is done:
+    // the request's results were copied back on this thread's stream)
+    const size_t bytes = e->st->has_engine() ? e->st->engine().device_bytes() : 0;
+    std::lock_guard<std::mutex> lk(m_);
+    e->busy = false;
+    e->bytes = bytes;
+    if (e->in_cache) st_.cached_bytes += bytes;
+}
+
+void Server::run(const std::vector<std::shared_ptr<Request>>& batch) {
+    // each member's resident structure; a member whose structure cannot be
+    // built fails alone
+    std::vector<std::shared_ptr<Entry>> es(batch.size());
+    for (size_t k = 0; k < batch.size(); ++k) {
+        try {
+            es[k] = acquire(*batch[k]);
+        } catch (const std::exception& ex) {
+            batch[k]->error = ex.what();
+        }
+    }
+    auto run_alone = [&](size_t k) {
+        Request& r = *batch[k];
+        try {
+            ResidentStructure& st = *es[k]->st;
+            auto h = st.calculate();
+            r.pairs = st.pairs();
+            r.kernel_pairs = st.engine().kernel_pairs();
+            r.I = h->debye_transform(r.q);
+        } catch (const std::exception& ex) {
+            r.error = ex.what();
+        }
+    };
+    std::vector<size_t> live;
+    for (size_t k = 0; k < batch.size(); ++k)
+        if (es[k]) live.push_back(k);
+    if (live.size() == 1) {
+        run_alone(live[0]);
+    } else if (live.size() > 1) {
+        std::vector<ResidentStructure*> ms;
+        for (size_t k : live) ms.push_back(es[k]->st.get());
+        bool merged_ok = false;
+        try {
+            int64_t pairs = 0, kpairs = 0;
+            auto hs = ResidentStructure::calculate_merged(ms, &pairs, &kpairs);
+            for (size_t i = 0; i < live.size(); ++i) {
+                Request& r = *batch[live[i]];
+                r.pairs = ms[i]->pairs();
+                r.kernel_pairs = i == 0 ? kpairs : 0;  // the launch's pair-kernel pairs, once
+                r.I = hs[i]->debye_transform(r.q);
+            }
+            merged_ok = true;
+            std::lock_guard<std::mutex> lk(m_);
+            ++st_.batches;
+            st_.batched += (int64_t)live.size();
+        } catch (const std::exception&) {
+        }
+        // a failure in the merged launch (one member's distances beyond its
+        // bins, say) is found again member by member, so each gets its own error
+        if (!merged_ok)
+            for (size_t k : live) {
+                batch[k]->error.clear();
+                run_alone(k);
+            }
+    }
+    for (auto& e : es)
+        if (e) release(e);
 }
 
 void Server::worker(int slot) {
     Context::set_thread_stream(streams_[slot]);
     for (;;) {
-        std::shared_ptr<Request> r;
+        std::vector<std::shared_ptr<Request>> batch;
         {
             std::unique_lock<std::mutex> lk(m_);
             // first come, first admitted: the head waits until its footprint fits
+            // (idle resident structures give way to it)
             cv_work_.wait(lk, [&] {
-                return stop_ || (!pending_.empty() && st_.in_use + pending_.front()->bytes <= st_.budget);
+                if (stop_) return true;
+                if (pending_.empty()) return false;
+                const size_t need = pending_.front()->bytes;
+                make_room(need);
+                return st_.in_use + st_.cached_bytes + need <= st_.budget ||
+                       (st_.in_use == 0 && st_.in_use + need <= st_.budget);
             });
             if (stop_) {
                 for (auto& p : pending_) {
@@ -145,34 +358,57 @@ void Server::worker(int slot) {
                 cv_done_.notify_all();
                 return;
             }
-            r = pending_.front();
+            batch.push_back(pending_.front());
             pending_.pop_front();
-            st_.in_use += r->bytes;
+            size_t bytes = batch[0]->bytes;
+            double pairs = batch[0]->est_pairs;
+            // small requests queued behind a small head join its launch
+            if (pairs < kSmallPairs)
+                while (!pending_.empty() && (int)batch.size() < kBatchMembers) {
+                    const Request& nx = *pending_.front();
+                    if (nx.est_pairs >= kSmallPairs || !mergeable(*batch[0], nx) || pairs + nx.est_pairs > kBatchPairs ||
+                        st_.in_use + st_.cached_bytes + bytes + nx.bytes > st_.budget)
+                        break;
+                    bytes += nx.bytes;
+                    pairs += nx.est_pairs;
+                    batch.push_back(pending_.front());
+                    pending_.pop_front();
+                }
+            const auto now = clock::now();
+            for (auto& r : batch) {
+                st_.in_use += r->bytes;
+                r->t_admit = now;
+            }
             st_.peak_in_use = std::max(st_.peak_in_use, st_.in_use);
-            st_.peak_in_flight = std::max(st_.peak_in_flight, ++st_.in_flight);
-            r->t_admit = clock::now();
+            st_.in_flight += (int)batch.size();
+            st_.peak_in_flight = std::max(st_.peak_in_flight, st_.in_flight);
         }
         // the next head may fit beside this one
         cv_work_.notify_one();
         try {
-            run(*r);
+            run(batch);
         } catch (const std::exception& e) {
-            r->error = e.what();
+            for (auto& r : batch)
+                if (r->error.empty() && r->I.empty()) r->error = e.what();
         } catch (...) {
-            r->error = "unknown error";
+            for (auto& r : batch)
+                if (r->error.empty() && r->I.empty()) r->error = "unknown error";
         }
         {
             std::lock_guard<std::mutex> lk(m_);
-            st_.in_use -= r->bytes;
-            --st_.in_flight;
-            r->t_done = clock::now();
-            r->done = true;
-            if (r->error.empty()) {
-                ++st_.completed;
-                st_.pairs += r->pairs;
-                st_.kernel_pairs += r->kernel_pairs;
-            } else {
-                ++st_.failed;
+            const auto now = clock::now();
+            for (auto& r : batch) {
+                st_.in_use -= r->bytes;
+                --st_.in_flight;
+                r->t_done = now;
+                r->done = true;
+                if (r->error.empty()) {
+                    ++st_.completed;
+                    st_.pairs += r->pairs;
+                    st_.kernel_pairs += r->kernel_pairs;
+                } else {
+                    ++st_.failed;
+                }
             }
         }
         cv_done_.notify_all();

@@ -241,7 +241,11 @@ typedef struct ausaxs_server ausaxs_server;
 /* A request loop with `slots` device slots (worker threads with their own
  * streams, 1..16) and a device-memory budget in GiB. Requests are admitted
  * in arrival order as soon as a slot is idle and their estimated footprint
- * fits the budget, and run concurrently with those already on the GPU. There
+ * fits the budget, and run concurrently with those already on the GPU.
+ * Structures stay resident across requests (keyed by their exact content,
+ * mode and settings; idle ones are evicted least recently used first when
+ * the budget needs the memory), and small requests queued together run as
+ * one merged pair launch. There
  * is no reference counterpart; each request is one manager evaluation
  * (HistogramManagerMTFF{Avg,Explicit,Grid}::calculate_all) plus
  * debye_transform on its q grid, identical to ausaxs_structure_create[_grid] +
@@ -270,8 +274,24 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
                         int* in_flight, int* peak_in_flight, double* peak_gb, long long* pairs,
                         long long* kernel_pairs);
 
+/* Batching and residency counters: requests served from a resident
+ * structure (cache_hits) or built (cache_misses), merged launches (batches)
+ * and the requests they ran (batched), evicted structures, and the device
+ * memory idle resident structures hold (GiB). No reference counterpart. */
+int ausaxs_server_batch_stats(ausaxs_server* s, long long* cache_hits, long long* cache_misses, long long* batches,
+                              long long* batched, long long* evictions, double* cached_gb);
+
 /* Stops the loop: running requests finish, queued ones fail. */
 int ausaxs_server_free(ausaxs_server* s);
+
+/* The library's caching allocator (every device and pinned host buffer):
+ * bytes of device blocks held and handed out, hipMalloc / hipHostMalloc calls
+ * so far, allocations served from freed blocks. A freed buffer returns to the
+ * pool, not to HIP (hipFree waits for the whole device); ausaxs_pool_trim
+ * returns the free blocks to HIP. No reference counterpart. */
+int ausaxs_pool_stats(size_t* device_held, size_t* device_in_use, long long* device_mallocs,
+                      long long* host_mallocs, long long* reuses);
+int ausaxs_pool_trim(void);
 
 /* The reference's default q sub-axis (ConstantsAxes.h:14, Axis::sub_axis,
  * Axis.cpp:40-59). q must hold 200 doubles; *nq receives the length. */

@@ -163,3 +163,79 @@ def test_serve_errors():
         srv.submit(A.MODE_SIMPLE, r["xyz"], r["ff"], r["wxyz"], r["q"])
     with pytest.raises(A.AusaxsError):
         A.Server(slots=0, budget_gb=1.0)
+
+
+def test_serve_merged_launches():
+    # small requests queued together run as merged launches (one engine, one
+    # pair launch); each result equals the request alone, bit for bit
+    reqs = [make_request(300 + k, n, MODES[k % 3]) for k, n in enumerate([250, 420, 333, 610, 180, 505, 777, 290])]
+    srv = A.Server(slots=1, budget_gb=8.0)
+    # a large request first keeps the one slot busy while the small ones queue
+    big = make_request(299, 20000, A.MODE_FFEXPLICIT)
+    ids = [srv.submit(big["mode"], big["xyz"], big["ff"], big["wxyz"], big["q"])]
+    ids += [srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]) for r in reqs]
+    res = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    print("stats:", st)
+    assert st["completed"] == len(ids) and st["failed"] == 0
+    assert st["batches"] >= 1 and st["batched"] >= 2
+    for r, I in zip([big] + reqs, res):
+        assert np.array_equal(I, direct(r)), r["mode"]
+
+
+def test_serve_resident_structures():
+    # a repeated structure is served from its resident copy: no rebuild and,
+    # after the first request, no new device allocation at all
+    r = make_request(41, 2500, A.MODE_FFGRID)
+    srv = A.Server(slots=1, budget_gb=8.0)
+    I0 = srv.wait(srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))[0]
+    m0 = A.pool_stats()["device_mallocs"]
+    for _ in range(6):
+        I = srv.wait(srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))[0]
+        assert np.array_equal(I, I0)
+    st = srv.stats()
+    print("stats:", st, "pool:", A.pool_stats())
+    assert st["cache_hits"] == 6 and st["cache_misses"] == 1
+    assert A.pool_stats()["device_mallocs"] == m0
+    assert np.array_equal(I0, direct(r))
+    # another mode of the same coordinates is another structure
+    I_avg = srv.wait(srv.submit(A.MODE_FFAVG, r["xyz"], r["ff"], r["wxyz"], r["q"]))[0]
+    assert srv.stats()["cache_misses"] == 2
+    assert np.array_equal(I_avg, direct(dict(r, mode=A.MODE_FFAVG)))
+
+
+def test_serve_steady_state_allocations():
+    # a second pass over the same request mix allocates far less than the
+    # first: resident structures and pooled buffers instead of hipMalloc
+    reqs = requests(12, seed=9)
+    srv = A.Server(slots=3, budget_gb=16.0)
+    m0 = A.pool_stats()["device_mallocs"]
+    first = [srv.wait(srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))[0] for r in reqs]
+    m1 = A.pool_stats()["device_mallocs"]
+    ids = [srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]) for r in reqs]
+    second = [srv.wait(i)[0] for i in ids]
+    m2 = A.pool_stats()["device_mallocs"]
+    print("device mallocs: first pass", m1 - m0, "second pass", m2 - m1, "stats", srv.stats())
+    for a, b in zip(first, second):
+        assert np.array_equal(a, b)
+    assert srv.stats()["cache_hits"] >= len(reqs) // 2
+    assert m2 - m1 <= max(4, (m1 - m0) // 4)
+
+
+def test_serve_eviction_under_budget():
+    # resident structures give way to new requests when the budget is tight
+    reqs = requests(10, seed=12)
+    sizes = []
+    for r in reqs:
+        s1 = A.Server(slots=1, budget_gb=64.0)
+        s1.wait(s1.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))
+        sizes.append(s1.stats()["peak_gb"])
+        s1.close()
+    srv = A.Server(slots=2, budget_gb=1.2 * max(sizes))
+    for rep in range(2):
+        for r in reqs:
+            I = srv.wait(srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))[0]
+            assert np.array_equal(I, direct(r))
+    st = srv.stats()
+    print("stats:", st)
+    assert st["failed"] == 0 and st["peak_gb"] <= 1.2 * max(sizes) + 1e-9
