@@ -617,18 +617,27 @@ void PairEngine::build(int tile_j) {
     layout_.clear();
     for (size_t k = 0; k < sets_.size(); ++k)
         layout_.push_back(placed[k] ? layout_set(sets_[k], buf, weighted_ ? &perm : nullptr) : layout_counts(sets_[k]));
-    WorkList plain, split;
-    plain.runs = split.runs = runs;
-    plain.set_runs = split.set_runs = set_runs;
-    plain.lattice_pairs = split.lattice_pairs = lattice_pairs;
-    for (size_t k = 0; k < jobs_.size(); ++k) {
-        const PairJob& job = jobs_[k];
-        append_job(plain, job, sets_, layout_, n_rows_, tile_j, false, !on_lattice[k]);
-        append_job(split, job, sets_, layout_, n_rows_, tile_j, true, !on_lattice[k]);
+    // Narrower items for small structures: a launch of a few full-width
+    // items leaves most CUs idle and runs as long as one item takes, so the
+    // column width halves (down to kMinTile) until there are two items per
+    // CU. The histograms are integer sums and do not depend on the tiling.
+    constexpr int kMinTile = 128;
+    const int target_items = 2 * ctx.prop.multiProcessorCount;
+    for (int tile = tile_j;; tile = std::max(kMinTile, tile / 2)) {
+        WorkList plain, split;
+        plain.runs = split.runs = runs;
+        plain.set_runs = split.set_runs = set_runs;
+        plain.lattice_pairs = split.lattice_pairs = lattice_pairs;
+        for (size_t k = 0; k < jobs_.size(); ++k) {
+            const PairJob& job = jobs_[k];
+            append_job(plain, job, sets_, layout_, n_rows_, tile, false, !on_lattice[k]);
+            append_job(split, job, sets_, layout_, n_rows_, tile, true, !on_lattice[k]);
+        }
+        const bool use_split = prefer_split(padded_work(plain.items), item_rows(plain.items),
+                                            padded_work(split.items), item_rows(split.items));
+        wl_ = use_split ? std::move(split) : std::move(plain);
+        if ((int)wl_.items.size() >= target_items || tile <= kMinTile) break;
     }
-    const bool use_split = prefer_split(padded_work(plain.items), item_rows(plain.items), padded_work(split.items),
-                                        item_rows(split.items));
-    wl_ = use_split ? std::move(split) : std::move(plain);
     sort_items(wl_.items);
     rows_ = item_rows(wl_.items);
     copies_ = pick_copies(nbins_, rows_);
@@ -776,8 +785,11 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     const size_t n = (size_t)n_rows_ * (nbins_ + 1);
     cnt.resize(n);
     dsum.resize(n);
-    check(hipMemcpyAsync(cnt.data(), d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
-    check(hipMemcpyAsync(dsum.data(), d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    // counts and delta sums through pinned memory (asynchronous copies, one wait)
+    h_stage_.ensure(2 * n);
+    check(hipMemcpyAsync(h_stage_.p, d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    check(hipMemcpyAsync(h_stage_.p + n, d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
+          "D2H");
     if (wsum) {
         if (!weighted_) throw std::runtime_error("ausaxs_hip: weight sums of an engine without weights");
         wsum->items.resize(n);
@@ -792,6 +804,8 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     if (!lat_.empty())
         check(hipMemcpyAsync(&flag, d_lat_flag_.p, sizeof(int), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     ctx.sync();
+    std::memcpy(cnt.data(), h_stage_.p, n * sizeof(uint64_t));
+    std::memcpy(dsum.data(), h_stage_.p + n, n * sizeof(int64_t));
     if (flag)
         throw std::runtime_error("ausaxs_hip: lattice correlation left a count off an integer (flag " +
                                  std::to_string(flag) + ")");

@@ -853,41 +853,61 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     if (rows == 0 || nq == 0) return cached_sinqd_;
 
     Context& ctx = Context::get();
-    // per-thread operand buffers, reused across calls (serving slots run
-    // concurrently, each on its own thread and stream)
-    thread_local DevBuf<double> dH, dd, dq, dout;
+    hipStream_t st = ctx.stream();
+    // Per-thread buffers and events, reused across calls (serving slots run
+    // concurrently, each on its own thread and stream). All operands go up in
+    // one copy through pinned memory ([q | per group: H rows, d axis]), the
+    // groups' contractions run back to back on the stream (one scratch area,
+    // used in stream order), and the result comes back in one copy.
+    thread_local DevBuf<double> dev;
     thread_local DevBuf<char> scratch;
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0), "hipEventCreate");
-    check(hipEventCreate(&e1), "hipEventCreate");
-    dq.upload(q.data(), q.size(), ctx.stream());
-    dout.ensure((size_t)rows * nq);
-    check(hipEventRecord(e0, ctx.stream()), "hipEventRecord");
-    size_t row0 = 0;
+    thread_local HostBuf<double> pin;
+    thread_local hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev)
+        if (!e) check(hipEventCreate(&e), "hipEventCreate");
+    size_t n_in = (size_t)nq, scratch_bytes = 0;
     for (int a = 0; a < 3; ++a) {
         const int ra = (int)keys[a].size();
         if (ra == 0) continue;
         if ((int)axes[a]->size() < K) throw std::runtime_error("ausaxs_hip: distance axis shorter than the histogram");
-        // the uploads are ordered on the stream before this group's kernels
-        dH.upload(H[a].data(), H[a].size(), ctx.stream());
-        dd.upload(axes[a]->data(), K, ctx.stream());
-        scratch.ensure(debye_scratch_bytes(ra, K, nq));
-        DebyeLaunch dl{dH.p, dd.p, dq.p, ra, K, nq, debye_terms, dout.p + row0 * nq, scratch.p};
-        check(debye_terms > 0 ? launch_debye(dl, ctx.stream()) : launch_debye_f64(dl, ctx.stream()), "debye");
-        // dH/dd are reused by the next group: wait before overwriting them
-        ctx.sync();
+        n_in += (size_t)ra * K + K;
+        scratch_bytes = std::max(scratch_bytes, debye_scratch_bytes(ra, K, nq));
+    }
+    const size_t n_out = (size_t)rows * nq;
+    dev.ensure(n_in + n_out);
+    pin.ensure(std::max(n_in, n_out));
+    scratch.ensure(scratch_bytes);
+    std::memcpy(pin.p, q.data(), (size_t)nq * sizeof(double));
+    size_t off = nq;
+    size_t h_off[3] = {0, 0, 0}, d_off[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+        if (keys[a].empty()) continue;
+        h_off[a] = off;
+        std::memcpy(pin.p + off, H[a].data(), H[a].size() * sizeof(double));
+        off += H[a].size();
+        d_off[a] = off;
+        std::memcpy(pin.p + off, axes[a]->data(), (size_t)K * sizeof(double));
+        off += K;
+    }
+    check(hipMemcpyAsync(dev.p, pin.p, n_in * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    check(hipEventRecord(ev[0], st), "hipEventRecord");
+    double* out = dev.p + n_in;
+    size_t row0 = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int ra = (int)keys[a].size();
+        if (ra == 0) continue;
+        DebyeLaunch dl{dev.p + h_off[a], dev.p + d_off[a], dev.p, ra, K, nq, debye_terms, out + row0 * nq, scratch.p};
+        check(debye_terms > 0 ? launch_debye(dl, st) : launch_debye_f64(dl, st), "debye");
         row0 += ra;
     }
-    check(hipEventRecord(e1, ctx.stream()), "hipEventRecord");
-    check(hipMemcpyAsync(cached_sinqd_.data(), dout.p, cached_sinqd_.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, ctx.stream()),
-          "D2H");
+    check(hipEventRecord(ev[1], st), "hipEventRecord");
+    // the pinned buffer's inputs were consumed by the copy above (stream order)
+    check(hipMemcpyAsync(pin.p, out, n_out * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
     ctx.sync();
+    std::memcpy(cached_sinqd_.data(), pin.p, n_out * sizeof(double));
     float ms = 0;
-    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
     last_debye_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return cached_sinqd_;
 }
 
