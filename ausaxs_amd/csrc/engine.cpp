@@ -785,11 +785,13 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     const size_t n = (size_t)n_rows_ * (nbins_ + 1);
     cnt.resize(n);
     dsum.resize(n);
-    // counts and delta sums through pinned memory (asynchronous copies, one wait)
-    h_stage_.ensure(2 * n);
-    check(hipMemcpyAsync(h_stage_.p, d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
-    check(hipMemcpyAsync(h_stage_.p + n, d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
-          "D2H");
+    // counts and delta sums in one per-thread host buffer, grown as needed
+    // (pageable: see CompositeDistanceHistogramHIP::sinqd), then one wait
+    thread_local std::vector<uint64_t> stage;
+    if (stage.size() < 2 * n) stage.resize(2 * n);
+    uint64_t* const h_stage = stage.data();
+    check(hipMemcpyAsync(h_stage, d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    check(hipMemcpyAsync(h_stage + n, d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     if (wsum) {
         if (!weighted_) throw std::runtime_error("ausaxs_hip: weight sums of an engine without weights");
         wsum->items.resize(n);
@@ -804,8 +806,8 @@ void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum
     if (!lat_.empty())
         check(hipMemcpyAsync(&flag, d_lat_flag_.p, sizeof(int), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
     ctx.sync();
-    std::memcpy(cnt.data(), h_stage_.p, n * sizeof(uint64_t));
-    std::memcpy(dsum.data(), h_stage_.p + n, n * sizeof(int64_t));
+    std::memcpy(cnt.data(), h_stage, n * sizeof(uint64_t));
+    std::memcpy(dsum.data(), h_stage + n, n * sizeof(int64_t));
     if (flag)
         throw std::runtime_error("ausaxs_hip: lattice correlation left a count off an integer (flag " +
                                  std::to_string(flag) + ")");

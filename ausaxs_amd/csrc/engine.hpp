@@ -339,7 +339,6 @@ class PairEngine {
     DevBuf<int32_t> d_row_list_, d_chunk_ptr_, d_row_chunk_, d_rowmaps_;
     DevBuf<uint64_t> d_partial_, d_cnt_, d_tmp_cnt_;
     DevBuf<int64_t> d_dsum_, d_tmp_dsum_;
-    HostBuf<uint64_t> h_stage_;  // pinned staging of the downloads
     hipEvent_t ev_[3]{};
     bool have_timing_ = false;
 

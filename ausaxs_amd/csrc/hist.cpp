@@ -856,12 +856,22 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     hipStream_t st = ctx.stream();
     // Per-thread buffers and events, reused across calls (serving slots run
     // concurrently, each on its own thread and stream). All operands go up in
-    // one copy through pinned memory ([q | per group: H rows, d axis]), the
-    // groups' contractions run back to back on the stream (one scratch area,
-    // used in stream order), and the result comes back in one copy.
+    // one copy ([q | per group: H rows, d axis]), the groups' contractions run
+    // back to back on the stream (one scratch area, used in stream order), and
+    // the result comes back in one copy. The host side is pageable memory:
+    // pinned staging made single requests faster but raised the median
+    // latency of concurrent serving requests (2.7 -> 4.0 ms on the serving
+    // benchmark), whose small asynchronous copies then queue on the DMA engines.
     thread_local DevBuf<double> dev;
     thread_local DevBuf<char> scratch;
-    thread_local HostBuf<double> pin;
+    thread_local std::vector<double> pin_v;
+    struct {
+        double* p = nullptr;
+        void ensure(std::vector<double>& v, size_t n) {
+            if (v.size() < n) v.resize(n);
+            p = v.data();
+        }
+    } pin;
     thread_local hipEvent_t ev[2] = {nullptr, nullptr};
     for (auto& e : ev)
         if (!e) check(hipEventCreate(&e), "hipEventCreate");
@@ -875,7 +885,7 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     }
     const size_t n_out = (size_t)rows * nq;
     dev.ensure(n_in + n_out);
-    pin.ensure(std::max(n_in, n_out));
+    pin.ensure(pin_v, std::max(n_in, n_out));
     scratch.ensure(scratch_bytes);
     std::memcpy(pin.p, q.data(), (size_t)nq * sizeof(double));
     size_t off = nq;
