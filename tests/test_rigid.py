@@ -362,3 +362,24 @@ def test_rigid_benchmark_bodies_hydrated():
         err = relerr(I[c], O.debye_simple(ref, q, cfg["bw"]))
         print(f"conformation {c}: I(q) rel err {err:.2e}")
         assert err <= 1e-5
+
+
+def test_rigid_scan_hydrated_across_batches():
+    # more conformations than one batch holds (1,024): the water sets of the
+    # second batch follow those of the first in the input; each conformation
+    # equals the same conformation scanned alone
+    bodies, weights = make_bodies([60, 45], seed=97, spacing=14.0)
+    rng = np.random.default_rng(98)
+    scan = A.RigidScan(bodies, weights, np.zeros((1, 3)), np.ones(1, np.float32))
+    n_conf = 1100
+    T = random_transforms(rng, bodies, n_conf)
+    counts = rng.integers(0, 24, n_conf)
+    waters = [(water_shell(rng, np.array([7.0, 0, 0]), int(m), 12, 16), 1.0) for m in counts]
+    q = np.linspace(0.01, 0.5, 20)
+    I = scan.scan_hydrated(T, waters, q)
+    assert np.all(np.isfinite(I))
+    for c in (0, 1023, 1024, 1099):
+        I1 = scan.scan_hydrated(T[c:c + 1], waters[c:c + 1], q)
+        np.testing.assert_allclose(I1[0], I[c], rtol=1e-13)
+    ref = oracle_conf(bodies, weights, T[1099], as_xyzw(waters[1099][0], 1.0))
+    check_hist(scan.histogram(0), ref)
