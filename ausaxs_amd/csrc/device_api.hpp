@@ -141,7 +141,7 @@ __host__ __device__ inline double weight_value(int64_t item_bits, int64_t strip_
 }
 
 // LDS bytes of one pair_hist_kernel workgroup (static column tile included)
-constexpr size_t kStaticLds = (512 + 4) * 16;
+constexpr size_t kStaticLds = (512 + 2) * 16;
 constexpr size_t pair_lds_bytes(int nbins, int rows, int copies) {
     return (size_t)copies * rows * (nbins + 2) * sizeof(uint64_t) + kStaticLds;
 }

@@ -6,7 +6,7 @@
 //     two parts that hold the same 512 row points and split each column tile;
 //   * row points in VGPRs (2 per lane); column points staged through LDS in
 //     tiles of 512 and read back as wave-wide broadcasts in double-buffered
-//     groups of 4, so each lane evaluates 2 pairs per column point (LDS reads
+//     groups of 2, so each lane evaluates 2 pairs per column point (LDS reads
 //     and LDS atomics retire in order, so the broadcast reads of the next
 //     group never wait on the atomics of the current one);
 //   * a workgroup-private histogram of packed uint64 cells in LDS updated
@@ -59,7 +59,7 @@ constexpr int kThreads = kRowLanes * kParts;
 constexpr int kRowsPerLane = 2;
 constexpr int kTileI = kRowLanes * kRowsPerLane;  // row points per work item
 constexpr int kColTile = 512;                    // column points per LDS tile (each part takes a share)
-constexpr int kColGroup = 4;                     // column points per double-buffered group
+constexpr int kColGroup = 2;                     // column points per double-buffered group
 static_assert(kColTile + kColGroup == kStaticLds / sizeof(float4), "static LDS size");
 
 // d2 of the reference's SIMD body: separately rounded squares and sums
@@ -356,9 +356,9 @@ __device__ __forceinline__ void run_weighted(const float4* __restrict__ pts, con
 // bins). An item with one output row in a two-row launch uses the same LDS as
 // 2 * copies one-row histograms. The copies are summed at the flush. The
 // kernel may use up to 80 VGPRs (6 waves per SIMD, what the LDS of a 0.1 A
-// workload allows anyway): held to 64, it spilled loop-invariant values to
-// scratch in every workgroup, ~200 MB of scratch writes per launch on the
-// 250k-atom workload, at the same pair rate.
+// workload allows anyway); with column groups of 2 it needs 64 and spills
+// nothing (groups of 4: 80 VGPRs and 0.7 % slower on the 250k-atom workload;
+// 1,024-thread workgroups at 8 waves per SIMD: 2 % slower).
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void pair_hist_kernel(const float4* __restrict__ pts,
                                                              const float4* __restrict__ runs,
                                                              const PairWork* __restrict__ items, int n_items,
