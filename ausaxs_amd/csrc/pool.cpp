@@ -30,7 +30,7 @@ struct Arena {
     bool host = false;
     std::unordered_map<void*, size_t> live;              // block -> size class
     std::unordered_map<size_t, std::vector<Block>> free;  // size class -> free blocks
-    std::vector<void*> parked;                            // released during graph capture
+    std::vector<std::pair<void*, size_t>> parked;         // released during graph capture (block, class)
     size_t held = 0, in_use = 0;
     int64_t mallocs = 0;
 };
@@ -83,6 +83,10 @@ class Pool {
 
     void release(Arena& a, void* p) {
         if (!p) return;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (!a.live.count(p)) return;  // not a pool block
+        }
         hipStream_t s = Context::get().stream();
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         const bool capturing = hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone;
@@ -97,14 +101,17 @@ class Pool {
         }
         std::lock_guard<std::mutex> lk(m_);
         auto it = a.live.find(p);
-        if (it == a.live.end()) return;  // not a pool block
+        if (it == a.live.end()) {  // released twice meanwhile
+            if (ev) events_.push_back(ev);
+            return;
+        }
         const size_t cls = it->second;
         a.live.erase(it);
         a.in_use -= cls;
         if (capturing) {
             // an event record would join the captured graph: keep the block
             // out of circulation until the next trim
-            a.parked.push_back(p);
+            a.parked.emplace_back(p, cls);
             return;
         }
         a.free[cls].push_back(Block{p, ev});
@@ -124,9 +131,10 @@ class Pool {
                 }
                 blocks.clear();
             }
-            for (void* p : a->parked) {
+            for (auto [p, cls] : a->parked) {
                 if (a->host) (void)hipHostFree(p);
                 else (void)hipFree(p);
+                a->held -= cls;
             }
             a->parked.clear();
         }
