@@ -126,6 +126,7 @@ def _load():
         "ausaxs_exact_debye": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
         "ausaxs_last_timing": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_longlong), P(C.c_int),
                                          P(C.c_longlong), P(C.c_longlong)]),
+        "ausaxs_last_merge_timing": (C.c_int, [vp, P(C.c_double), P(C.c_longlong)]),
         "ausaxs_fitter_create": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_uint, vp, C.c_int, P(vp)]),
         "ausaxs_scalable_create": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, P(_Settings), P(vp)]),
         "ausaxs_scalable_calculate": (C.c_int, [vp, C.c_double, P(vp)]),
@@ -303,8 +304,10 @@ class Structure:
         kp, lp = C.c_longlong(), C.c_longlong()
         _check(lib.ausaxs_last_timing(self._s, C.byref(pm), C.byref(tm), C.byref(pr), C.byref(ni), C.byref(kp),
                                       C.byref(lp)))
+        mm, mb = C.c_double(), C.c_longlong()
+        _check(lib.ausaxs_last_merge_timing(self._s, C.byref(mm), C.byref(mb)))
         return dict(pair_ms=pm.value, total_ms=tm.value, pairs=pr.value, n_items=ni.value, kernel_pairs=kp.value,
-                    lattice_pairs=lp.value)
+                    lattice_pairs=lp.value, merge_ms=mm.value, merge_bytes=mb.value)
 
 
 def calculate_all(mode, atoms, waters=None, settings=None, z_exv=1.0) -> CompositeHistogram:

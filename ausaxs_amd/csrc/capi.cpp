@@ -540,6 +540,14 @@ int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total
     });
 }
 
+int ausaxs_last_merge_timing(const ausaxs_structure* s, double* merge_ms, long long* merge_bytes) {
+    return guarded([&] {
+        require(s != nullptr, "NULL structure");
+        if (merge_ms) *merge_ms = s->last.merge_ms;
+        if (merge_bytes) *merge_bytes = s->last.merge_bytes;
+    });
+}
+
 int ausaxs_hist_calculate(int mode, const float* atoms, int n_atoms, const float* waters, int n_waters,
                           const ausaxs_settings* settings, double z_exv, ausaxs_histogram** out) {
     ausaxs_structure* s = nullptr;

@@ -153,8 +153,12 @@ hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s);
 // are cut into chunks of at most kMergeChunk: chunk c = row_list[chunk_ptr[c]
 // .. chunk_ptr[c+1]), and row r owns chunks row_chunk[r] .. row_chunk[r+1].
 // tmp_cnt/tmp_dsum: [n_chunks][nbins+1]; cnt/dsum: [n_rows][nbins+1]. Every
-// output cell is written (rows without entries become zero).
+// output cell is written (rows without entries become zero). The chunk pass
+// reads entries in batches of kMergeBatch, so row_list carries kMergeBatch - 1
+// entries of slack past its last chunk (any valid partial row; their values
+// are read and dropped).
 constexpr int kMergeChunk = 32;
+constexpr int kMergeBatch = 16;
 struct MergeLaunch {
     const uint64_t* partial;
     const int32_t* row_list;

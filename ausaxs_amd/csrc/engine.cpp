@@ -451,6 +451,8 @@ MergePlan plan_merge(std::vector<PairWork>& items, int n_rows) {
         }
         mp.row_chunk.push_back((int32_t)mp.chunk_ptr.size() - 1);
     }
+    // slack for the merge's batched reads: partial row 0 (read, then dropped)
+    mp.row_list.resize(mp.row_list.size() + kMergeBatch - 1, 0);
     return mp;
 }
 
@@ -725,6 +727,7 @@ void PairEngine::enqueue(bool timed) {
     MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
                    n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_cnt_.p, d_dsum_.p};
     check(launch_merge(ml, s), "merge kernels");
+    if (timed) check(hipEventRecord(ev_[3], s), "hipEventRecord");
     // lattice jobs add into the merged rows
     // the flag buffer holds two ints, cleared as one 8-byte word
     if (!lat_.empty())
@@ -825,9 +828,13 @@ EngineTiming PairEngine::timing() {
     t.kernel_pairs = kernel_pairs();
     t.n_items = (int)wl_.items.size();
     t.n_lattice = (int)lat_.size();
+    const int64_t row_bytes = (int64_t)(nbins_ + 1) * 8;
+    const int64_t entries = mp_.chunk_ptr.back();  // partial rows summed (row_list without its slack)
+    t.merge_bytes = row_bytes * (entries + 4 * (int64_t)mp_.n_chunks() + 2 * (int64_t)n_rows_);
     if (have_timing_) {
         check(hipEventSynchronize(ev_[2]), "hipEventSynchronize");
         check(hipEventElapsedTime(&t.pair_ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&t.merge_ms, ev_[1], ev_[3]), "hipEventElapsedTime");
         check(hipEventElapsedTime(&t.total_ms, ev_[0], ev_[2]), "hipEventElapsedTime");
     }
     return t;

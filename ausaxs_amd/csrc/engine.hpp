@@ -211,7 +211,9 @@ int item_rows(const std::vector<PairWork>& items);
 
 struct EngineTiming {
     float pair_ms = 0;          // pair_hist_kernel on the engine stream (HIP events)
+    float merge_ms = 0;         // the two merge kernels after it
     float total_ms = 0;         // pair + merge + lattice + strips
+    int64_t merge_bytes = 0;    // the merge's HBM bytes: partial rows read, chunk sums written and read, rows written
     int64_t pairs = 0;          // pairs in the histograms
     int64_t kernel_pairs = 0;   // of which evaluated by pair_hist_kernel
     int64_t lattice_pairs = 0;  // of which counted by lattice correlation
@@ -339,7 +341,7 @@ class PairEngine {
     DevBuf<int32_t> d_row_list_, d_chunk_ptr_, d_row_chunk_, d_rowmaps_;
     DevBuf<uint64_t> d_partial_, d_cnt_, d_tmp_cnt_;
     DevBuf<int64_t> d_dsum_, d_tmp_dsum_;
-    hipEvent_t ev_[3]{};
+    hipEvent_t ev_[4]{};
     bool have_timing_ = false;
 
     void build(int tile_j);

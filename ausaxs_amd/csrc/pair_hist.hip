@@ -73,15 +73,20 @@ __device__ __forceinline__ float simd_d2(float dx, float dy, float dz) {
 // addend of one pair: count 2^42 plus the offset code (kLowBias + k)
 __device__ __forceinline__ uint64_t cell_add(uint32_t code) { return (uint64_t(1) << 42) + (uint64_t)code; }
 
-// count and sum of k of one cell (see device_api.hpp)
+// count and sum of k of one cell (see device_api.hpp), without branches.
+// kCellUnit = 2^22 * M (kLowBias = 301 * 2^22), so n = floor(v / kCellUnit)
+// = floor(w / M) with w = v >> 22 < 2^42, exact in FP64; the FP64 quotient is
+// within 2^-30 of w / M, so its truncation is n, n - 1 or n + 1, and the
+// integer remainder w - c * M picks n.
+constexpr uint64_t kCellM = kCellUnit >> 22;
+static_assert(kCellM << 22 == kCellUnit, "cell unit is a multiple of 2^22");
 __device__ __forceinline__ void cell_decode(uint64_t v, uint64_t& n, uint64_t& ksum) {
-    const double inv = 1.0 / (double)kCellUnit;
-    uint64_t c = (uint64_t)((double)v * inv);
-    // the estimate is within one of floor(v / kCellUnit)
-    if (c * kCellUnit > v) --c;
-    else if (v - c * kCellUnit >= kCellUnit) ++c;
-    n = c;
-    ksum = v - c * kCellUnit;
+    const uint64_t w = v >> 22;
+    int64_t c = (int64_t)((double)w * (1.0 / (double)kCellM));
+    const int64_t r = (int64_t)w - c * (int64_t)kCellM;
+    c += (int64_t)(r >= (int64_t)kCellM) - (int64_t)(r < 0);
+    n = (uint64_t)c;
+    ksum = v - (((uint64_t)c * kCellM) << 22);
 }
 
 __device__ __forceinline__ void lds_add(uint64_t* cell, uint64_t v) {
@@ -400,6 +405,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
     }
 }
 
+// Merge phase 1: chunk c = sum of its partial rows row_list[chunk_ptr[c]] ..
+// (at most 32), one thread per bin. Each batch issues all its loads before
+// decoding any of them (the decode has no branches), so a thread keeps
+// kMergeBatch loads in flight instead of waiting on each in turn.
 __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __restrict__ partial,
                                                            const int32_t* __restrict__ row_list,
                                                            const int32_t* __restrict__ chunk_ptr, int nbins,
@@ -409,22 +418,36 @@ __global__ __launch_bounds__(256) void merge_chunks_kernel(const uint64_t* __res
     const int stride = nbins + 1;
     const int bin = blockIdx.x * blockDim.x + threadIdx.x;
     if (bin >= stride) return;
-    uint64_t n_sum = 0, low_sum = 0;
+    const uint64_t* __restrict__ col = partial + bin;
     const int e0 = chunk_ptr[c], e1 = chunk_ptr[c + 1];
     if (raw) {
         // weight pass: the items' exact integer sums, added in FP64 in a fixed order
         double w = 0;
-        for (int e = e0; e < e1; ++e) w += (double)(int64_t)partial[(size_t)row_list[e] * stride + bin];
+        for (int e = e0; e < e1; e += kMergeBatch) {
+            int64_t v[kMergeBatch];
+#pragma unroll
+            for (int u = 0; u < kMergeBatch; ++u) v[u] = (int64_t)col[(size_t)row_list[e + u] * stride];
+#pragma unroll
+            for (int u = 0; u < kMergeBatch; ++u) w += e + u < e1 ? (double)v[u] : 0.0;
+        }
         tcnt[(size_t)c * stride + bin] = 0;
         tdsum[(size_t)c * stride + bin] = __double_as_longlong(w);
         return;
     }
-#pragma unroll 8
-    for (int e = e0; e < e1; ++e) {
-        uint64_t n, k;
-        cell_decode(partial[(size_t)row_list[e] * stride + bin], n, k);
-        n_sum += n;
-        low_sum += k;
+    uint64_t n_sum = 0, low_sum = 0;
+    for (int e = e0; e < e1; e += kMergeBatch) {
+        uint64_t v[kMergeBatch];
+        // unconditional loads (row_list has kMergeBatch - 1 entries of slack);
+        // entries past the chunk are dropped as zero cells, which decode to 0, 0
+#pragma unroll
+        for (int u = 0; u < kMergeBatch; ++u) v[u] = col[(size_t)row_list[e + u] * stride];
+#pragma unroll
+        for (int u = 0; u < kMergeBatch; ++u) {
+            uint64_t n, k;
+            cell_decode(e + u < e1 ? v[u] : 0, n, k);
+            n_sum += n;
+            low_sum += k;
+        }
     }
     tcnt[(size_t)c * stride + bin] = n_sum;
     tdsum[(size_t)c * stride + bin] = (int64_t)low_sum - (int64_t)n_sum * kDeltaBias;

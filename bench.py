@@ -41,6 +41,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (AMD published)
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E (AMD published)
 FLOP_PER_PAIR = 11  # 3 sub + 3 mul + 2 add + sqrt + mul(inv_w) + add(sum d)
 METRIC = "Gpair-evals/s + I(q) profiles/s, p50/p99 latency; GPU vs CPU at ≤1e-4 rel err"
 
@@ -160,14 +161,16 @@ def bench_structure(cfg, args):
     st.set_timing(True)
     last = {}
 
-    kernel_ms = []
+    kernel_ms, merge_ms = [], []
 
     def step():
         h = st.calculate()
         last["h"] = h
-        # the pair kernel's launch time (HIP events on the structure's stream;
-        # calculate() has already synchronised on its read-back)
-        kernel_ms.append(st.last_timing()["pair_ms"])
+        # the pair kernel's and the merge's launch times (HIP events on the
+        # structure's stream; calculate() has already synchronised on its read-back)
+        tm = st.last_timing()
+        kernel_ms.append(tm["pair_ms"])
+        merge_ms.append(tm["merge_ms"])
         return h.debye_transform(q)
 
     res = timed_steps(step, args)
@@ -175,6 +178,7 @@ def bench_structure(cfg, args):
     timed_ms = kernel_ms[-args.steps:]  # the launches of the timed steps
     res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1,
                kernel_ms=float(np.mean(timed_ms)),
+               merge_ms=float(np.mean(merge_ms[-args.steps:])), merge_bytes=tm["merge_bytes"],
                items=tm["n_items"],
                extra={"atoms": len(atoms), "waters": 0 if waters is None else len(waters), "q_points": len(q),
                       "debye_terms": args.debye_terms})
@@ -235,17 +239,20 @@ def bench_grid(cfg, args):
     log(f"resident grid structure built in {time.perf_counter() - t0:.2f} s (dummies from the 1 A grid)")
     st.set_timing(True)
 
-    kernel_ms = []
+    kernel_ms, merge_ms = [], []
 
     def step():
         h = st.calculate()
-        kernel_ms.append(st.last_timing()["pair_ms"])
+        tm = st.last_timing()
+        kernel_ms.append(tm["pair_ms"])
+        merge_ms.append(tm["merge_ms"])
         return h.debye_transform(q)
 
     res = timed_steps(step, args)
     tm = st.last_timing()
     res.update(pairs=tm["pairs"], kernel_pairs=tm["kernel_pairs"], profiles_per_step=1,
                kernel_ms=float(np.mean(kernel_ms[-args.steps:])),
+               merge_ms=float(np.mean(merge_ms[-args.steps:])), merge_bytes=tm["merge_bytes"],
                items=tm["n_items"],
                extra={"atoms": len(xyz), "waters": len(wxyz), "q_points": len(q), "debye_terms": args.debye_terms,
                       "lattice_pairs_per_step": tm["lattice_pairs"]})
@@ -552,6 +559,13 @@ def main():
                     "traffic": traffic_from_profiles() if args.config == 3 else None,
                     "kernel_ms": round(res["kernel_ms"], 4), "kernel_pairs": kpairs,
                     "pairs_per_s": round(kpairs / kern_s, 1)}
+        if res.get("merge_ms"):
+            # the merge of the pair kernel's partial histograms: HBM-bound
+            gbs = res["merge_bytes"] / (res["merge_ms"] / 1e3) / 1e9
+            roofline["merge"] = {"bound": "hbm", "kernels": "merge_chunks_kernel + merge_rows_kernel",
+                                 "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": res["merge_bytes"],
+                                 "ms": round(res["merge_ms"], 4)}
     elif cfg["kind"] == "serve":
         # pairs the pair kernel evaluated over the window (grid requests count
         # their dummy-dummy pairs through lattice correlation, not here)

@@ -369,6 +369,11 @@ int ausaxs_copy_probe(size_t bytes, int reps, double* ms);
 int ausaxs_structure_set_timing(ausaxs_structure* s, int on);
 int ausaxs_last_timing(const ausaxs_structure* s, double* pair_ms, double* total_ms, long long* pairs,
                        int* n_items, long long* kernel_pairs, long long* lattice_pairs);
+/* The merge of that calculation (merge_chunks_kernel + merge_rows_kernel,
+ * HIP events): its time and its HBM bytes (partial rows read, chunk sums
+ * written and read back, result rows written), for its achieved GB/s (no
+ * counterpart in the reference). */
+int ausaxs_last_merge_timing(const ausaxs_structure* s, double* merge_ms, long long* merge_bytes);
 
 /* ---- bodies with symmetry copies ------------------------------------------ */
 
