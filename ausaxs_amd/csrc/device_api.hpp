@@ -228,30 +228,35 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // Pair histograms of integer-lattice point sets by FFT correlation
 // (lattice.hip). Grid index (i, j, k) holds the point origin + (i, j, k);
-// L: power-of-two lengths (x slowest, z fastest), each in [2, kLatticeMaxL];
-// E: extent of the occupied box (points at indices [0, E) per axis),
-// 2E - 1 <= L so that no displacement wraps.
+// L: line lengths 2^a 3^b 5^c (x slowest, z fastest), each in
+// [2, kLatticeMaxL]; E: extent of the occupied box (points at indices [0, E)
+// per axis), 2E - 1 <= L so that no displacement wraps.
 constexpr int kLatticeMaxL = 1024;
 struct LatticeGeom {
     int origin[3];
     int L[3];
     int E[3];
 };
-hipError_t launch_lattice_zero(double2* buf, size_t n, hipStream_t s);
+// L is a transform length the kernels take
+bool lattice_length_ok(int L);
+// the shortest such length >= 2E - 1 (and >= 2), or -1 beyond kLatticeMaxL
+int lattice_length(int E);
+// clears the occupied box (x < Ex, y < Ey, z < Ez); nothing else needs clearing
+hipError_t launch_lattice_zero_box(double2* buf, const LatticeGeom& g, hipStream_t s);
 hipError_t launch_lattice_zero_u64(unsigned long long* buf, size_t n, hipStream_t s);
 // occupancy += 1 at every point (integer coordinates inside the occupied box)
 hipError_t launch_lattice_fill(const float4* pts, int n, const LatticeGeom& g, double2* buf, hipStream_t s);
-// in-place forward 3D FFT, exp(-2 pi i k x / L), of an occupancy grid;
-// tw[a]: exp(-2 pi i k / L[a]) for k < L[a] / 2. store: 0 the transform,
-// 1 its power spectrum |F|^2, 2 conj(F) * other (other: a finished transform)
-hipError_t launch_fft3_forward(double2* buf, const LatticeGeom& g, const double2* const tw[3], int store,
-                               const double2* other, hipStream_t s);
-// inverse 3D FFT of a (power or cross) spectrum into pair counts per squared
-// displacement, scount[s] (s <= smax), for displacements |D| < E per axis;
-// self: each pair once, n_self points; flag |= 1 on a non-integer count,
-// 2 on s > smax. The buffer is left partly transformed.
-hipError_t launch_fft3_counts(double2* buf, const LatticeGeom& g, const double2* const tw[3], bool self,
-                              long long n_self, unsigned long long* scount, int smax, int* flag, hipStream_t s);
+// in-place forward 3D FFT, exp(-2 pi i k x / L), of a filled occupancy grid;
+// tw[a]: exp(-2 pi i m / L[a]) for m < L[a]. The whole transform is stored.
+hipError_t launch_fft3_forward(double2* buf, const LatticeGeom& g, const double2* const tw[3], hipStream_t s);
+// pair counts per squared displacement, scount[s] += (s <= smax), of a
+// filled occupancy grid: with itself (other = null; each pair once, n_self
+// points) or with a set whose finished transform is `other` (cross
+// correlation sum_v a(v) b(v + D)), for displacements |D| < E per axis.
+// flag |= 1 on a non-integer count, 2 on s > smax. buf is left partly
+// transformed.
+hipError_t launch_fft3_correlate(double2* buf, const LatticeGeom& g, const double2* const tw[3], const double2* other,
+                                 long long n_self, unsigned long long* scount, int smax, int* flag, hipStream_t s);
 // scount -> counts and delta sums of one histogram row (the pair kernel's
 // binning), added to cnt / dsum [nbins + 1] with atomics
 hipError_t launch_lattice_bins(const unsigned long long* scount, int smax, float inv_w, int nbins, uint64_t* cnt,

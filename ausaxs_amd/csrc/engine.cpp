@@ -396,20 +396,21 @@ bool plan_lattice(const PairJob& job, const std::vector<PointSet>& sets, int mod
     int64_t cells = 1, s_true = 0;
     for (int k = 0; k < 3; ++k) {
         const int64_t E = (int64_t)hi[k] - lo[k] + 1;
-        int64_t L = 2;
-        while (L < 2 * E - 1) L *= 2;
-        if (L > kLatticeMaxL) return false;
+        const int L = E <= kLatticeMaxL ? lattice_length((int)E) : -1;
+        if (L < 0) return false;
         j.g.origin[k] = lo[k];
-        j.g.L[k] = (int)L;
+        j.g.L[k] = L;
         j.g.E[k] = (int)E;
         cells *= L;
         s_true += (E - 1) * (E - 1);
-        j.smax += (int)((L / 2) * (L / 2));
     }
+    // counts per squared displacement, |D| < E per axis
+    j.smax = (int)std::min<int64_t>(s_true, INT32_MAX);
     // d2 = s exact in float: every squared displacement below 2^24
     if (s_true >= (int64_t(1) << 24) || cells > (int64_t(1) << 28)) return false;
-    // a transform costs about as much as 50 pair evaluations per cell
-    if (mode == 1 && j.pairs < 64 * cells * (self ? 1 : 2)) return false;
+    // automatic choice: a transform costs about as much as 50 pair
+    // evaluations per cell, and its ~10 launches as 1e8 pairs of the pair kernel
+    if (mode == 1 && (j.pairs < 64 * cells * (self ? 1 : 2) || j.pairs < 100000000)) return false;
     out = j;
     return true;
 }
@@ -689,7 +690,7 @@ void PairEngine::build(int tile_j) {
     d_cnt_.ensure((size_t)n_rows_ * (nbins_ + 1));
     d_dsum_.ensure((size_t)n_rows_ * (nbins_ + 1));
     if (!lat_.empty()) {
-        // twiddle tables exp(-2 pi i k / L), k < L / 2, per job and axis
+        // twiddle tables exp(-2 pi i m / L), m < L, per job and axis
         std::vector<double2> tw;
         size_t cells = 0, cross = 0;
         int smax = 0;
@@ -697,7 +698,7 @@ void PairEngine::build(int tile_j) {
             for (int a = 0; a < 3; ++a) {
                 j.tw_off[a] = (int)tw.size();
                 const int L = j.g.L[a];
-                for (int k = 0; k < std::max(1, L / 2); ++k) {
+                for (int k = 0; k < L; ++k) {
                     const double ang = -2.0 * 3.14159265358979323846 * (double)k / (double)L;
                     tw.push_back(make_double2(std::cos(ang), std::sin(ang)));
                 }
@@ -733,21 +734,19 @@ void PairEngine::enqueue(bool timed) {
     if (!lat_.empty())
         check(launch_lattice_zero_u64((unsigned long long*)d_lat_flag_.p, 1, s), "lattice_zero_u64_kernel");
     for (const LatticeJob& j : lat_) {
-        const size_t cells = (size_t)j.g.L[0] * j.g.L[1] * j.g.L[2];
         const double2* tw[3] = {d_tw_.p + j.tw_off[0], d_tw_.p + j.tw_off[1], d_tw_.p + j.tw_off[2]};
         const bool self = j.col_set < 0;
         if (!self) {
-            // the column set's transform first: the row set's last pass multiplies by it
-            check(launch_lattice_zero(d_lat_b_.p, cells, s), "lattice_zero_kernel");
+            // the column set's transform first: the row set's x pass multiplies by it
+            check(launch_lattice_zero_box(d_lat_b_.p, j.g, s), "lattice_zero_box_kernel");
             check(launch_lattice_fill(d_orig_[j.col_set].p, j.n_cols, j.g, d_lat_b_.p, s), "lattice_fill_kernel");
-            check(launch_fft3_forward(d_lat_b_.p, j.g, tw, 0, nullptr, s), "fft_lines_kernel");
+            check(launch_fft3_forward(d_lat_b_.p, j.g, tw, s), "fft_lines_kernel");
         }
-        check(launch_lattice_zero(d_lat_a_.p, cells, s), "lattice_zero_kernel");
+        check(launch_lattice_zero_box(d_lat_a_.p, j.g, s), "lattice_zero_box_kernel");
         check(launch_lattice_fill(d_orig_[j.row_set].p, j.n_rows, j.g, d_lat_a_.p, s), "lattice_fill_kernel");
-        check(launch_fft3_forward(d_lat_a_.p, j.g, tw, self ? 1 : 2, self ? nullptr : d_lat_b_.p, s),
-              "fft_lines_kernel");
         check(launch_lattice_zero_u64(d_scount_.p, (size_t)j.smax + 1, s), "lattice_zero_u64_kernel");
-        check(launch_fft3_counts(d_lat_a_.p, j.g, tw, self, self ? j.n_rows : 0, d_scount_.p, j.smax, d_lat_flag_.p, s),
+        check(launch_fft3_correlate(d_lat_a_.p, j.g, tw, self ? nullptr : d_lat_b_.p, self ? j.n_rows : 0, d_scount_.p,
+                                    j.smax, d_lat_flag_.p, s),
               "fft_lines_kernel");
         const size_t row = (size_t)j.out_row * (nbins_ + 1);
         check(launch_lattice_bins(d_scount_.p, j.smax, inv_w_, nbins_, d_cnt_.p + row, d_dsum_.p + row, s),

@@ -111,10 +111,10 @@ size_t Server::footprint(Mode mode, const double* atoms_xyz, int n, const double
         double cells = 1, smax = 0;
         for (int k = 0; k < 3; ++k) {
             const double E = std::ceil(ahi[k] - alo[k]) + 2 * 3 + 1;
-            double L = 1;
-            while (L < 2 * E - 1) L *= 2;
-            cells *= L;
-            smax += (L / 2) * (L / 2);
+            const int L = E <= kLatticeMaxL ? lattice_length((int)E) : -1;
+            const double Lk = L > 0 ? L : 2 * E;  // beyond the longest line the pair kernel takes the job
+            cells *= Lk;
+            smax += (E - 1) * (E - 1);
         }
         bytes += cells * 16 + smax * 8;
     }

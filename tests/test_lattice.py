@@ -109,3 +109,17 @@ def test_lattice_path_on_small_random_structures(seed):
         h2, t2 = run(cls, xyz, ff, w, 2, bin_width=bw)
         h0, _ = run(cls, xyz, ff, w, 0, bin_width=bw)
         same(h2, h0, dists)
+
+
+@pytest.mark.parametrize("n", [40, 150, 400, 900, 1800])
+def test_lattice_transform_lengths(n):
+    # growing globules: occupied boxes whose padded lengths (the shortest
+    # 2^a 3^b 5^c >= 2E - 1 per axis) take every radix mix of the transform,
+    # odd lengths included; one set (autocorrelation) and interior x surface
+    # (cross-correlation)
+    xyz, ff, w = structure(300 + n, n, max(1, n // 4))
+    for cls, dists in ((A.GridStructure, GRID_DISTS), (A.GridSurfaceStructure, SURFACE_DISTS)):
+        h2, t2 = run(cls, xyz, ff, w, 2)
+        h0, _ = run(cls, xyz, ff, w, 0)
+        assert t2["lattice_pairs"] > 0
+        same(h2, h0, dists)
