@@ -55,7 +55,18 @@ int bin_bound(const std::vector<const PointSet*>& sets, float inv_w) {
     if (!any) return 1;
     double d2 = 0;
     for (int k = 0; k < 3; ++k) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
-    const double x = std::sqrt(d2) * (double)inv_w * (1 + 1e-5) + 2;
+    // no pair is farther apart than twice the largest distance from the box
+    // centre (|p - q| <= |p - c| + |q - c|): for a round structure that is
+    // close to its diameter, while the box diagonal can exceed it by 40 %
+    const double c[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+    double r2 = 0;
+    for (auto* s : sets)
+        for (const auto& p : s->orig) {
+            const double dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
+            r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+        }
+    const double span = std::min(std::sqrt(d2), 2 * std::sqrt(r2));
+    const double x = span * (double)inv_w * (1 + 1e-5) + 2;
     if (!(x < 1e9)) throw std::runtime_error("ausaxs_hip: coordinates are not finite or too far apart");
     // the pair kernel's offset codes hold the Newton correction of pairs up
     // to kMaxExtentBins bins apart
