@@ -112,6 +112,18 @@ struct Rows {
     uint32_t base[kRowsPerLane];  // LDS byte address of the row's cell 0, minus 8 * kMagicBits (mod 2^32)
 };
 
+// The launch constants with m2 held in a vector register for the whole item.
+// The offset code's FMA fma(dl, q, m2) takes q from a scalar register, and a
+// VALU instruction reads at most one, so m2 must be a VGPR; left to itself the
+// compiler copies it from its scalar register again for every column (half
+// an instruction per pair). The empty asm makes the copy opaque, so it is
+// made once.
+__device__ __forceinline__ Consts vgpr_m2(const Consts& cs) {
+    Consts cv = cs;
+    asm volatile("" : "+v"(cv.m2));
+    return cv;
+}
+
 // Bins and adds the kRowsPerLane pairs of one column whose squared distances
 // (scaled) are d2: fast path, tie fallback, LDS add.
 // Oriented items hold the column's original index in tag_j as a float (the
@@ -210,6 +222,7 @@ __device__ __forceinline__ void run_item_runs(const float4* __restrict__ pts, co
     Rows r;
     load_rows<FULL, SPLIT, false>(pts, w, hist, cs, nbins, r);
     const unsigned ovf = (unsigned)nbins;
+    const Consts cv = vgpr_m2(cs);
     for (int rb = w.j_begin; rb < w.j_end; rb += kColTile) {
         const int n = min(kColTile, w.j_end - rb);
         __syncthreads();  // previous descriptors consumed
@@ -239,7 +252,7 @@ __device__ __forceinline__ void run_item_runs(const float4* __restrict__ pts, co
                     const float dz = r.z[k] - z;
                     d2[k] = sxy[k] + dz * dz;
                 }
-                add_pairs<FULL, false, false, CLAMP, false>(r, d2, 0.f, 0, cs, ovf, 0, 1.0);
+                add_pairs<FULL, false, false, CLAMP, false>(r, d2, 0.f, 0, cv, ovf, 0, 1.0);
                 z += cs.scale;
             }
         }
@@ -261,6 +274,7 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
     load_rows<FULL, SPLIT, WEIGHT>(pts, w, hist, cs, nbins, r);
     const int off1 = ORIENT ? nbins + 2 : 0;  // row stride: slot nbins overflow, nbins+1 masked lanes
     const unsigned ovf = (unsigned)nbins;
+    const Consts cv = vgpr_m2(cs);
 
     // the next column tile is loaded into a register while the current one is
     // evaluated, so the global-memory latency of the tile loads stays hidden
@@ -296,17 +310,17 @@ __device__ __forceinline__ void run_item(const float4* __restrict__ pts, const P
             for (int u = 0; u < kColGroup; ++u) bq[u] = cc[c + kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, a[u], jh + c + u, hist, cs, ovf, off1, wscale);
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, a[u], jh + c + u, hist, cv, ovf, off1, wscale);
             // may read up to kColGroup slots past the tile: cols is padded
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u) a[u] = cc[c + 2 * kColGroup + u];
 #pragma unroll
             for (int u = 0; u < kColGroup; ++u)
-                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, bq[u], jh + c + kColGroup + u, hist, cs, ovf, off1,
+                pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, bq[u], jh + c + kColGroup + u, hist, cv, ovf, off1,
                                                                      wscale);
         }
         for (int c = n2; c < n; ++c)
-            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, cc[c], jh + c, hist, cs, ovf, off1, wscale);
+            pair_column<FULL, DIAG, ORIENT, CLAMP, SPLIT, WEIGHT>(r, cc[c], jh + c, hist, cv, ovf, off1, wscale);
     }
 }
 
