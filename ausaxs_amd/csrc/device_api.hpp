@@ -222,6 +222,18 @@ inline int64_t strip_table_blocks(const StripLaunch& p) {
 hipError_t launch_bin_check(uint32_t lo_bits, uint32_t hi_bits, float inv_w, unsigned long long* out,
                             hipStream_t s);
 
+// Copies device buffers (32-bit words), back to back, into dst: pinned host
+// memory mapped for the device (its device address), so small results reach
+// the host through the kernel's own stores, without a DMA transfer.
+constexpr int kCopyOutMax = 6;
+struct CopyOut {
+    const uint32_t* src[kCopyOutMax];
+    size_t words[kCopyOutMax];
+    int n = 0;
+    uint32_t* dst = nullptr;
+};
+hipError_t launch_copy_out(const CopyOut& c, hipStream_t s);
+
 // Plain copy of `bytes` (a multiple of 16) from src to dst: the reference
 // stream for calibrating HBM traffic counters.
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

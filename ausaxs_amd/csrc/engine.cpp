@@ -796,31 +796,42 @@ void PairEngine::enqueue(bool timed) {
 void PairEngine::download(std::vector<uint64_t>& cnt, std::vector<int64_t>& dsum, WeightSums* wsum) {
     Context& ctx = Context::get();
     const size_t n = (size_t)n_rows_ * (nbins_ + 1);
+    if (wsum && !weighted_) throw std::runtime_error("ausaxs_hip: weight sums of an engine without weights");
+    // counts, delta sums (weight sums, lattice flag) gathered by one kernel
+    // into a per-thread pinned buffer, then one wait
+    CopyOut c{};
+    auto add = [&](const void* src, size_t bytes) {
+        c.src[c.n] = static_cast<const uint32_t*>(src);
+        c.words[c.n++] = bytes / sizeof(uint32_t);
+    };
+    add(d_cnt_.p, n * sizeof(uint64_t));
+    add(d_dsum_.p, n * sizeof(int64_t));
+    if (wsum) {
+        add(d_wsum_.p, n * sizeof(int64_t));
+        add(d_wstrip_.p, n * sizeof(int64_t));
+    }
+    if (!lat_.empty()) add(d_lat_flag_.p, sizeof(int));
+    size_t bytes = 0;
+    for (int b = 0; b < c.n; ++b) bytes += c.words[b] * sizeof(uint32_t);
+    thread_local MappedStage stage;
+    const char* h = stage.ensure(bytes);
+    c.dst = static_cast<uint32_t*>(stage.dev);
+    check(launch_copy_out(c, ctx.stream()), "copy_out_kernel");
+    ctx.sync();
     cnt.resize(n);
     dsum.resize(n);
-    // counts and delta sums in one per-thread host buffer, grown as needed
-    // (pageable: see CompositeDistanceHistogramHIP::sinqd), then one wait
-    thread_local std::vector<uint64_t> stage;
-    if (stage.size() < 2 * n) stage.resize(2 * n);
-    uint64_t* const h_stage = stage.data();
-    check(hipMemcpyAsync(h_stage, d_cnt_.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
-    check(hipMemcpyAsync(h_stage + n, d_dsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
+    std::memcpy(cnt.data(), h, n * sizeof(uint64_t));
+    std::memcpy(dsum.data(), h + n * sizeof(uint64_t), n * sizeof(int64_t));
+    size_t off = 2 * n * sizeof(uint64_t);
     if (wsum) {
-        if (!weighted_) throw std::runtime_error("ausaxs_hip: weight sums of an engine without weights");
         wsum->items.resize(n);
         wsum->strips.resize(n);
-        check(hipMemcpyAsync(wsum->items.data(), d_wsum_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx.stream()),
-              "D2H");
-        check(hipMemcpyAsync(wsum->strips.data(), d_wstrip_.p, n * sizeof(int64_t), hipMemcpyDeviceToHost,
-                             ctx.stream()),
-              "D2H");
+        std::memcpy(wsum->items.data(), h + off, n * sizeof(int64_t));
+        std::memcpy(wsum->strips.data(), h + off + n * sizeof(int64_t), n * sizeof(int64_t));
+        off += 2 * n * sizeof(int64_t);
     }
     int flag = 0;
-    if (!lat_.empty())
-        check(hipMemcpyAsync(&flag, d_lat_flag_.p, sizeof(int), hipMemcpyDeviceToHost, ctx.stream()), "D2H");
-    ctx.sync();
-    std::memcpy(cnt.data(), h_stage, n * sizeof(uint64_t));
-    std::memcpy(dsum.data(), h_stage + n, n * sizeof(int64_t));
+    if (!lat_.empty()) std::memcpy(&flag, h + off, sizeof(int));
     if (flag)
         throw std::runtime_error("ausaxs_hip: lattice correlation left a count off an integer (flag " +
                                  std::to_string(flag) + ")");

@@ -5,6 +5,7 @@
 #include "device_api.hpp"
 #include "pool.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <cstring>
@@ -88,6 +89,22 @@ struct HostBuf {
         n = 0;
         p = static_cast<T*>(pool_alloc_host(std::max<size_t>(count, 1) * sizeof(T)));
         n = std::max<size_t>(count, 1);
+    }
+};
+
+// A per-thread pinned host buffer that kernels read and write directly
+// through its device address (small operands and results cross the bus in
+// the kernels' own loads and stores: no DMA transfer to queue behind other
+// streams' copies). Grows by half again when too small.
+struct MappedStage {
+    HostBuf<char> buf;
+    void* dev = nullptr;
+    char* ensure(size_t bytes) {
+        if (bytes > buf.n || !buf.p) {
+            buf.ensure(std::max(bytes, buf.n + buf.n / 2));
+            check(hipHostGetDevicePointer(&dev, buf.p, 0), "hipHostGetDevicePointer");
+        }
+        return buf.p;
     }
 };
 

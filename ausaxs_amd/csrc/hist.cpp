@@ -855,23 +855,18 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
     Context& ctx = Context::get();
     hipStream_t st = ctx.stream();
     // Per-thread buffers and events, reused across calls (serving slots run
-    // concurrently, each on its own thread and stream). All operands go up in
-    // one copy ([q | per group: H rows, d axis]), the groups' contractions run
-    // back to back on the stream (one scratch area, used in stream order), and
-    // the result comes back in one copy. The host side is pageable memory:
-    // pinned staging made single requests faster but raised the median
-    // latency of concurrent serving requests (2.7 -> 4.0 ms on the serving
-    // benchmark), whose small asynchronous copies then queue on the DMA engines.
+    // concurrently, each on its own thread and stream). The operands ([q |
+    // per group: H rows, d axis]) are written into a per-thread pinned
+    // buffer, which one kernel copies to the device; the groups'
+    // contractions run back to back on the stream (one scratch area, used in
+    // stream order) and write their results straight into the pinned buffer;
+    // then one wait. No DMA transfer is involved: asynchronous copies from
+    // many serving threads queue on the DMA engines (pinned hipMemcpyAsync
+    // raised the median serving latency from 2.7 to 4.0 ms), and pageable
+    // ones cost tens of microseconds of host time each.
     thread_local DevBuf<double> dev;
     thread_local DevBuf<char> scratch;
-    thread_local std::vector<double> pin_v;
-    struct {
-        double* p = nullptr;
-        void ensure(std::vector<double>& v, size_t n) {
-            if (v.size() < n) v.resize(n);
-            p = v.data();
-        }
-    } pin;
+    thread_local MappedStage stage;
     thread_local hipEvent_t ev[2] = {nullptr, nullptr};
     for (auto& e : ev)
         if (!e) check(hipEventCreate(&e), "hipEventCreate");
@@ -884,24 +879,30 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
         scratch_bytes = std::max(scratch_bytes, debye_scratch_bytes(ra, K, nq));
     }
     const size_t n_out = (size_t)rows * nq;
-    dev.ensure(n_in + n_out);
-    pin.ensure(pin_v, std::max(n_in, n_out));
+    dev.ensure(n_in);
+    double* const pin = reinterpret_cast<double*>(stage.ensure((n_in + n_out) * sizeof(double)));
+    double* const pin_dev = static_cast<double*>(stage.dev);
     scratch.ensure(scratch_bytes);
-    std::memcpy(pin.p, q.data(), (size_t)nq * sizeof(double));
+    std::memcpy(pin, q.data(), (size_t)nq * sizeof(double));
     size_t off = nq;
     size_t h_off[3] = {0, 0, 0}, d_off[3] = {0, 0, 0};
     for (int a = 0; a < 3; ++a) {
         if (keys[a].empty()) continue;
         h_off[a] = off;
-        std::memcpy(pin.p + off, H[a].data(), H[a].size() * sizeof(double));
+        std::memcpy(pin + off, H[a].data(), H[a].size() * sizeof(double));
         off += H[a].size();
         d_off[a] = off;
-        std::memcpy(pin.p + off, axes[a]->data(), (size_t)K * sizeof(double));
+        std::memcpy(pin + off, axes[a]->data(), (size_t)K * sizeof(double));
         off += K;
     }
-    check(hipMemcpyAsync(dev.p, pin.p, n_in * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    CopyOut up{};
+    up.src[0] = reinterpret_cast<const uint32_t*>(pin_dev);
+    up.words[0] = n_in * 2;
+    up.n = 1;
+    up.dst = reinterpret_cast<uint32_t*>(dev.p);
     check(hipEventRecord(ev[0], st), "hipEventRecord");
-    double* out = dev.p + n_in;
+    check(launch_copy_out(up, st), "copy_out_kernel");
+    double* out = pin_dev + n_in;
     size_t row0 = 0;
     for (int a = 0; a < 3; ++a) {
         const int ra = (int)keys[a].size();
@@ -911,10 +912,8 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
         row0 += ra;
     }
     check(hipEventRecord(ev[1], st), "hipEventRecord");
-    // the pinned buffer's inputs were consumed by the copy above (stream order)
-    check(hipMemcpyAsync(pin.p, out, n_out * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
     ctx.sync();
-    std::memcpy(cached_sinqd_.data(), pin.p, n_out * sizeof(double));
+    std::memcpy(cached_sinqd_.data(), pin + n_in, n_out * sizeof(double));
     float ms = 0;
     check(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
     last_debye_ms = ms;

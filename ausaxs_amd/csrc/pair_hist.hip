@@ -664,7 +664,30 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
         dst[e] = src[e];
 }
 
+// Gathers up to kCopyOutMax device buffers, back to back, into one buffer
+// (pinned host memory: the kernel's stores cross the bus, no DMA engine)
+__global__ __launch_bounds__(256) void copy_out_kernel(CopyOut c) {
+    size_t base = 0;
+    for (int b = 0; b < c.n; ++b) {
+        const uint32_t* __restrict__ src = c.src[b];
+        const size_t n = c.words[b];
+        for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+            c.dst[base + e] = src[e];
+        base += n;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_copy_out(const CopyOut& c, hipStream_t s) {
+    if (c.n < 0 || c.n > kCopyOutMax || !c.dst) return hipErrorInvalidValue;
+    size_t total = 0;
+    for (int b = 0; b < c.n; ++b) total += c.words[b];
+    if (!total) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 256);
+    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / sizeof(uint4);
