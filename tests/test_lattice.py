@@ -123,3 +123,21 @@ def test_lattice_transform_lengths(n):
         h0, _ = run(cls, xyz, ff, w, 0)
         assert t2["lattice_pairs"] > 0
         same(h2, h0, dists)
+
+
+@pytest.mark.parametrize("sep", [90.0, 240.0, 430.0])
+def test_lattice_long_lines(sep):
+    # two small globules far apart on x (and a third off axis): occupied boxes
+    # hundreds of lattice steps long on one axis, so the x lines run to
+    # lengths of several hundred (few lines per workgroup, long radix-3 and
+    # radix-5 stage chains) while the other axes stay short
+    xyz, ff, w = structure(500 + int(sep), 150, 40)
+    shift = np.array([[sep, 0.0, 0.0]])
+    third = np.array([[sep / 3, 25.0, -10.0]])
+    xyz = np.concatenate([xyz, xyz + shift, xyz[:50] + third])
+    ff = np.concatenate([ff, ff, ff[:50]])
+    w = np.concatenate([w, w + shift])
+    h2, t2 = run(A.GridStructure, xyz, ff, w, 2)
+    h0, _ = run(A.GridStructure, xyz, ff, w, 0)
+    assert t2["lattice_pairs"] > 0
+    same(h2, h0, GRID_DISTS)
