@@ -83,10 +83,14 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
 // chunks of one tile are summed by debye_reduce_kernel in a fixed order, so
 // enough waves run to fill the chip while the result stays deterministic.
 // Lane l feeds row/column l%16 and K sub-block l/16 (8 consecutive bf16);
-// the accumulator holds D[4*(l/16)+i][l%16].
+// the accumulator holds D[4*(l/16)+i][l%16]. The term count is a template
+// parameter: with it known at compile time the operand registers are indexed
+// statically, and a step's loads are all in flight at once (the next step's
+// are issued before this step's MFMAs).
+template <int TERMS>
 __global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
                                                         int nq, int rows, int nq_pad, int rows_pad, int K_pad,
-                                                        int terms, int k_steps, double* __restrict__ part) {
+                                                        int k_steps, double* __restrict__ part) {
     const int l = threadIdx.x;
     const int q0 = blockIdx.x * 16, r0 = blockIdx.y * 16;
     const int m = l & 15, kg = l >> 4;
@@ -95,26 +99,37 @@ __global__ __launch_bounds__(64) void debye_mfma_kernel(const uint16_t* __restri
     const uint16_t* brow = B + (size_t)(r0 + m) * K_pad + kg * 8;
     double acc64[4] = {0, 0, 0, 0};
     const int kb = blockIdx.z * k_steps * 32, ke = min(K_pad, kb + k_steps * 32);
+    bf16x8 a[TERMS], b[TERMS];
+#pragma unroll
+    for (int t = 0; t < TERMS; ++t) {
+        a[t] = load8(arow + t * pa + kb);
+        b[t] = load8(brow + t * pb + kb);
+    }
     for (int k0 = kb; k0 < ke; k0 += 32) {
-        bf16x8 a[kMaxTerms], b[kMaxTerms];
-        for (int t = 0; t < terms; ++t) {
-            a[t] = load8(arow + t * pa + k0);
-            b[t] = load8(brow + t * pb + k0);
+        // the next step's operands (the last step reloads its own: in bounds, unused)
+        const int kn = k0 + 32 < ke ? k0 + 32 : k0;
+        bf16x8 na[TERMS], nb[TERMS];
+#pragma unroll
+        for (int t = 0; t < TERMS; ++t) {
+            na[t] = load8(arow + t * pa + kn);
+            nb[t] = load8(brow + t * pb + kn);
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
         // smallest products first, largest last
-        if (terms >= 3) {
+        if constexpr (TERMS >= 3) {
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
         }
-        if (terms >= 2) {
+        if constexpr (TERMS >= 2) {
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc64[i] += (double)acc[i];
+#pragma unroll
+        for (int t = 0; t < TERMS; ++t) a[t] = na[t], b[t] = nb[t];
     }
     double* dst = part + (size_t)blockIdx.z * rows_pad * nq_pad;
 #pragma unroll
@@ -220,8 +235,16 @@ hipError_t launch_debye(const DebyeLaunch& p, hipStream_t s) {
                        nq_pad, K_pad, p.terms, A);
     hipLaunchKernelGGL(h_split_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, p.H, p.rows, p.K, rows_pad,
                        K_pad, p.terms, B);
-    hipLaunchKernelGGL(debye_mfma_kernel, dim3(nq_pad / 16, rows_pad / 16, used), dim3(64), 0, s, A, B, p.nq, p.rows,
-                       nq_pad, rows_pad, K_pad, p.terms, per, part);
+    const dim3 grid(nq_pad / 16, rows_pad / 16, used);
+    if (p.terms == 3)
+        hipLaunchKernelGGL(debye_mfma_kernel<3>, grid, dim3(64), 0, s, A, B, p.nq, p.rows, nq_pad, rows_pad, K_pad,
+                           per, part);
+    else if (p.terms == 2)
+        hipLaunchKernelGGL(debye_mfma_kernel<2>, grid, dim3(64), 0, s, A, B, p.nq, p.rows, nq_pad, rows_pad, K_pad,
+                           per, part);
+    else
+        hipLaunchKernelGGL(debye_mfma_kernel<1>, grid, dim3(64), 0, s, A, B, p.nq, p.rows, nq_pad, rows_pad, K_pad,
+                           per, part);
     const int n = p.rows * p.nq;
     hipLaunchKernelGGL(debye_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, used, p.rows, p.nq,
                        rows_pad, nq_pad, p.out);
