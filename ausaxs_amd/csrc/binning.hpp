@@ -38,15 +38,15 @@ __device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
 // -(t - kMagic)) is the offset from the bin centre in bins, rounded once.
 constexpr float kMagic = 12582912.0f;  // 1.5 * 2^23
 constexpr uint32_t kMagicBits = 0x4B400000u;
-// Tie window slope per bin unit: 1.75 * 2^-22 (see bin_fast)
-constexpr float kTieSlope = -4.172325134277344e-7f;
+// Tie window per bin unit: 1.75 * 2^-22 (see bin_fast)
+constexpr float kTieWin = 4.172325134277344e-7f;
 
 // Constants of one launch: binning and offsets of a histogram with
 // inv_w = 1 / bin width, in the pair kernel's scaled coordinates.
 struct Consts {
     float scale;     // 2^(E-1): coordinate scale of the pair loop
     float inv_ws;    // inv_w / scale (exact): bins per scaled unit
-    float slope_ws;  // kTieSlope * inv_ws
+    float win;       // kTieWin * inv_ws: tie window per scaled unit of root
     float q;         // offset units per bin: 2^E / inv_w
     float m2;        // 1.5 * 2^23 + kDeltaBias: offset code of a zero offset
 };
@@ -56,7 +56,7 @@ __host__ __device__ inline Consts make_consts(float inv_w) {
     Consts c;
     c.scale = ldexpf(1.0f, E - 1);
     c.inv_ws = ldexpf(inv_w, 1 - E);
-    c.slope_ws = kTieSlope * c.inv_ws;
+    c.win = kTieWin * c.inv_ws;
     c.q = (float)(ldexp(1.0, E) / (double)inv_w);
     c.m2 = 12582912.0f + (float)kDeltaBias;
     return c;
@@ -79,14 +79,18 @@ __device__ __forceinline__ float rsq_clamped(float D) {
 // sqrt and rsq both). The reference's x is the correctly rounded root times
 // inv_w, rounded; the exact product s * inv_ws lies within 1.25 * 2^-22 * x
 // of it, so both round to the same bin unless x is that close to a .5 tie.
-// Returns true inside a window of 1.75 * 2^-22 * x around a tie (the margin
-// covers the rounding of dl and of the threshold); there the caller redoes
-// the bin with bin_tie.
+// Returns true inside a window of 1.75 * 2^-22 * x around a tie, tested as
+// |dl| + 1.75 * 2^-22 * x > 0.5 with one rounding (the margin covers it and
+// the rounding of dl); there the caller redoes the bin with bin_tie. The test
+// is an FMA with the absolute-value modifier and one compare against an
+// inline constant: with every operand in a vector register the FMA issues at
+// the full rate of the FP32 units, which an operand in a scalar register
+// halves on gfx950.
 __device__ __forceinline__ bool bin_fast(float D, const Consts& c, float& s, float& t, float& dl) {
     s = D * rsq_clamped(D);
     t = __builtin_fmaf(s, c.inv_ws, kMagic);
     dl = __builtin_fmaf(s, c.inv_ws, kMagic - t);
-    return fabsf(dl) > __builtin_fmaf(s, c.slope_ws, 0.5f);
+    return __builtin_fmaf(s, c.win, fabsf(dl)) > 0.5f;
 }
 
 // Offset code of a pair from its root s and offset dl (bins): the bit
