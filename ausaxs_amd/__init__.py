@@ -102,6 +102,7 @@ def _load():
         "ausaxs_bin_check": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, vp]),
         "ausaxs_structure_set_timing": (C.c_int, [vp, C.c_int]),
         "ausaxs_copy_probe": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
+        "ausaxs_issue_probe": (C.c_int, [C.c_int, C.c_int, P(C.c_double)]),
         "ausaxs_rigid_create": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, C.c_int, vp, P(vp)]),
         "ausaxs_rigid_scan": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]),
         "ausaxs_rigid_scan_hydrated": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_double, vp]),
@@ -356,6 +357,16 @@ def bin_check(lo_bits, hi_bits, bin_width):
     n = hi_bits - lo_bits + 1
     mean = [float(np.int64(np.uint64(v).view(np.int64))) / 65536.0 / n for v in (m[4], m[5])]
     return int(m[0]), int(m[1]), int(m[2]), int(m[3]), mean[0], mean[1]
+
+
+def issue_probe(nbins=4926, reps=5):
+    """The device's issue ceilings for the pair kernel's instruction classes,
+    in wave instructions per CU per ns: FP32 adds with vector-register
+    operands, with a scalar-register operand, v_rsq_f32, and ds_add_u64 at
+    hashed cells of an nbins-cell LDS histogram."""
+    r = (C.c_double * 4)()
+    _check(lib.ausaxs_issue_probe(int(nbins), int(reps), r))
+    return {"fp32_add_vgpr": r[0], "fp32_add_sgpr": r[1], "rsq": r[2], "lds_add_u64": r[3]}
 
 
 def synth_globule(seed, n, axes, r_excl):

@@ -32,7 +32,7 @@ COMMON = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["pair_hist.hip", "lattice.hip", "debye.hip", "rigid.hip", "fit.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "serve.cpp", "symmetry.cpp", "fitter.cpp", "hydrate.cpp",
+SOURCES = ["pair_hist.hip", "lattice.hip", "probe.hip", "debye.hip", "rigid.hip", "fit.hip", "engine.cpp", "hist.cpp", "grid.cpp", "rigid.cpp", "serve.cpp", "symmetry.cpp", "fitter.cpp", "hydrate.cpp",
            "synth.cpp", "pool.cpp", "capi.cpp"]
 
 

@@ -738,6 +738,36 @@ int ausaxs_copy_probe(size_t bytes, int reps, double* ms) {
     });
 }
 
+int ausaxs_issue_probe(int nbins, int reps, double* rates) {
+    return guarded([&] {
+        require(rates != nullptr && reps > 0, "rates is NULL or reps < 1");
+        require(nbins >= 1 && (size_t)nbins * sizeof(uint64_t) <= 64 * 1024, "nbins out of range");
+        Context& ctx = Context::get();
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        DevBuf<float> out;
+        out.ensure(probe_out_floats(cus));
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        for (int which = 0; which < 4; ++which) {
+            double wi = 0;
+            check(launch_issue_probe(which, cus, nbins, out.p, ctx.stream(), &wi), "issue probe");  // warm-up
+            check(hipEventRecord(e0, ctx.stream()), "hipEventRecord");
+            for (int r = 0; r < reps; ++r)
+                check(launch_issue_probe(which, cus, nbins, out.p, ctx.stream(), &wi), "issue probe");
+            check(hipEventRecord(e1, ctx.stream()), "hipEventRecord");
+            ctx.sync();
+            float t = 0;
+            check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+            rates[which] = wi * reps / ((double)t * 1e6) / cus;  // per CU per ns
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int ausaxs_synth_globule(uint64_t seed, int n, double ax, double ay, double az, double r_excl, float* out) {
     return guarded([&] {
         require(out != nullptr || n == 0, "out is NULL");

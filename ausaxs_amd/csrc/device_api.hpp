@@ -238,6 +238,15 @@ hipError_t launch_copy_out(const CopyOut& c, hipStream_t s);
 // stream for calibrating HBM traffic counters.
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
+// Issue-rate probes (probe.hip): which = 0 FP32 adds with vector-register
+// operands, 1 the same with a scalar-register operand, 2 v_rsq_f32, 3
+// ds_add_u64 at hashed cells of an nbins-cell LDS histogram (three 512-lane
+// workgroups per CU, nbins * 8 <= 64 KB). out: probe_out_floats(cus) floats
+// (results kept so the work is not removed). *wave_instr: the wave
+// instructions of the measured class that one launch issues.
+size_t probe_out_floats(int cus);
+hipError_t launch_issue_probe(int which, int cus, int nbins, float* out, hipStream_t s, double* wave_instr);
+
 // Pair histograms of integer-lattice point sets by FFT correlation
 // (lattice.hip). Grid index (i, j, k) holds the point origin + (i, j, k);
 // L: line lengths 2^a 3^b 5^c (x slowest, z fastest), each in

@@ -502,6 +502,22 @@ def timed_steps(step, args):
     return dict(elapsed=elapsed, lat_ms=np.array(lat) * 1e3)
 
 
+def measured_ceilings(A, pairs_per_s, achieved_tflops):
+    """The device's own issue ceilings (ausaxs_issue_probe, run after the timed
+    steps) for the pair kernel's two co-limiting pipes: FP32 VALU issue with
+    vector-register operands, as TFLOP/s at 2 flop per FMA, and ds_add_u64 at
+    hashed cells of a 4,926-cell LDS histogram (two rows of the 250k-atom
+    workload's 2,463), as pair accumulations per second (one atomic per pair)."""
+    cus = A.device_info()["compute_units"]
+    r = A.issue_probe(4926, 5)
+    fp32 = r["fp32_add_vgpr"] * cus * 64 * 2 / 1e3
+    lds = r["lds_add_u64"] * cus * 64 * 1e9
+    return {"probe": "ausaxs_issue_probe (wave instructions per CU per ns)",
+            "rates": {k: round(v, 4) for k, v in r.items()},
+            "fp32_tflops": round(fp32, 2), "frac_of_fp32": round(achieved_tflops / fp32, 4),
+            "lds_atomic_pairs_per_s": round(lds, 1), "frac_of_lds_atomic": round(pairs_per_s / lds, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +575,7 @@ def main():
                     "traffic": traffic_from_profiles() if args.config == 3 else None,
                     "kernel_ms": round(res["kernel_ms"], 4), "kernel_pairs": kpairs,
                     "pairs_per_s": round(kpairs / kern_s, 1)}
+        roofline["measured_ceilings"] = measured_ceilings(A, kpairs / kern_s, achieved)
         if res.get("merge_ms"):
             # the merge of the pair kernel's partial histograms: HBM-bound
             gbs = res["merge_bytes"] / (res["merge_ms"] / 1e3) / 1e9

@@ -363,6 +363,15 @@ int ausaxs_bin_check(uint32_t lo_bits, uint32_t hi_bits, double bin_width, unsig
  * known size (no counterpart in the reference). */
 int ausaxs_copy_probe(size_t bytes, int reps, double* ms);
 
+/* The device's own issue ceilings for the pair kernel's instruction classes
+ * (no counterpart in the reference), each the mean over `reps` launches after
+ * one warm-up, in wave instructions per CU per nanosecond: rates[0] FP32 adds
+ * with vector-register operands, rates[1] the same with a scalar-register
+ * operand, rates[2] v_rsq_f32, rates[3] ds_add_u64 at hashed cells of an
+ * nbins-cell LDS histogram (three 512-lane workgroups per CU; nbins * 8 <=
+ * 65536). rates: 4 entries. */
+int ausaxs_issue_probe(int nbins, int reps, double* rates);
+
 /* HIP-event timings of the last ausaxs_structure_calculate with timing on.
  * pairs: all pairs in the histograms; kernel_pairs: of which pair_hist_kernel
  * evaluated; lattice_pairs: of which lattice correlation counted. */
