@@ -28,11 +28,13 @@ def test_device_is_gfx950():
 
 @pytest.mark.parametrize("bw", [0.25, 0.1, 1.0, 0.05, 0.3, 2.0])
 def test_binning_arithmetic_exhaustive(bw):
-    # every float32 d2 in [0, 6.4e5] (distances up to 800 A): the fast path
-    # (raw v_sqrt_f32 + residual-corrected fallback near ties) bins like
-    # sqrt_rn, the device sqrtf is correctly rounded (verified in FP64), and the
-    # fallback's correction equals it on every normal d2
-    hi = int(np.array(6.4e5, np.float32).view(np.uint32))
+    # every float32 d2 up to the widest histogram the host accepts (2^15 bins,
+    # distances to 32768 bin widths): the fast path (rsq root, window test,
+    # correctly rounded fallback near ties) bins like the reference's
+    # sqrt_rn rule, the device sqrtf is correctly rounded (verified in FP64),
+    # the fallback's correction equals it on every normal d2, and every offset
+    # code is within one unit of the real root's offset
+    hi = int(np.array((32768 * bw) ** 2, np.float32).view(np.uint32))
     bad_bin, bad_cr, bad_fix, bad_off, _, _ = A.bin_check(0, hi, bw)
     assert (bad_bin, bad_cr, bad_fix, bad_off) == (0, 0, 0, 0)
 
