@@ -118,7 +118,7 @@ def _load():
         "ausaxs_server_stats": (C.c_int, [vp, P(C.c_longlong), P(C.c_longlong), P(C.c_longlong), P(C.c_int),
                                           P(C.c_int), P(C.c_double), P(C.c_longlong), P(C.c_longlong)]),
         "ausaxs_server_free": (C.c_int, [vp]),
-        "ausaxs_server_batch_stats": (C.c_int, [vp] + [P(C.c_longlong)] * 5 + [P(C.c_double)]),
+        "ausaxs_server_batch_stats": (C.c_int, [vp] + [P(C.c_longlong)] * 5 + [P(C.c_double), P(C.c_longlong)]),
         "ausaxs_pool_stats": (C.c_int, [P(C.c_size_t), P(C.c_size_t), P(C.c_longlong), P(C.c_longlong),
                                         P(C.c_longlong)]),
         "ausaxs_pool_trim": (C.c_int, []),
@@ -846,12 +846,12 @@ class Server:
         _check(lib.ausaxs_server_stats(self._s, *map(C.byref, v), C.byref(fl), C.byref(pk), C.byref(gb),
                                        C.byref(pairs), C.byref(kp)))
         b = [C.c_longlong() for _ in range(5)]
-        cgb = C.c_double()
-        _check(lib.ausaxs_server_batch_stats(self._s, *map(C.byref, b), C.byref(cgb)))
+        cgb, ex = C.c_double(), C.c_longlong()
+        _check(lib.ausaxs_server_batch_stats(self._s, *map(C.byref, b), C.byref(cgb), C.byref(ex)))
         return dict(submitted=v[0].value, completed=v[1].value, failed=v[2].value, in_flight=fl.value,
                     peak_in_flight=pk.value, peak_gb=gb.value, pairs=pairs.value, kernel_pairs=kp.value,
                     cache_hits=b[0].value, cache_misses=b[1].value, batches=b[2].value, batched=b[3].value,
-                    evictions=b[4].value, cached_gb=cgb.value)
+                    evictions=b[4].value, cached_gb=cgb.value, express=ex.value)
 
 
 def pool_stats():

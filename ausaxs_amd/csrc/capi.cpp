@@ -904,7 +904,7 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
 }
 
 int ausaxs_server_batch_stats(ausaxs_server* s, long long* cache_hits, long long* cache_misses, long long* batches,
-                              long long* batched, long long* evictions, double* cached_gb) {
+                              long long* batched, long long* evictions, double* cached_gb, long long* express) {
     return guarded([&] {
         require(s && s->s, "bad argument");
         const ServeStats st = s->s->stats();
@@ -914,6 +914,7 @@ int ausaxs_server_batch_stats(ausaxs_server* s, long long* cache_hits, long long
         if (batched) *batched = st.batched;
         if (evictions) *evictions = st.evictions;
         if (cached_gb) *cached_gb = (double)st.cached_bytes / (1ull << 30);
+        if (express) *express = st.express;
     });
 }
 

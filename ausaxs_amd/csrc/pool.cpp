@@ -23,8 +23,20 @@ namespace {
 
 struct Block {
     void* p = nullptr;
-    hipEvent_t ev = nullptr;  // recorded at release; null: nothing to wait for
+    hipEvent_t ev = nullptr;       // recorded at release; null: nothing to wait for
+    hipStream_t stream = nullptr;  // the stream ev was recorded on
 };
+
+// the block's last work is done (a query that finds it pending leaves
+// hipErrorNotReady as the thread's last error, which is cleared here so the
+// next kernel launch's error check does not see it)
+bool block_done(const Block& b) {
+    if (!b.ev) return true;
+    const hipError_t e = hipEventQuery(b.ev);
+    if (e == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
+}
 
 struct Arena {
     bool host = false;
@@ -43,29 +55,20 @@ class Pool {
         return *pool;
     }
 
+    // Reuse order: a free block whose last work is done, then one released on
+    // this thread's stream (device memory: stream order alone keeps the new
+    // work behind the old), then a new block; a block whose release is still
+    // pending on another stream is taken (with a wait for that stream's event)
+    // only when HIP has no memory left. Waiting on another stream would put
+    // this request behind whatever that stream had queued: a small serving
+    // request behind a large one's pair kernel.
     void* alloc(Arena& a, size_t bytes) {
         const size_t cls = pool_size_class(bytes);
+        const hipStream_t s = Context::get().stream();
         Block blk;
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            auto it = a.free.find(cls);
-            if (it != a.free.end() && !it->second.empty()) {
-                blk = it->second.back();
-                it->second.pop_back();
-                a.live[blk.p] = cls;
-                a.in_use += cls;
-                ++reuses_;
-            }
-        }
-        if (blk.p) {
-            if (blk.ev) {
-                if (a.host) check(hipEventSynchronize(blk.ev), "hipEventSynchronize");
-                else check(hipStreamWaitEvent(Context::get().stream(), blk.ev, 0), "hipStreamWaitEvent");
-                give_event(blk.ev);
-            }
-            return blk.p;
-        }
+        if (take(a, cls, s, false, blk)) return ready(a, s, blk);
         void* p = raw_alloc(a, cls);
+        if (!p && take(a, cls, s, true, blk)) return ready(a, s, blk);
         if (!p) {
             trim();
             p = raw_alloc(a, cls);
@@ -79,6 +82,38 @@ class Pool {
         a.in_use += cls;
         ++a.mallocs;
         return p;
+    }
+
+    // a free block of class cls: done or released on stream s (any = false),
+    // or any free block (any = true); marks it handed out
+    bool take(Arena& a, size_t cls, hipStream_t s, bool any, Block& blk) {
+        std::lock_guard<std::mutex> lk(m_);
+        auto it = a.free.find(cls);
+        if (it == a.free.end() || it->second.empty()) return false;
+        std::vector<Block>& v = it->second;
+        int pick = -1;
+        if (any) pick = (int)v.size() - 1;
+        for (int i = (int)v.size() - 1; i >= 0 && pick < 0; --i)
+            if (block_done(v[i])) pick = i;
+        for (int i = (int)v.size() - 1; i >= 0 && pick < 0; --i)
+            if (v[i].stream == s) pick = i;
+        if (pick < 0) return false;
+        blk = v[pick];
+        v.erase(v.begin() + pick);
+        a.live[blk.p] = cls;
+        a.in_use += cls;
+        ++reuses_;
+        return true;
+    }
+
+    // makes a reused block safe for new work on stream s
+    void* ready(Arena& a, hipStream_t s, Block& blk) {
+        if (blk.ev) {
+            if (a.host) check(hipEventSynchronize(blk.ev), "hipEventSynchronize");  // the host writes it
+            else if (blk.stream != s) check(hipStreamWaitEvent(s, blk.ev, 0), "hipStreamWaitEvent");
+            give_event(blk.ev);
+        }
+        return blk.p;
     }
 
     void release(Arena& a, void* p) {
@@ -114,7 +149,7 @@ class Pool {
             a.parked.emplace_back(p, cls);
             return;
         }
-        a.free[cls].push_back(Block{p, ev});
+        a.free[cls].push_back(Block{p, ev, ev ? s : nullptr});
     }
 
     void trim() {

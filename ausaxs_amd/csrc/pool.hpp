@@ -4,12 +4,16 @@
 // requests that run side by side on separate streams and finish at different
 // times. Here a freed block goes back to a free list of its size class
 // instead, with an event recorded on the freeing thread's stream (the stream
-// its last work was queued on). An allocation of that class takes the block
-// back: device memory makes the allocating thread's stream wait for the event,
-// so no new work touches the block before the old work is done; pinned host
-// memory, which the host writes directly, waits for the event on the host.
-// Blocks go back to HIP only in pool_trim, or when an allocation fails (the
-// pool trims and tries once more).
+// its last work was queued on). An allocation of that class takes back a block
+// whose event has completed, or else one freed on the allocating thread's own
+// stream (stream order keeps the new work behind the old; pinned host memory,
+// which the host writes directly, waits for the event on the host), and
+// otherwise allocates a new block: a block whose event is still pending on
+// another stream is reused (the allocating stream then waits for that event)
+// only when HIP is out of memory, since the wait would queue this thread's
+// work behind everything the other stream had queued. Blocks go back to HIP
+// only in pool_trim, or when an allocation fails (the pool trims and tries
+// once more).
 //
 // A block freed from another stream than the one its last work ran on is safe
 // only if that work has finished: the library frees a structure's buffers

@@ -160,7 +160,8 @@ int64_t Server::submit(Mode mode, const double* atoms_xyz, const int32_t* ff, in
     requests_[r->id] = r;
     pending_.push_back(r);
     ++st_.submitted;
-    cv_work_.notify_one();
+    // every idle worker looks: the express slot may take what the others cannot
+    cv_work_.notify_all();
     return r->id;
 }
 
@@ -335,16 +336,23 @@ void Server::run(const std::vector<std::shared_ptr<Request>>& batch) {
 
 void Server::worker(int slot) {
     Context::set_thread_stream(streams_[slot]);
+    // the last of two or more slots admits only small requests
+    const bool express = slot > 0 && slot == (int)streams_.size() - 1;
     for (;;) {
         std::vector<std::shared_ptr<Request>> batch;
         {
             std::unique_lock<std::mutex> lk(m_);
             // first come, first admitted: the head waits until its footprint fits
-            // (idle resident structures give way to it)
+            // (idle resident structures give way to it); the express slot takes
+            // the first small request, wherever it is queued
+            auto pick = pending_.end();
             cv_work_.wait(lk, [&] {
                 if (stop_) return true;
-                if (pending_.empty()) return false;
-                const size_t need = pending_.front()->bytes;
+                pick = pending_.begin();
+                if (express)
+                    while (pick != pending_.end() && (*pick)->est_pairs >= kExpressPairs) ++pick;
+                if (pick == pending_.end()) return false;
+                const size_t need = (*pick)->bytes;
                 make_room(need);
                 return st_.in_use + st_.cached_bytes + need <= st_.budget ||
                        (st_.in_use == 0 && st_.in_use + need <= st_.budget);
@@ -358,21 +366,22 @@ void Server::worker(int slot) {
                 cv_done_.notify_all();
                 return;
             }
-            batch.push_back(pending_.front());
-            pending_.pop_front();
+            batch.push_back(*pick);
+            auto next = pending_.erase(pick);
+            if (express) ++st_.express;
             size_t bytes = batch[0]->bytes;
             double pairs = batch[0]->est_pairs;
-            // small requests queued behind a small head join its launch
+            // small requests queued right behind a small first member join its launch
             if (pairs < kSmallPairs)
-                while (!pending_.empty() && (int)batch.size() < kBatchMembers) {
-                    const Request& nx = *pending_.front();
+                while (next != pending_.end() && (int)batch.size() < kBatchMembers) {
+                    const Request& nx = **next;
                     if (nx.est_pairs >= kSmallPairs || !mergeable(*batch[0], nx) || pairs + nx.est_pairs > kBatchPairs ||
                         st_.in_use + st_.cached_bytes + bytes + nx.bytes > st_.budget)
                         break;
                     bytes += nx.bytes;
                     pairs += nx.est_pairs;
-                    batch.push_back(pending_.front());
-                    pending_.pop_front();
+                    batch.push_back(*next);
+                    next = pending_.erase(next);
                 }
             const auto now = clock::now();
             for (auto& r : batch) {
@@ -384,7 +393,7 @@ void Server::worker(int slot) {
             st_.peak_in_flight = std::max(st_.peak_in_flight, st_.in_flight);
         }
         // the next head may fit beside this one
-        cv_work_.notify_one();
+        cv_work_.notify_all();
         try {
             run(batch);
         } catch (const std::exception& e) {

@@ -7,7 +7,12 @@
 // each with its own HIP stream. Requests are admitted in arrival order as soon
 // as a slot is idle and their estimated device footprint fits the memory
 // budget; admitted requests run concurrently, so the kernels of requests that
-// arrived while others were already on the GPU overlap with them.
+// arrived while others were already on the GPU overlap with them. With two or
+// more slots the last one is an express slot: it admits only small requests
+// (below kExpressPairs pair-kernel pairs, about a millisecond of device time),
+// the first one queued even if larger ones are ahead of it, so small requests
+// do not wait for a slot while large ones hold all the others (they share the
+// device with them once admitted).
 //
 // Three things keep the per-request cost down:
 //   * resident structures: a structure (its float points, grid dummies, work
@@ -55,6 +60,7 @@ struct ServeStats {
     int64_t batched = 0;       // requests that ran in a merged launch
     size_t cached_bytes = 0;   // device bytes held by idle resident structures
     int64_t evictions = 0;
+    int64_t express = 0;       // requests admitted by the express slot
 };
 
 class Server {
@@ -65,6 +71,9 @@ class Server {
     // at most this many pairs and members per merged launch
     static constexpr double kBatchPairs = 1.5e9;
     static constexpr int kBatchMembers = 32;
+    // requests below this many estimated pair-kernel pairs may take the
+    // express slot
+    static constexpr double kExpressPairs = 2e9;
     // resident structures kept at most (idle ones beyond are evicted)
     static constexpr size_t kCacheEntries = 256;
 

@@ -57,7 +57,7 @@ CONFIGS = {
             "XYZW weights, 0.25 A bins, default q axis", sizes=(3000, 5000, 8000, 12000), n_conf=512, bw=0.25),
     5: dict(kind="serve", name="serving: Poisson arrivals at 70 % of the measured service rate, structures "
             "log-uniform 1k-100k atoms (+30 % waters), modes uniform over FF-avg/FF-explicit/FF-grid, 200 q in 0-0.5",
-            pool=48, n_min=1000, n_max=100000, load=0.7, slots=4, budget_gb=200.0),
+            pool=48, n_min=1000, n_max=100000, load=0.7, slots=5, budget_gb=200.0),
 }
 I0 = {3: 6.0, 7: 7.0, 11: 8.0, 12: 16.0}  # XYZW weights: electron counts of C, N, O, S
 
@@ -401,7 +401,7 @@ def bench_serve(cfg, args):
     for i in [submit(r) for r in burst]:
         srv.wait(i)
     mu = len(burst) / (time.perf_counter() - t)
-    lam = cfg["load"] * mu
+    lam = args.arrival_rate if args.arrival_rate > 0 else cfg["load"] * mu
     log(f"service rate {mu:.2f} requests/s on a burst of {len(burst)} "
         f"({(srv.stats()['pairs'] - s0) / len(burst):.3g} pairs per request); arrivals at {lam:.2f}/s")
 
@@ -442,7 +442,8 @@ def bench_serve(cfg, args):
     res = dict(elapsed=elapsed, lat_ms=np.array(lat), pairs=pairs / n_req, kernel_pairs=kernel_pairs / n_req,
                profiles_per_step=1, kernel_ms=None, items=0, steps_override=n_req,
                extra={"requests": n_req, "arrival_rate_per_s": round(lam, 3), "service_rate_per_s": round(mu, 3),
-                      "slots": cfg["slots"], "memory_budget_gb": cfg["budget_gb"],
+                      "slots": cfg["slots"], "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                      "memory_budget_gb": cfg["budget_gb"],
                       "peak_in_flight": st["peak_in_flight"], "peak_admitted_gb": round(st["peak_gb"], 2),
                       "queue_ms_p50": round(float(np.percentile(queue, 50)), 3), "q_points": len(q)})
 
@@ -532,6 +533,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of CPU work for the baseline (1 warm-up + 5 timed runs of a sample)")
     ap.add_argument("--requests", type=int, default=400, help="serving: requests in the timed open-loop run")
+    ap.add_argument("--arrival-rate", type=float, default=0.0,
+                    help="serving: Poisson arrivals per second (default: 70 %% of the measured service rate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -550,9 +553,20 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    cfg = CONFIGS[args.config]
+    if cfg["kind"] == "serve":
+        # a hardware queue for every stream with work: the server's slots plus
+        # the library's default stream and HIP's null stream, with room to
+        # spare. Streams beyond the process's queues share one, in order, so a
+        # small request's kernels would wait behind a large request's on the
+        # same queue. HIP reads the setting when it starts: set it before the
+        # library loads.
+        need = cfg["slots"] + 4
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < need:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(need)
+
     import ausaxs_amd as A
 
-    cfg = CONFIGS[args.config]
     log("device:", A.device_info())
     fn = {"structure": bench_structure, "grid": bench_grid, "rigid": bench_rigid, "serve": bench_serve}[cfg["kind"]]
     res, cpu_fn = fn(cfg, args)

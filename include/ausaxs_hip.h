@@ -241,7 +241,10 @@ typedef struct ausaxs_server ausaxs_server;
 /* A request loop with `slots` device slots (worker threads with their own
  * streams, 1..16) and a device-memory budget in GiB. Requests are admitted
  * in arrival order as soon as a slot is idle and their estimated footprint
- * fits the budget, and run concurrently with those already on the GPU.
+ * fits the budget, and run concurrently with those already on the GPU. With
+ * two or more slots the last one is an express slot that admits only small
+ * requests (below 2e9 estimated pair evaluations), the first one queued, so
+ * they do not wait while large requests hold the other slots.
  * Structures stay resident across requests (keyed by their exact content,
  * mode and settings; idle ones are evicted least recently used first when
  * the budget needs the memory), and small requests queued together run as
@@ -276,10 +279,11 @@ int ausaxs_server_stats(ausaxs_server* s, long long* submitted, long long* compl
 
 /* Batching and residency counters: requests served from a resident
  * structure (cache_hits) or built (cache_misses), merged launches (batches)
- * and the requests they ran (batched), evicted structures, and the device
- * memory idle resident structures hold (GiB). No reference counterpart. */
+ * and the requests they ran (batched), evicted structures, the device
+ * memory idle resident structures hold (GiB), and the requests the express
+ * slot admitted. No reference counterpart. Any pointer may be NULL. */
 int ausaxs_server_batch_stats(ausaxs_server* s, long long* cache_hits, long long* cache_misses, long long* batches,
-                              long long* batched, long long* evictions, double* cached_gb);
+                              long long* batched, long long* evictions, double* cached_gb, long long* express);
 
 /* Stops the loop: running requests finish, queued ones fail. */
 int ausaxs_server_free(ausaxs_server* s);

@@ -154,6 +154,30 @@ def test_serve_benchmark_requests():
     assert checked >= 5
 
 
+def test_serve_express_slot():
+    # two slots: one takes requests in arrival order, the other (express) only
+    # small ones. Two large grid requests queued ahead of a small one: the
+    # first large one takes the ordinary slot, the small one the express slot
+    # at once, and the second large one waits for the first; so the small
+    # request is done before the second large one is admitted
+    big1 = make_request(71, 30000, A.MODE_FFGRID)
+    big2 = make_request(72, 30000, A.MODE_FFGRID)
+    small = make_request(73, 1000, A.MODE_FFEXPLICIT)
+    srv = A.Server(slots=2, budget_gb=100.0)
+    for r in (big1, big2, small):  # resident first: the timed round has no host set-up
+        srv.wait(srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]))
+    e0 = srv.stats()["express"]
+    ids = [srv.submit(r["mode"], r["xyz"], r["ff"], r["wxyz"], r["q"]) for r in (big1, big2, small)]
+    (I1, l1, q1), (I2, l2, q2), (Is, ls, qs) = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    print(f"big1 {l1:.2f} ms (queue {q1:.2f}), big2 {l2:.2f} ms (queue {q2:.2f}), small {ls:.2f} ms (queue {qs:.2f}),"
+          f" express admissions {st['express'] - e0}")
+    assert st["express"] - e0 == 1
+    assert ls < q2, (ls, q2)
+    assert np.array_equal(Is, direct(small))
+    assert np.array_equal(I2, direct(big2))
+
+
 def test_serve_errors():
     srv = A.Server(slots=2, budget_gb=1.0)
     r = make_request(5, 100, A.MODE_FFEXPLICIT)
