@@ -199,6 +199,20 @@ struct StripLaunch {
     double wscale = 1;
 };
 hipError_t launch_strip(const StripLaunch& p, hipStream_t s);
+// pairs per workgroup of a strip that accumulates in an LDS histogram first
+constexpr int kStripBlockPairs = 4096;
+// a strip with one output row, no replicas, a histogram that fits 64 KB of
+// LDS and more than kStripBlockPairs pairs accumulates in LDS first (long
+// strips crowd into few bins, where one global atomic per pair serialises)
+inline bool strip_uses_lds(const StripLaunch& p) {
+    const long long n = (long long)p.w.n_rows * (p.w.t_end - p.w.t_begin);
+    return p.fixed_row >= 0 && p.n_rep == 1 && (size_t)(p.nbins + 1) * sizeof(uint64_t) <= 65536 &&
+           n > kStripBlockPairs;
+}
+inline int64_t strip_lds_blocks(const StripLaunch& p) {
+    const int64_t n = (int64_t)p.w.n_rows * (p.w.t_end - p.w.t_begin);
+    return n > 0 ? (n + kStripBlockPairs - 1) / kStripBlockPairs : 0;
+}
 // Strips table[0..n) (device) in one launch, each with n_rep = 1: strip k
 // takes blocks [first[k], first[k+1]) of 256 pairs, first[n] = blocks
 // (device, int64). See strip_table_blocks for the host-side prefix.
@@ -209,6 +223,11 @@ inline int64_t strip_table_blocks(const StripLaunch& p) {
     const int64_t n = (int64_t)p.w.n_rows * (p.w.t_end - p.w.t_begin);
     return n > 0 ? (n + 255) / 256 : 0;
 }
+// LDS strips (strip_uses_lds) table[0..n) in one launch: strip k takes
+// blocks [first[k], first[k+1]) of kStripBlockPairs pairs (strip_lds_blocks),
+// every strip's histogram nbins + 1 cells wide.
+hipError_t launch_strip_lds_table(const StripLaunch* table, const int64_t* first, int n, int64_t blocks, int nbins,
+                                  hipStream_t s);
 
 // Exhaustive check of the binning arithmetic over float d2 in [lo, hi] (bit
 // patterns). out[0]: lanes where the pair kernel's bin (raw v_sqrt + tie

@@ -515,7 +515,7 @@ size_t PairEngine::device_bytes() const {
                d_items_.bytes() + d_row_list_.bytes() + d_chunk_ptr_.bytes() + d_row_chunk_.bytes() +
                d_rowmaps_.bytes() + d_partial_.bytes() + d_cnt_.bytes() + d_tmp_cnt_.bytes() + d_dsum_.bytes() +
                d_tmp_dsum_.bytes() + d_lat_a_.bytes() + d_lat_b_.bytes() + d_tw_.bytes() + d_scount_.bytes() +
-               d_lat_flag_.bytes();
+               d_lat_flag_.bytes() + strips_.bytes() + wstrips_.bytes();
     for (const auto& o : d_orig_) b += o.bytes();
     for (const auto& o : d_orig_w_) b += o.bytes();
     return b;
@@ -725,7 +725,60 @@ void PairEngine::build(int tile_j) {
         d_scount_.ensure((size_t)smax + 1);
         d_lat_flag_.ensure(2);
     }
+    // the strips' launch descriptors (the buffers above stay where they are)
+    std::vector<StripLaunch> st, wst;
+    for (const StripSpec& sp : wl_.strips) {
+        StripLaunch sl{d_orig_[sp.set_rows].p, d_orig_[sp.set_cols].p, sp.w, d_rowmaps_.p + sp.rowmap_off,
+                       d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};
+        if (sets_[sp.set_rows].n_classes == 1 && sets_[sp.set_cols].n_classes == 1)
+            sl.fixed_row = wl_.rowmaps[sp.rowmap_off];
+        st.push_back(sl);
+        if (weighted_) {
+            // the weight pass: weights in the points' fourth slot, one row per job
+            StripLaunch wl{d_orig_w_[sp.set_rows].p, d_orig_w_[sp.set_cols].p, sp.w, d_rowmaps_.p + sp.rowmap_off,
+                           d_wcnt_.p, d_wstrip_.p, inv_w_, nbins_, nbins_ + 1};
+            wl.fixed_row = wl_.rowmaps[sp.rowmap_off];
+            wl.wmode = 1;
+            wl.wscale = wscale_;
+            wst.push_back(wl);
+        }
+    }
+    build_strip_tables(st, strips_, s);
+    build_strip_tables(wst, wstrips_, s);
     ctx.sync();
+}
+
+void PairEngine::build_strip_tables(const std::vector<StripLaunch>& all, StripTables& t, hipStream_t s) {
+    std::vector<StripLaunch> a, l;
+    std::vector<int64_t> fa{0}, fl{0};
+    for (const StripLaunch& p : all) {
+        if (strip_uses_lds(p)) {
+            l.push_back(p);
+            fl.push_back(fl.back() + strip_lds_blocks(p));
+        } else if (const int64_t b = strip_table_blocks(p)) {
+            a.push_back(p);
+            fa.push_back(fa.back() + b);
+        }
+    }
+    if (fa.back() > 0x7fffffff || fl.back() > 0x7fffffff)
+        throw std::runtime_error("ausaxs_hip: strips beyond one launch's grid");
+    t.n_atomic = (int)a.size();
+    t.n_lds = (int)l.size();
+    t.atomic_blocks = fa.back();
+    t.lds_blocks = fl.back();
+    if (!a.empty()) {
+        t.atomic.upload(a.data(), a.size(), s);
+        t.atomic_first.upload(fa.data(), fa.size(), s);
+    }
+    if (!l.empty()) {
+        t.lds.upload(l.data(), l.size(), s);
+        t.lds_first.upload(fl.data(), fl.size(), s);
+    }
+}
+
+void PairEngine::launch_strip_tables(const StripTables& t, hipStream_t s, const char* what) const {
+    check(launch_strip_table(t.atomic.p, t.atomic_first.p, t.n_atomic, t.atomic_blocks, s), what);
+    check(launch_strip_lds_table(t.lds.p, t.lds_first.p, t.n_lds, t.lds_blocks, nbins_, s), what);
 }
 
 void PairEngine::enqueue(bool timed) {
@@ -763,13 +816,7 @@ void PairEngine::enqueue(bool timed) {
         check(launch_lattice_bins(d_scount_.p, j.smax, inv_w_, nbins_, d_cnt_.p + row, d_dsum_.p + row, s),
               "lattice_bins_kernel");
     }
-    for (const StripSpec& st : wl_.strips) {
-        StripLaunch sl{d_orig_[st.set_rows].p, d_orig_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
-                       d_cnt_.p, d_dsum_.p, inv_w_, nbins_, nbins_ + 1};
-        if (sets_[st.set_rows].n_classes == 1 && sets_[st.set_cols].n_classes == 1)
-            sl.fixed_row = wl_.rowmaps[st.rowmap_off];
-        check(launch_strip(sl, s), "strip_kernel");
-    }
+    launch_strip_tables(strips_, s, "strip table kernels");
     if (weighted_) {
         // the weight pass over the same work lists (wmode), its own outputs
         PairLaunch wl{d_pts_w_.p, d_runs_.p, d_items_.p, (int)wl_.items.size(), d_partial_.p, inv_w_, nbins_, clamp_ ? 1 : 0,
@@ -780,14 +827,7 @@ void PairEngine::enqueue(bool timed) {
         check(launch_merge(wm, s), "merge kernels (weights)");
         check(launch_lattice_zero_u64((unsigned long long*)d_wstrip_.p, (size_t)n_rows_ * (nbins_ + 1), s),
               "lattice_zero_u64_kernel");
-        for (const StripSpec& st : wl_.strips) {
-            StripLaunch sl{d_orig_w_[st.set_rows].p, d_orig_w_[st.set_cols].p, st.w, d_rowmaps_.p + st.rowmap_off,
-                           d_wcnt_.p, d_wstrip_.p, inv_w_, nbins_, nbins_ + 1};
-            sl.fixed_row = wl_.rowmaps[st.rowmap_off];
-            sl.wmode = 1;
-            sl.wscale = wscale_;
-            check(launch_strip(sl, s), "strip_kernel (weights)");
-        }
+        launch_strip_tables(wstrips_, s, "strip table kernels (weights)");
     }
     if (timed) check(hipEventRecord(ev_[2], s), "hipEventRecord");
     have_timing_ = timed;

@@ -358,10 +358,24 @@ class PairEngine {
     DevBuf<int32_t> d_row_list_, d_chunk_ptr_, d_row_chunk_, d_rowmaps_;
     DevBuf<uint64_t> d_partial_, d_cnt_, d_tmp_cnt_;
     DevBuf<int64_t> d_dsum_, d_tmp_dsum_;
+    // The strips of one pass, resident, launched as at most two kernels (an
+    // atomic table and an LDS table) instead of one launch per strip: a
+    // launch waits for free compute units, which on a device busy with other
+    // streams' pair kernels costs far more than the strip's own work.
+    struct StripTables {
+        DevBuf<StripLaunch> atomic, lds;
+        DevBuf<int64_t> atomic_first, lds_first;
+        int n_atomic = 0, n_lds = 0;
+        int64_t atomic_blocks = 0, lds_blocks = 0;
+        size_t bytes() const { return atomic.bytes() + lds.bytes() + atomic_first.bytes() + lds_first.bytes(); }
+    };
+    StripTables strips_, wstrips_;
     hipEvent_t ev_[4]{};
     bool have_timing_ = false;
 
     void build(int tile_j);
+    void build_strip_tables(const std::vector<StripLaunch>& all, StripTables& t, hipStream_t s);
+    void launch_strip_tables(const StripTables& t, hipStream_t s, const char* what) const;
 };
 
 // Upper bound on the bin of any pair in the given points (bounding-box

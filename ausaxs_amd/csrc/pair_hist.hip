@@ -576,20 +576,19 @@ __global__ __launch_bounds__(256) void strip_table_kernel(const StripLaunch* tab
     strip_one(p, (blk - first[lo]) * 256 + threadIdx.x, 0);
 }
 
-// Strip kernel for a strip with one output row (fixed_row) and no replicas:
-// kStripBlockPairs pairs per workgroup accumulate in an LDS histogram of
-// packed cells (as in the pair kernel), flushed with two global atomics per
-// occupied bin. Long strips (a dummy set's rows against its last columns)
-// crowd into few bins, where one global atomic per pair serialises.
-constexpr int kStripBlockPairs = 4096;
-__global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
-    extern __shared__ uint64_t cells[];  // [nbins + 1]
+// Block `block` of a strip with one output row (fixed_row) and no replicas:
+// its kStripBlockPairs pairs accumulate in an LDS histogram of packed cells
+// (as in the pair kernel), flushed with two global atomics per occupied bin.
+// Long strips (a dummy set's rows against its last columns) crowd into few
+// bins, where one global atomic per pair serialises. Every thread of the
+// workgroup calls it (it synchronises the workgroup).
+__device__ __forceinline__ void strip_lds_block(const StripLaunch& p, long long block, uint64_t* cells) {
     const StripWork& w = p.w;
     const int nt = w.t_end - w.t_begin;
     for (int b = threadIdx.x; b <= p.nbins; b += blockDim.x) cells[b] = 0;
     __syncthreads();
     const long long total = (long long)w.n_rows * nt;
-    const long long start = (long long)blockIdx.x * kStripBlockPairs;
+    const long long start = block * kStripBlockPairs;
     const long long end = min(start + kStripBlockPairs, total);
     const Consts cs = make_consts(p.inv_w);
     for (long long tid = start + threadIdx.x; tid < end; tid += blockDim.x) {
@@ -617,6 +616,27 @@ __global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
         atomicAdd((unsigned long long*)&p.dsum[row + b],
                   (unsigned long long)((int64_t)ksum - (int64_t)n * kDeltaBias));
     }
+}
+
+__global__ __launch_bounds__(256) void strip_lds_kernel(StripLaunch p) {
+    extern __shared__ uint64_t cells[];  // [nbins + 1]
+    strip_lds_block(p, blockIdx.x, cells);
+}
+
+// Many LDS strips in one launch (the layout of strip_table_kernel with
+// kStripBlockPairs pairs per block); the block's strip is uniform over the
+// workgroup, so all its threads reach strip_lds_block's barriers.
+__global__ __launch_bounds__(256) void strip_lds_table_kernel(const StripLaunch* table, const int64_t* first, int n) {
+    extern __shared__ uint64_t cells[];  // [nbins + 1]
+    const int64_t blk = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const StripLaunch p = table[lo];
+    strip_lds_block(p, blk - first[lo], cells);
 }
 
 __global__ __launch_bounds__(256) void bin_check_kernel(uint32_t lo, uint32_t hi, float inv_w,
@@ -734,9 +754,8 @@ hipError_t launch_strip(const StripLaunch& p, hipStream_t s) {
     const long long n = (long long)p.w.n_rows * (p.w.t_end - p.w.t_begin);
     if (n <= 0) return hipSuccess;
     if (p.n_rep <= 0) return hipSuccess;
-    if (p.fixed_row >= 0 && p.n_rep == 1 && (size_t)(p.nbins + 1) * sizeof(uint64_t) <= 65536 &&
-        n > kStripBlockPairs) {
-        const unsigned blocks = (unsigned)((n + kStripBlockPairs - 1) / kStripBlockPairs);
+    if (strip_uses_lds(p)) {
+        const unsigned blocks = (unsigned)strip_lds_blocks(p);
         hipLaunchKernelGGL(strip_lds_kernel, dim3(blocks), dim3(256), (size_t)(p.nbins + 1) * sizeof(uint64_t), s,
                            p);
         return hipGetLastError();
@@ -751,6 +770,15 @@ hipError_t launch_strip_table(const StripLaunch* table, const int64_t* first, in
     if (n <= 0 || blocks <= 0) return hipSuccess;
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL(strip_table_kernel, dim3((unsigned)blocks), dim3(256), 0, s, table, first, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_lds_table(const StripLaunch* table, const int64_t* first, int n, int64_t blocks, int nbins,
+                                  hipStream_t s) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    if (blocks > 0x7fffffff || nbins < 0 || (size_t)(nbins + 1) * sizeof(uint64_t) > 65536) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(strip_lds_table_kernel, dim3((unsigned)blocks), dim3(256), (size_t)(nbins + 1) * sizeof(uint64_t),
+                       s, table, first, n);
     return hipGetLastError();
 }
 
