@@ -171,6 +171,8 @@ struct MergeLaunch {
     int64_t* dsum;
     int raw = 0;  // 1: partials of a weight pass (integer k sums per item), summed in FP64 into dsum as
                   //    the bits of doubles (cnt zero)
+    int direct = 0;  // 1: every row has exactly one chunk (chunk r is row r): the chunk sums are the
+                     //    rows, written to cnt/dsum at once, and merge_rows_kernel is not launched
 };
 hipError_t launch_merge(const MergeLaunch& m, hipStream_t s);
 

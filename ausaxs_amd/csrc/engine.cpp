@@ -791,6 +791,7 @@ void PairEngine::enqueue(bool timed) {
     if (timed) check(hipEventRecord(ev_[1], s), "hipEventRecord");
     MergeLaunch ml{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
                    n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_cnt_.p, d_dsum_.p};
+    ml.direct = mp_.one_chunk_per_row() ? 1 : 0;  // small structures: one merge launch
     check(launch_merge(ml, s), "merge kernels");
     if (timed) check(hipEventRecord(ev_[3], s), "hipEventRecord");
     // lattice jobs add into the merged rows
@@ -824,6 +825,7 @@ void PairEngine::enqueue(bool timed) {
         check(launch_pair_hist(wl, s), "pair_hist_kernel (weights)");
         MergeLaunch wm{d_partial_.p, d_row_list_.p, d_chunk_ptr_.p, d_row_chunk_.p, mp_.n_chunks(),
                        n_rows_, nbins_, d_tmp_cnt_.p, d_tmp_dsum_.p, d_wcnt_.p, d_wsum_.p, 1};
+        wm.direct = ml.direct;
         check(launch_merge(wm, s), "merge kernels (weights)");
         check(launch_lattice_zero_u64((unsigned long long*)d_wstrip_.p, (size_t)n_rows_ * (nbins_ + 1), s),
               "lattice_zero_u64_kernel");
@@ -891,7 +893,10 @@ EngineTiming PairEngine::timing() {
     t.n_lattice = (int)lat_.size();
     const int64_t row_bytes = (int64_t)(nbins_ + 1) * 8;
     const int64_t entries = mp_.chunk_ptr.back();  // partial rows summed (row_list without its slack)
-    t.merge_bytes = row_bytes * (entries + 4 * (int64_t)mp_.n_chunks() + 2 * (int64_t)n_rows_);
+    // partial rows read; chunk sums written and read back (none when each row
+    // is one chunk); rows written
+    const int64_t chunk_rw = mp_.one_chunk_per_row() ? 0 : 4 * (int64_t)mp_.n_chunks();
+    t.merge_bytes = row_bytes * (entries + chunk_rw + 2 * (int64_t)n_rows_);
     if (have_timing_) {
         check(hipEventSynchronize(ev_[2]), "hipEventSynchronize");
         check(hipEventElapsedTime(&t.pair_ms, ev_[0], ev_[1]), "hipEventElapsedTime");

@@ -208,6 +208,12 @@ void sort_items(std::vector<PairWork>& items);
 struct MergePlan {
     std::vector<int32_t> row_list, chunk_ptr, row_chunk;
     int n_chunks() const { return (int)chunk_ptr.size() - 1; }
+    // every row has exactly one chunk (then chunk r is row r)
+    bool one_chunk_per_row() const {
+        for (size_t r = 0; r + 1 < row_chunk.size(); ++r)
+            if (row_chunk[r + 1] - row_chunk[r] != 1) return false;
+        return row_chunk.size() > 1;
+    }
 };
 // assigns slot = index to every item and plans the merge into n_rows rows
 MergePlan plan_merge(std::vector<PairWork>& items, int n_rows);

@@ -740,6 +740,13 @@ hipError_t launch_pair_hist(const PairLaunch& p, hipStream_t s) {
 hipError_t launch_merge(const MergeLaunch& m, hipStream_t s) {
     if (m.n_rows == 0) return hipSuccess;
     const unsigned bx = (unsigned)((m.nbins + 1 + 255) / 256);
+    if (m.direct) {
+        // the one chunk of each row is the row (the sums merge_rows would form: 0 + the chunk)
+        if (m.n_chunks != m.n_rows) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(merge_chunks_kernel, dim3(bx, m.n_chunks), dim3(256), 0, s, m.partial, m.row_list,
+                           m.chunk_ptr, m.nbins, m.cnt, m.dsum, m.raw);
+        return hipGetLastError();
+    }
     if (m.n_chunks > 0) {
         hipLaunchKernelGGL(merge_chunks_kernel, dim3(bx, m.n_chunks), dim3(256), 0, s, m.partial, m.row_list,
                            m.chunk_ptr, m.nbins, m.tmp_cnt, m.tmp_dsum, m.raw);
