@@ -260,7 +260,7 @@ struct EngineSpec {
     int n_rows = 0;
     float inv_w = 1;
     int nbins = 10;
-    int tile_j = 8000;
+    int tile_j = kDefaultTileJ;
     int lattice = 1;
     std::vector<std::vector<float>> weights;
     int clamp = -1;  // -1: from the sets' bounding box; else whether bins must be clamped
@@ -273,7 +273,7 @@ struct SpecView {
     int n_rows = 0;
     float inv_w = 1;
     int nbins = 10;
-    int tile_j = 8000;
+    int tile_j = kDefaultTileJ;
     int lattice = 1;
     bool weighted = false;
     int clamp = -1;
@@ -295,7 +295,7 @@ class PairEngine {
   public:
     // lattice: lattice-path mode for PairJob::lattice jobs (see plan_lattice)
     PairEngine(std::vector<PointSet> sets, std::vector<PairJob> jobs, int n_rows, float inv_w, int nbins,
-               int tile_j = 8000, int lattice = 1, std::vector<std::vector<float>> weights = {});
+               int tile_j = kDefaultTileJ, int lattice = 1, std::vector<std::vector<float>> weights = {});
     explicit PairEngine(EngineSpec spec);
     ~PairEngine();
     PairEngine(const PairEngine&) = delete;
@@ -339,7 +339,7 @@ class PairEngine {
     MergePlan mp_;
     int n_rows_, nbins_;
     float inv_w_;
-    int tile_j_ = 8000;
+    int tile_j_ = kDefaultTileJ;
     bool clamp_ = true;
     int grid_ = 0;
     int rows_ = 2, copies_ = 1;

@@ -71,7 +71,7 @@ struct HistSettings {
     uint32_t bin_count = 8000;  // settings::axes::bin_count
     bool weighted_bins = true;  // settings::hist::weighted_bins
     int debye_terms = 3;        // bf16 terms per operand; 0 = FP64 contraction
-    int tile_j = 8000;          // columns per work item (<= 8064)
+    int tile_j = kDefaultTileJ; // columns per work item (<= 8064)
     int lattice = 1;            // grid dummy-dummy pairs by lattice correlation: 0 off, 1 auto, 2 always
     double surface_thickness = 1.0;  // grid-surface excluded volume: surface layer thickness (A)
 };

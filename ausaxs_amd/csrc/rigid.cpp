@@ -242,7 +242,7 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
             for (int b = 0; b < n_bodies; ++b) weights.emplace_back(w_.begin() + body_off_[b], w_.begin() + body_off_[b + 1]);
             weights.push_back(water_w_);
         }
-        PairEngine eng(sets, jobs, 2 * R_aa_, inv_w_, base_nb_, s_.tile_j ? s_.tile_j : 8000, 1, std::move(weights));
+        PairEngine eng(sets, jobs, 2 * R_aa_, inv_w_, base_nb_, s_.tile_j ? s_.tile_j : kDefaultTileJ, 1, std::move(weights));
         eng.run(cnt, dsum, false, weighted_ ? &wsum : nullptr);
         if (eng.overflowed(cnt))
             throw std::runtime_error("ausaxs_hip: a pair distance falls beyond bin_count*bin_width");
@@ -1012,7 +1012,7 @@ void RigidScan::scan_hydrated(const double* T, int n_conf, const double* wxyz, c
         const std::vector<int> moving = moving_bodies(Tc, nc);
         const int nbins = bound_bins(Tc, nc, &hy);
         // the tile width of the scan's tuned plans, else the default
-        const int tile = s_.tile_j ? s_.tile_j : (info_.tile_j ? info_.tile_j : 8000);
+        const int tile = s_.tile_j ? s_.tile_j : (info_.tile_j ? info_.tile_j : kDefaultTileJ);
         if (last_ == hplan_.get()) last_ = nullptr;
         hplan_.reset();
         hplan_ = build_plan(nc, moving, nbins, tile, &hy);

@@ -95,9 +95,29 @@ size_t Server::footprint(Mode mode, const double* atoms_xyz, int n, const double
     const double nb = std::min<double>(s.bin_count, std::sqrt(diag2) / s.bin_width + 4);
     const double x = mode == Mode::FFGrid ? dummies_estimate(atoms_xyz, n) : 0.0;
     const double P = n + m + x;
+    // lattice correlation of the dummies: one transform grid of at least twice
+    // the padded atom box per axis, and the squared-distance counts
+    double cells = 1, smax = 0;
+    bool fits = mode == Mode::FFGrid && s.lattice != 0 && n > 0;
+    if (fits)
+        for (int k = 0; k < 3; ++k) {
+            const double E = std::ceil(ahi[k] - alo[k]) + 2 * 3 + 1;
+            const int L = E <= kLatticeMaxL ? lattice_length((int)E) : -1;
+            fits = fits && L > 0;
+            const double Lk = L > 0 ? L : 2 * E;
+            cells *= Lk;
+            smax += (E - 1) * (E - 1);
+        }
+    // pair-kernel rows: atoms and waters, and the dummies unless the lattice
+    // takes their block. With the lattice on and the box within its lines it
+    // declines only dummy sets below 1e8 pairs (14,143 points) or below 64
+    // pairs per transform cell (sqrt(128 cells) points), which then are rows.
+    const double dummy_rows = fits ? std::min(x, std::max(14143.0, std::sqrt(128.0 * cells))) : x;
+    const double rows_pts = n + m + dummy_rows;
     const double C = 16;
-    const double tile = s.tile_j > 0 ? s.tile_j : 8000;
-    const double items = (P / 512 + C) * (P / tile + C);
+    const double tile = s.tile_j > 0 ? s.tile_j : kDefaultTileJ;
+    // columns counted as points (a lattice set's z-runs are fewer)
+    const double items = (rows_pts / 512 + C) * (P / tile + C);
     const double rows = 15 * 15 + 32;
     const double cell = 8 * (nb + 1);
     double bytes = 2 * 16 * P                          // class-sorted + original-order points
@@ -105,19 +125,7 @@ size_t Server::footprint(Mode mode, const double* atoms_xyz, int n, const double
                    + (items / 16 + rows) * 2 * cell    // merge chunk sums
                    + rows * 2 * cell                   // counts + delta sums
                    + rows * nb * (8 + 6) + nq * nb * 6 + rows * nq * 8 + nq * 8;  // Debye operands
-    if (mode == Mode::FFGrid && s.lattice != 0 && n > 0) {
-        // lattice correlation of the dummies: one transform grid of at least
-        // twice the padded atom box per axis, and the squared-distance counts
-        double cells = 1, smax = 0;
-        for (int k = 0; k < 3; ++k) {
-            const double E = std::ceil(ahi[k] - alo[k]) + 2 * 3 + 1;
-            const int L = E <= kLatticeMaxL ? lattice_length((int)E) : -1;
-            const double Lk = L > 0 ? L : 2 * E;  // beyond the longest line the pair kernel takes the job
-            cells *= Lk;
-            smax += (E - 1) * (E - 1);
-        }
-        bytes += cells * 16 + smax * 8;
-    }
+    if (fits) bytes += cells * 16 + smax * 8;
     return (size_t)(bytes * 1.1) + (1u << 20);
 }
 

@@ -140,6 +140,11 @@ def traffic_from_profiles():
     return None
 
 
+def tile_kw(args):
+    """The work-item width option, if given (else the library default)."""
+    return {} if args.tile_j is None else {"tile_j": args.tile_j}
+
+
 def pairs_of(n, m):
     return n * (n - 1) / 2 + n * m + m * (m - 1) / 2
 
@@ -154,7 +159,7 @@ def bench_structure(cfg, args):
     waters = A.synth_shell(args.config + 1, cfg["waters"], cfg["axes"], 3.0) if cfg["waters"] else None
     q = np.linspace(*cfg["q"])
     mode = A.MODE_FFAVG if cfg["mode"] == "ffavg" else A.MODE_FFEXPLICIT
-    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, tile_j=args.tile_j)
+    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, **tile_kw(args))
     t0 = time.perf_counter()
     st = A.Structure(mode, atoms, waters, settings, z_exv=1.0)
     log(f"resident structure built in {time.perf_counter() - t0:.2f} s")
@@ -233,7 +238,7 @@ def bench_grid(cfg, args):
     xyz, ff = g[:, :3].astype(np.float64), g[:, 3].view(np.int32).copy()
     wxyz = w[:, :3].astype(np.float64)
     q = np.linspace(*cfg["q"])
-    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, tile_j=args.tile_j)
+    settings = A.default_settings(bin_width=cfg["bw"], debye_terms=args.debye_terms, **tile_kw(args))
     t0 = time.perf_counter()
     st = A.GridStructure(xyz, ff, wxyz, settings)
     log(f"resident grid structure built in {time.perf_counter() - t0:.2f} s (dummies from the 1 A grid)")
@@ -311,7 +316,8 @@ def bench_rigid(cfg, args):
     settings = A.default_settings(bin_width=cfg["bw"])
     t0 = time.perf_counter()
     if args.fixed_tiles:
-        settings.tile_j = args.tile_j
+        if args.tile_j is not None:
+            settings.tile_j = args.tile_j
     scan = A.RigidScan(bodies, weights, settings=settings, autotune=not args.fixed_tiles)
     log(f"bodies resident, self terms cached in {time.perf_counter() - t0:.2f} s")
     t0 = time.perf_counter()
@@ -526,7 +532,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
     ap.add_argument("--debye-terms", type=int, default=3)
-    ap.add_argument("--tile-j", type=int, default=8000, help="columns per work item")
+    ap.add_argument("--tile-j", type=int, default=None, help="columns per work item (default: the library's)")
     ap.add_argument("--fixed-tiles", action="store_true",
                     help="rigid scan: use --tile-j and one workgroup per item instead of the autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -52,7 +52,7 @@ typedef struct {
     int bin_count;     /* settings::axes::bin_count (default 8000)  */
     int weighted_bins; /* settings::hist::weighted_bins (default 1) */
     int debye_terms;   /* bf16 terms per GEMM operand: 2 = hi+lo, 3 = hi+mid+lo (default); 0 = FP64 */
-    int tile_j;        /* columns per pair work item (default 8000, at most 8064) */
+    int tile_j;        /* columns per pair work item (default 2000, at most 8064) */
     int lattice;       /* grid dummy-dummy pairs by lattice correlation instead of pair by pair
                           (identical histograms): 0 off, 1 when cheaper (default), 2 always */
     double surface_thickness; /* grid-surface excluded volume: settings::grid::exv::surface_thickness
