@@ -48,9 +48,14 @@ __host__ __device__ inline int delta_exp(float inv_w) {
     return ex + 19;
 }
 __host__ __device__ inline double delta_unit(float inv_w) { return ldexp(1.0, -delta_exp(inv_w)); }
-// Sum of the distances (A) of n pairs in bin b whose offsets sum to ds units.
+// Sum of the distances (A) of n pairs in bin b whose offsets sum to ds units,
+// with the unit (delta_unit) evaluated by the caller: loops over many bins
+// take it once instead of a frexpf and an ldexp per bin.
+__host__ __device__ inline double distance_sum_u(uint64_t n, int b, int64_t ds, float inv_w, double unit) {
+    return (double)n * (double)b / (double)inv_w + (double)ds * unit;
+}
 __host__ __device__ inline double distance_sum(uint64_t n, int b, int64_t ds, float inv_w) {
-    return (double)n * (double)b / (double)inv_w + (double)ds * delta_unit(inv_w);
+    return distance_sum_u(n, b, ds, inv_w, delta_unit(inv_w));
 }
 
 // One workgroup's rectangle of the pair space.

@@ -511,6 +511,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::finalize() {
                                      " A); increase bin_count or bin_width");
     const size_t nb = nbins;
     const float inv_w = inv_w_;
+    const double unit = delta_unit(inv_w);
 
     // entry of one global row and bin, times the reference's template factor
     auto entry = [&](int row, int b, int factor) {
@@ -519,7 +520,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> ResidentStructure::finalize() {
         WeightedEntry e;
         e.value = (double)factor * (double)c;
         e.count = (uint32_t)(factor * c);
-        e.bin_center = factor * distance_sum(c, b, s, inv_w);
+        e.bin_center = factor * distance_sum_u(c, b, s, inv_w, unit);
         return e;
     };
 
@@ -807,7 +808,10 @@ const std::vector<double>& CompositeDistanceHistogramHIP::sinqd(const std::vecto
         for (int b = 0; b < K; ++b) nz |= src[b].value != 0;
         if (!nz) return;
         keys[axis].push_back({kind, f1, f2});
-        for (int b = 0; b < K; ++b) H[axis].push_back(src[b].value);
+        const size_t o = H[axis].size();
+        H[axis].resize(o + (size_t)K);
+        double* h = H[axis].data() + o;
+        for (int b = 0; b < K; ++b) h[b] = src[b].value;
     };
     if (mode == Mode::Simple) {
         if (const WeightedEntry* r = aa.row(0, 0)) push_row(0, 0, 0, 0, r);
@@ -1151,9 +1155,10 @@ void raw_pair_histogram(const float* a, int n, const float* b, int m, const Hist
     std::vector<int64_t> ds;
     eng.run(c, ds);
     if (eng.overflowed(c)) throw std::runtime_error("ausaxs_hip: pair distance beyond bin_count*bin_width");
+    const double unit = delta_unit(inv_w);
     for (int k = 0; k < nbins; ++k) {
         count[k] = c[k];
-        dsum[k] = distance_sum(c[k], (int)k, ds[k], inv_w);
+        dsum[k] = distance_sum_u(c[k], (int)k, ds[k], inv_w, unit);
     }
 }
 

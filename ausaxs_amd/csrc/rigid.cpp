@@ -249,6 +249,7 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
         wscale = eng.wscale();
     }
     const int stride = base_nb_ + 1;
+    const double unit = delta_unit(inv_w_);
     base_aa_.assign(base_nb_, WeightedEntry{});
     base_ww_.assign(base_nb_, WeightedEntry{});
     for (int r = 0; r < 2 * R_aa_; ++r) {
@@ -263,7 +264,7 @@ RigidScan::RigidScan(const double* xyz, const float* w, const int32_t* sizes, in
             e.value = weighted_ ? 2.0 * weight_value(wsum.items[cell], wsum.strips[cell], wscale)
                                 : 2.0 * (double)c * row_w_[rr];
             e.count = (uint32_t)(2 * c);
-            e.bin_center = 2.0 * distance_sum(c, b, ds, inv_w_);
+            e.bin_center = 2.0 * distance_sum_u(c, b, ds, inv_w_, unit);
             entry_add(dst[b], e);
         }
     }
@@ -1055,6 +1056,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
     }
     ctx.sync();
     const int nb = p.nbins;
+    const double unit = delta_unit(inv_w_);
     std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
     for (int b = 0; b < std::min(nb, base_nb_); ++b) {
         aa[b] = base_aa_[b];
@@ -1069,7 +1071,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> RigidScan::histogram(int c) {
                 WeightedEntry e;  // add<2> (SimpleCalculator.h:195-207)
                 e.value = weighted_ ? 2.0 * weight_value(wsum[cell], wstrip[cell], wscale_) : 2.0 * (double)n * row_w[r];
                 e.count = (uint32_t)(2 * n);
-                e.bin_center = 2.0 * distance_sum(n, b, dsum[cell], inv_w_);
+                e.bin_center = 2.0 * distance_sum_u(n, b, dsum[cell], inv_w_, unit);
                 entry_add(row_kind[r] == 0 ? aa[b] : row_kind[r] == 1 ? aw[b] : ww[b], e);
             }
     if (p.hydrated)  // this conformation's water set's self term (SimpleCalculator.h:144-160)

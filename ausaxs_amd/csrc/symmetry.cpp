@@ -433,6 +433,8 @@ std::unique_ptr<CompositeDistanceHistogramHIP> SymmetryStructure::calculate(bool
                                  std::to_string(s_.bin_count) + " bins of " + std::to_string(s_.bin_width) +
                                  " A); increase bin_count or bin_width");
     const size_t nb = nbins;
+    const float inv_w = engine_->inv_w();
+    const double unit = delta_unit(inv_w);
     std::vector<WeightedEntry> aa(nb), aw(nb), ww(nb);
     for (size_t r = 0; r < rows_.size(); ++r) {
         const auto& k = rows_[r];
@@ -449,7 +451,7 @@ std::unique_ptr<CompositeDistanceHistogramHIP> SymmetryStructure::calculate(bool
             WeightedEntry e;
             e.value = (double)(factor * c) * wprod;
             e.count = (uint32_t)(factor * c);
-            e.bin_center = (double)factor * distance_sum(c, b, sd, engine_->inv_w());
+            e.bin_center = (double)factor * distance_sum_u(c, b, sd, inv_w, unit);
             entry_add((*dst)[b], e);
         }
     }
